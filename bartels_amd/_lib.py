@@ -1,0 +1,128 @@
+"""ctypes binding of the C ABI in ``include/bartels_hip.h`` (``bartels_amd/libbartels_hip.so``).
+
+The product path has no fallback: if the HIP library is missing this module raises, loudly.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbartels_hip.so")
+
+c_int = ctypes.c_int
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+P = ctypes.POINTER
+
+# status codes (bh_status)
+BH_OK, BH_BAD_ARG, BH_NONFINITE, BH_SOLVE_FAILED, BH_MAXIT, BH_DEVICE_ERROR, BH_UNSUPPORTED = range(7)
+STATUS_NAMES = ["BH_OK", "BH_BAD_ARG", "BH_NONFINITE", "BH_SOLVE_FAILED", "BH_MAXIT", "BH_DEVICE_ERROR",
+                "BH_UNSUPPORTED"]
+
+# models (bh_model)
+NEOHOOKEAN, STVK, COROTATIONAL, ARAP, NEOHOOKEAN_LEGACY = 0, 1, 2, 3, 4
+MODEL_PER_TET = 255
+
+
+class BhPsd(ctypes.Structure):
+    _fields_ = [("enabled", c_int), ("tol", c_double)]
+
+
+class BhElementOpts(ctypes.Structure):
+    _fields_ = [("stretch_reference_layout", c_int)]
+
+
+class BhNewtonOpts(ctypes.Structure):
+    _fields_ = [
+        ("model", c_int),
+        ("d_model_ids", c_void_p),
+        ("psd", BhPsd),
+        ("gradient_tol", c_double),
+        ("relative_tol", c_int),
+        ("max_iterations", c_int),
+        ("max_iterations_ls", c_int),
+        ("alpha", c_double),
+        ("c", c_double),
+        ("p", c_double),
+        ("armijo_with_alpha", c_int),
+        ("reject_nonfinite", c_int),
+        ("pcg_tol", c_double),
+        ("pcg_max_iterations", c_int),
+        ("precond", c_int),
+    ]
+
+
+class BhNewtonReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_int),
+        ("converged", c_int),
+        ("pcg_iterations_total", c_int),
+        ("pcg_failures", c_int),
+        ("ls_iterations_total", c_int),
+        ("energy", c_double),
+        ("gradient_norm", c_double),
+        ("gradient_norm0", c_double),
+    ]
+
+
+class BartelsError(RuntimeError):
+    def __init__(self, status, where):
+        name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"{where} failed with {name}")
+        self.status = status
+
+
+# every symbol include/bartels_hip.h declares: name -> (restype, argtypes)
+_VP = c_void_p
+SIGNATURES = {
+    "bh_linear_tetmesh_dphi_dX": (c_int, [c_int, _VP, c_int, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dphi_dX_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "bh_mesh_create": (c_int, [c_int, c_int, _VP, _VP, _VP, P(c_void_p)]),
+    "bh_mesh_destroy": (c_int, [_VP]),
+    "bh_mesh_nnz": (c_int, [_VP, P(c_int64)]),
+    "bh_mesh_pattern": (c_int, [_VP, _VP, _VP]),
+    "bh_mesh_block_pattern_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_int64)]),
+    "bh_mesh_set_element_opts": (c_int, [_VP, P(BhElementOpts)]),
+    "bh_linear_tetmesh_q": (c_int, [_VP, c_int, _VP, _VP, P(c_double)]),
+    "bh_linear_tetmesh_dq": (c_int, [_VP, c_int, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dq2": (c_int, [_VP, c_int, _VP, _VP, P(BhPsd), _VP]),
+    "bh_linear_tetmesh_generic_elastic_q": (c_int, [_VP, _VP, _VP, _VP, P(c_double)]),
+    "bh_linear_tetmesh_generic_elastic_dq": (c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_generic_elastic_dq2": (c_int, [_VP, _VP, _VP, _VP, P(BhPsd), _VP]),
+    "bh_linear_tetmesh_q_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dq_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dq2_device": (c_int, [_VP, c_int, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
+    "bh_linear_tetmesh_mass_matrix_device": (c_int, [_VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_const_ext_force_device": (c_int, [_VP, _VP, _VP, _VP]),
+    "bh_newton_default_opts": (None, [P(BhNewtonOpts)]),
+    "bh_newton_static_device": (c_int, [_VP, P(BhNewtonOpts), _VP, _VP, _VP, _VP, P(BhNewtonReport), _VP, _VP]),
+    "bh_spmv_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
+    "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),
+}
+
+KERNEL_CLASSES = ["energy", "gradient", "hessian_gather", "psd_classify", "psd_eigen"]
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m bartels_amd.build` (there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, where):
+    if status != BH_OK:
+        raise BartelsError(status, where)

@@ -1,0 +1,1233 @@
+// gfx950 kernels and the C ABI (include/bartels_hip.h) for the linear-tetmesh hot path.
+//
+// Data layout in HBM (built once by bh_mesh_create):
+//   tetE    int4 per tet (AoS, 16 B)                 -- from E (nT x 4 column-major)
+//   tetPhi  12 doubles per tet (AoS, 96 B), Phi(m,j) at 12e + m + 4j  -- from dphidX
+//   vol     1 double per tet
+//   incPtr  nV+1, inc[k] = (e << 2) | corner for the tets incident to a vertex, ascending e
+//   incRank packed ranks (4 x 8 bit) of the tet's four vertices in the vertex's sorted neighbour list
+//   bptr    nV+1 block-row pointer; bcol sorted neighbour vertices (the 3x3-block pattern)
+// CSR values of the reference pattern: row 3v+l starts at 9*bptr[v] + 3*l*nb(v); the block of
+// neighbour rank r sits at +3r.  Rows 3v..3v+2 therefore form one contiguous segment.
+//
+// Assembly is a deterministic gather: one lane per vertex walks its incident tets in ascending
+// element order, recomputes that tet's block row for the vertex and accumulates into LDS; the
+// workgroup then writes its contiguous CSR segment once.  Every slot is therefore a left fold in
+// ascending element order, the order of Eigen's setFromTriplets on the reference's triplet list
+// (src/assemble.cpp:68-94).  No floating-point atomics anywhere.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/bartels_hip.h"
+#include "elements.h"
+
+using namespace bh;
+
+#define BH_CHECK(x)                                                                            \
+  do {                                                                                         \
+    hipError_t err__ = (x);                                                                    \
+    if (err__ != hipSuccess) {                                                                 \
+      std::fprintf(stderr, "bartels_hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(err__), \
+                   __FILE__, __LINE__);                                                        \
+      return BH_DEVICE_ERROR;                                                                  \
+    }                                                                                          \
+  } while (0)
+
+static constexpr int kEnergyBlock = 256;
+static constexpr int kEnergyItems = 4;  // tets per thread in the energy kernel
+static constexpr int kGatherThreads = 64;
+
+// host-side mesh state ------------------------------------------------------------------------
+struct bh_mesh {
+  int nV = 0, nT = 0;
+  int64_t nnz = 0, nblocks = 0;
+  int device = 0;
+  bh_element_opts opts{1};
+  // host copies of the pattern
+  std::vector<int32_t> h_bptr, h_bcol;
+  // device arrays
+  int4* tetE = nullptr;
+  double* tetPhi = nullptr;
+  double* vol = nullptr;
+  int32_t* incPtr = nullptr;
+  int32_t* inc = nullptr;
+  uint32_t* incRank = nullptr;
+  int32_t* bptr = nullptr;
+  int32_t* bcol = nullptr;
+  int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max doubles per gather workgroup
+  // scratch
+  double* partials = nullptr;
+  int nPartials = 0;
+  double* d_energy = nullptr;
+  // host-API staging
+  double* stage_q = nullptr;
+  double* stage_params = nullptr;
+  double* stage_out = nullptr;
+  uint8_t* stage_ids = nullptr;
+  double* stage_vals = nullptr;
+  // PSD
+  uint8_t* psdFlag = nullptr;
+  int32_t* psdList = nullptr;
+  int32_t* psdCount = nullptr;
+  double* psdDM = nullptr;  // nT x 45
+  // solver workspace (lazily allocated)
+  double* ws = nullptr;
+  size_t wsBytes = 0;
+  hipStream_t own = nullptr;
+  // per-kernel-class HIP event pairs recorded while profiling is on
+  mutable bool prof = false;
+  mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> profEv[BH_KT_COUNT];
+};
+
+namespace {
+// records a start / stop event pair around the launches in its scope when profiling is on
+struct ProfScope {
+  const bh_mesh* m;
+  int k;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(const bh_mesh* m_, int k_, hipStream_t s_) : m(m_), k(k_), s(s_) {
+    if (m->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      m->profEv[k].emplace_back(a, b);
+    }
+  }
+};
+
+void clear_profile(const bh_mesh* m) {
+  for (auto& v : m->profEv) {
+    for (auto& e : v) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    v.clear();
+  }
+}
+}  // namespace
+
+namespace {
+
+struct MeshView {
+  int nV, nT;
+  const int4* __restrict__ tetE;
+  const double* __restrict__ tetPhi;
+  const double* __restrict__ vol;
+  const int32_t* __restrict__ incPtr;
+  const int32_t* __restrict__ inc;
+  const uint32_t* __restrict__ incRank;
+  const int32_t* __restrict__ bptr;
+  const int32_t* __restrict__ bcol;
+};
+
+MeshView view(const bh_mesh* m) {
+  return MeshView{m->nV, m->nT, m->tetE, m->tetPhi, m->vol, m->incPtr, m->inc, m->incRank, m->bptr, m->bcol};
+}
+
+struct PsdView {
+  double tolPos;  // max(tol, 0) / 4: clamp of the three translation eigenvalues, per block
+  double tol;
+  const uint8_t* __restrict__ flag;
+  const double* __restrict__ dM;
+};
+
+// ---- tet loads -------------------------------------------------------------------------------
+__device__ __forceinline__ void load_phi(const double* __restrict__ tetPhi, int e, double phi[4][3]) {
+  const double2* p2 = reinterpret_cast<const double2*>(tetPhi + 12ll * e);
+  double r[12];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double2 t = p2[i];
+    r[2 * i] = t.x;
+    r[2 * i + 1] = t.y;
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) phi[m][j] = r[m + 4 * j];
+}
+
+// F = sum_m x_m phi_m^T  (the reference's B * qe, include/flatten_multiply.h:52-75)
+__device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, const double phi[4][3], double* F) {
+  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
+  double x[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double* qp = q + 3ll * vid[m];
+    x[m][0] = qp[0];
+    x[m][1] = qp[1];
+    x[m][2] = qp[2];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j] + x[3][i] * phi[3][j];
+}
+
+template <int MODEL>
+__device__ __forceinline__ int model_at(const uint8_t* __restrict__ mids, int e) {
+  return MODEL == M_PER_TET ? (int)mids[e] : MODEL;
+}
+
+// ---- energy ----------------------------------------------------------------------------------
+template <int MODEL>
+__device__ __forceinline__ double tet_energy(const MeshView& m, const double* __restrict__ q,
+                                             const double* __restrict__ params, const uint8_t* __restrict__ mids,
+                                             int e) {
+  double phi[4][3], F[9];
+  load_phi(m.tetPhi, e, phi);
+  load_F(q, m.tetE[e], phi, F);
+  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  double v;
+  switch (model_at<MODEL>(mids, e)) {
+    case M_NH: v = psi<M_NH>(F, p0, p1); break;
+    case M_STVK: v = psi<M_STVK>(F, p0, p1); break;
+    case M_COROT: v = psi<M_COROT>(F, p0, p1); break;
+    case M_ARAP: v = psi<M_ARAP>(F, p0, p1); break;
+    default: v = psi<M_NH_LEGACY>(F, p0, p1); break;
+  }
+  return v * m.vol[e];
+}
+
+__device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
+  // fixed-shape tree: wave reduction by xor shuffles, then the per-wave partials in order
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) r += sh[w];
+  }
+  return r;
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(kEnergyBlock)
+    k_energy(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+             const uint8_t* __restrict__ mids, double* __restrict__ partials) {
+  __shared__ double sh[kEnergyBlock / 64];
+  const int base = blockIdx.x * kEnergyBlock * kEnergyItems + threadIdx.x;
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < kEnergyItems; ++i) {
+    const int e = base + i * kEnergyBlock;
+    if (e < m.nT) acc += tet_energy<MODEL>(m, q, params, mids, e);
+  }
+  const double s = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partials, int n,
+                                                          double* __restrict__ out) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+  const double s = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// ---- gradient (vertex gather, ascending element order per vertex) ------------------------------
+template <int MODEL>
+__global__ void __launch_bounds__(256)
+    k_gradient(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+               const uint8_t* __restrict__ mids, double* __restrict__ g) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= m.nV) return;
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  const int k1 = m.incPtr[v + 1];
+  for (int k = m.incPtr[v]; k < k1; ++k) {
+    const int code = m.inc[k];
+    const int e = code >> 2, a = code & 3;
+    double phi[4][3], F[9], P[9];
+    load_phi(m.tetPhi, e, phi);
+    load_F(q, m.tetE[e], phi, F);
+    const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+    switch (model_at<MODEL>(mids, e)) {
+      case M_NH: piola<M_NH>(F, p0, p1, P); break;
+      case M_STVK: piola<M_STVK>(F, p0, p1, P); break;
+      case M_COROT: piola<M_COROT>(F, p0, p1, P); break;
+      case M_ARAP: piola<M_ARAP>(F, p0, p1, P); break;
+      default: piola<M_NH_LEGACY>(F, p0, p1, P); break;
+    }
+    const double vl = m.vol[e];
+    double pa[3];
+    // phi row a, selected without dynamic register indexing
+#pragma unroll
+    for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
+    double f[3];
+    mv3(P, pa, f);  // (B^T vec P)_a = P phi_a
+    g0 += vl * f[0];
+    g1 += vl * f[1];
+    g2 += vl * f[2];
+  }
+  g[3ll * v] = g0;
+  g[3ll * v + 1] = g1;
+  g[3ll * v + 2] = g2;
+}
+
+// ---- Hessian (vertex gather into LDS) -----------------------------------------------------------
+template <int MODEL>
+__device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __restrict__ q,
+                                           const double* __restrict__ params, int e, int a, bool stretch_ref,
+                                           double blk[4][9]) {
+  if (MODEL == M_MASS) {
+    // consistent mass (src/linear_tet_mass_matrix.cpp:8-44); params[e] = density, q unused
+    const double mass = params[e] * m.vol[e];
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+      const double c = (mm == a) ? mass * (1.0 / 10.0) : mass * (1.0 / 20.0);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[mm][i] = (i % 4 == 0) ? c : 0.0;
+    }
+    return;
+  }
+  double phi[4][3];
+  Pre P;
+  load_phi(m.tetPhi, e, phi);
+  load_F(q, m.tetE[e], phi, P.F);
+  P.vol = m.vol[e];
+  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  double pa[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
+}
+
+// Q (4x3, orthonormal columns, orthogonal to the ones vector) of the thin QR of Phi
+__device__ __forceinline__ void phi_qr(const double phi[4][3], double Q[4][3], double R[3][3]) {
+  double c0[4], c1[4], c2[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    c0[m] = phi[m][0];
+    c1[m] = phi[m][1];
+    c2[m] = phi[m][2];
+  }
+  auto d4 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; };
+  const double r00 = sqrt(d4(c0, c0));
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c0[m] /= r00;
+  const double r01 = d4(c0, c1);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c1[m] -= r01 * c0[m];
+  const double r11 = sqrt(d4(c1, c1));
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c1[m] /= r11;
+  const double r02 = d4(c0, c2);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c2[m] -= r02 * c0[m];
+  const double r12 = d4(c1, c2);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c2[m] -= r12 * c1[m];
+  const double r22 = sqrt(d4(c2, c2));
+#pragma unroll
+  for (int m = 0; m < 4; ++m) c2[m] /= r22;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    Q[m][0] = c0[m];
+    Q[m][1] = c1[m];
+    Q[m][2] = c2[m];
+  }
+  R[0][0] = r00; R[0][1] = r01; R[0][2] = r02;
+  R[1][0] = 0.0; R[1][1] = r11; R[1][2] = r12;
+  R[2][0] = 0.0; R[2][1] = 0.0; R[2][2] = r22;
+}
+
+// packed upper-triangular index of a symmetric 9x9
+__device__ __forceinline__ int sym9(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a * 9 - (a * (a - 1)) / 2 + (b - a);
+}
+
+template <int MODEL, bool PSD, int VPW>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+              const uint8_t* __restrict__ mids, PsdView psd, int stretchRef, double* __restrict__ vals) {
+  extern __shared__ double acc[];
+  const int v0 = blockIdx.x * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = 0.0;
+  __syncthreads();
+  const int v = v0 + (int)threadIdx.x;
+  if ((int)threadIdx.x < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    double* row = acc + 9 * (m.bptr[v] - b0);
+    const int k1 = m.incPtr[v + 1];
+    for (int k = m.incPtr[v]; k < k1; ++k) {
+      const int code = m.inc[k];
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incRank[k];
+      double blk[4][9];
+      switch (model_at<MODEL>(mids, e)) {
+        case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
+        case M_STVK: tet_blocks<M_STVK>(m, q, params, e, a, stretchRef, blk); break;
+        case M_COROT: tet_blocks<M_COROT>(m, q, params, e, a, stretchRef, blk); break;
+        case M_ARAP: tet_blocks<M_ARAP>(m, q, params, e, a, stretchRef, blk); break;
+        case M_MASS: tet_blocks<M_MASS>(m, q, params, e, a, stretchRef, blk); break;
+        default: tet_blocks<M_NH_LEGACY>(m, q, params, e, a, stretchRef, blk); break;
+      }
+      if (PSD) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+          blk[mm][0] += psd.tolPos;
+          blk[mm][4] += psd.tolPos;
+          blk[mm][8] += psd.tolPos;
+        }
+        if (psd.flag[e]) {
+          // + Q dM Q^T restricted to block row a:  sum_{p,r} Q[a][p] dM_{pr} Q[m][r]
+          double phi[4][3], Q[4][3], R[3][3];
+          load_phi(m.tetPhi, e, phi);
+          phi_qr(phi, Q, R);
+          const double* dM = psd.dM + 45ll * e;
+          double qa[3];
+#pragma unroll
+          for (int p = 0; p < 3; ++p) qa[p] = (a == 0) ? Q[0][p] : (a == 1) ? Q[1][p] : (a == 2) ? Q[2][p] : Q[3][p];
+          double T[3][9];  // T[r] = sum_p qa[p] dM_{p r} (3x3 sub-blocks, row-major l,k)
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) {
+                double s = 0.0;
+#pragma unroll
+                for (int p = 0; p < 3; ++p) s += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
+                T[r][3 * l + kk] = s;
+              }
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) blk[mm][i] += Q[mm][0] * T[0][i] + Q[mm][1] * T[1][i] + Q[mm][2] * T[2][i];
+        }
+      }
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm) {
+        double* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[mm][3 * l + kk];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
+// ---- PSD projection (simple_psd_fix, src/simple_psd_fix.cpp:5-21) ------------------------------
+// K = vol B^T H B has the three translations in its null space; with Phi = Q R (thin QR),
+// K = (Q x I3) M (Q x I3)^T,  M = (R x I3) vol H (R x I3)^T (9x9), and
+// psd(K) = K + (Q x I3)(M' - M)(Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  M' = M with eigenvalues
+// clamped at tol.  Pass 1 flags tets whose M - tol I is not positive definite (Cholesky);
+// pass 2 eigen-decomposes M (cyclic Jacobi) for the flagged tets only and stores dM = M' - M.
+template <int MODEL>
+__device__ __forceinline__ void reduced_M(const MeshView& m, const double* __restrict__ q,
+                                          const double* __restrict__ params, int e, bool stretchRef,
+                                          double M[81]) {
+  double phi[4][3], Q[4][3], R[3][3];
+  Pre P;
+  load_phi(m.tetPhi, e, phi);
+  load_F(q, m.tetE[e], phi, P.F);
+  P.vol = m.vol[e];
+  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  phi_qr(phi, Q, R);
+  pre_hessian<MODEL>(P, p0, p1, stretchRef);
+#pragma unroll
+  for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      double b[9];
+      hess_block<MODEL>(P, R[pp], R[rr], b);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) M[(3 * pp + l) + 9 * (3 * rr + kk)] = b[3 * l + kk];
+    }
+}
+
+template <int MODEL>
+__device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const double* params,
+                                                   const uint8_t* mids, int e, bool stretchRef, double M[81]) {
+  switch (model_at<MODEL>(mids, e)) {
+    case M_NH: reduced_M<M_NH>(m, q, params, e, stretchRef, M); break;
+    case M_STVK: reduced_M<M_STVK>(m, q, params, e, stretchRef, M); break;
+    case M_COROT: reduced_M<M_COROT>(m, q, params, e, stretchRef, M); break;
+    case M_ARAP: reduced_M<M_ARAP>(m, q, params, e, stretchRef, M); break;
+    default: reduced_M<M_NH_LEGACY>(m, q, params, e, stretchRef, M); break;
+  }
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(256)
+    k_psd_classify(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+                   const uint8_t* __restrict__ mids, double tol, int stretchRef, uint8_t* __restrict__ flag,
+                   int32_t* __restrict__ list, int32_t* __restrict__ count) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  double M[81];
+  reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+  // Cholesky of the lower triangle of M - tol I (symmetrised)
+  bool pd = true;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    double d = 0.5 * (M[j + 9 * j] + M[j + 9 * j]) - tol;
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= M[j + 9 * k] * M[j + 9 * k];
+    pd = pd && (d > 0.0);
+    const double ljj = sqrt(fmax(d, 1e-300));
+#pragma unroll
+    for (int i = j + 1; i < 9; ++i) {
+      double s = 0.5 * (M[i + 9 * j] + M[j + 9 * i]);
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= M[i + 9 * k] * M[j + 9 * k];
+      M[i + 9 * j] = s / ljj;
+    }
+    M[j + 9 * j] = ljj;
+  }
+  flag[e] = pd ? 0 : 1;
+  if (!pd) {
+    const int idx = atomicAdd(count, 1);
+    list[idx] = e;
+  }
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(64)
+    k_psd_eigen(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+                const uint8_t* __restrict__ mids, double tol, int stretchRef, const int32_t* __restrict__ list,
+                const int32_t* __restrict__ count, double* __restrict__ dM) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  const int e = list[i];
+  double A[81], W[81];
+  reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, A);
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r + 1; c < 9; ++c) {
+      const double s = 0.5 * (A[r + 9 * c] + A[c + 9 * r]);
+      A[r + 9 * c] = s;
+      A[c + 9 * r] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? 1.0 : 0.0;
+  double scale = 0.0;
+#pragma unroll
+  for (int k = 0; k < 81; ++k) scale += A[k] * A[k];
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    double off = 0.0;
+#pragma unroll
+    for (int pp = 0; pp < 9; ++pp)
+#pragma unroll
+      for (int rr = pp + 1; rr < 9; ++rr) off += A[pp + 9 * rr] * A[pp + 9 * rr];
+    if (off <= 1e-32 * scale) break;
+#pragma unroll
+    for (int pp = 0; pp < 8; ++pp)
+#pragma unroll
+      for (int rr = pp + 1; rr < 9; ++rr) {
+        const double apq = A[pp + 9 * rr];
+        if (apq != 0.0) {
+          const double theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (2.0 * apq);
+          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const double akp = A[k + 9 * pp], akq = A[k + 9 * rr];
+            A[k + 9 * pp] = c * akp - s * akq;
+            A[k + 9 * rr] = s * akp + c * akq;
+          }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const double apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
+            A[pp + 9 * k] = c * apk - s * aqk;
+            A[rr + 9 * k] = s * apk + c * aqk;
+          }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const double wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
+            W[k + 9 * pp] = c * wkp - s * wkq;
+            W[k + 9 * rr] = s * wkp + c * wkq;
+          }
+        }
+      }
+  }
+  double lift[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const double lam = A[k + 9 * k];
+    lift[k] = (lam < tol) ? (tol - lam) : 0.0;
+  }
+  double* out = dM + 45ll * e;
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r; c < 9; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s += W[r + 9 * k] * lift[k] * W[c + 9 * k];
+      out[r * 9 - (r * (r - 1)) / 2 + (c - r)] = s;
+    }
+}
+
+// ---- mass / external force / setup / spmv -----------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ext_force(MeshView m, double t0, double t1, double t2, double* __restrict__ f) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= m.nV) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int k = m.incPtr[v]; k < m.incPtr[v + 1]; ++k) {
+    const double w = 0.25 * m.vol[m.inc[k] >> 2];
+    f0 += t0 * w;
+    f1 += t1 * w;
+    f2 += t2 * w;
+  }
+  f[3ll * v] = f0;
+  f[3ll * v + 1] = f1;
+  f[3ll * v + 2] = f2;
+}
+
+__global__ void __launch_bounds__(256) k_dphidX(MeshView m, const double* __restrict__ V, double* __restrict__ dphidX,
+                                                double* __restrict__ vol) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int4 ev = m.tetE[e];
+  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
+  double x[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[a][c] = V[vid[a] + (int64_t)c * m.nV];
+  double A[9];  // columns = edges x_m - x_0
+#pragma unroll
+  for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c + 3 * mm] = x[mm + 1][c] - x[0][c];
+  double cA[9];
+  cof3(A, cA);
+  const double det = A[0] * cA[0] + A[3] * cA[3] + A[6] * cA[6];
+  const double inv = 1.0 / det;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < 3; ++mm) {
+      const double v = cA[j + 3 * mm] * inv;  // (A^{-1})(mm, j) = cof(A)(j, mm) / det
+      dphidX[e + (int64_t)((mm + 1) + 4 * j) * m.nT] = v;
+      s += v;
+    }
+    dphidX[e + (int64_t)(4 * j) * m.nT] = -s;
+  }
+  double ad[3], bd[3], cd[3], cr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ad[k] = x[0][k] - x[3][k];
+    bd[k] = x[1][k] - x[3][k];
+    cd[k] = x[2][k] - x[3][k];
+  }
+  cross3(bd, cd, cr);
+  vol[e] = -dot3(ad, cr) / 6.0;
+}
+
+// y = H x on the block pattern; one lane per row
+__global__ void __launch_bounds__(256)
+    k_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const double* __restrict__ vals,
+           const double* __restrict__ x, double* __restrict__ y) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= 3 * nV) return;
+  const int v = row / 3, l = row - 3 * v;
+  const int b0 = bptr[v], b1 = bptr[v + 1], nb = b1 - b0;
+  const double* vr = vals + 9ll * b0 + 3ll * l * nb;
+  double s = 0.0;
+  for (int r = 0; r < nb; ++r) {
+    const double* xp = x + 3ll * bcol[b0 + r];
+    s += vr[3 * r] * xp[0] + vr[3 * r + 1] * xp[1] + vr[3 * r + 2] * xp[2];
+  }
+  y[row] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+bool is_model(int model) { return model >= 0 && model <= 4; }
+
+int max_lds_bytes() {
+  static int lim = -1;
+  if (lim < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    lim = v;
+  }
+  return lim;
+}
+
+template <int MODEL, bool PSD, int VPW>
+int launch_hessian_vpw(const bh_mesh* m, const double* q, const double* params, const uint8_t* mids, PsdView psd,
+                       double* vals, hipStream_t s) {
+  const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
+  const size_t lds = sizeof(double) * (size_t)std::max(seg, 1);
+  auto kern = k_hessian<MODEL, PSD, VPW>;
+  static std::once_flag once;
+  std::call_once(once, [&] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              max_lds_bytes());
+  });
+  const int grid = (m->nV + VPW - 1) / VPW;
+  ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kGatherThreads), lds, s, view(m), q, params, mids, psd,
+                     m->opts.stretch_reference_layout, vals);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+template <int MODEL, bool PSD>
+int launch_hessian(const bh_mesh* m, const double* q, const double* params, const uint8_t* mids, PsdView psd,
+                   double* vals, hipStream_t s) {
+  const int lim = max_lds_bytes();
+  if (sizeof(double) * (size_t)m->maxSeg64 <= (size_t)lim / 2)
+    return launch_hessian_vpw<MODEL, PSD, 64>(m, q, params, mids, psd, vals, s);
+  if (sizeof(double) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
+    return launch_hessian_vpw<MODEL, PSD, 32>(m, q, params, mids, psd, vals, s);
+  if (sizeof(double) * (size_t)m->maxSeg16 <= (size_t)lim)
+    return launch_hessian_vpw<MODEL, PSD, 16>(m, q, params, mids, psd, vals, s);
+  return BH_UNSUPPORTED;  // vertex valence too high for the LDS gather
+}
+
+template <int MODEL>
+int run_psd_passes(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double tol, hipStream_t s) {
+  if (!m->psdFlag) {
+    BH_CHECK(hipMalloc(&m->psdFlag, std::max<size_t>(1, m->nT)));
+    BH_CHECK(hipMalloc(&m->psdList, sizeof(int32_t) * std::max<size_t>(1, m->nT)));
+    BH_CHECK(hipMalloc(&m->psdCount, sizeof(int32_t)));
+    BH_CHECK(hipMalloc(&m->psdDM, sizeof(double) * 45 * std::max<size_t>(1, m->nT)));
+  }
+  BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
+  const int grid = (m->nT + 255) / 256;
+  {
+    ProfScope ps(m, BH_KT_PSD_CLASSIFY, s);
+    hipLaunchKernelGGL(k_psd_classify<MODEL>, dim3(grid), dim3(256), 0, s, view(m), q, params, mids, tol,
+                       m->opts.stretch_reference_layout, m->psdFlag, m->psdList, m->psdCount);
+  }
+  BH_CHECK(hipGetLastError());
+  // the eigen pass is sized for the worst case and exits early past *count
+  const int grid2 = (m->nT + 63) / 64;
+  ProfScope ps(m, BH_KT_PSD_EIGEN, s);
+  hipLaunchKernelGGL(k_psd_eigen<MODEL>, dim3(grid2), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdList, m->psdCount, m->psdDM);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+template <int MODEL>
+int hessian_model(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, const bh_psd* psd,
+                  double* vals, hipStream_t s) {
+  if (psd && psd->enabled) {
+    int st = run_psd_passes<MODEL>(m, q, params, mids, psd->tol, s);
+    if (st != BH_OK) return st;
+    PsdView pv{std::max(psd->tol, 0.0) * 0.25, psd->tol, m->psdFlag, m->psdDM};
+    return launch_hessian<MODEL, true>(m, q, params, mids, pv, vals, s);
+  }
+  PsdView pv{0.0, 0.0, nullptr, nullptr};
+  return launch_hessian<MODEL, false>(m, q, params, mids, pv, vals, s);
+}
+
+int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
+                     const bh_psd* psd, double* vals, hipStream_t s) {
+  if (m->nV == 0) return BH_OK;
+  if (mids) return hessian_model<M_PER_TET>(m, q, params, mids, psd, vals, s);
+  switch (model) {
+    case M_NH: return hessian_model<M_NH>(m, q, params, mids, psd, vals, s);
+    case M_STVK: return hessian_model<M_STVK>(m, q, params, mids, psd, vals, s);
+    case M_COROT: return hessian_model<M_COROT>(m, q, params, mids, psd, vals, s);
+    case M_ARAP: return hessian_model<M_ARAP>(m, q, params, mids, psd, vals, s);
+    case M_NH_LEGACY: return hessian_model<M_NH_LEGACY>(m, q, params, mids, psd, vals, s);
+    default: return BH_BAD_ARG;
+  }
+}
+
+template <int MODEL>
+int launch_energy(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double* out, hipStream_t s) {
+  const int per = kEnergyBlock * kEnergyItems;
+  const int grid = (m->nT + per - 1) / per;
+  ProfScope ps(m, BH_KT_ENERGY, s);
+  if (grid > 0) {
+    hipLaunchKernelGGL(k_energy<MODEL>, dim3(grid), dim3(kEnergyBlock), 0, s, view(m), q, params, mids, m->partials);
+    BH_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, m->partials, grid, out);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params, double* out,
+                    hipStream_t s) {
+  if (mids) return launch_energy<M_PER_TET>(m, q, params, mids, out, s);
+  switch (model) {
+    case M_NH: return launch_energy<M_NH>(m, q, params, mids, out, s);
+    case M_STVK: return launch_energy<M_STVK>(m, q, params, mids, out, s);
+    case M_COROT: return launch_energy<M_COROT>(m, q, params, mids, out, s);
+    case M_ARAP: return launch_energy<M_ARAP>(m, q, params, mids, out, s);
+    case M_NH_LEGACY: return launch_energy<M_NH_LEGACY>(m, q, params, mids, out, s);
+    default: return BH_BAD_ARG;
+  }
+}
+
+template <int MODEL>
+int launch_gradient(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double* g, hipStream_t s) {
+  const int grid = (m->nV + 255) / 256;
+  if (grid == 0) return BH_OK;
+  ProfScope ps(m, BH_KT_GRADIENT, s);
+  hipLaunchKernelGGL(k_gradient<MODEL>, dim3(grid), dim3(256), 0, s, view(m), q, params, mids, g);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int gradient_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params, double* g,
+                      hipStream_t s) {
+  if (mids) return launch_gradient<M_PER_TET>(m, q, params, mids, g, s);
+  switch (model) {
+    case M_NH: return launch_gradient<M_NH>(m, q, params, mids, g, s);
+    case M_STVK: return launch_gradient<M_STVK>(m, q, params, mids, g, s);
+    case M_COROT: return launch_gradient<M_COROT>(m, q, params, mids, g, s);
+    case M_ARAP: return launch_gradient<M_ARAP>(m, q, params, mids, g, s);
+    case M_NH_LEGACY: return launch_gradient<M_NH_LEGACY>(m, q, params, mids, g, s);
+    default: return BH_BAD_ARG;
+  }
+}
+
+int check_model_args(int model, const uint8_t* mids) {
+  if (mids) return BH_OK;
+  return is_model(model) ? BH_OK : BH_BAD_ARG;
+}
+
+hipStream_t as_stream(bh_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// host-API staging: copy q / params / ids in, run, copy out
+int stage_inputs(bh_mesh* m, const double* q, const double* params, const uint8_t* ids, hipStream_t s) {
+  if (q) BH_CHECK(hipMemcpyAsync(m->stage_q, q, sizeof(double) * 3 * (size_t)m->nV, hipMemcpyHostToDevice, s));
+  if (params)
+    BH_CHECK(hipMemcpyAsync(m->stage_params, params, sizeof(double) * 2 * (size_t)m->nT, hipMemcpyHostToDevice, s));
+  if (ids) BH_CHECK(hipMemcpyAsync(m->stage_ids, ids, (size_t)m->nT, hipMemcpyHostToDevice, s));
+  return BH_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// host-side pattern construction
+namespace {
+
+int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* volume) {
+  const int nV = m->nV, nT = m->nT;
+  // incidence lists (ascending element order by construction)
+  std::vector<int32_t> incPtr(nV + 1, 0);
+  for (int e = 0; e < nT; ++e)
+    for (int a = 0; a < 4; ++a) incPtr[E[e + (int64_t)a * nT] + 1]++;
+  for (int v = 0; v < nV; ++v) incPtr[v + 1] += incPtr[v];
+  std::vector<int32_t> inc((size_t)4 * nT);
+  {
+    std::vector<int32_t> pos(incPtr.begin(), incPtr.end() - 1);
+    for (int e = 0; e < nT; ++e)
+      for (int a = 0; a < 4; ++a) {
+        const int v = E[e + (int64_t)a * nT];
+        inc[pos[v]++] = (e << 2) | a;
+      }
+  }
+  // neighbour lists (sorted, unique, including the vertex itself)
+  std::vector<int32_t> bptr(nV + 1, 0);
+  std::vector<int32_t> bcol;
+  bcol.reserve((size_t)nV * 16);
+  std::vector<int32_t> tmp;
+  for (int v = 0; v < nV; ++v) {
+    tmp.clear();
+    for (int k = incPtr[v]; k < incPtr[v + 1]; ++k) {
+      const int e = inc[k] >> 2;
+      for (int a = 0; a < 4; ++a) tmp.push_back(E[e + (int64_t)a * nT]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    if (tmp.size() > 255) return BH_UNSUPPORTED;  // ranks are packed in 8 bits
+    bcol.insert(bcol.end(), tmp.begin(), tmp.end());
+    bptr[v + 1] = (int32_t)bcol.size();
+  }
+  const int64_t nblocks = (int64_t)bcol.size();
+  if (9 * nblocks > (int64_t)INT32_MAX) return BH_UNSUPPORTED;  // int32 CSR indices
+  std::vector<uint32_t> incRank((size_t)4 * nT);
+  for (int v = 0; v < nV; ++v) {
+    const int32_t* b = bcol.data() + bptr[v];
+    const int32_t* bend = bcol.data() + bptr[v + 1];
+    for (int k = incPtr[v]; k < incPtr[v + 1]; ++k) {
+      const int e = inc[k] >> 2;
+      uint32_t packed = 0;
+      for (int a = 0; a < 4; ++a) {
+        const int u = E[e + (int64_t)a * nT];
+        const uint32_t r = (uint32_t)(std::lower_bound(b, bend, u) - b);
+        packed |= r << (8 * a);
+      }
+      incRank[k] = packed;
+    }
+  }
+  auto maxseg = [&](int vpw) {
+    int best = 0;
+    for (int v0 = 0; v0 < nV; v0 += vpw) best = std::max(best, 9 * (bptr[std::min(v0 + vpw, nV)] - bptr[v0]));
+    return best;
+  };
+  m->maxSeg64 = maxseg(64);
+  m->maxSeg32 = maxseg(32);
+  m->maxSeg16 = maxseg(16);
+  m->nblocks = nblocks;
+  m->nnz = 9 * nblocks;
+  // AoS tet records
+  std::vector<int4> tetE(nT);
+  std::vector<double> tetPhi((size_t)12 * nT);
+  for (int e = 0; e < nT; ++e) {
+    tetE[e] = make_int4(E[e], E[e + (int64_t)nT], E[e + 2 * (int64_t)nT], E[e + 3 * (int64_t)nT]);
+    for (int c = 0; c < 12; ++c) tetPhi[(size_t)12 * e + c] = dphidX[e + (int64_t)c * nT];
+  }
+  BH_CHECK(hipMalloc(&m->tetE, sizeof(int4) * std::max(1, nT)));
+  BH_CHECK(hipMalloc(&m->tetPhi, sizeof(double) * 12 * std::max(1, nT)));
+  BH_CHECK(hipMalloc(&m->vol, sizeof(double) * std::max(1, nT)));
+  BH_CHECK(hipMalloc(&m->incPtr, sizeof(int32_t) * (nV + 1)));
+  BH_CHECK(hipMalloc(&m->inc, sizeof(int32_t) * std::max<size_t>(1, inc.size())));
+  BH_CHECK(hipMalloc(&m->incRank, sizeof(uint32_t) * std::max<size_t>(1, incRank.size())));
+  BH_CHECK(hipMalloc(&m->bptr, sizeof(int32_t) * (nV + 1)));
+  BH_CHECK(hipMalloc(&m->bcol, sizeof(int32_t) * std::max<int64_t>(1, nblocks)));
+  if (nT) {
+    BH_CHECK(hipMemcpy(m->tetE, tetE.data(), sizeof(int4) * nT, hipMemcpyHostToDevice));
+    BH_CHECK(hipMemcpy(m->tetPhi, tetPhi.data(), sizeof(double) * 12 * (size_t)nT, hipMemcpyHostToDevice));
+    BH_CHECK(hipMemcpy(m->vol, volume, sizeof(double) * nT, hipMemcpyHostToDevice));
+    BH_CHECK(hipMemcpy(m->inc, inc.data(), sizeof(int32_t) * inc.size(), hipMemcpyHostToDevice));
+    BH_CHECK(hipMemcpy(m->incRank, incRank.data(), sizeof(uint32_t) * incRank.size(), hipMemcpyHostToDevice));
+  }
+  BH_CHECK(hipMemcpy(m->incPtr, incPtr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice));
+  BH_CHECK(hipMemcpy(m->bptr, bptr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice));
+  if (nblocks) BH_CHECK(hipMemcpy(m->bcol, bcol.data(), sizeof(int32_t) * nblocks, hipMemcpyHostToDevice));
+  const int per = kEnergyBlock * kEnergyItems;
+  m->nPartials = std::max(1, (nT + per - 1) / per);
+  BH_CHECK(hipMalloc(&m->partials, sizeof(double) * m->nPartials));
+  BH_CHECK(hipMalloc(&m->d_energy, sizeof(double)));
+  BH_CHECK(hipMalloc(&m->stage_q, sizeof(double) * 3 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&m->stage_params, sizeof(double) * 2 * std::max(1, nT)));
+  BH_CHECK(hipMalloc(&m->stage_out, sizeof(double) * 3 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&m->stage_ids, std::max(1, nT)));
+  m->h_bptr.swap(bptr);
+  m->h_bcol.swap(bcol);
+  BH_CHECK(hipGetDevice(&m->device));
+  return BH_OK;
+}
+
+void free_mesh(bh_mesh* m) {
+  clear_profile(m);
+  void* ptrs[] = {m->tetE, m->tetPhi, m->vol, m->incPtr, m->inc, m->incRank, m->bptr, m->bcol, m->partials,
+                  m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
+                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (m->own) (void)hipStreamDestroy(m->own);
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+extern "C" {
+
+int bh_linear_tetmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX, double* volume) {
+  if (nV < 0 || nT < 0 || (nV && !V) || (nT && (!E || !dphidX || !volume))) return BH_BAD_ARG;
+  int status = BH_OK;
+  for (int e = 0; e < nT; ++e) {
+    double x[4][3];
+    for (int a = 0; a < 4; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      if (v < 0 || v >= nV) return BH_BAD_ARG;
+      for (int c = 0; c < 3; ++c) x[a][c] = V[v + (int64_t)c * nV];
+    }
+    double A[9];
+    for (int mm = 0; mm < 3; ++mm)
+      for (int c = 0; c < 3; ++c) A[c + 3 * mm] = x[mm + 1][c] - x[0][c];
+    double c[9];
+    c[0] = A[4] * A[8] - A[7] * A[5];
+    c[1] = A[7] * A[2] - A[1] * A[8];
+    c[2] = A[1] * A[5] - A[4] * A[2];
+    c[3] = A[6] * A[5] - A[3] * A[8];
+    c[4] = A[0] * A[8] - A[6] * A[2];
+    c[5] = A[3] * A[2] - A[0] * A[5];
+    c[6] = A[3] * A[7] - A[6] * A[4];
+    c[7] = A[6] * A[1] - A[0] * A[7];
+    c[8] = A[0] * A[4] - A[3] * A[1];
+    const double det = A[0] * c[0] + A[3] * c[3] + A[6] * c[6];
+    const double inv = 1.0 / det;
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int mm = 0; mm < 3; ++mm) {
+        const double v = c[j + 3 * mm] * inv;
+        dphidX[e + (int64_t)((mm + 1) + 4 * j) * nT] = v;
+        s += v;
+      }
+      dphidX[e + (int64_t)(4 * j) * nT] = -s;
+    }
+    double ad[3], bd[3], cd[3];
+    for (int k = 0; k < 3; ++k) {
+      ad[k] = x[0][k] - x[3][k];
+      bd[k] = x[1][k] - x[3][k];
+      cd[k] = x[2][k] - x[3][k];
+    }
+    const double cr0 = bd[1] * cd[2] - bd[2] * cd[1], cr1 = bd[2] * cd[0] - bd[0] * cd[2],
+                 cr2 = bd[0] * cd[1] - bd[1] * cd[0];
+    volume[e] = -(ad[0] * cr0 + ad[1] * cr1 + ad[2] * cr2) / 6.0;
+    if (!std::isfinite(inv) || det == 0.0) status = BH_NONFINITE;
+  }
+  return status;
+}
+
+int bh_mesh_create(int nV, int nT, const int* E, const double* dphidX, const double* volume, bh_mesh** out) {
+  if (!out || nV < 0 || nT < 0 || (nT && (!E || !dphidX || !volume))) return BH_BAD_ARG;
+  *out = nullptr;
+  if ((int64_t)nT >= (1ll << 29)) return BH_UNSUPPORTED;  // element id packed with the corner in int32
+  for (int64_t i = 0; i < 4ll * nT; ++i)
+    if (E[i] < 0 || E[i] >= nV) return BH_BAD_ARG;
+  for (int64_t i = 0; i < 12ll * nT; ++i)
+    if (!std::isfinite(dphidX[i])) return BH_NONFINITE;
+  for (int64_t i = 0; i < nT; ++i)
+    if (!std::isfinite(volume[i])) return BH_NONFINITE;
+  bh_mesh* m = new bh_mesh();
+  m->nV = nV;
+  m->nT = nT;
+  const int st = build_mesh(m, E, dphidX, volume);
+  if (st != BH_OK) {
+    free_mesh(m);
+    delete m;
+    return st;
+  }
+  *out = m;
+  return BH_OK;
+}
+
+int bh_mesh_destroy(bh_mesh* m) {
+  if (!m) return BH_BAD_ARG;
+  (void)hipDeviceSynchronize();
+  free_mesh(m);
+  delete m;
+  return BH_OK;
+}
+
+int bh_mesh_nnz(const bh_mesh* m, int64_t* nnz) {
+  if (!m || !nnz) return BH_BAD_ARG;
+  *nnz = m->nnz;
+  return BH_OK;
+}
+
+int bh_mesh_pattern(const bh_mesh* m, int32_t* rowptr, int32_t* colind) {
+  if (!m || !rowptr) return BH_BAD_ARG;
+  int64_t off = 0;
+  rowptr[0] = 0;
+  for (int v = 0; v < m->nV; ++v) {
+    const int b0 = m->h_bptr[v], b1 = m->h_bptr[v + 1], nb = b1 - b0;
+    for (int l = 0; l < 3; ++l) {
+      if (colind)
+        for (int r = 0; r < nb; ++r)
+          for (int k = 0; k < 3; ++k) colind[off + 3 * r + k] = 3 * m->h_bcol[b0 + r] + k;
+      off += 3 * nb;
+      rowptr[3 * v + l + 1] = (int32_t)off;
+    }
+  }
+  return BH_OK;
+}
+
+int bh_mesh_block_pattern_device(const bh_mesh* m, const int32_t** d_bptr, const int32_t** d_bcol, int64_t* nblocks) {
+  if (!m) return BH_BAD_ARG;
+  if (d_bptr) *d_bptr = m->bptr;
+  if (d_bcol) *d_bcol = m->bcol;
+  if (nblocks) *nblocks = m->nblocks;
+  return BH_OK;
+}
+
+int bh_mesh_set_element_opts(bh_mesh* m, const bh_element_opts* o) {
+  if (!m || !o) return BH_BAD_ARG;
+  m->opts = *o;
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_dphi_dX_device(bh_mesh* m, const double* d_V, double* d_dphidX, double* d_vol, bh_stream_t s) {
+  if (!m || !d_V || !d_dphidX || !d_vol) return BH_BAD_ARG;
+  if (m->nT == 0) return BH_OK;
+  hipLaunchKernelGGL(k_dphidX, dim3((m->nT + 255) / 256), dim3(256), 0, as_stream(s), view(m), d_V, d_dphidX, d_vol);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_q_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
+                               double* d_energy, bh_stream_t s) {
+  if (!m || !q || !params || !d_energy || check_model_args(model, mids)) return BH_BAD_ARG;
+  return energy_dispatch(m, model, mids, q, params, d_energy, as_stream(s));
+}
+
+int bh_linear_tetmesh_dq_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
+                                double* d_g, bh_stream_t s) {
+  if (!m || !q || !params || !d_g || check_model_args(model, mids)) return BH_BAD_ARG;
+  return gradient_dispatch(m, model, mids, q, params, d_g, as_stream(s));
+}
+
+int bh_linear_tetmesh_dq2_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
+                                 const bh_psd* psd, double* d_vals, bh_stream_t s) {
+  if (!m || !q || !params || !d_vals || check_model_args(model, mids)) return BH_BAD_ARG;
+  return hessian_dispatch(m, model, mids, q, params, psd, d_vals, as_stream(s));
+}
+
+int bh_linear_tetmesh_q(bh_mesh* m, int model, const double* q, const double* params, double* energy) {
+  if (!m || !q || !params || !energy || !is_model(model)) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = stage_inputs(m, q, params, nullptr, s);
+  if (st) return st;
+  st = energy_dispatch(m, model, nullptr, m->stage_q, m->stage_params, m->d_energy, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(energy, m->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_dq(bh_mesh* m, int model, const double* q, const double* params, double* g) {
+  if (!m || !q || !params || !g || !is_model(model)) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = stage_inputs(m, q, params, nullptr, s);
+  if (st) return st;
+  st = gradient_dispatch(m, model, nullptr, m->stage_q, m->stage_params, m->stage_out, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(g, m->stage_out, sizeof(double) * 3 * (size_t)m->nV, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+static int ensure_stage_vals(bh_mesh* m) {
+  if (!m->stage_vals) BH_CHECK(hipMalloc(&m->stage_vals, sizeof(double) * std::max<int64_t>(1, m->nnz)));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_dq2(bh_mesh* m, int model, const double* q, const double* params, const bh_psd* psd,
+                          double* vals) {
+  if (!m || !q || !params || !vals || !is_model(model)) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = ensure_stage_vals(m);
+  if (st) return st;
+  st = stage_inputs(m, q, params, nullptr, s);
+  if (st) return st;
+  st = hessian_dispatch(m, model, nullptr, m->stage_q, m->stage_params, psd, m->stage_vals, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(vals, m->stage_vals, sizeof(double) * m->nnz, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_generic_elastic_q(bh_mesh* m, const uint8_t* ids, const double* q, const double* params,
+                                        double* energy) {
+  if (!m || !ids || !q || !params || !energy) return BH_BAD_ARG;
+  for (int e = 0; e < m->nT; ++e)
+    if (ids[e] > 4) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = stage_inputs(m, q, params, ids, s);
+  if (st) return st;
+  st = energy_dispatch(m, M_PER_TET, m->stage_ids, m->stage_q, m->stage_params, m->d_energy, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(energy, m->d_energy, sizeof(double), hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_generic_elastic_dq(bh_mesh* m, const uint8_t* ids, const double* q, const double* params,
+                                         double* g) {
+  if (!m || !ids || !q || !params || !g) return BH_BAD_ARG;
+  for (int e = 0; e < m->nT; ++e)
+    if (ids[e] > 4) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = stage_inputs(m, q, params, ids, s);
+  if (st) return st;
+  st = gradient_dispatch(m, M_PER_TET, m->stage_ids, m->stage_q, m->stage_params, m->stage_out, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(g, m->stage_out, sizeof(double) * 3 * (size_t)m->nV, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_generic_elastic_dq2(bh_mesh* m, const uint8_t* ids, const double* q, const double* params,
+                                          const bh_psd* psd, double* vals) {
+  if (!m || !ids || !q || !params || !vals) return BH_BAD_ARG;
+  for (int e = 0; e < m->nT; ++e)
+    if (ids[e] > 4) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = ensure_stage_vals(m);
+  if (st) return st;
+  st = stage_inputs(m, q, params, ids, s);
+  if (st) return st;
+  st = hessian_dispatch(m, M_PER_TET, m->stage_ids, m->stage_q, m->stage_params, psd, m->stage_vals, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(vals, m->stage_vals, sizeof(double) * m->nnz, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_mass_matrix_device(bh_mesh* m, const double* d_density, double* d_vals, bh_stream_t s) {
+  if (!m || !d_density || !d_vals) return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  // params column 0 carries the density; column 1 is never read for the mass pseudo-model
+  PsdView pv{0.0, 0.0, nullptr, nullptr};
+  return launch_hessian<M_MASS, false>(m, m->stage_q, d_density, nullptr, pv, d_vals, as_stream(s));
+}
+
+int bh_linear_tetmesh_const_ext_force_device(bh_mesh* m, const double* traction, double* d_f, bh_stream_t s) {
+  if (!m || !traction || !d_f) return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  hipLaunchKernelGGL(k_ext_force, dim3((m->nV + 255) / 256), dim3(256), 0, as_stream(s), view(m), traction[0],
+                     traction[1], traction[2], d_f);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int bh_mesh_set_profiling(bh_mesh* m, int on) {
+  if (!m) return BH_BAD_ARG;
+  if (on) clear_profile(m);
+  m->prof = on != 0;
+  return BH_OK;
+}
+
+int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
+  if (!m || !ms || !launches) return BH_BAD_ARG;
+  for (int k = 0; k < BH_KT_COUNT; ++k) {
+    double t = 0.0;
+    for (auto& e : m->profEv[k]) {
+      BH_CHECK(hipEventSynchronize(e.second));
+      float x = 0.f;
+      BH_CHECK(hipEventElapsedTime(&x, e.first, e.second));
+      t += x;
+    }
+    ms[k] = t;
+    launches[k] = (int64_t)m->profEv[k].size();
+  }
+  return BH_OK;
+}
+
+int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* d_y, bh_stream_t s) {
+  if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  hipLaunchKernelGGL(k_spmv, dim3((3 * m->nV + 255) / 256), dim3(256), 0, as_stream(s), m->nV, m->bptr, m->bcol,
+                     d_vals, d_x, d_y);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+}  // extern "C"
+
+// device Newton (search direction by PCG, reference line search)
+#include "newton_impl.h"

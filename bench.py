@@ -1,0 +1,201 @@
+#!/usr/bin/env python
+"""Benchmark of the linear-tetmesh hot path on one MI355X.
+
+Workload (one "step"): the Neo-Hookean Hessian of a 64x64x256-vertex beam (6,072,570 tets,
+138,917,358 nonzeros) with the random smooth bend of the at-scale configuration, including the
+simple_psd_fix(1e-3) element projection and the deterministic CSR assembly -- i.e. one call of
+bh_linear_tetmesh_dq2_device with inputs already resident in HBM.  ``value`` is tets per second
+(Mtets/s) over exactly K timed steps.
+
+Also reported (not part of ``value``): the same call without the PSD projection, the gradient and
+the energy kernels, the roofline of the dominant kernel (the assembly gather), measured live with
+HIP events on the stream the kernels run on, and the FP64 CPU oracle (the reference algorithm
+restated in C++: dense B^T H B, 144 triplets per tet, 12x12 Jacobi PSD, sort-and-fold) timed on a
+bounded slab of the same mesh.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def algorithmic_bytes(nV, nT, nnz):
+    """per-tet algorithmic bytes: E 16 + dphidX 96 + volume 8 + params 16, q once per vertex,
+    and for the Hessian every CSR value written once (metadata excluded)."""
+    b_in = 16 + 96 + 8 + 16
+    b_q = b_in + 24.0 * nV / nT
+    return dict(q=b_q, dq=b_q + 24.0 * nV / nT, dq2=b_q + 8.0 * nnz / nT)
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 counters (profiles/*.json), if committed"""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(name)
+    except Exception:
+        return None
+
+
+def cpu_baseline(p, dX, vol, budget_s):
+    """the FP64 oracle (reference algorithm, 1 thread) on leading slabs of the same mesh and deformation"""
+    from oracle import oracle as O
+
+    O.load()
+    nV = p.V.shape[0]
+    slab = 40000
+    done, t0 = 0, time.perf_counter()
+    while True:
+        sl = slice(done, done + slab)
+        O.tetmesh_dq2(0, nV, p.E[sl], p.q, dX[sl], vol[sl], p.params[sl], psd=True, psd_tol=1e-3)
+        done += slab
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= p.E.shape[0]:
+            break
+    return dict(value=done / el / 1e6, unit="Mtets/s", cores=1, kind="port",
+                sample=f"first {done} tets of the 64x64x256 bend, neohookean dq2 + simple_psd_fix(1e-3), "
+                       f"{el:.1f} s on one host core")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", default="256,64,64", help="vertices along x,y,z")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not code); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from bartels_amd import _lib, meshgen
+    from bartels_amd import tetmesh as tm
+
+    grid = tuple(int(x) for x in args.grid.split(","))
+    t_setup = time.perf_counter()
+    p = meshgen.config2(grid=grid)
+    dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
+    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol)
+    nV, nT, nnz = mesh.nV, mesh.nT, mesh.nnz
+    dev = torch.device("cuda:0")
+    q = torch.tensor(p.q, device=dev)
+    params = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device=dev)
+    H = torch.empty(nnz, dtype=torch.float64, device=dev)
+    g = torch.empty(3 * nV, dtype=torch.float64, device=dev)
+    e = torch.empty(1, dtype=torch.float64, device=dev)
+    psd = tm.simple_psd_fix(1e-3)
+    stream = torch.cuda.current_stream()
+    setup_s = time.perf_counter() - t_setup
+
+    def step():
+        mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, callback=psd, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.bh_mesh_set_profiling(mesh.handle, 1)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    ms = (np.zeros(5)).astype(np.float64)
+    cnt = np.zeros(5, dtype=np.int64)
+    lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
+    lib.bh_mesh_set_profiling(mesh.handle, 0)
+    kt = {name: (ms[i] / cnt[i] if cnt[i] else None) for i, name in enumerate(_lib.KERNEL_CLASSES)}
+
+    # secondary kernels, each timed on its own (not part of value)
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    ms_nopsd = timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
+    lib.bh_mesh_set_profiling(mesh.handle, 1)
+    timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
+    lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
+    lib.bh_mesh_set_profiling(mesh.handle, 0)
+    gather_nopsd = ms[2] / cnt[2]
+    ms_dq = timed(lambda: mesh.dq_device(tm.NEOHOOKEAN, q, params, g, stream=stream))
+    ms_q = timed(lambda: mesh.q_device(tm.NEOHOOKEAN, q, params, e, stream=stream))
+
+    B = algorithmic_bytes(nV, nT, nnz)
+    gather_ms = kt["hessian_gather"]
+    achieved = B["dq2"] * nT / (gather_ms * 1e-3) / 1e9
+    traffic = load_traffic("hessian_gather_psd")
+    out = {
+        "metric": "Mtets/s for neohookean_dq2 assembly",
+        "value": nT / (ms_step * 1e-3) / 1e6,
+        "unit": "Mtets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"neohookean_dq2 + simple_psd_fix(1e-3) + CSR assembly, {grid[1]}x{grid[2]}x{grid[0]} beam "
+                        f"with random smooth bend, {nT} tets, {nnz} nnz, FP64",
+            "grid": list(grid), "tets": nT, "vertices": nV, "nnz": nnz,
+        },
+        "roofline": {
+            "kernel": "k_hessian (assembly gather, PSD step)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_tet": B["dq2"],
+        },
+        "kernels_ms": {k: v for k, v in kt.items() if v is not None},
+        "extra": {
+            "dq2_no_psd_ms": ms_nopsd,
+            "dq2_no_psd_mtets_s": nT / (ms_nopsd * 1e-3) / 1e6,
+            "gather_no_psd_ms": gather_nopsd,
+            "gather_no_psd_hbm_frac": B["dq2"] * nT / (gather_nopsd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "dq_ms": ms_dq,
+            "dq_hbm_frac": B["dq"] * nT / (ms_dq * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "q_ms": ms_q,
+            "q_hbm_frac": B["q"] * nT / (ms_q * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "wall_s_timed": wall,
+            "setup_s": setup_s,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(p, dX, vol, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/* bartels_hip.h -- C ABI of the MI355X (gfx950) linear-tetmesh hyperelastic hot path.
+ *
+ * Plain pointers and sizes only.  All host arrays use the reference's Eigen default
+ * (column-major) layouts, so an Eigen caller passes .data() without copies:
+ *   V       nV x 3  double, column-major          (V(v,c) at v + c*nV)
+ *   E       nT x 4  int32,  column-major          (E(e,k) at e + k*nT)
+ *   q       3*nV    double, interleaved xyz       (q[3v+c])
+ *   dphidX  nT x 12 double, column-major; row e is the column-major flattening of the 4x3 matrix
+ *           of shape-function gradients: dphidX(e, m + 4j) = d phi_m / d X_j
+ *   volume  nT      double
+ *   params  nT x 2  double, column-major; per-model meaning:
+ *             BH_NEOHOOKEAN, BH_NEOHOOKEAN_LEGACY : [C, D]       = [mu/2, lambda/2]
+ *             BH_STVK, BH_COROTATIONAL            : [lambda, mu]
+ *             BH_ARAP                             : [k, unused]
+ * Sparse outputs use the reference's pattern: full 3x3 blocks for every pair of vertices that
+ * share a tet, explicit zeros kept, sorted int32 indices.  The pattern is symmetric, so the CSR
+ * arrays returned here equal the CSC arrays of Eigen::SparseMatrix<double> (outerIndexPtr,
+ * innerIndexPtr); values follow CSR order (row-major within the pattern).
+ *
+ * Every function returns a bh_status.  Nothing calls exit().  Device (_device) variants take
+ * device pointers and a hipStream_t and are asynchronous on that stream; host variants copy
+ * in/out and synchronise.
+ */
+#ifndef BARTELS_HIP_H
+#define BARTELS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* bh_stream_t; /* == hipStream_t */
+
+typedef enum {
+  BH_OK = 0,
+  BH_BAD_ARG = 1,
+  BH_NONFINITE = 2,
+  BH_SOLVE_FAILED = 3,
+  BH_MAXIT = 4,
+  BH_DEVICE_ERROR = 5,
+  BH_UNSUPPORTED = 6
+} bh_status;
+
+typedef enum {
+  BH_NEOHOOKEAN = 0,        /* psi = C(tr F^T F - 3 - 2 ln J) + D (J-1)^2   src/psi_neohookean_F.cpp:31 */
+  BH_STVK = 1,              /* psi = mu E:E + lambda/2 (tr E)^2            src/psi_stvk_F.cpp:5-13 */
+  BH_COROTATIONAL = 2,      /* psi = mu sum (s-1)^2 + lambda/2 (sum s - 3)^2  src/psi_corotational_S.cpp:5-9 */
+  BH_ARAP = 3,              /* psi = k sum (s-1)^2                         src/psi_arap_S.cpp:5-11 */
+  BH_NEOHOOKEAN_LEGACY = 4, /* psi = C(J^{-2/3} tr F^T F - 3) + D (J-1)^2  src/psi_neohookean_F.cpp:8-14 (commented form) */
+  BH_MODEL_PER_TET = 255    /* per-tet model ids passed separately (generic_elastic) */
+} bh_model;
+
+/* The element-matrix callback every in-repo caller uses: simple_psd_fix(K, tol) on the
+ * volume-scaled 12x12 (src/simple_psd_fix.cpp:5-21; MATLAB passes tol = 1e-3).
+ * NULL or enabled == 0 means the reference's default_callback (no-op). */
+typedef struct {
+  int enabled;
+  double tol;
+} bh_psd;
+
+/* Options that are not part of the reference's argument lists. */
+typedef struct {
+  /* Stretch models only (corotational, ARAP).  1 (default): reproduce the reference's row-major
+   * write of the 9x9 (src/d2psi_stretch_dF2.cpp:36-45), i.e. the Hessian evaluated at F^T.
+   * 0: the true Hessian at F. */
+  int stretch_reference_layout;
+} bh_element_opts;
+
+typedef struct bh_mesh bh_mesh;
+
+/* ---- setup ------------------------------------------------------------------------------ */
+
+/* Replaces sim::linear_tetmesh_dphi_dX (src/linear_tetmesh_dphi_dX.cpp:5-18,
+ * src/linear_tet_dphi_dX.cpp:5-18) and the signed volume igl::volume
+ * (tests/test_assemble.cpp:27): vol = -(a-d).((b-d)x(c-d))/6.  Host arrays. */
+int bh_linear_tetmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX, double* volume);
+
+/* Same, computed on the device (device pointers; V column-major nV x 3). */
+int bh_linear_tetmesh_dphi_dX_device(bh_mesh* mesh, const double* d_V, double* d_dphidX, double* d_volume,
+                                     bh_stream_t stream);
+
+/* Builds the per-mesh device state once: the reference pattern (CSR == CSC), the vertex -> incident
+ * tet lists in ascending element order and the block ranks that map each element block onto its
+ * CSR slots.  Copies E, dphidX and volume (host arrays, reference layouts) to the device.
+ * Replaces the per-call triplet list of sim::assemble (src/assemble.cpp:36-97). */
+int bh_mesh_create(int nV, int nT, const int* E, const double* dphidX, const double* volume, bh_mesh** mesh);
+int bh_mesh_destroy(bh_mesh* mesh);
+int bh_mesh_nnz(const bh_mesh* mesh, int64_t* nnz);
+/* rowptr: 3*nV + 1 entries, colind: nnz entries (host arrays). */
+int bh_mesh_pattern(const bh_mesh* mesh, int32_t* rowptr, int32_t* colind);
+/* device copies of the block pattern (vertex level): bptr nV+1, bcol nblocks; returned pointers
+ * are owned by the mesh. */
+int bh_mesh_block_pattern_device(const bh_mesh* mesh, const int32_t** d_bptr, const int32_t** d_bcol,
+                                 int64_t* nblocks);
+int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
+
+/* ---- energy / gradient / Hessian (host arrays) -------------------------------------------- */
+
+/* Replaces sim::linear_tetmesh_{neohookean,stvk,corotational,arap}_q
+ * (include/linear_tetmesh_neohookean_q.h:13-18, src/linear_tetmesh_neohookean_q.cpp:8-24 and
+ * siblings).  V is implied by the mesh.  nT == 0 gives 0. */
+int bh_linear_tetmesh_q(bh_mesh* mesh, int model, const double* q, const double* params, double* energy);
+/* Replaces sim::linear_tetmesh_*_dq (src/linear_tetmesh_neohookean_dq.cpp:5-22, ...).  g: 3*nV. */
+int bh_linear_tetmesh_dq(bh_mesh* mesh, int model, const double* q, const double* params, double* g);
+/* Replaces sim::linear_tetmesh_*_dq2 (include/linear_tetmesh_neohookean_dq2.h:14-21,
+ * src/linear_tetmesh_neohookean_dq2.cpp:8-28, ...).  vals: nnz values in the pattern of
+ * bh_mesh_pattern; psd NULL = default_callback. */
+int bh_linear_tetmesh_dq2(bh_mesh* mesh, int model, const double* q, const double* params, const bh_psd* psd,
+                          double* vals);
+
+/* Replaces sim::linear_tetmesh_generic_elastic_{q,dq,dq2} (src/linear_tetmesh_generic_elastic_q.cpp:5-30,
+ * _dq.cpp:5-36, _dq2.cpp:8-38) for the built-in models: the user functor becomes a per-tet model id
+ * (uint8, bh_model values 0..4), params stay nT x 2 per tet. */
+int bh_linear_tetmesh_generic_elastic_q(bh_mesh* mesh, const uint8_t* model_ids, const double* q,
+                                        const double* params, double* energy);
+int bh_linear_tetmesh_generic_elastic_dq(bh_mesh* mesh, const uint8_t* model_ids, const double* q,
+                                         const double* params, double* g);
+int bh_linear_tetmesh_generic_elastic_dq2(bh_mesh* mesh, const uint8_t* model_ids, const double* q,
+                                          const double* params, const bh_psd* psd, double* vals);
+
+/* ---- device variants (device pointers, asynchronous on stream) ----------------------------- */
+/* model_ids may be NULL (then model is used for every tet; model must not be BH_MODEL_PER_TET). */
+int bh_linear_tetmesh_q_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                               const double* d_params, double* d_energy, bh_stream_t stream);
+int bh_linear_tetmesh_dq_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                const double* d_params, double* d_g, bh_stream_t stream);
+int bh_linear_tetmesh_dq2_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                 const double* d_params, const bh_psd* psd, double* d_vals, bh_stream_t stream);
+
+/* ---- objective terms (config objectives) ---------------------------------------------------- */
+
+/* Replaces sim::linear_tetmesh_mass_matrix (src/linear_tetmesh_mass_matrix.cpp:7-24,
+ * src/linear_tet_mass_matrix.cpp:8-44): m/10 diagonal, m/20 off-diagonal identity blocks,
+ * same pattern as the Hessian.  density: nT. */
+int bh_linear_tetmesh_mass_matrix_device(bh_mesh* mesh, const double* d_density, double* d_vals,
+                                         bh_stream_t stream);
+/* Replaces sim::linear_tetmesh_const_ext_force (src/linear_tetmesh_const_ext_force.cpp:5-18):
+ * f = sum_e traction * vol_e / 4 on each vertex of e.  traction: 3 doubles (host). */
+int bh_linear_tetmesh_const_ext_force_device(bh_mesh* mesh, const double* traction, double* d_f,
+                                             bh_stream_t stream);
+
+/* ---- Newton ------------------------------------------------------------------------------------ */
+
+typedef enum {
+  BH_PRECOND_JACOBI = 0, /* 3x3 block-Jacobi (block diagonal of H) */
+  BH_PRECOND_POINT_JACOBI = 1
+} bh_precond;
+
+typedef struct {
+  int model;               /* bh_model, or BH_MODEL_PER_TET with model_ids */
+  const uint8_t* d_model_ids;
+  bh_psd psd;              /* PSD fix applied to every element Hessian */
+  double gradient_tol;     /* reference default 1e-3 (absolute, on |g|_2) */
+  int relative_tol;        /* 0: |g| < tol (reference); 1: |g| < tol * |g_0| */
+  int max_iterations;      /* reference default 100; the loop runs up to max+1 times */
+  int max_iterations_ls;   /* reference default 100 */
+  double alpha;            /* 1.0 */
+  double c;                /* 1e-8 */
+  double p;                /* 0.5 */
+  int armijo_with_alpha;   /* 0: reference test f(x+a d) > f(x) + c g^T d; 1: textbook c a g^T d */
+  int reject_nonfinite;    /* 1: a non-finite trial energy counts as "not decreased" (the reference accepts it) */
+  double pcg_tol;          /* relative residual of the inner PCG solve */
+  int pcg_max_iterations;
+  int precond;             /* bh_precond */
+} bh_newton_opts;
+
+typedef struct {
+  int iterations;          /* Newton iterations run */
+  int converged;           /* 1 = SolverExitStatus::CONVERGED */
+  int pcg_iterations_total;
+  int pcg_failures;        /* inner solves that hit their cap */
+  int ls_iterations_total;
+  double energy;           /* final objective */
+  double gradient_norm;    /* final |g| */
+  double gradient_norm0;
+} bh_newton_report;
+
+/* Fills opts with the reference defaults (include/optimization_gradient_based.h:51-58). */
+void bh_newton_default_opts(bh_newton_opts* opts);
+
+/* Static equilibrium on the device: minimise Psi(q) - f_ext^T q over the DOFs with
+ * fixed_dof[i] == 0 (fixed DOFs keep their input values).  Replaces
+ * sim::newtons_method_bisection (include/optimization_gradient_based.h:33-110) with
+ * search_direction_newton_sparse (src/search_direction_newton_sparse.cpp:5-26; the direct solve
+ * becomes a preconditioned CG to opts->pcg_tol) and linesearch_backtracking_bisection
+ * (src/linesearch_backtracking_bisection.cpp:5-31).  All pointers are device pointers.
+ * d_f_ext may be NULL.  trace (host, optional, 4*(max_iterations+1) doubles):
+ * per iteration [objective, |g|, alpha, pcg iterations]. */
+int bh_newton_static_device(bh_mesh* mesh, const bh_newton_opts* opts, const double* d_params,
+                            const uint8_t* d_fixed_dof, const double* d_f_ext, double* d_q, bh_newton_report* report,
+                            double* trace, bh_stream_t stream);
+
+/* ---- instrumentation ---------------------------------------------------------------------------- */
+
+/* Kernel classes timed with HIP events (recorded on the launch stream) while profiling is on. */
+typedef enum {
+  BH_KT_ENERGY = 0,
+  BH_KT_GRADIENT = 1,
+  BH_KT_HESSIAN_GATHER = 2, /* the CSR assembly kernel (element blocks + deterministic gather) */
+  BH_KT_PSD_CLASSIFY = 3,
+  BH_KT_PSD_EIGEN = 4,
+  BH_KT_COUNT = 5
+} bh_kernel_class;
+
+/* on != 0 starts recording (and clears previous records); on == 0 stops. */
+int bh_mesh_set_profiling(bh_mesh* mesh, int on);
+/* Synchronises the recorded events and returns, per kernel class, the summed duration in ms and the
+ * number of launches since profiling was switched on.  ms, launches: BH_KT_COUNT entries. */
+int bh_mesh_kernel_times(bh_mesh* mesh, double* ms, int64_t* launches);
+
+/* Device sparse helpers on the mesh pattern (values in bh_mesh_pattern order). */
+int bh_spmv_device(bh_mesh* mesh, const double* d_vals, const double* d_x, double* d_y, bh_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BARTELS_HIP_H */
