@@ -84,12 +84,17 @@ __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __r
   const int b0 = bptr[v], nb = bptr[v + 1] - b0;
   int r = 0;
   while (r < nb && bcol[b0 + r] != v) ++r;
-  double A[9];  // row-major block
-  const double* base = vals + 9ll * b0 + 3ll * r;
+  double A[9];  // row-major block; a vertex that belongs to no tet gets the identity
+  if (r < nb) {
+    const double* base = vals + 9ll * b0 + 3ll * r;
 #pragma unroll
-  for (int l = 0; l < 3; ++l)
+    for (int l = 0; l < 3; ++l)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) A[3 * l + k] = base[3ll * l * nb + k];
+      for (int k = 0; k < 3; ++k) A[3 * l + k] = base[3ll * l * nb + k];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
   if (point) {
     A[1] = A[2] = A[3] = A[5] = A[6] = A[7] = 0.0;
   }
