@@ -97,6 +97,9 @@ SIGNATURES = {
     "bh_newton_default_opts": (None, [P(BhNewtonOpts)]),
     "bh_newton_static_device": (c_int, [_VP, P(BhNewtonOpts), _VP, _VP, _VP, _VP, P(BhNewtonReport), _VP, _VP]),
     "bh_spmv_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_q_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dq_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_dq2_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),
 }

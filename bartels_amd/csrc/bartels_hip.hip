@@ -55,6 +55,8 @@ struct bh_mesh {
   int4* tetE = nullptr;
   double* tetPhi = nullptr;
   double* vol = nullptr;
+  float* tetPhi32 = nullptr;  // precision form (made on first use)
+  float* vol32 = nullptr;
   int32_t* incPtr = nullptr;
   int32_t* inc = nullptr;
   uint32_t* incRank = nullptr;
@@ -76,6 +78,7 @@ struct bh_mesh {
   int32_t* psdList = nullptr;
   int32_t* psdCount = nullptr;
   double* psdDM = nullptr;  // nT x 45
+  float* psdDM32 = nullptr;
   // solver workspace (lazily allocated)
   double* ws = nullptr;
   size_t wsBytes = 0;
@@ -120,7 +123,9 @@ struct MeshView {
   int nV, nT;
   const int4* __restrict__ tetE;
   const double* __restrict__ tetPhi;
+  const float* __restrict__ tetPhi32;
   const double* __restrict__ vol;
+  const float* __restrict__ vol32;
   const int32_t* __restrict__ incPtr;
   const int32_t* __restrict__ inc;
   const uint32_t* __restrict__ incRank;
@@ -129,48 +134,86 @@ struct MeshView {
 };
 
 MeshView view(const bh_mesh* m) {
-  return MeshView{m->nV, m->nT, m->tetE, m->tetPhi, m->vol, m->incPtr, m->inc, m->incRank, m->bptr, m->bcol};
+  return MeshView{m->nV, m->nT, m->tetE, m->tetPhi, m->tetPhi32, m->vol, m->vol32, m->incPtr, m->inc, m->incRank,
+                  m->bptr, m->bcol};
 }
 
+template <typename T>
 struct PsdView {
-  double tolPos;  // max(tol, 0) / 4: clamp of the three translation eigenvalues, per block
-  double tol;
+  T tolPos;  // max(tol, 0) / 4: clamp of the three translation eigenvalues, per block
+  T tol;
   const uint8_t* __restrict__ flag;
-  const double* __restrict__ dM;
+  const T* __restrict__ dM;
 };
 
 // ---- tet loads -------------------------------------------------------------------------------
-__device__ __forceinline__ void load_phi(const double* __restrict__ tetPhi, int e, double phi[4][3]) {
-  const double2* p2 = reinterpret_cast<const double2*>(tetPhi + 12ll * e);
-  double r[12];
+template <typename T>
+__device__ __forceinline__ void load_phi(const MeshView& m, int e, T phi[4][3]) {
+  T r[12];
+  if constexpr (sizeof(T) == 8) {
+    const double2* p2 = reinterpret_cast<const double2*>(m.tetPhi + 12ll * e);
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const double2 t = p2[i];
-    r[2 * i] = t.x;
-    r[2 * i + 1] = t.y;
+    for (int i = 0; i < 6; ++i) {
+      const double2 t = p2[i];
+      r[2 * i] = t.x;
+      r[2 * i + 1] = t.y;
+    }
+  } else {
+    const float4* p4 = reinterpret_cast<const float4*>(m.tetPhi32 + 12ll * e);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float4 t = p4[i];
+      r[4 * i] = t.x;
+      r[4 * i + 1] = t.y;
+      r[4 * i + 2] = t.z;
+      r[4 * i + 3] = t.w;
+    }
   }
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) phi[m][j] = r[m + 4 * j];
+    for (int j = 0; j < 3; ++j) phi[mm][j] = r[mm + 4 * j];
 }
 
-// F = sum_m x_m phi_m^T  (the reference's B * qe, include/flatten_multiply.h:52-75)
-__device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, const double phi[4][3], double* F) {
+template <typename T>
+__device__ __forceinline__ T load_vol(const MeshView& m, int e) {
+  if constexpr (sizeof(T) == 8)
+    return m.vol[e];
+  else
+    return m.vol32[e];
+}
+
+// FP64: F = sum_m x_m phi_m^T  (the reference's B * qe, include/flatten_multiply.h:52-75)
+// FP32: F = sum_{m>=1} (x_m - x_0) phi_m^T with the edges formed in FP64 before the cast, so the
+//       beam's absolute coordinates (metres) do not swamp the millimetre edges in FP32
+template <typename T>
+__device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, const T phi[4][3], T* F) {
   const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
   double x[4][3];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const double* qp = q + 3ll * vid[m];
-    x[m][0] = qp[0];
-    x[m][1] = qp[1];
-    x[m][2] = qp[2];
+  for (int mm = 0; mm < 4; ++mm) {
+    const double* qp = q + 3ll * vid[mm];
+    x[mm][0] = qp[0];
+    x[mm][1] = qp[1];
+    x[mm][2] = qp[2];
   }
+  if constexpr (sizeof(T) == 8) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j] + x[3][i] * phi[3][j];
+      for (int j = 0; j < 3; ++j)
+        F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j] + x[3][i] * phi[3][j];
+  } else {
+    T ed[3][3];
+#pragma unroll
+    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(x[mm + 1][i] - x[0][i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+  }
 }
 
 template <int MODEL>
@@ -178,13 +221,18 @@ __device__ __forceinline__ int model_at(const uint8_t* __restrict__ mids, int e)
   return MODEL == M_PER_TET ? (int)mids[e] : MODEL;
 }
 
+template <typename T>
+__device__ __forceinline__ void select_row(const T phi[4][3], int a, T* pa) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
+}
+
 // ---- energy ----------------------------------------------------------------------------------
-template <int MODEL>
+template <int MODEL, typename T>
 __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __restrict__ q,
-                                             const double* __restrict__ params, const uint8_t* __restrict__ mids,
-                                             int e) {
-  double phi[4][3], F[9];
-  load_phi(m.tetPhi, e, phi);
+                                             const T* __restrict__ params, const uint8_t* __restrict__ mids, int e) {
+  T phi[4][3], F[9];
+  load_phi(m, e, phi);
   load_F(q, m.tetE[e], phi, F);
   const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
   double v;
@@ -195,7 +243,7 @@ __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __
     case M_ARAP: v = psi<M_ARAP>(F, p0, p1); break;
     default: v = psi<M_NH_LEGACY>(F, p0, p1); break;
   }
-  return v * m.vol[e];
+  return v * (double)load_vol<T>(m, e);
 }
 
 __device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
@@ -213,9 +261,9 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
   return r;
 }
 
-template <int MODEL>
+template <int MODEL, typename T>
 __global__ void __launch_bounds__(kEnergyBlock)
-    k_energy(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+    k_energy(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
              const uint8_t* __restrict__ mids, double* __restrict__ partials) {
   __shared__ double sh[kEnergyBlock / 64];
   const int base = blockIdx.x * kEnergyBlock * kEnergyItems + threadIdx.x;
@@ -238,10 +286,10 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restri
   if (threadIdx.x == 0) *out = s;
 }
 
-// ---- gradient (vertex gather, ascending element order per vertex) ------------------------------
-template <int MODEL>
+// ---- gradient (vertex gather, ascending element order per vertex; FP64 accumulation) ----------
+template <int MODEL, typename T>
 __global__ void __launch_bounds__(256)
-    k_gradient(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+    k_gradient(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                const uint8_t* __restrict__ mids, double* __restrict__ g) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= m.nV) return;
@@ -250,10 +298,10 @@ __global__ void __launch_bounds__(256)
   for (int k = m.incPtr[v]; k < k1; ++k) {
     const int code = m.inc[k];
     const int e = code >> 2, a = code & 3;
-    double phi[4][3], F[9], P[9];
-    load_phi(m.tetPhi, e, phi);
+    T phi[4][3], F[9], P[9];
+    load_phi(m, e, phi);
     load_F(q, m.tetE[e], phi, F);
-    const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+    const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
     switch (model_at<MODEL>(mids, e)) {
       case M_NH: piola<M_NH>(F, p0, p1, P); break;
       case M_STVK: piola<M_STVK>(F, p0, p1, P); break;
@@ -261,16 +309,13 @@ __global__ void __launch_bounds__(256)
       case M_ARAP: piola<M_ARAP>(F, p0, p1, P); break;
       default: piola<M_NH_LEGACY>(F, p0, p1, P); break;
     }
-    const double vl = m.vol[e];
-    double pa[3];
-    // phi row a, selected without dynamic register indexing
-#pragma unroll
-    for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
-    double f[3];
+    const T vl = load_vol<T>(m, e);
+    T pa[3], f[3];
+    select_row(phi, a, pa);
     mv3(P, pa, f);  // (B^T vec P)_a = P phi_a
-    g0 += vl * f[0];
-    g1 += vl * f[1];
-    g2 += vl * f[2];
+    g0 += (double)(vl * f[0]);
+    g1 += (double)(vl * f[1]);
+    g2 += (double)(vl * f[2]);
   }
   g[3ll * v] = g0;
   g[3ll * v + 1] = g1;
@@ -278,72 +323,72 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- Hessian (vertex gather into LDS) -----------------------------------------------------------
-template <int MODEL>
+template <int MODEL, typename T>
 __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __restrict__ q,
-                                           const double* __restrict__ params, int e, int a, bool stretch_ref,
-                                           double blk[4][9]) {
+                                           const T* __restrict__ params, int e, int a, bool stretch_ref,
+                                           T blk[4][9]) {
   if (MODEL == M_MASS) {
     // consistent mass (src/linear_tet_mass_matrix.cpp:8-44); params[e] = density, q unused
-    const double mass = params[e] * m.vol[e];
+    const T mass = params[e] * load_vol<T>(m, e);
 #pragma unroll
     for (int mm = 0; mm < 4; ++mm) {
-      const double c = (mm == a) ? mass * (1.0 / 10.0) : mass * (1.0 / 20.0);
+      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
 #pragma unroll
-      for (int i = 0; i < 9; ++i) blk[mm][i] = (i % 4 == 0) ? c : 0.0;
+      for (int i = 0; i < 9; ++i) blk[mm][i] = (i % 4 == 0) ? c : T(0);
     }
     return;
   }
-  double phi[4][3];
-  Pre P;
-  load_phi(m.tetPhi, e, phi);
+  T phi[4][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
   load_F(q, m.tetE[e], phi, P.F);
-  P.vol = m.vol[e];
-  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
-  double pa[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  T pa[3];
+  select_row(phi, a, pa);
   pre_hessian<MODEL>(P, p0, p1, stretch_ref);
 #pragma unroll
   for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
 }
 
 // Q (4x3, orthonormal columns, orthogonal to the ones vector) of the thin QR of Phi
-__device__ __forceinline__ void phi_qr(const double phi[4][3], double Q[4][3], double R[3][3]) {
-  double c0[4], c1[4], c2[4];
+template <typename T>
+__device__ __forceinline__ void phi_qr(const T phi[4][3], T Q[4][3], T R[3][3]) {
+  T c0[4], c1[4], c2[4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    c0[m] = phi[m][0];
-    c1[m] = phi[m][1];
-    c2[m] = phi[m][2];
+  for (int mm = 0; mm < 4; ++mm) {
+    c0[mm] = phi[mm][0];
+    c1[mm] = phi[mm][1];
+    c2[mm] = phi[mm][2];
   }
-  auto d4 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; };
-  const double r00 = sqrt(d4(c0, c0));
+  auto d4 = [](const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; };
+  const T r00 = sqrt(d4(c0, c0));
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c0[m] /= r00;
-  const double r01 = d4(c0, c1);
+  for (int mm = 0; mm < 4; ++mm) c0[mm] /= r00;
+  const T r01 = d4(c0, c1);
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c1[m] -= r01 * c0[m];
-  const double r11 = sqrt(d4(c1, c1));
+  for (int mm = 0; mm < 4; ++mm) c1[mm] -= r01 * c0[mm];
+  const T r11 = sqrt(d4(c1, c1));
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c1[m] /= r11;
-  const double r02 = d4(c0, c2);
+  for (int mm = 0; mm < 4; ++mm) c1[mm] /= r11;
+  const T r02 = d4(c0, c2);
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c2[m] -= r02 * c0[m];
-  const double r12 = d4(c1, c2);
+  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r02 * c0[mm];
+  const T r12 = d4(c1, c2);
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c2[m] -= r12 * c1[m];
-  const double r22 = sqrt(d4(c2, c2));
+  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r12 * c1[mm];
+  const T r22 = sqrt(d4(c2, c2));
 #pragma unroll
-  for (int m = 0; m < 4; ++m) c2[m] /= r22;
+  for (int mm = 0; mm < 4; ++mm) c2[mm] /= r22;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    Q[m][0] = c0[m];
-    Q[m][1] = c1[m];
-    Q[m][2] = c2[m];
+  for (int mm = 0; mm < 4; ++mm) {
+    Q[mm][0] = c0[mm];
+    Q[mm][1] = c1[mm];
+    Q[mm][2] = c2[mm];
   }
   R[0][0] = r00; R[0][1] = r01; R[0][2] = r02;
-  R[1][0] = 0.0; R[1][1] = r11; R[1][2] = r12;
-  R[2][0] = 0.0; R[2][1] = 0.0; R[2][2] = r22;
+  R[1][0] = T(0); R[1][1] = r11; R[1][2] = r12;
+  R[2][0] = T(0); R[2][1] = T(0); R[2][2] = r22;
 }
 
 // packed upper-triangular index of a symmetric 9x9
@@ -352,28 +397,29 @@ __device__ __forceinline__ int sym9(int i, int j) {
   return a * 9 - (a * (a - 1)) / 2 + (b - a);
 }
 
-template <int MODEL, bool PSD, int VPW>
+template <int MODEL, bool PSD, int VPW, typename T>
 __global__ void __launch_bounds__(kGatherThreads)
-    k_hessian(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
-              const uint8_t* __restrict__ mids, PsdView psd, int stretchRef, double* __restrict__ vals) {
-  extern __shared__ double acc[];
+    k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+              const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, T* __restrict__ vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
   const int v0 = blockIdx.x * VPW;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
   const int seglen = 9 * (m.bptr[vend] - b0);
-  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
   __syncthreads();
   const int v = v0 + (int)threadIdx.x;
   if ((int)threadIdx.x < VPW && v < m.nV) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
-    double* row = acc + 9 * (m.bptr[v] - b0);
+    T* row = acc + 9 * (m.bptr[v] - b0);
     const int k1 = m.incPtr[v + 1];
     for (int k = m.incPtr[v]; k < k1; ++k) {
       const int code = m.inc[k];
       const int e = code >> 2, a = code & 3;
       const uint32_t rk = m.incRank[k];
-      double blk[4][9];
+      T blk[4][9];
       switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
         case M_STVK: tet_blocks<M_STVK>(m, q, params, e, a, stretchRef, blk); break;
@@ -391,34 +437,33 @@ __global__ void __launch_bounds__(kGatherThreads)
         }
         if (psd.flag[e]) {
           // + Q dM Q^T restricted to block row a:  sum_{p,r} Q[a][p] dM_{pr} Q[m][r]
-          double phi[4][3], Q[4][3], R[3][3];
-          load_phi(m.tetPhi, e, phi);
+          T phi[4][3], Q[4][3], R[3][3];
+          load_phi(m, e, phi);
           phi_qr(phi, Q, R);
-          const double* dM = psd.dM + 45ll * e;
-          double qa[3];
-#pragma unroll
-          for (int p = 0; p < 3; ++p) qa[p] = (a == 0) ? Q[0][p] : (a == 1) ? Q[1][p] : (a == 2) ? Q[2][p] : Q[3][p];
-          double T[3][9];  // T[r] = sum_p qa[p] dM_{p r} (3x3 sub-blocks, row-major l,k)
+          const T* dM = psd.dM + 45ll * e;
+          T qa[3];
+          select_row(Q, a, qa);
+          T Tm[3][9];  // Tm[r] = sum_p qa[p] dM_{p r} (3x3 sub-blocks, row-major l,k)
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int l = 0; l < 3; ++l)
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk) {
-                double s = 0.0;
+                T s = T(0);
 #pragma unroll
                 for (int p = 0; p < 3; ++p) s += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
-                T[r][3 * l + kk] = s;
+                Tm[r][3 * l + kk] = s;
               }
 #pragma unroll
           for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-            for (int i = 0; i < 9; ++i) blk[mm][i] += Q[mm][0] * T[0][i] + Q[mm][1] * T[1][i] + Q[mm][2] * T[2][i];
+            for (int i = 0; i < 9; ++i) blk[mm][i] += Q[mm][0] * Tm[0][i] + Q[mm][1] * Tm[1][i] + Q[mm][2] * Tm[2][i];
         }
       }
 #pragma unroll
       for (int mm = 0; mm < 4; ++mm) {
-        double* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
@@ -436,23 +481,22 @@ __global__ void __launch_bounds__(kGatherThreads)
 // psd(K) = K + (Q x I3)(M' - M)(Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  M' = M with eigenvalues
 // clamped at tol.  Pass 1 flags tets whose M - tol I is not positive definite (Cholesky);
 // pass 2 eigen-decomposes M (cyclic Jacobi) for the flagged tets only and stores dM = M' - M.
-template <int MODEL>
+template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __restrict__ q,
-                                          const double* __restrict__ params, int e, bool stretchRef,
-                                          double M[81]) {
-  double phi[4][3], Q[4][3], R[3][3];
-  Pre P;
-  load_phi(m.tetPhi, e, phi);
+                                          const T* __restrict__ params, int e, bool stretchRef, T M[81]) {
+  T phi[4][3], Q[4][3], R[3][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
   load_F(q, m.tetE[e], phi, P.F);
-  P.vol = m.vol[e];
-  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
   pre_hessian<MODEL>(P, p0, p1, stretchRef);
 #pragma unroll
   for (int pp = 0; pp < 3; ++pp)
 #pragma unroll
     for (int rr = 0; rr < 3; ++rr) {
-      double b[9];
+      T b[9];
       hess_block<MODEL>(P, R[pp], R[rr], b);
 #pragma unroll
       for (int l = 0; l < 3; ++l)
@@ -461,9 +505,9 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
     }
 }
 
-template <int MODEL>
-__device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const double* params,
-                                                   const uint8_t* mids, int e, bool stretchRef, double M[81]) {
+template <int MODEL, typename T>
+__device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const T* params,
+                                                   const uint8_t* mids, int e, bool stretchRef, T M[81]) {
   switch (model_at<MODEL>(mids, e)) {
     case M_NH: reduced_M<M_NH>(m, q, params, e, stretchRef, M); break;
     case M_STVK: reduced_M<M_STVK>(m, q, params, e, stretchRef, M); break;
@@ -473,27 +517,27 @@ __device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const doub
   }
 }
 
-template <int MODEL>
+template <int MODEL, typename T>
 __global__ void __launch_bounds__(256)
-    k_psd_classify(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
-                   const uint8_t* __restrict__ mids, double tol, int stretchRef, uint8_t* __restrict__ flag,
+    k_psd_classify(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                   const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
                    int32_t* __restrict__ list, int32_t* __restrict__ count) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.nT) return;
-  double M[81];
+  T M[81];
   reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
   // Cholesky of the lower triangle of M - tol I (symmetrised)
   bool pd = true;
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
-    double d = 0.5 * (M[j + 9 * j] + M[j + 9 * j]) - tol;
+    T d = M[j + 9 * j] - tol;
 #pragma unroll
     for (int k = 0; k < j; ++k) d -= M[j + 9 * k] * M[j + 9 * k];
-    pd = pd && (d > 0.0);
-    const double ljj = sqrt(fmax(d, 1e-300));
+    pd = pd && (d > T(0));
+    const T ljj = sqrt(fmax(d, sizeof(T) == 8 ? T(1e-300) : T(1e-30)));
 #pragma unroll
     for (int i = j + 1; i < 9; ++i) {
-      double s = 0.5 * (M[i + 9 * j] + M[j + 9 * i]);
+      T s = T(0.5) * (M[i + 9 * j] + M[j + 9 * i]);
 #pragma unroll
       for (int k = 0; k < j; ++k) s -= M[i + 9 * k] * M[j + 9 * k];
       M[i + 9 * j] = s / ljj;
@@ -507,78 +551,79 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <int MODEL>
+template <int MODEL, typename T>
 __global__ void __launch_bounds__(64)
-    k_psd_eigen(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
-                const uint8_t* __restrict__ mids, double tol, int stretchRef, const int32_t* __restrict__ list,
-                const int32_t* __restrict__ count, double* __restrict__ dM) {
+    k_psd_eigen(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                const uint8_t* __restrict__ mids, T tol, int stretchRef, const int32_t* __restrict__ list,
+                const int32_t* __restrict__ count, T* __restrict__ dM) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= *count) return;
   const int e = list[i];
-  double A[81], W[81];
+  T A[81], W[81];
   reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, A);
 #pragma unroll
   for (int r = 0; r < 9; ++r)
 #pragma unroll
     for (int c = r + 1; c < 9; ++c) {
-      const double s = 0.5 * (A[r + 9 * c] + A[c + 9 * r]);
+      const T s = T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
       A[r + 9 * c] = s;
       A[c + 9 * r] = s;
     }
 #pragma unroll
-  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? 1.0 : 0.0;
-  double scale = 0.0;
+  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? T(1) : T(0);
+  T scale = T(0);
 #pragma unroll
   for (int k = 0; k < 81; ++k) scale += A[k] * A[k];
+  const T stop = sizeof(T) == 8 ? T(1e-32) : T(1e-14);
   for (int sweep = 0; sweep < 20; ++sweep) {
-    double off = 0.0;
+    T off = T(0);
 #pragma unroll
     for (int pp = 0; pp < 9; ++pp)
 #pragma unroll
       for (int rr = pp + 1; rr < 9; ++rr) off += A[pp + 9 * rr] * A[pp + 9 * rr];
-    if (off <= 1e-32 * scale) break;
+    if (off <= stop * scale) break;
 #pragma unroll
     for (int pp = 0; pp < 8; ++pp)
 #pragma unroll
       for (int rr = pp + 1; rr < 9; ++rr) {
-        const double apq = A[pp + 9 * rr];
-        if (apq != 0.0) {
-          const double theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (2.0 * apq);
-          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        const T apq = A[pp + 9 * rr];
+        if (apq != T(0)) {
+          const T theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (T(2) * apq);
+          const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+          const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            const double akp = A[k + 9 * pp], akq = A[k + 9 * rr];
+            const T akp = A[k + 9 * pp], akq = A[k + 9 * rr];
             A[k + 9 * pp] = c * akp - s * akq;
             A[k + 9 * rr] = s * akp + c * akq;
           }
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            const double apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
+            const T apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
             A[pp + 9 * k] = c * apk - s * aqk;
             A[rr + 9 * k] = s * apk + c * aqk;
           }
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            const double wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
+            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
             W[k + 9 * pp] = c * wkp - s * wkq;
             W[k + 9 * rr] = s * wkp + c * wkq;
           }
         }
       }
   }
-  double lift[9];
+  T lift[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const double lam = A[k + 9 * k];
-    lift[k] = (lam < tol) ? (tol - lam) : 0.0;
+    const T lam = A[k + 9 * k];
+    lift[k] = (lam < tol) ? (tol - lam) : T(0);
   }
-  double* out = dM + 45ll * e;
+  T* out = dM + 45ll * e;
 #pragma unroll
   for (int r = 0; r < 9; ++r)
 #pragma unroll
     for (int c = r; c < 9; ++c) {
-      double s = 0.0;
+      T s = T(0);
 #pragma unroll
       for (int k = 0; k < 9; ++k) s += W[r + 9 * k] * lift[k] * W[c + 9 * k];
       out[r * 9 - (r * (r - 1)) / 2 + (c - r)] = s;
@@ -643,19 +688,25 @@ __global__ void __launch_bounds__(256) k_dphidX(MeshView m, const double* __rest
   vol[e] = -dot3(ad, cr) / 6.0;
 }
 
+__global__ void __launch_bounds__(256) k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
 // y = H x on the block pattern; one lane per row
+template <typename V>
 __global__ void __launch_bounds__(256)
-    k_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const double* __restrict__ vals,
+    k_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const V* __restrict__ vals,
            const double* __restrict__ x, double* __restrict__ y) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= 3 * nV) return;
   const int v = row / 3, l = row - 3 * v;
   const int b0 = bptr[v], b1 = bptr[v + 1], nb = b1 - b0;
-  const double* vr = vals + 9ll * b0 + 3ll * l * nb;
+  const V* vr = vals + 9ll * b0 + 3ll * l * nb;
   double s = 0.0;
   for (int r = 0; r < nb; ++r) {
     const double* xp = x + 3ll * bcol[b0 + r];
-    s += vr[3 * r] * xp[0] + vr[3 * r + 1] * xp[1] + vr[3 * r + 2] * xp[2];
+    s += (double)vr[3 * r] * xp[0] + (double)vr[3 * r + 1] * xp[1] + (double)vr[3 * r + 2] * xp[2];
   }
   y[row] = s;
 }
@@ -674,12 +725,33 @@ int max_lds_bytes() {
   return lim;
 }
 
-template <int MODEL, bool PSD, int VPW>
-int launch_hessian_vpw(const bh_mesh* m, const double* q, const double* params, const uint8_t* mids, PsdView psd,
-                       double* vals, hipStream_t s) {
+// FP32 copies of the element geometry for the precision form (made on first use)
+int ensure_f32(bh_mesh* m, hipStream_t s) {
+  if (m->tetPhi32) return BH_OK;
+  BH_CHECK(hipMalloc(&m->tetPhi32, sizeof(float) * 12 * std::max(1, m->nT)));
+  BH_CHECK(hipMalloc(&m->vol32, sizeof(float) * std::max(1, m->nT)));
+  if (m->nT) {
+    hipLaunchKernelGGL(k_to_f32, dim3((12ll * m->nT + 255) / 256), dim3(256), 0, s, 12ll * m->nT, m->tetPhi, m->tetPhi32);
+    hipLaunchKernelGGL(k_to_f32, dim3((m->nT + 255) / 256), dim3(256), 0, s, (int64_t)m->nT, m->vol, m->vol32);
+    BH_CHECK(hipGetLastError());
+  }
+  return BH_OK;
+}
+
+template <typename T>
+int ensure_geometry(bh_mesh* m, hipStream_t s) {
+  if constexpr (sizeof(T) == 8)
+    return BH_OK;
+  else
+    return ensure_f32(m, s);
+}
+
+template <int MODEL, bool PSD, int VPW, typename T>
+int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd,
+                       T* vals, hipStream_t s) {
   const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
-  const size_t lds = sizeof(double) * (size_t)std::max(seg, 1);
-  auto kern = k_hessian<MODEL, PSD, VPW>;
+  const size_t lds = sizeof(T) * (size_t)std::max(seg, 1);
+  auto kern = k_hessian<MODEL, PSD, VPW, T>;
   static std::once_flag once;
   std::call_once(once, [&] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -693,60 +765,66 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const double* params, 
   return BH_OK;
 }
 
-template <int MODEL, bool PSD>
-int launch_hessian(const bh_mesh* m, const double* q, const double* params, const uint8_t* mids, PsdView psd,
-                   double* vals, hipStream_t s) {
+template <int MODEL, bool PSD, typename T>
+int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd, T* vals,
+                   hipStream_t s) {
   const int lim = max_lds_bytes();
-  if (sizeof(double) * (size_t)m->maxSeg64 <= (size_t)lim / 2)
+  if (sizeof(T) * (size_t)m->maxSeg64 <= (size_t)lim / 2)
     return launch_hessian_vpw<MODEL, PSD, 64>(m, q, params, mids, psd, vals, s);
-  if (sizeof(double) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
+  if (sizeof(T) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
     return launch_hessian_vpw<MODEL, PSD, 32>(m, q, params, mids, psd, vals, s);
-  if (sizeof(double) * (size_t)m->maxSeg16 <= (size_t)lim)
+  if (sizeof(T) * (size_t)m->maxSeg16 <= (size_t)lim)
     return launch_hessian_vpw<MODEL, PSD, 16>(m, q, params, mids, psd, vals, s);
   return BH_UNSUPPORTED;  // vertex valence too high for the LDS gather
 }
 
-template <int MODEL>
-int run_psd_passes(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double tol, hipStream_t s) {
+template <int MODEL, typename T>
+int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T tol, T** dM, hipStream_t s) {
   if (!m->psdFlag) {
     BH_CHECK(hipMalloc(&m->psdFlag, std::max<size_t>(1, m->nT)));
     BH_CHECK(hipMalloc(&m->psdList, sizeof(int32_t) * std::max<size_t>(1, m->nT)));
     BH_CHECK(hipMalloc(&m->psdCount, sizeof(int32_t)));
-    BH_CHECK(hipMalloc(&m->psdDM, sizeof(double) * 45 * std::max<size_t>(1, m->nT)));
   }
+  void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->psdDM) : reinterpret_cast<void**>(&m->psdDM32);
+  if (!*buf) BH_CHECK(hipMalloc(buf, sizeof(T) * 45 * std::max<size_t>(1, m->nT)));
+  *dM = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
   const int grid = (m->nT + 255) / 256;
   {
     ProfScope ps(m, BH_KT_PSD_CLASSIFY, s);
-    hipLaunchKernelGGL(k_psd_classify<MODEL>, dim3(grid), dim3(256), 0, s, view(m), q, params, mids, tol,
+    hipLaunchKernelGGL((k_psd_classify<MODEL, T>), dim3(grid), dim3(256), 0, s, view(m), q, params, mids, tol,
                        m->opts.stretch_reference_layout, m->psdFlag, m->psdList, m->psdCount);
   }
   BH_CHECK(hipGetLastError());
   // the eigen pass is sized for the worst case and exits early past *count
   const int grid2 = (m->nT + 63) / 64;
   ProfScope ps(m, BH_KT_PSD_EIGEN, s);
-  hipLaunchKernelGGL(k_psd_eigen<MODEL>, dim3(grid2), dim3(64), 0, s, view(m), q, params, mids, tol,
-                     m->opts.stretch_reference_layout, m->psdList, m->psdCount, m->psdDM);
+  hipLaunchKernelGGL((k_psd_eigen<MODEL, T>), dim3(grid2), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdList, m->psdCount, *dM);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
-template <int MODEL>
-int hessian_model(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, const bh_psd* psd,
-                  double* vals, hipStream_t s) {
+template <int MODEL, typename T>
+int hessian_model(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, const bh_psd* psd, T* vals,
+                  hipStream_t s) {
   if (psd && psd->enabled) {
-    int st = run_psd_passes<MODEL>(m, q, params, mids, psd->tol, s);
+    T* dM = nullptr;
+    int st = run_psd_passes<MODEL, T>(m, q, params, mids, (T)psd->tol, &dM, s);
     if (st != BH_OK) return st;
-    PsdView pv{std::max(psd->tol, 0.0) * 0.25, psd->tol, m->psdFlag, m->psdDM};
+    PsdView<T> pv{(T)(std::max(psd->tol, 0.0) * 0.25), (T)psd->tol, m->psdFlag, dM};
     return launch_hessian<MODEL, true>(m, q, params, mids, pv, vals, s);
   }
-  PsdView pv{0.0, 0.0, nullptr, nullptr};
+  PsdView<T> pv{T(0), T(0), nullptr, nullptr};
   return launch_hessian<MODEL, false>(m, q, params, mids, pv, vals, s);
 }
 
-int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
-                     const bh_psd* psd, double* vals, hipStream_t s) {
+template <typename T>
+int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const T* params,
+                     const bh_psd* psd, T* vals, hipStream_t s) {
   if (m->nV == 0) return BH_OK;
+  int st = ensure_geometry<T>(m, s);
+  if (st) return st;
   if (mids) return hessian_model<M_PER_TET>(m, q, params, mids, psd, vals, s);
   switch (model) {
     case M_NH: return hessian_model<M_NH>(m, q, params, mids, psd, vals, s);
@@ -758,13 +836,14 @@ int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q
   }
 }
 
-template <int MODEL>
-int launch_energy(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double* out, hipStream_t s) {
+template <int MODEL, typename T>
+int launch_energy(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, double* out, hipStream_t s) {
   const int per = kEnergyBlock * kEnergyItems;
   const int grid = (m->nT + per - 1) / per;
   ProfScope ps(m, BH_KT_ENERGY, s);
   if (grid > 0) {
-    hipLaunchKernelGGL(k_energy<MODEL>, dim3(grid), dim3(kEnergyBlock), 0, s, view(m), q, params, mids, m->partials);
+    hipLaunchKernelGGL((k_energy<MODEL, T>), dim3(grid), dim3(kEnergyBlock), 0, s, view(m), q, params, mids,
+                       m->partials);
     BH_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, m->partials, grid, out);
@@ -772,8 +851,11 @@ int launch_energy(bh_mesh* m, const double* q, const double* params, const uint8
   return BH_OK;
 }
 
-int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params, double* out,
+template <typename T>
+int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const T* params, double* out,
                     hipStream_t s) {
+  int st = ensure_geometry<T>(m, s);
+  if (st) return st;
   if (mids) return launch_energy<M_PER_TET>(m, q, params, mids, out, s);
   switch (model) {
     case M_NH: return launch_energy<M_NH>(m, q, params, mids, out, s);
@@ -785,18 +867,21 @@ int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q,
   }
 }
 
-template <int MODEL>
-int launch_gradient(bh_mesh* m, const double* q, const double* params, const uint8_t* mids, double* g, hipStream_t s) {
+template <int MODEL, typename T>
+int launch_gradient(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, double* g, hipStream_t s) {
   const int grid = (m->nV + 255) / 256;
   if (grid == 0) return BH_OK;
   ProfScope ps(m, BH_KT_GRADIENT, s);
-  hipLaunchKernelGGL(k_gradient<MODEL>, dim3(grid), dim3(256), 0, s, view(m), q, params, mids, g);
+  hipLaunchKernelGGL((k_gradient<MODEL, T>), dim3(grid), dim3(256), 0, s, view(m), q, params, mids, g);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
-int gradient_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params, double* g,
+template <typename T>
+int gradient_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const T* params, double* g,
                       hipStream_t s) {
+  int st = ensure_geometry<T>(m, s);
+  if (st) return st;
   if (mids) return launch_gradient<M_PER_TET>(m, q, params, mids, g, s);
   switch (model) {
     case M_NH: return launch_gradient<M_NH>(m, q, params, mids, g, s);
@@ -933,7 +1018,7 @@ void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->tetE, m->tetPhi, m->vol, m->incPtr, m->inc, m->incRank, m->bptr, m->bcol, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws};
+                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws, m->tetPhi32, m->vol32, m->psdDM32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -1087,6 +1172,24 @@ int bh_linear_tetmesh_dq2_device(bh_mesh* m, int model, const uint8_t* mids, con
   return hessian_dispatch(m, model, mids, q, params, psd, d_vals, as_stream(s));
 }
 
+int bh_linear_tetmesh_q_f32_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const float* params,
+                                   double* d_energy, bh_stream_t s) {
+  if (!m || !q || !params || !d_energy || check_model_args(model, mids)) return BH_BAD_ARG;
+  return energy_dispatch(m, model, mids, q, params, d_energy, as_stream(s));
+}
+
+int bh_linear_tetmesh_dq_f32_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const float* params,
+                                    double* d_g, bh_stream_t s) {
+  if (!m || !q || !params || !d_g || check_model_args(model, mids)) return BH_BAD_ARG;
+  return gradient_dispatch(m, model, mids, q, params, d_g, as_stream(s));
+}
+
+int bh_linear_tetmesh_dq2_f32_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const float* params,
+                                     const bh_psd* psd, float* d_vals, bh_stream_t s) {
+  if (!m || !q || !params || !d_vals || check_model_args(model, mids)) return BH_BAD_ARG;
+  return hessian_dispatch(m, model, mids, q, params, psd, d_vals, as_stream(s));
+}
+
 int bh_linear_tetmesh_q(bh_mesh* m, int model, const double* q, const double* params, double* energy) {
   if (!m || !q || !params || !energy || !is_model(model)) return BH_BAD_ARG;
   hipStream_t s = nullptr;
@@ -1182,7 +1285,7 @@ int bh_linear_tetmesh_mass_matrix_device(bh_mesh* m, const double* d_density, do
   if (!m || !d_density || !d_vals) return BH_BAD_ARG;
   if (m->nV == 0) return BH_OK;
   // params column 0 carries the density; column 1 is never read for the mass pseudo-model
-  PsdView pv{0.0, 0.0, nullptr, nullptr};
+  PsdView<double> pv{0.0, 0.0, nullptr, nullptr};
   return launch_hessian<M_MASS, false>(m, m->stage_q, d_density, nullptr, pv, d_vals, as_stream(s));
 }
 
@@ -1221,8 +1324,8 @@ int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
 int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* d_y, bh_stream_t s) {
   if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
   if (m->nV == 0) return BH_OK;
-  hipLaunchKernelGGL(k_spmv, dim3((3 * m->nV + 255) / 256), dim3(256), 0, as_stream(s), m->nV, m->bptr, m->bcol,
-                     d_vals, d_x, d_y);
+  hipLaunchKernelGGL(k_spmv<double>, dim3((3 * m->nV + 255) / 256), dim3(256), 0, as_stream(s), m->nV, m->bptr,
+                     m->bcol, d_vals, d_x, d_y);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }

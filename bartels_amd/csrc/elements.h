@@ -1,6 +1,9 @@
 // Per-tet constitutive math for the gfx950 kernels: energy density, first Piola stress and the
 // 3x3 blocks of the element Hessian, written in closed block form instead of the reference's
-// dense 12x9 * 9x9 * 9x12 product.
+// dense 12x9 * 9x9 * 9x12 product.  Templated on the element arithmetic type T:
+//   T = double : the FP64 path (parity with the reference to 1e-12);
+//   T = float  : the precision form -- F, stresses and Hessian blocks in FP32, while J = det F,
+//                1/J, log J and the energy invariants are evaluated in FP64 from the FP32 F.
 //
 // Conventions (reference include/flatten.h:59-72, include/flatten_multiply.h:52-75):
 //   F(i,j) at F[i + 3j];  phi_m = grad of shape function m (3-vector);  F = sum_m x_m phi_m^T.
@@ -33,13 +36,15 @@ namespace bh {
 
 enum : int { M_NH = 0, M_STVK = 1, M_COROT = 2, M_ARAP = 3, M_NH_LEGACY = 4, M_MASS = 100, M_PER_TET = 255 };
 
-__device__ __forceinline__ double det3(const double* F) {
+template <typename T>
+__device__ __forceinline__ T det3(const T* F) {
   return F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
          F[6] * (F[1] * F[5] - F[4] * F[2]);
 }
 
 // cof(F) = det(F) F^{-T}
-__device__ __forceinline__ void cof3(const double* F, double* c) {
+template <typename T>
+__device__ __forceinline__ void cof3(const T* F, T* c) {
   c[0] = F[4] * F[8] - F[7] * F[5];
   c[1] = F[7] * F[2] - F[1] * F[8];
   c[2] = F[1] * F[5] - F[4] * F[2];
@@ -52,109 +57,129 @@ __device__ __forceinline__ void cof3(const double* F, double* c) {
 }
 
 // y = A x, A col-major 3x3
-__device__ __forceinline__ void mv3(const double* A, const double* x, double* y) {
+template <typename T>
+__device__ __forceinline__ void mv3(const T* A, const T* x, T* y) {
   y[0] = A[0] * x[0] + A[3] * x[1] + A[6] * x[2];
   y[1] = A[1] * x[0] + A[4] * x[1] + A[7] * x[2];
   y[2] = A[2] * x[0] + A[5] * x[1] + A[8] * x[2];
 }
 // y = A^T x
-__device__ __forceinline__ void mtv3(const double* A, const double* x, double* y) {
+template <typename T>
+__device__ __forceinline__ void mtv3(const T* A, const T* x, T* y) {
   y[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
   y[1] = A[3] * x[0] + A[4] * x[1] + A[5] * x[2];
   y[2] = A[6] * x[0] + A[7] * x[1] + A[8] * x[2];
 }
-__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+template <typename T>
+__device__ __forceinline__ T dot3(const T* a, const T* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+template <typename T>
+__device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// determinant and cofactor in FP64 from an F of any precision (the precision form keeps J in FP64)
+template <typename T>
+__device__ __forceinline__ double det_cof_fp64(const T* F, double* c) {
+  double Fd[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
+  cof3(Fd, c);
+  return Fd[0] * c[0] + Fd[3] * c[3] + Fd[6] * c[6];
+}
+
 // ---- 3x3 SVD, rotation convention (U, V rotations, s sorted descending, s[2] signed) --------
 // Cyclic Jacobi on F^T F (bounded sweeps), then U from F V.  Same algorithm as the oracle.
-__device__ inline void jacobi_rot(double* A, double* V, int p, int q) {
-  const double apq = A[p + 3 * q];
-  if (apq == 0.0) return;
-  const double app = A[p + 3 * p], aqq = A[q + 3 * q];
-  const double theta = (aqq - app) / (2.0 * apq);
-  const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-  const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+template <typename T>
+__device__ inline void jacobi_rot(T* A, T* V, int p, int q) {
+  const T apq = A[p + 3 * q];
+  if (apq == T(0)) return;
+  const T app = A[p + 3 * p], aqq = A[q + 3 * q];
+  const T theta = (aqq - app) / (T(2) * apq);
+  const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+  const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double akp = A[k + 3 * p], akq = A[k + 3 * q];
+    const T akp = A[k + 3 * p], akq = A[k + 3 * q];
     A[k + 3 * p] = c * akp - s * akq;
     A[k + 3 * q] = s * akp + c * akq;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double apk = A[p + 3 * k], aqk = A[q + 3 * k];
+    const T apk = A[p + 3 * k], aqk = A[q + 3 * k];
     A[p + 3 * k] = c * apk - s * aqk;
     A[q + 3 * k] = s * apk + c * aqk;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double vkp = V[k + 3 * p], vkq = V[k + 3 * q];
+    const T vkp = V[k + 3 * p], vkq = V[k + 3 * q];
     V[k + 3 * p] = c * vkp - s * vkq;
     V[k + 3 * q] = s * vkp + c * vkq;
   }
 }
 
-__device__ inline void svd3(const double* F, double* U, double* s, double* V) {
-  double A[9];
+template <typename T>
+__device__ inline void svd3(const T* F, T* U, T* s, T* V) {
+  T A[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
       A[i + 3 * j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? T(1) : T(0);
+  const T eps2 = sizeof(T) == 8 ? T(1e-36) : T(1e-16);
   for (int sweep = 0; sweep < 12; ++sweep) {
-    const double off = A[3] * A[3] + A[6] * A[6] + A[7] * A[7];
-    const double dg = A[0] * A[0] + A[4] * A[4] + A[8] * A[8];
-    if (off <= 1e-36 * dg) break;
+    const T off = A[3] * A[3] + A[6] * A[6] + A[7] * A[7];
+    const T dg = A[0] * A[0] + A[4] * A[4] + A[8] * A[8];
+    if (off <= eps2 * dg) break;
     jacobi_rot(A, V, 0, 1);
     jacobi_rot(A, V, 0, 2);
     jacobi_rot(A, V, 1, 2);
   }
-  double w0 = A[0], w1 = A[4], w2 = A[8];
+  T w0 = A[0], w1 = A[4], w2 = A[8];
   // sort descending with column swaps (3-element network)
 #define BH_SWAPCOL(a, b, wa, wb)                                       \
   if (wa < wb) {                                                       \
-    double tw = wa; wa = wb; wb = tw;                                  \
+    T tw = wa; wa = wb; wb = tw;                                       \
     for (int r = 0; r < 3; ++r) {                                      \
-      double tv = V[r + 3 * a]; V[r + 3 * a] = V[r + 3 * b]; V[r + 3 * b] = tv; \
+      T tv = V[r + 3 * a]; V[r + 3 * a] = V[r + 3 * b]; V[r + 3 * b] = tv; \
     }                                                                  \
   }
   BH_SWAPCOL(0, 1, w0, w1)
   BH_SWAPCOL(0, 2, w0, w2)
   BH_SWAPCOL(1, 2, w1, w2)
 #undef BH_SWAPCOL
-  if (det3(V) < 0.0) {
+  if (det3(V) < T(0)) {
     V[6] = -V[6]; V[7] = -V[7]; V[8] = -V[8];
   }
-  double f0[3], f1[3], f2[3];
+  T f0[3], f1[3], f2[3];
   mv3(F, V, f0);
   mv3(F, V + 3, f1);
   mv3(F, V + 6, f2);
-  const double n0 = sqrt(dot3(f0, f0));
-  double u0[3], u1[3], u2[3];
-  if (n0 > 0.0) {
+  const T n0 = sqrt(dot3(f0, f0));
+  T u0[3], u1[3], u2[3];
+  if (n0 > T(0)) {
     u0[0] = f0[0] / n0; u0[1] = f0[1] / n0; u0[2] = f0[2] / n0;
   } else {
-    u0[0] = 1.0; u0[1] = 0.0; u0[2] = 0.0;
+    u0[0] = T(1); u0[1] = T(0); u0[2] = T(0);
   }
-  const double d01 = dot3(u0, f1);
+  const T d01 = dot3(u0, f1);
   u1[0] = f1[0] - d01 * u0[0]; u1[1] = f1[1] - d01 * u0[1]; u1[2] = f1[2] - d01 * u0[2];
-  const double n1 = sqrt(dot3(u1, u1));
-  if (n1 > 1e-300) {
+  const T n1 = sqrt(dot3(u1, u1));
+  const T tiny = sizeof(T) == 8 ? T(1e-300) : T(1e-30);
+  if (n1 > tiny) {
     u1[0] /= n1; u1[1] /= n1; u1[2] /= n1;
   } else {
-    const int k = (fabs(u0[0]) < 0.6) ? 0 : ((fabs(u0[1]) < 0.6) ? 1 : 2);
-    const double dd = u0[k];
-    u1[0] = (k == 0 ? 1.0 : 0.0) - dd * u0[0];
-    u1[1] = (k == 1 ? 1.0 : 0.0) - dd * u0[1];
-    u1[2] = (k == 2 ? 1.0 : 0.0) - dd * u0[2];
-    const double nn = sqrt(dot3(u1, u1));
+    const int k = (fabs(u0[0]) < T(0.6)) ? 0 : ((fabs(u0[1]) < T(0.6)) ? 1 : 2);
+    const T dd = u0[k];
+    u1[0] = (k == 0 ? T(1) : T(0)) - dd * u0[0];
+    u1[1] = (k == 1 ? T(1) : T(0)) - dd * u0[1];
+    u1[2] = (k == 2 ? T(1) : T(0)) - dd * u0[2];
+    const T nn = sqrt(dot3(u1, u1));
     u1[0] /= nn; u1[1] /= nn; u1[2] /= nn;
   }
   cross3(u0, u1, u2);
@@ -170,65 +195,75 @@ __device__ inline void svd3(const double* F, double* U, double* s, double* V) {
 }
 
 // ---- per-tet precomputation -----------------------------------------------------------------
+template <typename T>
 struct Pre {
-  double F[9];
-  double vol;
+  T F[9];
+  T vol;
   // NH / legacy / StVK
-  double cof[9];
-  double a0, a1, a2, a3;  // model scalars (pre-multiplied by vol)
-  double S[9];            // StVK stress-like matrix (vol * S)
-  double FFt[9];          // StVK (vol * mu * F F^T)
+  T cof[9];
+  T a0, a1, a2, a3;  // model scalars (pre-multiplied by vol)
+  T S[9];            // StVK stress-like matrix (vol * S)
+  T FFt[9];          // StVK (vol * mu * F F^T)
   // stretch
-  double U[9], V[9];      // frame of G (reference layout: G = F^T)
-  double psi2[9];         // vol * d2psi/ds2
-  double A[3], B[3];      // vol * A_ij, vol * B_ij for pairs (0,1), (0,2), (1,2)
+  T U[9], V[9];      // frame of G (reference layout: G = F^T)
+  T psi2[9];         // vol * d2psi/ds2
+  T A[3], B[3];      // vol * A_ij, vol * B_ij for pairs (0,1), (0,2), (1,2)
 };
 
 // stretch-energy derivatives in singular values. Corotational: params [lambda, mu]
 // (src/dpsi_corotational_dS.cpp:6-13); ARAP: k = params(0) (src/dpsi_arap_dS.cpp:6-12).
-template <int MODEL>
-__device__ __forceinline__ void dpsi_s(const double* s, double p0, double p1, double* d) {
+template <int MODEL, typename T>
+__device__ __forceinline__ void dpsi_s(const T* s, T p0, T p1, T* d) {
   if (MODEL == M_COROT) {
-    const double tr = s[0] + s[1] + s[2] - 3.0;
-    d[0] = 2.0 * p1 * (s[0] - 1.0) + p0 * tr;
-    d[1] = 2.0 * p1 * (s[1] - 1.0) + p0 * tr;
-    d[2] = 2.0 * p1 * (s[2] - 1.0) + p0 * tr;
+    const T tr = s[0] + s[1] + s[2] - T(3);
+    d[0] = T(2) * p1 * (s[0] - T(1)) + p0 * tr;
+    d[1] = T(2) * p1 * (s[1] - T(1)) + p0 * tr;
+    d[2] = T(2) * p1 * (s[2] - T(1)) + p0 * tr;
   } else {
-    d[0] = 2.0 * p0 * (s[0] - 1.0);
-    d[1] = 2.0 * p0 * (s[1] - 1.0);
-    d[2] = 2.0 * p0 * (s[2] - 1.0);
+    d[0] = T(2) * p0 * (s[0] - T(1));
+    d[1] = T(2) * p0 * (s[1] - T(1));
+    d[2] = T(2) * p0 * (s[2] - T(1));
   }
 }
 
 // Hessian precomputation.  stretch_ref: reference (F^T) layout for corotational / ARAP.
-template <int MODEL>
-__device__ __forceinline__ void pre_hessian(Pre& P, double p0, double p1, bool stretch_ref) {
-  const double* F = P.F;
-  const double vol = P.vol;
+template <int MODEL, typename T>
+__device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool stretch_ref) {
+  const T* F = P.F;
+  const T vol = P.vol;
   if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
-    cof3(F, P.cof);
-    const double J = F[0] * P.cof[0] + F[3] * P.cof[3] + F[6] * P.cof[6];
+    double J;
+    if constexpr (sizeof(T) == 8) {
+      cof3(F, P.cof);
+      J = (double)(F[0] * P.cof[0] + F[3] * P.cof[3] + F[6] * P.cof[6]);
+    } else {
+      double c[9];
+      J = det_cof_fp64(F, c);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) P.cof[i] = (T)c[i];
+    }
     const double iJ = 1.0 / J;
+    const double C = p0, D = p1, vd = vol;
     if (MODEL == M_NH) {
-      P.a0 = vol * 2.0 * p0;                                 // 2C
-      P.a1 = vol * (2.0 * p0 * iJ * iJ + 2.0 * p1);          // alpha
-      P.a2 = vol * (2.0 * p1 * (J - 1.0) - 2.0 * p0 * iJ);   // beta
-      P.a3 = 0.0;
+      P.a0 = (T)(vd * 2.0 * C);                              // 2C
+      P.a1 = (T)(vd * (2.0 * C * iJ * iJ + 2.0 * D));        // alpha
+      P.a2 = (T)(vd * (2.0 * D * (J - 1.0) - 2.0 * C * iJ)); // beta
+      P.a3 = T(0);
     } else {
       const double cb = cbrt(J);
       const double s = 1.0 / (cb * cb);
       double I1 = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
-      const double Cs = p0 * s;
-      P.a0 = vol * 2.0 * Cs;
-      P.a1 = vol * ((10.0 / 9.0) * Cs * I1 * iJ * iJ + 2.0 * p1);
-      P.a2 = vol * (2.0 * p1 * (J - 1.0) - (2.0 / 3.0) * Cs * I1 * iJ);
-      P.a3 = vol * (-(4.0 / 3.0) * Cs * iJ);
+      for (int k = 0; k < 9; ++k) I1 += (double)F[k] * (double)F[k];
+      const double Cs = C * s;
+      P.a0 = (T)(vd * 2.0 * Cs);
+      P.a1 = (T)(vd * ((10.0 / 9.0) * Cs * I1 * iJ * iJ + 2.0 * D));
+      P.a2 = (T)(vd * (2.0 * D * (J - 1.0) - (2.0 / 3.0) * Cs * I1 * iJ));
+      P.a3 = (T)(vd * (-(4.0 / 3.0) * Cs * iJ));
     }
   } else if (MODEL == M_STVK) {
-    const double lam = p0, mu = p1;
-    double C[9];
+    const T lam = p0, mu = p1;
+    T C[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -236,16 +271,16 @@ __device__ __forceinline__ void pre_hessian(Pre& P, double p0, double p1, bool s
         C[i + 3 * j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
         P.FFt[i + 3 * j] = vol * mu * (F[i] * F[j] + F[i + 3] * F[j + 3] + F[i + 6] * F[j + 6]);
       }
-    const double trE = 0.5 * (C[0] + C[4] + C[8] - 3.0);
+    const T trE = T(0.5) * (C[0] + C[4] + C[8] - T(3));
 #pragma unroll
-    for (int i = 0; i < 9; ++i) P.S[i] = vol * (mu * (C[i] - ((i % 4 == 0) ? 1.0 : 0.0)));
+    for (int i = 0; i < 9; ++i) P.S[i] = vol * (mu * (C[i] - ((i % 4 == 0) ? T(1) : T(0))));
     P.S[0] += vol * lam * trE;
     P.S[4] += vol * lam * trE;
     P.S[8] += vol * lam * trE;
     P.a0 = vol * mu;
     P.a1 = vol * lam;
   } else if (MODEL == M_COROT || MODEL == M_ARAP) {
-    double U[9], V[9], s[3];
+    T U[9], V[9], s[3];
     svd3(F, U, s, V);
     // G = F^T = V diag(s) U^T in the reference layout
 #pragma unroll
@@ -253,40 +288,40 @@ __device__ __forceinline__ void pre_hessian(Pre& P, double p0, double p1, bool s
       P.U[i] = stretch_ref ? V[i] : U[i];
       P.V[i] = stretch_ref ? U[i] : V[i];
     }
-    double d[3];
+    T d[3];
     dpsi_s<MODEL>(s, p0, p1, d);
-    double sij;
+    T sij;
     if (MODEL == M_COROT) {
-      sij = 2.0 * p1;
+      sij = T(2) * p1;
 #pragma unroll
-      for (int i = 0; i < 9; ++i) P.psi2[i] = vol * (p0 + ((i % 4 == 0) ? 2.0 * p1 : 0.0));
+      for (int i = 0; i < 9; ++i) P.psi2[i] = vol * (p0 + ((i % 4 == 0) ? T(2) * p1 : T(0)));
     } else {
-      sij = 2.0 * p0;
+      sij = T(2) * p0;
 #pragma unroll
-      for (int i = 0; i < 9; ++i) P.psi2[i] = vol * ((i % 4 == 0) ? 2.0 * p0 : 0.0);
+      for (int i = 0; i < 9; ++i) P.psi2[i] = vol * ((i % 4 == 0) ? T(2) * p0 : T(0));
     }
     const int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double den = s[pi[k]] + s[pj[k]];
-      const double t = (fabs(den) < 1e-8) ? 0.0 : (d[pi[k]] + d[pj[k]]) / den;
-      P.A[k] = vol * 0.5 * (sij + t);
-      P.B[k] = vol * 0.5 * (sij - t);
+      const T den = s[pi[k]] + s[pj[k]];
+      const T t = (fabs(den) < T(1e-8)) ? T(0) : (d[pi[k]] + d[pj[k]]) / den;
+      P.A[k] = vol * T(0.5) * (sij + t);
+      P.B[k] = vol * T(0.5) * (sij - t);
     }
   }
 }
 
 // 3x3 block K_am (row-major out[3l + k]).
-template <int MODEL>
-__device__ __forceinline__ void hess_block(const Pre& P, const double* pa, const double* pm, double* out) {
+template <int MODEL, typename T>
+__device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa, const T* pm, T* out) {
   if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
-    double ca[3], cm[3], x[3], w[3];
+    T ca[3], cm[3], x[3], w[3];
     mv3(P.cof, pa, ca);
     mv3(P.cof, pm, cm);
     cross3(pa, pm, x);
     mv3(P.F, x, w);
-    const double d = P.a0 * dot3(pa, pm);
-    const double b = P.a2;
+    const T d = P.a0 * dot3(pa, pm);
+    const T b = P.a2;
     out[0] = d + P.a1 * ca[0] * cm[0];
     out[1] = P.a1 * ca[0] * cm[1] + b * w[2];
     out[2] = P.a1 * ca[0] * cm[2] - b * w[1];
@@ -297,7 +332,7 @@ __device__ __forceinline__ void hess_block(const Pre& P, const double* pa, const
     out[7] = P.a1 * ca[2] * cm[1] - b * w[0];
     out[8] = d + P.a1 * ca[2] * cm[2];
     if (MODEL == M_NH_LEGACY) {
-      double ua[3], um[3];
+      T ua[3], um[3];
       mv3(P.F, pa, ua);
       mv3(P.F, pm, um);
 #pragma unroll
@@ -306,22 +341,22 @@ __device__ __forceinline__ void hess_block(const Pre& P, const double* pa, const
         for (int k = 0; k < 3; ++k) out[3 * l + k] += P.a3 * (ua[l] * cm[k] + ca[l] * um[k]);
     }
   } else if (MODEL == M_STVK) {
-    double ua[3], um[3], Sm[3];
+    T ua[3], um[3], Sm[3];
     mv3(P.F, pa, ua);
     mv3(P.F, pm, um);
     mv3(P.S, pm, Sm);
-    const double sd = dot3(pa, Sm);
-    const double dd = dot3(pa, pm);
+    const T sd = dot3(pa, Sm);
+    const T dd = dot3(pa, pm);
 #pragma unroll
     for (int l = 0; l < 3; ++l)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        out[3 * l + k] = ((l == k) ? sd : 0.0) + P.a0 * um[l] * ua[k] + dd * P.FFt[l + 3 * k] + P.a1 * ua[l] * um[k];
+        out[3 * l + k] = ((l == k) ? sd : T(0)) + P.a0 * um[l] * ua[k] + dd * P.FFt[l + 3 * k] + P.a1 * ua[l] * um[k];
   } else if (MODEL == M_COROT || MODEL == M_ARAP) {
-    double qa[3], qm[3];
+    T qa[3], qm[3];
     mtv3(P.V, pa, qa);
     mtv3(P.V, pm, qm);
-    double K[9];  // Khat col-major
+    T K[9];  // Khat col-major
     // diagonal entries
     K[0] = qa[0] * P.psi2[0] * qm[0] + P.A[0] * qa[1] * qm[1] + P.A[1] * qa[2] * qm[2];
     K[4] = qa[1] * P.psi2[4] * qm[1] + P.A[0] * qa[0] * qm[0] + P.A[2] * qa[2] * qm[2];
@@ -334,34 +369,51 @@ __device__ __forceinline__ void hess_block(const Pre& P, const double* pa, const
     K[7] = qa[1] * P.psi2[7] * qm[2] + P.B[2] * qa[2] * qm[1];  // (1,2)
     K[5] = qa[2] * P.psi2[5] * qm[1] + P.B[2] * qa[1] * qm[2];  // (2,1)
     // out = U K U^T
-    double T[9];  // T = K U^T  (col-major)
+    T Tm[9];  // K U^T  (col-major)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        T[i + 3 * j] = K[i] * P.U[j] + K[i + 3] * P.U[j + 3] + K[i + 6] * P.U[j + 6];
+        Tm[i + 3 * j] = K[i] * P.U[j] + K[i + 3] * P.U[j + 3] + K[i + 6] * P.U[j + 6];
 #pragma unroll
     for (int l = 0; l < 3; ++l)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        out[3 * l + k] = P.U[l] * T[3 * k] + P.U[l + 3] * T[1 + 3 * k] + P.U[l + 6] * T[2 + 3 * k];
+        out[3 * l + k] = P.U[l] * Tm[3 * k] + P.U[l + 3] * Tm[1 + 3 * k] + P.U[l + 6] * Tm[2 + 3 * k];
   }
 }
 
-// energy density psi(F) (not volume scaled)
-template <int MODEL>
-__device__ __forceinline__ double psi(const double* F, double p0, double p1) {
+// energy density psi(F) in FP64 (not volume scaled).  With an FP32 F the invariants are formed
+// from the displacement gradient G = F - I (exact in FP32 near I) and J - 1 in FP64, so the
+// energy keeps its relative accuracy at small strains (the line search compares energy sums).
+template <int MODEL, typename T>
+__device__ __forceinline__ double psi(const T* F, double p0, double p1) {
   if (MODEL == M_NH) {
-    double I1 = 0.0;
+    if constexpr (sizeof(T) == 8) {
+      double I1 = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
-    const double J = det3(F);
-    return p0 * (I1 - 3.0 - 2.0 * log(J)) + p1 * (J - 1.0) * (J - 1.0);
+      for (int k = 0; k < 9; ++k) I1 += (double)F[k] * (double)F[k];
+      const double J = det3((const double*)F);
+      return p0 * (I1 - 3.0 - 2.0 * log(J)) + p1 * (J - 1.0) * (J - 1.0);
+    } else {
+      double G[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) G[k] = (double)(F[k] - ((k % 4 == 0) ? T(1) : T(0)));
+      const double trG = G[0] + G[4] + G[8];
+      double GG = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) GG += G[k] * G[k];
+      // J - 1 = tr G + (sum of principal 2x2 minors of G) + det G
+      const double m2 = (G[0] * G[4] - G[1] * G[3]) + (G[0] * G[8] - G[2] * G[6]) + (G[4] * G[8] - G[5] * G[7]);
+      const double Jm1 = trG + m2 + det3(G);
+      return p0 * (2.0 * trG + GG - 2.0 * log1p(Jm1)) + p1 * Jm1 * Jm1;
+    }
   } else if (MODEL == M_NH_LEGACY) {
     double I1 = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
-    const double J = det3(F);
+    for (int k = 0; k < 9; ++k) I1 += (double)F[k] * (double)F[k];
+    double c[9];
+    const double J = det_cof_fp64(F, c);
     const double cb = cbrt(J);
     return p0 * (I1 / (cb * cb) - 3.0) + p1 * (J - 1.0) * (J - 1.0);
   } else if (MODEL == M_STVK) {
@@ -370,58 +422,65 @@ __device__ __forceinline__ double psi(const double* F, double p0, double p1) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        E[i + 3 * j] = 0.5 * (F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2] -
-                              ((i == j) ? 1.0 : 0.0));
+        E[i + 3 * j] = 0.5 * ((double)F[3 * i] * F[3 * j] + (double)F[3 * i + 1] * F[3 * j + 1] +
+                              (double)F[3 * i + 2] * F[3 * j + 2] - ((i == j) ? 1.0 : 0.0));
     double trEE = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) trEE += E[k] * E[k];  // E symmetric: tr(E E) = sum E_ij^2
     const double trE = E[0] + E[4] + E[8];
     return p1 * trEE + 0.5 * p0 * trE * trE;
   } else {
-    double U[9], V[9], s[3];
+    T U[9], V[9], s[3];
     svd3(F, U, s, V);
+    const double s0 = s[0], s1 = s[1], s2 = s[2];
     if (MODEL == M_COROT) {
-      const double t = s[0] + s[1] + s[2] - 3.0;
-      return p1 * ((s[0] - 1.0) * (s[0] - 1.0) + (s[1] - 1.0) * (s[1] - 1.0) + (s[2] - 1.0) * (s[2] - 1.0)) +
-             0.5 * p0 * t * t;
+      const double t = s0 + s1 + s2 - 3.0;
+      return p1 * ((s0 - 1.0) * (s0 - 1.0) + (s1 - 1.0) * (s1 - 1.0) + (s2 - 1.0) * (s2 - 1.0)) + 0.5 * p0 * t * t;
     }
-    return p0 * ((s[0] - 1.0) * (s[0] - 1.0) + (s[1] - 1.0) * (s[1] - 1.0) + (s[2] - 1.0) * (s[2] - 1.0));
+    return p0 * ((s0 - 1.0) * (s0 - 1.0) + (s1 - 1.0) * (s1 - 1.0) + (s2 - 1.0) * (s2 - 1.0));
   }
 }
 
 // first Piola stress P (col-major), not volume scaled.
 // Stretch models follow src/dpsi_stretch_dF.cpp:20-42: when s0 s1 s2 <= -1e-10 the negative
 // singular values are flipped positive (U, V unchanged).
-template <int MODEL>
-__device__ __forceinline__ void piola(const double* F, double p0, double p1, double* P) {
+template <int MODEL, typename T>
+__device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
   if (MODEL == M_NH) {
-    double c[9];
-    cof3(F, c);
-    const double J = F[0] * c[0] + F[3] * c[3] + F[6] * c[6];
-    const double iJ = 1.0 / J;
+    T c[9];
+    double J;
+    if constexpr (sizeof(T) == 8) {
+      cof3(F, c);
+      J = (double)(F[0] * c[0] + F[3] * c[3] + F[6] * c[6]);
+    } else {
+      double cd[9];
+      J = det_cof_fp64(F, cd);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) P[k] = 2.0 * p0 * (F[k] - c[k] * iJ) + 2.0 * p1 * (J - 1.0) * c[k];
+      for (int i = 0; i < 9; ++i) c[i] = (T)cd[i];
+    }
+    const T iJ = (T)(1.0 / J), Jm1 = (T)(J - 1.0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P[k] = T(2) * p0 * (F[k] - c[k] * iJ) + T(2) * p1 * Jm1 * c[k];
   } else if (MODEL == M_NH_LEGACY) {
-    double c[9];
-    cof3(F, c);
-    const double J = F[0] * c[0] + F[3] * c[3] + F[6] * c[6];
+    double cd[9];
+    const double J = det_cof_fp64(F, cd);
     double I1 = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
+    for (int k = 0; k < 9; ++k) I1 += (double)F[k] * (double)F[k];
     const double cb = cbrt(J);
     const double s = 1.0 / (cb * cb);
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-      P[k] = p0 * (2.0 * s * F[k] - (2.0 / 3.0) * (s / J) * I1 * c[k]) + 2.0 * p1 * (J - 1.0) * c[k];
+      P[k] = (T)(p0 * (2.0 * s * F[k] - (2.0 / 3.0) * (s / J) * I1 * cd[k]) + 2.0 * p1 * (J - 1.0) * cd[k]);
   } else if (MODEL == M_STVK) {
-    double S[9];
+    T S[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
         S[i + 3 * j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2] -
-                       ((i == j) ? 1.0 : 0.0);  // 2E
-    const double trE = 0.5 * (S[0] + S[4] + S[8]);
+                       ((i == j) ? T(1) : T(0));  // 2E
+    const T trE = T(0.5) * (S[0] + S[4] + S[8]);
 #pragma unroll
     for (int k = 0; k < 9; ++k) S[k] *= p1;  // 2 mu E
     S[0] += p0 * trE;
@@ -433,14 +492,14 @@ __device__ __forceinline__ void piola(const double* F, double p0, double p1, dou
       for (int j = 0; j < 3; ++j)
         P[i + 3 * j] = F[i] * S[3 * j] + F[i + 3] * S[3 * j + 1] + F[i + 6] * S[3 * j + 2];
   } else {
-    double U[9], V[9], s[3];
+    T U[9], V[9], s[3];
     svd3(F, U, s, V);
-    if (s[0] * s[1] * s[2] <= -1e-10) {
+    if (s[0] * s[1] * s[2] <= T(-1e-10)) {
       s[0] = fabs(s[0]);
       s[1] = fabs(s[1]);
       s[2] = fabs(s[2]);
     }
-    double d[3];
+    T d[3];
     dpsi_s<MODEL>(s, p0, p1, d);
 #pragma unroll
     for (int i = 0; i < 3; ++i)

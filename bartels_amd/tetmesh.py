@@ -188,6 +188,25 @@ class TetMesh:
                                                      self._dp(params), pp, self._dp(out), self._stream(stream)),
               "bh_linear_tetmesh_dq2_device")
 
+    # precision form: FP32 params (nT x 2 column-major, flattened) and FP32 Hessian values;
+    # energy and gradient come back in FP64
+    def q_f32_device(self, model, q, params32, out, model_ids=None, stream=None):
+        check(self._lib.bh_linear_tetmesh_q_f32_device(self._h, model, self._dp(model_ids), self._dp(q),
+                                                       self._dp(params32), self._dp(out), self._stream(stream)),
+              "bh_linear_tetmesh_q_f32_device")
+
+    def dq_f32_device(self, model, q, params32, out, model_ids=None, stream=None):
+        check(self._lib.bh_linear_tetmesh_dq_f32_device(self._h, model, self._dp(model_ids), self._dp(q),
+                                                        self._dp(params32), self._dp(out), self._stream(stream)),
+              "bh_linear_tetmesh_dq_f32_device")
+
+    def dq2_f32_device(self, model, q, params32, out, callback=None, model_ids=None, stream=None):
+        psd = _psd_struct(callback)
+        pp = ctypes.byref(psd) if psd is not None else None
+        check(self._lib.bh_linear_tetmesh_dq2_f32_device(self._h, model, self._dp(model_ids), self._dp(q),
+                                                         self._dp(params32), pp, self._dp(out),
+                                                         self._stream(stream)), "bh_linear_tetmesh_dq2_f32_device")
+
     def mass_matrix_device(self, density, out, stream=None):
         check(self._lib.bh_linear_tetmesh_mass_matrix_device(self._h, self._dp(density), self._dp(out),
                                                              self._stream(stream)), "mass_matrix_device")

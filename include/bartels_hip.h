@@ -128,6 +128,18 @@ int bh_linear_tetmesh_dq_device(bh_mesh* mesh, int model, const uint8_t* d_model
 int bh_linear_tetmesh_dq2_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
                                  const double* d_params, const bh_psd* psd, double* d_vals, bh_stream_t stream);
 
+/* ---- precision form (device pointers) --------------------------------------------------------------
+ * FP32 element path: FP32 copies of dphidX / volume (made on first use), FP32 params (nT x 2,
+ * column-major), F formed from FP64 edge differences then FP32; J = det F, 1/J, log J and the
+ * energy invariants in FP64; energy and gradient accumulated and returned in FP64; Hessian values
+ * FP32.  Same pattern, same models, same PSD semantics as the FP64 entry points. */
+int bh_linear_tetmesh_q_f32_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                   const float* d_params, double* d_energy, bh_stream_t stream);
+int bh_linear_tetmesh_dq_f32_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                    const float* d_params, double* d_g, bh_stream_t stream);
+int bh_linear_tetmesh_dq2_f32_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                     const float* d_params, const bh_psd* psd, float* d_vals, bh_stream_t stream);
+
 /* ---- objective terms (config objectives) ---------------------------------------------------- */
 
 /* Replaces sim::linear_tetmesh_mass_matrix (src/linear_tetmesh_mass_matrix.cpp:7-24,
