@@ -62,7 +62,8 @@ struct bh_mesh {
   uint32_t* incRank = nullptr;
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
-  int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max doubles per gather workgroup
+  int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
+  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
   // scratch
   double* partials = nullptr;
   int nPartials = 0;
@@ -351,6 +352,39 @@ __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __re
   for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
 }
 
+// two of the four blocks of the vertex's block row: corners h and h + 2 (h = 0 or 1)
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __restrict__ q,
+                                            const T* __restrict__ params, int e, int a, int h, bool stretch_ref,
+                                            T blk[2][9]) {
+  if (MODEL == M_MASS) {
+    const T mass = params[e] * load_vol<T>(m, e);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int mm = h + 2 * j;
+      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[j][i] = (i % 4 == 0) ? c : T(0);
+    }
+    return;
+  }
+  T phi[4][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
+  load_F(q, m.tetE[e], phi, P.F);
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  T pa[3];
+  select_row(phi, a, pa);
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    T pm[3];
+    select_row(phi, h + 2 * j, pm);
+    hess_block<MODEL>(P, pa, pm, blk[j]);
+  }
+}
+
 // Q (4x3, orthonormal columns, orthogonal to the ones vector) of the thin QR of Phi
 template <typename T>
 __device__ __forceinline__ void phi_qr(const T phi[4][3], T Q[4][3], T R[3][3]) {
@@ -468,6 +502,93 @@ __global__ void __launch_bounds__(kGatherThreads)
         for (int l = 0; l < 3; ++l)
 #pragma unroll
           for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[mm][3 * l + kk];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
+// Same gather with two lanes per vertex: lane h of the pair owns the blocks of corners h and h+2 of
+// every incident tet.  The four corners of a tet are four distinct vertices, so the pair never
+// updates the same LDS slot, and every slot still receives its contributions in ascending element
+// order.  Half the LDS per lane of the one-lane kernel (more waves per CU) for a duplicated
+// per-tet setup.
+template <int MODEL, bool PSD, int VPW, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_pair(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                   const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, T* __restrict__ vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int v0 = blockIdx.x * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 1, h = (int)threadIdx.x & 1;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const int k1 = m.incPtr[v + 1];
+    for (int k = m.incPtr[v]; k < k1; ++k) {
+      const int code = m.inc[k];
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incRank[k];
+      T blk[2][9];
+      switch (model_at<MODEL>(mids, e)) {
+        case M_NH: tet_blocks2<M_NH>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_STVK: tet_blocks2<M_STVK>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_COROT: tet_blocks2<M_COROT>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_ARAP: tet_blocks2<M_ARAP>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_MASS: tet_blocks2<M_MASS>(m, q, params, e, a, h, stretchRef, blk); break;
+        default: tet_blocks2<M_NH_LEGACY>(m, q, params, e, a, h, stretchRef, blk); break;
+      }
+      if (PSD) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          blk[j][0] += psd.tolPos;
+          blk[j][4] += psd.tolPos;
+          blk[j][8] += psd.tolPos;
+        }
+        if (psd.flag[e]) {
+          T phi[4][3], Q[4][3], R[3][3];
+          load_phi(m, e, phi);
+          phi_qr(phi, Q, R);
+          const T* dM = psd.dM + 45ll * e;
+          T qa[3];
+          select_row(Q, a, qa);
+          T Tm[3][9];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) {
+                T sacc = T(0);
+#pragma unroll
+                for (int p = 0; p < 3; ++p) sacc += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
+                Tm[r][3 * l + kk] = sacc;
+              }
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            T qm[3];
+            select_row(Q, h + 2 * j, qm);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) blk[j][i] += qm[0] * Tm[0][i] + qm[1] * Tm[1][i] + qm[2] * Tm[2][i];
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int mm = h + 2 * j;
+        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[j][3 * l + kk];
       }
     }
   }
@@ -746,12 +867,12 @@ int ensure_geometry(bh_mesh* m, hipStream_t s) {
     return ensure_f32(m, s);
 }
 
-template <int MODEL, bool PSD, int VPW, typename T>
+template <int MODEL, bool PSD, int VPW, int LPV, typename T>
 int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd,
                        T* vals, hipStream_t s) {
   const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
   const size_t lds = sizeof(T) * (size_t)std::max(seg, 1);
-  auto kern = k_hessian<MODEL, PSD, VPW, T>;
+  auto kern = (LPV == 1) ? k_hessian<MODEL, PSD, VPW, T> : k_hessian_pair<MODEL, PSD, VPW, T>;
   static std::once_flag once;
   std::call_once(once, [&] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -769,12 +890,18 @@ template <int MODEL, bool PSD, typename T>
 int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd, T* vals,
                    hipStream_t s) {
   const int lim = max_lds_bytes();
+  if (m->gatherLanes == 2) {
+    if (sizeof(T) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
+      return launch_hessian_vpw<MODEL, PSD, 32, 2>(m, q, params, mids, psd, vals, s);
+    if (sizeof(T) * (size_t)m->maxSeg16 <= (size_t)lim)
+      return launch_hessian_vpw<MODEL, PSD, 16, 2>(m, q, params, mids, psd, vals, s);
+  }
   if (sizeof(T) * (size_t)m->maxSeg64 <= (size_t)lim / 2)
-    return launch_hessian_vpw<MODEL, PSD, 64>(m, q, params, mids, psd, vals, s);
+    return launch_hessian_vpw<MODEL, PSD, 64, 1>(m, q, params, mids, psd, vals, s);
   if (sizeof(T) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
-    return launch_hessian_vpw<MODEL, PSD, 32>(m, q, params, mids, psd, vals, s);
+    return launch_hessian_vpw<MODEL, PSD, 32, 1>(m, q, params, mids, psd, vals, s);
   if (sizeof(T) * (size_t)m->maxSeg16 <= (size_t)lim)
-    return launch_hessian_vpw<MODEL, PSD, 16>(m, q, params, mids, psd, vals, s);
+    return launch_hessian_vpw<MODEL, PSD, 16, 1>(m, q, params, mids, psd, vals, s);
   return BH_UNSUPPORTED;  // vertex valence too high for the LDS gather
 }
 
@@ -1044,15 +1171,7 @@ int bh_linear_tetmesh_dphi_dX(int nV, const double* V, int nT, const int* E, dou
     for (int mm = 0; mm < 3; ++mm)
       for (int c = 0; c < 3; ++c) A[c + 3 * mm] = x[mm + 1][c] - x[0][c];
     double c[9];
-    c[0] = A[4] * A[8] - A[7] * A[5];
-    c[1] = A[7] * A[2] - A[1] * A[8];
-    c[2] = A[1] * A[5] - A[4] * A[2];
-    c[3] = A[6] * A[5] - A[3] * A[8];
-    c[4] = A[0] * A[8] - A[6] * A[2];
-    c[5] = A[3] * A[2] - A[0] * A[5];
-    c[6] = A[3] * A[7] - A[6] * A[4];
-    c[7] = A[6] * A[1] - A[0] * A[7];
-    c[8] = A[0] * A[4] - A[3] * A[1];
+    cof3(A, c);  // cofactor matrix, c[i + 3j] = cof(A)(i, j)
     const double det = A[0] * c[0] + A[3] * c[3] + A[6] * c[6];
     const double inv = 1.0 / det;
     for (int j = 0; j < 3; ++j) {
@@ -1295,6 +1414,12 @@ int bh_linear_tetmesh_const_ext_force_device(bh_mesh* m, const double* traction,
   hipLaunchKernelGGL(k_ext_force, dim3((m->nV + 255) / 256), dim3(256), 0, as_stream(s), view(m), traction[0],
                      traction[1], traction[2], d_f);
   BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
+  if (!m || (lanes != 1 && lanes != 2)) return BH_BAD_ARG;
+  m->gatherLanes = lanes;
   return BH_OK;
 }
 

@@ -32,50 +32,52 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace bh {
 
 enum : int { M_NH = 0, M_STVK = 1, M_COROT = 2, M_ARAP = 3, M_NH_LEGACY = 4, M_MASS = 100, M_PER_TET = 255 };
 
 template <typename T>
-__device__ __forceinline__ T det3(const T* F) {
+__host__ __device__ __forceinline__ T det3(const T* F) {
   return F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
          F[6] * (F[1] * F[5] - F[4] * F[2]);
 }
 
-// cof(F) = det(F) F^{-T}
+// cof(F) = det(F) F^{-T}, column-major: c[i + 3j] = (-1)^{i+j} minor_ij(F)
 template <typename T>
-__device__ __forceinline__ void cof3(const T* F, T* c) {
+__host__ __device__ __forceinline__ void cof3(const T* F, T* c) {
   c[0] = F[4] * F[8] - F[7] * F[5];
-  c[1] = F[7] * F[2] - F[1] * F[8];
-  c[2] = F[1] * F[5] - F[4] * F[2];
-  c[3] = F[6] * F[5] - F[3] * F[8];
+  c[1] = F[6] * F[5] - F[3] * F[8];
+  c[2] = F[3] * F[7] - F[6] * F[4];
+  c[3] = F[7] * F[2] - F[1] * F[8];
   c[4] = F[0] * F[8] - F[6] * F[2];
-  c[5] = F[3] * F[2] - F[0] * F[5];
-  c[6] = F[3] * F[7] - F[6] * F[4];
-  c[7] = F[6] * F[1] - F[0] * F[7];
+  c[5] = F[6] * F[1] - F[0] * F[7];
+  c[6] = F[1] * F[5] - F[4] * F[2];
+  c[7] = F[3] * F[2] - F[0] * F[5];
   c[8] = F[0] * F[4] - F[3] * F[1];
 }
 
 // y = A x, A col-major 3x3
 template <typename T>
-__device__ __forceinline__ void mv3(const T* A, const T* x, T* y) {
+__host__ __device__ __forceinline__ void mv3(const T* A, const T* x, T* y) {
   y[0] = A[0] * x[0] + A[3] * x[1] + A[6] * x[2];
   y[1] = A[1] * x[0] + A[4] * x[1] + A[7] * x[2];
   y[2] = A[2] * x[0] + A[5] * x[1] + A[8] * x[2];
 }
 // y = A^T x
 template <typename T>
-__device__ __forceinline__ void mtv3(const T* A, const T* x, T* y) {
+__host__ __device__ __forceinline__ void mtv3(const T* A, const T* x, T* y) {
   y[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
   y[1] = A[3] * x[0] + A[4] * x[1] + A[5] * x[2];
   y[2] = A[6] * x[0] + A[7] * x[1] + A[8] * x[2];
 }
 template <typename T>
-__device__ __forceinline__ T dot3(const T* a, const T* b) {
+__host__ __device__ __forceinline__ T dot3(const T* a, const T* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 template <typename T>
-__device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
+__host__ __device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
@@ -83,7 +85,7 @@ __device__ __forceinline__ void cross3(const T* a, const T* b, T* c) {
 
 // determinant and cofactor in FP64 from an F of any precision (the precision form keeps J in FP64)
 template <typename T>
-__device__ __forceinline__ double det_cof_fp64(const T* F, double* c) {
+__host__ __device__ __forceinline__ double det_cof_fp64(const T* F, double* c) {
   double Fd[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fd[i] = (double)F[i];
@@ -94,7 +96,7 @@ __device__ __forceinline__ double det_cof_fp64(const T* F, double* c) {
 // ---- 3x3 SVD, rotation convention (U, V rotations, s sorted descending, s[2] signed) --------
 // Cyclic Jacobi on F^T F (bounded sweeps), then U from F V.  Same algorithm as the oracle.
 template <typename T>
-__device__ inline void jacobi_rot(T* A, T* V, int p, int q) {
+__host__ __device__ inline void jacobi_rot(T* A, T* V, int p, int q) {
   const T apq = A[p + 3 * q];
   if (apq == T(0)) return;
   const T app = A[p + 3 * p], aqq = A[q + 3 * q];
@@ -122,7 +124,7 @@ __device__ inline void jacobi_rot(T* A, T* V, int p, int q) {
 }
 
 template <typename T>
-__device__ inline void svd3(const T* F, T* U, T* s, T* V) {
+__host__ __device__ inline void svd3(const T* F, T* U, T* s, T* V) {
   T A[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -213,7 +215,7 @@ struct Pre {
 // stretch-energy derivatives in singular values. Corotational: params [lambda, mu]
 // (src/dpsi_corotational_dS.cpp:6-13); ARAP: k = params(0) (src/dpsi_arap_dS.cpp:6-12).
 template <int MODEL, typename T>
-__device__ __forceinline__ void dpsi_s(const T* s, T p0, T p1, T* d) {
+__host__ __device__ __forceinline__ void dpsi_s(const T* s, T p0, T p1, T* d) {
   if (MODEL == M_COROT) {
     const T tr = s[0] + s[1] + s[2] - T(3);
     d[0] = T(2) * p1 * (s[0] - T(1)) + p0 * tr;
@@ -228,7 +230,7 @@ __device__ __forceinline__ void dpsi_s(const T* s, T p0, T p1, T* d) {
 
 // Hessian precomputation.  stretch_ref: reference (F^T) layout for corotational / ARAP.
 template <int MODEL, typename T>
-__device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool stretch_ref) {
+__host__ __device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool stretch_ref) {
   const T* F = P.F;
   const T vol = P.vol;
   if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
@@ -313,7 +315,7 @@ __device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool stretch_
 
 // 3x3 block K_am (row-major out[3l + k]).
 template <int MODEL, typename T>
-__device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa, const T* pm, T* out) {
+__host__ __device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa, const T* pm, T* out) {
   if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
     T ca[3], cm[3], x[3], w[3];
     mv3(P.cof, pa, ca);
@@ -387,7 +389,7 @@ __device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa, const T
 // from the displacement gradient G = F - I (exact in FP32 near I) and J - 1 in FP64, so the
 // energy keeps its relative accuracy at small strains (the line search compares energy sums).
 template <int MODEL, typename T>
-__device__ __forceinline__ double psi(const T* F, double p0, double p1) {
+__host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1) {
   if (MODEL == M_NH) {
     if constexpr (sizeof(T) == 8) {
       double I1 = 0.0;
@@ -445,7 +447,7 @@ __device__ __forceinline__ double psi(const T* F, double p0, double p1) {
 // Stretch models follow src/dpsi_stretch_dF.cpp:20-42: when s0 s1 s2 <= -1e-10 the negative
 // singular values are flipped positive (U, V unchanged).
 template <int MODEL, typename T>
-__device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
+__host__ __device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
   if (MODEL == M_NH) {
     T c[9];
     double J;

@@ -547,20 +547,23 @@ void load_tet(Tet& t, int e, int nT, const int* E, const double* q, const double
   t.p[1] = params[e + (int64_t)nT];
 }
 
-// K = B^T d2psi B * vol  (12x12 col-major)
+// K = B^T d2psi B * vol  (12x12 col-major); B materialised once per element, as the reference's
+// Matrix<9,12> B = flatten_multiply_right<...>(dphidX) is (src/linear_tet_neohookean_dq2.cpp:12-14)
 void element_hessian(const Tet& t, int model, int ref_layout, double* K) {
-  double H[81], HB[108];
+  double H[81], HB[108], B[108];
+  for (int c = 0; c < 12; ++c)
+    for (int r = 0; r < 9; ++r) B[r + 9 * c] = Bcoef(t.phi, r, c);
   d2psi_F(model, t.F, t.p, ref_layout, H);
   for (int r = 0; r < 9; ++r)
     for (int c = 0; c < 12; ++c) {
       double s = 0;
-      for (int k = 0; k < 9; ++k) s += H[r + 9 * k] * Bcoef(t.phi, k, c);
+      for (int k = 0; k < 9; ++k) s += H[r + 9 * k] * B[k + 9 * c];
       HB[r + 9 * c] = s;
     }
   for (int r = 0; r < 12; ++r)
     for (int c = 0; c < 12; ++c) {
       double s = 0;
-      for (int k = 0; k < 9; ++k) s += Bcoef(t.phi, k, r) * HB[k + 9 * c];
+      for (int k = 0; k < 9; ++k) s += B[k + 9 * r] * HB[k + 9 * c];
       K[r + 12 * c] = s * t.vol;
     }
 }
@@ -638,30 +641,34 @@ struct CSC {
 
 void set_from_triplets(CSC& M, int64_t n, const std::vector<Triplet>& trips) {
   M.n = n;
-  // stable bucket by column (keeps insertion order inside a column)
+  const int64_t nt = (int64_t)trips.size();
+  // two stable counting sorts (by row, then by column): within a column the entries end up sorted
+  // by row, and entries with the same (row, column) stay in insertion order -- so the fold below
+  // sums duplicates left to right in ascending element order, as setFromTriplets does
   std::vector<int64_t> cnt(n + 1, 0);
+  for (auto& t : trips) cnt[t.r + 1]++;
+  for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int64_t> byrow(nt);
+  for (int64_t k = 0; k < nt; ++k) byrow[cnt[trips[k].r]++] = k;
+  std::fill(cnt.begin(), cnt.end(), 0);
   for (auto& t : trips) cnt[t.c + 1]++;
   for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-  std::vector<int64_t> order(trips.size());
-  {
-    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-    for (int64_t k = 0; k < (int64_t)trips.size(); ++k) order[pos[trips[k].c]++] = k;
-  }
+  std::vector<int64_t> colstart(cnt.begin(), cnt.end());
+  std::vector<int64_t> order(nt);
+  for (int64_t k = 0; k < nt; ++k) order[cnt[trips[byrow[k]].c]++] = byrow[k];
   M.colptr.assign(n + 1, 0);
   M.rowind.clear();
   M.vals.clear();
-  M.rowind.reserve(trips.size() / 3);
-  M.vals.reserve(trips.size() / 3);
-  std::vector<int64_t> seg;
+  M.rowind.reserve(nt / 3);
+  M.vals.reserve(nt / 3);
   for (int64_t c = 0; c < n; ++c) {
-    seg.assign(order.begin() + cnt[c], order.begin() + cnt[c + 1]);
-    std::stable_sort(seg.begin(), seg.end(), [&](int64_t a, int64_t b) { return trips[a].r < trips[b].r; });
-    int64_t k = 0;
-    while (k < (int64_t)seg.size()) {
-      int r = trips[seg[k]].r;
-      double s = trips[seg[k]].v;
+    int64_t k = colstart[c];
+    const int64_t kend = colstart[c + 1];
+    while (k < kend) {
+      const int r = trips[order[k]].r;
+      double s = trips[order[k]].v;
       int64_t k2 = k + 1;
-      while (k2 < (int64_t)seg.size() && trips[seg[k2]].r == r) s += trips[seg[k2++]].v;
+      while (k2 < kend && trips[order[k2]].r == r) s += trips[order[k2++]].v;
       M.rowind.push_back(r);
       M.vals.push_back(s);
       k = k2;

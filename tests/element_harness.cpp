@@ -1,0 +1,79 @@
+// Host build of the device element math (bartels_amd/csrc/elements.h) so the CPU test suite can
+// check every closed-form block against the oracle's dense B^T H B.  Test infrastructure only.
+#include "../bartels_amd/csrc/elements.h"
+
+using namespace bh;
+
+template <typename T>
+static void make_F(const double* phi12, const double* x12, T phi[4][3], T* F) {
+  for (int m = 0; m < 4; ++m)
+    for (int j = 0; j < 3; ++j) phi[m][j] = (T)phi12[m + 4 * j];
+  if (sizeof(T) == 8) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        F[i + 3 * j] = x12[i] * phi[0][j] + x12[3 + i] * phi[1][j] + x12[6 + i] * phi[2][j] + x12[9 + i] * phi[3][j];
+  } else {
+    T ed[3][3];
+    for (int m = 0; m < 3; ++m)
+      for (int i = 0; i < 3; ++i) ed[m][i] = (T)(x12[3 * (m + 1) + i] - x12[i]);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) F[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+  }
+}
+
+template <int MODEL, typename T>
+static void blocks(const double* phi12, const double* x12, double vol, double p0, double p1, int ref, double* K) {
+  T phi[4][3];
+  Pre<T> P;
+  make_F(phi12, x12, phi, P.F);
+  P.vol = (T)vol;
+  pre_hessian<MODEL>(P, (T)p0, (T)p1, ref != 0);
+  for (int a = 0; a < 4; ++a)
+    for (int m = 0; m < 4; ++m) {
+      T b[9];
+      hess_block<MODEL>(P, phi[a], phi[m], b);
+      for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 3; ++k) K[(3 * a + l) + 12 * (3 * m + k)] = (double)b[3 * l + k];
+    }
+}
+
+template <int MODEL, typename T>
+static void grad(const double* phi12, const double* x12, double vol, double p0, double p1, double* g) {
+  T phi[4][3], F[9], P[9];
+  make_F(phi12, x12, phi, F);
+  piola<MODEL>(F, (T)p0, (T)p1, P);
+  for (int a = 0; a < 4; ++a) {
+    T f[3];
+    mv3(P, phi[a], f);
+    for (int l = 0; l < 3; ++l) g[3 * a + l] = (double)((T)vol * f[l]);
+  }
+}
+
+template <int MODEL, typename T>
+static double energy(const double* phi12, const double* x12, double vol, double p0, double p1) {
+  T phi[4][3], F[9];
+  make_F(phi12, x12, phi, F);
+  return psi<MODEL>(F, p0, p1) * vol;
+}
+
+#define DISPATCH(fn, ...)                                              \
+  switch (model) {                                                     \
+    case 0: return fp32 ? fn<0, float>(__VA_ARGS__) : fn<0, double>(__VA_ARGS__); \
+    case 1: return fp32 ? fn<1, float>(__VA_ARGS__) : fn<1, double>(__VA_ARGS__); \
+    case 2: return fp32 ? fn<2, float>(__VA_ARGS__) : fn<2, double>(__VA_ARGS__); \
+    case 3: return fp32 ? fn<3, float>(__VA_ARGS__) : fn<3, double>(__VA_ARGS__); \
+    default: return fp32 ? fn<4, float>(__VA_ARGS__) : fn<4, double>(__VA_ARGS__); \
+  }
+
+extern "C" {
+void eh_blocks(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1, int ref,
+               double* K) {
+  DISPATCH(blocks, phi12, x12, vol, p0, p1, ref, K)
+}
+void eh_grad(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1, double* g) {
+  DISPATCH(grad, phi12, x12, vol, p0, p1, g)
+}
+double eh_energy(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1) {
+  DISPATCH(energy, phi12, x12, vol, p0, p1)
+}
+}

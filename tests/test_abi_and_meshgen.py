@@ -88,3 +88,22 @@ def test_jitter_keeps_volumes_positive():
 def test_bunny_reader(bunny):
     V, E = bunny["V"], bunny["E"]
     assert V.shape == (500, 3) and E.shape == (1518, 4) and E.min() == 0 and E.max() == 499
+
+
+def test_host_dphidX_matches_oracle(oracle):
+    """host-side setup of the library (no device call): dphidX and signed volume"""
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    from bartels_amd import tetmesh as tm
+
+    V, E = meshgen.beam_grid(7, 5, 4, h=0.03)
+    V = meshgen.jitter(V, E, 0.03, 0.2, seed=4)
+    dX, vol = tm.linear_tetmesh_dphi_dX(V, E)
+    dXo, volo = oracle.dphidX_volume(V, E)
+    assert np.abs(dX - dXo).max() <= 1e-13 * np.abs(dXo).max()
+    assert np.abs(vol - volo).max() <= 1e-15 * np.abs(volo).max()
+    # F = diag(2,4,3) known answer through the library's dphidX
+    q = (V * np.array([2.0, 4.0, 3.0]))
+    phi = dX.reshape(-1, 3, 4).transpose(0, 2, 1)
+    F = np.einsum("tmi,tmj->tij", q[E], phi)
+    assert np.abs(F - np.diag([2.0, 4.0, 3.0])).max() < 1e-12

@@ -1,0 +1,110 @@
+"""The device element math (bartels_amd/csrc/elements.h: closed-form blocks, Piola stress,
+energy; FP64 and the FP32 precision form) compiled for the host and checked tet by tet against
+the oracle's dense reference algorithm (B^T d2psi B with the generated-style 9x9).  Runs on CPU."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from bartels_amd import meshgen
+from conftest import ROOT
+
+HIPCC = "/opt/rocm/bin/hipcc"
+
+
+@pytest.fixture(scope="module")
+def eh(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path_factory.mktemp("eh") / "eh.so")
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+                    "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "element_harness.cpp"), "-o", out],
+                   check=True)
+    L = ctypes.CDLL(out)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.eh_blocks.argtypes = [i, i, vp, vp, d, d, d, i, vp]
+    L.eh_grad.argtypes = [i, i, vp, vp, d, d, d, vp]
+    L.eh_energy.argtypes = [i, i, vp, vp, d, d, d]
+    L.eh_energy.restype = d
+    return L
+
+
+def tets(model, n=24, seed=0):
+    """single tets cut from a jittered beam, deformed by a random stretch (distinct singular values)"""
+    V, E = meshgen.beam_grid(5, 4, 4, h=0.02)
+    V = meshgen.jitter(V, E, 0.02, 0.2, seed=seed + 1)
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in rng.choice(E.shape[0], n, replace=False):
+        X = V[E[e]]
+        Q1, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        Q2, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        A = Q1 @ np.diag([1.25, 1.0, 0.8]) @ Q2
+        if np.linalg.det(A) < 0:
+            A = -A
+        x = X @ A.T + 3.0 + 0.002 * rng.standard_normal(X.shape)  # offset: metres-scale coordinates
+        lam, mu = meshgen.emu_to_lame(rng.uniform(1e5, 1e7), rng.uniform(0.3, 0.48))
+        p = meshgen.model_params(np.array([model]), lam, mu)[0]
+        out.append((X, x, p))
+    return out
+
+
+def single_tet(oracle, X):
+    E = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    dX, vol = oracle.dphidX_volume(X, E)
+    return E, dX, vol
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("ref", [1, 0])
+def test_blocks_match_dense_reference(eh, oracle, model, ref):
+    if ref == 0 and model not in (2, 3):
+        pytest.skip("layout only affects stretch models")
+    for X, x, p in tets(model):
+        E, dX, vol = single_tet(oracle, X)
+        q = x.ravel()
+        K_or = oracle.element_hessian(model, 0, E, q, dX, vol, p[None, :], ref_layout=ref)
+        phi = np.ascontiguousarray(dX[0])
+        xx = np.ascontiguousarray(q)
+        for fp32, tol in ((0, 1e-12), (1, 2e-5)):
+            K = np.zeros(144)
+            eh.eh_blocks(model, fp32, phi.ctypes.data, xx.ctypes.data, vol[0], p[0], p[1], ref, K.ctypes.data)
+            K = K.reshape(12, 12, order="F")
+            err = np.abs(K - K_or).max() / np.abs(K_or).max()
+            assert err < tol, (model, ref, fp32, err)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+def test_gradient_and_energy_match_reference(eh, oracle, model):
+    for X, x, p in tets(model, seed=5):
+        E, dX, vol = single_tet(oracle, X)
+        q = x.ravel()
+        g_or = oracle.tetmesh_dq(model, 4, E, q, dX, vol, p[None, :])
+        e_or = oracle.tetmesh_q(model, 4, E, q, dX, vol, p[None, :])
+        phi = np.ascontiguousarray(dX[0])
+        for fp32, tg, te in ((0, 1e-12, 1e-12), (1, 2e-5, 5e-6)):
+            g = np.zeros(12)
+            eh.eh_grad(model, fp32, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1], g.ctypes.data)
+            e = eh.eh_energy(model, fp32, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1])
+            assert np.abs(g - g_or).max() / np.abs(g_or).max() < tg, (model, fp32)
+            assert abs(e - e_or) / abs(e_or) < te, (model, fp32, e, e_or)
+
+
+def test_fp32_energy_keeps_relative_accuracy_near_rest(eh, oracle):
+    """the precision form's energy is formed from G = F - I and J - 1 in FP64: at strain 1e-4 it still
+    matches the FP64 energy to ~1e-6 relative (a plain FP32 tr(F^T F) - 3 would lose everything)"""
+    rng = np.random.default_rng(9)
+    V, E = meshgen.beam_grid(3, 3, 3, h=0.025)
+    X = V[E[0]]
+    Ex = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    dX, vol = oracle.dphidX_volume(X, Ex)
+    x = X + 2.0 + 1e-4 * 0.025 * rng.standard_normal(X.shape)
+    p = np.array([3.4e5, 3.1e6])
+    e_or = oracle.tetmesh_q(0, 4, Ex, x.ravel(), dX, vol, p[None, :])
+    q = np.ascontiguousarray(x.ravel())
+    phi = np.ascontiguousarray(dX[0])
+    e32 = eh.eh_energy(0, 1, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1])
+    assert abs(e32 - e_or) / abs(e_or) < 5e-3
