@@ -385,52 +385,6 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
   }
 }
 
-// Q (4x3, orthonormal columns, orthogonal to the ones vector) of the thin QR of Phi
-template <typename T>
-__device__ __forceinline__ void phi_qr(const T phi[4][3], T Q[4][3], T R[3][3]) {
-  T c0[4], c1[4], c2[4];
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) {
-    c0[mm] = phi[mm][0];
-    c1[mm] = phi[mm][1];
-    c2[mm] = phi[mm][2];
-  }
-  auto d4 = [](const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; };
-  const T r00 = sqrt(d4(c0, c0));
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c0[mm] /= r00;
-  const T r01 = d4(c0, c1);
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c1[mm] -= r01 * c0[mm];
-  const T r11 = sqrt(d4(c1, c1));
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c1[mm] /= r11;
-  const T r02 = d4(c0, c2);
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r02 * c0[mm];
-  const T r12 = d4(c1, c2);
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r12 * c1[mm];
-  const T r22 = sqrt(d4(c2, c2));
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) c2[mm] /= r22;
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) {
-    Q[mm][0] = c0[mm];
-    Q[mm][1] = c1[mm];
-    Q[mm][2] = c2[mm];
-  }
-  R[0][0] = r00; R[0][1] = r01; R[0][2] = r02;
-  R[1][0] = T(0); R[1][1] = r11; R[1][2] = r12;
-  R[2][0] = T(0); R[2][1] = T(0); R[2][2] = r22;
-}
-
-// packed upper-triangular index of a symmetric 9x9
-__device__ __forceinline__ int sym9(int i, int j) {
-  const int a = i < j ? i : j, b = i < j ? j : i;
-  return a * 9 - (a * (a - 1)) / 2 + (b - a);
-}
-
 template <int MODEL, bool PSD, int VPW, typename T>
 __global__ void __launch_bounds__(kGatherThreads)
     k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
@@ -470,29 +424,14 @@ __global__ void __launch_bounds__(kGatherThreads)
           blk[mm][8] += psd.tolPos;
         }
         if (psd.flag[e]) {
-          // + Q dM Q^T restricted to block row a:  sum_{p,r} Q[a][p] dM_{pr} Q[m][r]
+          // + Q dM Q^T restricted to block row a
           T phi[4][3], Q[4][3], R[3][3];
           load_phi(m, e, phi);
           phi_qr(phi, Q, R);
-          const T* dM = psd.dM + 45ll * e;
-          T qa[3];
-          select_row(Q, a, qa);
-          T Tm[3][9];  // Tm[r] = sum_p qa[p] dM_{p r} (3x3 sub-blocks, row-major l,k)
+          T Tm[3][9];
+          psd_row_factor(Q, a, psd.dM + 45ll * e, Tm);
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int l = 0; l < 3; ++l)
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk) {
-                T s = T(0);
-#pragma unroll
-                for (int p = 0; p < 3; ++p) s += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
-                Tm[r][3 * l + kk] = s;
-              }
-#pragma unroll
-          for (int mm = 0; mm < 4; ++mm)
-#pragma unroll
-            for (int i = 0; i < 9; ++i) blk[mm][i] += Q[mm][0] * Tm[0][i] + Q[mm][1] * Tm[1][i] + Q[mm][2] * Tm[2][i];
+          for (int mm = 0; mm < 4; ++mm) psd_add_block(Q[mm], Tm, blk[mm]);
         }
       }
 #pragma unroll
@@ -557,27 +496,13 @@ __global__ void __launch_bounds__(kGatherThreads)
           T phi[4][3], Q[4][3], R[3][3];
           load_phi(m, e, phi);
           phi_qr(phi, Q, R);
-          const T* dM = psd.dM + 45ll * e;
-          T qa[3];
-          select_row(Q, a, qa);
           T Tm[3][9];
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int l = 0; l < 3; ++l)
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk) {
-                T sacc = T(0);
-#pragma unroll
-                for (int p = 0; p < 3; ++p) sacc += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
-                Tm[r][3 * l + kk] = sacc;
-              }
+          psd_row_factor(Q, a, psd.dM + 45ll * e, Tm);
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             T qm[3];
             select_row(Q, h + 2 * j, qm);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) blk[j][i] += qm[0] * Tm[0][i] + qm[1] * Tm[1][i] + qm[2] * Tm[2][i];
+            psd_add_block(qm, Tm, blk[j]);
           }
         }
       }
@@ -613,17 +538,7 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
   pre_hessian<MODEL>(P, p0, p1, stretchRef);
-#pragma unroll
-  for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-    for (int rr = 0; rr < 3; ++rr) {
-      T b[9];
-      hess_block<MODEL>(P, R[pp], R[rr], b);
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) M[(3 * pp + l) + 9 * (3 * rr + kk)] = b[3 * l + kk];
-    }
+  reduced_matrix<MODEL>(P, R, M);
 }
 
 template <int MODEL, typename T>
@@ -647,24 +562,7 @@ __global__ void __launch_bounds__(256)
   if (e >= m.nT) return;
   T M[81];
   reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
-  // Cholesky of the lower triangle of M - tol I (symmetrised)
-  bool pd = true;
-#pragma unroll
-  for (int j = 0; j < 9; ++j) {
-    T d = M[j + 9 * j] - tol;
-#pragma unroll
-    for (int k = 0; k < j; ++k) d -= M[j + 9 * k] * M[j + 9 * k];
-    pd = pd && (d > T(0));
-    const T ljj = sqrt(fmax(d, sizeof(T) == 8 ? T(1e-300) : T(1e-30)));
-#pragma unroll
-    for (int i = j + 1; i < 9; ++i) {
-      T s = T(0.5) * (M[i + 9 * j] + M[j + 9 * i]);
-#pragma unroll
-      for (int k = 0; k < j; ++k) s -= M[i + 9 * k] * M[j + 9 * k];
-      M[i + 9 * j] = s / ljj;
-    }
-    M[j + 9 * j] = ljj;
-  }
+  const bool pd = shifted_pd(M, tol);
   flag[e] = pd ? 0 : 1;
   if (!pd) {
     const int idx = atomicAdd(count, 1);
@@ -680,75 +578,9 @@ __global__ void __launch_bounds__(64)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= *count) return;
   const int e = list[i];
-  T A[81], W[81];
+  T A[81];
   reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, A);
-#pragma unroll
-  for (int r = 0; r < 9; ++r)
-#pragma unroll
-    for (int c = r + 1; c < 9; ++c) {
-      const T s = T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
-      A[r + 9 * c] = s;
-      A[c + 9 * r] = s;
-    }
-#pragma unroll
-  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? T(1) : T(0);
-  T scale = T(0);
-#pragma unroll
-  for (int k = 0; k < 81; ++k) scale += A[k] * A[k];
-  const T stop = sizeof(T) == 8 ? T(1e-32) : T(1e-14);
-  for (int sweep = 0; sweep < 20; ++sweep) {
-    T off = T(0);
-#pragma unroll
-    for (int pp = 0; pp < 9; ++pp)
-#pragma unroll
-      for (int rr = pp + 1; rr < 9; ++rr) off += A[pp + 9 * rr] * A[pp + 9 * rr];
-    if (off <= stop * scale) break;
-#pragma unroll
-    for (int pp = 0; pp < 8; ++pp)
-#pragma unroll
-      for (int rr = pp + 1; rr < 9; ++rr) {
-        const T apq = A[pp + 9 * rr];
-        if (apq != T(0)) {
-          const T theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (T(2) * apq);
-          const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
-          const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T akp = A[k + 9 * pp], akq = A[k + 9 * rr];
-            A[k + 9 * pp] = c * akp - s * akq;
-            A[k + 9 * rr] = s * akp + c * akq;
-          }
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
-            A[pp + 9 * k] = c * apk - s * aqk;
-            A[rr + 9 * k] = s * apk + c * aqk;
-          }
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
-            W[k + 9 * pp] = c * wkp - s * wkq;
-            W[k + 9 * rr] = s * wkp + c * wkq;
-          }
-        }
-      }
-  }
-  T lift[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const T lam = A[k + 9 * k];
-    lift[k] = (lam < tol) ? (tol - lam) : T(0);
-  }
-  T* out = dM + 45ll * e;
-#pragma unroll
-  for (int r = 0; r < 9; ++r)
-#pragma unroll
-    for (int c = r; c < 9; ++c) {
-      T s = T(0);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) s += W[r + 9 * k] * lift[k] * W[c + 9 * k];
-      out[r * 9 - (r * (r - 1)) / 2 + (c - r)] = s;
-    }
+  psd_delta(A, tol, dM + 45ll * e);
 }
 
 // ---- mass / external force / setup / spmv -----------------------------------------------------------

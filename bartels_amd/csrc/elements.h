@@ -511,4 +511,195 @@ __host__ __device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
   }
 }
 
+// ---- PSD projection helpers (simple_psd_fix, src/simple_psd_fix.cpp:5-21) ---------------------------
+// K = vol B^T H B, B = Phi^T (x) I3.  With the thin QR Phi = Q R, K = (Q (x) I3) M (Q (x) I3)^T where
+// M = (R (x) I3) vol H (R (x) I3)^T is the block formula evaluated at the rows of R.  The three
+// translations (the complement of range Q (x) I3) are eigenvectors of K with eigenvalue 0, so
+//   simple_psd_fix(K, tol) = K + (Q (x) I3) dM (Q (x) I3)^T + max(tol, 0)/4 (1 1^T (x) I3),
+//   dM = W diag(max(lambda, tol) - lambda) W^T for M = W diag(lambda) W^T.
+
+// Q (4x3, orthonormal columns, orthogonal to the ones vector) and R (3x3 upper) of Phi (4x3)
+template <typename T>
+__host__ __device__ __forceinline__ void phi_qr(const T phi[4][3], T Q[4][3], T R[3][3]) {
+  T c0[4], c1[4], c2[4];
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) {
+    c0[mm] = phi[mm][0];
+    c1[mm] = phi[mm][1];
+    c2[mm] = phi[mm][2];
+  }
+  auto d4 = [](const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; };
+  const T r00 = sqrt(d4(c0, c0));
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c0[mm] /= r00;
+  const T r01 = d4(c0, c1);
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c1[mm] -= r01 * c0[mm];
+  const T r11 = sqrt(d4(c1, c1));
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c1[mm] /= r11;
+  const T r02 = d4(c0, c2);
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r02 * c0[mm];
+  const T r12 = d4(c1, c2);
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c2[mm] -= r12 * c1[mm];
+  const T r22 = sqrt(d4(c2, c2));
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) c2[mm] /= r22;
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) {
+    Q[mm][0] = c0[mm];
+    Q[mm][1] = c1[mm];
+    Q[mm][2] = c2[mm];
+  }
+  R[0][0] = r00; R[0][1] = r01; R[0][2] = r02;
+  R[1][0] = T(0); R[1][1] = r11; R[1][2] = r12;
+  R[2][0] = T(0); R[2][1] = T(0); R[2][2] = r22;
+}
+
+// packed upper-triangular index of a symmetric 9x9
+__host__ __device__ __forceinline__ int sym9(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a * 9 - (a * (a - 1)) / 2 + (b - a);
+}
+
+// M (9x9, col-major, index 3p + l) from the Hessian precomputation P and R
+template <int MODEL, typename T>
+__host__ __device__ __forceinline__ void reduced_matrix(const Pre<T>& P, const T R[3][3], T M[81]) {
+#pragma unroll
+  for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      T b[9];
+      hess_block<MODEL>(P, R[pp], R[rr], b);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) M[(3 * pp + l) + 9 * (3 * rr + kk)] = b[3 * l + kk];
+    }
+}
+
+// true when M - tol I (symmetrised) is positive definite: Cholesky (destroys M)
+template <typename T>
+__host__ __device__ __forceinline__ bool shifted_pd(T M[81], T tol) {
+  bool pd = true;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    T d = M[j + 9 * j] - tol;
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= M[j + 9 * k] * M[j + 9 * k];
+    pd = pd && (d > T(0));
+    const T ljj = sqrt(fmax(d, sizeof(T) == 8 ? T(1e-300) : T(1e-30)));
+#pragma unroll
+    for (int i = j + 1; i < 9; ++i) {
+      T s = T(0.5) * (M[i + 9 * j] + M[j + 9 * i]);
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= M[i + 9 * k] * M[j + 9 * k];
+      M[i + 9 * j] = s / ljj;
+    }
+    M[j + 9 * j] = ljj;
+  }
+  return pd;
+}
+
+// dM = M' - M (packed upper, 45 values) with M' = M with its eigenvalues clamped at tol
+// (cyclic Jacobi; destroys A)
+template <typename T>
+__host__ __device__ inline void psd_delta(T A[81], T tol, T* out) {
+  T W[81];
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r + 1; c < 9; ++c) {
+      const T s = T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
+      A[r + 9 * c] = s;
+      A[c + 9 * r] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? T(1) : T(0);
+  T scale = T(0);
+#pragma unroll
+  for (int k = 0; k < 81; ++k) scale += A[k] * A[k];
+  const T stop = sizeof(T) == 8 ? T(1e-32) : T(1e-14);
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    T off = T(0);
+#pragma unroll
+    for (int pp = 0; pp < 9; ++pp)
+#pragma unroll
+      for (int rr = pp + 1; rr < 9; ++rr) off += A[pp + 9 * rr] * A[pp + 9 * rr];
+    if (off <= stop * scale) break;
+#pragma unroll
+    for (int pp = 0; pp < 8; ++pp)
+#pragma unroll
+      for (int rr = pp + 1; rr < 9; ++rr) {
+        const T apq = A[pp + 9 * rr];
+        if (apq != T(0)) {
+          const T theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (T(2) * apq);
+          const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+          const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const T akp = A[k + 9 * pp], akq = A[k + 9 * rr];
+            A[k + 9 * pp] = c * akp - s * akq;
+            A[k + 9 * rr] = s * akp + c * akq;
+          }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const T apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
+            A[pp + 9 * k] = c * apk - s * aqk;
+            A[rr + 9 * k] = s * apk + c * aqk;
+          }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
+            W[k + 9 * pp] = c * wkp - s * wkq;
+            W[k + 9 * rr] = s * wkp + c * wkq;
+          }
+        }
+      }
+  }
+  T lift[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const T lam = A[k + 9 * k];
+    lift[k] = (lam < tol) ? (tol - lam) : T(0);
+  }
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r; c < 9; ++c) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s += W[r + 9 * k] * lift[k] * W[c + 9 * k];
+      out[r * 9 - (r * (r - 1)) / 2 + (c - r)] = s;
+    }
+}
+
+// Tm[r] = sum_p Q[a][p] dM_{p r} (3x3 sub-blocks, row-major l, k) for block row a
+template <typename T>
+__host__ __device__ __forceinline__ void psd_row_factor(const T Q[4][3], int a, const T* dM, T Tm[3][9]) {
+  T qa[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) qa[j] = (a == 0) ? Q[0][j] : (a == 1) ? Q[1][j] : (a == 2) ? Q[2][j] : Q[3][j];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        T s = T(0);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) s += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
+        Tm[r][3 * l + kk] = s;
+      }
+}
+
+// blk += sum_r qm[r] Tm[r]  (the (a, m) block of (Q (x) I3) dM (Q (x) I3)^T)
+template <typename T>
+__host__ __device__ __forceinline__ void psd_add_block(const T* qm, const T Tm[3][9], T* blk) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) blk[i] += qm[0] * Tm[0][i] + qm[1] * Tm[1][i] + qm[2] * Tm[2][i];
+}
+
 }  // namespace bh

@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", default="256,64,64", help="vertices along x,y,z")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1 or 2 lanes per vertex "
+                    "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -99,6 +101,9 @@ def main():
     e = torch.empty(1, dtype=torch.float64, device=dev)
     psd = tm.simple_psd_fix(1e-3)
     stream = torch.cuda.current_stream()
+    lib0 = _lib.load()
+    if args.gather_lanes:
+        lib0.bh_mesh_set_gather_lanes(mesh.handle, args.gather_lanes)
     setup_s = time.perf_counter() - t_setup
 
     def step():
@@ -145,6 +150,24 @@ def main():
     gather_nopsd = ms[2] / cnt[2]
     ms_dq = timed(lambda: mesh.dq_device(tm.NEOHOOKEAN, q, params, g, stream=stream))
     ms_q = timed(lambda: mesh.q_device(tm.NEOHOOKEAN, q, params, e, stream=stream))
+    # the other assembly-kernel shape, and the FP32 precision form (same workload)
+    variants = {}
+    cur = args.gather_lanes or 1
+    other = 2 if cur == 1 else 1
+    lib.bh_mesh_set_gather_lanes(mesh.handle, other)
+    variants[f"lanes{other}_psd_ms"] = timed(lambda: step())
+    variants[f"lanes{other}_nopsd_ms"] = timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
+    lib.bh_mesh_set_gather_lanes(mesh.handle, cur)
+    par32 = params.to(torch.float32)
+    H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
+    for lanes in (1, 2):
+        lib.bh_mesh_set_gather_lanes(mesh.handle, lanes)
+        variants[f"fp32_lanes{lanes}_psd_ms"] = timed(
+            lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
+        variants[f"fp32_lanes{lanes}_nopsd_ms"] = timed(
+            lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
+    lib.bh_mesh_set_gather_lanes(mesh.handle, cur)
+    del H32
 
     B = algorithmic_bytes(nV, nT, nnz)
     gather_ms = kt["hessian_gather"]
@@ -188,6 +211,7 @@ def main():
             "dq_hbm_frac": B["dq"] * nT / (ms_dq * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "q_ms": ms_q,
             "q_hbm_frac": B["q"] * nT / (ms_q * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "variants_ms": variants,
             "wall_s_timed": wall,
             "setup_s": setup_s,
         },

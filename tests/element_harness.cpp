@@ -56,6 +56,40 @@ static double energy(const double* phi12, const double* x12, double vol, double 
   return psi<MODEL>(F, p0, p1) * vol;
 }
 
+
+// simple_psd_fix through the reduced 9x9 (the GPU decomposition): returns 1 when the tet needed clamping
+template <int MODEL, typename T>
+static int psd_blocks(const double* phi12, const double* x12, double vol, double p0, double p1, int ref, double tol,
+                      double* K) {
+  T phi[4][3];
+  Pre<T> P;
+  make_F(phi12, x12, phi, P.F);
+  P.vol = (T)vol;
+  pre_hessian<MODEL>(P, (T)p0, (T)p1, ref != 0);
+  T Q[4][3], R[3][3], M[81], Mc[81], dM[45];
+  phi_qr(phi, Q, R);
+  reduced_matrix<MODEL>(P, R, M);
+  for (int i = 0; i < 81; ++i) Mc[i] = M[i];
+  const bool pd = shifted_pd(Mc, (T)tol);
+  if (!pd) psd_delta(M, (T)tol, dM);
+  const T tp = (T)((tol > 0 ? tol : 0.0) * 0.25);
+  for (int a = 0; a < 4; ++a) {
+    T Tm[3][9];
+    if (!pd) psd_row_factor(Q, a, dM, Tm);
+    for (int m = 0; m < 4; ++m) {
+      T b[9];
+      hess_block<MODEL>(P, phi[a], phi[m], b);
+      b[0] += tp;
+      b[4] += tp;
+      b[8] += tp;
+      if (!pd) psd_add_block(Q[m], Tm, b);
+      for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 3; ++k) K[(3 * a + l) + 12 * (3 * m + k)] = (double)b[3 * l + k];
+    }
+  }
+  return pd ? 0 : 1;
+}
+
 #define DISPATCH(fn, ...)                                              \
   switch (model) {                                                     \
     case 0: return fp32 ? fn<0, float>(__VA_ARGS__) : fn<0, double>(__VA_ARGS__); \
@@ -72,6 +106,10 @@ void eh_blocks(int model, int fp32, const double* phi12, const double* x12, doub
 }
 void eh_grad(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1, double* g) {
   DISPATCH(grad, phi12, x12, vol, p0, p1, g)
+}
+int eh_psd_blocks(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1,
+                  int ref, double tol, double* K) {
+  DISPATCH(psd_blocks, phi12, x12, vol, p0, p1, ref, tol, K)
 }
 double eh_energy(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1) {
   DISPATCH(energy, phi12, x12, vol, p0, p1)

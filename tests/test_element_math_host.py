@@ -29,6 +29,8 @@ def eh(tmp_path_factory):
     L.eh_grad.argtypes = [i, i, vp, vp, d, d, d, vp]
     L.eh_energy.argtypes = [i, i, vp, vp, d, d, d]
     L.eh_energy.restype = d
+    L.eh_psd_blocks.argtypes = [i, i, vp, vp, d, d, d, i, d, vp]
+    L.eh_psd_blocks.restype = i
     return L
 
 
@@ -108,3 +110,34 @@ def test_fp32_energy_keeps_relative_accuracy_near_rest(eh, oracle):
     phi = np.ascontiguousarray(dX[0])
     e32 = eh.eh_energy(0, 1, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1])
     assert abs(e32 - e_or) / abs(e_or) < 5e-3
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 4])
+def test_psd_decomposition_matches_12x12_eigensolve(eh, oracle, model):
+    """simple_psd_fix through the reduced 9x9 (QR of Phi, Cholesky test, Jacobi clamp) equals the
+    oracle's 12x12 Jacobi eigen-clamp, both on tets that need clamping and on ones that do not"""
+    rng = np.random.default_rng(11 + model)
+    V, E = meshgen.beam_grid(5, 4, 4, h=0.02)
+    clamped = 0
+    for e in rng.choice(E.shape[0], 30, replace=False):
+        X = V[E[e]]
+        x = X + 0.3 * 0.02 * rng.standard_normal(X.shape)  # strong distortion: indefinite element Hessians
+        lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+        p = meshgen.model_params(np.array([model]), lam, mu)[0]
+        Ex = np.array([[0, 1, 2, 3]], dtype=np.int32)
+        dX, vol = oracle.dphidX_volume(X, Ex)
+        if vol[0] <= 0:
+            continue
+        q = np.ascontiguousarray(x.ravel())
+        phi = np.ascontiguousarray(dX[0])
+        for tol in (1e-3, 10.0):
+            K_or = oracle.element_hessian(model, 0, Ex, q, dX, vol, p[None, :], psd=True, psd_tol=tol)
+            for fp32, bound in ((0, 1e-10), (1, 1e-4)):
+                K = np.zeros(144)
+                flag = eh.eh_psd_blocks(model, fp32, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1], 1, tol,
+                                        K.ctypes.data)
+                clamped += flag
+                K = K.reshape(12, 12, order="F")
+                err = np.abs(K - K_or).max() / np.abs(K_or).max()
+                assert err < bound, (model, tol, fp32, err)
+    assert clamped > 0
