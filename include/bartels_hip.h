@@ -95,7 +95,7 @@ int bh_mesh_block_pattern_device(const bh_mesh* mesh, const int32_t** d_bptr, co
                                  int64_t* nblocks);
 int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
 /* Assembly-kernel shape: 1 lane per vertex (default) or 2 lanes per vertex (each lane owns two of
- * the four corner blocks; half the LDS per lane).  Results are bit-identical either way. */
+ * the four corner blocks; half the LDS per lane).  Same per-slot fold order either way. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
 
 /* ---- energy / gradient / Hessian (host arrays) -------------------------------------------- */

@@ -236,3 +236,28 @@ def test_determinism(tm):
         outs.append((H.cpu().numpy(), g.cpu().numpy(), e.cpu().numpy()))
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_gather_variants_agree(tm):
+    """one and two lanes per vertex: same per-slot fold order, same results"""
+    import ctypes
+
+    import torch
+
+    from bartels_amd import _lib
+
+    p = meshgen.config1(grid=(24, 8, 8))
+    mesh, dX, vol = make_mesh(tm, p.V, p.E)
+    q = torch.tensor(strained(p.V, 4), device="cuda")
+    params = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
+    lib = _lib.load()
+    outs = []
+    for lanes in (1, 2):
+        assert lib.bh_mesh_set_gather_lanes(mesh.handle, lanes) == 0
+        H = torch.empty(mesh.nnz, dtype=torch.float64, device="cuda")
+        mesh.dq2_device(0, q, params, H, callback=tm.simple_psd_fix(1e-3))
+        torch.cuda.synchronize()
+        outs.append(H.cpu().numpy())
+    diff = np.abs(outs[0] - outs[1]).max() / np.abs(outs[0]).max()
+    print(f"gather lanes 1 vs 2: max rel diff {diff:.2e}, bitwise equal: {np.array_equal(outs[0], outs[1])}")
+    assert diff < 1e-14
