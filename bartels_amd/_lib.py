@@ -49,6 +49,8 @@ class BhNewtonOpts(ctypes.Structure):
         ("pcg_tol", c_double),
         ("pcg_max_iterations", c_int),
         ("precond", c_int),
+        ("precond_handle", c_void_p),
+        ("precond_shift", c_double),
     ]
 
 
@@ -101,6 +103,13 @@ SIGNATURES = {
     "bh_linear_tetmesh_q_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq2_f32_device": (c_int, [_VP, c_int, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
+    "bh_precond_create": (c_int, [_VP, P(c_void_p)]),
+    "bh_precond_destroy": (c_int, [_VP]),
+    "bh_precond_info": (c_int, [_VP, P(c_int), P(c_int64)]),
+    "bh_precond_factor_bic0_device": (c_int, [_VP, _VP, _VP, c_double, P(c_int), _VP]),
+    "bh_precond_set_factor_device": (c_int, [_VP, _VP, _VP, _VP]),
+    "bh_precond_lists_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p)]),
+    "bh_precond_apply_device": (c_int, [_VP, _VP, _VP, _VP]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),
 }

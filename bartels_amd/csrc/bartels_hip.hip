@@ -1289,5 +1289,6 @@ int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* 
 
 }  // extern "C"
 
-// device Newton (search direction by PCG, reference line search)
+// block lower-triangular preconditioners (colour sweeps), then the device Newton
+#include "precond_impl.h"
 #include "newton_impl.h"

@@ -297,6 +297,82 @@ __global__ void __launch_bounds__(256) k_pcg_p(int64_t n, const double* __restri
   if (i < n) p[i] = z[i] + beta * p[i];
 }
 
+// generic-preconditioner CG pieces: x = 0, r = b with partial r.r; x += a p, r -= a Ap with partial r.r
+__global__ void __launch_bounds__(kRedBlock) k_pcg_init_r(int64_t n, const double* __restrict__ b,
+                                                          double* __restrict__ x, double* __restrict__ r,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    rr = bi * bi;
+  }
+  rr = block_sum_256(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_pcg_update_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ Ap, double* __restrict__ part,
+                                                             const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  const int done = st->done;
+  if (done && done != 3) return;
+  const double alpha = st->alpha;
+  const int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    rr = ri * ri;
+  }
+  rr = block_sum_256(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
+}
+
+// st from separate r.r and r.z partial arrays (start: also bb; loop: beta and the convergence test)
+__global__ void __launch_bounds__(kRedBlock) k_pcg_scalars2(const double* __restrict__ prr, int nrr,
+                                                            const double* __restrict__ prz, int nrz, int start,
+                                                            double tol, int maxit, PcgState* st) {
+  __shared__ double sh[4];
+  if (!start && st->done) {
+    if (st->done == 3 && threadIdx.x == 0 && st->alpha != 0.0) {
+      st->alpha = 0.0;
+      st->iters += 1;
+    }
+    return;
+  }
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nrz; i += kRedBlock) a += prz[i];
+  for (int i = threadIdx.x; i < nrr; i += kRedBlock) b += prr[i];
+  a = block_sum_256(a, sh);
+  b = block_sum_256(b, sh);
+  if (threadIdx.x == 0) {
+    if (start) {
+      st->rz = a;
+      st->rr = b;
+      st->bb = b;
+      st->tol2 = tol * tol;
+      st->iters = 0;
+      st->maxit = maxit;
+      st->done = (b == 0.0) ? 1 : 0;
+    } else {
+      st->beta = a / st->rz;
+      st->rz = a;
+      st->rr = b;
+      st->iters += 1;
+      if (b <= st->tol2 * st->bb)
+        st->done = 1;
+      else if (st->iters >= st->maxit)
+        st->done = 2;
+    }
+  }
+}
+
 // y = a + s * b
 __global__ void __launch_bounds__(256) k_axpy_to(int64_t n, const double* __restrict__ a, double s,
                                                  const double* __restrict__ b, double* __restrict__ y) {
@@ -316,7 +392,8 @@ struct NewtonWs {
 
 int newton_workspace(bh_mesh* m, NewtonWs& w) {
   const int64_t n = 3ll * m->nV;
-  const int64_t nParts = 2ll * std::max<int64_t>(kRedMaxBlocks, (3ll * m->nV + kRedBlock - 1) / kRedBlock + 1);
+  const int64_t nParts = 2ll * std::max<int64_t>(kRedMaxBlocks, (3ll * m->nV + kRedBlock - 1) / kRedBlock + 1) +
+                         kRedMaxBlocks + 16;
   const size_t need = sizeof(double) * ((size_t)m->nnz + 8 * (size_t)n + 9 * (size_t)m->nV + nParts + 16) +
                       sizeof(PcgState) + 256;
   if (m->wsBytes < need) {
@@ -396,6 +473,46 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, NewtonW
   return BH_OK;
 }
 
+// PCG with a colour-sweep block-triangular preconditioner (same device-resident scalars)
+int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, const bh_precond* P, NewtonWs& w,
+                     int* iters, int* done, hipStream_t s) {
+  const int nV = m->nV;
+  const int64_t n = 3ll * nV;
+  const int gr = (int)((n + kRedBlock - 1) / kRedBlock);
+  const int nz = red_blocks(n);
+  double* part2 = w.part + gr + 8;  // second partial array (r.z), after the r.r / p.Ap partials
+  int* dflag = &w.st->done;
+  hipLaunchKernelGGL(k_pcg_init_r, dim3(gr), dim3(kRedBlock), 0, s, n, w.b, w.x, w.r, w.part);
+  int st = precond_apply(P, w.r, w.z, nullptr, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(w.p, w.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_dot_partials, dim3(nz), dim3(kRedBlock), 0, s, w.r, w.z, n, part2);
+  hipLaunchKernelGGL(k_pcg_scalars2, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, part2, nz, 1, o->pcg_tol,
+                     o->pcg_max_iterations, w.st);
+  BH_CHECK(hipGetLastError());
+  PcgState hs{};
+  for (;;) {
+    for (int k = 0; k < kPcgBatch; ++k) {
+      hipLaunchKernelGGL(k_pcg_spmv, dim3(gr), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, w.vals, fixed, w.p, w.Ap,
+                         w.part, w.st);
+      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, w.st);
+      hipLaunchKernelGGL(k_pcg_update_xr, dim3(gr), dim3(kRedBlock), 0, s, n, w.x, w.r, w.p, w.Ap, w.part, w.st);
+      if ((st = precond_apply(P, w.r, w.z, dflag, s))) return st;
+      hipLaunchKernelGGL(k_dot_partials, dim3(nz), dim3(kRedBlock), 0, s, w.r, w.z, n, part2);
+      hipLaunchKernelGGL(k_pcg_scalars2, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, part2, nz, 0, o->pcg_tol,
+                         o->pcg_max_iterations, w.st);
+      hipLaunchKernelGGL(k_pcg_p, dim3(gr), dim3(256), 0, s, n, w.z, w.p, w.st);
+    }
+    BH_CHECK(hipGetLastError());
+    BH_CHECK(hipMemcpyAsync(&hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    if (hs.done) break;
+  }
+  *iters = hs.iters;
+  *done = hs.done;
+  return BH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -418,6 +535,8 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->pcg_tol = 1e-10;
   o->pcg_max_iterations = 20000;
   o->precond = BH_PRECOND_JACOBI;
+  o->precond_handle = nullptr;
+  o->precond_shift = 0.0;
 }
 
 int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* params, const uint8_t* fixed,
@@ -468,7 +587,13 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
     if (st) return st;
     hipLaunchKernelGGL(k_negate, dim3(gn), dim3(256), 0, s, n, w.g, w.b);
     int pits = 0, pdone = 0;
-    if ((st = pcg_solve(m, o, fixed, w, &pits, &pdone, s))) return st;
+    if (o->precond == BH_PRECOND_BIC0) {
+      if (!o->precond_handle) return BH_BAD_ARG;
+      if ((st = precond_factor_bic0(o->precond_handle, w.vals, fixed, o->precond_shift, nullptr, s))) return st;
+      if ((st = pcg_solve_factor(m, o, fixed, o->precond_handle, w, &pits, &pdone, s))) return st;
+    } else if ((st = pcg_solve(m, o, fixed, w, &pits, &pdone, s))) {
+      return st;
+    }
     R.pcg_iterations_total += pits;
     if (pdone != 1) R.pcg_failures += 1;
     // line search on the objective (reference semantics unless the options say otherwise)

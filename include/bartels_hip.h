@@ -158,9 +158,32 @@ int bh_linear_tetmesh_const_ext_force_device(bh_mesh* mesh, const double* tracti
 /* ---- Newton ------------------------------------------------------------------------------------ */
 
 typedef enum {
-  BH_PRECOND_JACOBI = 0, /* 3x3 block-Jacobi (block diagonal of H) */
-  BH_PRECOND_POINT_JACOBI = 1
-} bh_precond;
+  BH_PRECOND_JACOBI = 0,       /* 3x3 block-Jacobi (block diagonal of H) */
+  BH_PRECOND_POINT_JACOBI = 1,
+  BH_PRECOND_BIC0 = 2,         /* block IC(0) in colour order, refactored from H every Newton iteration */
+  BH_PRECOND_FACTOR = 3        /* a block lower-triangular factor supplied through precond_handle */
+} bh_precond_kind;
+
+/* Block lower-triangular preconditioner M = L L^T on the Hessian's own 3x3-block pattern, in a
+ * greedy vertex-colour order (no two neighbours share a colour, so each triangular sweep is one
+ * parallel kernel per colour).  No counterpart in the reference (its direction is a direct solve). */
+typedef struct bh_precond bh_precond;
+int bh_precond_create(bh_mesh* mesh, bh_precond** precond);
+int bh_precond_destroy(bh_precond* precond);
+int bh_precond_info(const bh_precond* precond, int* ncolors, int64_t* nlower);
+/* block IC(0) of the masked H (fixed DOFs -> identity); shift > 0 adds shift*|H_ii| to the diagonal.
+ * failures (host, optional): number of 3x3 pivots that were not positive. */
+int bh_precond_factor_bic0_device(bh_precond* precond, const double* d_vals, const uint8_t* d_fixed_dof, double shift,
+                                  int* failures, bh_stream_t stream);
+/* install a factor: d_lower 9 floats per lower block (row-major, in the order of the lower lists),
+ * d_diag_inv 6 floats per vertex (inverse of L_vv, packed lower triangular, row-major). */
+int bh_precond_set_factor_device(bh_precond* precond, const float* d_lower, const float* d_diag_inv, bh_stream_t stream);
+/* device views of the lower lists: lptr (nV+1), lcol and lslot (mesh block index of (v, n)) per lower
+ * block, dslot (mesh block index of (v, v)), vertices grouped by colour */
+int bh_precond_lists_device(const bh_precond* precond, const int32_t** lptr, const int32_t** lcol,
+                            const int32_t** lslot, const int32_t** dslot, const int32_t** colverts);
+/* z = (L L^T)^{-1} r */
+int bh_precond_apply_device(bh_precond* precond, const double* d_r, double* d_z, bh_stream_t stream);
 
 typedef struct {
   int model;               /* bh_model, or BH_MODEL_PER_TET with model_ids */
@@ -177,7 +200,9 @@ typedef struct {
   int reject_nonfinite;    /* 1: a non-finite trial energy counts as "not decreased" (the reference accepts it) */
   double pcg_tol;          /* relative residual of the inner PCG solve */
   int pcg_max_iterations;
-  int precond;             /* bh_precond */
+  int precond;             /* bh_precond_kind */
+  bh_precond* precond_handle; /* BH_PRECOND_BIC0 / BH_PRECOND_FACTOR */
+  double precond_shift;    /* BH_PRECOND_BIC0 diagonal shift */
 } bh_newton_opts;
 
 typedef struct {
