@@ -111,6 +111,7 @@ SIGNATURES = {
     "bh_precond_lists_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p)]),
     "bh_precond_apply_device": (c_int, [_VP, _VP, _VP, _VP]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
+    "bh_mesh_psd_count": (c_int, [_VP, P(c_int64)]),
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),
 }
 

@@ -1,12 +1,15 @@
 // gfx950 kernels and the C ABI (include/bartels_hip.h) for the linear-tetmesh hot path.
 //
 // Data layout in HBM (built once by bh_mesh_create):
-//   tetE    int4 per tet (AoS, 16 B)                 -- from E (nT x 4 column-major)
-//   tetPhi  12 doubles per tet (AoS, 96 B), Phi(m,j) at 12e + m + 4j  -- from dphidX
-//   vol     1 double per tet
-//   incPtr  nV+1, inc[k] = (e << 2) | corner for the tets incident to a vertex, ascending e
-//   incRank packed ranks (4 x 8 bit) of the tet's four vertices in the vertex's sorted neighbour list
-//   bptr    nV+1 block-row pointer; bcol sorted neighbour vertices (the 3x3-block pattern)
+//   E, dX, vol  the reference's own column-major (structure-of-arrays) layouts: E(e,k) at e + k nT,
+//               Phi(m,j) = dphidX(e, m + 4j) at e + (m + 4j) nT -- lanes that read consecutive tets
+//               read consecutive addresses
+//   incE, incR  per-vertex incident-tet lists, interleaved per group of 64 vertices: slot k of the
+//               vertex in lane i of group g sits at gBase[g] + 64 k + i (slot-major, so a wave reads
+//               each slot with one coalesced access); incE = (e << 2) | corner in ascending e, -1 pads
+//               to the group's longest list; incR packs the ranks (4 x 8 bit) of the tet's four
+//               vertices in the vertex's sorted neighbour list
+//   bptr, bcol  block-row pointer and sorted neighbour vertices (the 3x3-block pattern)
 // CSR values of the reference pattern: row 3v+l starts at 9*bptr[v] + 3*l*nb(v); the block of
 // neighbour rank r sits at +3r.  Rows 3v..3v+2 therefore form one contiguous segment.
 //
@@ -52,14 +55,15 @@ struct bh_mesh {
   // host copies of the pattern
   std::vector<int32_t> h_bptr, h_bcol;
   // device arrays
-  int4* tetE = nullptr;
-  double* tetPhi = nullptr;
+  int32_t* E = nullptr;     // nT x 4, column-major
+  double* dX = nullptr;     // nT x 12, column-major
   double* vol = nullptr;
-  float* tetPhi32 = nullptr;  // precision form (made on first use)
+  float* dX32 = nullptr;    // precision form (made on first use)
   float* vol32 = nullptr;
-  int32_t* incPtr = nullptr;
-  int32_t* inc = nullptr;
-  uint32_t* incRank = nullptr;
+  int64_t* gBase = nullptr; // per group of 64 vertices: offset of its interleaved incidence lists
+  int32_t* gLen = nullptr;  // per group: longest list
+  int32_t* incE = nullptr;
+  uint32_t* incR = nullptr;
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
@@ -122,21 +126,32 @@ namespace {
 
 struct MeshView {
   int nV, nT;
-  const int4* __restrict__ tetE;
-  const double* __restrict__ tetPhi;
-  const float* __restrict__ tetPhi32;
+  const int32_t* __restrict__ E;
+  const double* __restrict__ dX;
+  const float* __restrict__ dX32;
   const double* __restrict__ vol;
   const float* __restrict__ vol32;
-  const int32_t* __restrict__ incPtr;
-  const int32_t* __restrict__ inc;
-  const uint32_t* __restrict__ incRank;
+  const int64_t* __restrict__ gBase;
+  const int32_t* __restrict__ gLen;
+  const int32_t* __restrict__ incE;
+  const uint32_t* __restrict__ incR;
   const int32_t* __restrict__ bptr;
   const int32_t* __restrict__ bcol;
 };
 
 MeshView view(const bh_mesh* m) {
-  return MeshView{m->nV, m->nT, m->tetE, m->tetPhi, m->tetPhi32, m->vol, m->vol32, m->incPtr, m->inc, m->incRank,
+  return MeshView{m->nV, m->nT, m->E, m->dX, m->dX32, m->vol, m->vol32, m->gBase, m->gLen, m->incE, m->incR,
                   m->bptr, m->bcol};
+}
+
+// the interleaved incidence list of vertex v: entries at first + 64 k, k < len (code < 0 ends it)
+struct IncList {
+  int64_t first;
+  int len;
+};
+__device__ __forceinline__ IncList inc_list(const MeshView& m, int v) {
+  const int g = v >> 6;
+  return IncList{m.gBase[g] + (v & 63), m.gLen[g]};
 }
 
 template <typename T>
@@ -150,30 +165,21 @@ struct PsdView {
 // ---- tet loads -------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void load_phi(const MeshView& m, int e, T phi[4][3]) {
-  T r[12];
-  if constexpr (sizeof(T) == 8) {
-    const double2* p2 = reinterpret_cast<const double2*>(m.tetPhi + 12ll * e);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const double2 t = p2[i];
-      r[2 * i] = t.x;
-      r[2 * i + 1] = t.y;
-    }
-  } else {
-    const float4* p4 = reinterpret_cast<const float4*>(m.tetPhi32 + 12ll * e);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const float4 t = p4[i];
-      r[4 * i] = t.x;
-      r[4 * i + 1] = t.y;
-      r[4 * i + 2] = t.z;
-      r[4 * i + 3] = t.w;
-    }
-  }
+  const int64_t nT = m.nT;
 #pragma unroll
   for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) phi[mm][j] = r[mm + 4 * j];
+    for (int j = 0; j < 3; ++j) {
+      if constexpr (sizeof(T) == 8)
+        phi[mm][j] = m.dX[e + (mm + 4 * j) * nT];
+      else
+        phi[mm][j] = m.dX32[e + (mm + 4 * j) * nT];
+    }
+}
+
+__device__ __forceinline__ int4 load_E(const MeshView& m, int e) {
+  const int64_t nT = m.nT;
+  return make_int4(m.E[e], m.E[e + nT], m.E[e + 2 * nT], m.E[e + 3 * nT]);
 }
 
 template <typename T>
@@ -234,7 +240,7 @@ __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __
                                              const T* __restrict__ params, const uint8_t* __restrict__ mids, int e) {
   T phi[4][3], F[9];
   load_phi(m, e, phi);
-  load_F(q, m.tetE[e], phi, F);
+  load_F(q, load_E(m, e), phi, F);
   const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
   double v;
   switch (model_at<MODEL>(mids, e)) {
@@ -295,13 +301,14 @@ __global__ void __launch_bounds__(256)
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= m.nV) return;
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-  const int k1 = m.incPtr[v + 1];
-  for (int k = m.incPtr[v]; k < k1; ++k) {
-    const int code = m.inc[k];
+  const IncList L = inc_list(m, v);
+  for (int k = 0; k < L.len; ++k) {
+    const int code = m.incE[L.first + 64ll * k];
+    if (code < 0) break;
     const int e = code >> 2, a = code & 3;
     T phi[4][3], F[9], P[9];
     load_phi(m, e, phi);
-    load_F(q, m.tetE[e], phi, F);
+    load_F(q, load_E(m, e), phi, F);
     const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
     switch (model_at<MODEL>(mids, e)) {
       case M_NH: piola<M_NH>(F, p0, p1, P); break;
@@ -342,7 +349,7 @@ __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __re
   T phi[4][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, m.tetE[e], phi, P.F);
+  load_F(q, load_E(m, e), phi, P.F);
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   T pa[3];
@@ -371,7 +378,7 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
   T phi[4][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, m.tetE[e], phi, P.F);
+  load_F(q, load_E(m, e), phi, P.F);
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   T pa[3];
@@ -402,11 +409,13 @@ __global__ void __launch_bounds__(kGatherThreads)
   if ((int)threadIdx.x < VPW && v < m.nV) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
-    const int k1 = m.incPtr[v + 1];
-    for (int k = m.incPtr[v]; k < k1; ++k) {
-      const int code = m.inc[k];
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
       const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incRank[k];
+      const uint32_t rk = m.incR[idx];
       T blk[4][9];
       switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
@@ -471,11 +480,13 @@ __global__ void __launch_bounds__(kGatherThreads)
   if (vl < VPW && v < m.nV) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
-    const int k1 = m.incPtr[v + 1];
-    for (int k = m.incPtr[v]; k < k1; ++k) {
-      const int code = m.inc[k];
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
       const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incRank[k];
+      const uint32_t rk = m.incR[idx];
       T blk[2][9];
       switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_blocks2<M_NH>(m, q, params, e, a, h, stretchRef, blk); break;
@@ -533,7 +544,7 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
   T phi[4][3], Q[4][3], R[3][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, m.tetE[e], phi, P.F);
+  load_F(q, load_E(m, e), phi, P.F);
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
@@ -588,8 +599,11 @@ __global__ void __launch_bounds__(256) k_ext_force(MeshView m, double t0, double
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= m.nV) return;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int k = m.incPtr[v]; k < m.incPtr[v + 1]; ++k) {
-    const double w = 0.25 * m.vol[m.inc[k] >> 2];
+  const IncList L = inc_list(m, v);
+  for (int k = 0; k < L.len; ++k) {
+    const int code = m.incE[L.first + 64ll * k];
+    if (code < 0) break;
+    const double w = 0.25 * m.vol[code >> 2];
     f0 += t0 * w;
     f1 += t1 * w;
     f2 += t2 * w;
@@ -603,7 +617,7 @@ __global__ void __launch_bounds__(256) k_dphidX(MeshView m, const double* __rest
                                                 double* __restrict__ vol) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.nT) return;
-  const int4 ev = m.tetE[e];
+  const int4 ev = load_E(m, e);
   const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
   double x[4][3];
 #pragma unroll
@@ -680,11 +694,11 @@ int max_lds_bytes() {
 
 // FP32 copies of the element geometry for the precision form (made on first use)
 int ensure_f32(bh_mesh* m, hipStream_t s) {
-  if (m->tetPhi32) return BH_OK;
-  BH_CHECK(hipMalloc(&m->tetPhi32, sizeof(float) * 12 * std::max(1, m->nT)));
+  if (m->dX32) return BH_OK;
+  BH_CHECK(hipMalloc(&m->dX32, sizeof(float) * 12 * std::max(1, m->nT)));
   BH_CHECK(hipMalloc(&m->vol32, sizeof(float) * std::max(1, m->nT)));
   if (m->nT) {
-    hipLaunchKernelGGL(k_to_f32, dim3((12ll * m->nT + 255) / 256), dim3(256), 0, s, 12ll * m->nT, m->tetPhi, m->tetPhi32);
+    hipLaunchKernelGGL(k_to_f32, dim3((12ll * m->nT + 255) / 256), dim3(256), 0, s, 12ll * m->nT, m->dX, m->dX32);
     hipLaunchKernelGGL(k_to_f32, dim3((m->nT + 255) / 256), dim3(256), 0, s, (int64_t)m->nT, m->vol, m->vol32);
     BH_CHECK(hipGetLastError());
   }
@@ -934,31 +948,40 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
   m->maxSeg16 = maxseg(16);
   m->nblocks = nblocks;
   m->nnz = 9 * nblocks;
-  // AoS tet records
-  std::vector<int4> tetE(nT);
-  std::vector<double> tetPhi((size_t)12 * nT);
-  for (int e = 0; e < nT; ++e) {
-    tetE[e] = make_int4(E[e], E[e + (int64_t)nT], E[e + 2 * (int64_t)nT], E[e + 3 * (int64_t)nT]);
-    for (int c = 0; c < 12; ++c) tetPhi[(size_t)12 * e + c] = dphidX[e + (int64_t)c * nT];
+  // interleaved incidence lists (groups of 64 vertices, slot-major)
+  const int nG = (nV + 63) / 64;
+  std::vector<int64_t> gBase(std::max(1, nG), 0);
+  std::vector<int32_t> gLen(std::max(1, nG), 0);
+  int64_t tot = 0;
+  for (int g = 0; g < nG; ++g) {
+    int L = 0;
+    for (int v = 64 * g; v < std::min(nV, 64 * g + 64); ++v) L = std::max(L, incPtr[v + 1] - incPtr[v]);
+    gBase[g] = tot;
+    gLen[g] = L;
+    tot += 64ll * L;
   }
-  BH_CHECK(hipMalloc(&m->tetE, sizeof(int4) * std::max(1, nT)));
-  BH_CHECK(hipMalloc(&m->tetPhi, sizeof(double) * 12 * std::max(1, nT)));
-  BH_CHECK(hipMalloc(&m->vol, sizeof(double) * std::max(1, nT)));
-  BH_CHECK(hipMalloc(&m->incPtr, sizeof(int32_t) * (nV + 1)));
-  BH_CHECK(hipMalloc(&m->inc, sizeof(int32_t) * std::max<size_t>(1, inc.size())));
-  BH_CHECK(hipMalloc(&m->incRank, sizeof(uint32_t) * std::max<size_t>(1, incRank.size())));
-  BH_CHECK(hipMalloc(&m->bptr, sizeof(int32_t) * (nV + 1)));
-  BH_CHECK(hipMalloc(&m->bcol, sizeof(int32_t) * std::max<int64_t>(1, nblocks)));
-  if (nT) {
-    BH_CHECK(hipMemcpy(m->tetE, tetE.data(), sizeof(int4) * nT, hipMemcpyHostToDevice));
-    BH_CHECK(hipMemcpy(m->tetPhi, tetPhi.data(), sizeof(double) * 12 * (size_t)nT, hipMemcpyHostToDevice));
-    BH_CHECK(hipMemcpy(m->vol, volume, sizeof(double) * nT, hipMemcpyHostToDevice));
-    BH_CHECK(hipMemcpy(m->inc, inc.data(), sizeof(int32_t) * inc.size(), hipMemcpyHostToDevice));
-    BH_CHECK(hipMemcpy(m->incRank, incRank.data(), sizeof(uint32_t) * incRank.size(), hipMemcpyHostToDevice));
+  std::vector<int32_t> incE(std::max<int64_t>(1, tot), -1);
+  std::vector<uint32_t> incR(std::max<int64_t>(1, tot), 0u);
+  for (int v = 0; v < nV; ++v) {
+    const int g = v >> 6, lane = v & 63;
+    for (int k = incPtr[v]; k < incPtr[v + 1]; ++k) {
+      const int64_t idx = gBase[g] + 64ll * (k - incPtr[v]) + lane;
+      incE[idx] = inc[k];
+      incR[idx] = incRank[k];
+    }
   }
-  BH_CHECK(hipMemcpy(m->incPtr, incPtr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice));
-  BH_CHECK(hipMemcpy(m->bptr, bptr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice));
-  if (nblocks) BH_CHECK(hipMemcpy(m->bcol, bcol.data(), sizeof(int32_t) * nblocks, hipMemcpyHostToDevice));
+  auto upload = [](auto** d, const auto* h, size_t n) -> int {
+    BH_CHECK(hipMalloc(d, sizeof(**d) * std::max<size_t>(1, n)));
+    if (n) BH_CHECK(hipMemcpy(*d, h, sizeof(**d) * n, hipMemcpyHostToDevice));
+    return BH_OK;
+  };
+  int st;
+  if ((st = upload(&m->E, E, 4 * (size_t)nT)) || (st = upload(&m->dX, dphidX, 12 * (size_t)nT)) ||
+      (st = upload(&m->vol, volume, (size_t)nT)) || (st = upload(&m->gBase, gBase.data(), gBase.size())) ||
+      (st = upload(&m->gLen, gLen.data(), gLen.size())) || (st = upload(&m->incE, incE.data(), incE.size())) ||
+      (st = upload(&m->incR, incR.data(), incR.size())) || (st = upload(&m->bptr, bptr.data(), bptr.size())) ||
+      (st = upload(&m->bcol, bcol.data(), bcol.size())))
+    return st;
   const int per = kEnergyBlock * kEnergyItems;
   m->nPartials = std::max(1, (nT + per - 1) / per);
   BH_CHECK(hipMalloc(&m->partials, sizeof(double) * m->nPartials));
@@ -975,9 +998,9 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
 
 void free_mesh(bh_mesh* m) {
   clear_profile(m);
-  void* ptrs[] = {m->tetE, m->tetPhi, m->vol, m->incPtr, m->inc, m->incRank, m->bptr, m->bcol, m->partials,
+  void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws, m->tetPhi32, m->vol32, m->psdDM32};
+                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws, m->dX32, m->vol32, m->psdDM32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -1246,6 +1269,17 @@ int bh_linear_tetmesh_const_ext_force_device(bh_mesh* m, const double* traction,
   hipLaunchKernelGGL(k_ext_force, dim3((m->nV + 255) / 256), dim3(256), 0, as_stream(s), view(m), traction[0],
                      traction[1], traction[2], d_f);
   BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int bh_mesh_psd_count(bh_mesh* m, int64_t* count) {
+  if (!m || !count) return BH_BAD_ARG;
+  *count = 0;
+  if (!m->psdCount) return BH_OK;
+  int32_t h = 0;
+  BH_CHECK(hipDeviceSynchronize());
+  BH_CHECK(hipMemcpy(&h, m->psdCount, sizeof(int32_t), hipMemcpyDeviceToHost));
+  *count = h;
   return BH_OK;
 }
 

@@ -603,58 +603,57 @@ __host__ __device__ __forceinline__ bool shifted_pd(T M[81], T tol) {
   return pd;
 }
 
-// dM = M' - M (packed upper, 45 values) with M' = M with its eigenvalues clamped at tol
-// (cyclic Jacobi; destroys A)
+// dM = M' - M (packed upper, 45 values) with M' = M with its eigenvalues clamped at tol.
+// Cyclic Jacobi on the packed upper triangle (symmetric rotation update, eigenvectors in W); stops
+// when the off-diagonal mass is below (rounding-level) 1e-28 (FP64) / 1e-12 (FP32) of the total.
+// Destroys A (col-major 9x9; only the upper triangle is read).
 template <typename T>
 __host__ __device__ inline void psd_delta(T A[81], T tol, T* out) {
-  T W[81];
+  T a[45];  // packed upper triangle, a[sym9(i, j)]
 #pragma unroll
   for (int r = 0; r < 9; ++r)
 #pragma unroll
-    for (int c = r + 1; c < 9; ++c) {
-      const T s = T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
-      A[r + 9 * c] = s;
-      A[c + 9 * r] = s;
-    }
+    for (int c = r; c < 9; ++c) a[sym9(r, c)] = (r == c) ? A[r + 9 * c] : T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
+  T W[81];
 #pragma unroll
   for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? T(1) : T(0);
-  T scale = T(0);
+  T total = T(0);
 #pragma unroll
-  for (int k = 0; k < 81; ++k) scale += A[k] * A[k];
-  const T stop = sizeof(T) == 8 ? T(1e-32) : T(1e-14);
-  for (int sweep = 0; sweep < 20; ++sweep) {
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int c = r; c < 9; ++c) total += (r == c ? T(1) : T(2)) * a[sym9(r, c)] * a[sym9(r, c)];
+  const T stop = sizeof(T) == 8 ? T(1e-28) : T(1e-12);
+  for (int sweep = 0; sweep < 12; ++sweep) {
     T off = T(0);
 #pragma unroll
-    for (int pp = 0; pp < 9; ++pp)
+    for (int r = 0; r < 9; ++r)
 #pragma unroll
-      for (int rr = pp + 1; rr < 9; ++rr) off += A[pp + 9 * rr] * A[pp + 9 * rr];
-    if (off <= stop * scale) break;
+      for (int c = r + 1; c < 9; ++c) off += a[sym9(r, c)] * a[sym9(r, c)];
+    if (!(off > stop * total)) break;
 #pragma unroll
     for (int pp = 0; pp < 8; ++pp)
 #pragma unroll
-      for (int rr = pp + 1; rr < 9; ++rr) {
-        const T apq = A[pp + 9 * rr];
+      for (int qq = pp + 1; qq < 9; ++qq) {
+        const T apq = a[sym9(pp, qq)];
         if (apq != T(0)) {
-          const T theta = (A[rr + 9 * rr] - A[pp + 9 * pp]) / (T(2) * apq);
+          const T theta = (a[sym9(qq, qq)] - a[sym9(pp, pp)]) / (T(2) * apq);
           const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
           const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
+          a[sym9(pp, pp)] -= t * apq;
+          a[sym9(qq, qq)] += t * apq;
+          a[sym9(pp, qq)] = T(0);
 #pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T akp = A[k + 9 * pp], akq = A[k + 9 * rr];
-            A[k + 9 * pp] = c * akp - s * akq;
-            A[k + 9 * rr] = s * akp + c * akq;
+          for (int r = 0; r < 9; ++r) {
+            if (r == pp || r == qq) continue;
+            const T arp = a[sym9(r, pp)], arq = a[sym9(r, qq)];
+            a[sym9(r, pp)] = c * arp - s * arq;
+            a[sym9(r, qq)] = s * arp + c * arq;
           }
 #pragma unroll
           for (int k = 0; k < 9; ++k) {
-            const T apk = A[pp + 9 * k], aqk = A[rr + 9 * k];
-            A[pp + 9 * k] = c * apk - s * aqk;
-            A[rr + 9 * k] = s * apk + c * aqk;
-          }
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * rr];
+            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * qq];
             W[k + 9 * pp] = c * wkp - s * wkq;
-            W[k + 9 * rr] = s * wkp + c * wkq;
+            W[k + 9 * qq] = s * wkp + c * wkq;
           }
         }
       }
@@ -662,17 +661,17 @@ __host__ __device__ inline void psd_delta(T A[81], T tol, T* out) {
   T lift[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const T lam = A[k + 9 * k];
+    const T lam = a[sym9(k, k)];
     lift[k] = (lam < tol) ? (tol - lam) : T(0);
   }
 #pragma unroll
   for (int r = 0; r < 9; ++r)
 #pragma unroll
     for (int c = r; c < 9; ++c) {
-      T s = T(0);
+      T sacc = T(0);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) s += W[r + 9 * k] * lift[k] * W[c + 9 * k];
-      out[r * 9 - (r * (r - 1)) / 2 + (c - r)] = s;
+      for (int k = 0; k < 9; ++k) sacc += W[r + 9 * k] * lift[k] * W[c + 9 * k];
+      out[sym9(r, c)] = sacc;
     }
 }
 

@@ -5,7 +5,8 @@ Box-Muller), so the same arrays can be regenerated bit-for-bit on any machine.
 
 Beam convention: the long axis is x; vertex (ix, iy, iz) has index (ix*ny + iy)*nz + iz; vertices
 with x == 0 are clamped; gravity is (0, 0, -9.8).  Each hex cell is split into 6 tets along its
-(0,0,0)-(1,1,1) diagonal (Kuhn split, conforming), oriented so that the signed volume
+(0,0,0)-(1,1,1) diagonal (Kuhn split, conforming), numbered type-major (all tets of the first
+type, cell by cell, then the second type, ...), oriented so that the signed volume
 vol = -(a-d).((b-d)x(c-d))/6 (igl::volume, as the reference uses in tests/test_assemble.cpp:27)
 is positive.  Arrays follow the reference layouts: V nV x 3, E nT x 4 int32.
 """
@@ -109,10 +110,12 @@ def beam_grid(nx, ny, nz, h=H_DEFAULT):
         return ((cx + dx) * ny + (cy + dy)) * nz + (cz + dz)
 
     tets = _kuhn_tets()
-    E = np.empty((cx.size, 6, 4), dtype=np.int32)
+    # type-major numbering: tet (type t, cell c) has index t * ncells + c, so the incident tets of
+    # consecutive vertices (consecutive cells) are consecutive in every per-tet array
+    E = np.empty((6, cx.size, 4), dtype=np.int32)
     for t, path in enumerate(tets):
         for k, (dx, dy, dz) in enumerate(path):
-            E[:, t, k] = vid(dx, dy, dz)
+            E[t, :, k] = vid(dx, dy, dz)
     return V, E.reshape(-1, 4)
 
 

@@ -129,6 +129,10 @@ def main():
     lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
     lib.bh_mesh_set_profiling(mesh.handle, 0)
     kt = {name: (ms[i] / cnt[i] if cnt[i] else None) for i, name in enumerate(_lib.KERNEL_CLASSES)}
+    import ctypes
+
+    flagged = ctypes.c_int64()
+    lib.bh_mesh_psd_count(mesh.handle, ctypes.byref(flagged))
 
     # secondary kernels, each timed on its own (not part of value)
     def timed(fn, reps=10):
@@ -212,6 +216,7 @@ def main():
             "q_ms": ms_q,
             "q_hbm_frac": B["q"] * nT / (ms_q * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "variants_ms": variants,
+            "psd_clamped_tets": int(flagged.value),
             "wall_s_timed": wall,
             "setup_s": setup_s,
         },

@@ -249,6 +249,10 @@ int bh_mesh_set_profiling(bh_mesh* mesh, int on);
  * number of launches since profiling was switched on.  ms, launches: BH_KT_COUNT entries. */
 int bh_mesh_kernel_times(bh_mesh* mesh, double* ms, int64_t* launches);
 
+/* number of tets whose element Hessian needed eigenvalue clamping in the last PSD call
+ * (synchronises the device) */
+int bh_mesh_psd_count(bh_mesh* mesh, int64_t* count);
+
 /* Device sparse helpers on the mesh pattern (values in bh_mesh_pattern order). */
 int bh_spmv_device(bh_mesh* mesh, const double* d_vals, const double* d_x, double* d_y, bh_stream_t stream);
 
