@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1 or 2 lanes per vertex "
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="timed steps only (for counter collection)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not code); "
@@ -133,6 +136,17 @@ def main():
 
     flagged = ctypes.c_int64()
     lib.bh_mesh_psd_count(mesh.handle, ctypes.byref(flagged))
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.quick:
+        print(json.dumps({"ms_per_step": ms_step, "kernels_ms": {k: v for k, v in kt.items() if v is not None}}))
+        return
 
     # secondary kernels, each timed on its own (not part of value)
     def timed(fn, reps=10):
