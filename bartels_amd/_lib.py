@@ -32,6 +32,11 @@ class BhElementOpts(ctypes.Structure):
     _fields_ = [("stretch_reference_layout", c_int)]
 
 
+class BhTuning(ctypes.Structure):
+    """kernel-schedule knobs of a mesh (gather lanes, XCD grouping); same per-slot fold order"""
+    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int)]
+
+
 class BhNewtonOpts(ctypes.Structure):
     _fields_ = [
         ("model", c_int),
@@ -86,6 +91,7 @@ SIGNATURES = {
     "bh_mesh_block_pattern_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_int64)]),
     "bh_mesh_set_element_opts": (c_int, [_VP, P(BhElementOpts)]),
     "bh_mesh_set_gather_lanes": (c_int, [_VP, c_int]),
+    "bh_mesh_set_tuning": (c_int, [_VP, P(BhTuning)]),
     "bh_linear_tetmesh_q": (c_int, [_VP, c_int, _VP, _VP, P(c_double)]),
     "bh_linear_tetmesh_dq": (c_int, [_VP, c_int, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq2": (c_int, [_VP, c_int, _VP, _VP, P(BhPsd), _VP]),
@@ -115,7 +121,7 @@ SIGNATURES = {
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),
 }
 
-KERNEL_CLASSES = ["energy", "gradient", "hessian_gather", "psd_classify", "psd_eigen"]
+KERNEL_CLASSES = ["energy", "gradient", "hessian_gather", "psd"]
 
 _lib = None
 

@@ -67,7 +67,8 @@ struct bh_mesh {
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
-  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
+  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1, 2 or 4)
+  int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
   // scratch
   double* partials = nullptr;
   int nPartials = 0;
@@ -80,7 +81,6 @@ struct bh_mesh {
   double* stage_vals = nullptr;
   // PSD
   uint8_t* psdFlag = nullptr;
-  int32_t* psdList = nullptr;
   int32_t* psdCount = nullptr;
   double* psdDM = nullptr;  // nT x 45
   float* psdDM32 = nullptr;
@@ -331,216 +331,239 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- Hessian (vertex gather into LDS) -----------------------------------------------------------
-template <int MODEL, typename T>
-__device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __restrict__ q,
-                                           const T* __restrict__ params, int e, int a, bool stretch_ref,
-                                           T blk[4][9]) {
-  if (MODEL == M_MASS) {
-    // consistent mass (src/linear_tet_mass_matrix.cpp:8-44); params[e] = density, q unused
-    const T mass = params[e] * load_vol<T>(m, e);
-#pragma unroll
-    for (int mm = 0; mm < 4; ++mm) {
-      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) blk[mm][i] = (i % 4 == 0) ? c : T(0);
-    }
+// One workgroup owns VPW consecutive vertices, i.e. one contiguous segment of the CSR values, and
+// accumulates it in LDS.  LPV lanes serve each vertex: lane h of the vertex owns corners
+// h, h + LPV, ... of every incident tet, i.e. NB = 4 / LPV blocks (a, m) of the vertex's block row.
+// The corners of a tet are distinct vertices, so the lanes of a vertex never update the same LDS
+// slot, and every slot receives its contributions in ascending element order.  LPV = 1 computes
+// the per-tet setup once per vertex; LPV = 2, 4 repeat it per lane for more lanes in flight.
+//
+// The walk over the incidence list is software-pipelined three deep: while tet k is evaluated,
+// the element data of tet k+1 and the list entry of tet k+2 are already in flight, so each step
+// waits on one memory latency (the vertex positions and stored PSD columns of tet k) instead of
+// the three dependent ones (list entry -> element data -> positions).
+
+// element data of one incident tet (everything that depends only on the element index)
+template <typename T>
+struct TetIn {
+  int4 ev;
+  T phi[4][3];
+  T vol, p0, p1;
+  int mid;
+  int fmt;  // stored PSD correction format (0 when none or PSD off)
+};
+
+template <int MODEL, bool PSD, typename T>
+__device__ __forceinline__ void load_tet(const MeshView& m, const T* __restrict__ params,
+                                         const uint8_t* __restrict__ mids, const PsdView<T>& psd, int e,
+                                         TetIn<T>& t) {
+  t.vol = load_vol<T>(m, e);
+  t.p0 = params[e];
+  t.mid = model_at<MODEL>(mids, e);
+  if (MODEL == M_MASS) {  // consistent mass: params = density (one value per tet), no geometry needed
+    t.p1 = T(0);
+    t.fmt = 0;
     return;
   }
-  T phi[4][3];
-  Pre<T> P;
-  load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, P.F);
-  P.vol = load_vol<T>(m, e);
-  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
-  T pa[3];
-  select_row(phi, a, pa);
-  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
+  t.ev = load_E(m, e);
+  load_phi(m, e, t.phi);
+  t.p1 = params[(int64_t)m.nT + e];
+  t.fmt = PSD ? (int)psd.flag[e] : 0;
 }
 
-// two of the four blocks of the vertex's block row: corners h and h + 2 (h = 0 or 1)
-template <int MODEL, typename T>
-__device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __restrict__ q,
-                                            const T* __restrict__ params, int e, int a, int h, bool stretch_ref,
-                                            T blk[2][9]) {
-  if (MODEL == M_MASS) {
-    const T mass = params[e] * load_vol<T>(m, e);
+__device__ __forceinline__ void load_x(const double* __restrict__ q, int4 ev, double x[4][3]) {
+  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int mm = h + 2 * j;
+  for (int mm = 0; mm < 4; ++mm) {
+    const double* qp = q + 3ll * vid[mm];
+    x[mm][0] = qp[0];
+    x[mm][1] = qp[1];
+    x[mm][2] = qp[2];
+  }
+}
+
+// F from loaded positions (same arithmetic as load_F)
+template <typename T>
+__device__ __forceinline__ void make_F(const double x[4][3], const T phi[4][3], T* F) {
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j] + x[3][i] * phi[3][j];
+  } else {
+    T ed[3][3];
+#pragma unroll
+    for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(x[mm + 1][i] - x[0][i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+  }
+}
+
+// blocks (a, h + LPV j), j < NB, of one tet for one element model
+template <int MODEL, int NB, int LPV, typename T>
+__device__ __forceinline__ void row_blocks(const TetIn<T>& t, const double x[4][3], int a, int h, bool stretchRef,
+                                           T blk[NB][9]) {
+  if (MODEL == M_MASS) {
+    const T mass = t.p0 * t.vol;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int mm = h + LPV * j;
       const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
 #pragma unroll
       for (int i = 0; i < 9; ++i) blk[j][i] = (i % 4 == 0) ? c : T(0);
     }
     return;
   }
-  T phi[4][3];
   Pre<T> P;
-  load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, P.F);
-  P.vol = load_vol<T>(m, e);
-  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  make_F(x, t.phi, P.F);
+  P.vol = t.vol;
   T pa[3];
-  select_row(phi, a, pa);
-  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  select_row(t.phi, a, pa);
+  pre_hessian<MODEL>(P, t.p0, t.p1, stretchRef);
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < NB; ++j) {
     T pm[3];
-    select_row(phi, h + 2 * j, pm);
+    select_row(t.phi, h + LPV * j, pm);
     hess_block<MODEL>(P, pa, pm, blk[j]);
   }
 }
 
-template <int MODEL, bool PSD, int VPW, typename T>
-__global__ void __launch_bounds__(kGatherThreads)
-    k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-              const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, T* __restrict__ vals) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* acc = reinterpret_cast<T*>(smem);
-  const int v0 = blockIdx.x * VPW;
-  const int vend = min(v0 + VPW, m.nV);
-  const int b0 = m.bptr[v0];
-  const int64_t seg0 = 9ll * b0;
-  const int seglen = 9 * (m.bptr[vend] - b0);
-  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
-  __syncthreads();
-  const int v = v0 + (int)threadIdx.x;
-  if ((int)threadIdx.x < VPW && v < m.nV) {
-    const int nb = m.bptr[v + 1] - m.bptr[v];
-    T* row = acc + 9 * (m.bptr[v] - b0);
-    const IncList L = inc_list(m, v);
-    for (int k = 0; k < L.len; ++k) {
-      const int64_t idx = L.first + 64ll * k;
-      const int code = m.incE[idx];
-      if (code < 0) break;
-      const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incR[idx];
-      T blk[4][9];
-      switch (model_at<MODEL>(mids, e)) {
-        case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
-        case M_STVK: tet_blocks<M_STVK>(m, q, params, e, a, stretchRef, blk); break;
-        case M_COROT: tet_blocks<M_COROT>(m, q, params, e, a, stretchRef, blk); break;
-        case M_ARAP: tet_blocks<M_ARAP>(m, q, params, e, a, stretchRef, blk); break;
-        case M_MASS: tet_blocks<M_MASS>(m, q, params, e, a, stretchRef, blk); break;
-        default: tet_blocks<M_NH_LEGACY>(m, q, params, e, a, stretchRef, blk); break;
-      }
-      if (PSD) {
+// blk_j += (a, mlist[j]) block of the stored clamp correction: k = fmt <= 3 columns already loaded
+// into w (27 values), or the packed 9x9 at Wp (fmt 4)
+template <typename T, int NB>
+__device__ __forceinline__ void add_psd_correction(const T phi[4][3], const T w[27], const T* __restrict__ Wp,
+                                                   int fmt, int a, const int* mlist, T blk[NB][9]) {
+  T Q[4][3], R[3][3];
+  phi_qr(phi, Q, R);
+  if (fmt <= 3) {
+    psd_lowrank_blocks<T, NB>(Q, a, mlist, fmt, w, blk);
+  } else {
+    T Tm[3][9];
+    psd_row_factor(Q, a, Wp, Tm);
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) {
-          blk[mm][0] += psd.tolPos;
-          blk[mm][4] += psd.tolPos;
-          blk[mm][8] += psd.tolPos;
-        }
-        if (psd.flag[e]) {
-          // + Q dM Q^T restricted to block row a
-          T phi[4][3], Q[4][3], R[3][3];
-          load_phi(m, e, phi);
-          phi_qr(phi, Q, R);
-          T Tm[3][9];
-          psd_row_factor(Q, a, psd.dM + 45ll * e, Tm);
-#pragma unroll
-          for (int mm = 0; mm < 4; ++mm) psd_add_block(Q[mm], Tm, blk[mm]);
-        }
-      }
-#pragma unroll
-      for (int mm = 0; mm < 4; ++mm) {
-        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[mm][3 * l + kk];
-      }
+    for (int j = 0; j < NB; ++j) {
+      T qm[3];
+      select_row(Q, mlist[j], qm);
+      psd_add_block(qm, Tm, blk[j]);
     }
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
 }
 
-// Same gather with two lanes per vertex: lane h of the pair owns the blocks of corners h and h+2 of
-// every incident tet.  The four corners of a tet are four distinct vertices, so the pair never
-// updates the same LDS slot, and every slot still receives its contributions in ascending element
-// order.  Half the LDS per lane of the one-lane kernel (more waves per CU) for a duplicated
-// per-tet setup.
-template <int MODEL, bool PSD, int VPW, typename T>
-__global__ void __launch_bounds__(kGatherThreads)
-    k_hessian_pair(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-                   const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, T* __restrict__ vals) {
+// Workgroup -> vertex group.  Workgroups are dealt round-robin to the 8 XCDs (each with its own
+// L2); with xcd set, XCD x takes the contiguous run of groups [x per, (x+1) per), so the groups
+// that share tets (neighbouring ids) meet in the same L2.  The grid is then 8 per workgroups and
+// the surplus ones return at once.
+__device__ __forceinline__ int gather_group(int xcd, int ngroups) {
+  if (!xcd) return (int)blockIdx.x;
+  const int per = (ngroups + 7) >> 3;
+  return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+}
+
+constexpr int gather_threads(int vpw, int lpv) { return vpw * lpv < 64 ? 64 : vpw * lpv; }
+
+template <int MODEL, bool PSD, int VPW, int LPV, typename T>
+__global__ void __launch_bounds__(gather_threads(VPW, LPV))
+    k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+              const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, int xcd, T* __restrict__ vals) {
+  constexpr int NB = 4 / LPV;
+  constexpr int NT = gather_threads(VPW, LPV);
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
-  const int v0 = blockIdx.x * VPW;
+  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
+  if (grp * VPW >= m.nV) return;
+  const int v0 = grp * VPW;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
   const int seglen = 9 * (m.bptr[vend] - b0);
-  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  for (int i = threadIdx.x; i < seglen; i += NT) acc[i] = T(0);
   __syncthreads();
-  const int vl = (int)threadIdx.x >> 1, h = (int)threadIdx.x & 1;
+  const int vl = (int)threadIdx.x / LPV, h = (int)threadIdx.x % LPV;
   const int v = v0 + vl;
-  if (vl < VPW && v < m.nV) {
+  if (vl < VPW && v < vend) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
     const IncList L = inc_list(m, v);
-    for (int k = 0; k < L.len; ++k) {
-      const int64_t idx = L.first + 64ll * k;
-      const int code = m.incE[idx];
-      if (code < 0) break;
-      const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incR[idx];
-      T blk[2][9];
-      switch (model_at<MODEL>(mids, e)) {
-        case M_NH: tet_blocks2<M_NH>(m, q, params, e, a, h, stretchRef, blk); break;
-        case M_STVK: tet_blocks2<M_STVK>(m, q, params, e, a, h, stretchRef, blk); break;
-        case M_COROT: tet_blocks2<M_COROT>(m, q, params, e, a, h, stretchRef, blk); break;
-        case M_ARAP: tet_blocks2<M_ARAP>(m, q, params, e, a, h, stretchRef, blk); break;
-        case M_MASS: tet_blocks2<M_MASS>(m, q, params, e, a, h, stretchRef, blk); break;
-        default: tet_blocks2<M_NH_LEGACY>(m, q, params, e, a, h, stretchRef, blk); break;
+    int mlist[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) mlist[j] = h + LPV * j;
+    // pipeline: A = tet being evaluated, B = element data in flight, C = list entry in flight
+    int codeA = (L.len > 0) ? m.incE[L.first] : -1;
+    uint32_t rkA = (L.len > 0) ? m.incR[L.first] : 0u;
+    int codeB = (L.len > 1) ? m.incE[L.first + 64] : -1;
+    uint32_t rkB = (L.len > 1) ? m.incR[L.first + 64] : 0u;
+    TetIn<T> tA;
+    if (codeA >= 0) load_tet<MODEL, PSD>(m, params, mids, psd, codeA >> 2, tA);
+    for (int k = 0; codeA >= 0; ++k) {
+      const int eA = codeA >> 2, a = codeA & 3;
+      double x[4][3];
+      if (MODEL != M_MASS) load_x(q, tA.ev, x);
+      T w[PSD ? 27 : 1];
+      if (PSD) {
+        // stored columns of tet A (the packed form is read inside the correction)
+        const T* W = psd.dM + 45ll * eA;
+#pragma unroll
+        for (int i = 0; i < 27; ++i) w[i] = (i < 9 * tA.fmt && tA.fmt <= 3) ? W[i] : T(0);
+      }
+      TetIn<T> tB;
+      if (codeB >= 0) load_tet<MODEL, PSD>(m, params, mids, psd, codeB >> 2, tB);
+      const bool more = k + 2 < L.len;
+      const int codeC = more ? m.incE[L.first + 64ll * (k + 2)] : -1;
+      const uint32_t rkC = more ? m.incR[L.first + 64ll * (k + 2)] : 0u;
+
+      T blk[NB][9];
+      switch (tA.mid) {
+        case M_NH: row_blocks<M_NH, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+        case M_STVK: row_blocks<M_STVK, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+        case M_COROT: row_blocks<M_COROT, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+        case M_ARAP: row_blocks<M_ARAP, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+        case M_MASS: row_blocks<M_MASS, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+        default: row_blocks<M_NH_LEGACY, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
       }
       if (PSD) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < NB; ++j) {
           blk[j][0] += psd.tolPos;
           blk[j][4] += psd.tolPos;
           blk[j][8] += psd.tolPos;
         }
-        if (psd.flag[e]) {
-          T phi[4][3], Q[4][3], R[3][3];
-          load_phi(m, e, phi);
-          phi_qr(phi, Q, R);
-          T Tm[3][9];
-          psd_row_factor(Q, a, psd.dM + 45ll * e, Tm);
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            T qm[3];
-            select_row(Q, h + 2 * j, qm);
-            psd_add_block(qm, Tm, blk[j]);
-          }
-        }
+        if (tA.fmt) add_psd_correction<T, NB>(tA.phi, w, psd.dM + 45ll * eA, tA.fmt, a, mlist, blk);
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int mm = h + 2 * j;
-        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+      for (int j = 0; j < NB; ++j) {
+        T* dst = row + 3 * ((rkA >> (8 * mlist[j])) & 0xffu);
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
           for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[j][3 * l + kk];
       }
+      codeA = codeB;
+      rkA = rkB;
+      tA = tB;
+      codeB = codeC;
+      rkB = rkC;
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+  for (int i = threadIdx.x; i < seglen; i += NT) vals[seg0 + i] = acc[i];
 }
 
 // ---- PSD projection (simple_psd_fix, src/simple_psd_fix.cpp:5-21) ------------------------------
 // K = vol B^T H B has the three translations in its null space; with Phi = Q R (thin QR),
 // K = (Q x I3) M (Q x I3)^T,  M = (R x I3) vol H (R x I3)^T (9x9), and
-// psd(K) = K + (Q x I3)(M' - M)(Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  M' = M with eigenvalues
-// clamped at tol.  Pass 1 flags tets whose M - tol I is not positive definite (Cholesky);
-// pass 2 eigen-decomposes M (cyclic Jacobi) for the flagged tets only and stores dM = M' - M.
+// psd(K) = K + (Q x I3) dM (Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  dM = M' - M, M' = M with its
+// eigenvalues clamped at tol.  A tet needs dM only when M - tol I is not positive definite
+// (Cholesky test); dM is then computed by the tridiagonal-QL eigensolver and stored as k <= 3
+// columns (dM = W W^T) or packed (k > 3); flag[e] records the storage format (0 = none).
+
 template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __restrict__ q,
-                                          const T* __restrict__ params, int e, bool stretchRef, T M[81]) {
+                                          const T* __restrict__ params, int e, bool stretchRef, T M[45]) {
   T phi[4][3], Q[4][3], R[3][3];
   Pre<T> P;
   load_phi(m, e, phi);
@@ -549,12 +572,12 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
   pre_hessian<MODEL>(P, p0, p1, stretchRef);
-  reduced_matrix<MODEL>(P, R, M);
+  reduced_packed<MODEL>(P, R, M);
 }
 
 template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const T* params,
-                                                   const uint8_t* mids, int e, bool stretchRef, T M[81]) {
+                                                   const uint8_t* mids, int e, bool stretchRef, T M[45]) {
   switch (model_at<MODEL>(mids, e)) {
     case M_NH: reduced_M<M_NH>(m, q, params, e, stretchRef, M); break;
     case M_STVK: reduced_M<M_STVK>(m, q, params, e, stretchRef, M); break;
@@ -564,34 +587,27 @@ __device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const doub
   }
 }
 
-template <int MODEL, typename T>
-__global__ void __launch_bounds__(256)
-    k_psd_classify(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-                   const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
-                   int32_t* __restrict__ list, int32_t* __restrict__ count) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m.nT) return;
-  T M[81];
-  reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
-  const bool pd = shifted_pd(M, tol);
-  flag[e] = pd ? 0 : 1;
-  if (!pd) {
-    const int idx = atomicAdd(count, 1);
-    list[idx] = e;
-  }
+// adds the wave's count of flagged lanes with one atomic (every lane of the wave must call it)
+__device__ __forceinline__ void count_flagged(bool flagged, int32_t* count) {
+  const unsigned long long bal = __ballot(flagged);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (int32_t)__popcll(bal));
 }
 
 template <int MODEL, typename T>
 __global__ void __launch_bounds__(64)
-    k_psd_eigen(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-                const uint8_t* __restrict__ mids, T tol, int stretchRef, const int32_t* __restrict__ list,
-                const int32_t* __restrict__ count, T* __restrict__ dM) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *count) return;
-  const int e = list[i];
-  T A[81];
-  reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, A);
-  psd_delta(A, tol, dM + 45ll * e);
+    k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
+                int32_t* __restrict__ count, T* __restrict__ dM) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  bool need = false;
+  T M[45];
+  if (e < m.nT) {
+    reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+    need = !shifted_pd(M, tol);
+  }
+  count_flagged(need, count);
+  if (e >= m.nT) return;
+  flag[e] = need ? (uint8_t)psd_format(psd_correction(M, tol, dM + 45ll * e)) : (uint8_t)0;
 }
 
 // ---- mass / external force / setup / spmv -----------------------------------------------------------
@@ -718,16 +734,18 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const
                        T* vals, hipStream_t s) {
   const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
   const size_t lds = sizeof(T) * (size_t)std::max(seg, 1);
-  auto kern = (LPV == 1) ? k_hessian<MODEL, PSD, VPW, T> : k_hessian_pair<MODEL, PSD, VPW, T>;
+  auto kern = k_hessian<MODEL, PSD, VPW, LPV, T>;
+  const int threads = gather_threads(VPW, LPV);
   static std::once_flag once;
   std::call_once(once, [&] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               max_lds_bytes());
   });
-  const int grid = (m->nV + VPW - 1) / VPW;
+  const int groups = (m->nV + VPW - 1) / VPW;
+  const int grid = m->xcdGroups ? 8 * ((groups + 7) / 8) : groups;
   ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kGatherThreads), lds, s, view(m), q, params, mids, psd,
-                     m->opts.stretch_reference_layout, vals);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, view(m), q, params, mids, psd,
+                     m->opts.stretch_reference_layout, m->xcdGroups, vals);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -735,45 +753,40 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const
 template <int MODEL, bool PSD, typename T>
 int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd, T* vals,
                    hipStream_t s) {
-  const int lim = max_lds_bytes();
-  if (m->gatherLanes == 2) {
-    if (sizeof(T) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
-      return launch_hessian_vpw<MODEL, PSD, 32, 2>(m, q, params, mids, psd, vals, s);
-    if (sizeof(T) * (size_t)m->maxSeg16 <= (size_t)lim)
-      return launch_hessian_vpw<MODEL, PSD, 16, 2>(m, q, params, mids, psd, vals, s);
+  const size_t lim = (size_t)max_lds_bytes();
+  const size_t s64 = sizeof(T) * (size_t)m->maxSeg64, s32 = sizeof(T) * (size_t)m->maxSeg32,
+               s16 = sizeof(T) * (size_t)m->maxSeg16;
+  if (m->gatherLanes == 4) {
+    if (s64 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 64, 4>(m, q, params, mids, psd, vals, s);
+    if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 4>(m, q, params, mids, psd, vals, s);
   }
-  if (sizeof(T) * (size_t)m->maxSeg64 <= (size_t)lim / 2)
-    return launch_hessian_vpw<MODEL, PSD, 64, 1>(m, q, params, mids, psd, vals, s);
-  if (sizeof(T) * (size_t)m->maxSeg32 <= (size_t)lim / 2)
-    return launch_hessian_vpw<MODEL, PSD, 32, 1>(m, q, params, mids, psd, vals, s);
-  if (sizeof(T) * (size_t)m->maxSeg16 <= (size_t)lim)
-    return launch_hessian_vpw<MODEL, PSD, 16, 1>(m, q, params, mids, psd, vals, s);
+  if (m->gatherLanes == 2) {
+    if (s32 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 32, 2>(m, q, params, mids, psd, vals, s);
+    if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 2>(m, q, params, mids, psd, vals, s);
+  }
+  if (s64 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 64, 1>(m, q, params, mids, psd, vals, s);
+  if (s32 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 32, 1>(m, q, params, mids, psd, vals, s);
+  if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 1>(m, q, params, mids, psd, vals, s);
   return BH_UNSUPPORTED;  // vertex valence too high for the LDS gather
 }
 
+// One pass over all tets: M, the Cholesky test, and for the tets that fail it the eigen-solve and
+// the stored correction.  A single schedule on purpose: the same element math compiled into two
+// kernels may contract multiply-adds differently, and a per-call choice between two schedules
+// would make repeated calls differ in the last bits.
 template <int MODEL, typename T>
 int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T tol, T** dM, hipStream_t s) {
   if (!m->psdFlag) {
     BH_CHECK(hipMalloc(&m->psdFlag, std::max<size_t>(1, m->nT)));
-    BH_CHECK(hipMalloc(&m->psdList, sizeof(int32_t) * std::max<size_t>(1, m->nT)));
     BH_CHECK(hipMalloc(&m->psdCount, sizeof(int32_t)));
   }
   void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->psdDM) : reinterpret_cast<void**>(&m->psdDM32);
   if (!*buf) BH_CHECK(hipMalloc(buf, sizeof(T) * 45 * std::max<size_t>(1, m->nT)));
   *dM = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
-  const int grid = (m->nT + 255) / 256;
-  {
-    ProfScope ps(m, BH_KT_PSD_CLASSIFY, s);
-    hipLaunchKernelGGL((k_psd_classify<MODEL, T>), dim3(grid), dim3(256), 0, s, view(m), q, params, mids, tol,
-                       m->opts.stretch_reference_layout, m->psdFlag, m->psdList, m->psdCount);
-  }
-  BH_CHECK(hipGetLastError());
-  // the eigen pass is sized for the worst case and exits early past *count
-  const int grid2 = (m->nT + 63) / 64;
-  ProfScope ps(m, BH_KT_PSD_EIGEN, s);
-  hipLaunchKernelGGL((k_psd_eigen<MODEL, T>), dim3(grid2), dim3(64), 0, s, view(m), q, params, mids, tol,
-                     m->opts.stretch_reference_layout, m->psdList, m->psdCount, *dM);
+  ProfScope ps(m, BH_KT_PSD, s);
+  hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, *dM);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -1000,7 +1013,7 @@ void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdList, m->psdCount, m->psdDM, m->ws, m->dX32, m->vol32, m->psdDM32};
+                  m->psdFlag, m->psdCount, m->psdDM, m->ws, m->dX32, m->vol32, m->psdDM32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -1284,8 +1297,18 @@ int bh_mesh_psd_count(bh_mesh* m, int64_t* count) {
 }
 
 int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
-  if (!m || (lanes != 1 && lanes != 2)) return BH_BAD_ARG;
+  if (!m || (lanes != 1 && lanes != 2 && lanes != 4)) return BH_BAD_ARG;
   m->gatherLanes = lanes;
+  return BH_OK;
+}
+
+int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
+  if (!m || !t) return BH_BAD_ARG;
+  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2 && t->gather_lanes != 4)
+    return BH_BAD_ARG;
+  if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
+  if (t->gather_lanes) m->gatherLanes = t->gather_lanes;
+  if (t->xcd_groups >= 0) m->xcdGroups = t->xcd_groups;
   return BH_OK;
 }
 

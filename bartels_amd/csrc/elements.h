@@ -564,115 +564,245 @@ __host__ __device__ __forceinline__ int sym9(int i, int j) {
   return a * 9 - (a * (a - 1)) / 2 + (b - a);
 }
 
-// M (9x9, col-major, index 3p + l) from the Hessian precomputation P and R
+// M (symmetric 9x9, index 3p + l) from the Hessian precomputation P and R, as its packed upper
+// triangle a[sym9(i, j)]: the six blocks (p, r), p <= r (the lower blocks are their transposes up
+// to rounding)
 template <int MODEL, typename T>
-__host__ __device__ __forceinline__ void reduced_matrix(const Pre<T>& P, const T R[3][3], T M[81]) {
+__host__ __device__ __forceinline__ void reduced_packed(const Pre<T>& P, const T R[3][3], T a[45]) {
 #pragma unroll
   for (int pp = 0; pp < 3; ++pp)
 #pragma unroll
-    for (int rr = 0; rr < 3; ++rr) {
+    for (int rr = pp; rr < 3; ++rr) {
       T b[9];
       hess_block<MODEL>(P, R[pp], R[rr], b);
 #pragma unroll
       for (int l = 0; l < 3; ++l)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) M[(3 * pp + l) + 9 * (3 * rr + kk)] = b[3 * l + kk];
+        for (int kk = 0; kk < 3; ++kk)
+          if (rr > pp || kk >= l) a[sym9(3 * pp + l, 3 * rr + kk)] = b[3 * l + kk];
     }
 }
 
-// true when M - tol I (symmetrised) is positive definite: Cholesky (destroys M)
+// true when M - tol I is positive definite (Cholesky of a packed copy)
 template <typename T>
-__host__ __device__ __forceinline__ bool shifted_pd(T M[81], T tol) {
+__host__ __device__ __forceinline__ bool shifted_pd(const T a[45], T tol) {
+  T L[45];
   bool pd = true;
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
-    T d = M[j + 9 * j] - tol;
+    T d = a[sym9(j, j)] - tol;
 #pragma unroll
-    for (int k = 0; k < j; ++k) d -= M[j + 9 * k] * M[j + 9 * k];
+    for (int k = 0; k < j; ++k) d -= L[sym9(k, j)] * L[sym9(k, j)];
     pd = pd && (d > T(0));
     const T ljj = sqrt(fmax(d, sizeof(T) == 8 ? T(1e-300) : T(1e-30)));
+    const T inv = T(1) / ljj;
 #pragma unroll
     for (int i = j + 1; i < 9; ++i) {
-      T s = T(0.5) * (M[i + 9 * j] + M[j + 9 * i]);
+      T s = a[sym9(j, i)];
 #pragma unroll
-      for (int k = 0; k < j; ++k) s -= M[i + 9 * k] * M[j + 9 * k];
-      M[i + 9 * j] = s / ljj;
+      for (int k = 0; k < j; ++k) s -= L[sym9(k, i)] * L[sym9(k, j)];
+      L[sym9(j, i)] = s * inv;
     }
-    M[j + 9 * j] = ljj;
+    L[sym9(j, j)] = ljj;
   }
   return pd;
 }
 
-// dM = M' - M (packed upper, 45 values) with M' = M with its eigenvalues clamped at tol.
-// Cyclic Jacobi on the packed upper triangle (symmetric rotation update, eigenvectors in W); stops
-// when the off-diagonal mass is below (rounding-level) 1e-28 (FP64) / 1e-12 (FP32) of the total.
-// Destroys A (col-major 9x9; only the upper triangle is read).
+// Eigen-decomposition of a symmetric 9x9 (packed upper triangle a[sym9(i, j)], destroyed):
+// Householder reduction to tridiagonal form, the reflectors accumulated into Z, then implicit QL
+// sweeps with the Wilkinson-type shift from the leading 2x2 (Golub & Van Loan, Matrix
+// Computations, 4th ed., sections 8.3.1 and 8.3.5, in the QL orientation that deflates from the
+// top).  On return d holds the eigenvalues (unsorted) and column c of Z (Z[r + 9 c]) the unit
+// eigenvector of d[c].  Every array index is a compile-time constant after unrolling, so a, d, e
+// and Z stay in registers.
 template <typename T>
-__host__ __device__ inline void psd_delta(T A[81], T tol, T* out) {
-  T a[45];  // packed upper triangle, a[sym9(i, j)]
+__host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
+  constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
+  constexpr T kTiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38);
+  T e[9], beta[7];
+  // reflector k maps row k's tail a(k, k+1..8) onto its first entry; v_k is kept in that row
 #pragma unroll
-  for (int r = 0; r < 9; ++r)
+  for (int k = 0; k < 7; ++k) {
+    T sig2 = T(0);
 #pragma unroll
-    for (int c = r; c < 9; ++c) a[sym9(r, c)] = (r == c) ? A[r + 9 * c] : T(0.5) * (A[r + 9 * c] + A[c + 9 * r]);
-  T W[81];
+    for (int j = k + 2; j < 9; ++j) sig2 += a[sym9(k, j)] * a[sym9(k, j)];
+    const T x0 = a[sym9(k, k + 1)];
+    const T nrm = sqrt(sig2 + x0 * x0);
+    const T alpha = x0 >= T(0) ? -nrm : nrm;
+    const T vtv = T(2) * nrm * (nrm + fabs(x0));
+    const bool act = sig2 > T(0) && vtv > kTiny;
+    const T bt = act ? T(2) / vtv : T(0);
+    e[k] = act ? alpha : x0;
+    T v[9];
 #pragma unroll
-  for (int k = 0; k < 81; ++k) W[k] = (k % 10 == 0) ? T(1) : T(0);
-  T total = T(0);
+    for (int j = k + 1; j < 9; ++j) v[j] = (j == k + 1) ? x0 - alpha : a[sym9(k, j)];
+    // A22 <- H A22 H = A22 - v w^T - w v^T with p = bt A22 v, w = p - (bt/2)(v.p) v
+    T p[9];
+    T vp = T(0);
 #pragma unroll
-  for (int r = 0; r < 9; ++r)
+    for (int i = k + 1; i < 9; ++i) {
+      T s = T(0);
 #pragma unroll
-    for (int c = r; c < 9; ++c) total += (r == c ? T(1) : T(2)) * a[sym9(r, c)] * a[sym9(r, c)];
-  const T stop = sizeof(T) == 8 ? T(1e-28) : T(1e-12);
-  for (int sweep = 0; sweep < 12; ++sweep) {
-    T off = T(0);
+      for (int j = k + 1; j < 9; ++j) s += a[sym9(i, j)] * v[j];
+      p[i] = bt * s;
+      vp += v[i] * p[i];
+    }
+    const T kk = T(0.5) * bt * vp;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) p[i] -= kk * v[i];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i)
+#pragma unroll
+      for (int j = i; j < 9; ++j) a[sym9(i, j)] -= v[i] * p[j] + p[i] * v[j];
+#pragma unroll
+    for (int j = k + 1; j < 9; ++j) a[sym9(k, j)] = v[j];
+    beta[k] = bt;
+  }
+  e[7] = a[sym9(7, 8)];
+  e[8] = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = a[sym9(i, i)];
+  // Z = H_0 H_1 ... H_6, accumulated from the right end (H_k touches rows and columns k+1..8)
+#pragma unroll
+  for (int i = 0; i < 81; ++i) Z[i] = (i % 10 == 0) ? T(1) : T(0);
+#pragma unroll
+  for (int k = 6; k >= 0; --k) {
+#pragma unroll
+    for (int c = k + 1; c < 9; ++c) {
+      T s = T(0);
+#pragma unroll
+      for (int r = k + 1; r < 9; ++r) s += a[sym9(k, r)] * Z[r + 9 * c];
+      s *= beta[k];
+#pragma unroll
+      for (int r = k + 1; r < 9; ++r) Z[r + 9 * c] -= s * a[sym9(k, r)];
+    }
+  }
+  // implicit QL on (d, e): e[i] couples d[i] and d[i+1].  Rotations at or below the split point m
+  // are evaluated and discarded (select), which keeps the sweep free of lane-divergent branches.
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    for (int iter = 0; iter < 40; ++iter) {
+      int m = 8;
+#pragma unroll
+      for (int j = 7; j >= l; --j)
+        if (fabs(e[j]) <= kEps * (fabs(d[j]) + fabs(d[j + 1]))) m = j;
+      if (m == l) break;
+      T dm = d[8];
+#pragma unroll
+      for (int j = l + 1; j < 8; ++j) dm = (j == m) ? d[j] : dm;
+      T g = (d[l + 1] - d[l]) / (T(2) * e[l]);
+      const T r0 = sqrt(g * g + T(1));
+      g = dm - d[l] + e[l] / (g + (g >= T(0) ? r0 : -r0));
+      T s = T(1), c = T(1), pacc = T(0);
+#pragma unroll
+      for (int i = 7; i >= l; --i) {
+        const bool act = i < m;
+        const T f = s * e[i], b = c * e[i];
+        const T r = sqrt(f * f + g * g);
+        const T inv = r > T(0) ? T(1) / r : T(0);
+        const T sn = r > T(0) ? f * inv : T(0);
+        const T cn = r > T(0) ? g * inv : T(1);
+        const T gi = d[i + 1] - pacc;
+        const T ri = (d[i] - gi) * sn + T(2) * cn * b;
+        const T pi = sn * ri;
+        e[i + 1] = act ? r : e[i + 1];
+        d[i + 1] = act ? gi + pi : d[i + 1];
+        s = act ? sn : s;
+        c = act ? cn : c;
+        pacc = act ? pi : pacc;
+        g = act ? cn * ri - b : g;
+        const T zs = act ? sn : T(0), zc = act ? cn : T(1);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const T zf = Z[k + 9 * (i + 1)], zi = Z[k + 9 * i];
+          Z[k + 9 * (i + 1)] = zs * zi + zc * zf;
+          Z[k + 9 * i] = zc * zi - zs * zf;
+        }
+      }
+      d[l] -= pacc;
+      e[l] = g;
+#pragma unroll
+      for (int j = l + 1; j < 9; ++j) e[j] = (j == m) ? T(0) : e[j];
+    }
+  }
+}
+
+// Clamp correction of a reduced matrix M (packed upper triangle, destroyed): with
+// M = sum_i lambda_i v_i v_i^T, dM = M' - M = sum_{lambda_i < tol} (tol - lambda_i) v_i v_i^T.
+// Returns the number k of eigenvalues below tol and stores dM in the form the assembly reads:
+//   k <= 3: k columns w_j = sqrt(tol - lambda_j) v_j at out[9 j .. 9 j + 8] (dM = W W^T);
+//   k >  3: the packed upper triangle, out[sym9(r, c)] (45 values).
+template <typename T>
+__host__ __device__ inline int psd_correction(T a[45], T tol, T* out) {
+  T d[9], Z[81];
+  sym9_eigen(a, d, Z);
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) k += (d[i] < tol) ? 1 : 0;
+  if (k <= 3) {
+    int j = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      if (d[i] < tol) {
+        const T w = sqrt(tol - d[i]);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) out[9 * j + r] = w * Z[r + 9 * i];
+        ++j;
+      }
+    }
+  } else {
+    T lift[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) lift[i] = (d[i] < tol) ? (tol - d[i]) : T(0);
 #pragma unroll
     for (int r = 0; r < 9; ++r)
 #pragma unroll
-      for (int c = r + 1; c < 9; ++c) off += a[sym9(r, c)] * a[sym9(r, c)];
-    if (!(off > stop * total)) break;
+      for (int c = r; c < 9; ++c) {
+        T sacc = T(0);
 #pragma unroll
-    for (int pp = 0; pp < 8; ++pp)
-#pragma unroll
-      for (int qq = pp + 1; qq < 9; ++qq) {
-        const T apq = a[sym9(pp, qq)];
-        if (apq != T(0)) {
-          const T theta = (a[sym9(qq, qq)] - a[sym9(pp, pp)]) / (T(2) * apq);
-          const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
-          const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
-          a[sym9(pp, pp)] -= t * apq;
-          a[sym9(qq, qq)] += t * apq;
-          a[sym9(pp, qq)] = T(0);
-#pragma unroll
-          for (int r = 0; r < 9; ++r) {
-            if (r == pp || r == qq) continue;
-            const T arp = a[sym9(r, pp)], arq = a[sym9(r, qq)];
-            a[sym9(r, pp)] = c * arp - s * arq;
-            a[sym9(r, qq)] = s * arp + c * arq;
-          }
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            const T wkp = W[k + 9 * pp], wkq = W[k + 9 * qq];
-            W[k + 9 * pp] = c * wkp - s * wkq;
-            W[k + 9 * qq] = s * wkp + c * wkq;
-          }
-        }
+        for (int i = 0; i < 9; ++i) sacc += Z[r + 9 * i] * lift[i] * Z[c + 9 * i];
+        out[sym9(r, c)] = sacc;
       }
   }
-  T lift[9];
+  return k;
+}
+
+// storage format code of a tet's clamp correction: 0 none, 1..3 low-rank columns, 4 packed
+__host__ __device__ __forceinline__ int psd_format(int k) { return k == 0 ? 0 : (k <= 3 ? k : 4); }
+
+// blk_j += (a, m_j) block of (Q (x) I3) W W^T (Q (x) I3)^T for the NB corners m_j, from the k <= 3
+// stored columns W (9 each): U_c = (Q_c (x) I3) W, block = U_a U_m^T
+template <typename T, int NB>
+__host__ __device__ __forceinline__ void psd_lowrank_blocks(const T Q[4][3], int a, const int* mlist, int k,
+                                                            const T* __restrict__ W, T blk[NB][9]) {
+  T qa[3];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const T lam = a[sym9(k, k)];
-    lift[k] = (lam < tol) ? (tol - lam) : T(0);
-  }
+  for (int j = 0; j < 3; ++j) qa[j] = (a == 0) ? Q[0][j] : (a == 1) ? Q[1][j] : (a == 2) ? Q[2][j] : Q[3][j];
 #pragma unroll
-  for (int r = 0; r < 9; ++r)
+  for (int col = 0; col < 3; ++col) {
+    if (col < k) {
+      T w[9];
 #pragma unroll
-    for (int c = r; c < 9; ++c) {
-      T sacc = T(0);
+      for (int r = 0; r < 9; ++r) w[r] = W[9 * col + r];
+      T ua[3];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) sacc += W[r + 9 * k] * lift[k] * W[c + 9 * k];
-      out[sym9(r, c)] = sacc;
+      for (int l = 0; l < 3; ++l) ua[l] = qa[0] * w[l] + qa[1] * w[3 + l] + qa[2] * w[6 + l];
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        const int mm = mlist[n];
+        T qm[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) qm[j] = (mm == 0) ? Q[0][j] : (mm == 1) ? Q[1][j] : (mm == 2) ? Q[2][j] : Q[3][j];
+        T um[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) um[l] = qm[0] * w[l] + qm[1] * w[3 + l] + qm[2] * w[6 + l];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) blk[n][3 * l + kk] += ua[l] * um[kk];
+      }
     }
+  }
 }
 
 // Tm[r] = sum_p Q[a][p] dM_{p r} (3x3 sub-blocks, row-major l, k) for block row a

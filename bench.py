@@ -14,6 +14,7 @@ restated in C++: dense B^T H B, 144 triplets per tet, 12x12 Jacobi PSD, sort-and
 bounded slab of the same mesh.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -73,7 +74,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", default="256,64,64", help="vertices along x,y,z")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1 or 2 lanes per vertex "
+    ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1, 2 or 4 lanes per vertex "
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="timed steps only (for counter collection)")
@@ -106,7 +107,7 @@ def main():
     stream = torch.cuda.current_stream()
     lib0 = _lib.load()
     if args.gather_lanes:
-        lib0.bh_mesh_set_gather_lanes(mesh.handle, args.gather_lanes)
+        assert lib0.bh_mesh_set_gather_lanes(mesh.handle, args.gather_lanes) == 0
     setup_s = time.perf_counter() - t_setup
 
     def step():
@@ -127,12 +128,11 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = ev0.elapsed_time(ev1) / args.steps
-    ms = (np.zeros(5)).astype(np.float64)
-    cnt = np.zeros(5, dtype=np.int64)
+    ms = np.zeros(len(_lib.KERNEL_CLASSES))
+    cnt = np.zeros(len(_lib.KERNEL_CLASSES), dtype=np.int64)
     lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
     lib.bh_mesh_set_profiling(mesh.handle, 0)
     kt = {name: (ms[i] / cnt[i] if cnt[i] else None) for i, name in enumerate(_lib.KERNEL_CLASSES)}
-    import ctypes
 
     flagged = ctypes.c_int64()
     lib.bh_mesh_psd_count(mesh.handle, ctypes.byref(flagged))
@@ -160,31 +160,39 @@ def main():
         torch.cuda.synchronize()
         return a.elapsed_time(b) / reps
 
-    ms_nopsd = timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
-    lib.bh_mesh_set_profiling(mesh.handle, 1)
-    timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
-    lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
-    lib.bh_mesh_set_profiling(mesh.handle, 0)
-    gather_nopsd = ms[2] / cnt[2]
+    def timed_kernels(fn, reps=10):
+        """(ms per call, {kernel class: ms per launch}) with the library's own event pairs"""
+        fn()
+        torch.cuda.synchronize()
+        lib.bh_mesh_set_profiling(mesh.handle, 1)
+        t = timed(fn, reps)
+        lib.bh_mesh_kernel_times(mesh.handle, ms.ctypes.data, cnt.ctypes.data)
+        lib.bh_mesh_set_profiling(mesh.handle, 0)
+        return t, {name: round(ms[i] / cnt[i], 4) for i, name in enumerate(_lib.KERNEL_CLASSES) if cnt[i]}
+
+    def tune(lanes=0, xcd=-1):
+        assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, xcd))) == 0
+
+    ms_nopsd, k_nopsd = timed_kernels(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
+    gather_nopsd = k_nopsd["hessian_gather"]
     ms_dq = timed(lambda: mesh.dq_device(tm.NEOHOOKEAN, q, params, g, stream=stream))
     ms_q = timed(lambda: mesh.q_device(tm.NEOHOOKEAN, q, params, e, stream=stream))
-    # the other assembly-kernel shape, and the FP32 precision form (same workload)
+    # the other kernel schedules (gather lanes x XCD grouping, PSD two-pass / fused) and the FP32
+    # precision form, same workload; none changes a result bit
     variants = {}
-    cur = args.gather_lanes or 1
-    other = 2 if cur == 1 else 1
-    lib.bh_mesh_set_gather_lanes(mesh.handle, other)
-    variants[f"lanes{other}_psd_ms"] = timed(lambda: step())
-    variants[f"lanes{other}_nopsd_ms"] = timed(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
-    lib.bh_mesh_set_gather_lanes(mesh.handle, cur)
     par32 = params.to(torch.float32)
     H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
-    for lanes in (1, 2):
-        lib.bh_mesh_set_gather_lanes(mesh.handle, lanes)
-        variants[f"fp32_lanes{lanes}_psd_ms"] = timed(
-            lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
-        variants[f"fp32_lanes{lanes}_nopsd_ms"] = timed(
-            lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
-    lib.bh_mesh_set_gather_lanes(mesh.handle, cur)
+    for lanes in (1, 2, 4):
+        for xcd in (0, 1):
+            tune(lanes, xcd)
+            key = f"lanes{lanes}_xcd{xcd}"
+            variants[key + "_psd"] = timed_kernels(step)
+            variants[key + "_nopsd"] = timed_kernels(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
+            variants[key + "_fp32_psd"] = timed_kernels(
+                lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
+            variants[key + "_fp32_nopsd"] = timed_kernels(
+                lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
+    tune(args.gather_lanes or 1, 0)
     del H32
 
     B = algorithmic_bytes(nV, nT, nnz)
@@ -224,6 +232,7 @@ def main():
             "dq2_no_psd_ms": ms_nopsd,
             "dq2_no_psd_mtets_s": nT / (ms_nopsd * 1e-3) / 1e6,
             "gather_no_psd_ms": gather_nopsd,
+            "kernels_no_psd_ms": k_nopsd,
             "gather_no_psd_hbm_frac": B["dq2"] * nT / (gather_nopsd * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "dq_ms": ms_dq,
             "dq_hbm_frac": B["dq"] * nT / (ms_dq * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -94,9 +94,18 @@ int bh_mesh_pattern(const bh_mesh* mesh, int32_t* rowptr, int32_t* colind);
 int bh_mesh_block_pattern_device(const bh_mesh* mesh, const int32_t** d_bptr, const int32_t** d_bcol,
                                  int64_t* nblocks);
 int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
-/* Assembly-kernel shape: 1 lane per vertex (default) or 2 lanes per vertex (each lane owns two of
- * the four corner blocks; half the LDS per lane).  Same per-slot fold order either way. */
+/* Assembly-kernel shape: 1, 2 or 4 lanes per vertex (with 2 or 4, each lane owns two or one of the
+ * four corner blocks of every incident tet).  Same per-slot fold order. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
+
+/* Kernel-schedule knobs of a mesh (no reference counterpart: the reference has one CPU schedule).
+ * Every setting keeps each slot's ascending-element fold, so results agree to the last bits; a
+ * fixed setting is bitwise reproducible. */
+typedef struct bh_tuning {
+  int gather_lanes; /* 0 keep, else 1, 2 or 4 lanes per vertex in the assembly gather */
+  int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
+} bh_tuning;
+int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
 
 /* ---- energy / gradient / Hessian (host arrays) -------------------------------------------- */
 
@@ -238,9 +247,8 @@ typedef enum {
   BH_KT_ENERGY = 0,
   BH_KT_GRADIENT = 1,
   BH_KT_HESSIAN_GATHER = 2, /* the CSR assembly kernel (element blocks + deterministic gather) */
-  BH_KT_PSD_CLASSIFY = 3,
-  BH_KT_PSD_EIGEN = 4,
-  BH_KT_COUNT = 5
+  BH_KT_PSD = 3,            /* simple_psd_fix pass: Cholesky test, eigen-clamp of the tets that fail */
+  BH_KT_COUNT = 4
 } bh_kernel_class;
 
 /* on != 0 starts recording (and clears previous records); on == 0 stops. */

@@ -66,28 +66,43 @@ static int psd_blocks(const double* phi12, const double* x12, double vol, double
   make_F(phi12, x12, phi, P.F);
   P.vol = (T)vol;
   pre_hessian<MODEL>(P, (T)p0, (T)p1, ref != 0);
-  T Q[4][3], R[3][3], M[81], Mc[81], dM[45];
+  T Q[4][3], R[3][3], M[45], dM[45];
   phi_qr(phi, Q, R);
-  reduced_matrix<MODEL>(P, R, M);
-  for (int i = 0; i < 81; ++i) Mc[i] = M[i];
-  const bool pd = shifted_pd(Mc, (T)tol);
-  if (!pd) psd_delta(M, (T)tol, dM);
+  reduced_packed<MODEL>(P, R, M);
+  const bool pd = shifted_pd(M, (T)tol);
+  const int fmt = pd ? 0 : psd_format(psd_correction(M, (T)tol, dM));
   const T tp = (T)((tol > 0 ? tol : 0.0) * 0.25);
+  const int all4[4] = {0, 1, 2, 3};
   for (int a = 0; a < 4; ++a) {
-    T Tm[3][9];
-    if (!pd) psd_row_factor(Q, a, dM, Tm);
+    T b[4][9];
     for (int m = 0; m < 4; ++m) {
-      T b[9];
-      hess_block<MODEL>(P, phi[a], phi[m], b);
-      b[0] += tp;
-      b[4] += tp;
-      b[8] += tp;
-      if (!pd) psd_add_block(Q[m], Tm, b);
-      for (int l = 0; l < 3; ++l)
-        for (int k = 0; k < 3; ++k) K[(3 * a + l) + 12 * (3 * m + k)] = (double)b[3 * l + k];
+      hess_block<MODEL>(P, phi[a], phi[m], b[m]);
+      b[m][0] += tp;
+      b[m][4] += tp;
+      b[m][8] += tp;
     }
+    if (fmt >= 1 && fmt <= 3) psd_lowrank_blocks<T, 4>(Q, a, all4, fmt, dM, b);
+    if (fmt == 4) {
+      T Tm[3][9];
+      psd_row_factor(Q, a, dM, Tm);
+      for (int m = 0; m < 4; ++m) psd_add_block(Q[m], Tm, b[m]);
+    }
+    for (int m = 0; m < 4; ++m)
+      for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 3; ++k) K[(3 * a + l) + 12 * (3 * m + k)] = (double)b[m][3 * l + k];
   }
-  return pd ? 0 : 1;
+  return fmt;
+}
+
+// eigen-decomposition of a symmetric 9x9 (col-major; upper triangle read): eigenvalues d, vectors Z
+template <typename T>
+static void eig9(const double* A, double* d, double* Z) {
+  T a[45], dd[9], ZZ[81];
+  for (int r = 0; r < 9; ++r)
+    for (int c = r; c < 9; ++c) a[sym9(r, c)] = (T)A[r + 9 * c];
+  sym9_eigen(a, dd, ZZ);
+  for (int i = 0; i < 9; ++i) d[i] = (double)dd[i];
+  for (int i = 0; i < 81; ++i) Z[i] = (double)ZZ[i];
 }
 
 #define DISPATCH(fn, ...)                                              \
@@ -110,6 +125,12 @@ void eh_grad(int model, int fp32, const double* phi12, const double* x12, double
 int eh_psd_blocks(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1,
                   int ref, double tol, double* K) {
   DISPATCH(psd_blocks, phi12, x12, vol, p0, p1, ref, tol, K)
+}
+void eh_eig9(int fp32, const double* A, double* d, double* Z) {
+  if (fp32)
+    eig9<float>(A, d, Z);
+  else
+    eig9<double>(A, d, Z);
 }
 double eh_energy(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1) {
   DISPATCH(energy, phi12, x12, vol, p0, p1)
