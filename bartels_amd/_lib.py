@@ -37,6 +37,11 @@ class BhTuning(ctypes.Structure):
     _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int)]
 
 
+# void (*on_system)(void* user, int iteration, const float* vals32, const double* vals64, const double* b,
+#                   const double* x)
+ON_SYSTEM = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p)
+
+
 class BhNewtonOpts(ctypes.Structure):
     _fields_ = [
         ("model", c_int),
@@ -56,6 +61,11 @@ class BhNewtonOpts(ctypes.Structure):
         ("precond", c_int),
         ("precond_handle", c_void_p),
         ("precond_shift", c_double),
+        ("element_precision", c_int),
+        ("jacobi_fallback", c_int),
+        ("mgn", c_void_p),
+        ("on_system", ON_SYSTEM),
+        ("user", c_void_p),
     ]
 
 
@@ -65,6 +75,7 @@ class BhNewtonReport(ctypes.Structure):
         ("converged", c_int),
         ("pcg_iterations_total", c_int),
         ("pcg_failures", c_int),
+        ("pcg_fallbacks", c_int),
         ("ls_iterations_total", c_int),
         ("energy", c_double),
         ("gradient_norm", c_double),
@@ -116,6 +127,15 @@ SIGNATURES = {
     "bh_precond_set_factor_device": (c_int, [_VP, _VP, _VP, _VP]),
     "bh_precond_lists_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p), P(c_void_p)]),
     "bh_precond_apply_device": (c_int, [_VP, _VP, _VP, _VP]),
+    "bh_mgn_create": (c_int, [ctypes.c_uint64, P(_VP)]),
+    "bh_mgn_destroy": (c_int, [_VP]),
+    "bh_mgn_num_params": (c_int, [_VP, P(c_int64)]),
+    "bh_mgn_get_params": (c_int, [_VP, _VP, _VP]),
+    "bh_mgn_set_params": (c_int, [_VP, _VP]),
+    "bh_mgn_loss_device": (c_int, [_VP, _VP, _VP, _VP, _VP, _VP, c_int, P(c_double), _VP]),
+    "bh_mgn_adam_step": (c_int, [_VP, c_double, c_double, c_double, c_double, _VP]),
+    "bh_mgn_factor_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "bh_mgn_debug_buffers": (c_int, [_VP, P(_VP), P(_VP), P(_VP), P(_VP)]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
     "bh_mesh_psd_count": (c_int, [_VP, P(c_int64)]),
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),

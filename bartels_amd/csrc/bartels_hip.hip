@@ -67,7 +67,7 @@ struct bh_mesh {
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
-  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1, 2 or 4)
+  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
   // scratch
   double* partials = nullptr;
@@ -332,125 +332,51 @@ __global__ void __launch_bounds__(256)
 
 // ---- Hessian (vertex gather into LDS) -----------------------------------------------------------
 // One workgroup owns VPW consecutive vertices, i.e. one contiguous segment of the CSR values, and
-// accumulates it in LDS.  LPV lanes serve each vertex: lane h of the vertex owns corners
-// h, h + LPV, ... of every incident tet, i.e. NB = 4 / LPV blocks (a, m) of the vertex's block row.
-// The corners of a tet are distinct vertices, so the lanes of a vertex never update the same LDS
-// slot, and every slot receives its contributions in ascending element order.  LPV = 1 computes
-// the per-tet setup once per vertex; LPV = 2, 4 repeat it per lane for more lanes in flight.
-//
-// The walk over the incidence list is software-pipelined three deep: while tet k is evaluated,
-// the element data of tet k+1 and the list entry of tet k+2 are already in flight, so each step
-// waits on one memory latency (the vertex positions and stored PSD columns of tet k) instead of
-// the three dependent ones (list entry -> element data -> positions).
+// accumulates it in LDS.  One lane per vertex (k_hessian) computes the vertex's four blocks of each
+// incident tet; two lanes per vertex (k_hessian_pair) split them as corners {h, h + 2}.  The
+// corners of a tet are distinct vertices, so the lanes of a vertex never update the same LDS slot,
+// and every slot receives its contributions in ascending element order.  A vertex row of the FP64
+// form is 9 nb doubles (1080 B at valence 14), so LDS caps one-lane workgroups of 64 vertices at
+// two per CU: occupancy and latency, not HBM, bound this kernel.
 
-// element data of one incident tet (everything that depends only on the element index)
-template <typename T>
-struct TetIn {
-  int4 ev;
-  T phi[4][3];
-  T vol, p0, p1;
-  int mid;
-  int fmt;  // stored PSD correction format (0 when none or PSD off)
-};
-
-template <int MODEL, bool PSD, typename T>
-__device__ __forceinline__ void load_tet(const MeshView& m, const T* __restrict__ params,
-                                         const uint8_t* __restrict__ mids, const PsdView<T>& psd, int e,
-                                         TetIn<T>& t) {
-  t.vol = load_vol<T>(m, e);
-  t.p0 = params[e];
-  t.mid = model_at<MODEL>(mids, e);
-  if (MODEL == M_MASS) {  // consistent mass: params = density (one value per tet), no geometry needed
-    t.p1 = T(0);
-    t.fmt = 0;
-    return;
-  }
-  t.ev = load_E(m, e);
-  load_phi(m, e, t.phi);
-  t.p1 = params[(int64_t)m.nT + e];
-  t.fmt = PSD ? (int)psd.flag[e] : 0;
-}
-
-__device__ __forceinline__ void load_x(const double* __restrict__ q, int4 ev, double x[4][3]) {
-  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
+// blk_j += (a, mlist[j]) block of the stored clamp correction of one tet: for fmt = k <= 3 the k
+// columns u = (Q (x) I3) w of the 12x12 correction U U^T (12 values each, u[3 c + l] for corner c);
+// for fmt 4 the packed reduced 9x9 dM, mapped through Q (x) I3 here
+template <typename T, int NB>
+__device__ __forceinline__ void add_psd_correction(const T phi[4][3], const T* __restrict__ W, int fmt, int a,
+                                                   const int* mlist, T blk[NB][9]) {
+  if (fmt <= 3) {
 #pragma unroll
-  for (int mm = 0; mm < 4; ++mm) {
-    const double* qp = q + 3ll * vid[mm];
-    x[mm][0] = qp[0];
-    x[mm][1] = qp[1];
-    x[mm][2] = qp[2];
-  }
-}
-
-// F from loaded positions (same arithmetic as load_F)
-template <typename T>
-__device__ __forceinline__ void make_F(const double x[4][3], const T phi[4][3], T* F) {
-  if constexpr (sizeof(T) == 8) {
+    for (int col = 0; col < 3; ++col) {
+      if (col < fmt) {
+        const T* u = W + 12 * col;
+        T ua[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int l = 0; l < 3; ++l) ua[l] = u[3 * a + l];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j] + x[3][i] * phi[3][j];
-  } else {
-    T ed[3][3];
+        for (int j = 0; j < NB; ++j) {
+          const T* um = u + 3 * mlist[j];
+          const T u0 = um[0], u1 = um[1], u2 = um[2];
 #pragma unroll
-    for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(x[mm + 1][i] - x[0][i]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) F[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
-  }
-}
-
-// blocks (a, h + LPV j), j < NB, of one tet for one element model
-template <int MODEL, int NB, int LPV, typename T>
-__device__ __forceinline__ void row_blocks(const TetIn<T>& t, const double x[4][3], int a, int h, bool stretchRef,
-                                           T blk[NB][9]) {
-  if (MODEL == M_MASS) {
-    const T mass = t.p0 * t.vol;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const int mm = h + LPV * j;
-      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) blk[j][i] = (i % 4 == 0) ? c : T(0);
+          for (int l = 0; l < 3; ++l) {
+            blk[j][3 * l] += ua[l] * u0;
+            blk[j][3 * l + 1] += ua[l] * u1;
+            blk[j][3 * l + 2] += ua[l] * u2;
+          }
+        }
+      }
     }
     return;
   }
-  Pre<T> P;
-  make_F(x, t.phi, P.F);
-  P.vol = t.vol;
-  T pa[3];
-  select_row(t.phi, a, pa);
-  pre_hessian<MODEL>(P, t.p0, t.p1, stretchRef);
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    T pm[3];
-    select_row(t.phi, h + LPV * j, pm);
-    hess_block<MODEL>(P, pa, pm, blk[j]);
-  }
-}
-
-// blk_j += (a, mlist[j]) block of the stored clamp correction: k = fmt <= 3 columns already loaded
-// into w (27 values), or the packed 9x9 at Wp (fmt 4)
-template <typename T, int NB>
-__device__ __forceinline__ void add_psd_correction(const T phi[4][3], const T w[27], const T* __restrict__ Wp,
-                                                   int fmt, int a, const int* mlist, T blk[NB][9]) {
   T Q[4][3], R[3][3];
   phi_qr(phi, Q, R);
-  if (fmt <= 3) {
-    psd_lowrank_blocks<T, NB>(Q, a, mlist, fmt, w, blk);
-  } else {
-    T Tm[3][9];
-    psd_row_factor(Q, a, Wp, Tm);
+  T Tm[3][9];
+  psd_row_factor(Q, a, W, Tm);
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      T qm[3];
-      select_row(Q, mlist[j], qm);
-      psd_add_block(qm, Tm, blk[j]);
-    }
+  for (int j = 0; j < NB; ++j) {
+    T qm[3];
+    select_row(Q, mlist[j], qm);
+    psd_add_block(qm, Tm, blk[j]);
   }
 }
 
@@ -464,14 +390,71 @@ __device__ __forceinline__ int gather_group(int xcd, int ngroups) {
   return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
 }
 
-constexpr int gather_threads(int vpw, int lpv) { return vpw * lpv < 64 ? 64 : vpw * lpv; }
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __restrict__ q,
+                                           const T* __restrict__ params, int e, int a, bool stretch_ref,
+                                           T blk[4][9]) {
+  if (MODEL == M_MASS) {
+    // consistent mass (src/linear_tet_mass_matrix.cpp:8-44); params[e] = density, q unused
+    const T mass = params[e] * load_vol<T>(m, e);
+#pragma unroll
+    for (int mm = 0; mm < 4; ++mm) {
+      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[mm][i] = (i % 4 == 0) ? c : T(0);
+    }
+    return;
+  }
+  T phi[4][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
+  load_F(q, load_E(m, e), phi, P.F);
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  T pa[3];
+  select_row(phi, a, pa);
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
+}
 
-template <int MODEL, bool PSD, int VPW, int LPV, typename T>
-__global__ void __launch_bounds__(gather_threads(VPW, LPV))
+// two of the four blocks of the vertex's block row: corners h and h + 2 (h = 0 or 1)
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __restrict__ q,
+                                            const T* __restrict__ params, int e, int a, int h, bool stretch_ref,
+                                            T blk[2][9]) {
+  if (MODEL == M_MASS) {
+    const T mass = params[e] * load_vol<T>(m, e);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int mm = h + 2 * j;
+      const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) blk[j][i] = (i % 4 == 0) ? c : T(0);
+    }
+    return;
+  }
+  T phi[4][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
+  load_F(q, load_E(m, e), phi, P.F);
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  T pa[3];
+  select_row(phi, a, pa);
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    T pm[3];
+    select_row(phi, h + 2 * j, pm);
+    hess_block<MODEL>(P, pa, pm, blk[j]);
+  }
+}
+
+template <int MODEL, bool PSD, int VPW, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
     k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
               const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, int xcd, T* __restrict__ vals) {
-  constexpr int NB = 4 / LPV;
-  constexpr int NT = gather_threads(VPW, LPV);
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
   const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
@@ -481,76 +464,126 @@ __global__ void __launch_bounds__(gather_threads(VPW, LPV))
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
   const int seglen = 9 * (m.bptr[vend] - b0);
-  for (int i = threadIdx.x; i < seglen; i += NT) acc[i] = T(0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
   __syncthreads();
-  const int vl = (int)threadIdx.x / LPV, h = (int)threadIdx.x % LPV;
-  const int v = v0 + vl;
-  if (vl < VPW && v < vend) {
+  const int v = v0 + (int)threadIdx.x;
+  if ((int)threadIdx.x < VPW && v < m.nV) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
     const IncList L = inc_list(m, v);
-    int mlist[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) mlist[j] = h + LPV * j;
-    // pipeline: A = tet being evaluated, B = element data in flight, C = list entry in flight
-    int codeA = (L.len > 0) ? m.incE[L.first] : -1;
-    uint32_t rkA = (L.len > 0) ? m.incR[L.first] : 0u;
-    int codeB = (L.len > 1) ? m.incE[L.first + 64] : -1;
-    uint32_t rkB = (L.len > 1) ? m.incR[L.first + 64] : 0u;
-    TetIn<T> tA;
-    if (codeA >= 0) load_tet<MODEL, PSD>(m, params, mids, psd, codeA >> 2, tA);
-    for (int k = 0; codeA >= 0; ++k) {
-      const int eA = codeA >> 2, a = codeA & 3;
-      double x[4][3];
-      if (MODEL != M_MASS) load_x(q, tA.ev, x);
-      T w[PSD ? 27 : 1];
-      if (PSD) {
-        // stored columns of tet A (the packed form is read inside the correction)
-        const T* W = psd.dM + 45ll * eA;
-#pragma unroll
-        for (int i = 0; i < 27; ++i) w[i] = (i < 9 * tA.fmt && tA.fmt <= 3) ? W[i] : T(0);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incR[idx];
+      T blk[4][9];
+      switch (model_at<MODEL>(mids, e)) {
+        case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
+        case M_STVK: tet_blocks<M_STVK>(m, q, params, e, a, stretchRef, blk); break;
+        case M_COROT: tet_blocks<M_COROT>(m, q, params, e, a, stretchRef, blk); break;
+        case M_ARAP: tet_blocks<M_ARAP>(m, q, params, e, a, stretchRef, blk); break;
+        case M_MASS: tet_blocks<M_MASS>(m, q, params, e, a, stretchRef, blk); break;
+        default: tet_blocks<M_NH_LEGACY>(m, q, params, e, a, stretchRef, blk); break;
       }
-      TetIn<T> tB;
-      if (codeB >= 0) load_tet<MODEL, PSD>(m, params, mids, psd, codeB >> 2, tB);
-      const bool more = k + 2 < L.len;
-      const int codeC = more ? m.incE[L.first + 64ll * (k + 2)] : -1;
-      const uint32_t rkC = more ? m.incR[L.first + 64ll * (k + 2)] : 0u;
+      if (PSD) {
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) {
+          blk[mm][0] += psd.tolPos;
+          blk[mm][4] += psd.tolPos;
+          blk[mm][8] += psd.tolPos;
+        }
+        const int fmt = psd.flag[e];
+        if (fmt) {
+          T phi[4][3];
+          load_phi(m, e, phi);
+          const int all4[4] = {0, 1, 2, 3};
+          add_psd_correction<T, 4>(phi, psd.dM + 45ll * e, fmt, a, all4, blk);
+        }
+      }
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm) {
+        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[mm][3 * l + kk];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
 
-      T blk[NB][9];
-      switch (tA.mid) {
-        case M_NH: row_blocks<M_NH, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
-        case M_STVK: row_blocks<M_STVK, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
-        case M_COROT: row_blocks<M_COROT, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
-        case M_ARAP: row_blocks<M_ARAP, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
-        case M_MASS: row_blocks<M_MASS, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
-        default: row_blocks<M_NH_LEGACY, NB, LPV>(tA, x, a, h, stretchRef != 0, blk); break;
+// Same gather with two lanes per vertex: lane h of the pair owns the blocks of corners h and h+2 of
+// every incident tet.  The four corners of a tet are four distinct vertices, so the pair never
+// updates the same LDS slot, and every slot still receives its contributions in ascending element
+// order.  Half the LDS per lane of the one-lane kernel (more waves per CU) for a duplicated
+// per-tet setup.
+template <int MODEL, bool PSD, int VPW, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_pair(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                   const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, int xcd, T* __restrict__ vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
+  if (grp * VPW >= m.nV) return;
+  const int v0 = grp * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 1, h = (int)threadIdx.x & 1;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incR[idx];
+      T blk[2][9];
+      switch (model_at<MODEL>(mids, e)) {
+        case M_NH: tet_blocks2<M_NH>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_STVK: tet_blocks2<M_STVK>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_COROT: tet_blocks2<M_COROT>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_ARAP: tet_blocks2<M_ARAP>(m, q, params, e, a, h, stretchRef, blk); break;
+        case M_MASS: tet_blocks2<M_MASS>(m, q, params, e, a, h, stretchRef, blk); break;
+        default: tet_blocks2<M_NH_LEGACY>(m, q, params, e, a, h, stretchRef, blk); break;
       }
       if (PSD) {
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
+        for (int j = 0; j < 2; ++j) {
           blk[j][0] += psd.tolPos;
           blk[j][4] += psd.tolPos;
           blk[j][8] += psd.tolPos;
         }
-        if (tA.fmt) add_psd_correction<T, NB>(tA.phi, w, psd.dM + 45ll * eA, tA.fmt, a, mlist, blk);
+        const int fmt = psd.flag[e];
+        if (fmt) {
+          T phi[4][3];
+          load_phi(m, e, phi);
+          const int two[2] = {h, h + 2};
+          add_psd_correction<T, 2>(phi, psd.dM + 45ll * e, fmt, a, two, blk);
+        }
       }
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        T* dst = row + 3 * ((rkA >> (8 * mlist[j])) & 0xffu);
+      for (int j = 0; j < 2; ++j) {
+        const int mm = h + 2 * j;
+        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
           for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[j][3 * l + kk];
       }
-      codeA = codeB;
-      rkA = rkB;
-      tA = tB;
-      codeB = codeC;
-      rkB = rkC;
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < seglen; i += NT) vals[seg0 + i] = acc[i];
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
 }
 
 // ---- PSD projection (simple_psd_fix, src/simple_psd_fix.cpp:5-21) ------------------------------
@@ -607,7 +640,30 @@ __global__ void __launch_bounds__(64)
   }
   count_flagged(need, count);
   if (e >= m.nT) return;
-  flag[e] = need ? (uint8_t)psd_format(psd_correction(M, tol, dM + 45ll * e)) : (uint8_t)0;
+  int fmt = 0;
+  if (need) {
+    T* out = dM + 45ll * e;
+    fmt = psd_format(psd_correction(M, tol, out));
+    if (fmt >= 1 && fmt <= 3) {
+      // columns w (9 each) -> u = (Q (x) I3) w (12 each, corner-major), written back in place from the last
+      T phi[4][3], Q[4][3], R[3][3];
+      load_phi(m, e, phi);
+      phi_qr(phi, Q, R);
+      T w[27];
+#pragma unroll
+      for (int i = 0; i < 27; ++i) w[i] = (i < 9 * fmt) ? out[i] : T(0);
+#pragma unroll
+      for (int col = 0; col < 3; ++col)
+        if (col < fmt) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+              out[12 * col + 3 * c + l] = Q[c][0] * w[9 * col + l] + Q[c][1] * w[9 * col + 3 + l] + Q[c][2] * w[9 * col + 6 + l];
+        }
+    }
+  }
+  flag[e] = (uint8_t)fmt;
 }
 
 // ---- mass / external force / setup / spmv -----------------------------------------------------------
@@ -676,24 +732,6 @@ __global__ void __launch_bounds__(256) k_to_f32(int64_t n, const double* __restr
   if (i < n) b[i] = (float)a[i];
 }
 
-// y = H x on the block pattern; one lane per row
-template <typename V>
-__global__ void __launch_bounds__(256)
-    k_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol, const V* __restrict__ vals,
-           const double* __restrict__ x, double* __restrict__ y) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= 3 * nV) return;
-  const int v = row / 3, l = row - 3 * v;
-  const int b0 = bptr[v], b1 = bptr[v + 1], nb = b1 - b0;
-  const V* vr = vals + 9ll * b0 + 3ll * l * nb;
-  double s = 0.0;
-  for (int r = 0; r < nb; ++r) {
-    const double* xp = x + 3ll * bcol[b0 + r];
-    s += (double)vr[3 * r] * xp[0] + (double)vr[3 * r + 1] * xp[1] + (double)vr[3 * r + 2] * xp[2];
-  }
-  y[row] = s;
-}
-
 // ---------------------------------------------------------------------------------------------
 bool is_model(int model) { return model >= 0 && model <= 4; }
 
@@ -734,8 +772,8 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const
                        T* vals, hipStream_t s) {
   const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
   const size_t lds = sizeof(T) * (size_t)std::max(seg, 1);
-  auto kern = k_hessian<MODEL, PSD, VPW, LPV, T>;
-  const int threads = gather_threads(VPW, LPV);
+  auto kern = (LPV == 1) ? k_hessian<MODEL, PSD, VPW, T> : k_hessian_pair<MODEL, PSD, VPW, T>;
+  const int threads = kGatherThreads;
   static std::once_flag once;
   std::call_once(once, [&] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -756,10 +794,6 @@ int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uin
   const size_t lim = (size_t)max_lds_bytes();
   const size_t s64 = sizeof(T) * (size_t)m->maxSeg64, s32 = sizeof(T) * (size_t)m->maxSeg32,
                s16 = sizeof(T) * (size_t)m->maxSeg16;
-  if (m->gatherLanes == 4) {
-    if (s64 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 64, 4>(m, q, params, mids, psd, vals, s);
-    if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 4>(m, q, params, mids, psd, vals, s);
-  }
   if (m->gatherLanes == 2) {
     if (s32 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 32, 2>(m, q, params, mids, psd, vals, s);
     if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 2>(m, q, params, mids, psd, vals, s);
@@ -1297,15 +1331,14 @@ int bh_mesh_psd_count(bh_mesh* m, int64_t* count) {
 }
 
 int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
-  if (!m || (lanes != 1 && lanes != 2 && lanes != 4)) return BH_BAD_ARG;
+  if (!m || (lanes != 1 && lanes != 2)) return BH_BAD_ARG;
   m->gatherLanes = lanes;
   return BH_OK;
 }
 
 int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (!m || !t) return BH_BAD_ARG;
-  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2 && t->gather_lanes != 4)
-    return BH_BAD_ARG;
+  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2) return BH_BAD_ARG;
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->gather_lanes) m->gatherLanes = t->gather_lanes;
   if (t->xcd_groups >= 0) m->xcdGroups = t->xcd_groups;
@@ -1335,17 +1368,10 @@ int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
   return BH_OK;
 }
 
-int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* d_y, bh_stream_t s) {
-  if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
-  if (m->nV == 0) return BH_OK;
-  hipLaunchKernelGGL(k_spmv<double>, dim3((3 * m->nV + 255) / 256), dim3(256), 0, as_stream(s), m->nV, m->bptr,
-                     m->bcol, d_vals, d_x, d_y);
-  BH_CHECK(hipGetLastError());
-  return BH_OK;
-}
 
 }  // extern "C"
 
 // block lower-triangular preconditioners (colour sweeps), then the device Newton
 #include "precond_impl.h"
+#include "mgn_impl.h"
 #include "newton_impl.h"

@@ -1,0 +1,998 @@
+// Learned preconditioner: a mesh-graph network that writes the block lower-triangular factor of
+// bh_precond (included at the end of bartels_hip.hip, after precond_impl.h; same translation unit).
+// No counterpart in the reference (its Newton direction is a direct sparse solve).
+//
+// Graph: the vertices of the mesh and the lower pairs k = (v, n) of the colour order (colour(n) <
+// colour(v)), i.e. exactly the off-diagonal blocks of the factor L.
+// Inputs, from the masked Hessian H (fixed DOFs -> identity rows / columns):
+//   s_v = sqrt(tr(H_vv) / 3),  Hn = S^{-1} H S^{-1}  (S = diag(s_v I3)),  C_v = chol(Hn_vv)
+//   node features (8): Hn_vv[00, 11, 22, 10, 20, 21], fixed DOFs of v / 3, (pattern degree - 1) / 16
+//   pair features (10): Hn_vn row-major (9), log(s_v / s_n)
+// Network (width 32, every MLP = Linear -> ReLU -> Linear, weights row-major [in][out]):
+//   h = MLP_node_enc(x_v),  e = MLP_pair_enc(f_k)
+//   3 x { e_k += MLP_edge([e_k, h_v, h_n]);  m_v = sum of e_k over the pairs of v (lower list, then
+//         upper list);  h_v += MLP_node([h_v, m_v]) }
+//   d_v = h_v W_dv + b_dv (6),  O_k = e_k W_do + b_do (9)       (decoders start at zero)
+// Factor (normalised): Lhat_vv = C_v T_v with T_v lower triangular, T_ii = exp(d), strict lower = d;
+//   Lhat_vn = O_k.  At zero decoders Lhat Lhat^T = blockdiag(Hn_vv): block Jacobi.  The factor of H
+//   is L = S Lhat (L_vv = s_v Lhat_vv, L_vn = s_v Lhat_vn).
+// Training loss on a Newton system H x = b (x from the plain solver):  with xh = S x, bh = S^{-1} b,
+//   loss = |Lhat Lhat^T xh - bh|^2 / |bh|^2.
+// Everything is FP32; every reduction has a fixed order (results are deterministic).
+
+struct bh_mgn {
+  int64_t nparam = 0;
+  float* w = nullptr;   // parameters
+  float* g = nullptr;   // gradient accumulator
+  float* m1 = nullptr;  // Adam moments
+  float* m2 = nullptr;
+  int64_t step = 0;
+  // per-graph workspace
+  int wsV = -1;
+  int64_t wsP = -1;
+  float* ws = nullptr;
+  size_t wsBytes = 0;
+  double* dscal = nullptr;  // [loss numerator, |bh|^2]
+  float* part = nullptr;    // weight-gradient partials
+  int32_t* pv = nullptr;    // pair -> higher vertex (owner row)
+  int64_t pvSize = 0;
+};
+
+namespace {
+
+constexpr int kW = 32;           // latent width
+constexpr int kLayers = 3;       // message-passing layers
+constexpr int kNodeIn = 8, kPairIn = 10;
+constexpr int kOuterBlocks = 512;  // blocks of the weight-gradient reductions
+
+// flat parameter layout
+struct MlpOff {
+  int64_t W1, b1, W2, b2;
+};
+struct MgnLayout {
+  MlpOff nenc, penc, edge[kLayers], node[kLayers];
+  int64_t Wdv, bdv, Wdo, bdo, total;
+};
+inline MlpOff mlp_off(int64_t& o, int in) {
+  MlpOff m;
+  m.W1 = o; o += (int64_t)in * kW;
+  m.b1 = o; o += kW;
+  m.W2 = o; o += kW * kW;
+  m.b2 = o; o += kW;
+  return m;
+}
+inline MgnLayout mgn_layout() {
+  MgnLayout L;
+  int64_t o = 0;
+  L.nenc = mlp_off(o, kNodeIn);
+  L.penc = mlp_off(o, kPairIn);
+  for (int l = 0; l < kLayers; ++l) {
+    L.edge[l] = mlp_off(o, 3 * kW);
+    L.node[l] = mlp_off(o, 2 * kW);
+  }
+  L.Wdv = o; o += kW * 6;
+  L.bdv = o; o += 6;
+  L.Wdo = o; o += kW * 9;
+  L.bdo = o; o += 9;
+  L.total = o;
+  return L;
+}
+
+// ---- workspace ------------------------------------------------------------------------------------
+struct MgnWs {
+  float *s, *C, *xn, *fp;             // scales, chol(Hn_vv), node / pair features
+  float *h[kLayers + 1], *e[kLayers + 1];
+  float *hidN[kLayers + 1], *hidE[kLayers + 1];  // MLP hidden activations (index 0: encoders)
+  float* m[kLayers];                  // aggregated messages
+  float *d, *O, *y, *r;               // decoder outputs, L^T xh, residual
+  float *gh, *ge, *gm, *gin, *ghid, *gd, *gO, *gy, *gz, *gh2, *ge2;
+};
+
+// carves the workspace from base (or only sizes it when base is null); returns the float count
+size_t mgn_ws_carve(float* base, int nV, int64_t nP, MgnWs& w) {
+  size_t used = 0;
+  auto take = [&](size_t n) {
+    float* r = base ? base + used : nullptr;
+    used += (n + 15) & ~size_t(15);
+    return r;
+  };
+  const size_t V = nV, P = nP;
+  w.s = take(V);
+  w.C = take(6 * V);
+  w.xn = take(kNodeIn * V);
+  w.fp = take(kPairIn * P);
+  for (int l = 0; l <= kLayers; ++l) {
+    w.h[l] = take(kW * V);
+    w.e[l] = take(kW * P);
+    w.hidN[l] = take(kW * V);
+    w.hidE[l] = take(kW * P);
+  }
+  for (int l = 0; l < kLayers; ++l) w.m[l] = take(kW * V);
+  w.d = take(6 * V);
+  w.O = take(9 * P);
+  w.y = take(3 * V);
+  w.r = take(3 * V);
+  w.gh = take(kW * V);
+  w.gh2 = take(kW * V);
+  w.gm = take(kW * V);
+  w.ge = take(kW * P);
+  w.ge2 = take(kW * P);
+  w.gin = take(kW * std::max(3 * P, 2 * V));
+  w.ghid = take(kW * std::max(V, P));
+  w.gd = take(6 * V);
+  w.gO = take(9 * P);
+  w.gy = take(3 * V);
+  w.gz = take(3 * V);
+  return used;
+}
+
+// ---- features -------------------------------------------------------------------------------------
+// masked 3x3 block (v, n) of H, row-major, from a block index of the mesh pattern
+template <typename V>
+__device__ __forceinline__ void mgn_block(const V* __restrict__ vals, const int32_t* __restrict__ bptr, int v, int slot,
+                                          const uint8_t* __restrict__ fixed, int n, float* A) {
+  const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+  const V* base = vals + 9ll * b0 + 3ll * (slot - b0);
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float a = (float)base[3ll * l * nb + k];
+      if (fixed && (fixed[3ll * v + l] || fixed[3ll * n + k])) a = (v == n && l == k) ? 1.f : 0.f;
+      A[3 * l + k] = a;
+    }
+}
+
+// per vertex: s_v, C_v, node features; per lower pair of v: pair features
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_mgn_prep(int nV, const int32_t* __restrict__ bptr, const V* __restrict__ vals, const uint8_t* __restrict__ fixed,
+               const int32_t* __restrict__ dslot, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+               const int32_t* __restrict__ lslot, float* __restrict__ s, float* __restrict__ C,
+               float* __restrict__ xn) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  float D[9];
+  mgn_block(vals, bptr, v, dslot[v], fixed, v, D);
+  const float sv = sqrtf(fmaxf((D[0] + D[4] + D[8]) * (1.f / 3.f), 1e-30f));
+  const float i2 = 1.f / (sv * sv);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) D[i] *= i2;
+  // Cholesky of the normalised diagonal block (pivots floored: a tiny positive value keeps L_vv
+  // invertible if a block is only semidefinite)
+  const float c00 = sqrtf(fmaxf(D[0], 1e-12f));
+  const float c10 = D[3] / c00, c20 = D[6] / c00;
+  const float c11 = sqrtf(fmaxf(D[4] - c10 * c10, 1e-12f));
+  const float c21 = (D[7] - c20 * c10) / c11;
+  const float c22 = sqrtf(fmaxf(D[8] - c20 * c20 - c21 * c21, 1e-12f));
+  s[v] = sv;
+  float* Cv = C + 6ll * v;
+  Cv[0] = c00; Cv[1] = c10; Cv[2] = c11; Cv[3] = c20; Cv[4] = c21; Cv[5] = c22;
+  int nf = 0;
+  if (fixed) nf = (int)(fixed[3ll * v] != 0) + (int)(fixed[3ll * v + 1] != 0) + (int)(fixed[3ll * v + 2] != 0);
+  float* x = xn + (int64_t)kNodeIn * v;
+  x[0] = D[0]; x[1] = D[4]; x[2] = D[8]; x[3] = D[3]; x[4] = D[6]; x[5] = D[7];
+  x[6] = nf * (1.f / 3.f);
+  x[7] = (float)(bptr[v + 1] - bptr[v] - 1) * (1.f / 16.f);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_mgn_pairfeat(int nV, const int32_t* __restrict__ bptr, const V* __restrict__ vals,
+                   const uint8_t* __restrict__ fixed, const int32_t* __restrict__ lptr,
+                   const int32_t* __restrict__ lcol, const int32_t* __restrict__ lslot, const float* __restrict__ s,
+                   float* __restrict__ fp) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const float sv = s[v];
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+    const int n = lcol[k];
+    float A[9];
+    mgn_block(vals, bptr, v, lslot[k], fixed, n, A);
+    const float sn = s[n];
+    const float inv = 1.f / (sv * sn);
+    float* f = fp + (int64_t)kPairIn * k;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i] = A[i] * inv;
+    f[9] = logf(sv / sn);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mgn_pair_owner(int nV, const int32_t* __restrict__ lptr,
+                                                        int32_t* __restrict__ pv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) pv[k] = v;
+}
+
+// ---- MLP rows -------------------------------------------------------------------------------------
+// An MLP input is either one segment of IN <= 32 features or IN / 32 segments of 32: segment i of
+// row r is p_i[(idx_i ? idx_i[r] : r) * width + j].  Widths are compile-time, so the gathered row
+// stays in registers.
+struct Seg3 {
+  const float* p[3];
+  const int32_t* idx[3];
+};
+
+template <int IN>
+__device__ __forceinline__ void gather_in(const Seg3& S, int64_t r, float* x) {
+  if constexpr (IN <= kW) {
+    const int64_t row = S.idx[0] ? (int64_t)S.idx[0][r] : r;
+    const float* src = S.p[0] + row * IN;
+#pragma unroll
+    for (int j = 0; j < IN; ++j) x[j] = src[j];
+  } else {
+#pragma unroll
+    for (int sg = 0; sg < IN / kW; ++sg) {
+      const int64_t row = S.idx[sg] ? (int64_t)S.idx[sg][r] : r;
+      const float* src = S.p[sg] + row * kW;
+#pragma unroll
+      for (int j = 0; j < kW; ++j) x[sg * kW + j] = src[j];
+    }
+  }
+}
+
+// out = [resid +] relu(x W1 + b1) W2 + b2; hid = relu(...) kept for the backward.  One row per
+// lane; the weights sit in LDS and are read as float4 rows (the same address across the wave, a
+// broadcast); the input is streamed segment by segment from global memory.
+template <int IN>
+__global__ void __launch_bounds__(256)
+    k_mlp_fwd(int64_t rows, Seg3 S, const float* __restrict__ W1, const float* __restrict__ b1,
+              const float* __restrict__ W2, const float* __restrict__ b2, const float* __restrict__ resid,
+              float* __restrict__ hid, float* __restrict__ out) {
+  __shared__ __align__(16) float sW1[IN * kW];
+  __shared__ __align__(16) float sW2[kW * kW];
+  __shared__ float sb1[kW], sb2[kW];
+  for (int i = threadIdx.x; i < IN * kW; i += blockDim.x) sW1[i] = W1[i];
+  for (int i = threadIdx.x; i < kW * kW; i += blockDim.x) sW2[i] = W2[i];
+  if (threadIdx.x < kW) {
+    sb1[threadIdx.x] = b1[threadIdx.x];
+    sb2[threadIdx.x] = b2[threadIdx.x];
+  }
+  __syncthreads();
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float hh[kW];
+#pragma unroll
+  for (int j = 0; j < kW; ++j) hh[j] = sb1[j];
+  constexpr int NSEG = IN <= kW ? 1 : IN / kW;
+  constexpr int SW = IN <= kW ? IN : kW;
+#pragma unroll
+  for (int sg = 0; sg < NSEG; ++sg) {
+    const int64_t row = S.idx[sg] ? (int64_t)S.idx[sg][r] : r;
+    const float* src = S.p[sg] + row * SW;
+#pragma unroll 2
+    for (int i = 0; i < SW; ++i) {
+      const float xi = src[i];
+      const float4* wr = reinterpret_cast<const float4*>(sW1 + (sg * SW + i) * kW);
+#pragma unroll
+      for (int q = 0; q < kW / 4; ++q) {
+        const float4 wv = wr[q];
+        hh[4 * q] += xi * wv.x;
+        hh[4 * q + 1] += xi * wv.y;
+        hh[4 * q + 2] += xi * wv.z;
+        hh[4 * q + 3] += xi * wv.w;
+      }
+    }
+  }
+  float* hp = hid + r * kW;
+#pragma unroll
+  for (int j = 0; j < kW; ++j) {
+    hh[j] = fmaxf(hh[j], 0.f);
+    hp[j] = hh[j];
+  }
+  float o[kW];
+#pragma unroll
+  for (int j = 0; j < kW; ++j) o[j] = sb2[j] + (resid ? resid[r * kW + j] : 0.f);
+#pragma unroll
+  for (int i = 0; i < kW; ++i) {
+    const float hi = hh[i];
+    const float4* wr = reinterpret_cast<const float4*>(sW2 + i * kW);
+#pragma unroll
+    for (int q = 0; q < kW / 4; ++q) {
+      const float4 wv = wr[q];
+      o[4 * q] += hi * wv.x;
+      o[4 * q + 1] += hi * wv.y;
+      o[4 * q + 2] += hi * wv.z;
+      o[4 * q + 3] += hi * wv.w;
+    }
+  }
+  float* op = out + r * kW;
+#pragma unroll
+  for (int j = 0; j < kW; ++j) op[j] = o[j];
+}
+
+// given gout: ghid = (gout W2^T) * [hid > 0]; gin = ghid W1^T (IN values per row)
+template <int IN>
+__global__ void __launch_bounds__(256)
+    k_mlp_bwd(int64_t rows, const float* __restrict__ W1, const float* __restrict__ W2, const float* __restrict__ hid,
+              const float* __restrict__ gout, float* __restrict__ ghid, float* __restrict__ gin) {
+  __shared__ float sW1[IN * kW], sW2[kW * kW];
+  for (int i = threadIdx.x; i < IN * kW; i += blockDim.x) sW1[i] = W1[i];
+  for (int i = threadIdx.x; i < kW * kW; i += blockDim.x) sW2[i] = W2[i];
+  __syncthreads();
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float go[kW], gh[kW];
+#pragma unroll
+  for (int j = 0; j < kW; ++j) go[j] = gout[r * kW + j];
+#pragma unroll
+  for (int i = 0; i < kW; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int j = 0; j < kW; ++j) a += go[j] * sW2[i * kW + j];
+    gh[i] = (hid[r * kW + i] > 0.f) ? a : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < kW; ++i) ghid[r * kW + i] = gh[i];
+  if (!gin) return;
+  for (int i = 0; i < IN; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int j = 0; j < kW; ++j) a += gh[j] * sW1[i * kW + j];
+    gin[r * IN + i] = a;
+  }
+}
+
+// Deterministic weight-gradient partials: block b sums, over its fixed share of the rows, the outer
+// products x_r^T g_r (XW x GW) and g_r (GW) -> part[b][XW * GW + GW].  Rows are staged through LDS
+// in chunks of 32; thread t owns entries t, t + 256, ...
+template <int XW, int GW>
+__global__ void __launch_bounds__(256)
+    k_outer(int64_t rows, Seg3 S, const float* __restrict__ g, float* __restrict__ part) {
+  constexpr int NE = XW * GW + GW;
+  constexpr int PER = (NE + 255) / 256;
+  constexpr int CH = 32;
+  __shared__ float sx[CH * XW], sg[CH * GW];
+  float acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.f;
+  const int64_t per = (rows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = blockIdx.x * per, r1 = min(rows, r0 + per);
+  for (int64_t c = r0; c < r1; c += CH) {
+    const int nr = (int)min<int64_t>(CH, r1 - c);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nr; t += blockDim.x) {
+      float x[XW];
+      gather_in<XW>(S, c + t, x);
+#pragma unroll
+      for (int i = 0; i < XW; ++i) sx[t * XW + i] = x[i];
+    }
+    for (int t = threadIdx.x; t < nr * GW; t += blockDim.x) sg[t] = g[(c + t / GW) * GW + t % GW];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int ent = threadIdx.x + 256 * q;
+      if (ent < NE) {
+        float a = acc[q];
+        if (ent < XW * GW) {
+          const int i = ent / GW, j = ent % GW;
+          for (int t = 0; t < nr; ++t) a += sx[t * XW + i] * sg[t * GW + j];
+        } else {
+          const int j = ent - XW * GW;
+          for (int t = 0; t < nr; ++t) a += sg[t * GW + j];
+        }
+        acc[q] = a;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int ent = threadIdx.x + 256 * q;
+    if (ent < NE) part[(int64_t)blockIdx.x * NE + ent] = acc[q];
+  }
+}
+
+// grad[W] += sum_b part[b][0 .. XW*GW), grad[bias] += sum_b part[b][XW*GW ..)
+__global__ void __launch_bounds__(256) k_outer_reduce(int nb, int XW, int GW, const float* __restrict__ part,
+                                                      float* __restrict__ gW, float* __restrict__ gb) {
+  const int NE = XW * GW + GW;
+  const int ent = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ent >= NE) return;
+  float a = 0.f;
+  for (int b = 0; b < nb; ++b) a += part[(int64_t)b * NE + ent];
+  if (ent < XW * GW)
+    gW[ent] += a;
+  else
+    gb[ent - XW * GW] += a;
+}
+
+// ---- message aggregation -------------------------------------------------------------------------
+// m_v = sum over v's lower pairs (list order) then upper pairs (list order) of e_k; 32 lanes per vertex
+__global__ void __launch_bounds__(256)
+    k_mgn_aggregate(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ uptr,
+                    const int32_t* __restrict__ ulink, const float* __restrict__ e, float* __restrict__ m) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int v = (int)(gid / kW), j = (int)(gid % kW);
+  if (v >= nV) return;
+  float a = 0.f;
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) a += e[(int64_t)k * kW + j];
+  for (int t = uptr[v]; t < uptr[v + 1]; ++t) a += e[(int64_t)ulink[t] * kW + j];
+  m[(int64_t)v * kW + j] = a;
+}
+
+// pair gradients from a per-vertex gradient: ge_k (+)= gm_v + gm_n
+__global__ void __launch_bounds__(256)
+    k_mgn_agg_bwd(int64_t nP, const int32_t* __restrict__ pv, const int32_t* __restrict__ lcol,
+                  const float* __restrict__ gm, const float* __restrict__ gin_e, float* __restrict__ ge) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t k = gid / kW;
+  const int j = (int)(gid % kW);
+  if (k >= nP) return;
+  ge[k * kW + j] = gin_e[k * kW + j] + gm[(int64_t)pv[k] * kW + j] + gm[(int64_t)lcol[k] * kW + j];
+}
+
+// vertex gradients from the edge-MLP input gradient gin (96 per pair: [e, h_v, h_n]) and the node
+// path: gh_v = ghn_v + sum_{lower pairs of v} gin_k[32:64] + sum_{upper pairs} gin_k[64:96]
+__global__ void __launch_bounds__(256)
+    k_mgn_gather_gh(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ uptr,
+                    const int32_t* __restrict__ ulink, const float* __restrict__ gin, const float* __restrict__ ghn,
+                    float* __restrict__ gh) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int v = (int)(gid / kW), j = (int)(gid % kW);
+  if (v >= nV) return;
+  float a = ghn[(int64_t)v * kW + j];
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) a += gin[(int64_t)k * 3 * kW + kW + j];
+  for (int t = uptr[v]; t < uptr[v + 1]; ++t) a += gin[(int64_t)ulink[t] * 3 * kW + 2 * kW + j];
+  gh[(int64_t)v * kW + j] = a;
+}
+
+// pair path of the edge MLP backward: ge_{l-1} = ge_l + gin[0:32]
+__global__ void __launch_bounds__(256) k_mgn_ge_prev(int64_t nP, const float* __restrict__ ge,
+                                                     const float* __restrict__ gin, float* __restrict__ out) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= nP * kW) return;
+  const int64_t k = gid / kW;
+  const int j = (int)(gid % kW);
+  out[gid] = ge[gid] + gin[k * 3 * kW + j];
+}
+
+// node path of the node MLP backward: gh_prev = gh + gin[0:32], gm = gin[32:64]
+__global__ void __launch_bounds__(256) k_mgn_gh_split(int nV, const float* __restrict__ gh, const float* __restrict__ gin,
+                                                      float* __restrict__ ghp, float* __restrict__ gm) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)nV * kW) return;
+  const int64_t v = gid / kW;
+  const int j = (int)(gid % kW);
+  ghp[gid] = gh[gid] + gin[v * 2 * kW + j];
+  gm[gid] = gin[v * 2 * kW + kW + j];
+}
+
+// ---- decoders and the factor -------------------------------------------------------------------------
+template <int GW>
+__global__ void __launch_bounds__(256) k_linear(int64_t rows, const float* __restrict__ x, const float* __restrict__ W,
+                                                const float* __restrict__ b, float* __restrict__ y) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float o[GW];
+#pragma unroll
+  for (int j = 0; j < GW; ++j) o[j] = b[j];
+  for (int i = 0; i < kW; ++i) {
+    const float xi = x[r * kW + i];
+#pragma unroll
+    for (int j = 0; j < GW; ++j) o[j] += xi * W[i * GW + j];
+  }
+#pragma unroll
+  for (int j = 0; j < GW; ++j) y[r * GW + j] = o[j];
+}
+
+// gx (+)= g W^T  (rows x 32)
+template <int GW>
+__global__ void __launch_bounds__(256) k_linear_bwd(int64_t rows, const float* __restrict__ g,
+                                                    const float* __restrict__ W, float* __restrict__ gx) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float gg[GW];
+#pragma unroll
+  for (int j = 0; j < GW; ++j) gg[j] = g[r * GW + j];
+  for (int i = 0; i < kW; ++i) {
+    float a = 0.f;
+#pragma unroll
+    for (int j = 0; j < GW; ++j) a += gg[j] * W[i * GW + j];
+    gx[r * kW + i] = a;
+  }
+}
+
+// Lhat_vv = C_v T(d_v), row-major 3x3 (lower triangular)
+__device__ __forceinline__ void mgn_lvv(const float* C, const float* d, float* L) {
+  const float t00 = expf(d[0]), t10 = d[1], t11 = expf(d[2]), t20 = d[3], t21 = d[4], t22 = expf(d[5]);
+  const float c00 = C[0], c10 = C[1], c11 = C[2], c20 = C[3], c21 = C[4], c22 = C[5];
+  L[0] = c00 * t00;
+  L[1] = 0.f;
+  L[2] = 0.f;
+  L[3] = c10 * t00 + c11 * t10;
+  L[4] = c11 * t11;
+  L[5] = 0.f;
+  L[6] = c20 * t00 + c21 * t10 + c22 * t20;
+  L[7] = c21 * t11 + c22 * t21;
+  L[8] = c22 * t22;
+}
+
+// out_n = Lhat_nn^T a_n + sum_{pairs k = (v, n)} O_k^T a_v, with a = sx ? s .* u : u
+__global__ void __launch_bounds__(256)
+    k_mgn_lt(int nV, const int32_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+             const int32_t* __restrict__ ulink, const float* __restrict__ C, const float* __restrict__ d,
+             const float* __restrict__ O, const float* __restrict__ s, const double* __restrict__ u, int scale,
+             const float* __restrict__ uf, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nV) return;
+  auto val = [&](int v, int l) -> float {
+    if (uf) return uf[3ll * v + l];
+    const float a = (float)u[3ll * v + l];
+    return scale ? a * s[v] : a;
+  };
+  float L[9];
+  mgn_lvv(C + 6ll * n, d + 6ll * n, L);
+  const float a0 = val(n, 0), a1 = val(n, 1), a2 = val(n, 2);
+  float o0 = L[0] * a0 + L[3] * a1 + L[6] * a2;
+  float o1 = L[1] * a0 + L[4] * a1 + L[7] * a2;
+  float o2 = L[2] * a0 + L[5] * a1 + L[8] * a2;
+  for (int t = uptr[n]; t < uptr[n + 1]; ++t) {
+    const float* B = O + 9ll * ulink[t];
+    const int v = ucol[t];
+    const float b0 = val(v, 0), b1 = val(v, 1), b2 = val(v, 2);
+    o0 += B[0] * b0 + B[3] * b1 + B[6] * b2;
+    o1 += B[1] * b0 + B[4] * b1 + B[7] * b2;
+    o2 += B[2] * b0 + B[5] * b1 + B[8] * b2;
+  }
+  out[3ll * n] = o0;
+  out[3ll * n + 1] = o1;
+  out[3ll * n + 2] = o2;
+}
+
+// r_v = Lhat_vv y_v + sum_{lower k of v} O_k y_n - b_v / s_v; per-block partials of |r|^2, |bh|^2
+__global__ void __launch_bounds__(256)
+    k_mgn_resid(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ O,
+                const float* __restrict__ s, const float* __restrict__ y, const double* __restrict__ b,
+                float* __restrict__ r, double* __restrict__ part) {
+  __shared__ double sh[8];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (v < nV) {
+    float L[9];
+    mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+    const float y0 = y[3ll * v], y1 = y[3ll * v + 1], y2 = y[3ll * v + 2];
+    float z0 = L[0] * y0 + L[1] * y1 + L[2] * y2;
+    float z1 = L[3] * y0 + L[4] * y1 + L[5] * y2;
+    float z2 = L[6] * y0 + L[7] * y1 + L[8] * y2;
+    for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+      const float* B = O + 9ll * k;
+      const int n = lcol[k];
+      const float w0 = y[3ll * n], w1 = y[3ll * n + 1], w2 = y[3ll * n + 2];
+      z0 += B[0] * w0 + B[1] * w1 + B[2] * w2;
+      z1 += B[3] * w0 + B[4] * w1 + B[5] * w2;
+      z2 += B[6] * w0 + B[7] * w1 + B[8] * w2;
+    }
+    const float is = 1.f / s[v];
+    const float bh0 = (float)b[3ll * v] * is, bh1 = (float)b[3ll * v + 1] * is, bh2 = (float)b[3ll * v + 2] * is;
+    const float r0 = z0 - bh0, r1 = z1 - bh1, r2 = z2 - bh2;
+    r[3ll * v] = r0;
+    r[3ll * v + 1] = r1;
+    r[3ll * v + 2] = r2;
+    rr = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+    bb = (double)bh0 * bh0 + (double)bh1 * bh1 + (double)bh2 * bh2;
+  }
+  rr = block_reduce_sum(rr, sh);
+  __syncthreads();
+  bb = block_reduce_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = rr;
+    part[2 * blockIdx.x + 1] = bb;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mgn_loss_sum(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[8];
+  double a = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += part[2 * i];
+    c += part[2 * i + 1];
+  }
+  a = block_reduce_sum(a, sh);
+  __syncthreads();
+  c = block_reduce_sum(c, sh);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = c;
+  }
+}
+
+// gz = 2 r / |bh|^2
+__global__ void __launch_bounds__(256) k_mgn_gz(int64_t n, const float* __restrict__ r, const double* __restrict__ sc,
+                                                float* __restrict__ gz) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  gz[i] = (float)(2.0 / sc[1]) * r[i];
+}
+
+// gO_k = gz_v y_n^T + xh_v gy_n^T ; gd_v from gLhat_vv = lower(gz_v y_v^T + xh_v gy_v^T), gT = C^T gLhat
+__global__ void __launch_bounds__(256)
+    k_mgn_gfactor(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                  const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ s,
+                  const double* __restrict__ x, const float* __restrict__ y, const float* __restrict__ gz,
+                  const float* __restrict__ gy, float* __restrict__ gO, float* __restrict__ gd) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const float sv = s[v];
+  const float xh[3] = {(float)x[3ll * v] * sv, (float)x[3ll * v + 1] * sv, (float)x[3ll * v + 2] * sv};
+  const float gzv[3] = {gz[3ll * v], gz[3ll * v + 1], gz[3ll * v + 2]};
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+    const int n = lcol[k];
+    float* o = gO + 9ll * k;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[3 * l + c] = gzv[l] * y[3ll * n + c] + xh[l] * gy[3ll * n + c];
+  }
+  float G[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) G[3 * l + c] = (c <= l) ? gzv[l] * y[3ll * v + c] + xh[l] * gy[3ll * v + c] : 0.f;
+  // Lhat = C T: gT = C^T G (lower part kept)
+  const float* Cv = C + 6ll * v;
+  const float c00 = Cv[0], c10 = Cv[1], c11 = Cv[2], c20 = Cv[3], c21 = Cv[4], c22 = Cv[5];
+  const float Cm[9] = {c00, 0.f, 0.f, c10, c11, 0.f, c20, c21, c22};
+  float gT[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * G[j] + Cm[3 + i] * G[3 + j] + Cm[6 + i] * G[6 + j];
+  const float* dv = d + 6ll * v;
+  float* g = gd + 6ll * v;
+  g[0] = gT[0] * expf(dv[0]);
+  g[1] = gT[3];
+  g[2] = gT[4] * expf(dv[2]);
+  g[3] = gT[6];
+  g[4] = gT[7];
+  g[5] = gT[8] * expf(dv[5]);
+}
+
+// the factor of H for bh_precond: L_vn = s_v O_k, inverse of L_vv = s_v Lhat_vv (packed lower)
+__global__ void __launch_bounds__(256)
+    k_mgn_factor_out(int nV, const int32_t* __restrict__ lptr, const float* __restrict__ C,
+                     const float* __restrict__ d, const float* __restrict__ O, const float* __restrict__ s,
+                     float* __restrict__ lval, float* __restrict__ dinv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const float sv = s[v];
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) lval[9ll * k + i] = sv * O[9ll * k + i];
+  float L[9];
+  mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+  const double l00 = (double)sv * L[0], l10 = (double)sv * L[3], l11 = (double)sv * L[4];
+  const double l20 = (double)sv * L[6], l21 = (double)sv * L[7], l22 = (double)sv * L[8];
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11;
+  const double i21 = -l21 * i11 * i22;
+  const double i20 = -(l20 * i00 + l21 * i10) * i22;
+  float* o = dinv + 6ll * v;
+  o[0] = (float)i00;
+  o[1] = (float)i10;
+  o[2] = (float)i11;
+  o[3] = (float)i20;
+  o[4] = (float)i21;
+  o[5] = (float)i22;
+}
+
+__global__ void __launch_bounds__(256) k_mgn_adam(int64_t n, float* __restrict__ w, float* __restrict__ g,
+                                                  float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
+                                                  float b2, float eps, float c1, float c2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+  g[i] = 0.f;
+}
+
+// ---- host drivers ----------------------------------------------------------------------------------
+inline dim3 g256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int mgn_prepare(bh_mgn* M, const bh_precond* P) {
+  const int nV = P->mesh->nV;
+  const int64_t nP = P->nlower;
+  if (M->wsV != nV || M->wsP != nP) {
+    MgnWs dry;
+    const size_t need = sizeof(float) * mgn_ws_carve(nullptr, nV, nP, dry);
+    if (need > M->wsBytes) {
+      if (M->ws) BH_CHECK(hipFree(M->ws));
+      M->ws = nullptr;
+      BH_CHECK(hipMalloc(&M->ws, need));
+      M->wsBytes = need;
+    }
+    if (M->pvSize < nP) {
+      if (M->pv) BH_CHECK(hipFree(M->pv));
+      BH_CHECK(hipMalloc(&M->pv, sizeof(int32_t) * std::max<int64_t>(1, nP)));
+      M->pvSize = nP;
+    }
+    M->wsV = nV;
+    M->wsP = nP;
+  }
+  return BH_OK;
+}
+
+template <typename V>
+int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, MgnWs& w, hipStream_t s) {
+  const bh_mesh* m = P->mesh;
+  const int nV = m->nV;
+  const int64_t nP = P->nlower;
+  const MgnLayout L = mgn_layout();
+  const float* W = M->w;
+  hipLaunchKernelGGL((k_mgn_prep<V>), g256(nV), dim3(256), 0, s, nV, m->bptr, vals, fixed, P->dslot, P->lptr, P->lcol,
+                     P->lslot, w.s, w.C, w.xn);
+  hipLaunchKernelGGL((k_mgn_pairfeat<V>), g256(nV), dim3(256), 0, s, nV, m->bptr, vals, fixed, P->lptr, P->lcol,
+                     P->lslot, w.s, w.fp);
+  hipLaunchKernelGGL(k_mgn_pair_owner, g256(nV), dim3(256), 0, s, nV, P->lptr, M->pv);
+  BH_CHECK(hipGetLastError());
+  auto mlp = [&](auto inTag, int64_t rows, Seg3 S, const MlpOff& o, const float* resid, float* hid, float* out) {
+    constexpr int IN = decltype(inTag)::value;
+    if (rows == 0) return;
+    hipLaunchKernelGGL((k_mlp_fwd<IN>), g256(rows), dim3(256), 0, s, rows, S, W + o.W1, W + o.b1, W + o.W2, W + o.b2,
+                       resid, hid, out);
+  };
+  using I8 = std::integral_constant<int, kNodeIn>;
+  using I10 = std::integral_constant<int, kPairIn>;
+  using I64 = std::integral_constant<int, 2 * kW>;
+  using I96 = std::integral_constant<int, 3 * kW>;
+  mlp(I8{}, nV, Seg3{{w.xn, nullptr, nullptr}, {nullptr, nullptr, nullptr}}, L.nenc, nullptr,
+      w.hidN[0], w.h[0]);
+  mlp(I10{}, nP, Seg3{{w.fp, nullptr, nullptr}, {nullptr, nullptr, nullptr}}, L.penc, nullptr,
+      w.hidE[0], w.e[0]);
+  for (int l = 0; l < kLayers; ++l) {
+    mlp(I96{}, nP, Seg3{{w.e[l], w.h[l], w.h[l]}, {nullptr, M->pv, P->lcol}}, L.edge[l], w.e[l],
+        w.hidE[l + 1], w.e[l + 1]);
+    hipLaunchKernelGGL(k_mgn_aggregate, g256((int64_t)nV * kW), dim3(256), 0, s, nV, P->lptr, P->uptr, P->ulink,
+                       w.e[l + 1], w.m[l]);
+    mlp(I64{}, nV, Seg3{{w.h[l], w.m[l], nullptr}, {nullptr, nullptr, nullptr}}, L.node[l], w.h[l],
+        w.hidN[l + 1], w.h[l + 1]);
+  }
+  hipLaunchKernelGGL(k_linear<6>, g256(nV), dim3(256), 0, s, (int64_t)nV, w.h[kLayers], W + L.Wdv, W + L.bdv, w.d);
+  if (nP) hipLaunchKernelGGL(k_linear<9>, g256(nP), dim3(256), 0, s, nP, w.e[kLayers], W + L.Wdo, W + L.bdo, w.O);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+// loss (and, with backward, the gradient accumulated into M->g) of one Newton system H x = b
+template <typename V>
+int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, const double* b, const double* x,
+             int backward, double* loss, hipStream_t s) {
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  MgnWs w;
+  mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
+  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  const int nV = P->mesh->nV;
+  const int64_t nP = P->nlower;
+  if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
+  const MgnLayout L = mgn_layout();
+  // y = Lhat^T xh ; r = Lhat y - bh
+  hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, x, 1,
+                     (const float*)nullptr, w.y);
+  const int nb = (int)((nV + 255) / 256);
+  double* part = reinterpret_cast<double*>(M->part);
+  hipLaunchKernelGGL(k_mgn_resid, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.O, w.s, w.y, b, w.r,
+                     part);
+  hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal);
+  BH_CHECK(hipGetLastError());
+  if (backward) {
+    float* G = M->g;
+    float* wp = M->part;  // weight-gradient partials (after the loss partials were consumed)
+    auto outer = [&](auto xwTag, auto gwTag, int64_t rows, Seg3 S, const float* g, float* gW, float* gb) {
+      constexpr int XW = decltype(xwTag)::value, GW = decltype(gwTag)::value;
+      if (rows == 0) return;
+      hipLaunchKernelGGL((k_outer<XW, GW>), dim3(kOuterBlocks), dim3(256), 0, s, rows, S, g, wp);
+      hipLaunchKernelGGL(k_outer_reduce, g256(XW * GW + GW), dim3(256), 0, s, kOuterBlocks, XW, GW, wp, gW, gb);
+    };
+    using I6 = std::integral_constant<int, 6>;
+    using I9 = std::integral_constant<int, 9>;
+    using I8 = std::integral_constant<int, kNodeIn>;
+    using I10 = std::integral_constant<int, kPairIn>;
+    using I32 = std::integral_constant<int, kW>;
+    using I64 = std::integral_constant<int, 2 * kW>;
+    using I96 = std::integral_constant<int, 3 * kW>;
+    auto plain = [](const float* p, int) { return Seg3{{p, nullptr, nullptr}, {nullptr, nullptr, nullptr}}; };
+    const float* W = M->w;
+    // factor -> gd, gO
+    hipLaunchKernelGGL(k_mgn_gz, g256(3ll * nV), dim3(256), 0, s, 3ll * nV, w.r, M->dscal, w.gz);
+    hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s,
+                       (const double*)nullptr, 0, w.gz, w.gy);
+    hipLaunchKernelGGL(k_mgn_gfactor, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.s, x, w.y, w.gz,
+                       w.gy, w.gO, w.gd);
+    // decoders
+    outer(I32{}, I6{}, nV, plain(w.h[kLayers], kW), w.gd, G + L.Wdv, G + L.bdv);
+    outer(I32{}, I9{}, nP, plain(w.e[kLayers], kW), w.gO, G + L.Wdo, G + L.bdo);
+    hipLaunchKernelGGL(k_linear_bwd<6>, g256(nV), dim3(256), 0, s, (int64_t)nV, w.gd, W + L.Wdv, w.gh);
+    if (nP) hipLaunchKernelGGL(k_linear_bwd<9>, g256(nP), dim3(256), 0, s, nP, w.gO, W + L.Wdo, w.ge);
+    for (int l = kLayers - 1; l >= 0; --l) {
+      // node MLP: h_{l+1} = h_l + MLP([h_l, m_l])
+      const MlpOff& on = L.node[l];
+      hipLaunchKernelGGL(k_mlp_bwd<2 * kW>, g256(nV), dim3(256), 0, s, (int64_t)nV, W + on.W1, W + on.W2,
+                         w.hidN[l + 1], w.gh, w.ghid, w.gin);
+      outer(I32{}, I32{}, nV, plain(w.hidN[l + 1], kW), w.gh, G + on.W2, G + on.b2);
+      outer(I64{}, I32{}, nV, Seg3{{w.h[l], w.m[l], nullptr}, {nullptr, nullptr, nullptr}}, w.ghid,
+            G + on.W1, G + on.b1);
+      hipLaunchKernelGGL(k_mgn_gh_split, g256((int64_t)nV * kW), dim3(256), 0, s, nV, w.gh, w.gin, w.gh2, w.gm);
+      // aggregation: ge_{l+1} += gm_v + gm_n
+      if (nP) {
+        hipLaunchKernelGGL(k_mgn_agg_bwd, g256(nP * kW), dim3(256), 0, s, nP, M->pv, P->lcol, w.gm, w.ge, w.ge2);
+        // edge MLP: e_{l+1} = e_l + MLP([e_l, h_l[v], h_l[n]])
+        const MlpOff& oe = L.edge[l];
+        hipLaunchKernelGGL(k_mlp_bwd<3 * kW>, g256(nP), dim3(256), 0, s, nP, W + oe.W1, W + oe.W2, w.hidE[l + 1],
+                           w.ge2, w.ghid, w.gin);
+        outer(I32{}, I32{}, nP, plain(w.hidE[l + 1], kW), w.ge2, G + oe.W2, G + oe.b2);
+        outer(I96{}, I32{}, nP, Seg3{{w.e[l], w.h[l], w.h[l]}, {nullptr, M->pv, P->lcol}}, w.ghid,
+              G + oe.W1, G + oe.b1);
+        hipLaunchKernelGGL(k_mgn_ge_prev, g256(nP * kW), dim3(256), 0, s, nP, w.ge2, w.gin, w.ge);
+      }
+      hipLaunchKernelGGL(k_mgn_gather_gh, g256((int64_t)nV * kW), dim3(256), 0, s, nV, P->lptr, P->uptr, P->ulink,
+                         w.gin, w.gh2, w.gh);
+    }
+    // encoders (weights only)
+    hipLaunchKernelGGL(k_mlp_bwd<kNodeIn>, g256(nV), dim3(256), 0, s, (int64_t)nV, W + L.nenc.W1, W + L.nenc.W2,
+                       w.hidN[0], w.gh, w.ghid, (float*)nullptr);
+    outer(I32{}, I32{}, nV, plain(w.hidN[0], kW), w.gh, G + L.nenc.W2, G + L.nenc.b2);
+    outer(I8{}, I32{}, nV, plain(w.xn, kNodeIn), w.ghid, G + L.nenc.W1, G + L.nenc.b1);
+    if (nP) {
+      hipLaunchKernelGGL(k_mlp_bwd<kPairIn>, g256(nP), dim3(256), 0, s, nP, W + L.penc.W1, W + L.penc.W2, w.hidE[0],
+                         w.ge, w.ghid, (float*)nullptr);
+      outer(I32{}, I32{}, nP, plain(w.hidE[0], kW), w.ge, G + L.penc.W2, G + L.penc.b2);
+      outer(I10{}, I32{}, nP, plain(w.fp, kPairIn), w.ghid, G + L.penc.W1, G + L.penc.b1);
+    }
+    BH_CHECK(hipGetLastError());
+  }
+  if (loss) {
+    double h[2];
+    BH_CHECK(hipMemcpyAsync(h, M->dscal, sizeof(h), hipMemcpyDeviceToHost, s));
+    BH_CHECK(hipStreamSynchronize(s));
+    *loss = h[1] > 0 ? h[0] / h[1] : 0.0;
+  }
+  return BH_OK;
+}
+
+template <typename V>
+int mgn_factor(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hipStream_t s) {
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  MgnWs w;
+  mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
+  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  const int nV = P->mesh->nV;
+  hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, w.C, w.d, w.O, w.s, P->lval, P->dinv);
+  BH_CHECK(hipGetLastError());
+  return precond_finalize(P, s);
+}
+
+// portable parameter initialisation (splitmix64 uniforms): He-uniform first layers, second layers
+// scaled by 1/4 (residual branches start small), zero biases and zero decoders
+void mgn_init_host(std::vector<float>& w, uint64_t seed) {
+  const MgnLayout L = mgn_layout();
+  w.assign((size_t)L.total, 0.f);
+  uint64_t ctr = 0;
+  auto uni = [&]() {
+    uint64_t z = seed + (++ctr) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int64_t off, int in, int out, double scale) {
+    const double bound = std::sqrt(6.0 / in) * scale;
+    for (int64_t i = 0; i < (int64_t)in * out; ++i) w[off + i] = (float)((2.0 * uni() - 1.0) * bound);
+  };
+  auto mlp = [&](const MlpOff& o, int in) {
+    fill(o.W1, in, kW, 1.0);
+    fill(o.W2, kW, kW, 0.25);
+  };
+  mlp(L.nenc, kNodeIn);
+  mlp(L.penc, kPairIn);
+  for (int l = 0; l < kLayers; ++l) {
+    mlp(L.edge[l], 3 * kW);
+    mlp(L.node[l], 2 * kW);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_mgn_create(uint64_t seed, bh_mgn** out) {
+  if (!out) return BH_BAD_ARG;
+  *out = nullptr;
+  auto* M = new bh_mgn();
+  const MgnLayout L = mgn_layout();
+  M->nparam = L.total;
+  std::vector<float> h;
+  mgn_init_host(h, seed);
+  const size_t bytes = sizeof(float) * (size_t)M->nparam;
+  const size_t partBytes = sizeof(float) * (size_t)kOuterBlocks * (3 * kW * kW + kW) + sizeof(double) * 2 * 65536;
+  if (hipMalloc(&M->w, bytes) != hipSuccess || hipMalloc(&M->g, bytes) != hipSuccess ||
+      hipMalloc(&M->m1, bytes) != hipSuccess || hipMalloc(&M->m2, bytes) != hipSuccess ||
+      hipMalloc(&M->part, partBytes) != hipSuccess || hipMalloc(&M->dscal, sizeof(double) * 4) != hipSuccess) {
+    delete M;
+    return BH_DEVICE_ERROR;
+  }
+  BH_CHECK(hipMemcpy(M->w, h.data(), bytes, hipMemcpyHostToDevice));
+  BH_CHECK(hipMemset(M->g, 0, bytes));
+  BH_CHECK(hipMemset(M->m1, 0, bytes));
+  BH_CHECK(hipMemset(M->m2, 0, bytes));
+  *out = M;
+  return BH_OK;
+}
+
+int bh_mgn_destroy(bh_mgn* M) {
+  if (!M) return BH_BAD_ARG;
+  (void)hipDeviceSynchronize();
+  void* ptrs[] = {M->w, M->g, M->m1, M->m2, M->part, M->dscal, M->ws, M->pv};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete M;
+  return BH_OK;
+}
+
+int bh_mgn_num_params(const bh_mgn* M, int64_t* n) {
+  if (!M || !n) return BH_BAD_ARG;
+  *n = M->nparam;
+  return BH_OK;
+}
+
+int bh_mgn_get_params(const bh_mgn* M, float* host_params, float* host_grad) {
+  if (!M) return BH_BAD_ARG;
+  BH_CHECK(hipDeviceSynchronize());
+  if (host_params) BH_CHECK(hipMemcpy(host_params, M->w, sizeof(float) * M->nparam, hipMemcpyDeviceToHost));
+  if (host_grad) BH_CHECK(hipMemcpy(host_grad, M->g, sizeof(float) * M->nparam, hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+int bh_mgn_set_params(bh_mgn* M, const float* host_params) {
+  if (!M || !host_params) return BH_BAD_ARG;
+  BH_CHECK(hipDeviceSynchronize());
+  BH_CHECK(hipMemcpy(M->w, host_params, sizeof(float) * M->nparam, hipMemcpyHostToDevice));
+  BH_CHECK(hipMemset(M->g, 0, sizeof(float) * M->nparam));
+  BH_CHECK(hipMemset(M->m1, 0, sizeof(float) * M->nparam));
+  BH_CHECK(hipMemset(M->m2, 0, sizeof(float) * M->nparam));
+  M->step = 0;
+  return BH_OK;
+}
+
+int bh_mgn_loss_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint8_t* d_fixed_dof, const double* d_b,
+                       const double* d_x, int backward, double* loss, bh_stream_t stream) {
+  if (!M || !P || !d_vals || !d_b || !d_x) return BH_BAD_ARG;
+  if (P->mesh->nV == 0) return BH_OK;
+  return mgn_loss<float>(M, P, d_vals, d_fixed_dof, d_b, d_x, backward, loss, as_stream(stream));
+}
+
+int bh_mgn_adam_step(bh_mgn* M, double lr, double beta1, double beta2, double eps, bh_stream_t stream) {
+  if (!M || !(lr > 0)) return BH_BAD_ARG;
+  M->step += 1;
+  const double c1 = 1.0 - std::pow(beta1, (double)M->step), c2 = 1.0 - std::pow(beta2, (double)M->step);
+  hipLaunchKernelGGL(k_mgn_adam, g256(M->nparam), dim3(256), 0, as_stream(stream), M->nparam, M->w, M->g, M->m1,
+                     M->m2, (float)lr, (float)beta1, (float)beta2, (float)eps, (float)c1, (float)c2);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+int bh_mgn_factor_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint8_t* d_fixed_dof,
+                         bh_stream_t stream) {
+  if (!M || !P || !d_vals) return BH_BAD_ARG;
+  if (P->mesh->nV == 0) return BH_OK;
+  return mgn_factor<float>(M, P, d_vals, d_fixed_dof, as_stream(stream));
+}
+
+int bh_mgn_debug_buffers(bh_mgn* M, const float** s, const float** C, const float** d, const float** O) {
+  if (!M || !M->ws) return BH_BAD_ARG;
+  MgnWs w;
+  mgn_ws_carve(M->ws, M->wsV, M->wsP, w);
+  if (s) *s = w.s;
+  if (C) *C = w.C;
+  if (d) *d = w.d;
+  if (O) *O = w.O;
+  return BH_OK;
+}
+
+}  // extern "C"

@@ -75,8 +75,9 @@ __global__ void __launch_bounds__(256) k_mask_gradient(int64_t n, const uint8_t*
 }
 
 // inverse of the 3x3 diagonal block of each vertex (fixed DOFs -> identity rows / columns)
+template <typename V>
 __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __restrict__ bptr,
-                                                      const int32_t* __restrict__ bcol, const double* __restrict__ vals,
+                                                      const int32_t* __restrict__ bcol, const V* __restrict__ vals,
                                                       const uint8_t* __restrict__ fixed, int point,
                                                       double* __restrict__ Dinv) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,11 +87,11 @@ __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __r
   while (r < nb && bcol[b0 + r] != v) ++r;
   double A[9];  // row-major block; a vertex that belongs to no tet gets the identity
   if (r < nb) {
-    const double* base = vals + 9ll * b0 + 3ll * r;
+    const V* base = vals + 9ll * b0 + 3ll * r;
 #pragma unroll
     for (int l = 0; l < 3; ++l)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) A[3 * l + k] = base[3ll * l * nb + k];
+      for (int k = 0; k < 3; ++k) A[3 * l + k] = (double)base[3ll * l * nb + k];
   } else {
 #pragma unroll
     for (int i = 0; i < 9; ++i) A[i] = (i % 4 == 0) ? 1.0 : 0.0;
@@ -183,29 +184,54 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_start(const double* __restric
   }
 }
 
-// Ap = mask(H p), partial p.Ap  (one lane per row)
+// Ap = mask(H p), partial p.Ap.  16 lanes per vertex: lane j takes blocks j, j+16, ... of the
+// vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
+// neighbouring 24-byte pieces), then the three row sums are reduced across the 16 lanes with a fixed
+// xor butterfly.  FP32 values are widened; products are summed in FP64.
+constexpr int kSpmvLanes = 16;
+inline int spmv_blocks(int nV) { return (int)((kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock); }
+
+template <typename V>
 __global__ void __launch_bounds__(kRedBlock)
     k_pcg_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
-               const double* __restrict__ vals, const uint8_t* __restrict__ fixed, const double* __restrict__ p,
+               const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const double* __restrict__ p,
                double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
   __shared__ double sh[4];
-  if (st->done) return;
-  const int row = blockIdx.x * kRedBlock + threadIdx.x;
-  double s = 0.0, pr = 0.0;
-  if (row < 3 * nV) {
-    const int v = row / 3, l = row - 3 * v;
+  if (st && st->done) return;
+  const int64_t gid = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
+  const int v = (int)(gid / kSpmvLanes), j = (int)(gid % kSpmvLanes);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (v < nV) {
     const int b0 = bptr[v], nb = bptr[v + 1] - b0;
-    const double* vr = vals + 9ll * b0 + 3ll * l * nb;
-    for (int r = 0; r < nb; ++r) {
+    const V* base = vals + 9ll * b0;
+    for (int r = j; r < nb; r += kSpmvLanes) {
       const double* xp = p + 3ll * bcol[b0 + r];
-      s += vr[3 * r] * xp[0] + vr[3 * r + 1] * xp[1] + vr[3 * r + 2] * xp[2];
+      const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+      const V* r0 = base + 3 * r;
+      const V* r1 = r0 + 3 * nb;
+      const V* r2 = r0 + 6 * nb;
+      s0 += (double)r0[0] * x0 + (double)r0[1] * x1 + (double)r0[2] * x2;
+      s1 += (double)r1[0] * x0 + (double)r1[1] * x1 + (double)r1[2] * x2;
+      s2 += (double)r2[0] * x0 + (double)r2[1] * x1 + (double)r2[2] * x2;
     }
-    pr = p[row];
-    if (fixed && fixed[row]) s = pr;
-    Ap[row] = s;
   }
-  const double d = block_sum_256(pr * s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = d;
+#pragma unroll
+  for (int off = kSpmvLanes / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double pap = 0.0;
+  if (v < nV && j < 3) {
+    const int64_t row = 3ll * v + j;
+    double sv = (j == 0) ? s0 : (j == 1) ? s1 : s2;
+    const double pr = p[row];
+    if (fixed && fixed[row]) sv = pr;
+    Ap[row] = sv;
+    pap = pr * sv;
+  }
+  const double d = block_sum_256(pap, sh);
+  if (part && threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
 __global__ void __launch_bounds__(kRedBlock) k_pcg_alpha(const double* __restrict__ part, int np, PcgState* st) {
@@ -387,15 +413,17 @@ __global__ void k_negate(int64_t n, const double* __restrict__ a, double* __rest
 
 struct NewtonWs {
   double *vals, *g, *b, *x, *r, *z, *p, *Ap, *qt, *Dinv, *part, *scal;
+  float* vals32;    // precision form: FP32 Hessian values
+  float* params32;  // precision form: FP32 parameters (2 nT)
   PcgState* st;
 };
 
 int newton_workspace(bh_mesh* m, NewtonWs& w) {
   const int64_t n = 3ll * m->nV;
   const int64_t nParts = 2ll * std::max<int64_t>(kRedMaxBlocks, (3ll * m->nV + kRedBlock - 1) / kRedBlock + 1) +
-                         kRedMaxBlocks + 16;
+                         kRedMaxBlocks + spmv_blocks(m->nV) + 16;
   const size_t need = sizeof(double) * ((size_t)m->nnz + 8 * (size_t)n + 9 * (size_t)m->nV + nParts + 16) +
-                      sizeof(PcgState) + 256;
+                      sizeof(float) * ((size_t)m->nnz + 2 * (size_t)m->nT) + sizeof(PcgState) + 512;
   if (m->wsBytes < need) {
     if (m->ws) BH_CHECK(hipFree(m->ws));
     m->ws = nullptr;
@@ -415,14 +443,19 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   w.Dinv = q; q += 9ll * m->nV;
   w.part = q; q += nParts;
   w.scal = q; q += 16;
-  w.st = reinterpret_cast<PcgState*>(q);
+  float* f = reinterpret_cast<float*>(q);
+  w.vals32 = f; f += m->nnz;
+  w.params32 = f; f += 2ll * m->nT;
+  w.st = reinterpret_cast<PcgState*>(reinterpret_cast<uintptr_t>(f + 64) & ~uintptr_t(63));
   return BH_OK;
 }
 
 // objective(q) = Psi(q) - f.q  -> *out (device)
 int objective(bh_mesh* m, const bh_newton_opts* o, const double* params, const double* f, const double* q, double* out,
               NewtonWs& w, hipStream_t s) {
-  int st = energy_dispatch(m, o->model, o->d_model_ids, q, params, out, s);
+  int st = (o->element_precision == 1)
+               ? energy_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, out, s)
+               : energy_dispatch(m, o->model, o->d_model_ids, q, params, out, s);
   if (st) return st;
   if (f) {
     st = device_dot(f, q, 3ll * m->nV, w.part, w.scal + 8, s);
@@ -440,13 +473,16 @@ int read_scalar(const double* d, double* h, hipStream_t s) {
 }
 
 // PCG solve of H x = b on the masked system; returns the iteration count in *iters, status in *done
-int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, NewtonWs& w, int* iters, int* done,
-              hipStream_t s) {
+template <typename V>
+int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const uint8_t* fixed, NewtonWs& w, int* iters,
+              int* done, hipStream_t s) {
   const int nV = m->nV;
   const int64_t n = 3ll * nV;
   const int gv = (nV + kRedBlock - 1) / kRedBlock;
   const int gr = (int)((n + kRedBlock - 1) / kRedBlock);
-  hipLaunchKernelGGL(k_block_jacobi, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, w.vals, fixed,
+  const int gs = spmv_blocks(nV);
+  (void)gr;
+  hipLaunchKernelGGL(k_block_jacobi<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, vals, fixed,
                      o->precond == BH_PRECOND_POINT_JACOBI ? 1 : 0, w.Dinv);
   hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kRedBlock), 0, s, nV, w.b, w.Dinv, w.x, w.r, w.z, w.p, w.part);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kRedBlock), 0, s, w.part, gv, o->pcg_tol, o->pcg_max_iterations,
@@ -455,9 +491,9 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, NewtonW
   PcgState hs{};
   for (int it = 0;; it += kPcgBatch) {
     for (int k = 0; k < kPcgBatch; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv, dim3(gr), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, w.vals, fixed, w.p, w.Ap,
+      hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, fixed, w.p, w.Ap,
                          w.part, w.st);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, w.st);
+      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gs, w.st);
       hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kRedBlock), 0, s, nV, w.Dinv, w.x, w.r, w.z, w.p, w.Ap, w.part,
                          w.st);
       hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(kRedBlock), 0, s, w.part, gv, w.st);
@@ -474,11 +510,13 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, NewtonW
 }
 
 // PCG with a colour-sweep block-triangular preconditioner (same device-resident scalars)
-int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, const bh_precond* P, NewtonWs& w,
-                     int* iters, int* done, hipStream_t s) {
+template <typename V>
+int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const uint8_t* fixed, const bh_precond* P,
+                     NewtonWs& w, int* iters, int* done, hipStream_t s) {
   const int nV = m->nV;
   const int64_t n = 3ll * nV;
   const int gr = (int)((n + kRedBlock - 1) / kRedBlock);
+  const int gs = spmv_blocks(nV);
   const int nz = red_blocks(n);
   double* part2 = w.part + gr + 8;  // second partial array (r.z), after the r.r / p.Ap partials
   int* dflag = &w.st->done;
@@ -493,9 +531,9 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, 
   PcgState hs{};
   for (;;) {
     for (int k = 0; k < kPcgBatch; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv, dim3(gr), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, w.vals, fixed, w.p, w.Ap,
+      hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, fixed, w.p, w.Ap,
                          w.part, w.st);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, w.st);
+      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gs, w.st);
       hipLaunchKernelGGL(k_pcg_update_xr, dim3(gr), dim3(kRedBlock), 0, s, n, w.x, w.r, w.p, w.Ap, w.part, w.st);
       if ((st = precond_apply(P, w.r, w.z, dflag, s))) return st;
       hipLaunchKernelGGL(k_dot_partials, dim3(nz), dim3(kRedBlock), 0, s, w.r, w.z, n, part2);
@@ -515,7 +553,52 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const uint8_t* fixed, 
 
 }  // namespace
 
+namespace {
+__global__ void __launch_bounds__(256) k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// one Newton direction: H(q) (PSD-fixed) and PCG on H d = -g, with the chosen preconditioner
+template <typename V>
+int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* vals, const uint8_t* fixed,
+                     NewtonWs& w, int* pits, int* pdone, int* fallback, hipStream_t s, const double* q) {
+  const bh_psd* psd = o->psd.enabled ? &o->psd : nullptr;
+  int st = hessian_dispatch(m, o->model, o->d_model_ids, q, params, psd, vals, s);
+  if (st) return st;
+  *fallback = 0;
+  if (o->precond == BH_PRECOND_BIC0 || o->precond == BH_PRECOND_FACTOR || o->precond == BH_PRECOND_LEARNED) {
+    if (!o->precond_handle) return BH_BAD_ARG;
+    if (o->precond == BH_PRECOND_BIC0 &&
+        (st = precond_factor_bic0(o->precond_handle, vals, fixed, o->precond_shift, nullptr, s)))
+      return st;
+    if (o->precond == BH_PRECOND_LEARNED) {
+      if (!o->mgn) return BH_BAD_ARG;
+      if ((st = mgn_factor<V>(o->mgn, o->precond_handle, vals, fixed, s))) return st;
+    }
+    if ((st = pcg_solve_factor(m, o, vals, fixed, o->precond_handle, w, pits, pdone, s))) return st;
+    if (*pdone == 2 && o->jacobi_fallback) {
+      int p2 = 0;
+      if ((st = pcg_solve(m, o, vals, fixed, w, &p2, pdone, s))) return st;
+      *pits += p2;
+      *fallback = 1;
+    }
+    return BH_OK;
+  }
+  return pcg_solve(m, o, vals, fixed, w, pits, pdone, s);
+}
+}  // namespace
+
 extern "C" {
+
+int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* d_y, bh_stream_t s) {
+  if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, as_stream(s), m->nV, m->bptr,
+                     m->bcol, d_vals, nullptr, d_x, d_y, nullptr, nullptr);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
 
 void bh_newton_default_opts(bh_newton_opts* o) {
   if (!o) return;
@@ -537,7 +620,14 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->precond = BH_PRECOND_JACOBI;
   o->precond_handle = nullptr;
   o->precond_shift = 0.0;
+  o->element_precision = 0;
+  o->jacobi_fallback = 1;
+  o->mgn = nullptr;
+  o->on_system = nullptr;
+  o->user = nullptr;
 }
+
+
 
 int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* params, const uint8_t* fixed,
                             const double* f, double* q, bh_newton_report* rep, double* trace, bh_stream_t stream) {
@@ -553,10 +643,16 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
   double g0 = -1.0;
   int it = 0;
   int status = BH_MAXIT;
-  const bh_psd* psd = o->psd.enabled ? &o->psd : nullptr;
+  const bool f32 = o->element_precision == 1;
+  if (f32) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((2ll * m->nT + 255) / 256), dim3(256), 0, s, 2ll * m->nT, params,
+                       w.params32);
+    BH_CHECK(hipGetLastError());
+  }
   do {
     // gradient of the objective, masked
-    st = gradient_dispatch(m, o->model, o->d_model_ids, q, params, w.g, s);
+    st = f32 ? gradient_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, w.g, s)
+             : gradient_dispatch(m, o->model, o->d_model_ids, q, params, w.g, s);
     if (st) return st;
     hipLaunchKernelGGL(k_mask_gradient, dim3(gn), dim3(256), 0, s, n, fixed, f, w.g);
     st = device_dot(w.g, w.g, n, w.part, w.scal, s);
@@ -583,19 +679,18 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
       break;
     }
     // Hessian (with the PSD callback) and direction d = -H^{-1} g
-    st = hessian_dispatch(m, o->model, o->d_model_ids, q, params, psd, w.vals, s);
-    if (st) return st;
     hipLaunchKernelGGL(k_negate, dim3(gn), dim3(256), 0, s, n, w.g, w.b);
-    int pits = 0, pdone = 0;
-    if (o->precond == BH_PRECOND_BIC0) {
-      if (!o->precond_handle) return BH_BAD_ARG;
-      if ((st = precond_factor_bic0(o->precond_handle, w.vals, fixed, o->precond_shift, nullptr, s))) return st;
-      if ((st = pcg_solve_factor(m, o, fixed, o->precond_handle, w, &pits, &pdone, s))) return st;
-    } else if ((st = pcg_solve(m, o, fixed, w, &pits, &pdone, s))) {
-      return st;
-    }
+    int pits = 0, pdone = 0, fb = 0;
+    st = f32 ? newton_direction<float>(m, o, w.params32, w.vals32, fixed, w, &pits, &pdone, &fb, s, q)
+             : newton_direction<double>(m, o, params, w.vals, fixed, w, &pits, &pdone, &fb, s, q);
+    if (st) return st;
     R.pcg_iterations_total += pits;
+    R.pcg_fallbacks += fb;
     if (pdone != 1) R.pcg_failures += 1;
+    if (o->on_system) {
+      BH_CHECK(hipStreamSynchronize(s));
+      o->on_system(o->user, it, f32 ? w.vals32 : nullptr, f32 ? nullptr : w.vals, w.b, w.x);
+    }
     // line search on the objective (reference semantics unless the options say otherwise)
     st = device_dot(w.g, w.x, n, w.part, w.scal + 2, s);
     if (st) return st;

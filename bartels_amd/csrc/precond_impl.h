@@ -25,6 +25,7 @@ struct bh_precond {
   int32_t* ulink = nullptr;     // index of the lower entry (n, v) in n's list
   int32_t* dslot = nullptr;     // block index of (v, v)
   float* lval = nullptr;        // 9 per lower entry, row-major
+  float* uval = nullptr;        // the same blocks in upper-list order (for the backward sweep)
   float* dinv = nullptr;        // 6 per vertex: inverse of L_vv (lower triangular, row-major packed)
   int32_t* fail = nullptr;      // factorisation breakdown counter
   int64_t nlower = 0;
@@ -46,55 +47,94 @@ __device__ __forceinline__ void ltri_tmv(const float* L, const double* x, double
   y[2] = (double)L[5] * x[2];
 }
 
+// Sweeps, 8 lanes per vertex: lane j takes list entries j, j+8, ... (a vertex's list is contiguous,
+// so the lanes read neighbouring 36-byte blocks), the three partial sums are reduced across the 8
+// lanes with a fixed xor butterfly, and lane 0 applies the diagonal inverse.
+constexpr int kSweepLanes = 8;
+
 // forward sweep over one colour: y_v = L_vv^{-1} (r_v - sum_n L_vn y_n)
 __global__ void __launch_bounds__(256)
     k_sweep_fwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ lptr,
                 const int32_t* __restrict__ lcol, const float* __restrict__ lval, const float* __restrict__ dinv,
                 const double* __restrict__ r, double* __restrict__ y, const int* __restrict__ done) {
   if (done && *done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int v = verts[i];
-  double s[3] = {r[3ll * v], r[3ll * v + 1], r[3ll * v + 2]};
-  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
-    const float* B = lval + 9ll * k;
-    const double* yn = y + 3ll * lcol[k];
-    const double y0 = yn[0], y1 = yn[1], y2 = yn[2];
-    s[0] -= (double)B[0] * y0 + (double)B[1] * y1 + (double)B[2] * y2;
-    s[1] -= (double)B[3] * y0 + (double)B[4] * y1 + (double)B[5] * y2;
-    s[2] -= (double)B[6] * y0 + (double)B[7] * y1 + (double)B[8] * y2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kSweepLanes, j = gid % kSweepLanes;
+  int v = 0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (i < n) {
+    v = verts[i];
+    for (int k = lptr[v] + j; k < lptr[v + 1]; k += kSweepLanes) {
+      const float* B = lval + 9ll * k;
+      const double* yn = y + 3ll * lcol[k];
+      const double y0 = yn[0], y1 = yn[1], y2 = yn[2];
+      s0 += (double)B[0] * y0 + (double)B[1] * y1 + (double)B[2] * y2;
+      s1 += (double)B[3] * y0 + (double)B[4] * y1 + (double)B[5] * y2;
+      s2 += (double)B[6] * y0 + (double)B[7] * y1 + (double)B[8] * y2;
+    }
   }
-  double o[3];
-  ltri_mv(dinv + 6ll * v, s, o);
-  y[3ll * v] = o[0];
-  y[3ll * v + 1] = o[1];
-  y[3ll * v + 2] = o[2];
+#pragma unroll
+  for (int off = kSweepLanes / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  if (i < n && j == 0) {
+    const double t[3] = {r[3ll * v] - s0, r[3ll * v + 1] - s1, r[3ll * v + 2] - s2};
+    double o[3];
+    ltri_mv(dinv + 6ll * v, t, o);
+    y[3ll * v] = o[0];
+    y[3ll * v + 1] = o[1];
+    y[3ll * v + 2] = o[2];
+  }
 }
 
-// backward sweep over one colour: z_v = L_vv^{-T} (y_v - sum_{n: colour(n) > colour(v)} L_nv^T z_n)
+// backward sweep over one colour: z_v = L_vv^{-T} (y_v - sum_{n: colour(n) > colour(v)} L_nv^T z_n), with
+// L_nv read from uval (the lower blocks copied into the upper-list order)
 __global__ void __launch_bounds__(256)
     k_sweep_bwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ uptr,
-                const int32_t* __restrict__ ucol, const int32_t* __restrict__ ulink, const float* __restrict__ lval,
-                const float* __restrict__ dinv, const double* __restrict__ y, double* __restrict__ z,
-                const int* __restrict__ done) {
+                const int32_t* __restrict__ ucol, const float* __restrict__ uval, const float* __restrict__ dinv,
+                const double* __restrict__ y, double* __restrict__ z, const int* __restrict__ done) {
   if (done && *done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int v = verts[i];
-  double s[3] = {y[3ll * v], y[3ll * v + 1], y[3ll * v + 2]};
-  for (int k = uptr[v]; k < uptr[v + 1]; ++k) {
-    const float* B = lval + 9ll * ulink[k];  // L_nv, use its transpose
-    const double* zn = z + 3ll * ucol[k];
-    const double z0 = zn[0], z1 = zn[1], z2 = zn[2];
-    s[0] -= (double)B[0] * z0 + (double)B[3] * z1 + (double)B[6] * z2;
-    s[1] -= (double)B[1] * z0 + (double)B[4] * z1 + (double)B[7] * z2;
-    s[2] -= (double)B[2] * z0 + (double)B[5] * z1 + (double)B[8] * z2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kSweepLanes, j = gid % kSweepLanes;
+  int v = 0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (i < n) {
+    v = verts[i];
+    for (int k = uptr[v] + j; k < uptr[v + 1]; k += kSweepLanes) {
+      const float* B = uval + 9ll * k;  // L_nv, use its transpose
+      const double* zn = z + 3ll * ucol[k];
+      const double z0 = zn[0], z1 = zn[1], z2 = zn[2];
+      s0 += (double)B[0] * z0 + (double)B[3] * z1 + (double)B[6] * z2;
+      s1 += (double)B[1] * z0 + (double)B[4] * z1 + (double)B[7] * z2;
+      s2 += (double)B[2] * z0 + (double)B[5] * z1 + (double)B[8] * z2;
+    }
   }
-  double o[3];
-  ltri_tmv(dinv + 6ll * v, s, o);
-  z[3ll * v] = o[0];
-  z[3ll * v + 1] = o[1];
-  z[3ll * v + 2] = o[2];
+#pragma unroll
+  for (int off = kSweepLanes / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  if (i < n && j == 0) {
+    const double t[3] = {y[3ll * v] - s0, y[3ll * v + 1] - s1, y[3ll * v + 2] - s2};
+    double o[3];
+    ltri_tmv(dinv + 6ll * v, t, o);
+    z[3ll * v] = o[0];
+    z[3ll * v + 1] = o[1];
+    z[3ll * v + 2] = o[2];
+  }
+}
+
+// uval[t] = lval[ulink[t]]: the factor's lower blocks in the order of the upper lists
+__global__ void __launch_bounds__(256) k_fill_uval(int64_t nu, const int32_t* __restrict__ ulink,
+                                                   const float* __restrict__ lval, float* __restrict__ uval) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nu) return;
+  const float* src = lval + 9ll * ulink[t];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) uval[9 * t + i] = src[i];
 }
 
 // ---- block IC(0), one colour at a time ----------------------------------------------------------------
@@ -282,6 +322,7 @@ int precond_build(bh_mesh* m, bh_precond* P) {
       (st = up(&P->ulink, ulink)) || (st = up(&P->dslot, dslot)))
     return st;
   BH_CHECK(hipMalloc(&P->lval, sizeof(float) * 9 * std::max<int64_t>(1, P->nlower)));
+  BH_CHECK(hipMalloc(&P->uval, sizeof(float) * 9 * std::max<int64_t>(1, P->nlower)));
   BH_CHECK(hipMalloc(&P->dinv, sizeof(float) * 6 * std::max(1, nV)));
   BH_CHECK(hipMalloc(&P->fail, sizeof(int32_t)));
   BH_CHECK(hipMalloc(&P->ytmp, sizeof(double) * 3 * std::max(1, nV)));
@@ -290,9 +331,17 @@ int precond_build(bh_mesh* m, bh_precond* P) {
 
 void precond_free(bh_precond* P) {
   void* ptrs[] = {P->colVerts, P->lptr, P->lcol, P->lslot, P->uptr, P->ucol, P->ulink, P->dslot,
-                  P->lval, P->dinv, P->fail, P->ytmp};
+                  P->lval, P->uval, P->dinv, P->fail, P->ytmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+}
+
+// after every change of lval: the upper-list copy read by the backward sweep
+int precond_finalize(bh_precond* P, hipStream_t s) {
+  if (P->nlower == 0) return BH_OK;
+  hipLaunchKernelGGL(k_fill_uval, dim3((P->nlower + 255) / 256), dim3(256), 0, s, P->nlower, P->ulink, P->lval, P->uval);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
 }
 
 // z = (L L^T)^{-1} r
@@ -301,18 +350,20 @@ int precond_apply(const bh_precond* P, const double* r, double* z, const int* do
   for (int c = 0; c < nc; ++c) {
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
-    hipLaunchKernelGGL(k_sweep_fwd, dim3((n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c], n, P->lptr,
-                       P->lcol, P->lval, P->dinv, r, P->ytmp, done);
+    hipLaunchKernelGGL(k_sweep_fwd, dim3((kSweepLanes * n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c],
+                       n, P->lptr, P->lcol, P->lval, P->dinv, r, P->ytmp, done);
   }
   for (int c = nc - 1; c >= 0; --c) {
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
-    hipLaunchKernelGGL(k_sweep_bwd, dim3((n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c], n, P->uptr,
-                       P->ucol, P->ulink, P->lval, P->dinv, P->ytmp, z, done);
+    hipLaunchKernelGGL(k_sweep_bwd, dim3((kSweepLanes * n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c],
+                       n, P->uptr, P->ucol, P->uval, P->dinv, P->ytmp, z, done);
   }
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
+
+int precond_finalize(bh_precond* P, hipStream_t s);
 
 template <typename V>
 int precond_factor_bic0(bh_precond* P, const V* vals, const uint8_t* fixed, double shift, int* failures,
@@ -326,6 +377,7 @@ int precond_factor_bic0(bh_precond* P, const V* vals, const uint8_t* fixed, doub
                        m->bptr, vals, fixed, P->lptr, P->lcol, P->lslot, P->dslot, shift, P->lval, P->dinv, P->fail);
   }
   BH_CHECK(hipGetLastError());
+  if (int st = precond_finalize(P, s)) return st;
   if (failures) {
     int32_t h = 0;
     BH_CHECK(hipMemcpyAsync(&h, P->fail, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -381,7 +433,7 @@ int bh_precond_set_factor_device(bh_precond* P, const float* d_lower, const floa
   if (P->nlower)
     BH_CHECK(hipMemcpyAsync(P->lval, d_lower, sizeof(float) * 9 * P->nlower, hipMemcpyDeviceToDevice, hs));
   BH_CHECK(hipMemcpyAsync(P->dinv, d_diag_inv, sizeof(float) * 6 * (size_t)P->mesh->nV, hipMemcpyDeviceToDevice, hs));
-  return BH_OK;
+  return precond_finalize(P, hs);
 }
 
 int bh_precond_lists_device(const bh_precond* P, const int32_t** lptr, const int32_t** lcol, const int32_t** lslot,

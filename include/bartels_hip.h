@@ -94,15 +94,15 @@ int bh_mesh_pattern(const bh_mesh* mesh, int32_t* rowptr, int32_t* colind);
 int bh_mesh_block_pattern_device(const bh_mesh* mesh, const int32_t** d_bptr, const int32_t** d_bcol,
                                  int64_t* nblocks);
 int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
-/* Assembly-kernel shape: 1, 2 or 4 lanes per vertex (with 2 or 4, each lane owns two or one of the
- * four corner blocks of every incident tet).  Same per-slot fold order. */
+/* Assembly-kernel shape: 1 or 2 lanes per vertex (with 2, each lane owns two of the four corner
+ * blocks of every incident tet).  Same per-slot fold order. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
 
 /* Kernel-schedule knobs of a mesh (no reference counterpart: the reference has one CPU schedule).
  * Every setting keeps each slot's ascending-element fold, so results agree to the last bits; a
  * fixed setting is bitwise reproducible. */
 typedef struct bh_tuning {
-  int gather_lanes; /* 0 keep, else 1, 2 or 4 lanes per vertex in the assembly gather */
+  int gather_lanes; /* 0 keep, else 1 or 2 lanes per vertex in the assembly gather */
   int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
 } bh_tuning;
 int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
@@ -170,7 +170,8 @@ typedef enum {
   BH_PRECOND_JACOBI = 0,       /* 3x3 block-Jacobi (block diagonal of H) */
   BH_PRECOND_POINT_JACOBI = 1,
   BH_PRECOND_BIC0 = 2,         /* block IC(0) in colour order, refactored from H every Newton iteration */
-  BH_PRECOND_FACTOR = 3        /* a block lower-triangular factor supplied through precond_handle */
+  BH_PRECOND_FACTOR = 3,       /* a block lower-triangular factor supplied through precond_handle */
+  BH_PRECOND_LEARNED = 4       /* factor written by the mesh-graph network opts.mgn from each Hessian */
 } bh_precond_kind;
 
 /* Block lower-triangular preconditioner M = L L^T on the Hessian's own 3x3-block pattern, in a
@@ -194,6 +195,34 @@ int bh_precond_lists_device(const bh_precond* precond, const int32_t** lptr, con
 /* z = (L L^T)^{-1} r */
 int bh_precond_apply_device(bh_precond* precond, const double* d_r, double* d_z, bh_stream_t stream);
 
+/* ---- learned preconditioner (no counterpart in the reference) -----------------------------------
+ * A mesh-graph network on the vertices and lower pairs of a bh_precond: node / pair features from
+ * the masked, diagonally normalised Hessian, 3 message-passing layers of width 32, decoders that
+ * write the factor L (L L^T approximates H; at initialisation the decoders are zero and L L^T is
+ * the block diagonal of H).  Parameters are FP32 (bh_mgn_num_params floats, layout fixed by the
+ * library).  Training minimises |L L^T x - b|^2 / |b|^2 in the diagonally scaled norm on Newton
+ * systems H x = b solved by the plain solver; gradients accumulate until bh_mgn_adam_step. */
+typedef struct bh_mgn bh_mgn;
+int bh_mgn_create(uint64_t seed, bh_mgn** mgn);
+int bh_mgn_destroy(bh_mgn* mgn);
+int bh_mgn_num_params(const bh_mgn* mgn, int64_t* nparams);
+/* host copies of the parameters and of the accumulated gradient (either may be NULL) */
+int bh_mgn_get_params(const bh_mgn* mgn, float* params, float* grad);
+/* sets the parameters and clears the gradient and the Adam state */
+int bh_mgn_set_params(bh_mgn* mgn, const float* params);
+/* loss of one system (d_vals: FP32 Hessian values in the mesh pattern, d_b / d_x: 3 nV, fixed DOFs
+ * zero); backward != 0 also accumulates the gradient.  loss (host, optional) synchronises. */
+int bh_mgn_loss_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, const uint8_t* d_fixed_dof,
+                       const double* d_b, const double* d_x, int backward, double* loss, bh_stream_t stream);
+/* one Adam update from the accumulated gradient (which it then clears) */
+int bh_mgn_adam_step(bh_mgn* mgn, double lr, double beta1, double beta2, double eps, bh_stream_t stream);
+/* inference: writes the factor of the given Hessian into the preconditioner */
+int bh_mgn_factor_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, const uint8_t* d_fixed_dof,
+                         bh_stream_t stream);
+/* device views of the last forward pass: scales s (nV), chol of the normalised diagonal blocks
+ * (6 per vertex, packed lower), decoder outputs d (6 per vertex) and O (9 per lower pair) */
+int bh_mgn_debug_buffers(bh_mgn* mgn, const float** s, const float** C, const float** d, const float** O);
+
 typedef struct {
   int model;               /* bh_model, or BH_MODEL_PER_TET with model_ids */
   const uint8_t* d_model_ids;
@@ -212,6 +241,21 @@ typedef struct {
   int precond;             /* bh_precond_kind */
   bh_precond* precond_handle; /* BH_PRECOND_BIC0 / BH_PRECOND_FACTOR */
   double precond_shift;    /* BH_PRECOND_BIC0 diagonal shift */
+  /* 0: FP64 element path.  1: the precision form -- element energy, gradient and Hessian in FP32
+   * (J = det F and log J in FP64), FP32 Hessian values; the energy sum of the line search, the
+   * gradient accumulation, SpMV sums, PCG vectors and dot products and the convergence test stay
+   * FP64.  params stay FP64 at the interface (converted once per solve). */
+  int element_precision;
+  /* 1: when a factor-preconditioned PCG (BIC0 / FACTOR) hits pcg_max_iterations, solve again with
+   * block-Jacobi PCG from zero (the fallback), counted in bh_newton_report.pcg_fallbacks */
+  int jacobi_fallback;
+  bh_mgn* mgn;             /* BH_PRECOND_LEARNED (precond_handle supplies the colour lists) */
+  /* optional, called after every direction solve with device pointers: the Hessian values (FP32 in
+   * the precision form, else FP64; the other pointer is NULL), the masked right-hand side b = -g
+   * and the PCG solution x (3 nV each); how the plain solver's systems reach a trainer */
+  void (*on_system)(void* user, int iteration, const float* d_vals32, const double* d_vals64, const double* d_b,
+                    const double* d_x);
+  void* user;
 } bh_newton_opts;
 
 typedef struct {
@@ -219,6 +263,7 @@ typedef struct {
   int converged;           /* 1 = SolverExitStatus::CONVERGED */
   int pcg_iterations_total;
   int pcg_failures;        /* inner solves that hit their cap */
+  int pcg_fallbacks;       /* factor-preconditioned solves redone with block Jacobi */
   int ls_iterations_total;
   double energy;           /* final objective */
   double gradient_norm;    /* final |g| */
