@@ -66,6 +66,9 @@ class BhNewtonOpts(ctypes.Structure):
         ("mgn", c_void_p),
         ("on_system", ON_SYSTEM),
         ("user", c_void_p),
+        ("d_mass_vals", c_void_p),
+        ("d_q_pred", c_void_p),
+        ("inertia", c_double),
     ]
 
 
@@ -134,6 +137,7 @@ SIGNATURES = {
     "bh_mgn_set_params": (c_int, [_VP, _VP]),
     "bh_mgn_loss_device": (c_int, [_VP, _VP, _VP, _VP, _VP, _VP, c_int, P(c_double), _VP]),
     "bh_mgn_adam_step": (c_int, [_VP, c_double, c_double, c_double, c_double, _VP]),
+    "bh_mgn_train_step_device": (c_int, [_VP, _VP, _VP, _VP, _VP, c_int, ctypes.c_uint64, c_double, P(c_double), _VP]),
     "bh_mgn_factor_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mgn_debug_buffers": (c_int, [_VP, P(_VP), P(_VP), P(_VP), P(_VP)]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
@@ -154,6 +158,12 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m bartels_amd.build` (there is no CPU fallback)")
+    # torch ships its own HIP runtime; importing it first lets the library bind to the runtime torch
+    # uses (loaded the other way round, torch finds no GPU in the same process)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

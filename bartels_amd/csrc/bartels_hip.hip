@@ -1372,6 +1372,7 @@ int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
 }  // extern "C"
 
 // block lower-triangular preconditioners (colour sweeps), then the device Newton
+#include "linalg_impl.h"
 #include "precond_impl.h"
 #include "mgn_impl.h"
 #include "newton_impl.h"

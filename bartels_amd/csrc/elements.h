@@ -558,6 +558,16 @@ __host__ __device__ __forceinline__ void phi_qr(const T phi[4][3], T Q[4][3], T 
   R[2][0] = T(0); R[2][1] = T(0); R[2][2] = r22;
 }
 
+// 1/sqrt(x): the hardware reciprocal square root with its library refinement on the device
+template <typename T>
+__host__ __device__ __forceinline__ T inv_sqrt(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrt(x);
+#else
+  return T(1) / std::sqrt(x);
+#endif
+}
+
 // packed upper-triangular index of a symmetric 9x9
 __host__ __device__ __forceinline__ int sym9(int i, int j) {
   const int a = i < j ? i : j, b = i < j ? j : i;
@@ -698,10 +708,11 @@ __host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
       for (int i = 7; i >= l; --i) {
         const bool act = i < m;
         const T f = s * e[i], b = c * e[i];
-        const T r = sqrt(f * f + g * g);
-        const T inv = r > T(0) ? T(1) / r : T(0);
-        const T sn = r > T(0) ? f * inv : T(0);
-        const T cn = r > T(0) ? g * inv : T(1);
+        const T r2 = f * f + g * g;
+        const T inv = r2 > T(0) ? inv_sqrt(r2) : T(0);  // one reciprocal square root per rotation
+        const T r = r2 * inv;
+        const T sn = r2 > T(0) ? f * inv : T(0);
+        const T cn = r2 > T(0) ? g * inv : T(1);
         const T gi = d[i + 1] - pacc;
         const T ri = (d[i] - gi) * sn + T(2) * cn * b;
         const T pi = sn * ri;

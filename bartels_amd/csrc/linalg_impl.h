@@ -1,0 +1,77 @@
+// Shared device linear-algebra pieces (included at the end of bartels_hip.hip, before the
+// preconditioner, network and Newton parts; same translation unit): the CG scalar block, the
+// fixed-shape block reduction and the 16-lane block-pattern SpMV.
+namespace {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxBlocks = 1024;
+constexpr int kPcgBatch = 16;
+
+struct PcgState {
+  double rz, pAp, alpha, beta, rr, bb, tol2;
+  int done;  // 0 running, 1 converged, 2 iteration cap, 3 non-positive curvature
+  int iters, maxit;
+};
+
+__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+// Ap = mask(H p), partial p.Ap.  16 lanes per vertex: lane j takes blocks j, j+16, ... of the
+// vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
+// neighbouring 24-byte pieces), then the three row sums are reduced across the 16 lanes with a fixed
+// xor butterfly.  FP32 values are widened; products are summed in FP64.
+constexpr int kSpmvLanes = 16;
+inline int spmv_blocks(int nV) { return (int)((kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock); }
+
+template <typename V>
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+               const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const double* __restrict__ p,
+               double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && st->done) return;
+  const int64_t gid = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
+  const int v = (int)(gid / kSpmvLanes), j = (int)(gid % kSpmvLanes);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (v < nV) {
+    const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+    const V* base = vals + 9ll * b0;
+    for (int r = j; r < nb; r += kSpmvLanes) {
+      const double* xp = p + 3ll * bcol[b0 + r];
+      const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+      const V* r0 = base + 3 * r;
+      const V* r1 = r0 + 3 * nb;
+      const V* r2 = r0 + 6 * nb;
+      s0 += (double)r0[0] * x0 + (double)r0[1] * x1 + (double)r0[2] * x2;
+      s1 += (double)r1[0] * x0 + (double)r1[1] * x1 + (double)r1[2] * x2;
+      s2 += (double)r2[0] * x0 + (double)r2[1] * x1 + (double)r2[2] * x2;
+    }
+  }
+#pragma unroll
+  for (int off = kSpmvLanes / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double pap = 0.0;
+  if (v < nV && j < 3) {
+    const int64_t row = 3ll * v + j;
+    double sv = (j == 0) ? s0 : (j == 1) ? s1 : s2;
+    const double pr = p[row];
+    if (fixed && fixed[row]) sv = pr;
+    Ap[row] = sv;
+    pap = pr * sv;
+  }
+  const double d = block_sum_256(pap, sh);
+  if (part && threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
+}  // namespace

@@ -16,8 +16,11 @@
 // Factor (normalised): Lhat_vv = C_v T_v with T_v lower triangular, T_ii = exp(d), strict lower = d;
 //   Lhat_vn = O_k.  At zero decoders Lhat Lhat^T = blockdiag(Hn_vv): block Jacobi.  The factor of H
 //   is L = S Lhat (L_vv = s_v Lhat_vv, L_vn = s_v Lhat_vn).
-// Training loss on a Newton system H x = b (x from the plain solver):  with xh = S x, bh = S^{-1} b,
-//   loss = |Lhat Lhat^T xh - bh|^2 / |bh|^2.
+// Loss of a system H x = b:  with xh = S x, bh = S^{-1} b,  |Lhat Lhat^T xh - bh|^2 / |bh|^2.
+// Training uses probe systems xh = w, bh = Hn w (w the normalised right-hand side of the plain
+// solver's Newton system and standard normal vectors): a Hutchinson estimate of the relative
+// Frobenius error |Lhat Lhat^T - Hn|_F^2 / |Hn|_F^2, whose scale does not depend on the condition
+// number (the loss on the solver's own solution x grows like cond(H)^2 and does not train).
 // Everything is FP32; every reduction has a fixed order (results are deterministic).
 
 struct bh_mgn {
@@ -36,6 +39,8 @@ struct bh_mgn {
   float* part = nullptr;    // weight-gradient partials
   int32_t* pv = nullptr;    // pair -> higher vertex (owner row)
   int64_t pvSize = 0;
+  double* probe = nullptr;  // training probes: x_j, b_j
+  size_t probeBytes = 0;
 };
 
 namespace {
@@ -493,9 +498,13 @@ __global__ void __launch_bounds__(256) k_linear_bwd(int64_t rows, const float* _
   }
 }
 
+// diagonal of T: exp(d), with d capped at 20 (the derivative is zero past the cap)
+__device__ __forceinline__ float texp(float d) { return expf(fminf(d, 20.f)); }
+__device__ __forceinline__ float dtexp(float d) { return d < 20.f ? expf(d) : 0.f; }
+
 // Lhat_vv = C_v T(d_v), row-major 3x3 (lower triangular)
 __device__ __forceinline__ void mgn_lvv(const float* C, const float* d, float* L) {
-  const float t00 = expf(d[0]), t10 = d[1], t11 = expf(d[2]), t20 = d[3], t21 = d[4], t22 = expf(d[5]);
+  const float t00 = texp(d[0]), t10 = d[1], t11 = texp(d[2]), t20 = d[3], t21 = d[4], t22 = texp(d[5]);
   const float c00 = C[0], c10 = C[1], c11 = C[2], c20 = C[3], c21 = C[4], c22 = C[5];
   L[0] = c00 * t00;
   L[1] = 0.f;
@@ -598,12 +607,12 @@ __global__ void __launch_bounds__(256) k_mgn_loss_sum(int nb, const double* __re
   }
 }
 
-// gz = 2 r / |bh|^2
+// gz = 2 r / (|bh|^2 ns): the gradient of the mean over ns systems of |r|^2 / |bh|^2
 __global__ void __launch_bounds__(256) k_mgn_gz(int64_t n, const float* __restrict__ r, const double* __restrict__ sc,
-                                                float* __restrict__ gz) {
+                                                int ns, float* __restrict__ gz) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  gz[i] = (float)(2.0 / sc[1]) * r[i];
+  gz[i] = (float)(2.0 / (sc[1] * ns)) * r[i];
 }
 
 // gO_k = gz_v y_n^T + xh_v gy_n^T ; gd_v from gLhat_vv = lower(gz_v y_v^T + xh_v gy_v^T), gT = C^T gLhat
@@ -611,7 +620,7 @@ __global__ void __launch_bounds__(256)
     k_mgn_gfactor(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
                   const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ s,
                   const double* __restrict__ x, const float* __restrict__ y, const float* __restrict__ gz,
-                  const float* __restrict__ gy, float* __restrict__ gO, float* __restrict__ gd) {
+                  const float* __restrict__ gy, int accumulate, float* __restrict__ gO, float* __restrict__ gd) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nV) return;
   const float sv = s[v];
@@ -623,7 +632,10 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int l = 0; l < 3; ++l)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) o[3 * l + c] = gzv[l] * y[3ll * n + c] + xh[l] * gy[3ll * n + c];
+      for (int c = 0; c < 3; ++c) {
+        const float a = gzv[l] * y[3ll * n + c] + xh[l] * gy[3ll * n + c];
+        o[3 * l + c] = accumulate ? o[3 * l + c] + a : a;
+      }
   }
   float G[9];
 #pragma unroll
@@ -641,27 +653,74 @@ __global__ void __launch_bounds__(256)
     for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * G[j] + Cm[3 + i] * G[3 + j] + Cm[6 + i] * G[6 + j];
   const float* dv = d + 6ll * v;
   float* g = gd + 6ll * v;
-  g[0] = gT[0] * expf(dv[0]);
-  g[1] = gT[3];
-  g[2] = gT[4] * expf(dv[2]);
-  g[3] = gT[6];
-  g[4] = gT[7];
-  g[5] = gT[8] * expf(dv[5]);
+  const float a0 = gT[0] * dtexp(dv[0]), a1 = gT[3], a2 = gT[4] * dtexp(dv[2]), a3 = gT[6], a4 = gT[7],
+              a5 = gT[8] * dtexp(dv[5]);
+  if (accumulate) {
+    g[0] += a0; g[1] += a1; g[2] += a2; g[3] += a3; g[4] += a4; g[5] += a5;
+  } else {
+    g[0] = a0; g[1] = a1; g[2] = a2; g[3] = a3; g[4] = a4; g[5] = a5;
+  }
 }
 
-// the factor of H for bh_precond: L_vn = s_v O_k, inverse of L_vv = s_v Lhat_vv (packed lower)
+// Probe systems for the Frobenius-type loss: in the normalised space a probe w gives xh = w and
+// bh = Hn w, i.e. x = S^-1 w and b = H x.  w is the normalised right-hand side S^-1 rhs (probe 0
+// when a right-hand side is given) or a standard normal vector from a counter-based hash
+// (splitmix64 + Box-Muller) of (seed, probe, DOF); fixed DOFs are zero.
+__host__ __device__ __forceinline__ uint64_t mgn_mix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 __global__ void __launch_bounds__(256)
-    k_mgn_factor_out(int nV, const int32_t* __restrict__ lptr, const float* __restrict__ C,
-                     const float* __restrict__ d, const float* __restrict__ O, const float* __restrict__ s,
-                     float* __restrict__ lval, float* __restrict__ dinv) {
+    k_mgn_probe_x(int nV, const float* __restrict__ s, const uint8_t* __restrict__ fixed,
+                  const double* __restrict__ rhs, uint64_t seed, double* __restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= 3ll * nV) return;
+  const int v = (int)(i / 3);
+  double w;
+  if (rhs) {
+    w = rhs[i] / (double)s[v];
+  } else {
+    const uint64_t a = mgn_mix(seed ^ mgn_mix((uint64_t)(2 * i))), b = mgn_mix(seed ^ mgn_mix((uint64_t)(2 * i + 1)));
+    const double u1 = ((double)(a >> 11) + 1.0) * (1.0 / 9007199254740993.0), u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+    w = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  x[i] = (fixed && fixed[i]) ? 0.0 : w / (double)s[v];
+}
+
+// The factor of H for bh_precond: L_vn = s_v O_k, inverse of L_vv = s_v Lhat_vv (packed lower).
+// Fixed DOFs get an identity row and column (zero off-diagonal entries, unit diagonal), so the
+// preconditioner maps vectors that vanish on the fixed DOFs to vectors that vanish there too, as
+// the masked system (identity rows / columns) requires.
+__global__ void __launch_bounds__(256)
+    k_mgn_factor_out(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                     const uint8_t* __restrict__ fixed, const float* __restrict__ C, const float* __restrict__ d,
+                     const float* __restrict__ O, const float* __restrict__ s, float* __restrict__ lval,
+                     float* __restrict__ dinv) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nV) return;
   const float sv = s[v];
-  for (int k = lptr[v]; k < lptr[v + 1]; ++k)
+  bool fv[3] = {false, false, false};
+  if (fixed)
 #pragma unroll
-    for (int i = 0; i < 9; ++i) lval[9ll * k + i] = sv * O[9ll * k + i];
+    for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+    const int n = lcol[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const bool fz = fixed && (fv[i / 3] || fixed[3ll * n + i % 3]);
+      lval[9ll * k + i] = fz ? 0.f : sv * O[9ll * k + i];
+    }
+  }
   float L[9];
   mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+  // identity rows / columns of the fixed components (C is already the identity there)
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (fv[l] || fv[c]) L[3 * l + c] = (l == c) ? 1.f / sv : 0.f;
   const double l00 = (double)sv * L[0], l10 = (double)sv * L[3], l11 = (double)sv * L[4];
   const double l20 = (double)sv * L[6], l21 = (double)sv * L[7], l22 = (double)sv * L[8];
   const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
@@ -758,10 +817,14 @@ int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fi
   return BH_OK;
 }
 
-// loss (and, with backward, the gradient accumulated into M->g) of one Newton system H x = b
+// Mean loss over ns systems (b_j, x_j) and, with backward, its gradient accumulated into M->g:
+// one forward, the factor-side gradients of every system summed, one backward through the network.
+constexpr int kMaxSystems = 16;
+
 template <typename V>
-int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, const double* b, const double* x,
-             int backward, double* loss, hipStream_t s) {
+int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, int ns, const double* const* bs,
+             const double* const* xs, int backward, double* loss, hipStream_t s) {
+  if (ns < 1 || ns > kMaxSystems) return BH_BAD_ARG;
   int st = mgn_prepare(M, P);
   if (st) return st;
   MgnWs w;
@@ -771,14 +834,23 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
   const int64_t nP = P->nlower;
   if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
   const MgnLayout L = mgn_layout();
-  // y = Lhat^T xh ; r = Lhat y - bh
-  hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, x, 1,
-                     (const float*)nullptr, w.y);
   const int nb = (int)((nV + 255) / 256);
   double* part = reinterpret_cast<double*>(M->part);
-  hipLaunchKernelGGL(k_mgn_resid, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.O, w.s, w.y, b, w.r,
-                     part);
-  hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal);
+  for (int j = 0; j < ns; ++j) {
+    // y = Lhat^T xh ; r = Lhat y - bh
+    hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, xs[j],
+                       1, (const float*)nullptr, w.y);
+    hipLaunchKernelGGL(k_mgn_resid, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.O, w.s, w.y, bs[j],
+                       w.r, part);
+    hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
+    if (backward) {
+      hipLaunchKernelGGL(k_mgn_gz, g256(3ll * nV), dim3(256), 0, s, 3ll * nV, w.r, M->dscal + 2 * j, ns, w.gz);
+      hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s,
+                         (const double*)nullptr, 0, w.gz, w.gy);
+      hipLaunchKernelGGL(k_mgn_gfactor, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.s, xs[j], w.y,
+                         w.gz, w.gy, j > 0 ? 1 : 0, w.gO, w.gd);
+    }
+  }
   BH_CHECK(hipGetLastError());
   if (backward) {
     float* G = M->g;
@@ -798,12 +870,6 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     using I96 = std::integral_constant<int, 3 * kW>;
     auto plain = [](const float* p, int) { return Seg3{{p, nullptr, nullptr}, {nullptr, nullptr, nullptr}}; };
     const float* W = M->w;
-    // factor -> gd, gO
-    hipLaunchKernelGGL(k_mgn_gz, g256(3ll * nV), dim3(256), 0, s, 3ll * nV, w.r, M->dscal, w.gz);
-    hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s,
-                       (const double*)nullptr, 0, w.gz, w.gy);
-    hipLaunchKernelGGL(k_mgn_gfactor, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.s, x, w.y, w.gz,
-                       w.gy, w.gO, w.gd);
     // decoders
     outer(I32{}, I6{}, nV, plain(w.h[kLayers], kW), w.gd, G + L.Wdv, G + L.bdv);
     outer(I32{}, I9{}, nP, plain(w.e[kLayers], kW), w.gO, G + L.Wdo, G + L.bdo);
@@ -847,11 +913,60 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     BH_CHECK(hipGetLastError());
   }
   if (loss) {
-    double h[2];
-    BH_CHECK(hipMemcpyAsync(h, M->dscal, sizeof(h), hipMemcpyDeviceToHost, s));
+    double h[2 * kMaxSystems];
+    BH_CHECK(hipMemcpyAsync(h, M->dscal, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, s));
     BH_CHECK(hipStreamSynchronize(s));
-    *loss = h[1] > 0 ? h[0] / h[1] : 0.0;
+    double a = 0.0;
+    for (int j = 0; j < ns; ++j) a += h[2 * j + 1] > 0 ? h[2 * j] / h[2 * j + 1] : 0.0;
+    *loss = a / ns;
   }
+  return BH_OK;
+}
+
+// One training step on a Hessian: nprobe probe systems (the normalised right-hand side first when
+// rhs is given, then standard normal vectors drawn from seed), loss, backward, Adam.
+template <typename V>
+int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, const double* rhs, int nprobe,
+                   uint64_t seed, double lr, double* loss, hipStream_t s) {
+  if (nprobe < 1 || nprobe > kMaxSystems) return BH_BAD_ARG;
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  const bh_mesh* m = P->mesh;
+  const int nV = m->nV;
+  const size_t need = sizeof(double) * 2 * (size_t)nprobe * 3 * (size_t)nV;
+  if (need > M->probeBytes) {
+    if (M->probe) BH_CHECK(hipFree(M->probe));
+    M->probe = nullptr;
+    BH_CHECK(hipMalloc(&M->probe, need));
+    M->probeBytes = need;
+  }
+  // the scales come from the forward pass of mgn_loss; probes need them first, so run the
+  // feature kernel once here (it is cheap and deterministic)
+  MgnWs w;
+  mgn_ws_carve(M->ws, nV, P->nlower, w);
+  hipLaunchKernelGGL((k_mgn_prep<V>), g256(nV), dim3(256), 0, s, nV, m->bptr, vals, fixed, P->dslot, P->lptr, P->lcol,
+                     P->lslot, w.s, w.C, w.xn);
+  const double* bs[kMaxSystems];
+  const double* xs[kMaxSystems];
+  for (int j = 0; j < nprobe; ++j) {
+    double* x = M->probe + (size_t)(2 * j) * 3 * nV;
+    double* b = x + 3ll * nV;
+    const bool useRhs = rhs && j == 0;
+    hipLaunchKernelGGL(k_mgn_probe_x, g256(3ll * nV), dim3(256), 0, s, nV, w.s, fixed, useRhs ? rhs : nullptr,
+                       mgn_mix(seed + 0x1000ull * (uint64_t)j), x);
+    hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(spmv_blocks(nV)), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, fixed,
+                       x, b, nullptr, nullptr);
+    bs[j] = b;
+    xs[j] = x;
+  }
+  BH_CHECK(hipGetLastError());
+  if ((st = mgn_loss<V>(M, P, vals, fixed, nprobe, bs, xs, 1, loss, s))) return st;
+  M->step += 1;
+  const double b1 = 0.9, b2 = 0.999;
+  const double c1 = 1.0 - std::pow(b1, (double)M->step), c2 = 1.0 - std::pow(b2, (double)M->step);
+  hipLaunchKernelGGL(k_mgn_adam, g256(M->nparam), dim3(256), 0, s, M->nparam, M->w, M->g, M->m1, M->m2, (float)lr,
+                     (float)b1, (float)b2, 1e-8f, (float)c1, (float)c2);
+  BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
@@ -863,7 +978,8 @@ int mgn_factor(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hi
   mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
   if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
   const int nV = P->mesh->nV;
-  hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, w.C, w.d, w.O, w.s, P->lval, P->dinv);
+  hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O, w.s,
+                     P->lval, P->dinv);
   BH_CHECK(hipGetLastError());
   return precond_finalize(P, s);
 }
@@ -913,7 +1029,7 @@ int bh_mgn_create(uint64_t seed, bh_mgn** out) {
   const size_t partBytes = sizeof(float) * (size_t)kOuterBlocks * (3 * kW * kW + kW) + sizeof(double) * 2 * 65536;
   if (hipMalloc(&M->w, bytes) != hipSuccess || hipMalloc(&M->g, bytes) != hipSuccess ||
       hipMalloc(&M->m1, bytes) != hipSuccess || hipMalloc(&M->m2, bytes) != hipSuccess ||
-      hipMalloc(&M->part, partBytes) != hipSuccess || hipMalloc(&M->dscal, sizeof(double) * 4) != hipSuccess) {
+      hipMalloc(&M->part, partBytes) != hipSuccess || hipMalloc(&M->dscal, sizeof(double) * 2 * kMaxSystems) != hipSuccess) {
     delete M;
     return BH_DEVICE_ERROR;
   }
@@ -928,7 +1044,7 @@ int bh_mgn_create(uint64_t seed, bh_mgn** out) {
 int bh_mgn_destroy(bh_mgn* M) {
   if (!M) return BH_BAD_ARG;
   (void)hipDeviceSynchronize();
-  void* ptrs[] = {M->w, M->g, M->m1, M->m2, M->part, M->dscal, M->ws, M->pv};
+  void* ptrs[] = {M->w, M->g, M->m1, M->m2, M->part, M->dscal, M->ws, M->pv, M->probe};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete M;
@@ -964,7 +1080,17 @@ int bh_mgn_loss_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint
                        const double* d_x, int backward, double* loss, bh_stream_t stream) {
   if (!M || !P || !d_vals || !d_b || !d_x) return BH_BAD_ARG;
   if (P->mesh->nV == 0) return BH_OK;
-  return mgn_loss<float>(M, P, d_vals, d_fixed_dof, d_b, d_x, backward, loss, as_stream(stream));
+  const double* bs[1] = {d_b};
+  const double* xs[1] = {d_x};
+  return mgn_loss<float>(M, P, d_vals, d_fixed_dof, 1, bs, xs, backward, loss, as_stream(stream));
+}
+
+int bh_mgn_train_step_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint8_t* d_fixed_dof,
+                             const double* d_rhs, int nprobe, uint64_t seed, double lr, double* loss,
+                             bh_stream_t stream) {
+  if (!M || !P || !d_vals || !(lr > 0)) return BH_BAD_ARG;
+  if (P->mesh->nV == 0) return BH_OK;
+  return mgn_train_step<float>(M, P, d_vals, d_fixed_dof, d_rhs, nprobe, seed, lr, loss, as_stream(stream));
 }
 
 int bh_mgn_adam_step(bh_mgn* M, double lr, double beta1, double beta2, double eps, bh_stream_t stream) {

@@ -12,27 +12,6 @@
 
 namespace {
 
-constexpr int kRedBlock = 256;
-constexpr int kRedMaxBlocks = 1024;
-constexpr int kPcgBatch = 16;
-
-struct PcgState {
-  double rz, pAp, alpha, beta, rr, bb, tol2;
-  int done;  // 0 running, 1 converged, 2 iteration cap, 3 non-positive curvature
-  int iters, maxit;
-};
-
-__device__ __forceinline__ double block_sum_256(double v, double* sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  __syncthreads();
-  return r;
-}
-
 int red_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(kRedMaxBlocks, (n + 4095) / 4096)); }
 
 // partial dot products over a fixed partition of [0, n)
@@ -182,56 +161,6 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_start(const double* __restric
     st->maxit = maxit;
     st->done = (b == 0.0) ? 1 : 0;
   }
-}
-
-// Ap = mask(H p), partial p.Ap.  16 lanes per vertex: lane j takes blocks j, j+16, ... of the
-// vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
-// neighbouring 24-byte pieces), then the three row sums are reduced across the 16 lanes with a fixed
-// xor butterfly.  FP32 values are widened; products are summed in FP64.
-constexpr int kSpmvLanes = 16;
-inline int spmv_blocks(int nV) { return (int)((kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock); }
-
-template <typename V>
-__global__ void __launch_bounds__(kRedBlock)
-    k_pcg_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
-               const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const double* __restrict__ p,
-               double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
-  __shared__ double sh[4];
-  if (st && st->done) return;
-  const int64_t gid = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
-  const int v = (int)(gid / kSpmvLanes), j = (int)(gid % kSpmvLanes);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  if (v < nV) {
-    const int b0 = bptr[v], nb = bptr[v + 1] - b0;
-    const V* base = vals + 9ll * b0;
-    for (int r = j; r < nb; r += kSpmvLanes) {
-      const double* xp = p + 3ll * bcol[b0 + r];
-      const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-      const V* r0 = base + 3 * r;
-      const V* r1 = r0 + 3 * nb;
-      const V* r2 = r0 + 6 * nb;
-      s0 += (double)r0[0] * x0 + (double)r0[1] * x1 + (double)r0[2] * x2;
-      s1 += (double)r1[0] * x0 + (double)r1[1] * x1 + (double)r1[2] * x2;
-      s2 += (double)r2[0] * x0 + (double)r2[1] * x1 + (double)r2[2] * x2;
-    }
-  }
-#pragma unroll
-  for (int off = kSpmvLanes / 2; off > 0; off >>= 1) {
-    s0 += __shfl_xor(s0, off, 64);
-    s1 += __shfl_xor(s1, off, 64);
-    s2 += __shfl_xor(s2, off, 64);
-  }
-  double pap = 0.0;
-  if (v < nV && j < 3) {
-    const int64_t row = 3ll * v + j;
-    double sv = (j == 0) ? s0 : (j == 1) ? s1 : s2;
-    const double pr = p[row];
-    if (fixed && fixed[row]) sv = pr;
-    Ap[row] = sv;
-    pap = pr * sv;
-  }
-  const double d = block_sum_256(pap, sh);
-  if (part && threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
 __global__ void __launch_bounds__(kRedBlock) k_pcg_alpha(const double* __restrict__ part, int np, PcgState* st) {
@@ -411,6 +340,23 @@ __global__ void k_negate(int64_t n, const double* __restrict__ a, double* __rest
   if (i < n) y[i] = -a[i];
 }
 
+// backward-Euler inertia pieces: d = q - q_pred; y += a x; vals += a M
+__global__ void __launch_bounds__(256) k_sub(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                             double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a[i] - b[i];
+}
+__global__ void __launch_bounds__(256) k_axpy(int64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+template <typename V>
+__global__ void __launch_bounds__(256) k_add_mass(int64_t n, double a, const double* __restrict__ mass,
+                                                  V* __restrict__ vals) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) vals[i] = (V)((double)vals[i] + a * mass[i]);
+}
+
 struct NewtonWs {
   double *vals, *g, *b, *x, *r, *z, *p, *Ap, *qt, *Dinv, *part, *scal;
   float* vals32;    // precision form: FP32 Hessian values
@@ -461,6 +407,17 @@ int objective(bh_mesh* m, const bh_newton_opts* o, const double* params, const d
     st = device_dot(f, q, 3ll * m->nV, w.part, w.scal + 8, s);
     if (st) return st;
     hipLaunchKernelGGL(k_axpy_to, dim3(1), dim3(1), 0, s, (int64_t)1, out, -1.0, w.scal + 8, out);
+    BH_CHECK(hipGetLastError());
+  }
+  if (o->d_mass_vals) {
+    // + 1/2 inertia d^T M d, d = q - q_pred (w.z, w.Ap are free outside the PCG solve)
+    const int64_t n = 3ll * m->nV;
+    hipLaunchKernelGGL(k_sub, dim3((n + 255) / 256), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
+    hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
+                       o->d_mass_vals, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr, (const PcgState*)nullptr);
+    st = device_dot(w.z, w.Ap, n, w.part, w.scal + 9, s);
+    if (st) return st;
+    hipLaunchKernelGGL(k_axpy_to, dim3(1), dim3(1), 0, s, (int64_t)1, out, 0.5 * o->inertia, w.scal + 9, out);
     BH_CHECK(hipGetLastError());
   }
   return BH_OK;
@@ -566,12 +523,25 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
   const bh_psd* psd = o->psd.enabled ? &o->psd : nullptr;
   int st = hessian_dispatch(m, o->model, o->d_model_ids, q, params, psd, vals, s);
   if (st) return st;
+  if (o->d_mass_vals) {
+    hipLaunchKernelGGL(k_add_mass<V>, dim3((m->nnz + 255) / 256), dim3(256), 0, s, m->nnz, o->inertia, o->d_mass_vals,
+                       vals);
+    BH_CHECK(hipGetLastError());
+  }
   *fallback = 0;
   if (o->precond == BH_PRECOND_BIC0 || o->precond == BH_PRECOND_FACTOR || o->precond == BH_PRECOND_LEARNED) {
     if (!o->precond_handle) return BH_BAD_ARG;
-    if (o->precond == BH_PRECOND_BIC0 &&
-        (st = precond_factor_bic0(o->precond_handle, vals, fixed, o->precond_shift, nullptr, s)))
-      return st;
+    if (o->precond == BH_PRECOND_BIC0) {
+      // IC(0) may break down (non-positive pivots) on strongly heterogeneous problems: refactor with a
+      // growing diagonal shift until every pivot is positive
+      double shift = o->precond_shift;
+      for (int attempt = 0;; ++attempt) {
+        int fails = 0;
+        if ((st = precond_factor_bic0(o->precond_handle, vals, fixed, shift, &fails, s))) return st;
+        if (fails == 0 || attempt >= 6) break;
+        shift = (shift > 0.0) ? 4.0 * shift : 1e-3;
+      }
+    }
     if (o->precond == BH_PRECOND_LEARNED) {
       if (!o->mgn) return BH_BAD_ARG;
       if ((st = mgn_factor<V>(o->mgn, o->precond_handle, vals, fixed, s))) return st;
@@ -625,6 +595,9 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->mgn = nullptr;
   o->on_system = nullptr;
   o->user = nullptr;
+  o->d_mass_vals = nullptr;
+  o->d_q_pred = nullptr;
+  o->inertia = 0.0;
 }
 
 
@@ -633,6 +606,7 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
                             const double* f, double* q, bh_newton_report* rep, double* trace, bh_stream_t stream) {
   if (!m || !o || !params || !q) return BH_BAD_ARG;
   if (check_model_args(o->model, o->d_model_ids)) return BH_BAD_ARG;
+  if (o->d_mass_vals && (!o->d_q_pred || !(o->inertia >= 0.0))) return BH_BAD_ARG;
   hipStream_t s = as_stream(stream);
   NewtonWs w;
   int st = newton_workspace(m, w);
@@ -654,6 +628,13 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
     st = f32 ? gradient_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, w.g, s)
              : gradient_dispatch(m, o->model, o->d_model_ids, q, params, w.g, s);
     if (st) return st;
+    if (o->d_mass_vals) {  // + inertia M (q - q_pred)
+      hipLaunchKernelGGL(k_sub, dim3(gn), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
+      hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
+                         o->d_mass_vals, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
+                         (const PcgState*)nullptr);
+      hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, s, n, o->inertia, w.Ap, w.g);
+    }
     hipLaunchKernelGGL(k_mask_gradient, dim3(gn), dim3(256), 0, s, n, fixed, f, w.g);
     st = device_dot(w.g, w.g, n, w.part, w.scal, s);
     if (st) return st;

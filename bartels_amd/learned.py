@@ -5,8 +5,10 @@ Training problems: jittered beams of 24x24x96 ... 40x40x160 vertices, per-tet Yo
 log-uniform in [1e5, 1e7] Pa, Poisson ratio uniform in [0.30, 0.48], 70 / 20 / 10 % Neo-Hookean /
 StVK / corotational tets, x = 0 clamped, gravity plus a random traction of magnitude up to 5e4 Pa in
 a random direction on the free end.  Each problem is solved by the precision-form Newton with
-block-Jacobi PCG (the plain solver) to 1e-6 relative residual; every system H x = b it solves is
-handed (on the device, through bh_newton_opts.on_system) to a few Adam steps of the network.
+block-Jacobi PCG (the plain solver) to 1e-6 relative residual; every Hessian it assembles is handed
+(on the device, through bh_newton_opts.on_system) to a few Adam steps of the network on probe
+systems: the normalised right-hand side of that Newton system and standard normal vectors (a
+Hutchinson estimate of the relative Frobenius error of L L^T).
 
     python -m bartels_amd.learned train --solves 120 --out weights.f32
     python -m bartels_amd.learned eval --weights weights.f32 --problems 10
@@ -158,7 +160,7 @@ class Network:
             self.h = ctypes.c_void_p()
 
 
-def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, log=print):
+def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, log=print):
     net = Network(seed=seed + 12345)
     rng = np.random.default_rng(seed)
     history = []
@@ -171,12 +173,12 @@ def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, log=print):
         losses = []
 
         def on_system(user, it, v32, v64, b, x):
-            for _ in range(steps):
+            for k in range(steps):
                 L = ctypes.c_double()
-                _lib.check(net.lib.bh_mgn_loss_device(net.h, S.P, ctypes.c_void_p(v32), S.dfixed.data_ptr(),
-                                                      ctypes.c_void_p(b), ctypes.c_void_p(x), 1, ctypes.byref(L),
-                                                      None), "mgn_loss")
-                _lib.check(net.lib.bh_mgn_adam_step(net.h, lr, 0.9, 0.999, 1e-8, None), "adam")
+                _lib.check(net.lib.bh_mgn_train_step_device(net.h, S.P, ctypes.c_void_p(v32), S.dfixed.data_ptr(),
+                                                            ctypes.c_void_p(b), probes,
+                                                            (seed * 1000003 + 7919 * i + 31 * it + k) & (2**64 - 1),
+                                                            lr, ctypes.byref(L), None), "mgn_train_step")
                 losses.append(L.value)
 
         _, rep, tr, wall = S.newton(0, precision=1, max_it=newton_its, on_system=on_system)
@@ -217,6 +219,7 @@ def main():
     a.add_argument("--solves", type=int, default=120)
     a.add_argument("--steps", type=int, default=4)
     a.add_argument("--lr", type=float, default=1e-3)
+    a.add_argument("--probes", type=int, default=4)
     a.add_argument("--seed", type=int, default=0)
     a.add_argument("--out", default=DEFAULT_WEIGHTS)
     b = sub.add_parser("eval")
@@ -225,7 +228,7 @@ def main():
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)  # noqa: E731
     if args.cmd == "train":
-        train(args.solves, args.out, seed=args.seed, steps=args.steps, lr=args.lr, log=log)
+        train(args.solves, args.out, seed=args.seed, steps=args.steps, lr=args.lr, probes=args.probes, log=log)
     else:
         rows = evaluate(args.weights, args.problems, log=log)
         j = np.array([r["jacobi_pcg"] for r in rows], float)

@@ -238,3 +238,18 @@ def config3(grid=(192, 48, 48), h=H_DEFAULT, seed=3):
     q[:, 2] = c[2] + s * (V[:, 2] - c[2])
     return Problem(V=V, E=E, q=q.ravel(), params=model_params(ids, lam, mu), model=255, model_ids=ids,
                    density=1000.0, gravity=np.array([0.0, 0.0, -9.8]), fixed=fixed_vertices(V), h=h)
+
+
+def config5(grid=(256, 64, 64), h=H_DEFAULT, seed=5):
+    """Full chosen form: the 64x64x256 Neo-Hookean beam at rest, x = 0 clamped, falling under gravity
+    (backward Euler, dt = 0.01 s), with per-tet Young's modulus log-uniform in [1e5, 1e7] Pa
+    (Poisson 0.45)."""
+    nx, ny, nz = grid
+    V, E = beam_grid(nx, ny, nz, h)
+    nT = E.shape[0]
+    u = uniform(seed, nT)
+    Ey = np.exp(np.log(1e5) + (np.log(1e7) - np.log(1e5)) * u)
+    lam, mu = emu_to_lame(Ey, 0.45)
+    ids = np.zeros(nT, dtype=np.uint8)
+    return Problem(V=V, E=E, q=V.ravel().copy(), params=model_params(ids, lam, mu), model=0, model_ids=None,
+                   density=1000.0, gravity=np.array([0.0, 0.0, -9.8]), fixed=fixed_vertices(V), h=h, dt=0.01)

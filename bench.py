@@ -67,6 +67,76 @@ def cpu_baseline(p, dX, vol, budget_s):
                        f"{el:.1f} s on one host core")
 
 
+def newton_steps(p, mesh, steps, pcg_max):
+    """Newton-step wall time on the same beam (bent state, gravity, x = 0 clamped, PSD 1e-3, PCG to a
+    1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
+    precision form with the learned preconditioner (block IC(0) is reported beside them)."""
+    import torch
+
+    from bartels_amd import _lib, learned
+
+    lib = _lib.load()
+    nV = p.V.shape[0]
+    fixed = np.zeros(3 * nV, np.uint8)
+    fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1
+    q0 = p.q.copy()
+    q0[fixed == 1] = p.V.ravel()[fixed == 1]
+    dev = torch.device("cuda:0")
+    dparams = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device=dev)
+    dfixed = torch.tensor(fixed, device=dev)
+    f = torch.zeros(3 * nV, dtype=torch.float64, device=dev)
+    lib.bh_linear_tetmesh_const_ext_force_device(mesh.handle, (ctypes.c_double * 3)(*(p.density * p.gravity)),
+                                                 ctypes.c_void_p(f.data_ptr()), None)
+    P = ctypes.c_void_p()
+    _lib.check(lib.bh_precond_create(mesh.handle, ctypes.byref(P)), "bh_precond_create")
+    net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
+    variants = [("fp64_jacobi", 0, 0), ("fp32_bic0", 1, 2)]
+    if net is not None:
+        variants.append(("fp32_learned", 1, 4))
+    res, qs = {}, {}
+    for name, prec, precond in variants:
+        o = _lib.BhNewtonOpts()
+        lib.bh_newton_default_opts(ctypes.byref(o))
+        o.model = 0
+        o.psd = _lib.BhPsd(1, 1e-3)
+        o.gradient_tol = 0.0
+        o.max_iterations = steps - 1
+        o.pcg_tol = 1e-6
+        o.pcg_max_iterations = pcg_max
+        o.precond = precond
+        o.precond_handle = P
+        o.element_precision = prec
+        if net is not None:
+            o.mgn = net.h
+        dq = torch.tensor(q0, device=dev)
+        rep = _lib.BhNewtonReport()
+        tr = np.zeros(4 * (steps + 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = lib.bh_newton_static_device(mesh.handle, ctypes.byref(o), ctypes.c_void_p(dparams.data_ptr()),
+                                         ctypes.c_void_p(dfixed.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                                         ctypes.c_void_p(dq.data_ptr()), ctypes.byref(rep),
+                                         tr.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        qs[name] = dq.cpu().numpy()
+        res[name] = dict(status=st, newton_steps=rep.iterations, s_per_step=wall / max(1, rep.iterations),
+                         pcg_iterations=[int(x) for x in tr.reshape(-1, 4)[:rep.iterations, 3]],
+                         pcg_failures=rep.pcg_failures, pcg_fallbacks=rep.pcg_fallbacks,
+                         objective=rep.energy)
+    lib.bh_precond_destroy(P)
+    if net is not None:
+        net.close()
+    u = np.linalg.norm(qs["fp64_jacobi"] - q0)
+    for name in qs:
+        res[name]["rel_displacement_vs_fp64_jacobi"] = float(np.linalg.norm(qs[name] - qs["fp64_jacobi"]) / u)
+    base = res["fp64_jacobi"]["s_per_step"]
+    for name in res:
+        res[name]["speedup_vs_fp64_jacobi"] = base / res[name]["s_per_step"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +147,9 @@ def main():
     ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1 or 2 lanes per vertex "
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--newton-steps", type=int, default=2,
+                    help="Newton steps per solver variant in the Newton-step comparison (0: skip)")
+    ap.add_argument("--pcg-max", type=int, default=20000)
     ap.add_argument("--quick", action="store_true", help="timed steps only (for counter collection)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference point for the HBM byte counters)")
@@ -244,6 +317,9 @@ def main():
             "setup_s": setup_s,
         },
     }
+    if args.newton_steps > 0:
+        del H, g
+        out["extra"]["newton"] = newton_steps(p, mesh, args.newton_steps, args.pcg_max)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, dX, vol, args.cpu_seconds)
     print(json.dumps(out), flush=True)

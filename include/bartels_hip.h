@@ -200,8 +200,9 @@ int bh_precond_apply_device(bh_precond* precond, const double* d_r, double* d_z,
  * the masked, diagonally normalised Hessian, 3 message-passing layers of width 32, decoders that
  * write the factor L (L L^T approximates H; at initialisation the decoders are zero and L L^T is
  * the block diagonal of H).  Parameters are FP32 (bh_mgn_num_params floats, layout fixed by the
- * library).  Training minimises |L L^T x - b|^2 / |b|^2 in the diagonally scaled norm on Newton
- * systems H x = b solved by the plain solver; gradients accumulate until bh_mgn_adam_step. */
+ * library).  The loss of a system H x = b is |L L^T x - b|^2 / |b|^2 in the diagonally scaled norm;
+ * bh_mgn_loss_device accumulates its gradient until bh_mgn_adam_step, bh_mgn_train_step_device
+ * trains on probe systems of a Hessian the plain solver produced. */
 typedef struct bh_mgn bh_mgn;
 int bh_mgn_create(uint64_t seed, bh_mgn** mgn);
 int bh_mgn_destroy(bh_mgn* mgn);
@@ -214,6 +215,13 @@ int bh_mgn_set_params(bh_mgn* mgn, const float* params);
  * zero); backward != 0 also accumulates the gradient.  loss (host, optional) synchronises. */
 int bh_mgn_loss_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, const uint8_t* d_fixed_dof,
                        const double* d_b, const double* d_x, int backward, double* loss, bh_stream_t stream);
+/* One training step on a Hessian: nprobe (<= 16) probe systems -- the normalised right-hand side
+ * d_rhs first when given, then standard normal vectors drawn from seed -- give a Hutchinson estimate
+ * of |L L^T - H|_F^2 / |H|_F^2 in the diagonally scaled norm; backward and one Adam step (lr,
+ * beta 0.9 / 0.999, eps 1e-8).  loss (host, optional) synchronises. */
+int bh_mgn_train_step_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, const uint8_t* d_fixed_dof,
+                             const double* d_rhs, int nprobe, uint64_t seed, double lr, double* loss,
+                             bh_stream_t stream);
 /* one Adam update from the accumulated gradient (which it then clears) */
 int bh_mgn_adam_step(bh_mgn* mgn, double lr, double beta1, double beta2, double eps, bh_stream_t stream);
 /* inference: writes the factor of the given Hessian into the preconditioner */
@@ -256,6 +264,13 @@ typedef struct {
   void (*on_system)(void* user, int iteration, const float* d_vals32, const double* d_vals64, const double* d_b,
                     const double* d_x);
   void* user;
+  /* Backward-Euler inertia (matlab/examples/test_tet_materials.m:97-112 in position form): when
+   * d_mass_vals != NULL the objective gains 1/2 inertia (q - q_pred)^T M (q - q_pred), with M the
+   * consistent mass matrix (values in the mesh pattern, bh_linear_tetmesh_mass_matrix_device),
+   * q_pred = q_t + dt v_t and inertia = 1 / dt^2.  bh_newton_static_device then performs one step. */
+  const double* d_mass_vals;
+  const double* d_q_pred;
+  double inertia;
 } bh_newton_opts;
 
 typedef struct {

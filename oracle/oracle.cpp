@@ -932,11 +932,52 @@ extern "C" {
 // The fixed DOFs are removed as P H P^T / P g (reference: src/fixed_point_constraint_matrix.cpp:7-38).
 // trace (optional, length 4*(max_it+1)): per iteration [energy, |g|, alpha, ls_iterations].
 // Returns the number of iterations run (>=0), or -1 when the direct solve fails.
+static int newton_impl(int model, int nV, int nT, const int* E, const double* dphidX, const double* vol,
+                       const double* params, const unsigned char* model_ids, const unsigned char* fixed_dof,
+                       const double* f_ext, int psd, double psd_tol, double tol, int max_it, int max_ls, double c,
+                       double p, double* q, double* trace, int* converged, const or_csc* mass, const double* q_pred,
+                       double inertia);
+
 int or_newton_static(int model, int nV, int nT, const int* E, const double* dphidX, const double* vol,
                      const double* params, const unsigned char* model_ids, const unsigned char* fixed_dof,
                      const double* f_ext, int psd, double psd_tol, double tol, int max_it, int max_ls, double c,
                      double p, double* q /*in/out, 3 nV*/, double* trace, int* converged) {
+  return newton_impl(model, nV, nT, E, dphidX, vol, params, model_ids, fixed_dof, f_ext, psd, psd_tol, tol, max_it,
+                     max_ls, c, p, q, trace, converged, nullptr, nullptr, 0.0);
+}
+
+// One backward-Euler step in the position form of the reference's velocity objective
+// (matlab/examples/test_tet_materials.m:97-112): with q = q_t + dt v and q_pred = q_t + dt v_t,
+//   1/2 v^T M v - v^T M v_t + Psi(q_t + dt v) - dt v^T f
+//     = 1/(2 dt^2) (q - q_pred)^T M (q - q_pred) + Psi(q) - f^T q + const,
+// so the minimiser is the same and the Newton iterates correspond (d_q = dt d_v).  Same Newton /
+// line-search semantics as or_newton_static; M = consistent mass with the given per-tet density.
+int or_newton_be(int model, int nV, int nT, const int* E, const double* dphidX, const double* vol,
+                 const double* params, const unsigned char* model_ids, const unsigned char* fixed_dof,
+                 const double* f_ext, const double* density, const double* q_pred, double dt, int psd,
+                 double psd_tol, double tol, int max_it, int max_ls, double c, double p, double* q, double* trace,
+                 int* converged) {
+  or_csc* M = or_tetmesh_mass_matrix(nV, nT, E, density, vol);
+  const int r = newton_impl(model, nV, nT, E, dphidX, vol, params, model_ids, fixed_dof, f_ext, psd, psd_tol, tol,
+                            max_it, max_ls, c, p, q, trace, converged, M, q_pred, 1.0 / (dt * dt));
+  or_csc_free(M);
+  return r;
+}
+
+static int newton_impl(int model, int nV, int nT, const int* E, const double* dphidX, const double* vol,
+                       const double* params, const unsigned char* model_ids, const unsigned char* fixed_dof,
+                       const double* f_ext, int psd, double psd_tol, double tol, int max_it, int max_ls, double c,
+                       double p, double* q, double* trace, int* converged, const or_csc* mass, const double* q_pred,
+                       double inertia) {
   const int64_t n = 3 * (int64_t)nV;
+  // M (q - q_pred), the inertia term's gradient
+  auto mass_times = [&](const std::vector<double>& qq, std::vector<double>& out) {
+    out.assign(n, 0.0);
+    for (int64_t cc = 0; cc < n; ++cc) {
+      const double dc = qq[cc] - q_pred[cc];
+      for (int64_t k = mass->colptr[cc]; k < mass->colptr[cc + 1]; ++k) out[mass->rowind[k]] += mass->vals[k] * dc;
+    }
+  };
   std::vector<int64_t> free_of(n, -1);
   std::vector<int64_t> dofs;
   for (int64_t i = 0; i < n; ++i)
@@ -945,10 +986,17 @@ int or_newton_static(int model, int nV, int nT, const int* E, const double* dphi
       dofs.push_back(i);
     }
   const int64_t nf = (int64_t)dofs.size();
+  std::vector<double> Md;
   auto energy = [&](const std::vector<double>& qq) {
     double e = or_tetmesh_q(model, nV, nT, E, qq.data(), dphidX, vol, params, model_ids);
     if (f_ext)
       for (int64_t i = 0; i < n; ++i) e -= f_ext[i] * qq[i];
+    if (mass) {
+      mass_times(qq, Md);
+      double a = 0;
+      for (int64_t i = 0; i < n; ++i) a += (qq[i] - q_pred[i]) * Md[i];
+      e += 0.5 * inertia * a;
+    }
     return e;
   };
   std::vector<double> qv(q, q + n), gfull(n), g(nf), d(nf);
@@ -956,6 +1004,10 @@ int or_newton_static(int model, int nV, int nT, const int* E, const double* dphi
   int it = 0;
   do {
     or_tetmesh_dq(model, nV, nT, E, qv.data(), dphidX, vol, params, model_ids, gfull.data());
+    if (mass) {
+      mass_times(qv, Md);
+      for (int64_t i = 0; i < n; ++i) gfull[i] += inertia * Md[i];
+    }
     double gg = 0;
     for (int64_t k = 0; k < nf; ++k) {
       g[k] = gfull[dofs[k]] - (f_ext ? f_ext[dofs[k]] : 0.0);
@@ -972,6 +1024,10 @@ int or_newton_static(int model, int nV, int nT, const int* E, const double* dphi
       break;
     }
     or_csc* H = or_tetmesh_dq2(model, nV, nT, E, qv.data(), dphidX, vol, params, model_ids, 1, psd, psd_tol);
+    if (mass) {  // same pattern (full blocks of every vertex pair that shares a tet)
+      if (mass->nnz != H->nnz) return -1;
+      for (int64_t k = 0; k < H->nnz; ++k) H->vals[k] += inertia * mass->vals[k];
+    }
     CSC R;  // reduced P H P^T
     R.n = nf;
     R.colptr.assign(nf + 1, 0);

@@ -56,6 +56,9 @@ def load():
     L.or_newton_static.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, i, d, d, i, i, d, d, vp, vp,
                                    ctypes.POINTER(ctypes.c_int)]
     L.or_newton_static.restype = i
+    L.or_newton_be.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, i, d, d, i, i, d, d, vp, vp,
+                               ctypes.POINTER(ctypes.c_int)]
+    L.or_newton_be.restype = i
     L.or_linesearch_backtracking.argtypes = [vp, vp, vp, i, ENERGY_CB, vp, i, d, d, d, ctypes.POINTER(ctypes.c_double)]
     L.or_linesearch_backtracking.restype = i
     L.or_band_solve.argtypes = [i64, vp, vp, vp, vp, vp]
@@ -199,6 +202,27 @@ def newton_static(model, nV, E, dX, vol, params, q0, fixed_dof=None, f_ext=None,
         raise RuntimeError("oracle direct solve failed")
     return q, it, bool(conv.value), trace.reshape(-1, 4)[:it + 1]
 
+
+
+def newton_be(model, nV, E, dX, vol, params, q0, q_pred, dt, density, fixed_dof=None, f_ext=None, model_ids=None,
+              psd=False, psd_tol=1e-3, tol=1e-3, max_it=100, max_ls=100, c=1e-8, p=0.5):
+    """One backward-Euler step (position form of matlab/examples/test_tet_materials.m:97-112) with the
+    reference Newton semantics and the banded direct solve.  density: per tet.  Returns
+    (q, iterations, converged, trace)."""
+    E, q, dX, vol, params, ids = _args(E, q0, dX, vol, params, model_ids)
+    q = q.copy()
+    fx = None if fixed_dof is None else np.ascontiguousarray(fixed_dof, dtype=np.uint8)
+    fe = None if f_ext is None else np.ascontiguousarray(f_ext, dtype=np.float64)
+    rho = np.ascontiguousarray(np.broadcast_to(np.asarray(density, dtype=np.float64), (E.shape[0],)))
+    qp = np.ascontiguousarray(q_pred, dtype=np.float64)
+    trace = np.zeros(4 * (max_it + 1))
+    conv = ctypes.c_int()
+    it = load().or_newton_be(model, nV, E.shape[0], _p(E), _p(dX), _p(vol), _p(params), _p(ids), _p(fx), _p(fe),
+                             _p(rho), _p(qp), dt, 1 if psd else 0, psd_tol, tol, max_it, max_ls, c, p, _p(q),
+                             _p(trace), ctypes.byref(conv))
+    if it < 0:
+        raise RuntimeError("oracle direct solve failed")
+    return q, it, bool(conv.value), trace.reshape(-1, 4)[:it + 1]
 
 def linesearch(x, d, g, f, max_it=100, alpha=1.0, c=1e-8, p=0.5):
     """reference linesearch_backtracking_bisection on a Python energy f(x)."""

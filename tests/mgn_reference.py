@@ -151,8 +151,9 @@ def loss(P, g, feats, x, b):
     return ((z - bh) ** 2).sum() / (bh ** 2).sum(), d, O
 
 
-def dense_factor(g, s, Lvv, Lk):
-    """the factor of H as a dense 3nV x 3nV lower-triangular-by-colour matrix: L = S Lhat"""
+def dense_factor(g, s, Lvv, Lk, fixed=None):
+    """the factor of H as a dense 3nV x 3nV lower-triangular-by-colour matrix: L = S Lhat, with
+    identity rows and columns on the fixed DOFs (the masked system)"""
     nV = g.nV
     L = np.zeros((3 * nV, 3 * nV))
     for v in range(nV):
@@ -160,4 +161,9 @@ def dense_factor(g, s, Lvv, Lk):
     for k in range(g.nP):
         v, n = g.pv[k], g.lcol[k]
         L[3 * v:3 * v + 3, 3 * n:3 * n + 3] = s[v] * Lk[k]
+    if fixed is not None:
+        fx = np.asarray(fixed).astype(bool)
+        L[fx, :] = 0.0
+        L[:, fx] = 0.0
+        L[fx, fx] = 1.0
     return L

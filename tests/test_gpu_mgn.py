@@ -169,7 +169,7 @@ def test_mgn_factor_inference_matches_reference(system):
         s, C, xn, fp = (torch.as_tensor(a) for a in S["feats"])
         d, O = R.forward(R.unflatten(torch.tensor(w, dtype=torch.float64)), g, xn, fp)
         Lvv, Lk = R.factor(C, d, O)
-        Ld = R.dense_factor(g, S["feats"][0], Lvv.detach().numpy(), Lk.detach().numpy())
+        Ld = R.dense_factor(g, S["feats"][0], Lvv.detach().numpy(), Lk.detach().numpy(), S["fixed"])
         r = rng.standard_normal(3 * g.nV)
         z_ref = np.linalg.solve(Ld @ Ld.T, r)
         dr = torch.tensor(r, device="cuda")
@@ -198,5 +198,26 @@ def test_mgn_training_reduces_loss(system):
         print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
         assert np.isfinite(losses).all()
         assert losses[-1] < 0.7 * losses[0]
+    finally:
+        lib.bh_mgn_destroy(M)
+
+
+def test_mgn_probe_training_step_reduces_frobenius_loss(system):
+    """bh_mgn_train_step_device (probe systems of one Hessian, right-hand side first) lowers the
+    relative Frobenius error of L L^T from its block-Jacobi value"""
+    S = system
+    lib = S["lib"]
+    M, n = make_mgn(lib, seed=9)
+    try:
+        losses = []
+        for it in range(80):
+            loss = ctypes.c_double()
+            assert lib.bh_mgn_train_step_device(M, S["P"], vp(S["vals"]), vp(S["dfixed"]), vp(S["db"]), 4, 100 + it,
+                                                3e-3, ctypes.byref(loss), None) == 0
+            losses.append(loss.value)
+        print(f"probe loss {losses[0]:.4f} -> {losses[-1]:.4f} (min {min(losses):.4f})")
+        assert np.isfinite(losses).all()
+        assert 0.0 < losses[0] < 10.0
+        assert np.mean(losses[-10:]) < 0.8 * losses[0]
     finally:
         lib.bh_mgn_destroy(M)

@@ -223,3 +223,65 @@ def _newton_mgn(mesh, p, q0, fixed, f, M, precond=4, precision=1, handle=None, p
                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert st in (_lib.BH_OK, _lib.BH_MAXIT), st
     return dq.cpu().numpy(), rep, None
+
+
+def test_backward_euler_steps_match_oracle(oracle):
+    """three backward-Euler steps (dt = 0.01, gravity, x = 0 clamped) of a small beam: device Newton
+    with the inertia term (FP64, and the FP32 precision form) against the oracle's direct-solve BE"""
+    import torch
+
+    from bartels_amd import _lib
+    from bartels_amd import tetmesh as tm
+
+    p = meshgen.config1(grid=(12, 4, 4))
+    nV = p.V.shape[0]
+    dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
+    mesh = tm.TetMesh(nV, p.E, dX, vol)
+    fixed = np.zeros(3 * nV, np.uint8)
+    fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1
+    f = oracle.const_ext_force(nV, p.E, vol, p.density * p.gravity)
+    dt = 0.01
+    lib = _lib.load()
+    dens = torch.full((p.E.shape[0],), p.density, dtype=torch.float64, device="cuda")
+    mvals = torch.empty(mesh.nnz, dtype=torch.float64, device="cuda")
+    assert lib.bh_linear_tetmesh_mass_matrix_device(mesh.handle, ctypes.c_void_p(dens.data_ptr()),
+                                                    ctypes.c_void_p(mvals.data_ptr()), None) == 0
+    dparams = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
+    dfixed = torch.tensor(fixed, device="cuda")
+    df = torch.tensor(f, device="cuda")
+    x_or = p.V.ravel().copy()
+    v_or = np.zeros_like(x_or)
+    for prec, bound in ((0, 1e-8), (1, 1e-5)):
+        x = p.V.ravel().copy()
+        v = np.zeros_like(x)
+        xo, vo = x_or.copy(), v_or.copy()
+        for step in range(3):
+            qp = x + dt * v
+            qpo = xo + dt * vo
+            q_or, _, _, _ = oracle.newton_be(0, nV, p.E, dX, vol, p.params, qpo, qpo, dt, p.density, fixed, f,
+                                            tol=1e-10, max_it=8)
+            o = _lib.BhNewtonOpts()
+            lib.bh_newton_default_opts(ctypes.byref(o))
+            o.model = 0
+            o.gradient_tol = 1e-10 if prec == 0 else 1e-4
+            o.relative_tol = 1
+            o.max_iterations = 8
+            o.pcg_tol = 1e-12
+            o.element_precision = prec
+            dqp = torch.tensor(qp, device="cuda")
+            o.d_mass_vals = ctypes.c_void_p(mvals.data_ptr())
+            o.d_q_pred = ctypes.c_void_p(dqp.data_ptr())
+            o.inertia = 1.0 / dt ** 2
+            dq = torch.tensor(qp, device="cuda")
+            rep = _lib.BhNewtonReport()
+            st = lib.bh_newton_static_device(mesh.handle, ctypes.byref(o), ctypes.c_void_p(dparams.data_ptr()),
+                                             ctypes.c_void_p(dfixed.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                                             ctypes.c_void_p(dq.data_ptr()), ctypes.byref(rep), None,
+                                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+            assert st in (_lib.BH_OK, _lib.BH_MAXIT)
+            q = dq.cpu().numpy()
+            v, x = (q - x) / dt, q
+            vo, xo = (q_or - xo) / dt, q_or
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo - p.V.ravel())
+        print(f"backward Euler, precision {prec}: relative displacement difference after 3 steps {err:.2e}")
+        assert err < bound
