@@ -104,6 +104,7 @@ SIGNATURES = {
     "bh_mesh_pattern": (c_int, [_VP, _VP, _VP]),
     "bh_mesh_block_pattern_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_int64)]),
     "bh_mesh_set_element_opts": (c_int, [_VP, P(BhElementOpts)]),
+    "bh_mesh_set_rest_positions": (c_int, [_VP, _VP]),
     "bh_mesh_set_gather_lanes": (c_int, [_VP, c_int]),
     "bh_mesh_set_tuning": (c_int, [_VP, P(BhTuning)]),
     "bh_linear_tetmesh_q": (c_int, [_VP, c_int, _VP, _VP, P(c_double)]),

@@ -60,6 +60,7 @@ struct bh_mesh {
   double* vol = nullptr;
   float* dX32 = nullptr;    // precision form (made on first use)
   float* vol32 = nullptr;
+  double* X = nullptr;      // rest positions (optional; the precision form's small-strain path)
   int64_t* gBase = nullptr; // per group of 64 vertices: offset of its interleaved incidence lists
   int32_t* gLen = nullptr;  // per group: longest list
   int32_t* incE = nullptr;
@@ -137,11 +138,12 @@ struct MeshView {
   const uint32_t* __restrict__ incR;
   const int32_t* __restrict__ bptr;
   const int32_t* __restrict__ bcol;
+  const double* __restrict__ X;  // rest positions (3 nV, interleaved) or null
 };
 
 MeshView view(const bh_mesh* m) {
   return MeshView{m->nV, m->nT, m->E, m->dX, m->dX32, m->vol, m->vol32, m->gBase, m->gLen, m->incE, m->incR,
-                  m->bptr, m->bcol};
+                  m->bptr, m->bcol, m->X};
 }
 
 // the interleaved incidence list of vertex v: entries at first + 64 k, k < len (code < 0 ends it)
@@ -223,6 +225,41 @@ __device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, co
   }
 }
 
+// F and G = F - I.  FP64: G from F.  FP32 with rest positions: G from displacement edges,
+// G = sum_{m>=1} ((x_m - x_0) - (X_m - X_0)) phi_m^T with the edges formed in FP64, so G keeps its
+// relative accuracy at small strain (F = I + G; stress and energy then use G); without rest
+// positions G = F - I in FP32.
+template <typename T>
+__device__ __forceinline__ void load_FG(const MeshView& m, const double* __restrict__ q, int4 ev, const T phi[4][3],
+                                        T* F, T* G) {
+  if constexpr (sizeof(T) == 4) {
+    if (m.X) {
+      const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
+      double u[4][3];
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u[mm][i] = q[3ll * vid[mm] + i] - m.X[3ll * vid[mm] + i];
+      T ed[3][3];
+#pragma unroll
+      for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(u[mm + 1][i] - u[0][i]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          G[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+          F[i + 3 * j] = G[i + 3 * j] + ((i == j) ? T(1) : T(0));
+        }
+      return;
+    }
+  }
+  load_F(q, ev, phi, F);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) G[k] = F[k] - ((k % 4 == 0) ? T(1) : T(0));
+}
+
 template <int MODEL>
 __device__ __forceinline__ int model_at(const uint8_t* __restrict__ mids, int e) {
   return MODEL == M_PER_TET ? (int)mids[e] : MODEL;
@@ -238,14 +275,15 @@ __device__ __forceinline__ void select_row(const T phi[4][3], int a, T* pa) {
 template <int MODEL, typename T>
 __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __restrict__ q,
                                              const T* __restrict__ params, const uint8_t* __restrict__ mids, int e) {
-  T phi[4][3], F[9];
+  T phi[4][3], F[9], G[9];
   load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, F);
+  load_FG(m, q, load_E(m, e), phi, F, G);
   const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  const T* Gp = (sizeof(T) == 4) ? G : nullptr;
   double v;
   switch (model_at<MODEL>(mids, e)) {
-    case M_NH: v = psi<M_NH>(F, p0, p1); break;
-    case M_STVK: v = psi<M_STVK>(F, p0, p1); break;
+    case M_NH: v = psi<M_NH>(F, p0, p1, Gp); break;
+    case M_STVK: v = psi<M_STVK>(F, p0, p1, Gp); break;
     case M_COROT: v = psi<M_COROT>(F, p0, p1); break;
     case M_ARAP: v = psi<M_ARAP>(F, p0, p1); break;
     default: v = psi<M_NH_LEGACY>(F, p0, p1); break;
@@ -306,13 +344,14 @@ __global__ void __launch_bounds__(256)
     const int code = m.incE[L.first + 64ll * k];
     if (code < 0) break;
     const int e = code >> 2, a = code & 3;
-    T phi[4][3], F[9], P[9];
+    T phi[4][3], F[9], G[9], P[9];
     load_phi(m, e, phi);
-    load_F(q, load_E(m, e), phi, F);
+    load_FG(m, q, load_E(m, e), phi, F, G);
     const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+    const T* Gp = (sizeof(T) == 4) ? G : nullptr;
     switch (model_at<MODEL>(mids, e)) {
-      case M_NH: piola<M_NH>(F, p0, p1, P); break;
-      case M_STVK: piola<M_STVK>(F, p0, p1, P); break;
+      case M_NH: piola<M_NH>(F, p0, p1, P, Gp); break;
+      case M_STVK: piola<M_STVK>(F, p0, p1, P, Gp); break;
       case M_COROT: piola<M_COROT>(F, p0, p1, P); break;
       case M_ARAP: piola<M_ARAP>(F, p0, p1, P); break;
       default: piola<M_NH_LEGACY>(F, p0, p1, P); break;
@@ -408,7 +447,10 @@ __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __re
   T phi[4][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, P.F);
+  {
+    T Gh[9];
+    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+  }
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   T pa[3];
@@ -437,7 +479,10 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
   T phi[4][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, P.F);
+  {
+    T Gh[9];
+    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+  }
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   T pa[3];
@@ -600,7 +645,10 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
   T phi[4][3], Q[4][3], R[3][3];
   Pre<T> P;
   load_phi(m, e, phi);
-  load_F(q, load_E(m, e), phi, P.F);
+  {
+    T Gh[9];
+    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+  }
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
@@ -1047,7 +1095,7 @@ void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdCount, m->psdDM, m->ws, m->dX32, m->vol32, m->psdDM32};
+                  m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -1327,6 +1375,21 @@ int bh_mesh_psd_count(bh_mesh* m, int64_t* count) {
   BH_CHECK(hipDeviceSynchronize());
   BH_CHECK(hipMemcpy(&h, m->psdCount, sizeof(int32_t), hipMemcpyDeviceToHost));
   *count = h;
+  return BH_OK;
+}
+
+int bh_mesh_set_rest_positions(bh_mesh* m, const double* V) {
+  if (!m) return BH_BAD_ARG;
+  if (!V) {
+    if (m->X) BH_CHECK(hipFree(m->X));
+    m->X = nullptr;
+    return BH_OK;
+  }
+  std::vector<double> h(3 * (size_t)m->nV);
+  for (int v = 0; v < m->nV; ++v)
+    for (int c = 0; c < 3; ++c) h[3 * (size_t)v + c] = V[v + (size_t)c * m->nV];
+  if (!m->X) BH_CHECK(hipMalloc(&m->X, sizeof(double) * std::max<size_t>(3, h.size())));
+  if (!h.empty()) BH_CHECK(hipMemcpy(m->X, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   return BH_OK;
 }
 

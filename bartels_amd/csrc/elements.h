@@ -389,9 +389,9 @@ __host__ __device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa
 // from the displacement gradient G = F - I (exact in FP32 near I) and J - 1 in FP64, so the
 // energy keeps its relative accuracy at small strains (the line search compares energy sums).
 template <int MODEL, typename T>
-__host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1) {
+__host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1, const T* Gin = nullptr) {
   if (MODEL == M_NH) {
-    if constexpr (sizeof(T) == 8) {
+    if (sizeof(T) == 8 && !Gin) {
       double I1 = 0.0;
 #pragma unroll
       for (int k = 0; k < 9; ++k) I1 += (double)F[k] * (double)F[k];
@@ -400,7 +400,7 @@ __host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1)
     } else {
       double G[9];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) G[k] = (double)(F[k] - ((k % 4 == 0) ? T(1) : T(0)));
+      for (int k = 0; k < 9; ++k) G[k] = Gin ? (double)Gin[k] : (double)(F[k] - ((k % 4 == 0) ? T(1) : T(0)));
       const double trG = G[0] + G[4] + G[8];
       double GG = 0.0;
 #pragma unroll
@@ -423,9 +423,15 @@ __host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        E[i + 3 * j] = 0.5 * ((double)F[3 * i] * F[3 * j] + (double)F[3 * i + 1] * F[3 * j + 1] +
-                              (double)F[3 * i + 2] * F[3 * j + 2] - ((i == j) ? 1.0 : 0.0));
+      for (int j = 0; j < 3; ++j) {
+        if (Gin)  // E = (G + G^T + G^T G) / 2: no cancellation at small strain
+          E[i + 3 * j] = 0.5 * ((double)Gin[3 * i + j] + (double)Gin[3 * j + i] +
+                                (double)Gin[3 * i] * Gin[3 * j] + (double)Gin[3 * i + 1] * Gin[3 * j + 1] +
+                                (double)Gin[3 * i + 2] * Gin[3 * j + 2]);
+        else
+          E[i + 3 * j] = 0.5 * ((double)F[3 * i] * F[3 * j] + (double)F[3 * i + 1] * F[3 * j + 1] +
+                                (double)F[3 * i + 2] * F[3 * j + 2] - ((i == j) ? 1.0 : 0.0));
+      }
     double trEE = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) trEE += E[k] * E[k];  // E symmetric: tr(E E) = sum E_ij^2
@@ -447,8 +453,30 @@ __host__ __device__ __forceinline__ double psi(const T* F, double p0, double p1)
 // Stretch models follow src/dpsi_stretch_dF.cpp:20-42: when s0 s1 s2 <= -1e-10 the negative
 // singular values are flipped positive (U, V unchanged).
 template <int MODEL, typename T>
-__host__ __device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
-  if (MODEL == M_NH) {
+__host__ __device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P, const T* G = nullptr) {
+  if (MODEL == M_NH && G && sizeof(T) == 4) {
+    // from G = F - I (cancellation-free at small strain):
+    //   cof F = (1 + tr G) I - G^T + cof G,  J = 1 + tr G + e2 + det G (e2 = tr cof G),
+    //   J F - cof F = (e2 + det G) I + (1 + tr G + e2 + det G) G + G^T - cof G,
+    //   P = 2C (J F - cof F) / J + 2D (J - 1) cof F
+    double Gd[9], cg[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Gd[k] = (double)G[k];
+    cof3(Gd, cg);
+    const double trG = Gd[0] + Gd[4] + Gd[8];
+    const double e2 = cg[0] + cg[4] + cg[8];
+    const double dG = Gd[0] * cg[0] + Gd[3] * cg[3] + Gd[6] * cg[6];
+    const double Jm1 = trG + e2 + dG, J = 1.0 + Jm1, iJ = 1.0 / J;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int k = i + 3 * j, kt = j + 3 * i;
+        const double cofF = ((i == j) ? 1.0 + trG : 0.0) - Gd[kt] + cg[k];
+        const double dev = ((i == j) ? e2 + dG : 0.0) + (1.0 + Jm1) * Gd[k] + Gd[kt] - cg[k];
+        P[k] = (T)(2.0 * p0 * dev * iJ + 2.0 * p1 * Jm1 * cofF);
+      }
+  } else if (MODEL == M_NH) {
     T c[9];
     double J;
     if constexpr (sizeof(T) == 8) {
@@ -480,8 +508,10 @@ __host__ __device__ __forceinline__ void piola(const T* F, T p0, T p1, T* P) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        S[i + 3 * j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2] -
-                       ((i == j) ? T(1) : T(0));  // 2E
+        S[i + 3 * j] = G ? G[3 * i + j] + G[3 * j + i] + G[3 * i] * G[3 * j] + G[3 * i + 1] * G[3 * j + 1] +
+                               G[3 * i + 2] * G[3 * j + 2]
+                         : F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2] -
+                               ((i == j) ? T(1) : T(0));  // 2E
     const T trE = T(0.5) * (S[0] + S[4] + S[8]);
 #pragma unroll
     for (int k = 0; k < 9; ++k) S[k] *= p1;  // 2 mu E

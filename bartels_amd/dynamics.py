@@ -24,7 +24,7 @@ class BackwardEuler:
         self.p = p
         nV = p.V.shape[0]
         dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-        self.mesh = tm.TetMesh(nV, p.E, dX, vol)
+        self.mesh = tm.TetMesh(nV, p.E, dX, vol, V=p.V)
         dev = torch.device("cuda:0")
         fixed = np.zeros(3 * nV, np.uint8)
         fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1

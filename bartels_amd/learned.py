@@ -63,7 +63,7 @@ class Solver:
         self.p = p
         nV = p.V.shape[0]
         dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-        self.mesh = tm.TetMesh(nV, p.E, dX, vol)
+        self.mesh = tm.TetMesh(nV, p.E, dX, vol, V=p.V)
         self.P = ctypes.c_void_p()
         _lib.check(self.lib.bh_precond_create(self.mesh.handle, ctypes.byref(self.P)), "bh_precond_create")
         dev = torch.device("cuda:0")

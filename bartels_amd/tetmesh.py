@@ -63,7 +63,9 @@ def linear_tetmesh_dphi_dX(V, E):
 class TetMesh:
     """Per-mesh device state (pattern, incidence lists, element geometry), built once."""
 
-    def __init__(self, nV, E, dphidX, volume):
+    def __init__(self, nV, E, dphidX, volume, V=None):
+        """V (rest positions, nV x 3), optional: lets the FP32 precision form evaluate stresses from
+        displacement differences (bh_mesh_set_rest_positions)."""
         lib = _lib.load()
         E = _colmajor(E, np.int32)
         dphidX = _colmajor(dphidX, np.float64)
@@ -78,6 +80,9 @@ class TetMesh:
         check(lib.bh_mesh_nnz(h, ctypes.byref(nnz)), "bh_mesh_nnz")
         self.nnz = int(nnz.value)
         self._pattern = None
+        if V is not None:
+            Vc = _colmajor(np.asarray(V, dtype=np.float64).reshape(self.nV, 3), np.float64)
+            check(lib.bh_mesh_set_rest_positions(h, _ptr(Vc)), "bh_mesh_set_rest_positions")
 
     @property
     def handle(self):

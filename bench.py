@@ -168,7 +168,7 @@ def main():
     t_setup = time.perf_counter()
     p = meshgen.config2(grid=grid)
     dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol)
+    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol, V=p.V)
     nV, nT, nnz = mesh.nV, mesh.nT, mesh.nnz
     dev = torch.device("cuda:0")
     q = torch.tensor(p.q, device=dev)

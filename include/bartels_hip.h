@@ -94,6 +94,13 @@ int bh_mesh_pattern(const bh_mesh* mesh, int32_t* rowptr, int32_t* colind);
 int bh_mesh_block_pattern_device(const bh_mesh* mesh, const int32_t** d_bptr, const int32_t** d_bcol,
                                  int64_t* nblocks);
 int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
+/* Rest positions V (host, nV x 3 column-major; NULL clears).  Optional: the FP32 precision form
+ * then forms G = F - I from displacement edges in FP64 and evaluates the Neo-Hookean / StVK energy
+ * and stress from G, keeping their relative accuracy at small strain (without V, F is rounded to
+ * FP32 first and the stress carries an absolute error of about 1e-7 of the modulus).  No
+ * counterpart in the reference (its V argument is passed on every call and unused by the
+ * element math). */
+int bh_mesh_set_rest_positions(bh_mesh* mesh, const double* V);
 /* Assembly-kernel shape: 1 or 2 lanes per vertex (with 2, each lane owns two of the four corner
  * blocks of every incident tet).  Same per-slot fold order. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);

@@ -49,6 +49,30 @@ static void grad(const double* phi12, const double* x12, double vol, double p0, 
   }
 }
 
+// FP32 stress / energy from G = F - I formed from displacement edges (rest positions X12)
+template <int MODEL, typename T>
+static void grad_g(const double* phi12, const double* x12, const double* X12, double vol, double p0, double p1,
+                   double* g) {
+  T phi[4][3], F[9], G[9], P[9];
+  for (int m = 0; m < 4; ++m)
+    for (int j = 0; j < 3; ++j) phi[m][j] = (T)phi12[m + 4 * j];
+  T ed[3][3];
+  for (int m = 0; m < 3; ++m)
+    for (int i = 0; i < 3; ++i)
+      ed[m][i] = (T)((x12[3 * (m + 1) + i] - X12[3 * (m + 1) + i]) - (x12[i] - X12[i]));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      G[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+      F[i + 3 * j] = G[i + 3 * j] + ((i == j) ? T(1) : T(0));
+    }
+  piola<MODEL>(F, (T)p0, (T)p1, P, G);
+  for (int a = 0; a < 4; ++a) {
+    T f[3];
+    mv3(P, phi[a], f);
+    for (int l = 0; l < 3; ++l) g[3 * a + l] = (double)((T)vol * f[l]);
+  }
+}
+
 template <int MODEL, typename T>
 static double energy(const double* phi12, const double* x12, double vol, double p0, double p1) {
   T phi[4][3], F[9];
@@ -131,6 +155,13 @@ void eh_eig9(int fp32, const double* A, double* d, double* Z) {
     eig9<float>(A, d, Z);
   else
     eig9<double>(A, d, Z);
+}
+void eh_grad_g(int model, const double* phi12, const double* x12, const double* X12, double vol, double p0,
+               double p1, double* g) {
+  if (model == 0)
+    grad_g<0, float>(phi12, x12, X12, vol, p0, p1, g);
+  else
+    grad_g<1, float>(phi12, x12, X12, vol, p0, p1, g);
 }
 double eh_energy(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1) {
   DISPATCH(energy, phi12, x12, vol, p0, p1)

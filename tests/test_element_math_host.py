@@ -32,6 +32,7 @@ def eh(tmp_path_factory):
     L.eh_psd_blocks.argtypes = [i, i, vp, vp, d, d, d, i, d, vp]
     L.eh_psd_blocks.restype = i
     L.eh_eig9.argtypes = [i, vp, vp, vp]
+    L.eh_grad_g.argtypes = [i, vp, vp, vp, d, d, d, vp]
     return L
 
 
@@ -187,3 +188,34 @@ def test_sym9_eigen_against_closed_forms_and_eigh(eh):
         d32, Z32 = _eig9(eh, A, fp32=1)
         assert np.abs(np.sort(d32) - np.linalg.eigvalsh(A)).max() <= 3e-6 * nA + 1e-30
         assert np.abs(Z32.T @ Z32 - np.eye(9)).max() < 3e-6
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_fp32_stress_from_displacements_small_strain(eh, oracle, model):
+    """the precision form's small-strain path: G = F - I from displacement edges (FP64), stress from G
+    in FP32.  At strain 1e-5 its gradient stays within 1e-5 of the FP64 oracle, where the path through
+    F rounded to FP32 loses ~1e-2"""
+    rng = np.random.default_rng(21 + model)
+    worst_g, worst_f = 0.0, 0.0
+    for _ in range(20):
+        base = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])  # positive signed volume
+        X = 3.0 + 0.02 * (base + 0.1 * rng.standard_normal((4, 3)))
+        x = X + 1e-5 * 0.02 * rng.standard_normal((4, 3))
+        lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+        p = meshgen.model_params(np.array([model]), lam, mu)[0]
+        Ex = np.array([[0, 1, 2, 3]], dtype=np.int32)
+        dX, vol = oracle.dphidX_volume(X, Ex)
+        assert vol[0] > 0
+        q = np.ascontiguousarray(x.ravel())
+        g_or = oracle.tetmesh_dq(model, 4, Ex, q, dX, vol, p[None, :])
+        phi = np.ascontiguousarray(dX[0])
+        Xr = np.ascontiguousarray(X.ravel())
+        g = np.zeros(12)
+        eh.eh_grad_g(model, phi.ctypes.data, q.ctypes.data, Xr.ctypes.data, vol[0], p[0], p[1], g.ctypes.data)
+        gf = np.zeros(12)
+        eh.eh_grad(model, 1, phi.ctypes.data, q.ctypes.data, vol[0], p[0], p[1], gf.ctypes.data)
+        n = np.linalg.norm(g_or)
+        worst_g = max(worst_g, np.linalg.norm(g - g_or) / n)
+        worst_f = max(worst_f, np.linalg.norm(gf - g_or) / n)
+    print(f"model {model}: FP32 gradient error at strain 1e-5, from G {worst_g:.1e}, from F {worst_f:.1e}")
+    assert worst_g < 1e-5

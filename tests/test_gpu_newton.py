@@ -107,7 +107,7 @@ def gravity_problem(oracle):
     p = meshgen.config1()
     nV = p.V.shape[0]
     dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-    mesh = tm.TetMesh(nV, p.E, dX, vol)
+    mesh = tm.TetMesh(nV, p.E, dX, vol, V=p.V)
     fixed = np.zeros(3 * nV, np.uint8)
     fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1
     q0 = p.V.ravel().copy()
@@ -236,7 +236,7 @@ def test_backward_euler_steps_match_oracle(oracle):
     p = meshgen.config1(grid=(12, 4, 4))
     nV = p.V.shape[0]
     dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-    mesh = tm.TetMesh(nV, p.E, dX, vol)
+    mesh = tm.TetMesh(nV, p.E, dX, vol, V=p.V)
     fixed = np.zeros(3 * nV, np.uint8)
     fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1
     f = oracle.const_ext_force(nV, p.E, vol, p.density * p.gravity)
@@ -251,7 +251,9 @@ def test_backward_euler_steps_match_oracle(oracle):
     df = torch.tensor(f, device="cuda")
     x_or = p.V.ravel().copy()
     v_or = np.zeros_like(x_or)
-    for prec, bound in ((0, 1e-8), (1, 1e-5)):
+    # FP64: the oracle's energy-based Armijo test stalls near 1e-8 relative (rounding of the energy
+    # differences), which sets the agreement floor
+    for prec, bound in ((0, 1e-7), (1, 1e-5)):
         x = p.V.ravel().copy()
         v = np.zeros_like(x)
         xo, vo = x_or.copy(), v_or.copy()
@@ -263,7 +265,7 @@ def test_backward_euler_steps_match_oracle(oracle):
             o = _lib.BhNewtonOpts()
             lib.bh_newton_default_opts(ctypes.byref(o))
             o.model = 0
-            o.gradient_tol = 1e-10 if prec == 0 else 1e-4
+            o.gradient_tol = 1e-10  # the FP32 form stops at max_iterations (its gradient floor is higher)
             o.relative_tol = 1
             o.max_iterations = 8
             o.pcg_tol = 1e-12

@@ -21,7 +21,7 @@ def test_precision_form_mixed(oracle):
 
     p = meshgen.config3(grid=(40, 12, 12))
     dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
-    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol)
+    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol, V=p.V)
     nV = p.V.shape[0]
     dev = "cuda"
     q = torch.tensor(p.q, device=dev)
@@ -53,3 +53,39 @@ def test_precision_form_mixed(oracle):
     eP = sp.linalg.norm(Hpg - Hp_or) / sp.linalg.norm(Hp_or)
     print(f"precision form PSD Hessian vs FP64 oracle: {eP:.2e}")
     assert eP < 5e-5
+
+
+def test_precision_form_small_strain(oracle):
+    """At strain ~1e-4 the FP32 stress computed from F = I + G loses relative accuracy to the
+    rounding of F near I; with rest positions the form evaluates it from G (displacement edges in
+    FP64), which restores it: gradient and energy against the FP64 oracle, Neo-Hookean and StVK"""
+    import torch
+
+    from bartels_amd import tetmesh as tm
+
+    V, E = meshgen.beam_grid(16, 6, 6, h=0.025)
+    V = V + 3.0  # metre-scale absolute coordinates
+    rng = np.random.default_rng(8)
+    nV = V.shape[0]
+    q = (V + 1e-4 * 0.025 * rng.standard_normal(V.shape)).ravel()
+    dX, vol = tm.linear_tetmesh_dphi_dX(V, E)
+    lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+    dq = torch.tensor(q, device="cuda")
+    for model in (0, 1):
+        params = meshgen.model_params(np.full(E.shape[0], model, np.uint8), lam, mu)
+        par32 = torch.tensor(np.asfortranarray(params).ravel(order="F"), device="cuda", dtype=torch.float32)
+        g_or = oracle.tetmesh_dq(model, nV, E, q, dX, vol, params)
+        e_or = oracle.tetmesh_q(model, nV, E, q, dX, vol, params)
+        errs = {}
+        for name, VV in (("F", None), ("G", V)):
+            mesh = tm.TetMesh(nV, E, dX, vol, V=VV)
+            g = torch.empty(3 * nV, dtype=torch.float64, device="cuda")
+            e = torch.empty(1, dtype=torch.float64, device="cuda")
+            mesh.dq_f32_device(model, dq, par32, g)
+            mesh.q_f32_device(model, dq, par32, e)
+            torch.cuda.synchronize()
+            errs[name] = (np.linalg.norm(g.cpu().numpy() - g_or) / np.linalg.norm(g_or), abs(e.item() - e_or) / e_or)
+            mesh.close()
+        print(f"model {model}, strain 1e-4: gradient / energy error from F {errs['F'][0]:.1e} / {errs['F'][1]:.1e}, "
+              f"from G {errs['G'][0]:.1e} / {errs['G'][1]:.1e}")
+        assert errs["G"][0] < 1e-5 and errs["G"][1] < 1e-5

@@ -253,3 +253,26 @@ def test_newton_small_beam_converges(oracle):
     # quadratic convergence at the end
     gn = trace[:, 1]
     assert gn[-1] < 1e-3 * gn[-3]
+
+
+def test_backward_euler_free_fall_translation(oracle):
+    """Pins the oracle's backward-Euler inertia term (scale 1/dt^2, sign, mass): an unconstrained body at
+    rest under the uniform load f = M g moves as a rigid translation by exactly dt^2 g in one step
+    (elastic energy and gradient vanish on translations), and a second step continues with velocity
+    dt g (x2 - x1 = 2 dt^2 g)"""
+    V, E = meshgen.beam_grid(5, 3, 3, h=0.05)
+    nV = V.shape[0]
+    dX, vol = oracle.dphidX_volume(V, E)
+    lam, mu = meshgen.emu_to_lame(1e6, 0.4)
+    params = meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu)
+    rho, dt = 1000.0, 0.01
+    g = np.array([0.3, -1.1, -9.8])
+    M = oracle.mass_matrix(nV, E, np.full(E.shape[0], rho), vol)
+    f = M @ np.tile(g, nV)
+    x0 = V.ravel().copy()
+    x1, it, conv, _ = oracle.newton_be(0, nV, E, dX, vol, params, x0, x0, dt, rho, None, f, tol=1e-12, max_it=5)
+    assert np.abs(x1 - x0 - dt * dt * np.tile(g, nV)).max() < 1e-12
+    v1 = (x1 - x0) / dt
+    qp = x1 + dt * v1
+    x2, _, _, _ = oracle.newton_be(0, nV, E, dX, vol, params, qp, qp, dt, rho, None, f, tol=1e-12, max_it=5)
+    assert np.abs(x2 - x1 - 2 * dt * dt * np.tile(g, nV)).max() < 1e-12
