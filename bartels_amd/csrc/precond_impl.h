@@ -29,7 +29,8 @@ struct bh_precond {
   float* dinv = nullptr;        // 6 per vertex: inverse of L_vv (lower triangular, row-major packed)
   int32_t* fail = nullptr;      // factorisation breakdown counter
   int64_t nlower = 0;
-  double* ytmp = nullptr;       // forward-sweep result (3 nV)
+  float* ytmp = nullptr;        // forward-sweep result (3 nV)
+  float* ztmp = nullptr;        // FP32 shadow of the backward-sweep result (3 nV)
 };
 
 namespace {
@@ -41,36 +42,33 @@ __device__ __forceinline__ void ltri_mv(const float* L, const double* x, double*
   y[1] = (double)L[1] * x[0] + (double)L[2] * x[1];
   y[2] = (double)L[3] * x[0] + (double)L[4] * x[1] + (double)L[5] * x[2];
 }
-__device__ __forceinline__ void ltri_tmv(const float* L, const double* x, double* y) {  // L^T x
-  y[0] = (double)L[0] * x[0] + (double)L[1] * x[1] + (double)L[3] * x[2];
-  y[1] = (double)L[2] * x[1] + (double)L[4] * x[2];
-  y[2] = (double)L[5] * x[2];
-}
 
 // Sweeps, 8 lanes per vertex: lane j takes list entries j, j+8, ... (a vertex's list is contiguous,
 // so the lanes read neighbouring 36-byte blocks), the three partial sums are reduced across the 8
-// lanes with a fixed xor butterfly, and lane 0 applies the diagonal inverse.
+// lanes with a fixed xor butterfly, and lane 0 applies the diagonal inverse.  The factor is FP32
+// and so is the sweep arithmetic: the intermediate y and an FP32 shadow of z are what the gathers
+// read (12 bytes per neighbour); r comes in and z goes out in FP64 for the CG recurrences.
 constexpr int kSweepLanes = 8;
 
 // forward sweep over one colour: y_v = L_vv^{-1} (r_v - sum_n L_vn y_n)
 __global__ void __launch_bounds__(256)
     k_sweep_fwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ lptr,
                 const int32_t* __restrict__ lcol, const float* __restrict__ lval, const float* __restrict__ dinv,
-                const double* __restrict__ r, double* __restrict__ y, const int* __restrict__ done) {
+                const double* __restrict__ r, float* __restrict__ y, const int* __restrict__ done) {
   if (done && *done) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kSweepLanes, j = gid % kSweepLanes;
   int v = 0;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
   if (i < n) {
     v = verts[i];
     for (int k = lptr[v] + j; k < lptr[v + 1]; k += kSweepLanes) {
       const float* B = lval + 9ll * k;
-      const double* yn = y + 3ll * lcol[k];
-      const double y0 = yn[0], y1 = yn[1], y2 = yn[2];
-      s0 += (double)B[0] * y0 + (double)B[1] * y1 + (double)B[2] * y2;
-      s1 += (double)B[3] * y0 + (double)B[4] * y1 + (double)B[5] * y2;
-      s2 += (double)B[6] * y0 + (double)B[7] * y1 + (double)B[8] * y2;
+      const float* yn = y + 3ll * lcol[k];
+      const float y0 = yn[0], y1 = yn[1], y2 = yn[2];
+      s0 += B[0] * y0 + B[1] * y1 + B[2] * y2;
+      s1 += B[3] * y0 + B[4] * y1 + B[5] * y2;
+      s2 += B[6] * y0 + B[7] * y1 + B[8] * y2;
     }
   }
 #pragma unroll
@@ -80,12 +78,11 @@ __global__ void __launch_bounds__(256)
     s2 += __shfl_xor(s2, off, 64);
   }
   if (i < n && j == 0) {
-    const double t[3] = {r[3ll * v] - s0, r[3ll * v + 1] - s1, r[3ll * v + 2] - s2};
-    double o[3];
-    ltri_mv(dinv + 6ll * v, t, o);
-    y[3ll * v] = o[0];
-    y[3ll * v + 1] = o[1];
-    y[3ll * v + 2] = o[2];
+    const float* L = dinv + 6ll * v;
+    const float t0 = (float)r[3ll * v] - s0, t1 = (float)r[3ll * v + 1] - s1, t2 = (float)r[3ll * v + 2] - s2;
+    y[3ll * v] = L[0] * t0;
+    y[3ll * v + 1] = L[1] * t0 + L[2] * t1;
+    y[3ll * v + 2] = L[3] * t0 + L[4] * t1 + L[5] * t2;
   }
 }
 
@@ -94,21 +91,22 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_sweep_bwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ uptr,
                 const int32_t* __restrict__ ucol, const float* __restrict__ uval, const float* __restrict__ dinv,
-                const double* __restrict__ y, double* __restrict__ z, const int* __restrict__ done) {
+                const float* __restrict__ y, float* __restrict__ z32, double* __restrict__ z,
+                const int* __restrict__ done) {
   if (done && *done) return;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kSweepLanes, j = gid % kSweepLanes;
   int v = 0;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
   if (i < n) {
     v = verts[i];
     for (int k = uptr[v] + j; k < uptr[v + 1]; k += kSweepLanes) {
       const float* B = uval + 9ll * k;  // L_nv, use its transpose
-      const double* zn = z + 3ll * ucol[k];
-      const double z0 = zn[0], z1 = zn[1], z2 = zn[2];
-      s0 += (double)B[0] * z0 + (double)B[3] * z1 + (double)B[6] * z2;
-      s1 += (double)B[1] * z0 + (double)B[4] * z1 + (double)B[7] * z2;
-      s2 += (double)B[2] * z0 + (double)B[5] * z1 + (double)B[8] * z2;
+      const float* zn = z32 + 3ll * ucol[k];
+      const float z0 = zn[0], z1 = zn[1], z2 = zn[2];
+      s0 += B[0] * z0 + B[3] * z1 + B[6] * z2;
+      s1 += B[1] * z0 + B[4] * z1 + B[7] * z2;
+      s2 += B[2] * z0 + B[5] * z1 + B[8] * z2;
     }
   }
 #pragma unroll
@@ -118,12 +116,18 @@ __global__ void __launch_bounds__(256)
     s2 += __shfl_xor(s2, off, 64);
   }
   if (i < n && j == 0) {
-    const double t[3] = {y[3ll * v] - s0, y[3ll * v + 1] - s1, y[3ll * v + 2] - s2};
-    double o[3];
-    ltri_tmv(dinv + 6ll * v, t, o);
-    z[3ll * v] = o[0];
-    z[3ll * v + 1] = o[1];
-    z[3ll * v + 2] = o[2];
+    const float* L = dinv + 6ll * v;
+    const float t0 = y[3ll * v] - s0, t1 = y[3ll * v + 1] - s1, t2 = y[3ll * v + 2] - s2;
+    // L^T t with L packed lower [l00, l10, l11, l20, l21, l22]
+    const float o0 = L[0] * t0 + L[1] * t1 + L[3] * t2;
+    const float o1 = L[2] * t1 + L[4] * t2;
+    const float o2 = L[5] * t2;
+    z32[3ll * v] = o0;
+    z32[3ll * v + 1] = o1;
+    z32[3ll * v + 2] = o2;
+    z[3ll * v] = o0;
+    z[3ll * v + 1] = o1;
+    z[3ll * v + 2] = o2;
   }
 }
 
@@ -325,13 +329,14 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   BH_CHECK(hipMalloc(&P->uval, sizeof(float) * 9 * std::max<int64_t>(1, P->nlower)));
   BH_CHECK(hipMalloc(&P->dinv, sizeof(float) * 6 * std::max(1, nV)));
   BH_CHECK(hipMalloc(&P->fail, sizeof(int32_t)));
-  BH_CHECK(hipMalloc(&P->ytmp, sizeof(double) * 3 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->ytmp, sizeof(float) * 3 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->ztmp, sizeof(float) * 3 * std::max(1, nV)));
   return BH_OK;
 }
 
 void precond_free(bh_precond* P) {
   void* ptrs[] = {P->colVerts, P->lptr, P->lcol, P->lslot, P->uptr, P->ucol, P->ulink, P->dslot,
-                  P->lval, P->uval, P->dinv, P->fail, P->ytmp};
+                  P->lval, P->uval, P->dinv, P->fail, P->ytmp, P->ztmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -357,7 +362,7 @@ int precond_apply(const bh_precond* P, const double* r, double* z, const int* do
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
     hipLaunchKernelGGL(k_sweep_bwd, dim3((kSweepLanes * n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c],
-                       n, P->uptr, P->ucol, P->uval, P->dinv, P->ytmp, z, done);
+                       n, P->uptr, P->ucol, P->uval, P->dinv, P->ytmp, P->ztmp, z, done);
   }
   BH_CHECK(hipGetLastError());
   return BH_OK;

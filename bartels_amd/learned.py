@@ -194,19 +194,21 @@ def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, log=pri
     return history
 
 
-def evaluate(weights, problems=10, seed=777, newton_its=3, log=print):
-    """held-out 32x32x128 problems: PCG iterations and time to 1e-6 per Newton step, block-Jacobi PCG
-    versus learned PCG (both in the precision form)"""
+def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, log=print):
+    """held-out 32x32x128 problems: PCG iterations and time to 1e-6 over the first Newton steps,
+    block-Jacobi PCG versus learned PCG versus block IC(0) (all in the precision form)"""
     net = Network(weights=weights)
     rows = []
     for i in range(problems):
         p = random_problem((128, 32, 32), seed=90000 + seed + 13 * i)
         S = Solver(p)
-        _, rj, _, wj = S.newton(0, precision=1, max_it=newton_its)
-        _, rl, _, wl = S.newton(4, mgn=net.h, precision=1, max_it=newton_its)
+        _, rj, _, wj = S.newton(0, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        _, rl, _, wl = S.newton(4, mgn=net.h, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        _, rb, _, wb = S.newton(2, precision=1, max_it=newton_its, pcg_max=pcg_max)
         S.close()
         rows.append(dict(problem=i, jacobi_pcg=rj.pcg_iterations_total, learned_pcg=rl.pcg_iterations_total,
-                         jacobi_s=wj, learned_s=wl, fallbacks=rl.pcg_fallbacks))
+                         ic0_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, ic0_s=wb,
+                         fallbacks=rl.pcg_fallbacks))
         log(json.dumps(rows[-1]))
     net.close()
     return rows
@@ -233,7 +235,9 @@ def main():
         rows = evaluate(args.weights, args.problems, log=log)
         j = np.array([r["jacobi_pcg"] for r in rows], float)
         l = np.array([r["learned_pcg"] for r in rows], float)
-        log(json.dumps(dict(mean_jacobi_pcg=j.mean(), mean_learned_pcg=l.mean(), ratio=(j / l).mean())))
+        b = np.array([r["ic0_pcg"] for r in rows], float)
+        log(json.dumps(dict(mean_jacobi_pcg=j.mean(), mean_learned_pcg=l.mean(), mean_ic0_pcg=b.mean(),
+                            jacobi_over_learned=(j / l).mean(), jacobi_over_ic0=(j / b).mean())))
 
 
 if __name__ == "__main__":

@@ -90,7 +90,7 @@ def newton_steps(p, mesh, steps, pcg_max):
     P = ctypes.c_void_p()
     _lib.check(lib.bh_precond_create(mesh.handle, ctypes.byref(P)), "bh_precond_create")
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
-    variants = [("fp64_jacobi", 0, 0), ("fp32_bic0", 1, 2)]
+    variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2)]
     if net is not None:
         variants.append(("fp32_learned", 1, 4))
     res, qs = {}, {}
