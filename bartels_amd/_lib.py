@@ -33,8 +33,9 @@ class BhElementOpts(ctypes.Structure):
 
 
 class BhTuning(ctypes.Structure):
-    """kernel-schedule knobs of a mesh (gather lanes, XCD grouping); same per-slot fold order"""
-    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int)]
+    """schedule knobs of a mesh: gather lanes and XCD grouping (same per-slot fold order), and the vertex
+    colouring of the colour sweeps (0 keep, 1 DSatur, 2 greedy)"""
+    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int), ("coloring", c_int)]
 
 
 # void (*on_system)(void* user, int iteration, const float* vals32, const double* vals64, const double* b,

@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <queue>
 #include <vector>
 
 #include "../../include/bartels_hip.h"
@@ -67,9 +68,12 @@ struct bh_mesh {
   uint32_t* incR = nullptr;
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
+  int32_t* vdeg = nullptr;  // tets incident to each vertex
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
   int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
+  int coloring = 1;                               // colour sweeps: 1 DSatur, 2 greedy in vertex order
+  bh_precond* solveOrder = nullptr;               // colour lists for the precision form's block Jacobi
   // scratch
   double* partials = nullptr;
   int nPartials = 0;
@@ -1077,6 +1081,11 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
       (st = upload(&m->incR, incR.data(), incR.size())) || (st = upload(&m->bptr, bptr.data(), bptr.size())) ||
       (st = upload(&m->bcol, bcol.data(), bcol.size())))
     return st;
+  {
+    std::vector<int32_t> vdeg(std::max(1, nV), 0);
+    for (int v = 0; v < nV; ++v) vdeg[v] = incPtr[v + 1] - incPtr[v];
+    if ((st = upload(&m->vdeg, vdeg.data(), vdeg.size()))) return st;
+  }
   const int per = kEnergyBlock * kEnergyItems;
   m->nPartials = std::max(1, (nT + per - 1) / per);
   BH_CHECK(hipMalloc(&m->partials, sizeof(double) * m->nPartials));
@@ -1093,7 +1102,7 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
 
 void free_mesh(bh_mesh* m) {
   clear_profile(m);
-  void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->partials,
+  void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->vdeg, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
                   m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32};
   for (void* p : ptrs)
@@ -1173,6 +1182,7 @@ int bh_mesh_create(int nV, int nT, const int* E, const double* dphidX, const dou
 int bh_mesh_destroy(bh_mesh* m) {
   if (!m) return BH_BAD_ARG;
   (void)hipDeviceSynchronize();
+  if (m->solveOrder) bh_precond_destroy(m->solveOrder);
   free_mesh(m);
   delete m;
   return BH_OK;
@@ -1403,6 +1413,14 @@ int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (!m || !t) return BH_BAD_ARG;
   if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2) return BH_BAD_ARG;
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
+  if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
+  if (t->coloring && t->coloring != m->coloring) {
+    m->coloring = t->coloring;
+    if (m->solveOrder) {  // made with the old colouring
+      bh_precond_destroy(m->solveOrder);
+      m->solveOrder = nullptr;
+    }
+  }
   if (t->gather_lanes) m->gatherLanes = t->gather_lanes;
   if (t->xcd_groups >= 0) m->xcdGroups = t->xcd_groups;
   return BH_OK;

@@ -7,47 +7,74 @@ constexpr int kRedBlock = 256;
 constexpr int kRedMaxBlocks = 1024;
 constexpr int kPcgBatch = 16;
 
+// Device-resident CG scalars (Chronopoulos-Gear form: one reduction of r.u, r.r and w.u per
+// iteration).  done: 0 running, 1 converged, 2 iteration cap, 3 non-positive curvature.
 struct PcgState {
-  double rz, pAp, alpha, beta, rr, bb, tol2;
-  int done;  // 0 running, 1 converged, 2 iteration cap, 3 non-positive curvature
-  int iters, maxit;
+  double gamma, delta, alpha, beta, rr, bb, tol2;
+  int done, iters, maxit;
 };
 
-__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+// kernels of an iteration stop once the solve is over, except for the one update that applies
+// alpha = 1 after non-positive curvature in the first iteration
+__device__ __forceinline__ bool cg_skip(const PcgState* st) {
+  const int d = st->done;
+  return d != 0 && !(d == 3 && st->alpha != 0.0);
+}
+
+// sum over a block of NT threads (NT a multiple of 64), fixed order; the result is valid in thread 0
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = 0.0;
-  if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  }
   __syncthreads();
   return r;
 }
+__device__ __forceinline__ double block_sum_256(double v, double* sh) { return block_sum<256>(v, sh); }
 
-// Ap = mask(H p), partial p.Ap.  16 lanes per vertex: lane j takes blocks j, j+16, ... of the
-// vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
-// neighbouring 24-byte pieces), then the three row sums are reduced across the 16 lanes with a fixed
-// xor butterfly.  FP32 values are widened; products are summed in FP64.
+// Ap = mask(H p), partial p.Ap per block.  16 lanes per vertex: lane j takes blocks j, j+16, ... of
+// the vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
+// neighbouring 3-value pieces), then the three row sums are reduced across the 16 lanes with a fixed
+// xor butterfly.  FP32 values are widened; products are summed in FP64.  One pass per block (the
+// loads of a row are a dependent chain; many resident blocks hide it).  hdiag (optional, FP64, 9 per
+// vertex) replaces the stored diagonal blocks.
 constexpr int kSpmvLanes = 16;
-inline int spmv_blocks(int nV) { return (int)((kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock); }
+inline int spmv_blocks(int nV) {
+  return (int)std::max<int64_t>(1, (kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock);
+}
 
 template <typename V>
 __global__ void __launch_bounds__(kRedBlock)
     k_pcg_spmv(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
-               const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const double* __restrict__ p,
+               const V* __restrict__ vals, const double* __restrict__ hdiag, const uint8_t* __restrict__ fixed,
+               const double* __restrict__ p,
                double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
   __shared__ double sh[4];
-  if (st && st->done) return;
+  if (st && cg_skip(st)) return;
   const int64_t gid = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
   const int v = (int)(gid / kSpmvLanes), j = (int)(gid % kSpmvLanes);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (v < nV) {
     const int b0 = bptr[v], nb = bptr[v + 1] - b0;
-    const V* base = vals + 9ll * b0;
+    const V* row = vals + 9ll * b0;
     for (int r = j; r < nb; r += kSpmvLanes) {
-      const double* xp = p + 3ll * bcol[b0 + r];
+      const int c = bcol[b0 + r];
+      const double* xp = p + 3ll * c;
       const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
-      const V* r0 = base + 3 * r;
+      if (hdiag && c == v) {
+        const double* D = hdiag + 9ll * v;
+        s0 += D[0] * x0 + D[1] * x1 + D[2] * x2;
+        s1 += D[3] * x0 + D[4] * x1 + D[5] * x2;
+        s2 += D[6] * x0 + D[7] * x1 + D[8] * x2;
+        continue;
+      }
+      const V* r0 = row + 3 * r;
       const V* r1 = r0 + 3 * nb;
       const V* r2 = r0 + 6 * nb;
       s0 += (double)r0[0] * x0 + (double)r0[1] * x1 + (double)r0[2] * x2;
@@ -63,15 +90,17 @@ __global__ void __launch_bounds__(kRedBlock)
   }
   double pap = 0.0;
   if (v < nV && j < 3) {
-    const int64_t row = 3ll * v + j;
+    const int64_t rowi = 3ll * v + j;
     double sv = (j == 0) ? s0 : (j == 1) ? s1 : s2;
-    const double pr = p[row];
-    if (fixed && fixed[row]) sv = pr;
-    Ap[row] = sv;
+    const double pr = p[rowi];
+    if (fixed && fixed[rowi]) sv = pr;
+    Ap[rowi] = sv;
     pap = pr * sv;
   }
-  const double d = block_sum_256(pap, sh);
-  if (part && threadIdx.x == 0) part[blockIdx.x] = d;
+  if (part) {
+    const double d = block_sum_256(pap, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
 }
 
 }  // namespace

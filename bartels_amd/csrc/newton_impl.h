@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(256) k_mask_gradient(int64_t n, const uint8_t*
 template <typename V>
 __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __restrict__ bptr,
                                                       const int32_t* __restrict__ bcol, const V* __restrict__ vals,
+                                                      const double* __restrict__ hdiag,
                                                       const uint8_t* __restrict__ fixed, int point,
                                                       double* __restrict__ Dinv) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,7 +66,10 @@ __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __r
   int r = 0;
   while (r < nb && bcol[b0 + r] != v) ++r;
   double A[9];  // row-major block; a vertex that belongs to no tet gets the identity
-  if (r < nb) {
+  if (r < nb && hdiag) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = hdiag[9ll * v + i];
+  } else if (r < nb) {
     const V* base = vals + 9ll * b0 + 3ll * r;
 #pragma unroll
     for (int l = 0; l < 3; ++l)
@@ -109,221 +113,203 @@ __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __r
   o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
 }
 
-// x = 0, r = b, z = Dinv r, p = z, and partials of r.z and r.r
-__global__ void __launch_bounds__(kRedBlock) k_pcg_init(int nV, const double* __restrict__ b,
-                                                        const double* __restrict__ Dinv, double* __restrict__ x,
-                                                        double* __restrict__ r, double* __restrict__ z,
-                                                        double* __restrict__ p, double* __restrict__ part) {
-  __shared__ double sh[4];
-  const int v = blockIdx.x * kRedBlock + threadIdx.x;
-  double rz = 0.0, rr = 0.0;
-  if (v < nV) {
-    double rv[3], zv[3];
+// inverse of 3x3 blocks (row-major, 9 per block; point = 1 keeps only the diagonal); singular blocks
+// fall back to the inverse of their diagonal
+__device__ __forceinline__ void inv3_block(double* A, int point, double* o) {
+  if (point) A[1] = A[2] = A[3] = A[5] = A[6] = A[7] = 0.0;
+  const double c0 = A[4] * A[8] - A[5] * A[7], c1 = A[5] * A[6] - A[3] * A[8], c2 = A[3] * A[7] - A[4] * A[6];
+  const double det = A[0] * c0 + A[1] * c1 + A[2] * c2;
+  if (!(fabs(det) > 0.0) || !isfinite(det)) {
 #pragma unroll
-    for (int l = 0; l < 3; ++l) rv[l] = b[3ll * v + l];
-    const double* D = Dinv + 9ll * v;
-#pragma unroll
-    for (int l = 0; l < 3; ++l) zv[l] = D[3 * l] * rv[0] + D[3 * l + 1] * rv[1] + D[3 * l + 2] * rv[2];
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      x[3ll * v + l] = 0.0;
-      r[3ll * v + l] = rv[l];
-      z[3ll * v + l] = zv[l];
-      p[3ll * v + l] = zv[l];
-      rz += rv[l] * zv[l];
-      rr += rv[l] * rv[l];
-    }
+    for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? (A[i] != 0.0 ? 1.0 / A[i] : 1.0) : 0.0;
+    return;
   }
-  rz = block_sum_256(rz, sh);
-  rr = block_sum_256(rr, sh);
+  const double id = 1.0 / det;
+  o[0] = c0 * id;
+  o[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+  o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+  o[3] = c1 * id;
+  o[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+  o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+  o[6] = c2 * id;
+  o[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+  o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
+__global__ void __launch_bounds__(256) k_block_inv(int nV, const double* __restrict__ hd, int point,
+                                                   double* __restrict__ Dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  double A[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A[k] = hd[9ll * i + k];
+  inv3_block(A, point, Dinv + 9ll * i);
+}
+
+// ---- PCG in the Chronopoulos-Gear form -----------------------------------------------------------
+// (Chronopoulos & Gear 1989): with u = M^{-1} r, w = H u and s = H p carried by recurrence, the three
+// inner products an iteration needs (r.u, r.r, w.u) are all available after the product w = H u, so
+// one reduction per iteration suffices:
+//   beta = gamma' / gamma,  alpha = gamma' / (delta' - beta gamma' / alpha)
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,  u = M^{-1} r,  w = H u.
+// The vector recurrences are fused into the kernel that applies M^{-1} (block Jacobi: one pass; a
+// factor: the forward colour sweep), and the partial sums go to three fixed regions of the
+// workspace, reduced by one small kernel that also runs the convergence test.
+// partial-sum regions of the workspace (r.u, r.r, w.u), each large enough for one partial per block
+// of any kernel of the solve, and the second-stage partials after them
+inline int64_t part_region(int nV) { return spmv_blocks(nV) + 2048; }
+constexpr int kRed1Blocks = 64;
+
+// block-Jacobi step, one thread per DOF row (coalesced), 64 vertices per block (their three rows
+// meet in LDS for the 3x3 inverse): (start) x = 0, r = b, p = s = 0; (loop) the CG recurrences;
+// then u = Dinv r, partials r.u and r.r
+constexpr int kJacVerts = 64;
+__global__ void __launch_bounds__(3 * kJacVerts)
+    k_cg_vec_jacobi(int nV, int start, const double* __restrict__ b, const double* __restrict__ Dinv,
+                    double* __restrict__ x, double* __restrict__ r, double* __restrict__ u, double* __restrict__ p,
+                    double* __restrict__ s, const double* __restrict__ w, double* __restrict__ part_ru,
+                    double* __restrict__ part_rr, const PcgState* __restrict__ st) {
+  __shared__ double rs[3 * kJacVerts];
+  __shared__ double sh[4];
+  if (!start && cg_skip(st)) return;
+  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
+  const int64_t n = 3ll * nV;
+  double ri = 0.0;
+  if (i < n) {
+    if (start) {
+      x[i] = 0.0;
+      p[i] = 0.0;
+      s[i] = 0.0;
+      ri = b[i];
+    } else {
+      const double alpha = st->alpha, beta = st->beta;
+      const double pi = u[i] + beta * p[i];
+      const double si = w[i] + beta * s[i];
+      p[i] = pi;
+      s[i] = si;
+      x[i] += alpha * pi;
+      ri = r[i] - alpha * si;
+    }
+    r[i] = ri;
+  }
+  rs[threadIdx.x] = ri;
+  __syncthreads();
+  double ru = 0.0, rr = 0.0;
+  if (i < n) {
+    const int l = (int)(i % 3);
+    const int base = threadIdx.x - l;
+    const double* D = Dinv + 3 * i;  // row l of the vertex's 3x3 inverse (9 v + 3 l = 3 i)
+    const double ui = D[0] * rs[base] + D[1] * rs[base + 1] + D[2] * rs[base + 2];
+    u[i] = ui;
+    ru = ri * ui;
+    rr = ri * ri;
+  }
+  ru = block_sum<3 * kJacVerts>(ru, sh);
+  rr = block_sum<3 * kJacVerts>(rr, sh);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = rz;
-    part[2 * blockIdx.x + 1] = rr;
+    part_ru[blockIdx.x] = ru;
+    part_rr[blockIdx.x] = rr;
+  }
+}
+inline int jac_blocks(int nV) { return std::max(1, (nV + kJacVerts - 1) / kJacVerts); }
+inline int vec_blocks(int nV) { return std::max(1, std::min((nV + kRedBlock - 1) / kRedBlock, 2048)); }
+
+// first stage of the CG reduction: kRed1Blocks blocks sum fixed slices of the three regions
+__global__ void __launch_bounds__(1024)
+    k_cg_reduce1(const double* __restrict__ part, int64_t region, int nru, int nrr, int nwu, double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int cnt[3] = {nru, nrr, nwu};
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const double* src = part + g * region;
+    const int per = (cnt[g] + kRed1Blocks - 1) / kRed1Blocks;
+    const int lo = blockIdx.x * per, hi = min(cnt[g], lo + per);
+    double a = 0.0;
+    for (int k = lo + threadIdx.x; k < hi; k += 1024) a += src[k];
+    a = block_sum<1024>(a, sh);
+    if (threadIdx.x == 0) out[g * kRed1Blocks + blockIdx.x] = a;
   }
 }
 
-__global__ void __launch_bounds__(kRedBlock) k_pcg_start(const double* __restrict__ part, int np, double tol,
-                                                         int maxit, PcgState* st) {
+// start of a factor-preconditioned solve: x = 0, r = b, p = s = 0, partial r.r
+__global__ void __launch_bounds__(kRedBlock)
+    k_cg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r,
+              double* __restrict__ p, double* __restrict__ s, double* __restrict__ part_rr) {
   __shared__ double sh[4];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedBlock) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    p[i] = 0.0;
+    s[i] = 0.0;
+    r[i] = bi;
+    rr += bi * bi;
   }
-  a = block_sum_256(a, sh);
-  b = block_sum_256(b, sh);
-  if (threadIdx.x == 0) {
-    st->rz = a;
-    st->rr = b;
-    st->bb = b;
+  rr = block_sum_256(rr, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// second stage of the reduction (kRed1Blocks partials per quantity) and the CG scalar step
+__global__ void __launch_bounds__(1024)
+    k_cg_scalars(const double* __restrict__ part, int start, double tol, int maxit, PcgState* __restrict__ st) {
+  __shared__ double sh[16];
+  if (!start && st->done) {
+    // a non-positive-curvature exit in the first iteration applied its alpha = 1 update: count it once
+    if (st->done == 3 && threadIdx.x == 0 && st->alpha != 0.0) {
+      st->alpha = 0.0;
+      st->iters += 1;
+    }
+    return;
+  }
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (threadIdx.x < kRed1Blocks) {
+    a = part[threadIdx.x];
+    b = part[kRed1Blocks + threadIdx.x];
+    c = part[2 * kRed1Blocks + threadIdx.x];
+  }
+  a = block_sum<1024>(a, sh);
+  b = block_sum<1024>(b, sh);
+  c = block_sum<1024>(c, sh);
+  if (threadIdx.x != 0) return;
+  const double g = a, rr = b, d = c;
+  if (start) {
+    st->bb = rr;
+    st->rr = rr;
     st->tol2 = tol * tol;
     st->iters = 0;
     st->maxit = maxit;
-    st->done = (b == 0.0) ? 1 : 0;
-  }
-}
-
-__global__ void __launch_bounds__(kRedBlock) k_pcg_alpha(const double* __restrict__ part, int np, PcgState* st) {
-  __shared__ double sh[4];
-  if (st->done) return;
-  double a = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedBlock) a += part[i];
-  a = block_sum_256(a, sh);
-  if (threadIdx.x == 0) {
-    st->pAp = a;
-    if (!(a > 0.0)) {  // non-positive curvature: keep the current iterate (or the preconditioned gradient)
-      st->alpha = (st->iters == 0) ? 1.0 : 0.0;
+    st->beta = 0.0;
+    st->gamma = g;
+    st->delta = d;
+    if (rr == 0.0) {
+      st->alpha = 0.0;
+      st->done = 1;
+    } else if (!(d > 0.0)) {  // non-positive curvature: take the preconditioned gradient
+      st->alpha = 1.0;
       st->done = 3;
     } else {
-      st->alpha = st->rz / a;
-    }
-  }
-}
-
-__global__ void __launch_bounds__(kRedBlock)
-    k_pcg_update(int nV, const double* __restrict__ Dinv, double* __restrict__ x, double* __restrict__ r,
-                 double* __restrict__ z, const double* __restrict__ p, const double* __restrict__ Ap,
-                 double* __restrict__ part, PcgState* st) {
-  __shared__ double sh[4];
-  const int done = st->done;
-  if (done && done != 3) return;
-  const double alpha = st->alpha;
-  const int v = blockIdx.x * kRedBlock + threadIdx.x;
-  double rz = 0.0, rr = 0.0;
-  if (v < nV) {
-    double rv[3];
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      x[3ll * v + l] += alpha * p[3ll * v + l];
-      rv[l] = r[3ll * v + l] - alpha * Ap[3ll * v + l];
-      r[3ll * v + l] = rv[l];
-    }
-    const double* D = Dinv + 9ll * v;
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      const double zl = D[3 * l] * rv[0] + D[3 * l + 1] * rv[1] + D[3 * l + 2] * rv[2];
-      z[3ll * v + l] = zl;
-      rz += rv[l] * zl;
-      rr += rv[l] * rv[l];
-    }
-  }
-  rz = block_sum_256(rz, sh);
-  rr = block_sum_256(rr, sh);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = rz;
-    part[2 * blockIdx.x + 1] = rr;
-  }
-}
-
-__global__ void __launch_bounds__(kRedBlock) k_pcg_beta(const double* __restrict__ part, int np, PcgState* st) {
-  __shared__ double sh[4];
-  if (st->done) {
-    // a non-positive-curvature exit applied its last (alpha) update in k_pcg_update: count it once
-    if (st->done == 3 && threadIdx.x == 0 && st->alpha != 0.0) {
-      st->alpha = 0.0;
-      st->iters += 1;
+      st->alpha = g / d;
+      st->done = 0;
     }
     return;
   }
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < np; i += kRedBlock) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
-  }
-  a = block_sum_256(a, sh);
-  b = block_sum_256(b, sh);
-  if (threadIdx.x == 0) {
-    st->beta = a / st->rz;
-    st->rz = a;
-    st->rr = b;
-    st->iters += 1;
-    if (b <= st->tol2 * st->bb)
-      st->done = 1;
-    else if (st->iters >= st->maxit)
-      st->done = 2;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_pcg_p(int64_t n, const double* __restrict__ z, double* __restrict__ p,
-                                               const PcgState* __restrict__ st) {
-  if (st->done) return;
-  const double beta = st->beta;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-// generic-preconditioner CG pieces: x = 0, r = b with partial r.r; x += a p, r -= a Ap with partial r.r
-__global__ void __launch_bounds__(kRedBlock) k_pcg_init_r(int64_t n, const double* __restrict__ b,
-                                                          double* __restrict__ x, double* __restrict__ r,
-                                                          double* __restrict__ part) {
-  __shared__ double sh[4];
-  const int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
-  double rr = 0.0;
-  if (i < n) {
-    const double bi = b[i];
-    x[i] = 0.0;
-    r[i] = bi;
-    rr = bi * bi;
-  }
-  rr = block_sum_256(rr, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = rr;
-}
-
-__global__ void __launch_bounds__(kRedBlock) k_pcg_update_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
-                                                             const double* __restrict__ p,
-                                                             const double* __restrict__ Ap, double* __restrict__ part,
-                                                             const PcgState* __restrict__ st) {
-  __shared__ double sh[4];
-  const int done = st->done;
-  if (done && done != 3) return;
-  const double alpha = st->alpha;
-  const int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
-  double rr = 0.0;
-  if (i < n) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * Ap[i];
-    r[i] = ri;
-    rr = ri * ri;
-  }
-  rr = block_sum_256(rr, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = rr;
-}
-
-// st from separate r.r and r.z partial arrays (start: also bb; loop: beta and the convergence test)
-__global__ void __launch_bounds__(kRedBlock) k_pcg_scalars2(const double* __restrict__ prr, int nrr,
-                                                            const double* __restrict__ prz, int nrz, int start,
-                                                            double tol, int maxit, PcgState* st) {
-  __shared__ double sh[4];
-  if (!start && st->done) {
-    if (st->done == 3 && threadIdx.x == 0 && st->alpha != 0.0) {
+  st->iters += 1;
+  st->rr = rr;
+  if (rr <= st->tol2 * st->bb) {
+    st->done = 1;
+  } else if (st->iters >= st->maxit) {
+    st->done = 2;
+  } else {
+    const double beta = g / st->gamma;
+    const double den = d - beta * g / st->alpha;  // = p'.H p' for the next direction
+    st->beta = beta;
+    st->gamma = g;
+    st->delta = d;
+    if (!(den > 0.0)) {  // non-positive curvature: keep the current iterate
       st->alpha = 0.0;
-      st->iters += 1;
-    }
-    return;
-  }
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nrz; i += kRedBlock) a += prz[i];
-  for (int i = threadIdx.x; i < nrr; i += kRedBlock) b += prr[i];
-  a = block_sum_256(a, sh);
-  b = block_sum_256(b, sh);
-  if (threadIdx.x == 0) {
-    if (start) {
-      st->rz = a;
-      st->rr = b;
-      st->bb = b;
-      st->tol2 = tol * tol;
-      st->iters = 0;
-      st->maxit = maxit;
-      st->done = (b == 0.0) ? 1 : 0;
+      st->done = 3;
     } else {
-      st->beta = a / st->rz;
-      st->rz = a;
-      st->rr = b;
-      st->iters += 1;
-      if (b <= st->tol2 * st->bb)
-        st->done = 1;
-      else if (st->iters >= st->maxit)
-        st->done = 2;
+      st->alpha = g / den;
     }
   }
 }
@@ -357,8 +343,43 @@ __global__ void __launch_bounds__(256) k_add_mass(int64_t n, double a, const dou
   if (i < n) vals[i] = (V)((double)vals[i] + a * mass[i]);
 }
 
+// The diagonal blocks of the precision form's Hessian, rebuilt in FP64 from its off-diagonal blocks.
+// Every element Hessian is translation invariant (it depends on q only through F, and the shape
+// gradients of a tet sum to zero) and the PSD clamp adds exactly tol/4 (1 1^T (x) I) to every tet, so
+// block row v of H sums to tol * (tets at v) * I, plus inertia * (row sum of M) with the
+// backward-Euler term.  H_vv = that row sum - sum_{n != v} H_vn keeps the FP32 operator exactly
+// translation invariant: its soft (smooth, low-stiffness) modes are then as accurate as the FP32
+// off-diagonal entries themselves, where a rounded FP32 diagonal perturbs them by ~1e-7 of the
+// largest stiffness (on the 6.07M-tet beam that alone cost 1.8x the CG iterations).
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_implied_diag(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                   const V* __restrict__ vals, const int32_t* __restrict__ vdeg, double psd_tol,
+                   const double* __restrict__ mass, double inertia, double* __restrict__ hdiag) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+  double S[9], Ms[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = Ms[i] = 0.0;
+  for (int r = 0; r < nb; ++r) {
+    const bool off = bcol[b0 + r] != v;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int64_t at = 9ll * b0 + 3ll * l * nb + 3 * r + k;
+        if (off) S[3 * l + k] += (double)vals[at];
+        if (mass) Ms[3 * l + k] += mass[at];
+      }
+  }
+  const double g = psd_tol * (double)vdeg[v];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) hdiag[9ll * v + i] = ((i % 4 == 0) ? g : 0.0) + inertia * Ms[i] - S[i];
+}
+
 struct NewtonWs {
-  double *vals, *g, *b, *x, *r, *z, *p, *Ap, *qt, *Dinv, *part, *scal;
+  double *vals, *g, *b, *x, *r, *z, *p, *s, *Ap, *qt, *Dinv, *hdiag, *part, *scal;
   float* vals32;    // precision form: FP32 Hessian values
   float* params32;  // precision form: FP32 parameters (2 nT)
   PcgState* st;
@@ -366,9 +387,8 @@ struct NewtonWs {
 
 int newton_workspace(bh_mesh* m, NewtonWs& w) {
   const int64_t n = 3ll * m->nV;
-  const int64_t nParts = 2ll * std::max<int64_t>(kRedMaxBlocks, (3ll * m->nV + kRedBlock - 1) / kRedBlock + 1) +
-                         kRedMaxBlocks + spmv_blocks(m->nV) + 16;
-  const size_t need = sizeof(double) * ((size_t)m->nnz + 8 * (size_t)n + 9 * (size_t)m->nV + nParts + 16) +
+  const int64_t nParts = 3 * part_region(m->nV) + 3 * kRed1Blocks + 16;
+  const size_t need = sizeof(double) * ((size_t)m->nnz + 9 * (size_t)n + 18 * (size_t)m->nV + nParts + 16) +
                       sizeof(float) * ((size_t)m->nnz + 2 * (size_t)m->nT) + sizeof(PcgState) + 512;
   if (m->wsBytes < need) {
     if (m->ws) BH_CHECK(hipFree(m->ws));
@@ -384,9 +404,11 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   w.r = q; q += n;
   w.z = q; q += n;
   w.p = q; q += n;
+  w.s = q; q += n;
   w.Ap = q; q += n;
   w.qt = q; q += n;
   w.Dinv = q; q += 9ll * m->nV;
+  w.hdiag = q; q += 9ll * m->nV;
   w.part = q; q += nParts;
   w.scal = q; q += 16;
   float* f = reinterpret_cast<float*>(q);
@@ -414,7 +436,8 @@ int objective(bh_mesh* m, const bh_newton_opts* o, const double* params, const d
     const int64_t n = 3ll * m->nV;
     hipLaunchKernelGGL(k_sub, dim3((n + 255) / 256), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
     hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
-                       o->d_mass_vals, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr, (const PcgState*)nullptr);
+                       o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
+                       (const PcgState*)nullptr);
     st = device_dot(w.z, w.Ap, n, w.part, w.scal + 9, s);
     if (st) return st;
     hipLaunchKernelGGL(k_axpy_to, dim3(1), dim3(1), 0, s, (int64_t)1, out, 0.5 * o->inertia, w.scal + 9, out);
@@ -429,36 +452,52 @@ int read_scalar(const double* d, double* h, hipStream_t s) {
   return BH_OK;
 }
 
-// PCG solve of H x = b on the masked system; returns the iteration count in *iters, status in *done
-template <typename V>
-int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const uint8_t* fixed, NewtonWs& w, int* iters,
-              int* done, hipStream_t s) {
-  const int nV = m->nV;
-  const int64_t n = 3ll * nV;
-  const int gv = (nV + kRedBlock - 1) / kRedBlock;
-  const int gr = (int)((n + kRedBlock - 1) / kRedBlock);
-  const int gs = spmv_blocks(nV);
-  (void)gr;
-  hipLaunchKernelGGL(k_block_jacobi<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, vals, fixed,
-                     o->precond == BH_PRECOND_POINT_JACOBI ? 1 : 0, w.Dinv);
-  hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(kRedBlock), 0, s, nV, w.b, w.Dinv, w.x, w.r, w.z, w.p, w.part);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kRedBlock), 0, s, w.part, gv, o->pcg_tol, o->pcg_max_iterations,
-                     w.st);
+// the two reduction stages and the scalar step (partial counts of the three regions)
+inline void cg_reduce(const NewtonWs& w, int nV, int nru, int nrr, int nwu, int start, const bh_newton_opts* o,
+                      hipStream_t s) {
+  const int64_t R = part_region(nV);
+  double* stage2 = w.part + 3 * R;
+  hipLaunchKernelGGL(k_cg_reduce1, dim3(kRed1Blocks), dim3(1024), 0, s, w.part, R, nru, nrr, nwu, stage2);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, s, stage2, start, o->pcg_tol, o->pcg_max_iterations, w.st);
+}
+
+int cg_wait(const NewtonWs& w, PcgState* hs, hipStream_t s) {
   BH_CHECK(hipGetLastError());
+  BH_CHECK(hipMemcpyAsync(hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+// PCG solve of H x = b on the masked system with block (or point) Jacobi; the iteration count goes to
+// *iters, the exit state to *done
+template <typename V>
+int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag, const uint8_t* fixed,
+              NewtonWs& w, int* iters, int* done, hipStream_t s) {
+  const int nV = m->nV;
+  const int gj = jac_blocks(nV);
+  const int gs = spmv_blocks(nV);
+  const int64_t R = part_region(nV);
+  double* pru = w.part;
+  double* prr = w.part + R;
+  double* pwu = w.part + 2 * R;
+  hipLaunchKernelGGL(k_block_jacobi<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, vals, hdiag,
+                     fixed, o->precond == BH_PRECOND_POINT_JACOBI ? 1 : 0, w.Dinv);
+  hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 1, w.b, w.Dinv, w.x, w.r, w.z, w.p,
+                     w.s, w.Ap, pru, prr, w.st);
+  hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, hdiag, fixed, w.z,
+                     w.Ap, pwu, (const PcgState*)nullptr);
+  cg_reduce(w, nV, gj, gj, gs, 1, o, s);
   PcgState hs{};
-  for (int it = 0;; it += kPcgBatch) {
+  int st;
+  for (;;) {
     for (int k = 0; k < kPcgBatch; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, fixed, w.p, w.Ap,
-                         w.part, w.st);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gs, w.st);
-      hipLaunchKernelGGL(k_pcg_update, dim3(gv), dim3(kRedBlock), 0, s, nV, w.Dinv, w.x, w.r, w.z, w.p, w.Ap, w.part,
-                         w.st);
-      hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(kRedBlock), 0, s, w.part, gv, w.st);
-      hipLaunchKernelGGL(k_pcg_p, dim3(gr), dim3(256), 0, s, n, w.z, w.p, w.st);
+      hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 0, w.b, w.Dinv, w.x, w.r, w.z,
+                         w.p, w.s, w.Ap, pru, prr, w.st);
+      hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, hdiag, fixed,
+                         w.z, w.Ap, pwu, w.st);
+      cg_reduce(w, nV, gj, gj, gs, 0, o, s);
     }
-    BH_CHECK(hipGetLastError());
-    BH_CHECK(hipMemcpyAsync(&hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    BH_CHECK(hipStreamSynchronize(s));
+    if ((st = cg_wait(w, &hs, s))) return st;
     if (hs.done) break;
   }
   *iters = hs.iters;
@@ -466,43 +505,102 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const uint8_t*
   return BH_OK;
 }
 
-// PCG with a colour-sweep block-triangular preconditioner (same device-resident scalars)
+// Block-Jacobi PCG on the split operator in the solve order of P (the precision form: the split
+// operator takes every upper block as the transpose of the lower one, so it is exactly symmetric,
+// where the FP32 CSR pattern holds H_vn and H_nv computed separately; CG on that slightly
+// non-symmetric matrix needed up to 1.8x the iterations).  split: build the split Hessian first.
 template <typename V>
-int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const uint8_t* fixed, const bh_precond* P,
-                     NewtonWs& w, int* iters, int* done, hipStream_t s) {
+int pcg_solve_split_jacobi(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag,
+                           const uint8_t* fixed, const bh_precond* P, int split, NewtonWs& w, int* iters, int* done,
+                           hipStream_t s) {
   const int nV = m->nV;
-  const int64_t n = 3ll * nV;
-  const int gr = (int)((n + kRedBlock - 1) / kRedBlock);
-  const int gs = spmv_blocks(nV);
-  const int nz = red_blocks(n);
-  double* part2 = w.part + gr + 8;  // second partial array (r.z), after the r.r / p.Ap partials
-  int* dflag = &w.st->done;
-  hipLaunchKernelGGL(k_pcg_init_r, dim3(gr), dim3(kRedBlock), 0, s, n, w.b, w.x, w.r, w.part);
-  int st = precond_apply(P, w.r, w.z, nullptr, s);
-  if (st) return st;
-  BH_CHECK(hipMemcpyAsync(w.p, w.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(k_dot_partials, dim3(nz), dim3(kRedBlock), 0, s, w.r, w.z, n, part2);
-  hipLaunchKernelGGL(k_pcg_scalars2, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, part2, nz, 1, o->pcg_tol,
-                     o->pcg_max_iterations, w.st);
-  BH_CHECK(hipGetLastError());
+  const int gj = jac_blocks(nV);
+  const int gsp = sweep_grid(nV);
+  const int gr = (int)((3ll * nV + 255) / 256);
+  const int64_t R = part_region(nV);
+  double* pru = w.part;
+  double* prr = w.part + R;
+  double* pwu = w.part + 2 * R;
+  int st;
+  if (split && (st = precond_split_hessian<V>(P, vals, hdiag, fixed, s))) return st;
+  const SplitH<V> h = split_view<V>(P);
+  double *b = svec(P, SV_B), *x = svec(P, SV_X), *r = svec(P, SV_R), *p = svec(P, SV_P);
+  double *sv = svec(P, SV_S), *u = svec(P, SV_U), *wv = svec(P, SV_W);
+  hipLaunchKernelGGL(k_block_inv, dim3((nV + 255) / 256), dim3(256), 0, s, nV, (const double*)P->hdiag,
+                     o->precond == BH_PRECOND_POINT_JACOBI ? 1 : 0, w.Dinv);
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)w.b, b);
+  hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 1, b, w.Dinv, x, r, u, p, sv, wv, pru,
+                     prr, w.st);
+  hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl, P->nuptr,
+                     P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                     (const PcgState*)nullptr);
+  cg_reduce(w, nV, gj, gj, gsp, 1, o, s);
   PcgState hs{};
   for (;;) {
     for (int k = 0; k < kPcgBatch; ++k) {
-      hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, fixed, w.p, w.Ap,
-                         w.part, w.st);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(kRedBlock), 0, s, w.part, gs, w.st);
-      hipLaunchKernelGGL(k_pcg_update_xr, dim3(gr), dim3(kRedBlock), 0, s, n, w.x, w.r, w.p, w.Ap, w.part, w.st);
-      if ((st = precond_apply(P, w.r, w.z, dflag, s))) return st;
-      hipLaunchKernelGGL(k_dot_partials, dim3(nz), dim3(kRedBlock), 0, s, w.r, w.z, n, part2);
-      hipLaunchKernelGGL(k_pcg_scalars2, dim3(1), dim3(kRedBlock), 0, s, w.part, gr, part2, nz, 0, o->pcg_tol,
-                         o->pcg_max_iterations, w.st);
-      hipLaunchKernelGGL(k_pcg_p, dim3(gr), dim3(256), 0, s, n, w.z, w.p, w.st);
+      hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 0, b, w.Dinv, x, r, u, p, sv, wv,
+                         pru, prr, w.st);
+      hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
+                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                         w.st);
+      cg_reduce(w, nV, gj, gj, gsp, 0, o, s);
     }
-    BH_CHECK(hipGetLastError());
-    BH_CHECK(hipMemcpyAsync(&hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    BH_CHECK(hipStreamSynchronize(s));
+    if ((st = cg_wait(w, &hs, s))) return st;
     if (hs.done) break;
   }
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)x, w.x);
+  BH_CHECK(hipGetLastError());
+  *iters = hs.iters;
+  *done = hs.done;
+  return BH_OK;
+}
+
+// PCG with a colour-sweep block-triangular preconditioner, run in the preconditioner's solve order
+// (colours contiguous): the masked Hessian is split into lower / transposed-lower / diagonal blocks in
+// that order, b is permuted in and x out.  Same recurrences as above; the vector update runs inside
+// the forward sweep, r.z comes from the backward sweep.
+template <typename V>
+int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag, const uint8_t* fixed,
+                     const bh_precond* P, NewtonWs& w, int* iters, int* done, hipStream_t s) {
+  const int nV = m->nV;
+  const int64_t n = 3ll * nV;
+  const int gsp = sweep_grid(nV);
+  const int gi = vec_blocks(nV);
+  const int gr = (int)((n + 255) / 256);
+  const int64_t R = part_region(nV);
+  if (precond_sweep_blocks(P) > R || gsp > R) return BH_UNSUPPORTED;  // more colours than the regions hold
+  double* pru = w.part;
+  double* prr = w.part + R;
+  double* pwu = w.part + 2 * R;
+  int st = precond_split_hessian<V>(P, vals, hdiag, fixed, s);
+  if (st) return st;
+  const SplitH<V> h = split_view<V>(P);
+  double *b = svec(P, SV_B), *x = svec(P, SV_X), *r = svec(P, SV_R), *p = svec(P, SV_P);
+  double *sv = svec(P, SV_S), *u = svec(P, SV_U), *wv = svec(P, SV_W);
+  int np[2] = {0, 0};
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)w.b, b);
+  hipLaunchKernelGGL(k_cg_init, dim3(gi), dim3(kRedBlock), 0, s, n, b, x, r, p, sv, prr);
+  CgVecs start{x, p, sv, u, wv, nullptr, nullptr};
+  if ((st = precond_apply_solve(P, r, u, s, &start, 0, pru, np))) return st;
+  hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl, P->nuptr,
+                     P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                     (const PcgState*)nullptr);
+  cg_reduce(w, nV, np[1], gi, gsp, 1, o, s);
+  CgVecs loop{x, p, sv, u, wv, prr, w.st};
+  PcgState hs{};
+  for (;;) {
+    for (int k = 0; k < kPcgBatch; ++k) {
+      if ((st = precond_apply_solve(P, r, u, s, &loop, 1, pru, np))) return st;
+      hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
+                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                         w.st);
+      cg_reduce(w, nV, np[1], np[0], gsp, 0, o, s);
+    }
+    if ((st = cg_wait(w, &hs, s))) return st;
+    if (hs.done) break;
+  }
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)x, w.x);
+  BH_CHECK(hipGetLastError());
   *iters = hs.iters;
   *done = hs.done;
   return BH_OK;
@@ -528,6 +626,15 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
                        vals);
     BH_CHECK(hipGetLastError());
   }
+  // precision form: FP64 diagonal blocks rebuilt from the FP32 off-diagonal blocks (exact row sums)
+  const double* hdiag = nullptr;
+  if (sizeof(V) == 4 && m->nV) {
+    hipLaunchKernelGGL(k_implied_diag<V>, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, m->bptr, m->bcol,
+                       (const V*)vals, m->vdeg, psd ? psd->tol : 0.0, o->d_mass_vals, o->d_mass_vals ? o->inertia : 0.0,
+                       w.hdiag);
+    BH_CHECK(hipGetLastError());
+    hdiag = w.hdiag;
+  }
   *fallback = 0;
   if (o->precond == BH_PRECOND_BIC0 || o->precond == BH_PRECOND_FACTOR || o->precond == BH_PRECOND_LEARNED) {
     if (!o->precond_handle) return BH_BAD_ARG;
@@ -537,7 +644,7 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
       double shift = o->precond_shift;
       for (int attempt = 0;; ++attempt) {
         int fails = 0;
-        if ((st = precond_factor_bic0(o->precond_handle, vals, fixed, shift, &fails, s))) return st;
+        if ((st = precond_factor_bic0(o->precond_handle, vals, hdiag, fixed, shift, &fails, s))) return st;
         if (fails == 0 || attempt >= 6) break;
         shift = (shift > 0.0) ? 4.0 * shift : 1e-3;
       }
@@ -546,16 +653,22 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
       if (!o->mgn) return BH_BAD_ARG;
       if ((st = mgn_factor<V>(o->mgn, o->precond_handle, vals, fixed, s))) return st;
     }
-    if ((st = pcg_solve_factor(m, o, vals, fixed, o->precond_handle, w, pits, pdone, s))) return st;
+    if ((st = pcg_solve_factor(m, o, vals, hdiag, fixed, o->precond_handle, w, pits, pdone, s))) return st;
     if (*pdone == 2 && o->jacobi_fallback) {
       int p2 = 0;
-      if ((st = pcg_solve(m, o, vals, fixed, w, &p2, pdone, s))) return st;
+      st = hdiag ? pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, 0, w, &p2, pdone, s)
+                 : pcg_solve(m, o, vals, hdiag, fixed, w, &p2, pdone, s);
+      if (st) return st;
       *pits += p2;
       *fallback = 1;
     }
     return BH_OK;
   }
-  return pcg_solve(m, o, vals, fixed, w, pits, pdone, s);
+  if (hdiag) {  // precision form: block Jacobi on the symmetric split operator (colour lists made once)
+    if (!m->solveOrder && (st = bh_precond_create(m, &m->solveOrder))) return st;
+    return pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, m->solveOrder, 1, w, pits, pdone, s);
+  }
+  return pcg_solve(m, o, vals, hdiag, fixed, w, pits, pdone, s);
 }
 }  // namespace
 
@@ -565,7 +678,7 @@ int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* 
   if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
   if (m->nV == 0) return BH_OK;
   hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, as_stream(s), m->nV, m->bptr,
-                     m->bcol, d_vals, nullptr, d_x, d_y, nullptr, nullptr);
+                     m->bcol, d_vals, (const double*)nullptr, nullptr, d_x, d_y, nullptr, nullptr);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -631,7 +744,7 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
     if (o->d_mass_vals) {  // + inertia M (q - q_pred)
       hipLaunchKernelGGL(k_sub, dim3(gn), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
       hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
-                         o->d_mass_vals, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
+                         o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
                          (const PcgState*)nullptr);
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, s, n, o->inertia, w.Ap, w.g);
     }

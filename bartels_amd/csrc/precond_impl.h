@@ -1,22 +1,26 @@
 // Block lower-triangular preconditioners on the Hessian's own 3x3-block pattern (included at the end
 // of bartels_hip.hip; same translation unit).
 //
-// Vertices are coloured greedily so that no two neighbours share a colour; in colour order the
+// Vertices are coloured (DSatur) so that no two neighbours share a colour; in colour order the
 // factor L has, for every vertex v, a lower-triangular 3x3 diagonal block L_vv and one 3x3 block
 // L_vn per neighbour n of lower colour -- exactly the lower half of H's pattern.  M = L L^T is
 // applied as two colour sweeps (forward L y = r, backward L^T z = y); inside one colour every
 // vertex is independent, so each sweep is one parallel kernel per colour.
 //   * block IC(0): L from H by an incomplete Cholesky restricted to the pattern (colour by colour);
 //   * learned: L written by the mesh-graph network (bh_precond_set_factor).
-// Storage (FP32): per vertex the inverse of L_vv (6 lower-triangular entries), the lower blocks of
-// its row (lptr/lcol/lval) and, for the backward sweep, the transposed blocks of its column
-// (uptr/ucol/ulink into lval), so each sweep streams contiguous per-vertex lists.
+// Storage: the factor is written in vertex order (lptr/lcol/lval, FP32, and the inverse of each L_vv as
+// 6 packed entries) by IC(0) or the network; the solve runs in the colour order -- position i holds
+// vertex colVerts[i], so every colour is one contiguous range of positions, rows, factor blocks and
+// Hessian blocks.  precond_finalize copies the factor into that order (lvs; uvs holds the same blocks
+// in the order of the upper lists, for the backward sweep), and precond_split_hessian copies the
+// masked Hessian into lower blocks, the transposed-lower copy for the upper part and FP64 diagonal
+// blocks, which is what the solve's products read.
 
 struct bh_precond {
   bh_mesh* mesh = nullptr;
   int ncolors = 0;
   std::vector<int32_t> h_colPtr;
-  int32_t* colVerts = nullptr;  // vertices grouped by colour
+  int32_t* colVerts = nullptr;  // vertices grouped by colour (position -> vertex)
   int32_t* lptr = nullptr;      // nV + 1
   int32_t* lcol = nullptr;      // lower neighbour (colour(n) < colour(v)), sorted by colour then id
   int32_t* lslot = nullptr;     // block index of (v, n) in the mesh pattern
@@ -25,17 +29,27 @@ struct bh_precond {
   int32_t* ulink = nullptr;     // index of the lower entry (n, v) in n's list
   int32_t* dslot = nullptr;     // block index of (v, v)
   float* lval = nullptr;        // 9 per lower entry, row-major
-  float* uval = nullptr;        // the same blocks in upper-list order (for the backward sweep)
   float* dinv = nullptr;        // 6 per vertex: inverse of L_vv (lower triangular, row-major packed)
   int32_t* fail = nullptr;      // factorisation breakdown counter
   int64_t nlower = 0;
-  float* ytmp = nullptr;        // forward-sweep result (3 nV)
-  float* ztmp = nullptr;        // FP32 shadow of the backward-sweep result (3 nV)
+  // the solve order
+  int32_t* nlptr = nullptr;     // nV + 1 per position
+  int32_t* nlcol = nullptr;     // position of the lower neighbour
+  int32_t* nlsrc = nullptr;     // index of the entry in the vertex-order lists
+  int32_t* nuptr = nullptr;     // nV + 1 per position
+  int32_t* nucol = nullptr;     // position of the upper neighbour
+  int32_t* nulink = nullptr;    // solve-order index of the lower entry (n, v) in n's list
+  float* lvs = nullptr;         // factor lower blocks (9 each, row-major)
+  float* uvs = nullptr;         // the lower blocks (n, i) in i's upper-list order
+  float* dinvs = nullptr;       // 6 per position
+  double* hsplit = nullptr;     // masked Hessian: lower blocks and their transposed copy (FP32 or FP64)
+  double* hdiag = nullptr;      // masked diagonal blocks (FP64, 9 per position)
+  double* sv = nullptr;         // solve vectors, 3 nV each: b, x, r, p, s, u, w, y
 };
 
 namespace {
 
-// ---- factor application ------------------------------------------------------------------------
+// ---- factor application (solve order) ----------------------------------------------------------
 // packed lower-triangular 3x3: [l00, l10, l11, l20, l21, l22]
 __device__ __forceinline__ void ltri_mv(const float* L, const double* x, double* y) {
   y[0] = (double)L[0] * x[0];
@@ -43,32 +57,62 @@ __device__ __forceinline__ void ltri_mv(const float* L, const double* x, double*
   y[2] = (double)L[3] * x[0] + (double)L[4] * x[1] + (double)L[5] * x[2];
 }
 
-// Sweeps, 8 lanes per vertex: lane j takes list entries j, j+8, ... (a vertex's list is contiguous,
-// so the lanes read neighbouring 36-byte blocks), the three partial sums are reduced across the 8
-// lanes with a fixed xor butterfly, and lane 0 applies the diagonal inverse.  The factor is FP32
-// and so is the sweep arithmetic: the intermediate y and an FP32 shadow of z are what the gathers
-// read (12 bytes per neighbour); r comes in and z goes out in FP64 for the CG recurrences.
+// Sweeps, 8 lanes per position: lane j takes list entries j, j+8, ... (a position's list is
+// contiguous and so are the lists of a colour, so the lanes stream neighbouring 36-byte blocks), the
+// three partial sums are reduced across the 8 lanes with a fixed xor butterfly, and lane 0 applies the
+// diagonal inverse.  The factor is stored in FP32 and widened; the sweep arithmetic and the vectors y,
+// z are FP64, so M^{-1} stays a fixed linear operator to FP64 rounding (an FP32 z perturbs CG like an
+// inexact preconditioner and costs iterations at tight tolerances).  FP16 factor blocks were tried and
+// dropped: their 5e-4 rounding swamps the near-cancellation that makes an incomplete factor accurate
+// on the smooth, low-stiffness modes, and IC(0) then needed more iterations than block Jacobi.
 constexpr int kSweepLanes = 8;
+constexpr int kSweepVpb = 256 / kSweepLanes;  // positions per block
 
-// forward sweep over one colour: y_v = L_vv^{-1} (r_v - sum_n L_vn y_n)
+// CG vectors a forward sweep updates before it reads r (Chronopoulos-Gear recurrences, rows of the
+// colour's positions only): p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s
+struct CgVecs {
+  double *x, *p, *s;
+  const double *u, *w;
+  double* part_rr;  // partial r.r per block (after the update)
+  const PcgState* st;
+};
+
+// forward sweep over the colour's positions [c0, c0 + n): y_i = L_ii^{-1} (r_i - sum_n L_in y_n),
+// optionally after the CG update
 __global__ void __launch_bounds__(256)
-    k_sweep_fwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ lptr,
-                const int32_t* __restrict__ lcol, const float* __restrict__ lval, const float* __restrict__ dinv,
-                const double* __restrict__ r, float* __restrict__ y, const int* __restrict__ done) {
-  if (done && *done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kSweepLanes, j = gid % kSweepLanes;
-  int v = 0;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  if (i < n) {
-    v = verts[i];
-    for (int k = lptr[v] + j; k < lptr[v + 1]; k += kSweepLanes) {
-      const float* B = lval + 9ll * k;
-      const float* yn = y + 3ll * lcol[k];
-      const float y0 = yn[0], y1 = yn[1], y2 = yn[2];
-      s0 += B[0] * y0 + B[1] * y1 + B[2] * y2;
-      s1 += B[3] * y0 + B[4] * y1 + B[5] * y2;
-      s2 += B[6] * y0 + B[7] * y1 + B[8] * y2;
+    k_sweep_fwd(int c0, int n, const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlcol,
+                const float* __restrict__ lv, const float* __restrict__ dinv, double* __restrict__ r,
+                double* __restrict__ y, CgVecs cg, int update) {
+  __shared__ double sh[4];
+  if (cg.st && cg_skip(cg.st)) return;
+  const int j = threadIdx.x % kSweepLanes;
+  const int li = blockIdx.x * kSweepVpb + threadIdx.x / kSweepLanes;
+  const int i = c0 + li;
+  double rj = 0.0, rr = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (li < n) {
+    if (j < 3) {
+      const int64_t row = 3ll * i + j;
+      rj = r[row];
+      if (update) {
+        const double alpha = cg.st->alpha, beta = cg.st->beta;
+        const double pj = cg.u[row] + beta * cg.p[row];
+        const double sj = cg.w[row] + beta * cg.s[row];
+        cg.p[row] = pj;
+        cg.s[row] = sj;
+        cg.x[row] += alpha * pj;
+        rj -= alpha * sj;
+        r[row] = rj;
+        rr = rj * rj;
+      }
+    }
+    for (int k = nlptr[i] + j; k < nlptr[i + 1]; k += kSweepLanes) {
+      const float* B = lv + 9ll * k;
+      const double* yn = y + 3ll * nlcol[k];
+      const double y0 = yn[0], y1 = yn[1], y2 = yn[2];
+      s0 += (double)B[0] * y0 + (double)B[1] * y1 + (double)B[2] * y2;
+      s1 += (double)B[3] * y0 + (double)B[4] * y1 + (double)B[5] * y2;
+      s2 += (double)B[6] * y0 + (double)B[7] * y1 + (double)B[8] * y2;
     }
   }
 #pragma unroll
@@ -77,36 +121,43 @@ __global__ void __launch_bounds__(256)
     s1 += __shfl_xor(s1, off, 64);
     s2 += __shfl_xor(s2, off, 64);
   }
-  if (i < n && j == 0) {
-    const float* L = dinv + 6ll * v;
-    const float t0 = (float)r[3ll * v] - s0, t1 = (float)r[3ll * v + 1] - s1, t2 = (float)r[3ll * v + 2] - s2;
-    y[3ll * v] = L[0] * t0;
-    y[3ll * v + 1] = L[1] * t0 + L[2] * t1;
-    y[3ll * v + 2] = L[3] * t0 + L[4] * t1 + L[5] * t2;
+  const int lb = (int)(threadIdx.x & 63) - j;  // group base lane within the wavefront
+  const double r0 = __shfl(rj, lb, 64), r1 = __shfl(rj, lb + 1, 64), r2 = __shfl(rj, lb + 2, 64);
+  if (li < n && j == 0) {
+    const float* L = dinv + 6ll * i;
+    const double t0 = r0 - s0, t1 = r1 - s1, t2 = r2 - s2;
+    y[3ll * i] = (double)L[0] * t0;
+    y[3ll * i + 1] = (double)L[1] * t0 + (double)L[2] * t1;
+    y[3ll * i + 2] = (double)L[3] * t0 + (double)L[4] * t1 + (double)L[5] * t2;
+  }
+  if (cg.part_rr) {
+    const double d = block_sum_256(rr, sh);
+    if (threadIdx.x == 0) cg.part_rr[blockIdx.x] = d;
   }
 }
 
-// backward sweep over one colour: z_v = L_vv^{-T} (y_v - sum_{n: colour(n) > colour(v)} L_nv^T z_n), with
-// L_nv read from uval (the lower blocks copied into the upper-list order)
+// backward sweep over the colour's positions: z_i = L_ii^{-T} (y_i - sum_{n upper} L_ni^T z_n), with L_ni
+// read from uvs (the lower blocks in upper-list order); optional partial r.z
 __global__ void __launch_bounds__(256)
-    k_sweep_bwd(const int32_t* __restrict__ verts, int n, const int32_t* __restrict__ uptr,
-                const int32_t* __restrict__ ucol, const float* __restrict__ uval, const float* __restrict__ dinv,
-                const float* __restrict__ y, float* __restrict__ z32, double* __restrict__ z,
-                const int* __restrict__ done) {
-  if (done && *done) return;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kSweepLanes, j = gid % kSweepLanes;
-  int v = 0;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  if (i < n) {
-    v = verts[i];
-    for (int k = uptr[v] + j; k < uptr[v + 1]; k += kSweepLanes) {
-      const float* B = uval + 9ll * k;  // L_nv, use its transpose
-      const float* zn = z32 + 3ll * ucol[k];
-      const float z0 = zn[0], z1 = zn[1], z2 = zn[2];
-      s0 += B[0] * z0 + B[3] * z1 + B[6] * z2;
-      s1 += B[1] * z0 + B[4] * z1 + B[7] * z2;
-      s2 += B[2] * z0 + B[5] * z1 + B[8] * z2;
+    k_sweep_bwd(int c0, int n, const int32_t* __restrict__ nuptr, const int32_t* __restrict__ nucol,
+                const float* __restrict__ uv, const float* __restrict__ dinv, const double* __restrict__ y,
+                double* __restrict__ z, const double* __restrict__ r, double* __restrict__ part_rz,
+                const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && cg_skip(st)) return;
+  const int j = threadIdx.x % kSweepLanes;
+  const int li = blockIdx.x * kSweepVpb + threadIdx.x / kSweepLanes;
+  const int i = c0 + li;
+  double rz = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (li < n) {
+    for (int k = nuptr[i] + j; k < nuptr[i + 1]; k += kSweepLanes) {
+      const float* B = uv + 9ll * k;  // L_ni, use its transpose
+      const double* zn = z + 3ll * nucol[k];
+      const double z0 = zn[0], z1 = zn[1], z2 = zn[2];
+      s0 += (double)B[0] * z0 + (double)B[3] * z1 + (double)B[6] * z2;
+      s1 += (double)B[1] * z0 + (double)B[4] * z1 + (double)B[7] * z2;
+      s2 += (double)B[2] * z0 + (double)B[5] * z1 + (double)B[8] * z2;
     }
   }
 #pragma unroll
@@ -115,30 +166,133 @@ __global__ void __launch_bounds__(256)
     s1 += __shfl_xor(s1, off, 64);
     s2 += __shfl_xor(s2, off, 64);
   }
-  if (i < n && j == 0) {
-    const float* L = dinv + 6ll * v;
-    const float t0 = y[3ll * v] - s0, t1 = y[3ll * v + 1] - s1, t2 = y[3ll * v + 2] - s2;
+  if (li < n && j == 0) {
+    const float* L = dinv + 6ll * i;
+    const double t0 = y[3ll * i] - s0, t1 = y[3ll * i + 1] - s1, t2 = y[3ll * i + 2] - s2;
     // L^T t with L packed lower [l00, l10, l11, l20, l21, l22]
-    const float o0 = L[0] * t0 + L[1] * t1 + L[3] * t2;
-    const float o1 = L[2] * t1 + L[4] * t2;
-    const float o2 = L[5] * t2;
-    z32[3ll * v] = o0;
-    z32[3ll * v + 1] = o1;
-    z32[3ll * v + 2] = o2;
-    z[3ll * v] = o0;
-    z[3ll * v + 1] = o1;
-    z[3ll * v + 2] = o2;
+    const double o0 = (double)L[0] * t0 + (double)L[1] * t1 + (double)L[3] * t2;
+    const double o1 = (double)L[2] * t1 + (double)L[4] * t2;
+    const double o2 = (double)L[5] * t2;
+    z[3ll * i] = o0;
+    z[3ll * i + 1] = o1;
+    z[3ll * i + 2] = o2;
+    if (r) rz = r[3ll * i] * o0 + r[3ll * i + 1] * o1 + r[3ll * i + 2] * o2;
+  }
+  if (part_rz) {
+    const double d = block_sum_256(rz, sh);
+    if (threadIdx.x == 0) part_rz[blockIdx.x] = d;
   }
 }
 
-// uval[t] = lval[ulink[t]]: the factor's lower blocks in the order of the upper lists
-__global__ void __launch_bounds__(256) k_fill_uval(int64_t nu, const int32_t* __restrict__ ulink,
-                                                   const float* __restrict__ lval, float* __restrict__ uval) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nu) return;
-  const float* src = lval + 9ll * ulink[t];
+// Ap = H p in the solve order from the split Hessian (lower blocks, the transposed copy of the lower
+// blocks for the upper part, diagonal blocks), partial p.Ap.  8 lanes per position over the joined
+// lower + upper list; lane 0 adds the diagonal block.  The masked system is symmetric by construction.
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_split_spmv(int nV, const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlcol,
+                 const V* __restrict__ hl, const int32_t* __restrict__ nuptr, const int32_t* __restrict__ nucol,
+                 const V* __restrict__ hu, const double* __restrict__ hd, const double* __restrict__ p,
+                 double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && cg_skip(st)) return;
+  const int j = threadIdx.x % kSweepLanes;
+  const int i = blockIdx.x * kSweepVpb + threadIdx.x / kSweepLanes;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (i < nV) {
+    const int l0 = nlptr[i], nl = nlptr[i + 1] - l0;
+    const int u0 = nuptr[i], nt = nl + nuptr[i + 1] - u0;
+    for (int t = j; t < nt; t += kSweepLanes) {
+      if (t < nl) {
+        const V* B = hl + 9ll * (l0 + t);
+        const double* xp = p + 3ll * nlcol[l0 + t];
+        const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+        s0 += (double)B[0] * x0 + (double)B[1] * x1 + (double)B[2] * x2;
+        s1 += (double)B[3] * x0 + (double)B[4] * x1 + (double)B[5] * x2;
+        s2 += (double)B[6] * x0 + (double)B[7] * x1 + (double)B[8] * x2;
+      } else {
+        const int k = u0 + t - nl;
+        const V* B = hu + 9ll * k;  // H_ni: use its transpose
+        const double* xp = p + 3ll * nucol[k];
+        const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
+        s0 += (double)B[0] * x0 + (double)B[3] * x1 + (double)B[6] * x2;
+        s1 += (double)B[1] * x0 + (double)B[4] * x1 + (double)B[7] * x2;
+        s2 += (double)B[2] * x0 + (double)B[5] * x1 + (double)B[8] * x2;
+      }
+    }
+    if (j == 0) {
+      const double* B = hd + 9ll * i;
+      const double x0 = p[3ll * i], x1 = p[3ll * i + 1], x2 = p[3ll * i + 2];
+      s0 += (double)B[0] * x0 + (double)B[1] * x1 + (double)B[2] * x2;
+      s1 += (double)B[3] * x0 + (double)B[4] * x1 + (double)B[5] * x2;
+      s2 += (double)B[6] * x0 + (double)B[7] * x1 + (double)B[8] * x2;
+    }
+  }
 #pragma unroll
-  for (int i = 0; i < 9; ++i) uval[9 * t + i] = src[i];
+  for (int off = kSweepLanes / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double pap = 0.0;
+  if (i < nV && j < 3) {
+    const int64_t row = 3ll * i + j;
+    const double sv = (j == 0) ? s0 : (j == 1) ? s1 : s2;
+    Ap[row] = sv;
+    pap = p[row] * sv;
+  }
+  if (part) {
+    const double d = block_sum_256(pap, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
+}
+
+// dst block t = src block link[t] (9 values each)
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k_copy_blocks(int64_t nb, const int32_t* __restrict__ link,
+                                                     const S* __restrict__ src, D* __restrict__ dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nb) return;
+  const S* b = src + 9ll * link[t];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dst[9 * t + i] = (D)b[i];
+}
+
+// the factor in the solve order, one thread per position i (vertex colVerts[i]): the 6 packed inverse
+// entries of L_ii, the lower blocks and, for the upper list, the lower blocks (n, i) of the neighbours
+__global__ void __launch_bounds__(256)
+    k_factor_solve_order(int nV, const int32_t* __restrict__ colVerts, const float* __restrict__ dinv,
+                         const float* __restrict__ lval, const int32_t* __restrict__ nlptr,
+                         const int32_t* __restrict__ nlsrc, const int32_t* __restrict__ nuptr,
+                         const int32_t* __restrict__ nulink, float* __restrict__ dinvs, float* __restrict__ lv,
+                         float* __restrict__ uv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const float* a = dinv + 6ll * colVerts[i];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dinvs[6ll * i + k] = a[k];
+  for (int t = nlptr[i]; t < nlptr[i + 1]; ++t) {
+    const float* b = lval + 9ll * nlsrc[t];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) lv[9ll * t + e] = b[e];
+  }
+  for (int t = nuptr[i]; t < nuptr[i + 1]; ++t) {
+    const float* b = lval + 9ll * nlsrc[nulink[t]];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) uv[9ll * t + e] = b[e];
+  }
+}
+
+// rows of a vector between vertex order and the solve order (to = 1: vertex -> position)
+__global__ void __launch_bounds__(256) k_permute_rows(int nV, const int32_t* __restrict__ colVerts, int to,
+                                                      const double* __restrict__ a, double* __restrict__ b) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3ll * nV) return;
+  const int i = (int)(t / 3), l = (int)(t % 3);
+  const int64_t v = 3ll * colVerts[i] + l;
+  if (to)
+    b[t] = a[v];
+  else
+    b[v] = a[t];
 }
 
 // ---- block IC(0), one colour at a time ----------------------------------------------------------------
@@ -164,19 +318,65 @@ __device__ __forceinline__ void masked_block(const V* __restrict__ vals, const i
   }
 }
 
+// the masked diagonal block of v: from hdiag (FP64, 9 per vertex) when given, else from the pattern
+template <typename V>
+__device__ __forceinline__ void masked_diag(const V* __restrict__ vals, const int32_t* __restrict__ bptr, int v,
+                                            int slot, const double* __restrict__ hdiag,
+                                            const uint8_t* __restrict__ fixed, double* A) {
+  if (!hdiag) {
+    masked_block(vals, bptr, v, slot, fixed, v, A);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) A[i] = hdiag[9ll * v + i];
+  if (fixed) {
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (fixed[3ll * v + l] || fixed[3ll * v + k]) A[3 * l + k] = (l == k) ? 1.0 : 0.0;
+  }
+}
+
+// the masked Hessian in the solve order: for position i (vertex v = colVerts[i]) its diagonal block
+// (symmetrised) and its lower blocks, one thread per position
+template <typename V>
+__global__ void __launch_bounds__(256)
+    k_split_hessian(int nV, const int32_t* __restrict__ colVerts, const int32_t* __restrict__ bptr,
+                    const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const int32_t* __restrict__ dslot,
+                    const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlsrc,
+                    const int32_t* __restrict__ lslot, const int32_t* __restrict__ lcol,
+                    const double* __restrict__ hdiag, V* __restrict__ hl, double* __restrict__ hd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const int v = colVerts[i];
+  double A[9];
+  masked_diag(vals, bptr, v, dslot[v], hdiag, fixed, A);
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hd[9ll * i + 3 * l + k] = 0.5 * (A[3 * l + k] + A[3 * k + l]);
+  for (int t = nlptr[i]; t < nlptr[i + 1]; ++t) {
+    const int k = nlsrc[t];
+    masked_block(vals, bptr, v, lslot[k], fixed, lcol[k], A);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) hl[9ll * t + e] = (V)A[e];
+  }
+}
+
 template <typename V>
 __global__ void __launch_bounds__(256)
     k_bic0_color(const int32_t* __restrict__ verts, int nverts, const int32_t* __restrict__ bptr,
                  const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const int32_t* __restrict__ lptr,
                  const int32_t* __restrict__ lcol, const int32_t* __restrict__ lslot,
-                 const int32_t* __restrict__ dslot, double shift, float* __restrict__ lval, float* __restrict__ dinv,
-                 int32_t* __restrict__ fail) {
+                 const int32_t* __restrict__ dslot, const double* __restrict__ hdiag, double shift,
+                 float* __restrict__ lval, float* __restrict__ dinv, int32_t* __restrict__ fail) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nverts) return;
   const int v = verts[i];
   const int k0 = lptr[v], k1 = lptr[v + 1];
   double D[9];
-  masked_block(vals, bptr, v, dslot[v], fixed, v, D);
+  masked_diag(vals, bptr, v, dslot[v], hdiag, fixed, D);
   const double sd0 = D[0], sd4 = D[4], sd8 = D[8];
   D[0] += shift * fabs(sd0);
   D[4] += shift * fabs(sd4);
@@ -256,20 +456,53 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   const int nV = m->nV;
   const std::vector<int32_t>& bp = m->h_bptr;
   const std::vector<int32_t>& bc = m->h_bcol;
-  // greedy colouring in vertex order
+  // DSatur colouring (Brélaz 1979): repeatedly colour the uncoloured vertex whose neighbours already
+  // use the most distinct colours (ties: lowest index) with the smallest free colour.  On the Kuhn
+  // beams this finds 4 balanced colours where greedy colouring in vertex order needs 8; every colour is
+  // one kernel launch per sweep.  (Greedy colouring in vertex order is kept as an option.)
   std::vector<int32_t> color(nV, -1);
-  std::vector<char> used;
   int nc = 0;
-  for (int v = 0; v < nV; ++v) {
-    used.assign(nc + 1, 0);
-    for (int k = bp[v]; k < bp[v + 1]; ++k) {
-      const int c = color[bc[k]];
-      if (c >= 0) used[c] = 1;
+  if (m->coloring == 2) {  // greedy in vertex order
+    std::vector<char> used;
+    for (int v = 0; v < nV; ++v) {
+      used.assign(nc + 1, 0);
+      for (int k = bp[v]; k < bp[v + 1]; ++k) {
+        const int c = color[bc[k]];
+        if (c >= 0) used[c] = 1;
+      }
+      int c = 0;
+      while (used[c]) ++c;
+      color[v] = c;
+      nc = std::max(nc, c + 1);
     }
-    int c = 0;
-    while (used[c]) ++c;
-    color[v] = c;
-    nc = std::max(nc, c + 1);
+  } else {
+    std::vector<uint64_t> sat(nV, 0);  // bit c: some neighbour has colour c (colours >= 64 share bit 63)
+    std::priority_queue<std::pair<int, int>> heap;  // (saturation, -vertex), stale entries skipped
+    for (int v = 0; v < nV; ++v) heap.emplace(0, -v);
+    std::vector<char> used;
+    while (!heap.empty()) {
+      const auto top = heap.top();
+      heap.pop();
+      const int v = -top.second;
+      if (color[v] >= 0 || top.first != __builtin_popcountll(sat[v])) continue;
+      used.assign(nc + 1, 0);
+      for (int k = bp[v]; k < bp[v + 1]; ++k) {
+        const int c = color[bc[k]];
+        if (c >= 0) used[c] = 1;
+      }
+      int c = 0;
+      while (used[c]) ++c;
+      color[v] = c;
+      nc = std::max(nc, c + 1);
+      const uint64_t bit = uint64_t(1) << std::min(c, 63);
+      for (int k = bp[v]; k < bp[v + 1]; ++k) {
+        const int n = bc[k];
+        if (color[n] < 0 && !(sat[n] & bit)) {
+          sat[n] |= bit;
+          heap.emplace(__builtin_popcountll(sat[n]), -n);
+        }
+      }
+    }
   }
   P->ncolors = nc;
   std::vector<int32_t> colPtr(nc + 1, 0), colVerts(nV);
@@ -315,6 +548,30 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   }
   P->nlower = (int64_t)lcol.size();
   P->h_colPtr = colPtr;
+  // the solve order: position i <- vertex colVerts[i]; lower / upper lists re-indexed by position
+  std::vector<int32_t> pos(nV), nlptr(nV + 1, 0), nlcol, nlsrc, nuptr(nV + 1, 0), nucol, nulink;
+  for (int i = 0; i < nV; ++i) pos[colVerts[i]] = i;
+  std::vector<int32_t> newLower(P->nlower);  // vertex-order lower index -> solve-order index
+  nlcol.reserve(P->nlower);
+  nlsrc.reserve(P->nlower);
+  for (int i = 0; i < nV; ++i) {
+    const int v = colVerts[i];
+    for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+      newLower[k] = (int32_t)nlcol.size();
+      nlcol.push_back(pos[lcol[k]]);
+      nlsrc.push_back(k);
+    }
+    nlptr[i + 1] = (int32_t)nlcol.size();
+  }
+  for (int i = 0; i < nV; ++i) {
+    const int v = colVerts[i];
+    for (int k = uptr[v]; k < uptr[v + 1]; ++k) {
+      if (ulink[k] < 0) return BH_BAD_ARG;  // the pattern is not symmetric
+      nucol.push_back(pos[ucol[k]]);
+      nulink.push_back(newLower[ulink[k]]);
+    }
+    nuptr[i + 1] = (int32_t)nucol.size();
+  }
   auto up = [](int32_t** d, const std::vector<int32_t>& h) -> int {
     BH_CHECK(hipMalloc(d, sizeof(int32_t) * std::max<size_t>(1, h.size())));
     if (!h.empty()) BH_CHECK(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
@@ -323,47 +580,131 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   int st;
   if ((st = up(&P->colVerts, colVerts)) || (st = up(&P->lptr, lptr)) || (st = up(&P->lcol, lcol)) ||
       (st = up(&P->lslot, lslot)) || (st = up(&P->uptr, uptr)) || (st = up(&P->ucol, ucol)) ||
-      (st = up(&P->ulink, ulink)) || (st = up(&P->dslot, dslot)))
+      (st = up(&P->ulink, ulink)) || (st = up(&P->dslot, dslot)) || (st = up(&P->nlptr, nlptr)) ||
+      (st = up(&P->nlcol, nlcol)) || (st = up(&P->nlsrc, nlsrc)) || (st = up(&P->nuptr, nuptr)) ||
+      (st = up(&P->nucol, nucol)) || (st = up(&P->nulink, nulink)))
     return st;
-  BH_CHECK(hipMalloc(&P->lval, sizeof(float) * 9 * std::max<int64_t>(1, P->nlower)));
-  BH_CHECK(hipMalloc(&P->uval, sizeof(float) * 9 * std::max<int64_t>(1, P->nlower)));
+  const int64_t nl1 = std::max<int64_t>(1, P->nlower);
+  BH_CHECK(hipMalloc(&P->lval, sizeof(float) * 9 * nl1));
   BH_CHECK(hipMalloc(&P->dinv, sizeof(float) * 6 * std::max(1, nV)));
   BH_CHECK(hipMalloc(&P->fail, sizeof(int32_t)));
-  BH_CHECK(hipMalloc(&P->ytmp, sizeof(float) * 3 * std::max(1, nV)));
-  BH_CHECK(hipMalloc(&P->ztmp, sizeof(float) * 3 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->lvs, sizeof(float) * 9 * nl1));
+  BH_CHECK(hipMalloc(&P->uvs, sizeof(float) * 9 * nl1));
+  BH_CHECK(hipMalloc(&P->dinvs, sizeof(float) * 6 * std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->hsplit, sizeof(double) * 18 * nl1));
+  BH_CHECK(hipMalloc(&P->hdiag, sizeof(double) * 9 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->sv, sizeof(double) * 8 * 3 * (int64_t)std::max(1, nV)));
   return BH_OK;
 }
 
 void precond_free(bh_precond* P) {
-  void* ptrs[] = {P->colVerts, P->lptr, P->lcol, P->lslot, P->uptr, P->ucol, P->ulink, P->dslot,
-                  P->lval, P->uval, P->dinv, P->fail, P->ytmp, P->ztmp};
+  void* ptrs[] = {P->colVerts, P->lptr, P->lcol, P->lslot, P->uptr, P->ucol, P->ulink, P->dslot, P->lval, P->dinv,
+                  P->fail, P->nlptr, P->nlcol, P->nlsrc, P->nuptr, P->nucol, P->nulink, P->lvs, P->uvs,
+                  P->dinvs, P->hsplit, P->hdiag, P->sv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
 
-// after every change of lval: the upper-list copy read by the backward sweep
+// solve vectors (3 nV doubles each)
+enum { SV_B, SV_X, SV_R, SV_P, SV_S, SV_U, SV_W, SV_Y };
+inline double* svec(const bh_precond* P, int k) { return P->sv + (int64_t)k * 3 * P->mesh->nV; }
+
+// the split Hessian (type V): lower blocks, transposed copy, diagonal blocks
+template <typename V>
+struct SplitH {
+  V *hl, *hu;
+};
+template <typename V>
+SplitH<V> split_view(const bh_precond* P) {
+  V* base = reinterpret_cast<V*>(P->hsplit);
+  const int64_t nl = 9 * std::max<int64_t>(1, P->nlower);
+  return SplitH<V>{base, base + nl};
+}
+
+// after every change of lval / dinv: the factor in the solve order (both list orders)
 int precond_finalize(bh_precond* P, hipStream_t s) {
-  if (P->nlower == 0) return BH_OK;
-  hipLaunchKernelGGL(k_fill_uval, dim3((P->nlower + 255) / 256), dim3(256), 0, s, P->nlower, P->ulink, P->lval, P->uval);
+  const int nV = P->mesh->nV;
+  if (nV)
+    hipLaunchKernelGGL(k_factor_solve_order, dim3((nV + 255) / 256), dim3(256), 0, s, nV, P->colVerts, P->dinv,
+                       P->lval, P->nlptr, P->nlsrc, P->nuptr, P->nulink, P->dinvs, P->lvs, P->uvs);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
-// z = (L L^T)^{-1} r
-int precond_apply(const bh_precond* P, const double* r, double* z, const int* done, hipStream_t s) {
+// the masked Hessian in the solve order (once per Newton system)
+template <typename V>
+int precond_split_hessian(const bh_precond* P, const V* vals, const double* hdiag, const uint8_t* fixed,
+                          hipStream_t s) {
+  const bh_mesh* m = P->mesh;
+  const int nV = m->nV;
+  SplitH<V> h = split_view<V>(P);
+  if (nV)
+    hipLaunchKernelGGL(k_split_hessian<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, P->colVerts, m->bptr, vals,
+                       fixed, P->dslot, P->nlptr, P->nlsrc, P->lslot, P->lcol, hdiag, h.hl, P->hdiag);
+  if (P->nlower)
+    hipLaunchKernelGGL((k_copy_blocks<V, V>), dim3((P->nlower + 255) / 256), dim3(256), 0, s, P->nlower, P->nulink,
+                       (const V*)h.hl, h.hu);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+// blocks of one colour's sweep kernel (one pass per block)
+inline int sweep_grid(int n) { return std::max(1, (n + kSweepVpb - 1) / kSweepVpb); }
+
+// blocks of one whole sweep (= dot-product partials it leaves)
+int64_t precond_sweep_blocks(const bh_precond* P) {
+  int64_t t = 0;
+  for (int c = 0; c < P->ncolors; ++c)
+    if (P->h_colPtr[c + 1] > P->h_colPtr[c]) t += sweep_grid(P->h_colPtr[c + 1] - P->h_colPtr[c]);
+  return t;
+}
+
+// z = (L L^T)^{-1} r in the solve order.  With cg set, the forward sweep first applies the CG vector
+// update of each colour's rows and the sweeps leave r.r (forward) and r.z (backward) partials, colour
+// after colour, in cg->part_rr and part_rz; nparts receives how many each wrote.
+int precond_apply_solve(const bh_precond* P, double* r, double* z, hipStream_t s, const CgVecs* cg = nullptr,
+                        int update = 0, double* part_rz = nullptr, int* nparts = nullptr) {
   const int nc = P->ncolors;
+  double* y = svec(P, SV_Y);
+  CgVecs c0{};
+  int off = 0;
   for (int c = 0; c < nc; ++c) {
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
-    hipLaunchKernelGGL(k_sweep_fwd, dim3((kSweepLanes * n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c],
-                       n, P->lptr, P->lcol, P->lval, P->dinv, r, P->ytmp, done);
+    const int g = sweep_grid(n);
+    CgVecs cc = cg ? *cg : c0;
+    if (cg && cg->part_rr) cc.part_rr = cg->part_rr + off;
+    hipLaunchKernelGGL(k_sweep_fwd, dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol, P->lvs,
+                       P->dinvs, r, y, cc, update);
+    off += g;
   }
+  if (nparts) nparts[0] = off;
+  off = 0;
   for (int c = nc - 1; c >= 0; --c) {
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
-    hipLaunchKernelGGL(k_sweep_bwd, dim3((kSweepLanes * n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c],
-                       n, P->uptr, P->ucol, P->uval, P->dinv, P->ytmp, P->ztmp, z, done);
+    const int g = sweep_grid(n);
+    hipLaunchKernelGGL(k_sweep_bwd, dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol, P->uvs,
+                       P->dinvs, (const double*)y, z, part_rz ? (const double*)r : nullptr,
+                       part_rz ? part_rz + off : nullptr, cg ? cg->st : nullptr);
+    off += g;
   }
+  if (nparts) nparts[1] = off;
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+// z = (L L^T)^{-1} r with r, z in vertex order
+int precond_apply(const bh_precond* P, const double* r, double* z, hipStream_t s) {
+  const int nV = P->mesh->nV;
+  if (!nV) return BH_OK;
+  const int g = (int)((3ll * nV + 255) / 256);
+  double* rs = svec(P, SV_R);
+  double* zs = svec(P, SV_U);
+  hipLaunchKernelGGL(k_permute_rows, dim3(g), dim3(256), 0, s, nV, P->colVerts, 1, r, rs);
+  int st = precond_apply_solve(P, rs, zs, s);
+  if (st) return st;
+  hipLaunchKernelGGL(k_permute_rows, dim3(g), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)zs, z);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -371,15 +712,16 @@ int precond_apply(const bh_precond* P, const double* r, double* z, const int* do
 int precond_finalize(bh_precond* P, hipStream_t s);
 
 template <typename V>
-int precond_factor_bic0(bh_precond* P, const V* vals, const uint8_t* fixed, double shift, int* failures,
-                        hipStream_t s) {
+int precond_factor_bic0(bh_precond* P, const V* vals, const double* hdiag, const uint8_t* fixed, double shift,
+                        int* failures, hipStream_t s) {
   bh_mesh* m = P->mesh;
   BH_CHECK(hipMemsetAsync(P->fail, 0, sizeof(int32_t), s));
   for (int c = 0; c < P->ncolors; ++c) {
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
     hipLaunchKernelGGL(k_bic0_color<V>, dim3((n + 255) / 256), dim3(256), 0, s, P->colVerts + P->h_colPtr[c], n,
-                       m->bptr, vals, fixed, P->lptr, P->lcol, P->lslot, P->dslot, shift, P->lval, P->dinv, P->fail);
+                       m->bptr, vals, fixed, P->lptr, P->lcol, P->lslot, P->dslot, hdiag, shift, P->lval, P->dinv,
+                       P->fail);
   }
   BH_CHECK(hipGetLastError());
   if (int st = precond_finalize(P, s)) return st;
@@ -429,7 +771,7 @@ int bh_precond_info(const bh_precond* P, int* ncolors, int64_t* nlower) {
 int bh_precond_factor_bic0_device(bh_precond* P, const double* d_vals, const uint8_t* d_fixed_dof, double shift,
                                   int* failures, bh_stream_t s) {
   if (!P || !d_vals) return BH_BAD_ARG;
-  return precond_factor_bic0(P, d_vals, d_fixed_dof, shift, failures, as_stream(s));
+  return precond_factor_bic0(P, d_vals, (const double*)nullptr, d_fixed_dof, shift, failures, as_stream(s));
 }
 
 int bh_precond_set_factor_device(bh_precond* P, const float* d_lower, const float* d_diag_inv, bh_stream_t s) {
@@ -454,7 +796,7 @@ int bh_precond_lists_device(const bh_precond* P, const int32_t** lptr, const int
 
 int bh_precond_apply_device(bh_precond* P, const double* d_r, double* d_z, bh_stream_t s) {
   if (!P || !d_r || !d_z) return BH_BAD_ARG;
-  return precond_apply(P, d_r, d_z, nullptr, as_stream(s));
+  return precond_apply(P, d_r, d_z, as_stream(s));
 }
 
 }  // extern "C"

@@ -111,6 +111,8 @@ int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
 typedef struct bh_tuning {
   int gather_lanes; /* 0 keep, else 1 or 2 lanes per vertex in the assembly gather */
   int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
+  int coloring;     /* 0 keep, 1 DSatur (default), 2 greedy in vertex order: the vertex colouring that
+                       bh_precond_create uses for the colour sweeps */
 } bh_tuning;
 int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
 
