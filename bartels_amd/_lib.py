@@ -34,8 +34,9 @@ class BhElementOpts(ctypes.Structure):
 
 class BhTuning(ctypes.Structure):
     """schedule knobs of a mesh: gather lanes and XCD grouping (same per-slot fold order), and the vertex
-    colouring of the colour sweeps (0 keep, 1 DSatur, 2 greedy)"""
-    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int), ("coloring", c_int)]
+    colouring of the colour sweeps (0 keep, 1 DSatur, 2 greedy) and the lanes per vertex of the Gauss-Seidel
+    sweeps (0 keep, 4, 8)"""
+    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int), ("coloring", c_int), ("sweep_lanes", c_int)]
 
 
 # void (*on_system)(void* user, int iteration, const float* vals32, const double* vals64, const double* b,
@@ -70,6 +71,7 @@ class BhNewtonOpts(ctypes.Structure):
         ("d_mass_vals", c_void_p),
         ("d_q_pred", c_void_p),
         ("inertia", c_double),
+        ("relaxation", c_double),
     ]
 
 

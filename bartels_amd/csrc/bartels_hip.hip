@@ -73,6 +73,7 @@ struct bh_mesh {
   int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
   int coloring = 1;                               // colour sweeps: 1 DSatur, 2 greedy in vertex order
+  int sweepLanes = 4;                             // lanes per vertex in the Gauss-Seidel sweeps (4 or 8)
   bh_precond* solveOrder = nullptr;               // colour lists for the precision form's block Jacobi
   // scratch
   double* partials = nullptr;
@@ -1414,6 +1415,8 @@ int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2) return BH_BAD_ARG;
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
+  if (t->sweep_lanes != 0 && t->sweep_lanes != 4 && t->sweep_lanes != 8) return BH_BAD_ARG;
+  if (t->sweep_lanes) m->sweepLanes = t->sweep_lanes;
   if (t->coloring && t->coloring != m->coloring) {
     m->coloring = t->coloring;
     if (m->solveOrder) {  // made with the old colouring

@@ -38,6 +38,29 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 __device__ __forceinline__ double block_sum_256(double v, double* sh) { return block_sum<256>(v, sh); }
 
+// inverse of 3x3 blocks (row-major, 9 per block; point = 1 keeps only the diagonal); singular blocks
+// fall back to the inverse of their diagonal
+__device__ __forceinline__ void inv3_block(double* A, int point, double* o) {
+  if (point) A[1] = A[2] = A[3] = A[5] = A[6] = A[7] = 0.0;
+  const double c0 = A[4] * A[8] - A[5] * A[7], c1 = A[5] * A[6] - A[3] * A[8], c2 = A[3] * A[7] - A[4] * A[6];
+  const double det = A[0] * c0 + A[1] * c1 + A[2] * c2;
+  if (!(fabs(det) > 0.0) || !isfinite(det)) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? (A[i] != 0.0 ? 1.0 / A[i] : 1.0) : 0.0;
+    return;
+  }
+  const double id = 1.0 / det;
+  o[0] = c0 * id;
+  o[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+  o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+  o[3] = c1 * id;
+  o[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+  o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+  o[6] = c2 * id;
+  o[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+  o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+}
+
 // Ap = mask(H p), partial p.Ap per block.  16 lanes per vertex: lane j takes blocks j, j+16, ... of
 // the vertex's block row (its three rows are one contiguous CSR segment, so the 16 lanes read
 // neighbouring 3-value pieces), then the three row sums are reduced across the 16 lanes with a fixed

@@ -985,6 +985,52 @@ int mgn_factor(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hi
   return precond_finalize(P, s);
 }
 
+// The Eisenstat-form diagonal from the network, one thread per solve position i (vertex colVerts[i]):
+// Dt_i = s_v^2 Lhat_vv Lhat_vv^T with Lhat_vv = C_v T(d_v) (C_v the Cholesky factor of the scaled
+// masked H_vv), so zero decoders give Dt = D, block symmetric Gauss-Seidel; fixed components get
+// identity rows / columns.  Writes Dt, its inverse and K = 2 Dt - D for the sweeps.
+__global__ void __launch_bounds__(256)
+    k_mgn_sgs_out(int nV, const int32_t* __restrict__ colVerts, const uint8_t* __restrict__ fixed,
+                  const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ s,
+                  const double* __restrict__ hd, float* __restrict__ dt, float* __restrict__ dtinv,
+                  double* __restrict__ kk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const int v = colVerts[i];
+  const double sv = s[v];
+  float L[9];
+  mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+  bool fv[3] = {false, false, false};
+  if (fixed)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+  double D[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double x = sv * sv *
+                       ((double)L[3 * l] * L[3 * c] + (double)L[3 * l + 1] * L[3 * c + 1] + (double)L[3 * l + 2] * L[3 * c + 2]);
+      D[3 * l + c] = (fv[l] || fv[c]) ? ((l == c) ? 1.0 : 0.0) : x;
+    }
+  eis_store_diag(i, D, hd, dt, dtinv, kk);
+}
+
+// the network's Eisenstat-form diagonal for the split Hessian already in P
+template <typename V>
+int mgn_sgs(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hipStream_t s) {
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  MgnWs w;
+  mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
+  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  const int nV = P->mesh->nV;
+  hipLaunchKernelGGL(k_mgn_sgs_out, g256(nV), dim3(256), 0, s, nV, P->colVerts, fixed, w.C, w.d, w.s,
+                     (const double*)P->hdiag, P->dt, P->dtinv, P->kk);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
 // portable parameter initialisation (splitmix64 uniforms): He-uniform first layers, second layers
 // scaled by 1/4 (residual branches start small), zero biases and zero decoders
 void mgn_init_host(std::vector<float>& w, uint64_t seed) {

@@ -113,29 +113,6 @@ __global__ void __launch_bounds__(256) k_block_jacobi(int nV, const int32_t* __r
   o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
 }
 
-// inverse of 3x3 blocks (row-major, 9 per block; point = 1 keeps only the diagonal); singular blocks
-// fall back to the inverse of their diagonal
-__device__ __forceinline__ void inv3_block(double* A, int point, double* o) {
-  if (point) A[1] = A[2] = A[3] = A[5] = A[6] = A[7] = 0.0;
-  const double c0 = A[4] * A[8] - A[5] * A[7], c1 = A[5] * A[6] - A[3] * A[8], c2 = A[3] * A[7] - A[4] * A[6];
-  const double det = A[0] * c0 + A[1] * c1 + A[2] * c2;
-  if (!(fabs(det) > 0.0) || !isfinite(det)) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? (A[i] != 0.0 ? 1.0 / A[i] : 1.0) : 0.0;
-    return;
-  }
-  const double id = 1.0 / det;
-  o[0] = c0 * id;
-  o[1] = (A[2] * A[7] - A[1] * A[8]) * id;
-  o[2] = (A[1] * A[5] - A[2] * A[4]) * id;
-  o[3] = c1 * id;
-  o[4] = (A[0] * A[8] - A[2] * A[6]) * id;
-  o[5] = (A[2] * A[3] - A[0] * A[5]) * id;
-  o[6] = c2 * id;
-  o[7] = (A[1] * A[6] - A[0] * A[7]) * id;
-  o[8] = (A[0] * A[4] - A[1] * A[3]) * id;
-}
-
 __global__ void __launch_bounds__(256) k_block_inv(int nV, const double* __restrict__ hd, int point,
                                                    double* __restrict__ Dinv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,12 +193,14 @@ inline int vec_blocks(int nV) { return std::max(1, std::min((nV + kRedBlock - 1)
 
 // first stage of the CG reduction: kRed1Blocks blocks sum fixed slices of the three regions
 __global__ void __launch_bounds__(1024)
-    k_cg_reduce1(const double* __restrict__ part, int64_t region, int nru, int nrr, int nwu, double* __restrict__ out) {
+    k_cg_reduce1(const double* __restrict__ pru, const double* __restrict__ prr, const double* __restrict__ pwu,
+                 int nru, int nrr, int nwu, double* __restrict__ out) {
   __shared__ double sh[16];
   const int cnt[3] = {nru, nrr, nwu};
+  const double* srcs[3] = {pru, prr, pwu};
 #pragma unroll
   for (int g = 0; g < 3; ++g) {
-    const double* src = part + g * region;
+    const double* src = srcs[g];
     const int per = (cnt[g] + kRed1Blocks - 1) / kRed1Blocks;
     const int lo = blockIdx.x * per, hi = min(cnt[g], lo + per);
     double a = 0.0;
@@ -247,6 +226,16 @@ __global__ void __launch_bounds__(kRedBlock)
   }
   rr = block_sum_256(rr, sh);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// Eisenstat-form start: x~ = 0, p~ = s~ = 0
+__global__ void __launch_bounds__(kRedBlock) k_eis_init(int64_t n, double* __restrict__ xt, double* __restrict__ pt,
+                                                       double* __restrict__ st) {
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    xt[i] = 0.0;
+    pt[i] = 0.0;
+    st[i] = 0.0;
+  }
 }
 
 // second stage of the reduction (kRed1Blocks partials per quantity) and the CG scalar step
@@ -453,12 +442,18 @@ int read_scalar(const double* d, double* h, hipStream_t s) {
 }
 
 // the two reduction stages and the scalar step (partial counts of the three regions)
+inline void cg_reduce_regions(const NewtonWs& w, int nV, const double* pru, int nru, const double* prr, int nrr,
+                              const double* pwu, int nwu, int start, double tol, int maxit, hipStream_t s) {
+  const int64_t R = part_region(nV);
+  double* stage2 = w.part + 3 * R;
+  hipLaunchKernelGGL(k_cg_reduce1, dim3(kRed1Blocks), dim3(1024), 0, s, pru, prr, pwu, nru, nrr, nwu, stage2);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, s, stage2, start, tol, maxit, w.st);
+}
 inline void cg_reduce(const NewtonWs& w, int nV, int nru, int nrr, int nwu, int start, const bh_newton_opts* o,
                       hipStream_t s) {
   const int64_t R = part_region(nV);
-  double* stage2 = w.part + 3 * R;
-  hipLaunchKernelGGL(k_cg_reduce1, dim3(kRed1Blocks), dim3(1024), 0, s, w.part, R, nru, nrr, nwu, stage2);
-  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, s, stage2, start, o->pcg_tol, o->pcg_max_iterations, w.st);
+  cg_reduce_regions(w, nV, w.part, nru, w.part + R, nrr, w.part + 2 * R, nwu, start, o->pcg_tol,
+                    o->pcg_max_iterations, s);
 }
 
 int cg_wait(const NewtonWs& w, PcgState* hs, hipStream_t s) {
@@ -555,6 +550,93 @@ int pcg_solve_split_jacobi(bh_mesh* m, const bh_newton_opts* o, const V* vals, c
   return BH_OK;
 }
 
+// PCG with the Eisenstat-form preconditioner M = (Dt + L) Dt^{-1} (Dt + L)^T in the solve order of P (Dt = D:
+// block symmetric Gauss-Seidel; or the network's Dt):
+// CG on A~ = P^{-1} A P^{-T} with Dt as its preconditioner (one backward and one forward sweep per
+// iteration, Chronopoulos-Gear recurrences, one reduction).  The transformed residual is an
+// M^{-1}-norm estimate of the true one; once it is within kEisCheck * tol the residual of A x = b
+// itself (x = P^{-T} x~) is computed at the end of a batch and decides convergence.  The first check
+// comes once the transformed residual has fallen kEisCheck^2 below its start; every check then measures
+// rho = (|r|^2 / |b|^2) / (gamma / gamma0), and the next one waits until rho * gamma / gamma0 predicts
+// the tolerance (within a factor 4).
+constexpr double kEisCheck = 100.0;
+
+template <typename V>
+int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag,
+                        const uint8_t* fixed, bh_precond* P, int learned, NewtonWs& w, int* iters, int* done,
+                        hipStream_t s) {
+  const int nV = m->nV;
+  const int64_t n = 3ll * nV;
+  const int gr = (int)((n + 255) / 256);
+  const int gi = vec_blocks(nV);
+  const int gsp = sweep_grid(nV);
+  const int64_t R = part_region(nV);
+  if (precond_sweep_blocks(P) > R) return BH_UNSUPPORTED;  // more colours than the regions hold
+  double* pru = w.part;
+  double* prr = w.part + R;
+  double* pwu = w.part + 2 * R;
+  int st = precond_split_hessian<V>(P, vals, hdiag, fixed, s);
+  if (st) return st;
+  if (learned) {
+    if (!o->mgn) return BH_BAD_ARG;
+    st = mgn_sgs<V>(o->mgn, P, vals, fixed, s);
+  } else {
+    const double w = (o->relaxation > 0.0 && o->relaxation < 2.0) ? o->relaxation : 1.0;
+    st = precond_sgs(P, 1.0 / w, s);
+  }
+  if (st) return st;
+  const SplitH<V> h = split_view<V>(P);
+  double* b = svec(P, SV_B);
+  double* xs = svec(P, SV_Y);  // x = P^{-T} x~ (FP64)
+  EisVecs v{svec(P, SV_R), svec(P, SV_X), svec(P, SV_P), svec(P, SV_S),
+            svec(P, SV_U), svec(P, SV_V), svec(P, SV_T), svec(P, SV_W)};
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)w.b, b);
+  double bb = 0.0;
+  if ((st = device_dot(b, b, n, w.part, w.scal + 10, s)) || (st = read_scalar(w.scal + 10, &bb, s))) return st;
+  const double tol = o->pcg_tol;
+  hipLaunchKernelGGL(k_eis_init, dim3(gi), dim3(kRedBlock), 0, s, n, v.xt, v.pt, v.st);
+  int ng = 0, nd = 0;
+  eis_fwd<V, double>(P, 0, v, b, v.rt, nullptr, nullptr, s);      // r~ = P^{-1} b
+  eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s, &ng);  // u = Dt r~, t = P^{-T} u
+  eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s, &nd);  // w~ = A~ u
+  cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 1, 0.0, o->pcg_max_iterations, s);
+  BH_CHECK(hipGetLastError());
+  PcgState hs{};
+  bool converged = bb == 0.0, fresh = false;
+  double rho = 1.0 / (kEisCheck * kEisCheck);  // predicted (|r|^2/|b|^2) per unit of gamma/gamma0
+  while (!converged) {
+    for (int k = 0; k < kPcgBatch; ++k) {
+      eis_bwd<V, double>(P, 2, v, nullptr, v.t, pru, w.st, s);
+      eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, w.st, s);
+      cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 0, 0.0, o->pcg_max_iterations, s);
+    }
+    if ((st = cg_wait(w, &hs, s))) return st;
+    fresh = false;
+    const double g = hs.bb > 0.0 ? hs.rr / hs.bb : 0.0;
+    if (rho * g <= 4.0 * tol * tol || hs.done) {
+      // x = P^{-T} x~ and the residual of A x = b (FP64)
+      eis_bwd<V, double>(P, 0, v, v.xt, xs, nullptr, nullptr, s);
+      double* r = svec(P, SV_Z);
+      hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
+                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)xs, r,
+                         (double*)nullptr, (const PcgState*)nullptr);
+      hipLaunchKernelGGL(k_resid, dim3(gr), dim3(256), 0, s, n, (const double*)b, r);
+      double rr = 0.0;
+      if ((st = device_dot(r, r, n, w.part, w.scal + 11, s)) || (st = read_scalar(w.scal + 11, &rr, s))) return st;
+      fresh = true;
+      if (rr <= tol * tol * bb) converged = true;
+      else if (g > 0.0) rho = (rr / bb) / g;
+    }
+    if (hs.done) break;
+  }
+  if (!fresh) eis_bwd<V, double>(P, 0, v, v.xt, xs, nullptr, nullptr, s);
+  hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)xs, w.x);
+  BH_CHECK(hipGetLastError());
+  *iters = hs.iters;
+  *done = converged ? 1 : (hs.done ? hs.done : 2);
+  return BH_OK;
+}
+
 // PCG with a colour-sweep block-triangular preconditioner, run in the preconditioner's solve order
 // (colours contiguous): the masked Hessian is split into lower / transposed-lower / diagonal blocks in
 // that order, b is permuted in and x out.  Same recurrences as above; the vector update runs inside
@@ -636,6 +718,21 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
     hdiag = w.hdiag;
   }
   *fallback = 0;
+  if (o->precond == BH_PRECOND_SGS || o->precond == BH_PRECOND_LEARNED_SGS) {
+    if (!o->precond_handle) return BH_BAD_ARG;
+    if ((st = pcg_solve_eisenstat(m, o, vals, hdiag, fixed, o->precond_handle,
+                                  o->precond == BH_PRECOND_LEARNED_SGS ? 1 : 0, w, pits, pdone, s)))
+      return st;
+    if (*pdone == 2 && o->jacobi_fallback) {
+      int p2 = 0;
+      st = hdiag ? pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, 0, w, &p2, pdone, s)
+                 : pcg_solve(m, o, vals, hdiag, fixed, w, &p2, pdone, s);
+      if (st) return st;
+      *pits += p2;
+      *fallback = 1;
+    }
+    return BH_OK;
+  }
   if (o->precond == BH_PRECOND_BIC0 || o->precond == BH_PRECOND_FACTOR || o->precond == BH_PRECOND_LEARNED) {
     if (!o->precond_handle) return BH_BAD_ARG;
     if (o->precond == BH_PRECOND_BIC0) {
@@ -711,6 +808,7 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->d_mass_vals = nullptr;
   o->d_q_pred = nullptr;
   o->inertia = 0.0;
+  o->relaxation = 1.0;
 }
 
 

@@ -44,7 +44,14 @@ struct bh_precond {
   float* dinvs = nullptr;       // 6 per position
   double* hsplit = nullptr;     // masked Hessian: lower blocks and their transposed copy (FP32 or FP64)
   double* hdiag = nullptr;      // masked diagonal blocks (FP64, 9 per position)
-  double* sv = nullptr;         // solve vectors, 3 nV each: b, x, r, p, s, u, w, y
+  double* sv = nullptr;         // solve vectors, 3 nV each: b, x, r, p, s, u, w, y, t, v, z
+  // Eisenstat form: the block diagonal Dt and its inverse (FP32, 9 per position) and K = 2 Dt' - D in
+  // FP64, with Dt' the exact inverse of the stored FP32 inverse (the identity A = P + P^T - K that the
+  // sweeps rely on then holds to FP64 rounding)
+  float* dt = nullptr;
+  float* dtinv = nullptr;
+  double* kk = nullptr;
+  int eisLanes = 4;             // lanes per position in the Eisenstat sweeps (4 or 8)
 };
 
 namespace {
@@ -244,6 +251,223 @@ __global__ void __launch_bounds__(256)
     const double d = block_sum_256(pap, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = d;
   }
+}
+
+// ---- Eisenstat form ------------------------------------------------------------------------------
+// M = P Dt^{-1} P^T with P = Dt + L, L the strictly lower blocks of the masked H in colour order and Dt
+// a block diagonal (D: block symmetric Gauss-Seidel, or written by the network).  With A = L + D + L^T = P + P^T - K, K = 2 Dt - D,
+//   P^{-1} A P^{-T} v = t + P^{-1} (v - K t),   t = P^{-T} v        (Eisenstat 1981),
+// one backward sweep over the upper blocks and one forward sweep over the lower blocks: a single read
+// of H per CG iteration and no separate product with H.  CG runs on P^{-1} A P^{-T} with Dt as its
+// block-diagonal preconditioner (in exact arithmetic PCG with M on A); the residual of A x = b is
+// checked directly (x = P^{-T} x~, r = b - A x) whenever the transformed residual says it may be met.
+
+// Store an Eisenstat diagonal Dt (FP64 in, one position): FP32 Dt and inverse, FP64 K = 2 Dt' - D
+__device__ __forceinline__ void eis_store_diag(int64_t i, const double* Dt, const double* __restrict__ hd,
+                                               float* __restrict__ dt, float* __restrict__ dtinv,
+                                               double* __restrict__ kk) {
+  double A[9], Ai[9], Ae[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) A[e] = Dt[e];
+  inv3_block(A, 0, Ai);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    dt[9 * i + e] = (float)Dt[e];
+    const float fi = (float)Ai[e];
+    dtinv[9 * i + e] = fi;
+    Ai[e] = (double)fi;
+  }
+  inv3_block(Ai, 0, Ae);  // Dt' = (stored inverse)^{-1}
+#pragma unroll
+  for (int e = 0; e < 9; ++e) kk[9 * i + e] = 2.0 * Ae[e] - hd[9 * i + e];
+}
+
+// block symmetric Gauss-Seidel: Dt = D, the masked diagonal blocks.  (A D-ILU diagonal,
+// Dt_i = D_i - sum H_in Dt_n^{-1} H_in^T, was tried first: on these elasticity Hessians, which are not
+// M-matrices, it turns indefinite and CG stalls.)
+__global__ void __launch_bounds__(256) k_sgs_diag(int nV, const double* __restrict__ hd, double scale,
+                                                  float* __restrict__ dt, float* __restrict__ dtinv,
+                                                  double* __restrict__ kk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  double D[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) D[e] = scale * hd[9ll * i + e];
+  eis_store_diag(i, D, hd, dt, dtinv, kk);
+}
+
+// transformed-space CG vectors and the sweep outputs (solve order), all FP64: with the search and
+// product vectors in FP32 the Chronopoulos-Gear recurrences lost conjugacy and CG needed 7x the
+// iterations on the 6.07M-tet beam.
+struct EisVecs {
+  double *rt, *xt;                 // r~, x~
+  double *pt, *st, *ut, *wt;       // p~, s~ (= A~ p~), u = Dt r~, w~ = A~ u
+  double *t, *w;                   // backward / forward sweep results of the product (gather sources)
+};
+
+template <typename T>
+__device__ __forceinline__ void mv3d(const T* A, double x0, double x1, double x2, double* y) {
+  y[0] = (double)A[0] * x0 + (double)A[1] * x1 + (double)A[2] * x2;
+  y[1] = (double)A[3] * x0 + (double)A[4] * x1 + (double)A[5] * x2;
+  y[2] = (double)A[6] * x0 + (double)A[7] * x1 + (double)A[8] * x2;
+}
+
+// backward sweep over one colour (colours high to low), LANES lanes per position, writing tb (type TT,
+// also the gather source):
+//   mode 0: tb = P^{-T} src
+//   mode 1: u = Dt r~, tb = P^{-T} u, partial r~.u                       (tb = v.t)
+//   mode 2: first the CG update of the position's rows (p~ = u + beta p~, s~ = w~ + beta s~,
+//           x~ += alpha p~, r~ -= alpha s~), then mode 1
+template <typename V, int LANES, typename TT>
+__global__ void __launch_bounds__(256)
+    k_eis_bwd(int c0, int n, const int32_t* __restrict__ nuptr, const int32_t* __restrict__ nucol,
+              const V* __restrict__ hu, const float* __restrict__ dt, const float* __restrict__ dtinv, int mode,
+              EisVecs v, const double* __restrict__ src, TT* __restrict__ tb, double* __restrict__ part,
+              const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && cg_skip(st)) return;
+  const int j = threadIdx.x % LANES;
+  const int li = blockIdx.x * (256 / LANES) + threadIdx.x / LANES;
+  const int i = c0 + li;
+  double aj = 0.0;  // lane j < 3: row j of the right-hand side
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (li < n) {
+    if (j < 3) {
+      const int64_t row = 3ll * i + j;
+      if (mode == 0) {
+        aj = src[row];
+      } else {
+        double rj = v.rt[row];
+        if (mode == 2) {
+          const double alpha = st->alpha, beta = st->beta;
+          const double pj = v.ut[row] + beta * v.pt[row];
+          const double sj = v.wt[row] + beta * v.st[row];
+          v.pt[row] = pj;
+          v.st[row] = sj;
+          v.xt[row] += alpha * pj;
+          rj -= alpha * sj;
+          v.rt[row] = rj;
+        }
+        aj = rj;
+      }
+    }
+    for (int k = nuptr[i] + j; k < nuptr[i + 1]; k += LANES) {
+      const V* B = hu + 9ll * k;  // H_ni; the product needs H_in = H_ni^T
+      const TT* tn = tb + 3ll * nucol[k];
+      const double t0 = tn[0], t1 = tn[1], t2 = tn[2];
+      s0 += (double)B[0] * t0 + (double)B[3] * t1 + (double)B[6] * t2;
+      s1 += (double)B[1] * t0 + (double)B[4] * t1 + (double)B[7] * t2;
+      s2 += (double)B[2] * t0 + (double)B[5] * t1 + (double)B[8] * t2;
+    }
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  const int lb = (int)(threadIdx.x & 63) - j;
+  const double a0 = __shfl(aj, lb, 64), a1 = __shfl(aj, lb + 1, 64), a2 = __shfl(aj, lb + 2, 64);
+  double g = 0.0;
+  if (li < n && j == 0) {
+    double rhs[3] = {a0, a1, a2};
+    if (mode >= 1) {
+      double u[3];
+      mv3d(dt + 9ll * i, a0, a1, a2, u);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        v.ut[3ll * i + l] = u[l];
+        rhs[l] = u[l];
+      }
+      g = a0 * rhs[0] + a1 * rhs[1] + a2 * rhs[2];
+    }
+    double tt[3];
+    mv3d(dtinv + 9ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, tt);
+    tb[3ll * i] = (TT)tt[0];
+    tb[3ll * i + 1] = (TT)tt[1];
+    tb[3ll * i + 2] = (TT)tt[2];
+  }
+  if (part) {
+    const double d = block_sum_256(g, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
+}
+
+// forward sweep over one colour (colours low to high), LANES lanes per position, writing db (type TT,
+// also the gather source):
+//   mode 0: db = P^{-1} src
+//   mode 1: db = w = P^{-1} (u - K t), w~ = t + w, partial w~.u           (db = v.w)
+template <typename V, int LANES, typename TT>
+__global__ void __launch_bounds__(256)
+    k_eis_fwd(int c0, int n, const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlcol,
+              const V* __restrict__ hl, const float* __restrict__ dtinv, const double* __restrict__ kk, int mode,
+              EisVecs v, const double* __restrict__ src, TT* __restrict__ db, double* __restrict__ part,
+              const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && cg_skip(st)) return;
+  const int j = threadIdx.x % LANES;
+  const int li = blockIdx.x * (256 / LANES) + threadIdx.x / LANES;
+  const int i = c0 + li;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (li < n) {
+    for (int k = nlptr[i] + j; k < nlptr[i + 1]; k += LANES) {
+      const V* B = hl + 9ll * k;  // H_in
+      const TT* wn = db + 3ll * nlcol[k];
+      const double w0 = wn[0], w1 = wn[1], w2 = wn[2];
+      s0 += (double)B[0] * w0 + (double)B[1] * w1 + (double)B[2] * w2;
+      s1 += (double)B[3] * w0 + (double)B[4] * w1 + (double)B[5] * w2;
+      s2 += (double)B[6] * w0 + (double)B[7] * w1 + (double)B[8] * w2;
+    }
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double d = 0.0;
+  if (li < n && j == 0) {
+    double rhs[3];
+    double t[3] = {0.0, 0.0, 0.0}, u[3] = {0.0, 0.0, 0.0};
+    if (mode == 0) {
+      rhs[0] = src[3ll * i];
+      rhs[1] = src[3ll * i + 1];
+      rhs[2] = src[3ll * i + 2];
+    } else {
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        t[l] = v.t[3ll * i + l];
+        u[l] = v.ut[3ll * i + l];
+      }
+      double kt[3];
+      mv3d(kk + 9ll * i, t[0], t[1], t[2], kt);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) rhs[l] = u[l] - kt[l];
+    }
+    double ww[3];
+    mv3d(dtinv + 9ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, ww);
+    db[3ll * i] = (TT)ww[0];
+    db[3ll * i + 1] = (TT)ww[1];
+    db[3ll * i + 2] = (TT)ww[2];
+    if (mode == 1) {
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const double wt = t[l] + ww[l];
+        v.wt[3ll * i + l] = wt;
+        d += wt * u[l];
+      }
+    }
+  }
+  if (part) {
+    const double dd = block_sum_256(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = dd;
+  }
+}
+
+// r = b - y in place of y
+__global__ void __launch_bounds__(256) k_resid(int64_t n, const double* __restrict__ b, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = b[i] - y[i];
 }
 
 // dst block t = src block link[t] (9 values each)
@@ -548,6 +772,7 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   }
   P->nlower = (int64_t)lcol.size();
   P->h_colPtr = colPtr;
+  P->eisLanes = m->sweepLanes;
   // the solve order: position i <- vertex colVerts[i]; lower / upper lists re-indexed by position
   std::vector<int32_t> pos(nV), nlptr(nV + 1, 0), nlcol, nlsrc, nuptr(nV + 1, 0), nucol, nulink;
   for (int i = 0; i < nV; ++i) pos[colVerts[i]] = i;
@@ -593,20 +818,23 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   BH_CHECK(hipMalloc(&P->dinvs, sizeof(float) * 6 * std::max(1, nV)));
   BH_CHECK(hipMalloc(&P->hsplit, sizeof(double) * 18 * nl1));
   BH_CHECK(hipMalloc(&P->hdiag, sizeof(double) * 9 * (int64_t)std::max(1, nV)));
-  BH_CHECK(hipMalloc(&P->sv, sizeof(double) * 8 * 3 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->sv, sizeof(double) * 11 * 3 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->dt, sizeof(float) * 9 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->dtinv, sizeof(float) * 9 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->kk, sizeof(double) * 9 * (int64_t)std::max(1, nV)));
   return BH_OK;
 }
 
 void precond_free(bh_precond* P) {
   void* ptrs[] = {P->colVerts, P->lptr, P->lcol, P->lslot, P->uptr, P->ucol, P->ulink, P->dslot, P->lval, P->dinv,
                   P->fail, P->nlptr, P->nlcol, P->nlsrc, P->nuptr, P->nucol, P->nulink, P->lvs, P->uvs,
-                  P->dinvs, P->hsplit, P->hdiag, P->sv};
+                  P->dinvs, P->hsplit, P->hdiag, P->sv, P->dt, P->dtinv, P->kk};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
 
 // solve vectors (3 nV doubles each)
-enum { SV_B, SV_X, SV_R, SV_P, SV_S, SV_U, SV_W, SV_Y };
+enum { SV_B, SV_X, SV_R, SV_P, SV_S, SV_U, SV_W, SV_Y, SV_T, SV_V, SV_Z };
 inline double* svec(const bh_precond* P, int k) { return P->sv + (int64_t)k * 3 * P->mesh->nV; }
 
 // the split Hessian (type V): lower blocks, transposed copy, diagonal blocks
@@ -692,6 +920,63 @@ int precond_apply_solve(const bh_precond* P, double* r, double* z, hipStream_t s
   if (nparts) nparts[1] = off;
   BH_CHECK(hipGetLastError());
   return BH_OK;
+}
+
+// the block SSOR diagonal Dt = scale * D of the split Hessian (scale = 1 / w; 1: symmetric Gauss-Seidel)
+inline int precond_sgs(bh_precond* P, double scale, hipStream_t s) {
+  const int nV = P->mesh->nV;
+  if (nV)
+    hipLaunchKernelGGL(k_sgs_diag, dim3((nV + 255) / 256), dim3(256), 0, s, nV, (const double*)P->hdiag, scale, P->dt,
+                       P->dtinv, P->kk);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
+// Eisenstat sweeps over all colours (tb / db: output and gather buffer; double for mode 0, v.t / v.w
+// otherwise); nparts (optional) receives the partial count
+template <typename V, typename TT>
+void eis_bwd(const bh_precond* P, int mode, const EisVecs& v, const double* src, TT* tb, double* part,
+             const PcgState* st, hipStream_t s, int* nparts = nullptr) {
+  const SplitH<V> h = split_view<V>(P);
+  const int lanes = P->eisLanes, vpb = 256 / lanes;
+  int off = 0;
+  for (int c = P->ncolors - 1; c >= 0; --c) {
+    const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
+    if (!n) continue;
+    const int g = (n + vpb - 1) / vpb;
+    if (lanes == 4)
+      hipLaunchKernelGGL((k_eis_bwd<V, 4, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol,
+                         (const V*)h.hu, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
+                         part ? part + off : nullptr, st);
+    else
+      hipLaunchKernelGGL((k_eis_bwd<V, 8, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol,
+                         (const V*)h.hu, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
+                         part ? part + off : nullptr, st);
+    off += g;
+  }
+  if (nparts) *nparts = off;
+}
+template <typename V, typename TT>
+void eis_fwd(const bh_precond* P, int mode, const EisVecs& v, const double* src, TT* db, double* part,
+             const PcgState* st, hipStream_t s, int* nparts = nullptr) {
+  const SplitH<V> h = split_view<V>(P);
+  const int lanes = P->eisLanes, vpb = 256 / lanes;
+  int off = 0;
+  for (int c = 0; c < P->ncolors; ++c) {
+    const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
+    if (!n) continue;
+    const int g = (n + vpb - 1) / vpb;
+    if (lanes == 4)
+      hipLaunchKernelGGL((k_eis_fwd<V, 4, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol,
+                         (const V*)h.hl, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
+                         part ? part + off : nullptr, st);
+    else
+      hipLaunchKernelGGL((k_eis_fwd<V, 8, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol,
+                         (const V*)h.hl, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
+                         part ? part + off : nullptr, st);
+    off += g;
+  }
+  if (nparts) *nparts = off;
 }
 
 // z = (L L^T)^{-1} r with r, z in vertex order

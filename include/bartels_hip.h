@@ -113,6 +113,8 @@ typedef struct bh_tuning {
   int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
   int coloring;     /* 0 keep, 1 DSatur (default), 2 greedy in vertex order: the vertex colouring that
                        bh_precond_create uses for the colour sweeps */
+  int sweep_lanes;  /* 0 keep, 4 (default) or 8: lanes per vertex in the symmetric Gauss-Seidel sweeps of
+                       preconditioners created afterwards */
 } bh_tuning;
 int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
 
@@ -180,7 +182,12 @@ typedef enum {
   BH_PRECOND_POINT_JACOBI = 1,
   BH_PRECOND_BIC0 = 2,         /* block IC(0) in colour order, refactored from H every Newton iteration */
   BH_PRECOND_FACTOR = 3,       /* a block lower-triangular factor supplied through precond_handle */
-  BH_PRECOND_LEARNED = 4       /* factor written by the mesh-graph network opts.mgn from each Hessian */
+  BH_PRECOND_LEARNED = 4,      /* factor written by the mesh-graph network opts.mgn from each Hessian */
+  BH_PRECOND_SGS = 5,          /* block symmetric Gauss-Seidel in colour order, M = (D + L) D^-1 (D + L)^T
+                                  with L the strictly lower part of H, applied with Eisenstat's trick (one read
+                                  of H per CG iteration, no separate product); needs precond_handle */
+  BH_PRECOND_LEARNED_SGS = 6   /* M = (Dt + L) Dt^-1 (Dt + L)^T with the block diagonal Dt written by the
+                                  mesh-graph network opts.mgn (Dt = D at zero decoders) */
 } bh_precond_kind;
 
 /* Block lower-triangular preconditioner M = L L^T on the Hessian's own 3x3-block pattern, in a
@@ -280,6 +287,7 @@ typedef struct {
   const double* d_mass_vals;
   const double* d_q_pred;
   double inertia;
+  double relaxation;       /* BH_PRECOND_SGS: SSOR relaxation w in (0, 2), Dt = D / w (0 means 1) */
 } bh_newton_opts;
 
 typedef struct {

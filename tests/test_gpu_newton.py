@@ -118,8 +118,9 @@ def gravity_problem(oracle):
 
 
 def test_newton_precision_form_and_preconditioners(gravity_problem):
-    """FP64 block-Jacobi, FP64 block IC(0) and the FP32 precision form reach the oracle's equilibrium;
-    the precision form within 1e-5 relative displacement"""
+    """FP64 block-Jacobi, block IC(0) and block symmetric Gauss-Seidel (Eisenstat form), FP64 and the FP32
+    precision form, reach the oracle's equilibrium; the precision form within 1e-5 relative displacement.
+    IC(0) and Gauss-Seidel need fewer CG iterations than block Jacobi."""
     from bartels_amd import _lib
 
     p, mesh, q0, fixed, f, q_or = gravity_problem
@@ -130,7 +131,9 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
     try:
         results = {}
         for name, kw in (("fp64_jacobi", dict()), ("fp64_bic0", dict(precond=2, handle=P)),
-                         ("fp32_jacobi", dict(precision=1)), ("fp32_bic0", dict(precision=1, precond=2, handle=P))):
+                         ("fp64_sgs", dict(precond=5, handle=P)), ("fp32_jacobi", dict(precision=1)),
+                         ("fp32_bic0", dict(precision=1, precond=2, handle=P)),
+                         ("fp32_sgs", dict(precision=1, precond=5, handle=P))):
             q, rep, tr = _newton(mesh, p, q0, fixed, f, **kw)
             err = np.linalg.norm(q - q_or) / u
             results[name] = (err, rep.iterations, rep.pcg_iterations_total, rep.gradient_norm / rep.gradient_norm0)
@@ -140,7 +143,11 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
         assert results["fp64_bic0"][0] < 1e-8
         assert results["fp32_jacobi"][0] < 1e-5
         assert results["fp32_bic0"][0] < 1e-5
+        assert results["fp64_sgs"][0] < 1e-8
+        assert results["fp32_sgs"][0] < 1e-5
         assert results["fp64_bic0"][2] < results["fp64_jacobi"][2]  # IC(0) needs fewer CG iterations
+        assert results["fp64_sgs"][2] < results["fp64_jacobi"][2]
+        assert results["fp32_sgs"][2] < results["fp32_jacobi"][2]
     finally:
         lib.bh_precond_destroy(P)
 
