@@ -41,6 +41,8 @@ struct bh_mgn {
   int64_t pvSize = 0;
   double* probe = nullptr;  // training probes: x_j, b_j
   size_t probeBytes = 0;
+  int form = 0;             // 0: general factor; 1: Eisenstat (symmetric Gauss-Seidel) form, O tied to H
+  int probeSolution = 0;    // 1: the vector handed to a training step is a solution x of H x = b (probe 0)
 };
 
 namespace {
@@ -662,6 +664,93 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The Eisenstat (symmetric Gauss-Seidel) form as a factor: L = (Dt + L_H) Lhat^{-T} in the normalised
+// space, i.e. the diagonal blocks Lhat_vv = C_v T(d_v) as in the general form and the off-diagonal
+// blocks tied to the Hessian, O_k = Hn_vn Lhat_nn^{-T} for the pair k = (v, n), n of lower colour
+// (Hn_vn: the masked, normalised block, the first 9 pair features).  Then L L^T = (Dt + L_H) Dt^{-1}
+// (Dt + L_H)^T with Dt = Lhat Lhat^T, the preconditioner that bh_newton applies with Eisenstat's trick.
+__device__ __forceinline__ void ltri_inv(const float* L, float* I) {  // row-major lower-triangular 3x3
+  const float i00 = 1.f / L[0], i11 = 1.f / L[4], i22 = 1.f / L[8];
+  const float i10 = -L[3] * i00 * i11;
+  const float i21 = -L[7] * i11 * i22;
+  const float i20 = -(L[6] * i00 + L[7] * i10) * i22;
+  I[0] = i00; I[1] = 0.f; I[2] = 0.f;
+  I[3] = i10; I[4] = i11; I[5] = 0.f;
+  I[6] = i20; I[7] = i21; I[8] = i22;
+}
+
+__global__ void __launch_bounds__(256)
+    k_mgn_tied_O(int64_t nP, const int32_t* __restrict__ pv, const int32_t* __restrict__ lcol,
+                 const float* __restrict__ fp, const float* __restrict__ C, const float* __restrict__ d,
+                 float* __restrict__ O) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nP) return;
+  const int n = lcol[k];
+  float X[9], Xi[9];
+  mgn_lvv(C + 6ll * n, d + 6ll * n, X);
+  ltri_inv(X, Xi);
+  const float* B = fp + (int64_t)kPairIn * k;
+  // O = B Xi^T: O[l][c] = sum_m B[l][m] Xi[c][m]
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) O[9 * k + 3 * l + c] = B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
+  (void)pv;
+}
+
+// the tied blocks' share of the gradient of Lhat_nn, gathered over the pairs in which n is the lower
+// vertex (its upper list): with O = B X^{-T}, dL/dX = -X^{-T} G^T O for G = dL/dO; added (through
+// X = C T(d)) to gd_n.  gO is cleared afterwards: the pair decoder does not act in this form.
+__global__ void __launch_bounds__(256)
+    k_mgn_tied_bwd(int nV, const int32_t* __restrict__ uptr, const int32_t* __restrict__ ulink,
+                   const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ O,
+                   const float* __restrict__ gO, float* __restrict__ gd) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nV) return;
+  float X[9], Xi[9];
+  mgn_lvv(C + 6ll * n, d + 6ll * n, X);
+  ltri_inv(X, Xi);
+  float A[9];  // sum over pairs of G^T O
+#pragma unroll
+  for (int e = 0; e < 9; ++e) A[e] = 0.f;
+  for (int t = uptr[n]; t < uptr[n + 1]; ++t) {
+    const int64_t k = ulink[t];
+    const float* G = gO + 9 * k;
+    const float* Ok = O + 9 * k;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) A[3 * i + j] += G[i] * Ok[j] + G[3 + i] * Ok[3 + j] + G[6 + i] * Ok[6 + j];
+  }
+  // GX = -Xi^T A (lower part)
+  float GX[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      GX[3 * i + j] = (j <= i) ? -(Xi[i] * A[j] + Xi[3 + i] * A[3 + j] + Xi[6 + i] * A[6 + j]) : 0.f;
+  const float* Cv = C + 6ll * n;
+  const float Cm[9] = {Cv[0], 0.f, 0.f, Cv[1], Cv[2], 0.f, Cv[3], Cv[4], Cv[5]};
+  float gT[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * GX[j] + Cm[3 + i] * GX[3 + j] + Cm[6 + i] * GX[6 + j];
+  const float* dv = d + 6ll * n;
+  float* g = gd + 6ll * n;
+  g[0] += gT[0] * dtexp(dv[0]);
+  g[1] += gT[3];
+  g[2] += gT[4] * dtexp(dv[2]);
+  g[3] += gT[6];
+  g[4] += gT[7];
+  g[5] += gT[8] * dtexp(dv[5]);
+}
+
+__global__ void __launch_bounds__(256) k_zero_f(int64_t n, float* __restrict__ a) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.f;
+}
+
 // Probe systems for the Frobenius-type loss: in the normalised space a probe w gives xh = w and
 // bh = Hn w, i.e. x = S^-1 w and b = H x.  w is the normalised right-hand side S^-1 rhs (probe 0
 // when a right-hand side is given) or a standard normal vector from a counter-based hash
@@ -674,11 +763,15 @@ __host__ __device__ __forceinline__ uint64_t mgn_mix(uint64_t z) {
 }
 __global__ void __launch_bounds__(256)
     k_mgn_probe_x(int nV, const float* __restrict__ s, const uint8_t* __restrict__ fixed,
-                  const double* __restrict__ rhs, uint64_t seed, double* __restrict__ x) {
+                  const double* __restrict__ rhs, int is_solution, uint64_t seed, double* __restrict__ x) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= 3ll * nV) return;
   const int v = (int)(i / 3);
   double w;
+  if (rhs && is_solution) {  // a solution of the system itself: the smooth modes CG finds hardest
+    x[i] = (fixed && fixed[i]) ? 0.0 : rhs[i];
+    return;
+  }
   if (rhs) {
     w = rhs[i] / (double)s[v];
   } else {
@@ -833,6 +926,8 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
   const int nV = P->mesh->nV;
   const int64_t nP = P->nlower;
   if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
+  if (M->form == 1 && nP)
+    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O);
   const MgnLayout L = mgn_layout();
   const int nb = (int)((nV + 255) / 256);
   double* part = reinterpret_cast<double*>(M->part);
@@ -852,6 +947,11 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     }
   }
   BH_CHECK(hipGetLastError());
+  if (backward && M->form == 1) {
+    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.O, w.gO, w.gd);
+    if (nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
+    BH_CHECK(hipGetLastError());
+  }
   if (backward) {
     float* G = M->g;
     float* wp = M->part;  // weight-gradient partials (after the loss partials were consumed)
@@ -953,7 +1053,7 @@ int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t*
     double* b = x + 3ll * nV;
     const bool useRhs = rhs && j == 0;
     hipLaunchKernelGGL(k_mgn_probe_x, g256(3ll * nV), dim3(256), 0, s, nV, w.s, fixed, useRhs ? rhs : nullptr,
-                       mgn_mix(seed + 0x1000ull * (uint64_t)j), x);
+                       M->probeSolution, mgn_mix(seed + 0x1000ull * (uint64_t)j), x);
     hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(spmv_blocks(nV)), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals,
                        (const double*)nullptr, fixed,
                        x, b, nullptr, nullptr);
@@ -961,7 +1061,14 @@ int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t*
     xs[j] = x;
   }
   BH_CHECK(hipGetLastError());
-  if ((st = mgn_loss<V>(M, P, vals, fixed, nprobe, bs, xs, 1, loss, s))) return st;
+  double lv = 0.0;
+  if ((st = mgn_loss<V>(M, P, vals, fixed, nprobe, bs, xs, 1, &lv, s))) return st;
+  if (loss) *loss = lv;
+  if (!std::isfinite(lv)) {  // a diverged step: drop its gradient, keep the parameters
+    hipLaunchKernelGGL(k_zero_f, g256(M->nparam), dim3(256), 0, s, M->nparam, M->g);
+    BH_CHECK(hipGetLastError());
+    return BH_NONFINITE;
+  }
   M->step += 1;
   const double b1 = 0.9, b2 = 0.999;
   const double c1 = 1.0 - std::pow(b1, (double)M->step), c2 = 1.0 - std::pow(b2, (double)M->step);
@@ -1095,6 +1202,13 @@ int bh_mgn_destroy(bh_mgn* M) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete M;
+  return BH_OK;
+}
+
+int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution) {
+  if (!M || form < 0 || form > 1) return BH_BAD_ARG;
+  M->form = form;
+  M->probeSolution = probe_is_solution ? 1 : 0;
   return BH_OK;
 }
 

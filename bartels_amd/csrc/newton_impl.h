@@ -228,6 +228,15 @@ __global__ void __launch_bounds__(kRedBlock)
   if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
 }
 
+// Eisenstat-form restart: p~ = s~ = 0
+__global__ void __launch_bounds__(kRedBlock) k_eis_restart(int64_t n, double* __restrict__ pt,
+                                                          double* __restrict__ st) {
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    pt[i] = 0.0;
+    st[i] = 0.0;
+  }
+}
+
 // Eisenstat-form start: x~ = 0, p~ = s~ = 0
 __global__ void __launch_bounds__(kRedBlock) k_eis_init(int64_t n, double* __restrict__ xt, double* __restrict__ pt,
                                                        double* __restrict__ st) {
@@ -558,7 +567,10 @@ int pcg_solve_split_jacobi(bh_mesh* m, const bh_newton_opts* o, const V* vals, c
 // itself (x = P^{-T} x~) is computed at the end of a batch and decides convergence.  The first check
 // comes once the transformed residual has fallen kEisCheck^2 below its start; every check then measures
 // rho = (|r|^2 / |b|^2) / (gamma / gamma0), and the next one waits until rho * gamma / gamma0 predicts
-// the tolerance (within a factor 4).
+// the tolerance (within a factor 4).  When the true residual stagnates between two checks (it fell by
+// less than 4x), the recursive residual has drifted from it (most at tight tolerances): it is
+// replaced by the one just computed (r~ = P^{-1} (b - A x)) and the recurrences restart from the
+// current x~.  (Restarting at every failed check instead cost ~50 % more iterations at 1e-6.)
 constexpr double kEisCheck = 100.0;
 
 template <typename V>
@@ -604,11 +616,13 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
   PcgState hs{};
   bool converged = bb == 0.0, fresh = false;
   double rho = 1.0 / (kEisCheck * kEisCheck);  // predicted (|r|^2/|b|^2) per unit of gamma/gamma0
+  int itBase = 0;                              // iterations before the last restart
+  double prevRr = -1.0;                        // true residual at the previous check
   while (!converged) {
     for (int k = 0; k < kPcgBatch; ++k) {
       eis_bwd<V, double>(P, 2, v, nullptr, v.t, pru, w.st, s);
       eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, w.st, s);
-      cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 0, 0.0, o->pcg_max_iterations, s);
+      cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 0, 0.0, o->pcg_max_iterations - itBase, s);
     }
     if ((st = cg_wait(w, &hs, s))) return st;
     fresh = false;
@@ -624,15 +638,33 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
       double rr = 0.0;
       if ((st = device_dot(r, r, n, w.part, w.scal + 11, s)) || (st = read_scalar(w.scal + 11, &rr, s))) return st;
       fresh = true;
-      if (rr <= tol * tol * bb) converged = true;
-      else if (g > 0.0) rho = (rr / bb) / g;
+      const bool stagnant = prevRr > 0.0 && rr > 0.25 * prevRr;
+      prevRr = rr;
+      if (rr <= tol * tol * bb) {
+        converged = true;
+      } else if (!stagnant) {
+        if (g > 0.0) rho = (rr / bb) / g;
+      } else if (!hs.done && itBase + hs.iters < o->pcg_max_iterations) {
+        // residual replacement and restart from x~ (r~ = P^{-1} r, p~ = s~ = 0)
+        itBase += hs.iters;
+        eis_fwd<V, double>(P, 0, v, r, v.rt, nullptr, nullptr, s);
+        hipLaunchKernelGGL(k_eis_restart, dim3(gi), dim3(kRedBlock), 0, s, n, v.pt, v.st);
+        eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s);
+        eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s);
+        cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 1, 0.0, o->pcg_max_iterations - itBase, s);
+        BH_CHECK(hipGetLastError());
+        rho = rr / bb;  // at the restart gamma / gamma0 = 1
+        prevRr = -1.0;
+        fresh = false;
+        continue;
+      }
     }
     if (hs.done) break;
   }
   if (!fresh) eis_bwd<V, double>(P, 0, v, v.xt, xs, nullptr, nullptr, s);
   hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)xs, w.x);
   BH_CHECK(hipGetLastError());
-  *iters = hs.iters;
+  *iters = itBase + hs.iters;
   *done = converged ? 1 : (hs.done ? hs.done : 2);
   return BH_OK;
 }

@@ -5,10 +5,15 @@ Training problems: jittered beams of 24x24x96 ... 40x40x160 vertices, per-tet Yo
 log-uniform in [1e5, 1e7] Pa, Poisson ratio uniform in [0.30, 0.48], 70 / 20 / 10 % Neo-Hookean /
 StVK / corotational tets, x = 0 clamped, gravity plus a random traction of magnitude up to 5e4 Pa in
 a random direction on the free end.  Each problem is solved by the precision-form Newton with
-block-Jacobi PCG (the plain solver) to 1e-6 relative residual; every Hessian it assembles is handed
+block symmetric Gauss-Seidel PCG to 1e-6 relative residual; every Hessian it assembles is handed
 (on the device, through bh_newton_opts.on_system) to a few Adam steps of the network on probe
-systems: the normalised right-hand side of that Newton system and standard normal vectors (a
-Hutchinson estimate of the relative Frobenius error of L L^T).
+systems: the Newton direction the solver found (a smooth vector: the modes CG converges on last)
+and standard normal vectors (a Hutchinson estimate of the relative Frobenius error of M).
+
+Two forms (bh_mgn_set_form): "sgs" (default), the Eisenstat form M = (Dt + L) Dt^-1 (Dt + L)^T with
+the network writing the block diagonal Dt (Dt = D, block symmetric Gauss-Seidel, at zero decoders),
+solved with BH_PRECOND_LEARNED_SGS; and "factor", a general block lower-triangular factor
+(BH_PRECOND_LEARNED).
 
     python -m bartels_amd.learned train --solves 120 --out weights.f32
     python -m bartels_amd.learned eval --weights weights.f32 --problems 10
@@ -129,11 +134,18 @@ class Solver:
             self.P = ctypes.c_void_p()
 
 
+FORMS = {"factor": 0, "sgs": 1}
+PRECOND = {"factor": 4, "sgs": 6}  # BH_PRECOND_LEARNED, BH_PRECOND_LEARNED_SGS
+
+
 class Network:
-    def __init__(self, seed=1, weights=None):
+    def __init__(self, seed=1, weights=None, form="sgs", probe_solution=True):
         self.lib = _lib.load()
         self.h = ctypes.c_void_p()
         _lib.check(self.lib.bh_mgn_create(seed, ctypes.byref(self.h)), "bh_mgn_create")
+        self.form = form
+        self.precond = PRECOND[form]
+        _lib.check(self.lib.bh_mgn_set_form(self.h, FORMS[form], 1 if probe_solution else 0), "bh_mgn_set_form")
         n = ctypes.c_int64()
         self.lib.bh_mgn_num_params(self.h, ctypes.byref(n))
         self.n = n.value
@@ -160,8 +172,9 @@ class Network:
             self.h = ctypes.c_void_p()
 
 
-def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, log=print):
-    net = Network(seed=seed + 12345)
+def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, form="sgs", probe_solution=True,
+          log=print):
+    net = Network(seed=seed + 12345, form=form, probe_solution=probe_solution)
     rng = np.random.default_rng(seed)
     history = []
     t0 = time.perf_counter()
@@ -175,39 +188,45 @@ def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, log=pri
         def on_system(user, it, v32, v64, b, x):
             for k in range(steps):
                 L = ctypes.c_double()
-                _lib.check(net.lib.bh_mgn_train_step_device(net.h, S.P, ctypes.c_void_p(v32), S.dfixed.data_ptr(),
-                                                            ctypes.c_void_p(b), probes,
-                                                            (seed * 1000003 + 7919 * i + 31 * it + k) & (2**64 - 1),
-                                                            lr, ctypes.byref(L), None), "mgn_train_step")
+                st = net.lib.bh_mgn_train_step_device(net.h, S.P, ctypes.c_void_p(v32), S.dfixed.data_ptr(),
+                                                      ctypes.c_void_p(x if probe_solution else b), probes,
+                                                      (seed * 1000003 + 7919 * i + 31 * it + k) & (2**64 - 1),
+                                                      lr, ctypes.byref(L), None)
+                if st == _lib.BH_NONFINITE:  # the step was dropped (parameters unchanged)
+                    skipped.append(1)
+                    continue
+                _lib.check(st, "mgn_train_step")
                 losses.append(L.value)
 
-        _, rep, tr, wall = S.newton(0, precision=1, max_it=newton_its, on_system=on_system)
+        skipped = []
+        _, rep, tr, wall = S.newton(5, precision=1, max_it=newton_its, on_system=on_system)
         S.close()
         history.append(dict(solve=i, grid=grid, newton=rep.iterations, pcg=rep.pcg_iterations_total,
                             loss_first=losses[0] if losses else None, loss_last=losses[-1] if losses else None,
                             wall=wall))
         log(f"solve {i} grid {grid}: newton {rep.iterations}, pcg {rep.pcg_iterations_total}, "
             f"loss {losses[0] if losses else float('nan'):.4f} -> {losses[-1] if losses else float('nan'):.4f}, "
-            f"{wall:.1f} s (total {time.perf_counter() - t0:.0f} s)")
+            f"dropped {len(skipped)}, {wall:.1f} s (total {time.perf_counter() - t0:.0f} s)")
     net.save(out)
     net.close()
     return history
 
 
-def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, log=print):
+def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="sgs", log=print):
     """held-out 32x32x128 problems: PCG iterations and time to 1e-6 over the first Newton steps,
-    block-Jacobi PCG versus learned PCG versus block IC(0) (all in the precision form)"""
-    net = Network(weights=weights)
+    block-Jacobi PCG, block symmetric Gauss-Seidel and the learned preconditioner (all in the precision
+    form)"""
+    net = Network(weights=weights, form=form)
     rows = []
     for i in range(problems):
         p = random_problem((128, 32, 32), seed=90000 + seed + 13 * i)
         S = Solver(p)
         _, rj, _, wj = S.newton(0, precision=1, max_it=newton_its, pcg_max=pcg_max)
-        _, rl, _, wl = S.newton(4, mgn=net.h, precision=1, max_it=newton_its, pcg_max=pcg_max)
-        _, rb, _, wb = S.newton(2, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        _, rl, _, wl = S.newton(net.precond, mgn=net.h, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        _, rb, _, wb = S.newton(5, precision=1, max_it=newton_its, pcg_max=pcg_max)
         S.close()
         rows.append(dict(problem=i, jacobi_pcg=rj.pcg_iterations_total, learned_pcg=rl.pcg_iterations_total,
-                         ic0_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, ic0_s=wb,
+                         sgs_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, sgs_s=wb,
                          fallbacks=rl.pcg_fallbacks))
         log(json.dumps(rows[-1]))
     net.close()
@@ -223,21 +242,26 @@ def main():
     a.add_argument("--lr", type=float, default=1e-3)
     a.add_argument("--probes", type=int, default=4)
     a.add_argument("--seed", type=int, default=0)
+    a.add_argument("--form", choices=sorted(FORMS), default="sgs")
+    a.add_argument("--probe", choices=["solution", "rhs"], default="solution",
+                   help="probe 0: the Newton direction (solution) or the scaled right-hand side")
     a.add_argument("--out", default=DEFAULT_WEIGHTS)
     b = sub.add_parser("eval")
     b.add_argument("--weights", default=DEFAULT_WEIGHTS)
     b.add_argument("--problems", type=int, default=10)
+    b.add_argument("--form", choices=sorted(FORMS), default="sgs")
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)  # noqa: E731
     if args.cmd == "train":
-        train(args.solves, args.out, seed=args.seed, steps=args.steps, lr=args.lr, probes=args.probes, log=log)
+        train(args.solves, args.out, seed=args.seed, steps=args.steps, lr=args.lr, probes=args.probes,
+              form=args.form, probe_solution=args.probe == "solution", log=log)
     else:
-        rows = evaluate(args.weights, args.problems, log=log)
+        rows = evaluate(args.weights, args.problems, form=args.form, log=log)
         j = np.array([r["jacobi_pcg"] for r in rows], float)
         l = np.array([r["learned_pcg"] for r in rows], float)
-        b = np.array([r["ic0_pcg"] for r in rows], float)
-        log(json.dumps(dict(mean_jacobi_pcg=j.mean(), mean_learned_pcg=l.mean(), mean_ic0_pcg=b.mean(),
-                            jacobi_over_learned=(j / l).mean(), jacobi_over_ic0=(j / b).mean())))
+        b = np.array([r["sgs_pcg"] for r in rows], float)
+        log(json.dumps(dict(mean_jacobi_pcg=j.mean(), mean_learned_pcg=l.mean(), mean_sgs_pcg=b.mean(),
+                            sgs_over_learned=(b / l).mean(), jacobi_over_learned=(j / l).mean())))
 
 
 if __name__ == "__main__":

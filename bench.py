@@ -70,7 +70,8 @@ def cpu_baseline(p, dX, vol, budget_s):
 def newton_steps(p, mesh, steps, pcg_max):
     """Newton-step wall time on the same beam (bent state, gravity, x = 0 clamped, PSD 1e-3, PCG to a
     1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
-    precision form with the learned preconditioner (block IC(0) is reported beside them)."""
+    precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and
+    the learned Eisenstat diagonal (when trained weights are present)."""
     import torch
 
     from bartels_amd import _lib, learned
@@ -90,9 +91,9 @@ def newton_steps(p, mesh, steps, pcg_max):
     P = ctypes.c_void_p()
     _lib.check(lib.bh_precond_create(mesh.handle, ctypes.byref(P)), "bh_precond_create")
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
-    variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2)]
+    variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5)]
     if net is not None:
-        variants.append(("fp32_learned", 1, 4))
+        variants.append(("fp32_learned_sgs", 1, net.precond))
     res, qs = {}, {}
     for name, prec, precond in variants:
         o = _lib.BhNewtonOpts()

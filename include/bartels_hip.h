@@ -222,6 +222,13 @@ int bh_precond_apply_device(bh_precond* precond, const double* d_r, double* d_z,
 typedef struct bh_mgn bh_mgn;
 int bh_mgn_create(uint64_t seed, bh_mgn** mgn);
 int bh_mgn_destroy(bh_mgn* mgn);
+/* The preconditioner form the loss and training steps use: 0 the general block lower-triangular factor
+ * (BH_PRECOND_LEARNED), 1 the Eisenstat form M = (Dt + L) Dt^-1 (Dt + L)^T whose off-diagonal blocks
+ * are tied to H (BH_PRECOND_LEARNED_SGS; only the node decoder acts).  probe_is_solution = 1: the vector
+ * bh_mgn_train_step_device receives is a solution x of the Newton system (probe 0 is x itself) instead of
+ * its right-hand side.  No counterpart in the reference (the learned preconditioner is this build's
+ * addition).  A training step whose loss is not finite returns BH_NONFINITE and leaves the parameters. */
+int bh_mgn_set_form(bh_mgn* mgn, int form, int probe_is_solution);
 int bh_mgn_num_params(const bh_mgn* mgn, int64_t* nparams);
 /* host copies of the parameters and of the accumulated gradient (either may be NULL) */
 int bh_mgn_get_params(const bh_mgn* mgn, float* params, float* grad);

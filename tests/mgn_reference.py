@@ -116,8 +116,10 @@ def forward(P, g, xn, fp):
     return d, O
 
 
-def factor(C, d, O):
-    """normalised factor blocks: Lvv (nV x 3 x 3) = C T(d), Lk (nP x 3 x 3) = O"""
+def factor(C, d, O, g=None, fp=None):
+    """normalised factor blocks: Lvv (nV x 3 x 3) = C T(d), Lk (nP x 3 x 3) = O; or, in the Eisenstat
+    (symmetric Gauss-Seidel) form (g and the pair features fp given), Lk = Hn_vn Lvv_n^{-T} with Hn_vn
+    the first 9 pair features, so that L L^T = (Dt + L_H) Dt^{-1} (Dt + L_H)^T, Dt = Lvv Lvv^T"""
     nV = d.shape[0]
     T = torch.zeros(nV, 3, 3, dtype=d.dtype)
     T[:, 0, 0] = torch.exp(d[:, 0])
@@ -126,7 +128,13 @@ def factor(C, d, O):
     T[:, 2, 0] = d[:, 3]
     T[:, 2, 1] = d[:, 4]
     T[:, 2, 2] = torch.exp(d[:, 5])
-    return torch.bmm(C, T), O.reshape(-1, 3, 3)
+    Lvv = torch.bmm(C, T)
+    if fp is None:
+        return Lvv, O.reshape(-1, 3, 3)
+    pn = torch.as_tensor(g.lcol.astype(np.int64))
+    Hn = fp[:, :9].reshape(-1, 3, 3)
+    Xinv_t = torch.linalg.inv(Lvv[pn]).transpose(1, 2).contiguous()
+    return Lvv, torch.bmm(Hn, Xinv_t)
 
 
 def apply_LLt(g, Lvv, Lk, u):
@@ -140,11 +148,13 @@ def apply_LLt(g, Lvv, Lk, u):
     return z
 
 
-def loss(P, g, feats, x, b):
-    """|Lhat Lhat^T xh - bh|^2 / |bh|^2 with xh = s x, bh = b / s"""
+def loss(P, g, feats, x, b, tied=False):
+    """|Lhat Lhat^T xh - bh|^2 / |bh|^2 with xh = s x, bh = b / s (tied: the Eisenstat form)"""
     s, C, xn, fp = (torch.as_tensor(a) for a in feats)
     d, O = forward(P, g, xn, fp)
-    Lvv, Lk = factor(C, d, O)
+    Lvv, Lk = factor(C, d, O, g, fp) if tied else factor(C, d, O)
+    if tied:
+        O = Lk.reshape(-1, 9)
     xh = torch.as_tensor(x.reshape(-1, 3)) * s[:, None]
     bh = torch.as_tensor(b.reshape(-1, 3)) / s[:, None]
     z = apply_LLt(g, Lvv, Lk, xh)

@@ -91,11 +91,11 @@ def make_mgn(lib, seed=7):
     return M, n.value
 
 
-def test_mgn_forward_loss_and_gradient_match_reference(system):
-    S = system
+def _gradient_check(S, tied):
     lib = S["lib"]
     M, n = make_mgn(lib)
     try:
+        assert lib.bh_mgn_set_form(M, 1 if tied else 0, 0) == 0
         lay, ntot = R.layout()
         assert n == ntot
         rng = np.random.default_rng(1)
@@ -127,11 +127,12 @@ def test_mgn_forward_loss_and_gradient_match_reference(system):
         O_gpu = d2h(bufs[3].value, 9 * g.nP, np.float32).reshape(-1, 9)
         # reference
         Pt = {k: v.clone().requires_grad_(True) for k, v in R.unflatten(torch.tensor(w, dtype=torch.float64)).items()}
-        L, d_ref, O_ref = R.loss(Pt, g, S["feats"], S["x"], S["b"])
+        L, d_ref, O_ref = R.loss(Pt, g, S["feats"], S["x"], S["b"], tied=tied)
         L.backward()
         g_ref = np.zeros(n)
         for k, (o, sh) in lay.items():
-            g_ref[o:o + int(np.prod(sh))] = Pt[k].grad.numpy().ravel()
+            if Pt[k].grad is not None:  # (the pair decoder is unused in the tied form)
+                g_ref[o:o + int(np.prod(sh))] = Pt[k].grad.numpy().ravel()
         rs = np.abs(s_gpu - S["feats"][0]).max() / np.abs(S["feats"][0]).max()
         rd = np.abs(d_gpu - d_ref.detach().numpy()).max() / max(np.abs(d_ref.detach().numpy()).max(), 1e-30)
         rO = np.abs(O_gpu - O_ref.detach().numpy()).max() / max(np.abs(O_ref.detach().numpy()).max(), 1e-30)
@@ -141,12 +142,26 @@ def test_mgn_forward_loss_and_gradient_match_reference(system):
         worst = 0.0
         for k, (o, sh) in lay.items():
             a, r = grad[o:o + int(np.prod(sh))], g_ref[o:o + int(np.prod(sh))]
+            if tied and k in ("Wdo", "bdo"):  # the pair decoder does not act in the tied form
+                assert not np.any(a) and not np.any(r)
+                continue
             e = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
             worst = max(worst, e)
             assert e < 2e-3, (k, e, np.linalg.norm(r))
         print(f"worst per-tensor gradient error {worst:.1e} over {len(lay)} tensors")
     finally:
         lib.bh_mgn_destroy(M)
+
+
+def test_mgn_forward_loss_and_gradient_match_reference(system):
+    """general factor: features, decoder outputs, loss and every parameter gradient against FP64 torch"""
+    _gradient_check(system, tied=False)
+
+
+def test_mgn_tied_form_gradient_matches_reference(system):
+    """Eisenstat (symmetric Gauss-Seidel) form: off-diagonal blocks Hn_vn Lhat_nn^{-T}, the gradient through
+    them into the node decoder, against FP64 torch"""
+    _gradient_check(system, tied=True)
 
 
 def test_mgn_factor_inference_matches_reference(system):
