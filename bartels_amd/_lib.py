@@ -136,7 +136,7 @@ SIGNATURES = {
     "bh_precond_apply_device": (c_int, [_VP, _VP, _VP, _VP]),
     "bh_mgn_create": (c_int, [ctypes.c_uint64, P(_VP)]),
     "bh_mgn_destroy": (c_int, [_VP]),
-    "bh_mgn_set_form": (c_int, [_VP, c_int, c_int]),
+    "bh_mgn_set_form": (c_int, [_VP, c_int, c_int, c_int]),
     "bh_mgn_num_params": (c_int, [_VP, P(c_int64)]),
     "bh_mgn_get_params": (c_int, [_VP, _VP, _VP]),
     "bh_mgn_set_params": (c_int, [_VP, _VP]),

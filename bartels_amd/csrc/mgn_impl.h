@@ -43,6 +43,7 @@ struct bh_mgn {
   size_t probeBytes = 0;
   int form = 0;             // 0: general factor; 1: Eisenstat (symmetric Gauss-Seidel) form, O tied to H
   int probeSolution = 0;    // 1: the vector handed to a training step is a solution x of H x = b (probe 0)
+  int energyLoss = 0;       // 1 (Eisenstat form): the energy-ratio loss instead of the residual loss
 };
 
 namespace {
@@ -746,6 +747,106 @@ __global__ void __launch_bounds__(256)
   g[5] += gT[8] * dtexp(dv[5]);
 }
 
+// Energy-ratio loss of the Eisenstat form, per probe x (normalised: xh = s x):
+//   q = x^T M x = sum_v [xh_v^T Dh_v xh_v + 2 xh_v^T z_v + z_v^T Dh_v^{-1} z_v],  z_v = sum_{n upper of v} Hn_vn xh_n,
+//   a = x^T H x,   loss = mean over probes of (log q - log a)^2,
+// with Dh_v = Lhat_vv Lhat_vv^T.  It compares the energies of M and H on the probes (both ends of the
+// spectrum of M^{-1} H when the probes are smooth and rough vectors) and is local per vertex, unlike
+// the residual loss, whose minimiser on the tied form made CG slower than symmetric Gauss-Seidel.
+__global__ void __launch_bounds__(256)
+    k_mgn_energy_fwd(int nV, const int32_t* __restrict__ uptr, const int32_t* __restrict__ ucol,
+                     const int32_t* __restrict__ ulink, const float* __restrict__ fp, const float* __restrict__ C,
+                     const float* __restrict__ d, const float* __restrict__ s, const double* __restrict__ x,
+                     const double* __restrict__ b, float* __restrict__ z, double* __restrict__ part) {
+  __shared__ double sh[8];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double q = 0.0, a = 0.0;
+  if (v < nV) {
+    const double sv = s[v];
+    const double xh[3] = {sv * x[3ll * v], sv * x[3ll * v + 1], sv * x[3ll * v + 2]};
+    double zz[3] = {0.0, 0.0, 0.0};
+    for (int t = uptr[v]; t < uptr[v + 1]; ++t) {
+      const float* B = fp + (int64_t)kPairIn * ulink[t];  // Hn_nv, n the higher vertex
+      const int n = ucol[t];
+      const double sn = s[n];
+      const double x0 = sn * x[3ll * n], x1 = sn * x[3ll * n + 1], x2 = sn * x[3ll * n + 2];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) zz[l] += (double)B[l] * x0 + (double)B[3 + l] * x1 + (double)B[6 + l] * x2;
+    }
+    float L[9], Li[9];
+    mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+    ltri_inv(L, Li);
+    double ltx[3], liz[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      ltx[l] = (double)L[l] * xh[0] + (double)L[3 + l] * xh[1] + (double)L[6 + l] * xh[2];   // L^T xh
+      liz[l] = (double)Li[3 * l] * zz[0] + (double)Li[3 * l + 1] * zz[1] + (double)Li[3 * l + 2] * zz[2];  // L^-1 z
+    }
+    q = ltx[0] * ltx[0] + ltx[1] * ltx[1] + ltx[2] * ltx[2] + 2.0 * (xh[0] * zz[0] + xh[1] * zz[1] + xh[2] * zz[2]) +
+        liz[0] * liz[0] + liz[1] * liz[1] + liz[2] * liz[2];
+    a = x[3ll * v] * b[3ll * v] + x[3ll * v + 1] * b[3ll * v + 1] + x[3ll * v + 2] * b[3ll * v + 2];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) z[3ll * v + l] = (float)zz[l];
+  }
+  q = block_reduce_sum(q, sh);
+  __syncthreads();
+  a = block_reduce_sum(a, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = q;
+    part[2 * blockIdx.x + 1] = a;
+  }
+}
+
+// its gradient: dloss/dDh_v = c (xh xh^T - Dh^-1 z z^T Dh^-1), c = 2 (log q - log a) / (q ns); through
+// Dh = Lhat Lhat^T (dloss/dLhat = 2 G Lhat, lower part) and Lhat = C T(d) into gd
+__global__ void __launch_bounds__(256)
+    k_mgn_energy_bwd(int nV, const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ s,
+                     const double* __restrict__ x, const float* __restrict__ z, const double* __restrict__ qa, int ns,
+                     int accumulate, float* __restrict__ gd) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const double q = qa[0], a = qa[1];
+  const double c = (q > 0.0 && a > 0.0) ? 2.0 * (log(q) - log(a)) / (q * ns) : 0.0;
+  const double sv = s[v];
+  const double xh[3] = {sv * x[3ll * v], sv * x[3ll * v + 1], sv * x[3ll * v + 2]};
+  float L[9], Li[9];
+  mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+  ltri_inv(L, Li);
+  const double zz[3] = {z[3ll * v], z[3ll * v + 1], z[3ll * v + 2]};
+  double liz[3], w[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) liz[l] = (double)Li[3 * l] * zz[0] + (double)Li[3 * l + 1] * zz[1] + (double)Li[3 * l + 2] * zz[2];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) w[l] = (double)Li[l] * liz[0] + (double)Li[3 + l] * liz[1] + (double)Li[6 + l] * liz[2];  // Dh^-1 z
+  double G[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[3 * i + j] = c * (xh[i] * xh[j] - w[i] * w[j]);
+  float GL[9];  // lower part of 2 G Lhat
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      GL[3 * i + j] = (j <= i) ? (float)(2.0 * (G[3 * i] * L[j] + G[3 * i + 1] * L[3 + j] + G[3 * i + 2] * L[6 + j])) : 0.f;
+  const float* Cv = C + 6ll * v;
+  const float Cm[9] = {Cv[0], 0.f, 0.f, Cv[1], Cv[2], 0.f, Cv[3], Cv[4], Cv[5]};
+  float gT[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * GL[j] + Cm[3 + i] * GL[3 + j] + Cm[6 + i] * GL[6 + j];
+  const float* dv = d + 6ll * v;
+  float* g = gd + 6ll * v;
+  const float a0 = gT[0] * dtexp(dv[0]), a1 = gT[3], a2 = gT[4] * dtexp(dv[2]), a3 = gT[6], a4 = gT[7],
+              a5 = gT[8] * dtexp(dv[5]);
+  if (accumulate) {
+    g[0] += a0; g[1] += a1; g[2] += a2; g[3] += a3; g[4] += a4; g[5] += a5;
+  } else {
+    g[0] = a0; g[1] = a1; g[2] = a2; g[3] = a3; g[4] = a4; g[5] = a5;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_zero_f(int64_t n, float* __restrict__ a) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.f;
@@ -931,7 +1032,18 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
   const MgnLayout L = mgn_layout();
   const int nb = (int)((nV + 255) / 256);
   double* part = reinterpret_cast<double*>(M->part);
-  for (int j = 0; j < ns; ++j) {
+  if (M->energyLoss) {
+    for (int j = 0; j < ns; ++j) {
+      hipLaunchKernelGGL(k_mgn_energy_fwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.fp, w.C, w.d,
+                         w.s, xs[j], bs[j], w.y, part);
+      hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
+      if (backward)
+        hipLaunchKernelGGL(k_mgn_energy_bwd, g256(nV), dim3(256), 0, s, nV, w.C, w.d, w.s, xs[j], w.y,
+                           M->dscal + 2 * j, ns, j > 0 ? 1 : 0, w.gd);
+    }
+    if (backward && nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
+  }
+  for (int j = 0; j < ns && !M->energyLoss; ++j) {
     // y = Lhat^T xh ; r = Lhat y - bh
     hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, xs[j],
                        1, (const float*)nullptr, w.y);
@@ -947,7 +1059,7 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     }
   }
   BH_CHECK(hipGetLastError());
-  if (backward && M->form == 1) {
+  if (backward && M->form == 1 && !M->energyLoss) {
     hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.O, w.gO, w.gd);
     if (nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
     BH_CHECK(hipGetLastError());
@@ -1017,7 +1129,14 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     BH_CHECK(hipMemcpyAsync(h, M->dscal, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, s));
     BH_CHECK(hipStreamSynchronize(s));
     double a = 0.0;
-    for (int j = 0; j < ns; ++j) a += h[2 * j + 1] > 0 ? h[2 * j] / h[2 * j + 1] : 0.0;
+    for (int j = 0; j < ns; ++j) {
+      if (M->energyLoss) {
+        const double e = (h[2 * j] > 0 && h[2 * j + 1] > 0) ? std::log(h[2 * j] / h[2 * j + 1]) : 0.0;
+        a += e * e;
+      } else {
+        a += h[2 * j + 1] > 0 ? h[2 * j] / h[2 * j + 1] : 0.0;
+      }
+    }
     *loss = a / ns;
   }
   return BH_OK;
@@ -1205,10 +1324,11 @@ int bh_mgn_destroy(bh_mgn* M) {
   return BH_OK;
 }
 
-int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution) {
-  if (!M || form < 0 || form > 1) return BH_BAD_ARG;
+int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution, int energy_loss) {
+  if (!M || form < 0 || form > 1 || (energy_loss && form != 1)) return BH_BAD_ARG;
   M->form = form;
   M->probeSolution = probe_is_solution ? 1 : 0;
+  M->energyLoss = energy_loss ? 1 : 0;
   return BH_OK;
 }
 

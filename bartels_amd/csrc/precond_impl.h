@@ -324,6 +324,7 @@ __global__ void __launch_bounds__(256)
               const V* __restrict__ hu, const float* __restrict__ dt, const float* __restrict__ dtinv, int mode,
               EisVecs v, const double* __restrict__ src, TT* __restrict__ tb, double* __restrict__ part,
               const PcgState* __restrict__ st) {
+  static_assert(LANES >= 3, "lanes 0..2 carry the three rows of a position");
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
   const int j = threadIdx.x % LANES;
@@ -403,6 +404,7 @@ __global__ void __launch_bounds__(256)
               const V* __restrict__ hl, const float* __restrict__ dtinv, const double* __restrict__ kk, int mode,
               EisVecs v, const double* __restrict__ src, TT* __restrict__ db, double* __restrict__ part,
               const PcgState* __restrict__ st) {
+  static_assert(LANES >= 3, "lanes 0..2 carry the three rows of a position");
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
   const int j = threadIdx.x % LANES;

@@ -139,13 +139,14 @@ PRECOND = {"factor": 4, "sgs": 6}  # BH_PRECOND_LEARNED, BH_PRECOND_LEARNED_SGS
 
 
 class Network:
-    def __init__(self, seed=1, weights=None, form="sgs", probe_solution=True):
+    def __init__(self, seed=1, weights=None, form="sgs", probe_solution=True, energy_loss=True):
         self.lib = _lib.load()
         self.h = ctypes.c_void_p()
         _lib.check(self.lib.bh_mgn_create(seed, ctypes.byref(self.h)), "bh_mgn_create")
         self.form = form
         self.precond = PRECOND[form]
-        _lib.check(self.lib.bh_mgn_set_form(self.h, FORMS[form], 1 if probe_solution else 0), "bh_mgn_set_form")
+        _lib.check(self.lib.bh_mgn_set_form(self.h, FORMS[form], 1 if probe_solution else 0,
+                                            1 if (energy_loss and form == "sgs") else 0), "bh_mgn_set_form")
         n = ctypes.c_int64()
         self.lib.bh_mgn_num_params(self.h, ctypes.byref(n))
         self.n = n.value

@@ -152,6 +152,8 @@ def main():
                     help="Newton steps per solver variant in the Newton-step comparison (0: skip)")
     ap.add_argument("--pcg-max", type=int, default=20000)
     ap.add_argument("--quick", action="store_true", help="timed steps only (for counter collection)")
+    ap.add_argument("--newton-only", action="store_true",
+                    help="only the Newton-step comparison (for the kernel profile of the solvers)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference point for the HBM byte counters)")
     args = ap.parse_args()
@@ -183,6 +185,9 @@ def main():
     if args.gather_lanes:
         assert lib0.bh_mesh_set_gather_lanes(mesh.handle, args.gather_lanes) == 0
     setup_s = time.perf_counter() - t_setup
+    if args.newton_only:
+        print(json.dumps(newton_steps(p, mesh, max(1, args.newton_steps), args.pcg_max)), flush=True)
+        return
 
     def step():
         mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, callback=psd, stream=stream)

@@ -227,8 +227,10 @@ int bh_mgn_destroy(bh_mgn* mgn);
  * are tied to H (BH_PRECOND_LEARNED_SGS; only the node decoder acts).  probe_is_solution = 1: the vector
  * bh_mgn_train_step_device receives is a solution x of the Newton system (probe 0 is x itself) instead of
  * its right-hand side.  No counterpart in the reference (the learned preconditioner is this build's
- * addition).  A training step whose loss is not finite returns BH_NONFINITE and leaves the parameters. */
-int bh_mgn_set_form(bh_mgn* mgn, int form, int probe_is_solution);
+ * addition).  energy_loss = 1 (form 1 only): the loss is the mean over probes x of
+ * (log(x^T M x) - log(x^T H x))^2 instead of |M x - H x|^2 / |H x|^2.  A training step whose loss is not
+ * finite returns BH_NONFINITE and leaves the parameters. */
+int bh_mgn_set_form(bh_mgn* mgn, int form, int probe_is_solution, int energy_loss);
 int bh_mgn_num_params(const bh_mgn* mgn, int64_t* nparams);
 /* host copies of the parameters and of the accumulated gradient (either may be NULL) */
 int bh_mgn_get_params(const bh_mgn* mgn, float* params, float* grad);

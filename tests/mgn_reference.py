@@ -177,3 +177,26 @@ def dense_factor(g, s, Lvv, Lk, fixed=None):
         L[:, fx] = 0.0
         L[fx, fx] = 1.0
     return L
+
+
+def energy_loss(P, g, feats, xs, bs):
+    """Eisenstat form, energy-ratio loss: mean over probes of (log x^T M x - log x^T H x)^2 with
+    M = (Dt + L) Dt^{-1} (Dt + L)^T in the normalised space, Dt = Lvv Lvv^T, L the lower pairs of Hn.
+    x^T M x = sum_v [xh_v^T Dt_v xh_v + 2 xh_v^T z_v + z_v^T Dt_v^{-1} z_v], z_v = sum_{pairs (n, v)} Hn_nv^T xh_n"""
+    s, C, xn, fp = (torch.as_tensor(a) for a in feats)
+    d, O = forward(P, g, xn, fp)
+    Lvv, _ = factor(C, d, O)
+    D = torch.bmm(Lvv, Lvv.transpose(1, 2).contiguous())
+    Dinv = torch.linalg.inv(D)
+    pv = torch.as_tensor(g.pv)
+    pn = torch.as_tensor(g.lcol.astype(np.int64))
+    Hn = fp[:, :9].reshape(-1, 3, 3)
+    total = 0.0
+    for x, b in zip(xs, bs):
+        xh = torch.as_tensor(x.reshape(-1, 3)) * s[:, None]
+        z = torch.zeros_like(xh).index_add(0, pn, torch.einsum("kml,km->kl", Hn, xh[pv]))
+        q = (torch.einsum("vi,vij,vj->v", xh, D, xh) + 2.0 * (xh * z).sum(1) +
+             torch.einsum("vi,vij,vj->v", z, Dinv, z)).sum()
+        a = float(np.dot(x, b))
+        total = total + (torch.log(q) - np.log(a)) ** 2
+    return total / len(xs), d

@@ -95,7 +95,7 @@ def _gradient_check(S, tied):
     lib = S["lib"]
     M, n = make_mgn(lib)
     try:
-        assert lib.bh_mgn_set_form(M, 1 if tied else 0, 0) == 0
+        assert lib.bh_mgn_set_form(M, 1 if tied else 0, 0, 0) == 0
         lay, ntot = R.layout()
         assert n == ntot
         rng = np.random.default_rng(1)
@@ -162,6 +162,51 @@ def test_mgn_tied_form_gradient_matches_reference(system):
     """Eisenstat (symmetric Gauss-Seidel) form: off-diagonal blocks Hn_vn Lhat_nn^{-T}, the gradient through
     them into the node decoder, against FP64 torch"""
     _gradient_check(system, tied=True)
+
+
+def test_mgn_energy_loss_gradient_matches_reference(system):
+    """Eisenstat form with the energy-ratio loss (log x^T M x - log x^T H x)^2: loss and every parameter
+    gradient against FP64 torch"""
+    S = system
+    lib = S["lib"]
+    M, n = make_mgn(lib, seed=5)
+    try:
+        assert lib.bh_mgn_set_form(M, 1, 0, 1) == 0
+        lay, _ = R.layout()
+        rng = np.random.default_rng(4)
+        w = np.empty(n, np.float32)
+        lib.bh_mgn_get_params(M, w.ctypes.data_as(ctypes.c_void_p), None)
+        for k, (o, sh) in lay.items():
+            if k == "Wdv":
+                w[o:o + int(np.prod(sh))] = 0.05 * rng.standard_normal(int(np.prod(sh)))
+            elif k.endswith("b1") or k.endswith("b2") or k == "bdv":
+                w[o:o + int(np.prod(sh))] = 0.02 * rng.standard_normal(int(np.prod(sh)))
+        assert lib.bh_mgn_set_params(M, w.ctypes.data_as(ctypes.c_void_p)) == 0
+        xs, bs = [S["x"]], [S["b"]]
+        loss = ctypes.c_double()
+        assert lib.bh_mgn_loss_device(M, S["P"], vp(S["vals"]), vp(S["dfixed"]), vp(S["db"]), vp(S["dx"]), 1,
+                                      ctypes.byref(loss), None) == 0
+        grad = np.empty(n, np.float32)
+        lib.bh_mgn_get_params(M, None, grad.ctypes.data_as(ctypes.c_void_p))
+        Pt = {k: v.clone().requires_grad_(True) for k, v in R.unflatten(torch.tensor(w, dtype=torch.float64)).items()}
+        L, _ = R.energy_loss(Pt, S["g"], S["feats"], xs, bs)
+        L.backward()
+        rl = abs(loss.value - L.item()) / max(L.item(), 1e-30)
+        print(f"energy loss {loss.value:.6e} vs {L.item():.6e} ({rl:.1e})")
+        assert rl < 1e-4
+        worst = 0.0
+        for k, (o, sh) in lay.items():
+            a = grad[o:o + int(np.prod(sh))]
+            r = Pt[k].grad.numpy().ravel() if Pt[k].grad is not None else np.zeros_like(a, dtype=np.float64)
+            if k in ("Wdo", "bdo"):
+                assert not np.any(a) and not np.any(r)
+                continue
+            e = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30)
+            worst = max(worst, e)
+            assert e < 2e-3, (k, e, np.linalg.norm(r))
+        print(f"worst per-tensor gradient error {worst:.1e}")
+    finally:
+        lib.bh_mgn_destroy(M)
 
 
 def test_mgn_factor_inference_matches_reference(system):
