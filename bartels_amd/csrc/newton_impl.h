@@ -419,7 +419,7 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
 // objective(q) = Psi(q) - f.q  -> *out (device)
 int objective(bh_mesh* m, const bh_newton_opts* o, const double* params, const double* f, const double* q, double* out,
               NewtonWs& w, hipStream_t s) {
-  int st = (o->element_precision == 1)
+  int st = (o->element_precision == 2)
                ? energy_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, out, s)
                : energy_dispatch(m, o->model, o->d_model_ids, q, params, out, s);
   if (st) return st;
@@ -860,7 +860,11 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
   double g0 = -1.0;
   int it = 0;
   int status = BH_MAXIT;
-  const bool f32 = o->element_precision == 1;
+  // precision form: 1 = FP32 element Hessian and CG operator (energy and gradient in FP64, so the
+  // equilibrium is the FP64 one); 2 = energy, gradient and Hessian all from FP32 element math
+  const bool f32 = o->element_precision == 1 || o->element_precision == 2;
+  const bool f32g = o->element_precision == 2;
+  if (o->element_precision < 0 || o->element_precision > 2) return BH_BAD_ARG;
   if (f32) {
     hipLaunchKernelGGL(k_f64_to_f32, dim3((2ll * m->nT + 255) / 256), dim3(256), 0, s, 2ll * m->nT, params,
                        w.params32);
@@ -868,7 +872,7 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
   }
   do {
     // gradient of the objective, masked
-    st = f32 ? gradient_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, w.g, s)
+    st = f32g ? gradient_dispatch(m, o->model, o->d_model_ids, q, (const float*)w.params32, w.g, s)
              : gradient_dispatch(m, o->model, o->d_model_ids, q, params, w.g, s);
     if (st) return st;
     if (o->d_mass_vals) {  // + inertia M (q - q_pred)

@@ -91,7 +91,9 @@ def newton_steps(p, mesh, steps, pcg_max):
     P = ctypes.c_void_p()
     _lib.check(lib.bh_precond_create(mesh.handle, ctypes.byref(P)), "bh_precond_create")
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
-    variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5)]
+    # precision 1: FP32 Hessian / CG operator, FP64 element gradient; 2: FP32 element gradient too
+    variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5),
+                ("fp32full_sgs", 2, 5)]
     if net is not None:
         variants.append(("fp32_learned_sgs", 1, net.precond))
     res, qs = {}, {}

@@ -274,10 +274,16 @@ typedef struct {
   int precond;             /* bh_precond_kind */
   bh_precond* precond_handle; /* BH_PRECOND_BIC0 / BH_PRECOND_FACTOR */
   double precond_shift;    /* BH_PRECOND_BIC0 diagonal shift */
-  /* 0: FP64 element path.  1: the precision form -- element energy, gradient and Hessian in FP32
-   * (J = det F and log J in FP64), FP32 Hessian values; the energy sum of the line search, the
-   * gradient accumulation, SpMV sums, PCG vectors and dot products and the convergence test stay
-   * FP64.  params stay FP64 at the interface (converted once per solve). */
+  /* 0: FP64 element path.
+   * 1: the precision form for the solve -- FP32 element Hessian and FP32 Hessian blocks for every
+   *    product and sweep (FP64 diagonal blocks rebuilt from exact row sums), element energy and
+   *    gradient in FP64, so the equilibrium Newton converges to is the FP64 one.
+   * 2: the full precision form -- element energy, gradient and Hessian all in FP32 (J = det F and
+   *    log J in FP64; G = F - I from FP64 displacement edges when rest positions are set).  Its
+   *    equilibrium carries the FP32 stress error, largest under big rotations.
+   * In both, the energy sum of the line search, the gradient accumulation, every CG vector and dot
+   * product, the residual check and the convergence test stay FP64; params stay FP64 at the
+   * interface (converted once per solve). */
   int element_precision;
   /* 1: when a factor-preconditioned PCG (BIC0 / FACTOR) hits pcg_max_iterations, solve again with
    * block-Jacobi PCG from zero (the fallback), counted in bh_newton_report.pcg_fallbacks */
