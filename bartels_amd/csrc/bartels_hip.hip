@@ -26,6 +26,7 @@
 #include <cstring>
 #include <mutex>
 #include <queue>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bartels_hip.h"
@@ -70,7 +71,7 @@ struct bh_mesh {
   int32_t* bcol = nullptr;
   int32_t* vdeg = nullptr;  // tets incident to each vertex
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
-  int gatherLanes = 1;                            // lanes per vertex in the assembly gather (1 or 2)
+  int gatherLanes = 4;                            // lanes per vertex in the assembly gather (1, 2 or 4)
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
   int coloring = 1;                               // colour sweeps: 1 DSatur, 2 greedy in vertex order
   int sweepLanes = 4;                             // lanes per vertex in the Gauss-Seidel sweeps (4 or 8)
@@ -88,8 +89,10 @@ struct bh_mesh {
   // PSD
   uint8_t* psdFlag = nullptr;
   int32_t* psdCount = nullptr;
-  double* psdDM = nullptr;  // nT x 45
+  double* psdDM = nullptr;  // nT x 45: the tets' records M' (packed reduced 9x9)
   float* psdDM32 = nullptr;
+  double* tetQ = nullptr;   // nT x 12: Q of Phi = Q R, tet-contiguous (made on first use)
+  float* tetQ32 = nullptr;
   // solver workspace (lazily allocated)
   double* ws = nullptr;
   size_t wsBytes = 0;
@@ -160,14 +163,6 @@ __device__ __forceinline__ IncList inc_list(const MeshView& m, int v) {
   const int g = v >> 6;
   return IncList{m.gBase[g] + (v & 63), m.gLen[g]};
 }
-
-template <typename T>
-struct PsdView {
-  T tolPos;  // max(tol, 0) / 4: clamp of the three translation eigenvalues, per block
-  T tol;
-  const uint8_t* __restrict__ flag;
-  const T* __restrict__ dM;
-};
 
 // ---- tet loads -------------------------------------------------------------------------------
 template <typename T>
@@ -383,47 +378,6 @@ __global__ void __launch_bounds__(256)
 // form is 9 nb doubles (1080 B at valence 14), so LDS caps one-lane workgroups of 64 vertices at
 // two per CU: occupancy and latency, not HBM, bound this kernel.
 
-// blk_j += (a, mlist[j]) block of the stored clamp correction of one tet: for fmt = k <= 3 the k
-// columns u = (Q (x) I3) w of the 12x12 correction U U^T (12 values each, u[3 c + l] for corner c);
-// for fmt 4 the packed reduced 9x9 dM, mapped through Q (x) I3 here
-template <typename T, int NB>
-__device__ __forceinline__ void add_psd_correction(const T phi[4][3], const T* __restrict__ W, int fmt, int a,
-                                                   const int* mlist, T blk[NB][9]) {
-  if (fmt <= 3) {
-#pragma unroll
-    for (int col = 0; col < 3; ++col) {
-      if (col < fmt) {
-        const T* u = W + 12 * col;
-        T ua[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) ua[l] = u[3 * a + l];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-          const T* um = u + 3 * mlist[j];
-          const T u0 = um[0], u1 = um[1], u2 = um[2];
-#pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            blk[j][3 * l] += ua[l] * u0;
-            blk[j][3 * l + 1] += ua[l] * u1;
-            blk[j][3 * l + 2] += ua[l] * u2;
-          }
-        }
-      }
-    }
-    return;
-  }
-  T Q[4][3], R[3][3];
-  phi_qr(phi, Q, R);
-  T Tm[3][9];
-  psd_row_factor(Q, a, W, Tm);
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    T qm[3];
-    select_row(Q, mlist[j], qm);
-    psd_add_block(qm, Tm, blk[j]);
-  }
-}
-
 // Workgroup -> vertex group.  Workgroups are dealt round-robin to the 8 XCDs (each with its own
 // L2); with xcd set, XCD x takes the contiguous run of groups [x per, (x+1) per), so the groups
 // that share tets (neighbouring ids) meet in the same L2.  The grid is then 8 per workgroups and
@@ -501,10 +455,38 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
   }
 }
 
-template <int MODEL, bool PSD, int VPW, typename T>
+// one block of the vertex's block row: corner mm
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_block1(const MeshView& m, const double* __restrict__ q,
+                                           const T* __restrict__ params, int e, int a, int mm, bool stretch_ref,
+                                           T blk[1][9]) {
+  if (MODEL == M_MASS) {
+    const T mass = params[e] * load_vol<T>(m, e);
+    const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) blk[0][i] = (i % 4 == 0) ? c : T(0);
+    return;
+  }
+  T phi[4][3];
+  Pre<T> P;
+  load_phi(m, e, phi);
+  {
+    T Gh[9];
+    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+  }
+  P.vol = load_vol<T>(m, e);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  T pa[3], pm[3];
+  select_row(phi, a, pa);
+  select_row(phi, mm, pm);
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  hess_block<MODEL>(P, pa, pm, blk[0]);
+}
+
+template <int MODEL, int VPW, typename T>
 __global__ void __launch_bounds__(kGatherThreads)
     k_hessian(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-              const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, int xcd, T* __restrict__ vals) {
+              const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
   const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
@@ -536,21 +518,6 @@ __global__ void __launch_bounds__(kGatherThreads)
         case M_MASS: tet_blocks<M_MASS>(m, q, params, e, a, stretchRef, blk); break;
         default: tet_blocks<M_NH_LEGACY>(m, q, params, e, a, stretchRef, blk); break;
       }
-      if (PSD) {
-#pragma unroll
-        for (int mm = 0; mm < 4; ++mm) {
-          blk[mm][0] += psd.tolPos;
-          blk[mm][4] += psd.tolPos;
-          blk[mm][8] += psd.tolPos;
-        }
-        const int fmt = psd.flag[e];
-        if (fmt) {
-          T phi[4][3];
-          load_phi(m, e, phi);
-          const int all4[4] = {0, 1, 2, 3};
-          add_psd_correction<T, 4>(phi, psd.dM + 45ll * e, fmt, a, all4, blk);
-        }
-      }
 #pragma unroll
       for (int mm = 0; mm < 4; ++mm) {
         T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
@@ -570,10 +537,10 @@ __global__ void __launch_bounds__(kGatherThreads)
 // updates the same LDS slot, and every slot still receives its contributions in ascending element
 // order.  Half the LDS per lane of the one-lane kernel (more waves per CU) for a duplicated
 // per-tet setup.
-template <int MODEL, bool PSD, int VPW, typename T>
+template <int MODEL, int VPW, typename T>
 __global__ void __launch_bounds__(kGatherThreads)
     k_hessian_pair(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-                   const uint8_t* __restrict__ mids, PsdView<T> psd, int stretchRef, int xcd, T* __restrict__ vals) {
+                   const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
   const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
@@ -606,21 +573,6 @@ __global__ void __launch_bounds__(kGatherThreads)
         case M_MASS: tet_blocks2<M_MASS>(m, q, params, e, a, h, stretchRef, blk); break;
         default: tet_blocks2<M_NH_LEGACY>(m, q, params, e, a, h, stretchRef, blk); break;
       }
-      if (PSD) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          blk[j][0] += psd.tolPos;
-          blk[j][4] += psd.tolPos;
-          blk[j][8] += psd.tolPos;
-        }
-        const int fmt = psd.flag[e];
-        if (fmt) {
-          T phi[4][3];
-          load_phi(m, e, phi);
-          const int two[2] = {h, h + 2};
-          add_psd_correction<T, 2>(phi, psd.dM + 45ll * e, fmt, a, two, blk);
-        }
-      }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int mm = h + 2 * j;
@@ -636,13 +588,68 @@ __global__ void __launch_bounds__(kGatherThreads)
   for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
 }
 
+// Same gather with four lanes per vertex: lane c of the quad owns the block of corner c of every
+// incident tet.  The four lanes of a vertex walk the same incidence list in lockstep and update four
+// distinct LDS slots per tet (its four corners are distinct vertices), so every slot still receives
+// its contributions in ascending element order.  A quarter of the LDS per lane of the one-lane
+// kernel: sixteen vertices per 64-lane workgroup, so LDS no longer caps the waves per CU, at the
+// price of a per-tet setup repeated on four lanes.
+template <int MODEL, int VPW, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_quad(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                   const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
+  if (grp * VPW >= m.nV) return;
+  const int v0 = grp * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 2, c = (int)threadIdx.x & 3;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incR[idx];
+      T blk[1][9];
+      switch (model_at<MODEL>(mids, e)) {
+        case M_NH: tet_block1<M_NH>(m, q, params, e, a, c, stretchRef, blk); break;
+        case M_STVK: tet_block1<M_STVK>(m, q, params, e, a, c, stretchRef, blk); break;
+        case M_COROT: tet_block1<M_COROT>(m, q, params, e, a, c, stretchRef, blk); break;
+        case M_ARAP: tet_block1<M_ARAP>(m, q, params, e, a, c, stretchRef, blk); break;
+        case M_MASS: tet_block1<M_MASS>(m, q, params, e, a, c, stretchRef, blk); break;
+        default: tet_block1<M_NH_LEGACY>(m, q, params, e, a, c, stretchRef, blk); break;
+      }
+      T* dst = row + 3 * ((rk >> (8 * c)) & 0xffu);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[0][3 * l + kk];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
 // ---- PSD projection (simple_psd_fix, src/simple_psd_fix.cpp:5-21) ------------------------------
 // K = vol B^T H B has the three translations in its null space; with Phi = Q R (thin QR),
 // K = (Q x I3) M (Q x I3)^T,  M = (R x I3) vol H (R x I3)^T (9x9), and
-// psd(K) = K + (Q x I3) dM (Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  dM = M' - M, M' = M with its
-// eigenvalues clamped at tol.  A tet needs dM only when M - tol I is not positive definite
-// (Cholesky test); dM is then computed by the tridiagonal-QL eigensolver and stored as k <= 3
-// columns (dM = W W^T) or packed (k > 3); flag[e] records the storage format (0 = none).
+// psd(K) = (Q x I3) M' (Q x I3)^T + max(tol,0)/4 (1 1^T x I3),  M' = M with its eigenvalues clamped
+// at tol.  One pass over the tets forms M and stores M' packed (45 values, the tet's record);
+// M' = M unless M - tol I fails the Cholesky test, and then M' = M + dM, dM = sum (tol - l) v v^T
+// over the k eigenvalues l below tol: by QL eigenvalues and inverse iteration for k <= 3, by the
+// full tridiagonal-QL eigensolver in a second pass for k > 3.  The assembly gather then reads only
+// the records and the tets' Q: the per-tet element math runs once, not once per corner.
 
 template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __restrict__ q,
@@ -679,44 +686,186 @@ __device__ __forceinline__ void count_flagged(bool flagged, int32_t* count) {
   if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (int32_t)__popcll(bal));
 }
 
+// flag of a tet whose record the first pass left to the second (more than three eigenvalues below
+// tol); never left in flag once both passes have run
+constexpr int kPsdPending = 1;
+
+// First pass: M for every tet, stored as its record; for a tet that fails the Cholesky test with at
+// most three eigenvalues below tol (the common case: the rotational modes of a mildly stressed tet)
+// the FP64 record becomes M + W W^T, W's columns from inverse iteration without an eigenvector
+// matrix, which keeps the pass at half the registers of the full eigensolver; tets with more are
+// marked pending.  The FP32 pass runs the full eigensolver on every flagged tet.  The records of the workgroup's 64 tets are built in LDS (M parked there while the
+// eigensolver overwrites it, each column added as it is formed) and leave as one contiguous,
+// coalesced block.
+constexpr int kRecPitch = 65;  // LDS pitch of one record entry over the 64 lanes (odd: no bank conflicts)
+
 template <int MODEL, typename T>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
-                int32_t* __restrict__ count, T* __restrict__ dM) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+                int32_t* __restrict__ count, T* __restrict__ rec) {
+  __shared__ T stash[45 * kRecPitch];
+  const int lane = (int)threadIdx.x;
+  const int e0 = (int)blockIdx.x * 64;
+  const int e = e0 + lane;
   bool need = false;
   T M[45];
   if (e < m.nT) {
     reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+#pragma unroll
+    for (int i = 0; i < 45; ++i) stash[i * kRecPitch + lane] = M[i];
     need = !shifted_pd(M, tol);
   }
   count_flagged(need, count);
+  if (e < m.nT) {
+    int fl = 0;
+    if (need) {
+      if constexpr (sizeof(T) == 8) {
+        const int k = psd_correction_low(M, tol, [&](int, const T* w) {
+#pragma unroll
+          for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = i; j < 9; ++j) stash[sym9(i, j) * kRecPitch + lane] += w[i] * w[j];
+        });
+        if (k > 3) fl = kPsdPending;
+      } else {
+        // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
+        // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
+        // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
+        T dm[45];
+        const int k = psd_correction(M, tol, dm);
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+          for (int j = i; j < 9; ++j) {
+            T acc = stash[sym9(i, j) * kRecPitch + lane];
+            if (k > 3) {
+              acc += dm[sym9(i, j)];
+            } else {
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
+            }
+            stash[sym9(i, j) * kRecPitch + lane] = acc;
+          }
+      }
+    }
+    flag[e] = (uint8_t)fl;
+  }
+  __syncthreads();
+  const int n = 45 * min(64, m.nT - e0);
+  T* out = rec + 45ll * e0;
+  for (int idx = lane; idx < n; idx += 64) {
+    const int t = idx / 45;
+    out[idx] = stash[(idx - 45 * t) * kRecPitch + t];
+  }
+}
+
+// Second pass: the pending tets, by the full eigensolver (M formed again; its rounding may differ
+// from the first pass's in the last bits, so every k is handled)
+template <int MODEL, typename T>
+__global__ void __launch_bounds__(64)
+    k_psd_full(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+               const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
+               T* __restrict__ rec) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT || flag[e] != kPsdPending) return;
+  T M[45], dm[45];
+  reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+  T* r = rec + 45ll * e;
+#pragma unroll
+  for (int i = 0; i < 45; ++i) r[i] = M[i];
+  const int k = psd_correction(M, tol, dm);
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) {
+      T acc = r[sym9(i, j)];
+      if (k > 3) {
+        acc += dm[sym9(i, j)];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
+      }
+      r[sym9(i, j)] = acc;
+    }
+  flag[e] = 0;
+}
+
+// Q of each tet's Phi = Q R (12 values, corner-major), stored tet-contiguous for the gather
+template <typename T>
+__global__ void __launch_bounds__(256) k_tet_q(MeshView m, T* __restrict__ Qout) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.nT) return;
-  int fmt = 0;
-  if (need) {
-    T* out = dM + 45ll * e;
-    fmt = psd_format(psd_correction(M, tol, out));
-    if (fmt >= 1 && fmt <= 3) {
-      // columns w (9 each) -> u = (Q (x) I3) w (12 each, corner-major), written back in place from the last
-      T phi[4][3], Q[4][3], R[3][3];
-      load_phi(m, e, phi);
-      phi_qr(phi, Q, R);
-      T w[27];
+  T phi[4][3], Q[4][3], R[3][3];
+  load_phi(m, e, phi);
+  phi_qr(phi, Q, R);
 #pragma unroll
-      for (int i = 0; i < 27; ++i) w[i] = (i < 9 * fmt) ? out[i] : T(0);
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int col = 0; col < 3; ++col)
-        if (col < fmt) {
+    for (int l = 0; l < 3; ++l) Qout[12ll * e + 3 * c + l] = Q[c][l];
+}
+
+// Assembly gather from the PSD records: the (a, m) block of tet e is tolPos I3 + sum_{p,r}
+// Q[a][p] Q[m][r] M'(p, r) (3x3 blocks of the record), evaluated so that the (m, a) block is its
+// exact transpose (the assembled matrix is exactly symmetric: the precision form's implied
+// diagonal relies on it).  LPV lanes per vertex: lane c owns
+// corners c, c + LPV, ...; as in the element gathers every LDS slot receives its contributions in
+// ascending element order.  No element math and no q, model or parameter reads here.
+template <int VPW, int LPV, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_rec(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd,
+                  T* __restrict__ vals) {
+  constexpr int NB = 4 / LPV;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
+  if (grp * VPW >= m.nV) return;
+  const int v0 = grp * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x / LPV, c = (int)threadIdx.x % LPV;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
+      const int e = code >> 2, a = code & 3;
+      const uint32_t rk = m.incR[idx];
+      const T* S = rec + 45ll * e;
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
+      for (int j = 0; j < NB; ++j) {
+        const int mm = c + LPV * j;
+        const int lo = min(a, mm), hi = max(a, mm);
+        // the two rows of Q by address (no per-lane selection among four loaded rows)
+        T ql[3], qh[3];
 #pragma unroll
-            for (int l = 0; l < 3; ++l)
-              out[12 * col + 3 * c + l] = Q[c][0] * w[9 * col + l] + Q[c][1] * w[9 * col + 3 + l] + Q[c][2] * w[9 * col + 6 + l];
+        for (int l = 0; l < 3; ++l) {
+          ql[l] = tq[12ll * e + 3 * lo + l];
+          qh[l] = tq[12ll * e + 3 * hi + l];
         }
+        T blk[9];
+        rec_block(ql, qh, a, mm, S, tolPos, blk);
+        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
+      }
     }
   }
-  flag[e] = (uint8_t)fmt;
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
 }
 
 // ---- mass / external force / setup / spmv -----------------------------------------------------------
@@ -820,60 +969,116 @@ int ensure_geometry(bh_mesh* m, hipStream_t s) {
     return ensure_f32(m, s);
 }
 
-template <int MODEL, bool PSD, int VPW, int LPV, typename T>
-int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd,
-                       T* vals, hipStream_t s) {
-  const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
-  const size_t lds = sizeof(T) * (size_t)std::max(seg, 1);
-  auto kern = (LPV == 1) ? k_hessian<MODEL, PSD, VPW, T> : k_hessian_pair<MODEL, PSD, VPW, T>;
-  const int threads = kGatherThreads;
+// LDS limit of the gather kernel `kern`, raised once per kernel to what the device allows
+template <typename K>
+void gather_lds_attr(K kern) {
   static std::once_flag once;
   std::call_once(once, [&] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               max_lds_bytes());
   });
-  const int groups = (m->nV + VPW - 1) / VPW;
-  const int grid = m->xcdGroups ? 8 * ((groups + 7) / 8) : groups;
+}
+
+int gather_grid(const bh_mesh* m, int vpw) {
+  const int groups = (m->nV + vpw - 1) / vpw;
+  return m->xcdGroups ? 8 * ((groups + 7) / 8) : groups;
+}
+
+template <int VPW, typename T>
+size_t gather_lds(const bh_mesh* m) {
+  const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
+  return sizeof(T) * (size_t)std::max(seg, 1);
+}
+
+template <int MODEL, int VPW, int LPV, typename T>
+int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T* vals,
+                       hipStream_t s) {
+  auto kern = (LPV == 1) ? k_hessian<MODEL, VPW, T>
+              : (LPV == 2) ? k_hessian_pair<MODEL, VPW, T> : k_hessian_quad<MODEL, VPW, T>;
+  gather_lds_attr(kern);
+  const size_t lds = gather_lds<VPW, T>(m);
   ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, s, view(m), q, params, mids, psd,
+  hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), q, params, mids,
                      m->opts.stretch_reference_layout, m->xcdGroups, vals);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
-template <int MODEL, bool PSD, typename T>
-int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, PsdView<T> psd, T* vals,
-                   hipStream_t s) {
+// Picks the workgroup shape (vertices per workgroup, lanes per vertex) from the mesh's tuning and
+// the LDS the densest vertex group needs, and calls launch(std::integral_constant VPW, LPV).
+template <typename T, typename F>
+int pick_gather_shape(const bh_mesh* m, F&& launch) {
+  using std::integral_constant;
   const size_t lim = (size_t)max_lds_bytes();
   const size_t s64 = sizeof(T) * (size_t)m->maxSeg64, s32 = sizeof(T) * (size_t)m->maxSeg32,
                s16 = sizeof(T) * (size_t)m->maxSeg16;
+  if (m->gatherLanes == 4 && s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 4>());
   if (m->gatherLanes == 2) {
-    if (s32 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 32, 2>(m, q, params, mids, psd, vals, s);
-    if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 2>(m, q, params, mids, psd, vals, s);
+    if (s32 <= lim / 2) return launch(integral_constant<int, 32>(), integral_constant<int, 2>());
+    if (s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 2>());
   }
-  if (s64 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 64, 1>(m, q, params, mids, psd, vals, s);
-  if (s32 <= lim / 2) return launch_hessian_vpw<MODEL, PSD, 32, 1>(m, q, params, mids, psd, vals, s);
-  if (s16 <= lim) return launch_hessian_vpw<MODEL, PSD, 16, 1>(m, q, params, mids, psd, vals, s);
+  if (s64 <= lim / 2) return launch(integral_constant<int, 64>(), integral_constant<int, 1>());
+  if (s32 <= lim / 2) return launch(integral_constant<int, 32>(), integral_constant<int, 1>());
+  if (s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 1>());
   return BH_UNSUPPORTED;  // vertex valence too high for the LDS gather
 }
 
-// One pass over all tets: M, the Cholesky test, and for the tets that fail it the eigen-solve and
-// the stored correction.  A single schedule on purpose: the same element math compiled into two
-// kernels may contract multiply-adds differently, and a per-call choice between two schedules
-// would make repeated calls differ in the last bits.
 template <int MODEL, typename T>
-int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T tol, T** dM, hipStream_t s) {
+int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T* vals, hipStream_t s) {
+  return pick_gather_shape<T>(m, [&](auto vpw, auto lpv) {
+    return launch_hessian_vpw<MODEL, decltype(vpw)::value, decltype(lpv)::value, T>(m, q, params, mids, vals, s);
+  });
+}
+
+template <typename T>
+int launch_hessian_rec(const bh_mesh* m, const T* tq, const T* rec, T tolPos, T* vals, hipStream_t s) {
+  return pick_gather_shape<T>(m, [&](auto vpw, auto lpv) {
+    constexpr int VPW = decltype(vpw)::value, LPV = decltype(lpv)::value;
+    auto kern = k_hessian_rec<VPW, LPV, T>;
+    gather_lds_attr(kern);
+    const size_t lds = gather_lds<VPW, T>(m);
+    ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
+    hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
+                       m->xcdGroups, vals);
+    BH_CHECK(hipGetLastError());
+    return BH_OK;
+  });
+}
+
+// the tets' Q factors (made on first use; FP32 copy for the precision form)
+template <typename T>
+int ensure_tet_q(bh_mesh* m, const T** tq, hipStream_t s) {
+  void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->tetQ) : reinterpret_cast<void**>(&m->tetQ32);
+  if (!*buf) {
+    BH_CHECK(hipMalloc(buf, sizeof(T) * 12 * std::max<size_t>(1, m->nT)));
+    if (m->nT) {
+      hipLaunchKernelGGL((k_tet_q<T>), dim3((m->nT + 255) / 256), dim3(256), 0, s, view(m), reinterpret_cast<T*>(*buf));
+      BH_CHECK(hipGetLastError());
+    }
+  }
+  *tq = reinterpret_cast<const T*>(*buf);
+  return BH_OK;
+}
+
+// Two passes over all tets (the second only works on the tets the first marked pending).  Which
+// pass serves a tet depends only on that tet's data, never on the call, so repeated calls agree to
+// the last bit.
+template <int MODEL, typename T>
+int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T tol, T** rec, hipStream_t s) {
   if (!m->psdFlag) {
     BH_CHECK(hipMalloc(&m->psdFlag, std::max<size_t>(1, m->nT)));
     BH_CHECK(hipMalloc(&m->psdCount, sizeof(int32_t)));
   }
   void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->psdDM) : reinterpret_cast<void**>(&m->psdDM32);
   if (!*buf) BH_CHECK(hipMalloc(buf, sizeof(T) * 45 * std::max<size_t>(1, m->nT)));
-  *dM = reinterpret_cast<T*>(*buf);
+  *rec = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
   ProfScope ps(m, BH_KT_PSD, s);
   hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
-                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, *dM);
+                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, *rec);
+  BH_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((k_psd_full<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdFlag, *rec);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -882,14 +1087,15 @@ template <int MODEL, typename T>
 int hessian_model(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, const bh_psd* psd, T* vals,
                   hipStream_t s) {
   if (psd && psd->enabled) {
-    T* dM = nullptr;
-    int st = run_psd_passes<MODEL, T>(m, q, params, mids, (T)psd->tol, &dM, s);
+    T* rec = nullptr;
+    const T* tq = nullptr;
+    int st = ensure_tet_q<T>(m, &tq, s);
     if (st != BH_OK) return st;
-    PsdView<T> pv{(T)(std::max(psd->tol, 0.0) * 0.25), (T)psd->tol, m->psdFlag, dM};
-    return launch_hessian<MODEL, true>(m, q, params, mids, pv, vals, s);
+    st = run_psd_passes<MODEL, T>(m, q, params, mids, (T)psd->tol, &rec, s);
+    if (st != BH_OK) return st;
+    return launch_hessian_rec<T>(m, tq, rec, (T)(std::max(psd->tol, 0.0) * 0.25), vals, s);
   }
-  PsdView<T> pv{T(0), T(0), nullptr, nullptr};
-  return launch_hessian<MODEL, false>(m, q, params, mids, pv, vals, s);
+  return launch_hessian<MODEL>(m, q, params, mids, vals, s);
 }
 
 template <typename T>
@@ -1105,7 +1311,7 @@ void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->vdeg, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32};
+                  m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32, m->tetQ, m->tetQ32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -1365,8 +1571,7 @@ int bh_linear_tetmesh_mass_matrix_device(bh_mesh* m, const double* d_density, do
   if (!m || !d_density || !d_vals) return BH_BAD_ARG;
   if (m->nV == 0) return BH_OK;
   // params column 0 carries the density; column 1 is never read for the mass pseudo-model
-  PsdView<double> pv{0.0, 0.0, nullptr, nullptr};
-  return launch_hessian<M_MASS, false>(m, m->stage_q, d_density, nullptr, pv, d_vals, as_stream(s));
+  return launch_hessian<M_MASS>(m, m->stage_q, d_density, nullptr, d_vals, as_stream(s));
 }
 
 int bh_linear_tetmesh_const_ext_force_device(bh_mesh* m, const double* traction, double* d_f, bh_stream_t s) {
@@ -1405,14 +1610,14 @@ int bh_mesh_set_rest_positions(bh_mesh* m, const double* V) {
 }
 
 int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
-  if (!m || (lanes != 1 && lanes != 2)) return BH_BAD_ARG;
+  if (!m || (lanes != 1 && lanes != 2 && lanes != 4)) return BH_BAD_ARG;
   m->gatherLanes = lanes;
   return BH_OK;
 }
 
 int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (!m || !t) return BH_BAD_ARG;
-  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2) return BH_BAD_ARG;
+  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2 && t->gather_lanes != 4) return BH_BAD_ARG;
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
   if (t->sweep_lanes != 0 && t->sweep_lanes != 4 && t->sweep_lanes != 8) return BH_BAD_ARG;

@@ -648,19 +648,71 @@ __host__ __device__ __forceinline__ bool shifted_pd(const T a[45], T tol) {
   return pd;
 }
 
-// Eigen-decomposition of a symmetric 9x9 (packed upper triangle a[sym9(i, j)], destroyed):
-// Householder reduction to tridiagonal form, the reflectors accumulated into Z, then implicit QL
-// sweeps with the Wilkinson-type shift from the leading 2x2 (Golub & Van Loan, Matrix
-// Computations, 4th ed., sections 8.3.1 and 8.3.5, in the QL orientation that deflates from the
-// top).  On return d holds the eigenvalues (unsorted) and column c of Z (Z[r + 9 c]) the unit
-// eigenvector of d[c].  Every array index is a compile-time constant after unrolling, so a, d, e
-// and Z stay in registers.
-template <typename T>
-__host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
+// Implicit QL on the tridiagonal (d, e) (e[i] couples d[i] and d[i+1]; e[8] = 0): on return d holds
+// the eigenvalues (unsorted).  With WANTZ the rotations are also applied to the columns of Z.
+template <bool WANTZ, typename T>
+__host__ __device__ __forceinline__ void tridiag_ql(T d[9], T e[9], T* Z) {
   constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
+  // Rotations at or below the split point m are evaluated and discarded (select), which keeps the
+  // sweep free of lane-divergent branches.
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    for (int iter = 0; iter < 40; ++iter) {
+      int m = 8;
+#pragma unroll
+      for (int j = 7; j >= l; --j)
+        if (fabs(e[j]) <= kEps * (fabs(d[j]) + fabs(d[j + 1]))) m = j;
+      if (m == l) break;
+      T dm = d[8];
+#pragma unroll
+      for (int j = l + 1; j < 8; ++j) dm = (j == m) ? d[j] : dm;
+      T g = (d[l + 1] - d[l]) / (T(2) * e[l]);
+      const T r0 = sqrt(g * g + T(1));
+      g = dm - d[l] + e[l] / (g + (g >= T(0) ? r0 : -r0));
+      T s = T(1), c = T(1), pacc = T(0);
+#pragma unroll
+      for (int i = 7; i >= l; --i) {
+        const bool act = i < m;
+        const T f = s * e[i], b = c * e[i];
+        const T r2 = f * f + g * g;
+        const T inv = r2 > T(0) ? inv_sqrt(r2) : T(0);  // one reciprocal square root per rotation
+        const T r = r2 * inv;
+        const T sn = r2 > T(0) ? f * inv : T(0);
+        const T cn = r2 > T(0) ? g * inv : T(1);
+        const T gi = d[i + 1] - pacc;
+        const T ri = (d[i] - gi) * sn + T(2) * cn * b;
+        const T pi = sn * ri;
+        e[i + 1] = act ? r : e[i + 1];
+        d[i + 1] = act ? gi + pi : d[i + 1];
+        s = act ? sn : s;
+        c = act ? cn : c;
+        pacc = act ? pi : pacc;
+        g = act ? cn * ri - b : g;
+        if constexpr (WANTZ) {
+          const T zs = act ? sn : T(0), zc = act ? cn : T(1);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            const T zf = Z[k + 9 * (i + 1)], zi = Z[k + 9 * i];
+            Z[k + 9 * (i + 1)] = zs * zi + zc * zf;
+            Z[k + 9 * i] = zc * zi - zs * zf;
+          }
+        }
+      }
+      d[l] -= pacc;
+      e[l] = g;
+#pragma unroll
+      for (int j = l + 1; j < 9; ++j) e[j] = (j == m) ? T(0) : e[j];
+    }
+  }
+}
+
+// Householder reduction of a symmetric 9x9 (packed upper triangle a[sym9(i, j)]) to tridiagonal form
+// (d, e), e[i] coupling d[i] and d[i+1]: reflector k (k = 0..6) maps row k's tail a(k, k+1..8) onto its
+// first entry and is kept in that row of a with its scale beta[k], so Z = H_0 H_1 ... H_6 (the
+// eigenvectors' back-transformation) can be applied later.
+template <typename T>
+__host__ __device__ __forceinline__ void sym9_tridiag(T a[45], T d[9], T e[9], T beta[7]) {
   constexpr T kTiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38);
-  T e[9], beta[7];
-  // reflector k maps row k's tail a(k, k+1..8) onto its first entry; v_k is kept in that row
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
     T sig2 = T(0);
@@ -702,6 +754,19 @@ __host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
   e[8] = T(0);
 #pragma unroll
   for (int i = 0; i < 9; ++i) d[i] = a[sym9(i, i)];
+}
+
+// Eigen-decomposition of a symmetric 9x9 (packed upper triangle a[sym9(i, j)], destroyed):
+// Householder reduction to tridiagonal form, the reflectors accumulated into Z, then implicit QL
+// sweeps with the Wilkinson-type shift from the leading 2x2 (Golub & Van Loan, Matrix
+// Computations, 4th ed., sections 8.3.1 and 8.3.5, in the QL orientation that deflates from the
+// top).  On return d holds the eigenvalues (unsorted) and column c of Z (Z[r + 9 c]) the unit
+// eigenvector of d[c].  Every array index is a compile-time constant after unrolling, so a, d, e
+// and Z stay in registers.
+template <typename T>
+__host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
+  T e[9], beta[7];
+  sym9_tridiag(a, d, e, beta);
   // Z = H_0 H_1 ... H_6, accumulated from the right end (H_k touches rows and columns k+1..8)
 #pragma unroll
   for (int i = 0; i < 81; ++i) Z[i] = (i % 10 == 0) ? T(1) : T(0);
@@ -717,55 +782,7 @@ __host__ __device__ inline void sym9_eigen(T a[45], T d[9], T Z[81]) {
       for (int r = k + 1; r < 9; ++r) Z[r + 9 * c] -= s * a[sym9(k, r)];
     }
   }
-  // implicit QL on (d, e): e[i] couples d[i] and d[i+1].  Rotations at or below the split point m
-  // are evaluated and discarded (select), which keeps the sweep free of lane-divergent branches.
-#pragma unroll
-  for (int l = 0; l < 8; ++l) {
-    for (int iter = 0; iter < 40; ++iter) {
-      int m = 8;
-#pragma unroll
-      for (int j = 7; j >= l; --j)
-        if (fabs(e[j]) <= kEps * (fabs(d[j]) + fabs(d[j + 1]))) m = j;
-      if (m == l) break;
-      T dm = d[8];
-#pragma unroll
-      for (int j = l + 1; j < 8; ++j) dm = (j == m) ? d[j] : dm;
-      T g = (d[l + 1] - d[l]) / (T(2) * e[l]);
-      const T r0 = sqrt(g * g + T(1));
-      g = dm - d[l] + e[l] / (g + (g >= T(0) ? r0 : -r0));
-      T s = T(1), c = T(1), pacc = T(0);
-#pragma unroll
-      for (int i = 7; i >= l; --i) {
-        const bool act = i < m;
-        const T f = s * e[i], b = c * e[i];
-        const T r2 = f * f + g * g;
-        const T inv = r2 > T(0) ? inv_sqrt(r2) : T(0);  // one reciprocal square root per rotation
-        const T r = r2 * inv;
-        const T sn = r2 > T(0) ? f * inv : T(0);
-        const T cn = r2 > T(0) ? g * inv : T(1);
-        const T gi = d[i + 1] - pacc;
-        const T ri = (d[i] - gi) * sn + T(2) * cn * b;
-        const T pi = sn * ri;
-        e[i + 1] = act ? r : e[i + 1];
-        d[i + 1] = act ? gi + pi : d[i + 1];
-        s = act ? sn : s;
-        c = act ? cn : c;
-        pacc = act ? pi : pacc;
-        g = act ? cn * ri - b : g;
-        const T zs = act ? sn : T(0), zc = act ? cn : T(1);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          const T zf = Z[k + 9 * (i + 1)], zi = Z[k + 9 * i];
-          Z[k + 9 * (i + 1)] = zs * zi + zc * zf;
-          Z[k + 9 * i] = zc * zi - zs * zf;
-        }
-      }
-      d[l] -= pacc;
-      e[l] = g;
-#pragma unroll
-      for (int j = l + 1; j < 9; ++j) e[j] = (j == m) ? T(0) : e[j];
-    }
-  }
+  tridiag_ql<true>(d, e, Z);
 }
 
 // Clamp correction of a reduced matrix M (packed upper triangle, destroyed): with
@@ -808,50 +825,164 @@ __host__ __device__ inline int psd_correction(T a[45], T tol, T* out) {
   return k;
 }
 
-// storage format code of a tet's clamp correction: 0 none, 1..3 low-rank columns, 4 packed
-__host__ __device__ __forceinline__ int psd_format(int k) { return k == 0 ? 0 : (k <= 3 ? k : 4); }
-
-// blk_j += (a, m_j) block of (Q (x) I3) W W^T (Q (x) I3)^T for the NB corners m_j, from the k <= 3
-// stored columns W (9 each): U_c = (Q_c (x) I3) W, block = U_a U_m^T
-template <typename T, int NB>
-__host__ __device__ __forceinline__ void psd_lowrank_blocks(const T Q[4][3], int a, const int* mlist, int k,
-                                                            const T* __restrict__ W, T blk[NB][9]) {
-  T qa[3];
+// One step of inverse iteration on a symmetric tridiagonal T (d, e): x <- (T - mu I)^{-1} x by
+// Gaussian elimination with row interchanges (at step i the pivot row is whichever of the active
+// row i and row i+1 has the larger entry in column i, so U gains a second superdiagonal), fused with
+// the forward substitution; pivots smaller than pmin in magnitude are replaced by +-pmin, as
+// inverse iteration at a computed eigenvalue needs the nearly singular solve (Golub & Van Loan,
+// section 8.2.2; Wilkinson, The Algebraic Eigenvalue Problem, ch. 5).
+template <typename T>
+__host__ __device__ __forceinline__ void tridiag_shift_solve(const T d[9], const T e[9], T mu, T pmin, T x[9]) {
+  T r0[9], u1[8], u2[7];
+  T al = d[0] - mu, be = e[0], act = x[0];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) qa[j] = (a == 0) ? Q[0][j] : (a == 1) ? Q[1][j] : (a == 2) ? Q[2][j] : Q[3][j];
-#pragma unroll
-  for (int col = 0; col < 3; ++col) {
-    if (col < k) {
-      T w[9];
-#pragma unroll
-      for (int r = 0; r < 9; ++r) w[r] = W[9 * col + r];
-      T ua[3];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) ua[l] = qa[0] * w[l] + qa[1] * w[3 + l] + qa[2] * w[6 + l];
-#pragma unroll
-      for (int n = 0; n < NB; ++n) {
-        const int mm = mlist[n];
-        T qm[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) qm[j] = (mm == 0) ? Q[0][j] : (mm == 1) ? Q[1][j] : (mm == 2) ? Q[2][j] : Q[3][j];
-        T um[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) um[l] = qm[0] * w[l] + qm[1] * w[3 + l] + qm[2] * w[6 + l];
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) blk[n][3 * l + kk] += ua[l] * um[kk];
-      }
-    }
+  for (int i = 0; i < 8; ++i) {
+    const T c = e[i], dn = d[i + 1] - mu, en = (i < 7) ? e[i + 1] : T(0);
+    const bool sw = fabs(c) > fabs(al);
+    T p0 = sw ? c : al;
+    p0 = fabs(p0) < pmin ? (p0 < T(0) ? -pmin : pmin) : p0;
+    const T rp = T(1) / p0;
+    const T p1 = sw ? dn : be, p2 = sw ? en : T(0);
+    const T ml = (sw ? al : c) * rp;
+    al = (sw ? be : dn) - ml * p1;
+    be = (sw ? T(0) : en) - ml * p2;
+    const T nx = x[i + 1];
+    const T piv = sw ? nx : act;
+    act = (sw ? act : nx) - ml * piv;
+    x[i] = piv;
+    r0[i] = rp;
+    u1[i] = p1;
+    if (i < 7) u2[i] = p2;
   }
+  al = fabs(al) < pmin ? (al < T(0) ? -pmin : pmin) : al;
+  x[8] = act / al;
+  x[7] = (x[7] - u1[7] * x[8]) * r0[7];
+#pragma unroll
+  for (int i = 6; i >= 0; --i) x[i] = (x[i] - u1[i] * x[i + 1] - u2[i] * x[i + 2]) * r0[i];
 }
 
-// Tm[r] = sum_p Q[a][p] dM_{p r} (3x3 sub-blocks, row-major l, k) for block row a
+// x <- x / |x|, scaled by its largest entry first so the squares cannot overflow
 template <typename T>
-__host__ __device__ __forceinline__ void psd_row_factor(const T Q[4][3], int a, const T* dM, T Tm[3][9]) {
-  T qa[3];
+__host__ __device__ __forceinline__ void normalize9(T x[9]) {
+  T mx = T(0);
 #pragma unroll
-  for (int j = 0; j < 3; ++j) qa[j] = (a == 0) ? Q[0][j] : (a == 1) ? Q[1][j] : (a == 2) ? Q[2][j] : Q[3][j];
+  for (int i = 0; i < 9; ++i) mx = fmax(mx, fabs(x[i]));
+  const T rs = mx > T(0) ? T(1) / mx : T(1);
+  T n2 = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    x[i] *= rs;
+    n2 += x[i] * x[i];
+  }
+  const T rn = n2 > T(0) ? inv_sqrt(n2) : T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] *= rn;
+}
+
+// x <- x - (x.y) y for unit y
+template <typename T>
+__host__ __device__ __forceinline__ void deflate9(T x[9], const T y[9]) {
+  T s = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s += x[i] * y[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] -= s * y[i];
+}
+
+// Clamp correction of a reduced matrix M (packed upper triangle, destroyed) when at most three
+// eigenvalues lie below tol, without forming the eigenvector matrix: Householder tridiagonalisation
+// (reflectors kept), eigenvalues by implicit QL on a copy of (d, e) with no rotations applied to
+// vectors, then for each eigenvalue mu_j < tol three steps of inverse iteration on the tridiagonal
+// form from a fixed start vector, each followed by Gram-Schmidt against the vectors already found
+// (members of a cluster then span its invariant subspace; the clamp only needs that subspace, and
+// the error any rotation within a cluster leaves in dM is bounded by the cluster's width), and the
+// result mapped back through the reflectors.  Hands each column w_j = sqrt(tol - mu_j) v_j to
+// emit(j, w_j) as soon as it is formed and returns k; for k > 3 it emits nothing (the caller runs
+// psd_correction on the matrix again).
+template <typename T, typename Emit>
+__host__ __device__ inline int psd_correction_low(T a[45], T tol, Emit&& emit) {
+  constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
+  constexpr T kFloor = sizeof(T) == 8 ? T(1e-290) : T(1e-30);
+  T d[9], e[9], beta[7];
+  sym9_tridiag(a, d, e, beta);
+  T lam[9], le[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    lam[i] = d[i];
+    le[i] = e[i];
+  }
+  tridiag_ql<false>(lam, le, (T*)nullptr);
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) k += (lam[i] < tol) ? 1 : 0;
+  if (k > 3) return k;
+  T nrm = T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) nrm = fmax(nrm, fabs(d[i]) + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : T(0)));
+  const T pmin = fmax(kEps * nrm, kFloor);
+  // fixed start vectors, entries of magnitude 0.25..1 with mixed signs
+  const T start[27] = {-0.878349, 0.700260, -0.389749, 0.521367, 0.650314, -0.272610, 0.266693, -0.944151, 0.636744,
+                       0.754404,  0.638914, -0.455676, -0.561953, 0.431300, 0.811135, 0.623978, 0.274232, -0.751266,
+                       -0.314536, 0.953812, 0.275041, -0.520199, 0.844775, -0.664799, 0.968093, -0.325816, -0.891424};
+  // mus[j] = the j-th eigenvalue below tol (in QL order)
+  T mus[3] = {T(0), T(0), T(0)};
+  {
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const bool sel = lam[i] < tol;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) mus[j] = (sel && cnt == j) ? lam[i] : mus[j];
+      cnt += sel ? 1 : 0;
+    }
+  }
+  T y[3][9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < k) {
+      const T mu = mus[j];
+      T x[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) x[i] = start[9 * j + i];
+#pragma unroll
+      for (int it = 0; it < 3; ++it) {
+        tridiag_shift_solve(d, e, mu, pmin, x);
+#pragma unroll
+        for (int q = 0; q < j; ++q) deflate9(x, y[q]);
+        if (it == 2) {
+#pragma unroll
+          for (int q = 0; q < j; ++q) deflate9(x, y[q]);
+        }
+        normalize9(x);
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) y[j][i] = x[i];
+      // v = H_0 H_1 ... H_6 x
+#pragma unroll
+      for (int r = 6; r >= 0; --r) {
+        T sdot = T(0);
+#pragma unroll
+        for (int c = r + 1; c < 9; ++c) sdot += a[sym9(r, c)] * x[c];
+        sdot *= beta[r];
+#pragma unroll
+        for (int c = r + 1; c < 9; ++c) x[c] -= sdot * a[sym9(r, c)];
+      }
+      const T w = sqrt(tol - mu);
+#pragma unroll
+      for (int r = 0; r < 9; ++r) x[r] *= w;
+      emit(j, x);
+    }
+  }
+  return k;
+}
+
+// class of a tet's clamp: 0 none, 1..3 eigenvalues below tol, 4 more
+__host__ __device__ __forceinline__ int psd_format(int k) { return k == 0 ? 0 : (k <= 3 ? k : 4); }
+
+// Tm[r] = sum_p qa[p] S_{p r} (3x3 sub-blocks of the packed symmetric 9x9 S, row-major l, k): the
+// row factor of the corner whose row of Q is qa
+template <typename T>
+__host__ __device__ __forceinline__ void row_factor(const T qa[3], const T* S, T Tm[3][9]) {
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -860,16 +991,33 @@ __host__ __device__ __forceinline__ void psd_row_factor(const T Q[4][3], int a, 
       for (int kk = 0; kk < 3; ++kk) {
         T s = T(0);
 #pragma unroll
-        for (int p = 0; p < 3; ++p) s += qa[p] * dM[sym9(3 * p + l, 3 * r + kk)];
+        for (int p = 0; p < 3; ++p) s += qa[p] * S[sym9(3 * p + l, 3 * r + kk)];
         Tm[r][3 * l + kk] = s;
       }
 }
 
-// blk += sum_r qm[r] Tm[r]  (the (a, m) block of (Q (x) I3) dM (Q (x) I3)^T)
+// blk = the (a, m) block of (Q (x) I3) S (Q (x) I3)^T + tolPos I3 for the packed symmetric 9x9 S,
+// from the rows ql = Q[lo] and qh = Q[hi] of the corners lo = min(a, m), hi = max(a, m): computed as
+// the (lo, hi) block (row factor of lo, then hi's combination) and transposed when a > m, so the
+// (m, a) block of a tet is bit for bit the transpose of its (a, m) block; a diagonal block is
+// symmetrized as (B + B^T) / 2.
 template <typename T>
-__host__ __device__ __forceinline__ void psd_add_block(const T* qm, const T Tm[3][9], T* blk) {
+__host__ __device__ __forceinline__ void rec_block(const T ql[3], const T qh[3], int a, int m, const T* S, T tolPos,
+                                                   T blk[9]) {
+  T Tm[3][9];
+  row_factor(ql, S, Tm);
+  T b[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) blk[i] += qm[0] * Tm[0][i] + qm[1] * Tm[1][i] + qm[2] * Tm[2][i];
+  for (int i = 0; i < 9; ++i) b[i] = qh[0] * Tm[0][i] + qh[1] * Tm[1][i] + qh[2] * Tm[2][i];
+  const bool tr = a > m, sy = a == m;
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const T x = b[3 * l + k], y = b[3 * k + l];
+      const T v = sy ? T(0.5) * (x + y) : (tr ? y : x);
+      blk[3 * l + k] = (l == k) ? v + tolPos : v;
+    }
 }
 
 }  // namespace bh

@@ -101,15 +101,15 @@ int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
  * counterpart in the reference (its V argument is passed on every call and unused by the
  * element math). */
 int bh_mesh_set_rest_positions(bh_mesh* mesh, const double* V);
-/* Assembly-kernel shape: 1 or 2 lanes per vertex (with 2, each lane owns two of the four corner
- * blocks of every incident tet).  Same per-slot fold order. */
+/* Assembly-kernel shape: 1, 2 or 4 (the default) lanes per vertex (with 2, each lane owns two of the
+ * four corner blocks of every incident tet; with 4, one).  Same per-slot fold order. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
 
 /* Kernel-schedule knobs of a mesh (no reference counterpart: the reference has one CPU schedule).
  * Every setting keeps each slot's ascending-element fold, so results agree to the last bits; a
  * fixed setting is bitwise reproducible. */
 typedef struct bh_tuning {
-  int gather_lanes; /* 0 keep, else 1 or 2 lanes per vertex in the assembly gather */
+  int gather_lanes; /* 0 keep, else 1, 2 or 4 lanes per vertex in the assembly gather */
   int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
   int coloring;     /* 0 keep, 1 DSatur (default), 2 greedy in vertex order: the vertex colouring that
                        bh_precond_create uses for the colour sweeps */

@@ -81,7 +81,10 @@ static double energy(const double* phi12, const double* x12, double vol, double 
 }
 
 
-// simple_psd_fix through the reduced 9x9 (the GPU decomposition): returns 1 when the tet needed clamping
+// simple_psd_fix through the reduced 9x9 as the device computes it: the record M' = M (+ dM when M -
+// tol I fails the Cholesky test: inverse iteration for k <= 3 eigenvalues below tol, the full
+// eigensolver on a fresh M for k > 3), then every block from M' and Q.  Returns the storage format
+// code of k (0 none, 1..3 low rank, 4 more)
 template <int MODEL, typename T>
 static int psd_blocks(const double* phi12, const double* x12, double vol, double p0, double p1, int ref, double tol,
                       double* K) {
@@ -90,32 +93,43 @@ static int psd_blocks(const double* phi12, const double* x12, double vol, double
   make_F(phi12, x12, phi, P.F);
   P.vol = (T)vol;
   pre_hessian<MODEL>(P, (T)p0, (T)p1, ref != 0);
-  T Q[4][3], R[3][3], M[45], dM[45];
+  T Q[4][3], R[3][3], M[45], rec[45];
   phi_qr(phi, Q, R);
   reduced_packed<MODEL>(P, R, M);
-  const bool pd = shifted_pd(M, (T)tol);
-  const int fmt = pd ? 0 : psd_format(psd_correction(M, (T)tol, dM));
-  const T tp = (T)((tol > 0 ? tol : 0.0) * 0.25);
-  const int all4[4] = {0, 1, 2, 3};
-  for (int a = 0; a < 4; ++a) {
-    T b[4][9];
-    for (int m = 0; m < 4; ++m) {
-      hess_block<MODEL>(P, phi[a], phi[m], b[m]);
-      b[m][0] += tp;
-      b[m][4] += tp;
-      b[m][8] += tp;
+  for (int i = 0; i < 45; ++i) rec[i] = M[i];
+  int k = 0;
+  if (!shifted_pd(M, (T)tol)) {
+    T Mc[45];
+    for (int i = 0; i < 45; ++i) Mc[i] = M[i];
+    // FP64: inverse iteration for k <= 3, the full eigensolver for k > 3; FP32: the full eigensolver
+    k = sizeof(T) == 8 ? psd_correction_low(Mc, (T)tol, [&](int, const T* w) {
+      for (int i = 0; i < 9; ++i)
+        for (int j = i; j < 9; ++j) rec[sym9(i, j)] += w[i] * w[j];
+    }) : 4;
+    if (k > 3) {
+      T dM[45];
+      k = psd_correction(M, (T)tol, dM);
+      for (int i = 0; i < 9; ++i)
+        for (int j = i; j < 9; ++j) {
+          T acc = rec[sym9(i, j)];
+          if (k > 3)
+            acc += dM[sym9(i, j)];
+          else
+            for (int c = 0; c < k; ++c) acc += dM[9 * c + i] * dM[9 * c + j];
+          rec[sym9(i, j)] = acc;
+        }
     }
-    if (fmt >= 1 && fmt <= 3) psd_lowrank_blocks<T, 4>(Q, a, all4, fmt, dM, b);
-    if (fmt == 4) {
-      T Tm[3][9];
-      psd_row_factor(Q, a, dM, Tm);
-      for (int m = 0; m < 4; ++m) psd_add_block(Q[m], Tm, b[m]);
-    }
-    for (int m = 0; m < 4; ++m)
-      for (int l = 0; l < 3; ++l)
-        for (int k = 0; k < 3; ++k) K[(3 * a + l) + 12 * (3 * m + k)] = (double)b[m][3 * l + k];
   }
-  return fmt;
+  const T tp = (T)((tol > 0 ? tol : 0.0) * 0.25);
+  for (int a = 0; a < 4; ++a) {
+    for (int m = 0; m < 4; ++m) {
+      T b[9];
+      rec_block(Q[a < m ? a : m], Q[a < m ? m : a], a, m, rec, tp, b);
+      for (int l = 0; l < 3; ++l)
+        for (int kk = 0; kk < 3; ++kk) K[(3 * a + l) + 12 * (3 * m + kk)] = (double)b[3 * l + kk];
+    }
+  }
+  return psd_format(k);
 }
 
 // eigen-decomposition of a symmetric 9x9 (col-major; upper triangle read): eigenvalues d, vectors Z
@@ -138,6 +152,19 @@ static void eig9(const double* A, double* d, double* Z) {
     default: return fp32 ? fn<4, float>(__VA_ARGS__) : fn<4, double>(__VA_ARGS__); \
   }
 
+// clamp correction of a symmetric 9x9 (col-major; upper triangle read) by the inverse-iteration
+// path: returns k and, for k <= 3, the columns w_j in W[9 j + r]
+template <typename T>
+static int psd_low9(const double* A, double tol, double* W) {
+  T a[45], w[27];
+  for (int r = 0; r < 9; ++r)
+    for (int c = r; c < 9; ++c) a[sym9(r, c)] = (T)A[r + 9 * c];
+  const int k = psd_correction_low(a, (T)tol, [&](int j, const T* col) {
+    for (int r = 0; r < 9; ++r) w[9 * j + r] = col[r];
+  });
+  for (int i = 0; i < 9 * (k <= 3 ? k : 0); ++i) W[i] = (double)w[i];
+  return k;
+}
 extern "C" {
 void eh_blocks(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1, int ref,
                double* K) {
@@ -149,6 +176,9 @@ void eh_grad(int model, int fp32, const double* phi12, const double* x12, double
 int eh_psd_blocks(int model, int fp32, const double* phi12, const double* x12, double vol, double p0, double p1,
                   int ref, double tol, double* K) {
   DISPATCH(psd_blocks, phi12, x12, vol, p0, p1, ref, tol, K)
+}
+int eh_psd_low9(int fp32, const double* A, double tol, double* W) {
+  return fp32 ? psd_low9<float>(A, tol, W) : psd_low9<double>(A, tol, W);
 }
 void eh_eig9(int fp32, const double* A, double* d, double* Z) {
   if (fp32)

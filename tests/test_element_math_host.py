@@ -32,6 +32,8 @@ def eh(tmp_path_factory):
     L.eh_psd_blocks.argtypes = [i, i, vp, vp, d, d, d, i, d, vp]
     L.eh_psd_blocks.restype = i
     L.eh_eig9.argtypes = [i, vp, vp, vp]
+    L.eh_psd_low9.argtypes = [i, vp, d, vp]
+    L.eh_psd_low9.restype = i
     L.eh_grad_g.argtypes = [i, vp, vp, vp, d, d, d, vp]
     return L
 
@@ -116,9 +118,10 @@ def test_fp32_energy_keeps_relative_accuracy_near_rest(eh, oracle):
 
 @pytest.mark.parametrize("model", [0, 1, 2, 4])
 def test_psd_decomposition_matches_12x12_eigensolve(eh, oracle, model):
-    """simple_psd_fix through the reduced 9x9 (QR of Phi, Cholesky test, tridiagonal-QL clamp, stored
-    as k <= 3 low-rank columns or as the packed 9x9) equals the oracle's 12x12 Jacobi eigen-clamp,
-    both on tets that need clamping and on ones that do not; the large tolerance forces k > 3"""
+    """simple_psd_fix through the reduced 9x9 (QR of Phi, Cholesky test, the clamped record M' by
+    inverse iteration for k <= 3 eigenvalues below tol and by the full tridiagonal-QL eigensolver for
+    k > 3, blocks from M' and Q) equals the oracle's 12x12 Jacobi eigen-clamp, both on tets that need
+    clamping and on ones that do not, and is exactly symmetric; the large tolerance forces k > 3"""
     rng = np.random.default_rng(11 + model)
     V, E = meshgen.beam_grid(5, 4, 4, h=0.02)
     formats = set()
@@ -141,6 +144,7 @@ def test_psd_decomposition_matches_12x12_eigensolve(eh, oracle, model):
                                        K.ctypes.data)
                 formats.add(fmt)
                 K = K.reshape(12, 12, order="F")
+                assert np.array_equal(K, K.T)  # blocks (m, a) are the exact transposes of (a, m)
                 err = np.abs(K - K_or).max() / np.abs(K_or).max()
                 assert err < bound, (model, tol, fp32, fmt, err)
     assert 4 in formats and formats & {1, 2, 3}, formats
@@ -188,6 +192,60 @@ def test_sym9_eigen_against_closed_forms_and_eigh(eh):
         d32, Z32 = _eig9(eh, A, fp32=1)
         assert np.abs(np.sort(d32) - np.linalg.eigvalsh(A)).max() <= 3e-6 * nA + 1e-30
         assert np.abs(Z32.T @ Z32 - np.eye(9)).max() < 3e-6
+
+
+def test_psd_low_rank_by_inverse_iteration(eh):
+    """the clamp correction for k <= 3 eigenvalues below tol without the eigenvector matrix (QL
+    eigenvalues, inverse iteration on the tridiagonal form, reflectors applied back): W W^T equals
+    sum (tol - lambda) v v^T from numpy's eigh, on isolated, clustered (1e-9 and exactly repeated),
+    zero and wide-range low eigenvalues and on diagonal and block-diagonal inputs (split tridiagonal
+    forms); for k > 3 it declines"""
+    rng = np.random.default_rng(17)
+    cases = []
+    for trial in range(300):
+        Q, _ = np.linalg.qr(rng.standard_normal((9, 9)))
+        hi = rng.uniform(1.0, 1e3, 9)
+        kind = trial % 6
+        k = 1 + trial % 3
+        lam = hi.copy()
+        if kind == 0:
+            lam[:k] = rng.uniform(-1.0, 0.9, k)  # isolated
+        elif kind == 1:
+            lam[:k] = 0.3 + 1e-9 * rng.standard_normal(k)  # tight cluster
+        elif kind == 2:
+            lam[:k] = 0.0  # exactly repeated (rigid-rotation-like null modes)
+        elif kind == 3:
+            lam[:k] = -10 ** rng.uniform(-8, 3, k)  # wide range
+        elif kind == 4:
+            lam[:k] = rng.uniform(-1.0, 0.9, k)
+            lam[k] = 1.0 + 1e-9  # a partner just above tol
+        else:
+            lam[:k] = rng.uniform(-1.0, 0.9, k)
+            Q = np.eye(9)[rng.permutation(9)]  # diagonal input: every off-diagonal of T is zero
+        cases.append(((Q * lam) @ Q.T, 1.0))
+    B = np.zeros((9, 9))
+    B[:3, :3] = [[2, 1, 0], [1, 2, 0], [0, 0, 5]]
+    B[3:, 3:] = np.diag([-1.0, 9.0, 7.0, 7.0, 7.0, 0.5])
+    cases.append((B, 1.5))  # three below tol across decoupled blocks (eigenvalues 1, -1, 0.5)
+    for A, tol in cases:
+        A = 0.5 * (A + A.T)
+        w, V = np.linalg.eigh(A)
+        sel = w < tol
+        dM = (V[:, sel] * (tol - w[sel])) @ V[:, sel].T
+        nA = np.abs(A).max()
+        for fp32, bound in ((0, 2e-14), (1, 2e-5)):
+            W = np.zeros(27)
+            k = eh.eh_psd_low9(fp32, np.asfortranarray(A).ctypes.data, tol, W.ctypes.data)
+            if fp32:  # an eigenvalue within FP32 rounding of tol may count either way
+                if k > 3:
+                    continue
+            else:
+                assert k == sel.sum()
+            W = W.reshape(3, 9)[:k].T
+            err = np.abs(W @ W.T - dM).max() / nA
+            assert err < bound, (fp32, k, err)
+    A = np.diag(np.r_[np.zeros(4), np.ones(5) * 5.0])
+    assert eh.eh_psd_low9(0, np.asfortranarray(A).ctypes.data, 1.0, np.zeros(27).ctypes.data) == 4
 
 
 @pytest.mark.parametrize("model", [0, 1])

@@ -239,8 +239,45 @@ def test_determinism(tm):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("psd", [False, True])
+def test_assembled_hessian_exactly_symmetric(tm, fp32, psd):
+    """with the PSD projection the assembled matrix is symmetric to the last bit, FP64 and FP32:
+    block (n, v) is the exact transpose of block (v, n) (the precision form's implied diagonal and
+    the split operator both rely on it); the element-math gathers without it are symmetric to
+    rounding"""
+    import torch
+
+    p = meshgen.config1(grid=(24, 8, 8))
+    mesh, dX, vol = make_mesh(tm, p.V, p.E)
+    q = torch.tensor(strained(p.V, 5), device="cuda")
+    params = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
+    if fp32:
+        params = params.to(torch.float32)
+    import ctypes
+
+    from bartels_amd import _lib
+
+    lib = _lib.load()
+    for lanes in (1, 2, 4):
+        assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, 0))) == 0
+        H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
+        fn = mesh.dq2_f32_device if fp32 else mesh.dq2_device
+        fn(0, q, params, H, callback=tm.simple_psd_fix(1e-3) if psd else None)
+        torch.cuda.synchronize()
+        A = csr_from_mesh(mesh, H.cpu().numpy().astype(np.float64))
+        D = (A - A.T).tocoo()
+        asym = np.abs(D.data).max(initial=0.0) / np.abs(A.data).max()
+        print(f"fp32={fp32} psd={psd} lanes={lanes}: max |H - H^T| / max |H| = {asym:.3e}")
+        if psd:
+            assert asym == 0.0
+        else:
+            assert asym < (1e-6 if fp32 else 1e-14)
+    assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(4, 0))) == 0
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 def test_gather_variants_agree(tm, fp32):
-    """every kernel schedule -- 1 or 2 lanes per vertex, XCD grouping on/off -- keeps each slot's
+    """every kernel schedule -- 1, 2 or 4 lanes per vertex, XCD grouping on/off -- keeps each slot's
     ascending-element fold and the same element math: same results (to rounding; printed whether
     bitwise)"""
     import ctypes
@@ -257,7 +294,7 @@ def test_gather_variants_agree(tm, fp32):
         params = params.to(torch.float32)
     lib = _lib.load()
     outs = {}
-    for lanes in (1, 2):
+    for lanes in (1, 2, 4):
         for xcd in (0, 1):
             assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, xcd))) == 0
             H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
