@@ -191,66 +191,10 @@ __global__ void __launch_bounds__(3 * kJacVerts)
 inline int jac_blocks(int nV) { return std::max(1, (nV + kJacVerts - 1) / kJacVerts); }
 inline int vec_blocks(int nV) { return std::max(1, std::min((nV + kRedBlock - 1) / kRedBlock, 2048)); }
 
-// first stage of the CG reduction: kRed1Blocks blocks sum fixed slices of the three regions
-__global__ void __launch_bounds__(1024)
-    k_cg_reduce1(const double* __restrict__ pru, const double* __restrict__ prr, const double* __restrict__ pwu,
-                 int nru, int nrr, int nwu, double* __restrict__ out) {
-  __shared__ double sh[16];
-  const int cnt[3] = {nru, nrr, nwu};
-  const double* srcs[3] = {pru, prr, pwu};
-#pragma unroll
-  for (int g = 0; g < 3; ++g) {
-    const double* src = srcs[g];
-    const int per = (cnt[g] + kRed1Blocks - 1) / kRed1Blocks;
-    const int lo = blockIdx.x * per, hi = min(cnt[g], lo + per);
-    double a = 0.0;
-    for (int k = lo + threadIdx.x; k < hi; k += 1024) a += src[k];
-    a = block_sum<1024>(a, sh);
-    if (threadIdx.x == 0) out[g * kRed1Blocks + blockIdx.x] = a;
-  }
-}
-
-// start of a factor-preconditioned solve: x = 0, r = b, p = s = 0, partial r.r
-__global__ void __launch_bounds__(kRedBlock)
-    k_cg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r,
-              double* __restrict__ p, double* __restrict__ s, double* __restrict__ part_rr) {
-  __shared__ double sh[4];
-  double rr = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
-    const double bi = b[i];
-    x[i] = 0.0;
-    p[i] = 0.0;
-    s[i] = 0.0;
-    r[i] = bi;
-    rr += bi * bi;
-  }
-  rr = block_sum_256(rr, sh);
-  if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
-}
-
-// Eisenstat-form restart: p~ = s~ = 0
-__global__ void __launch_bounds__(kRedBlock) k_eis_restart(int64_t n, double* __restrict__ pt,
-                                                          double* __restrict__ st) {
-  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
-    pt[i] = 0.0;
-    st[i] = 0.0;
-  }
-}
-
-// Eisenstat-form start: x~ = 0, p~ = s~ = 0
-__global__ void __launch_bounds__(kRedBlock) k_eis_init(int64_t n, double* __restrict__ xt, double* __restrict__ pt,
-                                                       double* __restrict__ st) {
-  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
-    xt[i] = 0.0;
-    pt[i] = 0.0;
-    st[i] = 0.0;
-  }
-}
-
-// second stage of the reduction (kRed1Blocks partials per quantity) and the CG scalar step
-__global__ void __launch_bounds__(1024)
-    k_cg_scalars(const double* __restrict__ part, int start, double tol, int maxit, PcgState* __restrict__ st) {
-  __shared__ double sh[16];
+// second stage of the reduction (kRed1Blocks partials per quantity) and the CG scalar step, run by
+// one workgroup of 1024 threads
+__device__ void cg_scalar_step(const double* __restrict__ part, int start, double tol, int maxit,
+                               PcgState* __restrict__ st, double* sh) {
   if (!start && st->done) {
     // a non-positive-curvature exit in the first iteration applied its alpha = 1 update: count it once
     if (st->done == 3 && threadIdx.x == 0 && st->alpha != 0.0) {
@@ -309,6 +253,75 @@ __global__ void __launch_bounds__(1024)
     } else {
       st->alpha = g / den;
     }
+  }
+}
+
+// first stage of the CG reduction: kRed1Blocks blocks sum fixed slices of the three regions
+__global__ void __launch_bounds__(1024)
+    k_cg_reduce1(const double* __restrict__ pru, const double* __restrict__ prr, const double* __restrict__ pwu,
+                 int nru, int nrr, int nwu, double* __restrict__ out, int* __restrict__ arrived, int start,
+                 double tol, int maxit, PcgState* __restrict__ st) {
+  __shared__ double sh[16];
+  __shared__ int last;
+  const int cnt[3] = {nru, nrr, nwu};
+  const double* srcs[3] = {pru, prr, pwu};
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    const double* src = srcs[g];
+    const int per = (cnt[g] + kRed1Blocks - 1) / kRed1Blocks;
+    const int lo = blockIdx.x * per, hi = min(cnt[g], lo + per);
+    double a = 0.0;
+    for (int k = lo + threadIdx.x; k < hi; k += 1024) a += src[k];
+    a = block_sum<1024>(a, sh);
+    if (threadIdx.x == 0) out[g * kRed1Blocks + blockIdx.x] = a;
+  }
+  // the workgroup that arrives last runs the second stage and the scalar step (its partials are
+  // summed in workgroup order, so the result does not depend on the arrival order)
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(arrived, 1) == kRed1Blocks - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  cg_scalar_step(out, start, tol, maxit, st, sh);
+  if (threadIdx.x == 0) *arrived = 0;
+}
+
+// start of a factor-preconditioned solve: x = 0, r = b, p = s = 0, partial r.r
+__global__ void __launch_bounds__(kRedBlock)
+    k_cg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r,
+              double* __restrict__ p, double* __restrict__ s, double* __restrict__ part_rr) {
+  __shared__ double sh[4];
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    p[i] = 0.0;
+    s[i] = 0.0;
+    r[i] = bi;
+    rr += bi * bi;
+  }
+  rr = block_sum_256(rr, sh);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+}
+
+// Eisenstat-form restart: p~ = s~ = 0
+__global__ void __launch_bounds__(kRedBlock) k_eis_restart(int64_t n, double* __restrict__ pt,
+                                                          double* __restrict__ st) {
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    pt[i] = 0.0;
+    st[i] = 0.0;
+  }
+}
+
+// Eisenstat-form start: x~ = 0, p~ = s~ = 0
+__global__ void __launch_bounds__(kRedBlock) k_eis_init(int64_t n, double* __restrict__ xt, double* __restrict__ pt,
+                                                       double* __restrict__ st) {
+  for (int64_t i = blockIdx.x * (int64_t)kRedBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRedBlock) {
+    xt[i] = 0.0;
+    pt[i] = 0.0;
+    st[i] = 0.0;
   }
 }
 
@@ -381,6 +394,7 @@ struct NewtonWs {
   float* vals32;    // precision form: FP32 Hessian values
   float* params32;  // precision form: FP32 parameters (2 nT)
   PcgState* st;
+  int* arrived;     // workgroups of the reduction done so far (zero between reductions)
 };
 
 int newton_workspace(bh_mesh* m, NewtonWs& w) {
@@ -388,11 +402,13 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   const int64_t nParts = 3 * part_region(m->nV) + 3 * kRed1Blocks + 16;
   const size_t need = sizeof(double) * ((size_t)m->nnz + 9 * (size_t)n + 18 * (size_t)m->nV + nParts + 16) +
                       sizeof(float) * ((size_t)m->nnz + 2 * (size_t)m->nT) + sizeof(PcgState) + 512;
+  bool fresh = false;
   if (m->wsBytes < need) {
     if (m->ws) BH_CHECK(hipFree(m->ws));
     m->ws = nullptr;
     BH_CHECK(hipMalloc(&m->ws, need));
     m->wsBytes = need;
+    fresh = true;
   }
   double* q = m->ws;
   w.vals = q; q += m->nnz;
@@ -413,6 +429,8 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   w.vals32 = f; f += m->nnz;
   w.params32 = f; f += 2ll * m->nT;
   w.st = reinterpret_cast<PcgState*>(reinterpret_cast<uintptr_t>(f + 64) & ~uintptr_t(63));
+  w.arrived = reinterpret_cast<int*>(w.st + 1);
+  if (fresh) BH_CHECK(hipMemset(w.arrived, 0, sizeof(int)));
   return BH_OK;
 }
 
@@ -455,8 +473,8 @@ inline void cg_reduce_regions(const NewtonWs& w, int nV, const double* pru, int 
                               const double* pwu, int nwu, int start, double tol, int maxit, hipStream_t s) {
   const int64_t R = part_region(nV);
   double* stage2 = w.part + 3 * R;
-  hipLaunchKernelGGL(k_cg_reduce1, dim3(kRed1Blocks), dim3(1024), 0, s, pru, prr, pwu, nru, nrr, nwu, stage2);
-  hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(1024), 0, s, stage2, start, tol, maxit, w.st);
+  hipLaunchKernelGGL(k_cg_reduce1, dim3(kRed1Blocks), dim3(1024), 0, s, pru, prr, pwu, nru, nrr, nwu, stage2,
+                     w.arrived, start, tol, maxit, w.st);
 }
 inline void cg_reduce(const NewtonWs& w, int nV, int nru, int nrr, int nwu, int start, const bh_newton_opts* o,
                       hipStream_t s) {
@@ -536,7 +554,7 @@ int pcg_solve_split_jacobi(bh_mesh* m, const bh_newton_opts* o, const V* vals, c
   hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 1, b, w.Dinv, x, r, u, p, sv, wv, pru,
                      prr, w.st);
   hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl, P->nuptr,
-                     P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                     P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                      (const PcgState*)nullptr);
   cg_reduce(w, nV, gj, gj, gsp, 1, o, s);
   PcgState hs{};
@@ -545,7 +563,7 @@ int pcg_solve_split_jacobi(bh_mesh* m, const bh_newton_opts* o, const V* vals, c
       hipLaunchKernelGGL(k_cg_vec_jacobi, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, 0, b, w.Dinv, x, r, u, p, sv, wv,
                          pru, prr, w.st);
       hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
-                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                         P->nuptr, P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                          w.st);
       cg_reduce(w, nV, gj, gj, gsp, 0, o, s);
     }
@@ -608,9 +626,9 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
   const double tol = o->pcg_tol;
   hipLaunchKernelGGL(k_eis_init, dim3(gi), dim3(kRedBlock), 0, s, n, v.xt, v.pt, v.st);
   int ng = 0, nd = 0;
-  eis_fwd<V, double>(P, 0, v, b, v.rt, nullptr, nullptr, s);      // r~ = P^{-1} b
-  eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s, &ng);  // u = Dt r~, t = P^{-T} u
-  eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s, &nd);  // w~ = A~ u
+  eis_fwd<V, double>(P, 0, v, b, v.rt, nullptr, nullptr, s);           // r~ = P^{-1} b
+  eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s, &ng, pwu);  // u = Dt r~, t = P^{-T} u
+  eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s, &nd, true);  // w~ = A~ u
   cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 1, 0.0, o->pcg_max_iterations, s);
   BH_CHECK(hipGetLastError());
   PcgState hs{};
@@ -620,8 +638,8 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
   double prevRr = -1.0;                        // true residual at the previous check
   while (!converged) {
     for (int k = 0; k < kPcgBatch; ++k) {
-      eis_bwd<V, double>(P, 2, v, nullptr, v.t, pru, w.st, s);
-      eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, w.st, s);
+      eis_bwd<V, double>(P, 2, v, nullptr, v.t, pru, w.st, s, nullptr, pwu);
+      eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, w.st, s, nullptr, true);
       cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 0, 0.0, o->pcg_max_iterations - itBase, s);
     }
     if ((st = cg_wait(w, &hs, s))) return st;
@@ -632,7 +650,7 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
       eis_bwd<V, double>(P, 0, v, v.xt, xs, nullptr, nullptr, s);
       double* r = svec(P, SV_Z);
       hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
-                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)xs, r,
+                         P->nuptr, P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)xs, r,
                          (double*)nullptr, (const PcgState*)nullptr);
       hipLaunchKernelGGL(k_resid, dim3(gr), dim3(256), 0, s, n, (const double*)b, r);
       double rr = 0.0;
@@ -649,8 +667,8 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
         itBase += hs.iters;
         eis_fwd<V, double>(P, 0, v, r, v.rt, nullptr, nullptr, s);
         hipLaunchKernelGGL(k_eis_restart, dim3(gi), dim3(kRedBlock), 0, s, n, v.pt, v.st);
-        eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s);
-        eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s);
+        eis_bwd<V, double>(P, 1, v, nullptr, v.t, pru, nullptr, s, nullptr, pwu);
+        eis_fwd<V, double>(P, 1, v, nullptr, v.w, pwu, nullptr, s, nullptr, true);
         cg_reduce_regions(w, nV, pru, ng, pru, ng, pwu, nd, 1, 0.0, o->pcg_max_iterations - itBase, s);
         BH_CHECK(hipGetLastError());
         rho = rr / bb;  // at the restart gamma / gamma0 = 1
@@ -697,7 +715,7 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const d
   CgVecs start{x, p, sv, u, wv, nullptr, nullptr};
   if ((st = precond_apply_solve(P, r, u, s, &start, 0, pru, np))) return st;
   hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl, P->nuptr,
-                     P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                     P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                      (const PcgState*)nullptr);
   cg_reduce(w, nV, np[1], gi, gsp, 1, o, s);
   CgVecs loop{x, p, sv, u, wv, prr, w.st};
@@ -706,7 +724,7 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const d
     for (int k = 0; k < kPcgBatch; ++k) {
       if ((st = precond_apply_solve(P, r, u, s, &loop, 1, pru, np))) return st;
       hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
-                         P->nuptr, P->nucol, (const V*)h.hu, (const double*)P->hdiag, (const double*)u, wv, pwu,
+                         P->nuptr, P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                          w.st);
       cg_reduce(w, nV, np[1], np[0], gsp, 0, o, s);
     }

@@ -191,6 +191,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// the split Hessian's blocks are stored component-major: component q of block k at h[q * ld + k]
+// (ld = number of lower blocks), so the lanes of a wave, which walk consecutive blocks, read each
+// component as one contiguous run
+template <typename V>
+__device__ __forceinline__ void load_block(const V* __restrict__ h, int64_t ld, int64_t k, double B[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) B[q] = (double)h[q * ld + k];
+}
+
 // Ap = H p in the solve order from the split Hessian (lower blocks, the transposed copy of the lower
 // blocks for the upper part, diagonal blocks), partial p.Ap.  8 lanes per position over the joined
 // lower + upper list; lane 0 adds the diagonal block.  The masked system is symmetric by construction.
@@ -198,7 +207,7 @@ template <typename V>
 __global__ void __launch_bounds__(256)
     k_split_spmv(int nV, const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlcol,
                  const V* __restrict__ hl, const int32_t* __restrict__ nuptr, const int32_t* __restrict__ nucol,
-                 const V* __restrict__ hu, const double* __restrict__ hd, const double* __restrict__ p,
+                 const V* __restrict__ hu, int64_t ld, const double* __restrict__ hd, const double* __restrict__ p,
                  double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
@@ -210,7 +219,8 @@ __global__ void __launch_bounds__(256)
     const int u0 = nuptr[i], nt = nl + nuptr[i + 1] - u0;
     for (int t = j; t < nt; t += kSweepLanes) {
       if (t < nl) {
-        const V* B = hl + 9ll * (l0 + t);
+        double B[9];
+        load_block(hl, ld, l0 + t, B);
         const double* xp = p + 3ll * nlcol[l0 + t];
         const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
         s0 += (double)B[0] * x0 + (double)B[1] * x1 + (double)B[2] * x2;
@@ -218,7 +228,8 @@ __global__ void __launch_bounds__(256)
         s2 += (double)B[6] * x0 + (double)B[7] * x1 + (double)B[8] * x2;
       } else {
         const int k = u0 + t - nl;
-        const V* B = hu + 9ll * k;  // H_ni: use its transpose
+        double B[9];
+        load_block(hu, ld, k, B);  // H_ni: use its transpose
         const double* xp = p + 3ll * nucol[k];
         const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
         s0 += (double)B[0] * x0 + (double)B[3] * x1 + (double)B[6] * x2;
@@ -321,9 +332,9 @@ __device__ __forceinline__ void mv3d(const T* A, double x0, double x1, double x2
 template <typename V, int LANES, typename TT>
 __global__ void __launch_bounds__(256)
     k_eis_bwd(int c0, int n, const int32_t* __restrict__ nuptr, const int32_t* __restrict__ nucol,
-              const V* __restrict__ hu, const float* __restrict__ dt, const float* __restrict__ dtinv, int mode,
+              const V* __restrict__ hu, int64_t ld, const float* __restrict__ dt, const float* __restrict__ dtinv, int mode,
               EisVecs v, const double* __restrict__ src, TT* __restrict__ tb, double* __restrict__ part,
-              const PcgState* __restrict__ st) {
+              const PcgState* __restrict__ st, const double* __restrict__ kk, double* __restrict__ fpart) {
   static_assert(LANES >= 3, "lanes 0..2 carry the three rows of a position");
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
@@ -353,7 +364,8 @@ __global__ void __launch_bounds__(256)
       }
     }
     for (int k = nuptr[i] + j; k < nuptr[i + 1]; k += LANES) {
-      const V* B = hu + 9ll * k;  // H_ni; the product needs H_in = H_ni^T
+      double B[9];
+      load_block(hu, ld, k, B);  // H_ni; the product needs H_in = H_ni^T
       const TT* tn = tb + 3ll * nucol[k];
       const double t0 = tn[0], t1 = tn[1], t2 = tn[2];
       s0 += (double)B[0] * t0 + (double)B[3] * t1 + (double)B[6] * t2;
@@ -369,7 +381,7 @@ __global__ void __launch_bounds__(256)
   }
   const int lb = (int)(threadIdx.x & 63) - j;
   const double a0 = __shfl(aj, lb, 64), a1 = __shfl(aj, lb + 1, 64), a2 = __shfl(aj, lb + 2, 64);
-  double g = 0.0;
+  double g = 0.0, fd = 0.0;
   if (li < n && j == 0) {
     double rhs[3] = {a0, a1, a2};
     if (mode >= 1) {
@@ -387,10 +399,28 @@ __global__ void __launch_bounds__(256)
     tb[3ll * i] = (TT)tt[0];
     tb[3ll * i + 1] = (TT)tt[1];
     tb[3ll * i + 2] = (TT)tt[2];
+    if (kk) {
+      // the forward sweep's step on this colour when it is the first of the solve order (no lower
+      // blocks): w = D~^{-1} (u - K t), w~ = t + w, partial w~.u -- what k_eis_fwd mode 1 computes
+      double kt[3], ww[3];
+      mv3d(kk + 9ll * i, (double)(TT)tt[0], (double)(TT)tt[1], (double)(TT)tt[2], kt);
+      mv3d(dtinv + 9ll * i, rhs[0] - kt[0], rhs[1] - kt[1], rhs[2] - kt[2], ww);
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        v.w[3ll * i + l] = ww[l];
+        const double wt = (double)(TT)tt[l] + ww[l];
+        v.wt[3ll * i + l] = wt;
+        fd += wt * rhs[l];
+      }
+    }
   }
   if (part) {
     const double d = block_sum_256(g, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
+  if (fpart) {
+    const double d = block_sum_256(fd, sh);
+    if (threadIdx.x == 0) fpart[blockIdx.x] = d;
   }
 }
 
@@ -401,7 +431,7 @@ __global__ void __launch_bounds__(256)
 template <typename V, int LANES, typename TT>
 __global__ void __launch_bounds__(256)
     k_eis_fwd(int c0, int n, const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlcol,
-              const V* __restrict__ hl, const float* __restrict__ dtinv, const double* __restrict__ kk, int mode,
+              const V* __restrict__ hl, int64_t ld, const float* __restrict__ dtinv, const double* __restrict__ kk, int mode,
               EisVecs v, const double* __restrict__ src, TT* __restrict__ db, double* __restrict__ part,
               const PcgState* __restrict__ st) {
   static_assert(LANES >= 3, "lanes 0..2 carry the three rows of a position");
@@ -413,7 +443,8 @@ __global__ void __launch_bounds__(256)
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (li < n) {
     for (int k = nlptr[i] + j; k < nlptr[i + 1]; k += LANES) {
-      const V* B = hl + 9ll * k;  // H_in
+      double B[9];
+      load_block(hl, ld, k, B);  // H_in
       const TT* wn = db + 3ll * nlcol[k];
       const double w0 = wn[0], w1 = wn[1], w2 = wn[2];
       s0 += (double)B[0] * w0 + (double)B[1] * w1 + (double)B[2] * w2;
@@ -473,14 +504,15 @@ __global__ void __launch_bounds__(256) k_resid(int64_t n, const double* __restri
 }
 
 // dst block t = src block link[t] (9 values each)
-template <typename S, typename D>
-__global__ void __launch_bounds__(256) k_copy_blocks(int64_t nb, const int32_t* __restrict__ link,
-                                                     const S* __restrict__ src, D* __restrict__ dst) {
+// the upper list's blocks: hu[., t] = hl[., link[t]] (the transposes are taken by the readers)
+template <typename V>
+__global__ void __launch_bounds__(256) k_copy_blocks(int64_t nb, int64_t ld, const int32_t* __restrict__ link,
+                                                     const V* __restrict__ src, V* __restrict__ dst) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nb) return;
-  const S* b = src + 9ll * link[t];
+  const int64_t k = link[t];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) dst[9 * t + i] = (D)b[i];
+  for (int q = 0; q < 9; ++q) dst[q * ld + t] = src[q * ld + k];
 }
 
 // the factor in the solve order, one thread per position i (vertex colVerts[i]): the 6 packed inverse
@@ -572,7 +604,7 @@ __global__ void __launch_bounds__(256)
                     const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const int32_t* __restrict__ dslot,
                     const int32_t* __restrict__ nlptr, const int32_t* __restrict__ nlsrc,
                     const int32_t* __restrict__ lslot, const int32_t* __restrict__ lcol,
-                    const double* __restrict__ hdiag, V* __restrict__ hl, double* __restrict__ hd) {
+                    const double* __restrict__ hdiag, int64_t ld, V* __restrict__ hl, double* __restrict__ hd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nV) return;
   const int v = colVerts[i];
@@ -586,7 +618,7 @@ __global__ void __launch_bounds__(256)
     const int k = nlsrc[t];
     masked_block(vals, bptr, v, lslot[k], fixed, lcol[k], A);
 #pragma unroll
-    for (int e = 0; e < 9; ++e) hl[9ll * t + e] = (V)A[e];
+    for (int e = 0; e < 9; ++e) hl[e * ld + t] = (V)A[e];
   }
 }
 
@@ -843,12 +875,13 @@ inline double* svec(const bh_precond* P, int k) { return P->sv + (int64_t)k * 3 
 template <typename V>
 struct SplitH {
   V *hl, *hu;
+  int64_t ld;  // component stride: component q of block k at h[q * ld + k]
 };
 template <typename V>
 SplitH<V> split_view(const bh_precond* P) {
   V* base = reinterpret_cast<V*>(P->hsplit);
-  const int64_t nl = 9 * std::max<int64_t>(1, P->nlower);
-  return SplitH<V>{base, base + nl};
+  const int64_t ld = std::max<int64_t>(1, P->nlower);
+  return SplitH<V>{base, base + 9 * ld, ld};
 }
 
 // after every change of lval / dinv: the factor in the solve order (both list orders)
@@ -870,10 +903,10 @@ int precond_split_hessian(const bh_precond* P, const V* vals, const double* hdia
   SplitH<V> h = split_view<V>(P);
   if (nV)
     hipLaunchKernelGGL(k_split_hessian<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, P->colVerts, m->bptr, vals,
-                       fixed, P->dslot, P->nlptr, P->nlsrc, P->lslot, P->lcol, hdiag, h.hl, P->hdiag);
+                       fixed, P->dslot, P->nlptr, P->nlsrc, P->lslot, P->lcol, hdiag, h.ld, h.hl, P->hdiag);
   if (P->nlower)
-    hipLaunchKernelGGL((k_copy_blocks<V, V>), dim3((P->nlower + 255) / 256), dim3(256), 0, s, P->nlower, P->nulink,
-                       (const V*)h.hl, h.hu);
+    hipLaunchKernelGGL((k_copy_blocks<V>), dim3((P->nlower + 255) / 256), dim3(256), 0, s, P->nlower, h.ld,
+                       P->nulink, (const V*)h.hl, h.hu);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -936,9 +969,12 @@ inline int precond_sgs(bh_precond* P, double scale, hipStream_t s) {
 
 // Eisenstat sweeps over all colours (tb / db: output and gather buffer; double for mode 0, v.t / v.w
 // otherwise); nparts (optional) receives the partial count
+// fpart non-null (modes >= 1, tb = v.t): the last launch (the solve order's first colour) also takes
+// the forward sweep's step on that colour, writing its dot-product partials to fpart; eis_fwd then
+// starts at the second colour (first_done)
 template <typename V, typename TT>
 void eis_bwd(const bh_precond* P, int mode, const EisVecs& v, const double* src, TT* tb, double* part,
-             const PcgState* st, hipStream_t s, int* nparts = nullptr) {
+             const PcgState* st, hipStream_t s, int* nparts = nullptr, double* fpart = nullptr) {
   const SplitH<V> h = split_view<V>(P);
   const int lanes = P->eisLanes, vpb = 256 / lanes;
   int off = 0;
@@ -946,21 +982,24 @@ void eis_bwd(const bh_precond* P, int mode, const EisVecs& v, const double* src,
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
     const int g = (n + vpb - 1) / vpb;
+    const bool fuse = fpart && c == 0;
+    const double* kk = fuse ? (const double*)P->kk : nullptr;
+    double* fp = fuse ? fpart : nullptr;
     if (lanes == 4)
       hipLaunchKernelGGL((k_eis_bwd<V, 4, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol,
-                         (const V*)h.hu, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
-                         part ? part + off : nullptr, st);
+                         (const V*)h.hu, h.ld, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
+                         part ? part + off : nullptr, st, kk, fp);
     else
       hipLaunchKernelGGL((k_eis_bwd<V, 8, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol,
-                         (const V*)h.hu, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
-                         part ? part + off : nullptr, st);
+                         (const V*)h.hu, h.ld, (const float*)P->dt, (const float*)P->dtinv, mode, v, src, tb,
+                         part ? part + off : nullptr, st, kk, fp);
     off += g;
   }
   if (nparts) *nparts = off;
 }
 template <typename V, typename TT>
 void eis_fwd(const bh_precond* P, int mode, const EisVecs& v, const double* src, TT* db, double* part,
-             const PcgState* st, hipStream_t s, int* nparts = nullptr) {
+             const PcgState* st, hipStream_t s, int* nparts = nullptr, bool first_done = false) {
   const SplitH<V> h = split_view<V>(P);
   const int lanes = P->eisLanes, vpb = 256 / lanes;
   int off = 0;
@@ -968,13 +1007,17 @@ void eis_fwd(const bh_precond* P, int mode, const EisVecs& v, const double* src,
     const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
     if (!n) continue;
     const int g = (n + vpb - 1) / vpb;
+    if (c == 0 && first_done) {  // taken by the backward sweep's last launch
+      off += g;
+      continue;
+    }
     if (lanes == 4)
       hipLaunchKernelGGL((k_eis_fwd<V, 4, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol,
-                         (const V*)h.hl, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
+                         (const V*)h.hl, h.ld, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
                          part ? part + off : nullptr, st);
     else
       hipLaunchKernelGGL((k_eis_fwd<V, 8, TT>), dim3(g), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol,
-                         (const V*)h.hl, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
+                         (const V*)h.hl, h.ld, (const float*)P->dtinv, (const double*)P->kk, mode, v, src, db,
                          part ? part + off : nullptr, st);
     off += g;
   }
