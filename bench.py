@@ -299,7 +299,7 @@ def main():
             "grid": list(grid), "tets": nT, "vertices": nV, "nnz": nnz,
         },
         "roofline": {
-            "kernel": "k_hessian (assembly gather, PSD step)",
+            "kernel": "k_hessian_rec (assembly gather from the PSD records)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
