@@ -690,11 +690,12 @@ __device__ __forceinline__ void count_flagged(bool flagged, int32_t* count) {
 // tol); never left in flag once both passes have run
 constexpr int kPsdPending = 1;
 
-// First pass: M for every tet, stored as its record; for a tet that fails the Cholesky test with at
-// most three eigenvalues below tol (the common case: the rotational modes of a mildly stressed tet)
-// the FP64 record becomes M + W W^T, W's columns from inverse iteration without an eigenvector
-// matrix, which keeps the pass at half the registers of the full eigensolver; tets with more are
-// marked pending.  The FP32 pass runs the full eigensolver on every flagged tet.  The records of the workgroup's 64 tets are built in LDS (M parked there while the
+// First pass: M for every tet, stored as its record.  FP64: the eigenvalues of M (QL, no vectors)
+// decide; for a tet with at most three below tol (the common case: the rotational modes of a mildly
+// stressed tet) the record becomes M + W W^T, W's columns from inverse iteration without an
+// eigenvector matrix, which keeps the pass at half the registers of the full eigensolver; tets with
+// more are marked pending.  FP32: a Cholesky test of M - tol I, then the full eigensolver on the
+// tets that fail it.  The records of the workgroup's 64 tets are built in LDS (M parked there while the
 // eigensolver overwrites it, each column added as it is formed) and leave as one contiguous,
 // coalesced block.
 constexpr int kRecPitch = 65;  // LDS pitch of one record entry over the 64 lanes (odd: no bank conflicts)
@@ -708,50 +709,49 @@ __global__ void __launch_bounds__(64, 2)
   const int lane = (int)threadIdx.x;
   const int e0 = (int)blockIdx.x * 64;
   const int e = e0 + lane;
-  bool need = false;
-  T M[45];
+  bool clamped = false;
   if (e < m.nT) {
+    T M[45];
     reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
 #pragma unroll
     for (int i = 0; i < 45; ++i) stash[i * kRecPitch + lane] = M[i];
-    need = !shifted_pd(M, tol);
-  }
-  count_flagged(need, count);
-  if (e < m.nT) {
     int fl = 0;
-    if (need) {
-      if constexpr (sizeof(T) == 8) {
-        const int k = psd_correction_low(M, tol, [&](int, const T* w) {
-#pragma unroll
-          for (int i = 0; i < 9; ++i)
-#pragma unroll
-            for (int j = i; j < 9; ++j) stash[sym9(i, j) * kRecPitch + lane] += w[i] * w[j];
-        });
-        if (k > 3) fl = kPsdPending;
-      } else {
-        // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
-        // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
-        // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
-        T dm[45];
-        const int k = psd_correction(M, tol, dm);
+    if constexpr (sizeof(T) == 8) {
+      // the QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the
+      // rotational modes put nearly every tet of a mildly stressed mesh below it)
+      const int k = psd_correction_low(M, tol, [&](int, const T* w) {
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
-          for (int j = i; j < 9; ++j) {
-            T acc = stash[sym9(i, j) * kRecPitch + lane];
-            if (k > 3) {
-              acc += dm[sym9(i, j)];
-            } else {
+          for (int j = i; j < 9; ++j) stash[sym9(i, j) * kRecPitch + lane] += w[i] * w[j];
+      });
+      clamped = k > 0;
+      if (k > 3) fl = kPsdPending;
+    } else if (!shifted_pd(M, tol)) {
+      // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
+      // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
+      // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
+      clamped = true;
+      T dm[45];
+      const int k = psd_correction(M, tol, dm);
 #pragma unroll
-              for (int c = 0; c < 3; ++c)
-                if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
-            }
-            stash[sym9(i, j) * kRecPitch + lane] = acc;
+      for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = i; j < 9; ++j) {
+          T acc = stash[sym9(i, j) * kRecPitch + lane];
+          if (k > 3) {
+            acc += dm[sym9(i, j)];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
           }
-      }
+          stash[sym9(i, j) * kRecPitch + lane] = acc;
+        }
     }
     flag[e] = (uint8_t)fl;
   }
+  count_flagged(clamped, count);
   __syncthreads();
   const int n = 45 * min(64, m.nT - e0);
   T* out = rec + 45ll * e0;

@@ -98,10 +98,11 @@ static int psd_blocks(const double* phi12, const double* x12, double vol, double
   reduced_packed<MODEL>(P, R, M);
   for (int i = 0; i < 45; ++i) rec[i] = M[i];
   int k = 0;
-  if (!shifted_pd(M, (T)tol)) {
+  // FP64: the QL eigenvalues decide, inverse iteration for k <= 3, the full eigensolver for k > 3;
+  // FP32: a Cholesky test of M - tol I, then the full eigensolver
+  if (sizeof(T) == 8 || !shifted_pd(M, (T)tol)) {
     T Mc[45];
     for (int i = 0; i < 45; ++i) Mc[i] = M[i];
-    // FP64: inverse iteration for k <= 3, the full eigensolver for k > 3; FP32: the full eigensolver
     k = sizeof(T) == 8 ? psd_correction_low(Mc, (T)tol, [&](int, const T* w) {
       for (int i = 0; i < 9; ++i)
         for (int j = i; j < 9; ++j) rec[sym9(i, j)] += w[i] * w[j];
