@@ -45,9 +45,9 @@ struct bh_precond {
   double* hsplit = nullptr;     // masked Hessian: lower blocks and their transposed copy (FP32 or FP64)
   double* hdiag = nullptr;      // masked diagonal blocks (FP64, 9 per position)
   double* sv = nullptr;         // solve vectors, 3 nV each: b, x, r, p, s, u, w, y, t, v, z
-  // Eisenstat form: the block diagonal Dt and its inverse (FP32, 9 per position) and K = 2 Dt' - D in
-  // FP64, with Dt' the exact inverse of the stored FP32 inverse (the identity A = P + P^T - K that the
-  // sweeps rely on then holds to FP64 rounding)
+  // Eisenstat form: the block diagonal Dt and its inverse (FP32, packed symmetric, 6 per position)
+  // and K = 2 Dt' - D in FP64 (6 per position), with Dt' the exact inverse of the stored FP32
+  // inverse (the identity A = P + P^T - K that the sweeps rely on then holds to FP64 rounding)
   float* dt = nullptr;
   float* dtinv = nullptr;
   double* kk = nullptr;
@@ -273,24 +273,41 @@ __global__ void __launch_bounds__(256)
 // block-diagonal preconditioner (in exact arithmetic PCG with M on A); the residual of A x = b is
 // checked directly (x = P^{-T} x~, r = b - A x) whenever the transformed residual says it may be met.
 
-// Store an Eisenstat diagonal Dt (FP64 in, one position): FP32 Dt and inverse, FP64 K = 2 Dt' - D
+// packed upper triangle of a symmetric 3x3 (s00, s01, s02, s11, s12, s22) times x
+template <typename T>
+__device__ __forceinline__ void mv3s(const T* S, double x0, double x1, double x2, double* y) {
+  const double a = S[0], b = S[1], c = S[2], d = S[3], e = S[4], f = S[5];
+  y[0] = a * x0 + b * x1 + c * x2;
+  y[1] = b * x0 + d * x1 + e * x2;
+  y[2] = c * x0 + e * x1 + f * x2;
+}
+
+// Store an Eisenstat diagonal Dt (FP64 in, symmetric, one position), each as a packed upper triangle:
+// FP32 Dt and its inverse, FP64 K = 2 Dt' - D (Dt' = the inverse of the stored FP32 inverse).  The
+// symmetric packing reads 24 B per FP32 block and 48 B for K instead of 36 and 72.
 __device__ __forceinline__ void eis_store_diag(int64_t i, const double* Dt, const double* __restrict__ hd,
                                                float* __restrict__ dt, float* __restrict__ dtinv,
                                                double* __restrict__ kk) {
+  constexpr int kUp[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
   double A[9], Ai[9], Ae[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) A[e] = Dt[e];
   inv3_block(A, 0, Ai);
+  double As[9];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    dt[9 * i + e] = (float)Dt[e];
-    const float fi = (float)Ai[e];
-    dtinv[9 * i + e] = fi;
-    Ai[e] = (double)fi;
+  for (int k = 0; k < 6; ++k) {
+    const int r = kUp[k][0], c = kUp[k][1];
+    dt[6 * i + k] = (float)(0.5 * (Dt[3 * r + c] + Dt[3 * c + r]));
+    const float fi = (float)(0.5 * (Ai[3 * r + c] + Ai[3 * c + r]));
+    dtinv[6 * i + k] = fi;
+    As[3 * r + c] = As[3 * c + r] = (double)fi;
   }
-  inv3_block(Ai, 0, Ae);  // Dt' = (stored inverse)^{-1}
+  inv3_block(As, 0, Ae);  // Dt' = (stored inverse)^{-1}
 #pragma unroll
-  for (int e = 0; e < 9; ++e) kk[9 * i + e] = 2.0 * Ae[e] - hd[9 * i + e];
+  for (int k = 0; k < 6; ++k) {
+    const int r = kUp[k][0], c = kUp[k][1];
+    kk[6 * i + k] = (Ae[3 * r + c] + Ae[3 * c + r]) - hd[9 * i + 3 * r + c];
+  }
 }
 
 // block symmetric Gauss-Seidel: Dt = D, the masked diagonal blocks.  (A D-ILU diagonal,
@@ -316,12 +333,6 @@ struct EisVecs {
   double *t, *w;                   // backward / forward sweep results of the product (gather sources)
 };
 
-template <typename T>
-__device__ __forceinline__ void mv3d(const T* A, double x0, double x1, double x2, double* y) {
-  y[0] = (double)A[0] * x0 + (double)A[1] * x1 + (double)A[2] * x2;
-  y[1] = (double)A[3] * x0 + (double)A[4] * x1 + (double)A[5] * x2;
-  y[2] = (double)A[6] * x0 + (double)A[7] * x1 + (double)A[8] * x2;
-}
 
 // backward sweep over one colour (colours high to low), LANES lanes per position, writing tb (type TT,
 // also the gather source):
@@ -386,7 +397,7 @@ __global__ void __launch_bounds__(256)
     double rhs[3] = {a0, a1, a2};
     if (mode >= 1) {
       double u[3];
-      mv3d(dt + 9ll * i, a0, a1, a2, u);
+      mv3s(dt + 6ll * i, a0, a1, a2, u);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
         v.ut[3ll * i + l] = u[l];
@@ -395,7 +406,7 @@ __global__ void __launch_bounds__(256)
       g = a0 * rhs[0] + a1 * rhs[1] + a2 * rhs[2];
     }
     double tt[3];
-    mv3d(dtinv + 9ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, tt);
+    mv3s(dtinv + 6ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, tt);
     tb[3ll * i] = (TT)tt[0];
     tb[3ll * i + 1] = (TT)tt[1];
     tb[3ll * i + 2] = (TT)tt[2];
@@ -403,8 +414,8 @@ __global__ void __launch_bounds__(256)
       // the forward sweep's step on this colour when it is the first of the solve order (no lower
       // blocks): w = D~^{-1} (u - K t), w~ = t + w, partial w~.u -- what k_eis_fwd mode 1 computes
       double kt[3], ww[3];
-      mv3d(kk + 9ll * i, (double)(TT)tt[0], (double)(TT)tt[1], (double)(TT)tt[2], kt);
-      mv3d(dtinv + 9ll * i, rhs[0] - kt[0], rhs[1] - kt[1], rhs[2] - kt[2], ww);
+      mv3s(kk + 6ll * i, (double)(TT)tt[0], (double)(TT)tt[1], (double)(TT)tt[2], kt);
+      mv3s(dtinv + 6ll * i, rhs[0] - kt[0], rhs[1] - kt[1], rhs[2] - kt[2], ww);
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
         v.w[3ll * i + l] = ww[l];
@@ -473,12 +484,12 @@ __global__ void __launch_bounds__(256)
         u[l] = v.ut[3ll * i + l];
       }
       double kt[3];
-      mv3d(kk + 9ll * i, t[0], t[1], t[2], kt);
+      mv3s(kk + 6ll * i, t[0], t[1], t[2], kt);
 #pragma unroll
       for (int l = 0; l < 3; ++l) rhs[l] = u[l] - kt[l];
     }
     double ww[3];
-    mv3d(dtinv + 9ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, ww);
+    mv3s(dtinv + 6ll * i, rhs[0] - s0, rhs[1] - s1, rhs[2] - s2, ww);
     db[3ll * i] = (TT)ww[0];
     db[3ll * i + 1] = (TT)ww[1];
     db[3ll * i + 2] = (TT)ww[2];
@@ -853,9 +864,9 @@ int precond_build(bh_mesh* m, bh_precond* P) {
   BH_CHECK(hipMalloc(&P->hsplit, sizeof(double) * 18 * nl1));
   BH_CHECK(hipMalloc(&P->hdiag, sizeof(double) * 9 * (int64_t)std::max(1, nV)));
   BH_CHECK(hipMalloc(&P->sv, sizeof(double) * 11 * 3 * (int64_t)std::max(1, nV)));
-  BH_CHECK(hipMalloc(&P->dt, sizeof(float) * 9 * (int64_t)std::max(1, nV)));
-  BH_CHECK(hipMalloc(&P->dtinv, sizeof(float) * 9 * (int64_t)std::max(1, nV)));
-  BH_CHECK(hipMalloc(&P->kk, sizeof(double) * 9 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->dt, sizeof(float) * 6 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->dtinv, sizeof(float) * 6 * (int64_t)std::max(1, nV)));
+  BH_CHECK(hipMalloc(&P->kk, sizeof(double) * 6 * (int64_t)std::max(1, nV)));
   return BH_OK;
 }
 
