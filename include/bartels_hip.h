@@ -338,8 +338,8 @@ int bh_newton_static_device(bh_mesh* mesh, const bh_newton_opts* opts, const dou
 typedef enum {
   BH_KT_ENERGY = 0,
   BH_KT_GRADIENT = 1,
-  BH_KT_HESSIAN_GATHER = 2, /* the CSR assembly kernel (element blocks + deterministic gather) */
-  BH_KT_PSD = 3,            /* simple_psd_fix pass: Cholesky test, eigen-clamp of the tets that fail */
+  BH_KT_HESSIAN_GATHER = 2, /* the CSR assembly kernel (element or record blocks + deterministic gather) */
+  BH_KT_PSD = 3,            /* simple_psd_fix passes: each tet's reduced matrix, eigen-clamped into its record */
   BH_KT_COUNT = 4
 } bh_kernel_class;
 
@@ -349,8 +349,9 @@ int bh_mesh_set_profiling(bh_mesh* mesh, int on);
  * number of launches since profiling was switched on.  ms, launches: BH_KT_COUNT entries. */
 int bh_mesh_kernel_times(bh_mesh* mesh, double* ms, int64_t* launches);
 
-/* number of tets whose element Hessian needed eigenvalue clamping in the last PSD call
- * (synchronises the device) */
+/* number of tets whose element Hessian needed eigenvalue clamping in the last PSD call: FP64, tets
+ * with an eigenvalue below tol; FP32, tets whose M - tol I failed the Cholesky test (synchronises
+ * the device) */
 int bh_mesh_psd_count(bh_mesh* mesh, int64_t* count);
 
 /* Device sparse helpers on the mesh pattern (values in bh_mesh_pattern order). */
