@@ -72,6 +72,7 @@ class BhNewtonOpts(ctypes.Structure):
         ("d_q_pred", c_void_p),
         ("inertia", c_double),
         ("relaxation", c_double),
+        ("coarse_aggregates", c_int),
     ]
 
 

@@ -96,6 +96,8 @@ struct bh_mesh {
   // solver workspace (lazily allocated)
   double* ws = nullptr;
   size_t wsBytes = 0;
+  double* cws = nullptr;  // two-level coarse space: centroids, E, E^-1, coarse vectors, partials
+  size_t cwsBytes = 0;
   hipStream_t own = nullptr;
   // per-kernel-class HIP event pairs recorded while profiling is on
   mutable bool prof = false;
@@ -1311,7 +1313,8 @@ void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->vdeg, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
-                  m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32, m->tetQ, m->tetQ32};
+                  m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32, m->tetQ, m->tetQ32,
+                  m->cws};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);

@@ -527,6 +527,372 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* 
   return BH_OK;
 }
 
+// ---- two-level block Jacobi (BH_PRECOND_JACOBI_2L) -------------------------------------------
+// M^-1 = D^-1 + Z E^-1 Z^T: block Jacobi plus an additive coarse correction on aggregates of
+// consecutive vertices (a multiple of the vector kernel's 64 vertices each, so every workgroup of
+// that kernel lies in one aggregate).  Z holds six columns per aggregate I, the rigid-body modes
+// about its centroid c_I of the current positions: translations e_l and rotations e_c x (q_v - c_I);
+// rows of fixed DOFs are zero.  E = Z^T A Z (6 nagg square, dense) is formed once per system and
+// inverted in place (Gauss-Jordan on the SPD matrix, no pivoting); per CG iteration the correction
+// costs a reduction of Z^T r per aggregate, one dense product with E^-1 and a vertex pass
+// u += Z y.  On slender, bending-dominated problems the rigid modes of slabs carry the slow
+// components that block Jacobi resolves last.  E is accumulated with atomic adds (LDS per
+// workgroup, then global), so its rounding, and with it the iterates, can differ between runs in
+// the last bits.
+struct Coarse {
+  int nagg = 0, per = 0, k = 0;
+  double *cent, *E, *g, *y, *pzr;
+};
+
+// aggregate layout for nV vertices and a requested count (0: one per 16384 vertices)
+inline void coarse_layout(int nV, int want, int& nagg, int& per) {
+  if (want <= 0) want = std::max(1, (nV + 16383) / 16384);
+  want = std::min(want, 256);
+  per = (nV + want - 1) / want;
+  per = ((per + kJacVerts - 1) / kJacVerts) * kJacVerts;
+  nagg = std::max(1, (nV + per - 1) / per);
+}
+
+int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
+  coarse_layout(m->nV, want, c.nagg, c.per);
+  c.k = 6 * c.nagg;
+  const size_t need = sizeof(double) * (3 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + 6 * (size_t)gj + 64);
+  if (m->cwsBytes < need) {
+    if (m->cws) BH_CHECK(hipFree(m->cws));
+    m->cws = nullptr;
+    BH_CHECK(hipMalloc(&m->cws, need));
+    m->cwsBytes = need;
+  }
+  double* q = m->cws;
+  c.cent = q; q += 3 * c.nagg;
+  c.E = q; q += (size_t)c.k * c.k;
+  c.g = q; q += c.k;
+  c.y = q; q += c.k;
+  c.pzr = q;
+  return BH_OK;
+}
+
+// centroid of q over each aggregate (one workgroup per aggregate, fixed-order sums)
+__global__ void __launch_bounds__(256) k_agg_centroid(int nV, int per, const double* __restrict__ q,
+                                                      double* __restrict__ cent) {
+  __shared__ double sh[4];
+  const int v0 = blockIdx.x * per, v1 = min(nV, v0 + per);
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int v = v0 + threadIdx.x; v < v1; v += 256)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) a[l] += q[3ll * v + l];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const double t = block_sum_256(a[l], sh);
+    if (threadIdx.x == 0) cent[3 * blockIdx.x + l] = t / (double)max(1, v1 - v0);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero(int64_t n, double* __restrict__ a) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// E += Z^T A Z: one lane per row v (64 rows per workgroup, all in aggregate I); each block (v, n)
+// adds Z_v^T A_vn Z_n to E(I, J), J = aggregate of n.  The masked operator of the CG: fixed rows and
+// columns zero (Z's rows of fixed DOFs are zero), the diagonal block from hdiag when given.
+template <typename V>
+__global__ void __launch_bounds__(64)
+    k_coarse_E(int nV, int per, int nagg, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+               const V* __restrict__ vals, const double* __restrict__ hdiag, const uint8_t* __restrict__ fixed,
+               const double* __restrict__ q, const double* __restrict__ cent, double* __restrict__ E) {
+  __shared__ double acc[3][36];  // J = I - 1, I, I + 1
+  for (int t = threadIdx.x; t < 108; t += 64) acc[t / 36][t % 36] = 0.0;
+  __syncthreads();
+  const int v = blockIdx.x * 64 + threadIdx.x;
+  const int I = (blockIdx.x * 64) / per;
+  const int k = 6 * nagg;
+  if (v < nV) {
+    const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+    double dv[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) dv[l] = q[3ll * v + l] - cent[3 * I + l];
+    bool fv[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) fv[l] = fixed && fixed[3ll * v + l];
+    for (int r = 0; r < nb; ++r) {
+      const int n = bcol[b0 + r];
+      const int J = n / per;
+      bool fn[3];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) fn[kk] = fixed && fixed[3ll * n + kk];
+      // masked entries are selected at the load (a conditional store into the array after the loads
+      // left the fixed-column mask without effect in the compiled code)
+      const double* src = (n == v && hdiag) ? hdiag + 9ll * v : nullptr;
+      double A[9];
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+          const double a = src ? src[3 * l + kk] : (double)vals[9ll * b0 + 3ll * l * nb + 3 * r + kk];
+          A[3 * l + kk] = (fv[l] || fn[kk]) ? 0.0 : a;
+        }
+      double dn[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) dn[l] = q[3ll * n + l] - cent[3 * J + l];
+      // B = A Z_n (3 x 6): translations, then rotations e_c x d_n
+      double B[3][6];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        B[l][0] = A[3 * l];
+        B[l][1] = A[3 * l + 1];
+        B[l][2] = A[3 * l + 2];
+        B[l][3] = -A[3 * l + 1] * dn[2] + A[3 * l + 2] * dn[1];  // e_x x d = (0, -dz, dy)
+        B[l][4] = A[3 * l] * dn[2] - A[3 * l + 2] * dn[0];       // e_y x d = (dz, 0, -dx)
+        B[l][5] = -A[3 * l] * dn[1] + A[3 * l + 1] * dn[0];      // e_z x d = (-dy, dx, 0)
+      }
+      // C = Z_v^T B (6 x 6)
+      double C[36];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        C[c] = B[0][c];
+        C[6 + c] = B[1][c];
+        C[12 + c] = B[2][c];
+        C[18 + c] = -dv[2] * B[1][c] + dv[1] * B[2][c];
+        C[24 + c] = dv[2] * B[0][c] - dv[0] * B[2][c];
+        C[30 + c] = -dv[1] * B[0][c] + dv[0] * B[1][c];
+      }
+      const int slot = J - I + 1;
+      if (slot >= 0 && slot <= 2) {
+#pragma unroll
+        for (int t = 0; t < 36; ++t) atomicAdd(&acc[slot][t], C[t]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 36; ++t) atomicAdd(&E[(size_t)(6 * I + t / 6) * k + 6 * J + t % 6], C[t]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 108; t += 64) {
+    const int J = I - 1 + t / 36, e = t % 36;
+    if (J >= 0 && J < nagg) atomicAdd(&E[(size_t)(6 * I + e / 6) * k + 6 * J + e % 6], acc[t / 36][e]);
+  }
+}
+
+// E <- E^-1 in place (Gauss-Jordan without pivoting, one workgroup); a zero pivot (a mode with no
+// free DOF) gets a zero row and column
+__global__ void __launch_bounds__(1024) k_coarse_inv(int k, double* __restrict__ E) {
+  extern __shared__ double col[];  // 2k: column p, row p
+  double* row = col + k;
+  for (int p = 0; p < k; ++p) {
+    for (int i = threadIdx.x; i < k; i += 1024) {
+      col[i] = E[(size_t)i * k + p];
+      row[i] = E[(size_t)p * k + i];
+    }
+    __syncthreads();
+    const double d = col[p];
+    const double piv = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+    // wave wv takes rows wv, wv + 16, ...; its lanes the columns
+    for (int i = (int)threadIdx.x >> 6; i < k; i += 16) {
+      double* Ei = E + (size_t)i * k;
+      const double ci = col[i];
+      for (int j = (int)threadIdx.x & 63; j < k; j += 64) {
+        if (i == p && j == p)
+          Ei[j] = piv;
+        else if (i == p)
+          Ei[j] = row[j] * piv;
+        else if (j == p)
+          Ei[j] = -ci * piv;
+        else
+          Ei[j] -= ci * row[j] * piv;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// the coarse right-hand side g = Z^T r: the partials of each aggregate's vector-kernel workgroups,
+// summed in workgroup order (one workgroup per aggregate)
+__global__ void __launch_bounds__(256) k_coarse_g(int per, int nblocks, const double* __restrict__ pzr,
+                                                  double* __restrict__ g, const PcgState* __restrict__ st) {
+  __shared__ double sh[4];
+  if (st && cg_skip(st)) return;
+  const int w0 = blockIdx.x * (per / kJacVerts), w1 = min(nblocks, w0 + per / kJacVerts);
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int w = w0 + threadIdx.x; w < w1; w += 256)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a[c] += pzr[6ll * w + c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double t = block_sum_256(a[c], sh);
+    if (threadIdx.x == 0) g[6 * blockIdx.x + c] = t;
+  }
+}
+
+// y = E^-1 g (one wave per row, its lanes along the row: coalesced) and the partials of g.y
+// (= r . Z y, the coarse part of r.u), one per workgroup of four rows
+constexpr int kCoarseRowsPerBlock = 4;
+__global__ void __launch_bounds__(64 * kCoarseRowsPerBlock)
+    k_coarse_y(int k, const double* __restrict__ Einv, const double* __restrict__ g, double* __restrict__ y,
+               double* __restrict__ part, const PcgState* __restrict__ st) {
+  __shared__ double sh[kCoarseRowsPerBlock];
+  if (st && cg_skip(st)) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * kCoarseRowsPerBlock + wv;
+  double a = 0.0;
+  if (i < k)
+    for (int j = lane; j < k; j += 64) a += Einv[(size_t)i * k + j] * g[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+  if (lane == 0) {
+    if (i < k) y[i] = a;
+    sh[wv] = (i < k) ? a * g[i] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kCoarseRowsPerBlock; ++w) t += sh[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// u += Z y (rows of fixed DOFs untouched)
+__global__ void __launch_bounds__(256) k_coarse_add(int nV, int per, const double* __restrict__ q,
+                                                    const double* __restrict__ cent, const double* __restrict__ y,
+                                                    const uint8_t* __restrict__ fixed, double* __restrict__ u,
+                                                    const PcgState* __restrict__ st) {
+  if (st && cg_skip(st)) return;
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v >= nV) return;
+  const int I = v / per;
+  const double* a = y + 6 * I;
+  double d[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) d[l] = q[3ll * v + l] - cent[3 * I + l];
+  // a[0..2] + (a[3], a[4], a[5]) x d
+  const double z[3] = {a[0] + a[4] * d[2] - a[5] * d[1], a[1] + a[5] * d[0] - a[3] * d[2],
+                       a[2] + a[3] * d[1] - a[4] * d[0]};
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+    if (!(fixed && fixed[3ll * v + l])) u[3ll * v + l] += z[l];
+}
+
+// the vector kernel of block-Jacobi PCG plus the partials of Z^T r per workgroup (r of fixed DOFs
+// is zero: b is masked and the fixed rows of the operator are the identity)
+__global__ void __launch_bounds__(3 * kJacVerts)
+    k_cg_vec_jacobi_2l(int nV, int start, int per, const double* __restrict__ b, const double* __restrict__ Dinv,
+                       double* __restrict__ x, double* __restrict__ r, double* __restrict__ u, double* __restrict__ p,
+                       double* __restrict__ s, const double* __restrict__ w, const double* __restrict__ q,
+                       const double* __restrict__ cent, double* __restrict__ part_ru, double* __restrict__ part_rr,
+                       double* __restrict__ pzr, const PcgState* __restrict__ st) {
+  __shared__ double rs[3 * kJacVerts];
+  __shared__ double sh[4];
+  if (!start && cg_skip(st)) return;
+  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
+  const int64_t n = 3ll * nV;
+  double ri = 0.0;
+  if (i < n) {
+    if (start) {
+      x[i] = 0.0;
+      p[i] = 0.0;
+      s[i] = 0.0;
+      ri = b[i];
+    } else {
+      const double alpha = st->alpha, beta = st->beta;
+      const double pi = u[i] + beta * p[i];
+      const double si = w[i] + beta * s[i];
+      p[i] = pi;
+      s[i] = si;
+      x[i] += alpha * pi;
+      ri = r[i] - alpha * si;
+    }
+    r[i] = ri;
+  }
+  rs[threadIdx.x] = ri;
+  __syncthreads();
+  double ru = 0.0, rr = 0.0;
+  double zr[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const int l = (int)(i % 3);
+    const int base = threadIdx.x - l;
+    const double* D = Dinv + 3 * i;
+    const double ui = D[0] * rs[base] + D[1] * rs[base + 1] + D[2] * rs[base + 2];
+    u[i] = ui;
+    ru = ri * ui;
+    rr = ri * ri;
+    zr[l] = ri;
+    if (l == 0) {
+      const int v = (int)(i / 3), I = v / per;
+      double d[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) d[c] = q[3ll * v + c] - cent[3 * I + c];
+      const double r0 = rs[base], r1 = rs[base + 1], r2 = rs[base + 2];
+      zr[3] = d[1] * r2 - d[2] * r1;  // (d x r)
+      zr[4] = d[2] * r0 - d[0] * r2;
+      zr[5] = d[0] * r1 - d[1] * r0;
+    }
+  }
+  ru = block_sum<3 * kJacVerts>(ru, sh);
+  rr = block_sum<3 * kJacVerts>(rr, sh);
+  if (threadIdx.x == 0) {
+    part_ru[blockIdx.x] = ru;
+    part_rr[blockIdx.x] = rr;
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double t = block_sum<3 * kJacVerts>(zr[c], sh);
+    if (threadIdx.x == 0) pzr[6ll * blockIdx.x + c] = t;
+  }
+}
+
+// Two-level block-Jacobi PCG in vertex order on the CSR operator (FP64 or FP32 values, optional
+// FP64 diagonal).  The FP32 CSR operator is exactly symmetric when the Hessian came through the
+// PSD records; the element-math gathers leave it symmetric to FP32 rounding.
+template <typename V>
+int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag, const uint8_t* fixed,
+                 const double* q, NewtonWs& w, int* iters, int* done, hipStream_t s) {
+  const int nV = m->nV;
+  const int gj = jac_blocks(nV);
+  const int gs = spmv_blocks(nV);
+  const int64_t R = part_region(nV);
+  Coarse c;
+  int st = coarse_workspace(m, o->coarse_aggregates, gj, c);
+  if (st) return st;
+  const int gy = (c.k + kCoarseRowsPerBlock - 1) / kCoarseRowsPerBlock;
+  if (gj + gy > R) return BH_UNSUPPORTED;
+  double* pru = w.part;
+  double* prr = w.part + R;
+  double* pwu = w.part + 2 * R;
+  hipLaunchKernelGGL(k_block_jacobi<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, vals, hdiag,
+                     fixed, 0, w.Dinv);
+  hipLaunchKernelGGL(k_agg_centroid, dim3(c.nagg), dim3(256), 0, s, nV, c.per, q, c.cent);
+  const int64_t kk = (int64_t)c.k * c.k;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, s, kk, c.E);
+  hipLaunchKernelGGL((k_coarse_E<V>), dim3((nV + 63) / 64), dim3(64), 0, s, nV, c.per, c.nagg, m->bptr, m->bcol, vals,
+                     hdiag, fixed, q, (const double*)c.cent, c.E);
+  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(1024), sizeof(double) * 2 * c.k, s, c.k, c.E);
+  BH_CHECK(hipGetLastError());
+  auto precond_tail = [&](int start) {
+    const PcgState* sk = start ? nullptr : w.st;  // kernels skip once the solve is done
+    hipLaunchKernelGGL(k_cg_vec_jacobi_2l, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, start, c.per, w.b, w.Dinv, w.x,
+                       w.r, w.z, w.p, w.s, w.Ap, q, (const double*)c.cent, pru, prr, c.pzr, w.st);
+    hipLaunchKernelGGL(k_coarse_g, dim3(c.nagg), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
+    hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
+                       (const double*)c.g, c.y, pru + gj, sk);
+    hipLaunchKernelGGL(k_coarse_add, dim3((nV + 255) / 256), dim3(256), 0, s, nV, c.per, q, (const double*)c.cent,
+                       (const double*)c.y, fixed, w.z, sk);
+    hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, hdiag, fixed, w.z,
+                       w.Ap, pwu, start ? (const PcgState*)nullptr : (const PcgState*)w.st);
+    cg_reduce(w, nV, gj + gy, gj, gs, start, o, s);
+  };
+  precond_tail(1);
+  PcgState hs{};
+  for (;;) {
+    for (int k = 0; k < kPcgBatch; ++k) precond_tail(0);
+    if ((st = cg_wait(w, &hs, s))) return st;
+    if (hs.done) break;
+  }
+  *iters = hs.iters;
+  *done = hs.done;
+  return BH_OK;
+}
+
 // Block-Jacobi PCG on the split operator in the solve order of P (the precision form: the split
 // operator takes every upper block as the transpose of the lower one, so it is exactly symmetric,
 // where the FP32 CSR pattern holds H_vn and H_nv computed separately; CG on that slightly
@@ -811,6 +1177,7 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
     }
     return BH_OK;
   }
+  if (o->precond == BH_PRECOND_JACOBI_2L) return pcg_solve_2l(m, o, vals, hdiag, fixed, q, w, pits, pdone, s);
   if (hdiag) {  // precision form: block Jacobi on the symmetric split operator (colour lists made once)
     if (!m->solveOrder && (st = bh_precond_create(m, &m->solveOrder))) return st;
     return pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, m->solveOrder, 1, w, pits, pdone, s);
@@ -859,6 +1226,7 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->d_q_pred = nullptr;
   o->inertia = 0.0;
   o->relaxation = 1.0;
+  o->coarse_aggregates = 0;
 }
 
 

@@ -93,7 +93,7 @@ def newton_steps(p, mesh, steps, pcg_max):
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
     # precision 1: FP32 Hessian / CG operator, FP64 element gradient; 2: FP32 element gradient too
     variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5),
-                ("fp32full_sgs", 2, 5)]
+                ("fp32full_sgs", 2, 5), ("fp32_jacobi2l", 1, 7)]
     if net is not None:
         variants.append(("fp32_learned_sgs", 1, net.precond))
     res, qs = {}, {}

@@ -118,9 +118,10 @@ def gravity_problem(oracle):
 
 
 def test_newton_precision_form_and_preconditioners(gravity_problem):
-    """FP64 block-Jacobi, block IC(0) and block symmetric Gauss-Seidel (Eisenstat form), FP64 and the FP32
-    precision form, reach the oracle's equilibrium; the precision form within 1e-5 relative displacement.
-    IC(0) and Gauss-Seidel need fewer CG iterations than block Jacobi."""
+    """FP64 block-Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and two-level block
+    Jacobi (rigid-mode coarse correction), FP64 and the FP32 precision form, reach the oracle's
+    equilibrium; the precision form within 1e-5 relative displacement.  IC(0), Gauss-Seidel and the
+    two-level form need fewer CG iterations than block Jacobi."""
     from bartels_amd import _lib
 
     p, mesh, q0, fixed, f, q_or = gravity_problem
@@ -134,7 +135,8 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
                          ("fp64_sgs", dict(precond=5, handle=P)), ("fp32_jacobi", dict(precision=1)),
                          ("fp32_bic0", dict(precision=1, precond=2, handle=P)),
                          ("fp32_sgs", dict(precision=1, precond=5, handle=P)),
-                         ("fp32full_sgs", dict(precision=2, precond=5, handle=P))):
+                         ("fp32full_sgs", dict(precision=2, precond=5, handle=P)),
+                         ("fp64_2l", dict(precond=7)), ("fp32_2l", dict(precision=1, precond=7))):
             q, rep, tr = _newton(mesh, p, q0, fixed, f, **kw)
             err = np.linalg.norm(q - q_or) / u
             results[name] = (err, rep.iterations, rep.pcg_iterations_total, rep.gradient_norm / rep.gradient_norm0)
@@ -150,6 +152,9 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
         assert results["fp64_bic0"][2] < results["fp64_jacobi"][2]  # IC(0) needs fewer CG iterations
         assert results["fp64_sgs"][2] < results["fp64_jacobi"][2]
         assert results["fp32_sgs"][2] < results["fp32_jacobi"][2]
+        assert results["fp64_2l"][0] < 1e-8
+        assert results["fp32_2l"][0] < 1e-5
+        assert results["fp64_2l"][2] < results["fp64_jacobi"][2]  # the coarse correction saves iterations
     finally:
         lib.bh_precond_destroy(P)
 
