@@ -361,7 +361,11 @@ __global__ void __launch_bounds__(256) k_add_mass(int64_t n, double a, const dou
 // backward-Euler term.  H_vv = that row sum - sum_{n != v} H_vn keeps the FP32 operator exactly
 // translation invariant: its soft (smooth, low-stiffness) modes are then as accurate as the FP32
 // off-diagonal entries themselves, where a rounded FP32 diagonal perturbs them by ~1e-7 of the
-// largest stiffness (on the 6.07M-tet beam that alone cost 1.8x the CG iterations).
+// largest stiffness (on the 6.07M-tet beam that alone cost 1.8x the CG iterations).  The sum of
+// the off-diagonal blocks is symmetric in exact arithmetic (it is the row sum minus the symmetric
+// H_vv) but not after FP32 rounding; its symmetric part is kept, so the assembled operator is exactly
+// symmetric wherever the off-diagonal blocks are exact transposes of each other (CG on the
+// unsymmetrised diagonal ended twice as far from the FP64 Newton iterate).
 template <typename V>
 __global__ void __launch_bounds__(256)
     k_implied_diag(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
@@ -386,7 +390,10 @@ __global__ void __launch_bounds__(256)
   }
   const double g = psd_tol * (double)vdeg[v];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) hdiag[9ll * v + i] = ((i % 4 == 0) ? g : 0.0) + inertia * Ms[i] - S[i];
+  for (int i = 0; i < 9; ++i) {
+    const int t = 3 * (i % 3) + i / 3;  // transposed position
+    hdiag[9ll * v + i] = ((i % 4 == 0) ? g : 0.0) + inertia * 0.5 * (Ms[i] + Ms[t]) - 0.5 * (S[i] + S[t]);
+  }
 }
 
 struct NewtonWs {
@@ -544,10 +551,14 @@ struct Coarse {
   double *cent, *E, *g, *y, *pzr;
 };
 
-// aggregate layout for nV vertices and a requested count (0: one per 16384 vertices)
+// aggregate layout for nV vertices and a requested count (0: one per 1024 vertices, 4 to 32; on
+// the 6.07M-tet beam 16 to 128 aggregates gave the same iteration counts within 2%, and the dense
+// inverse of E, formed by one workgroup, costs ~20 ms per Newton step at 64 aggregates and ~170 ms
+// at 128, hence the cap)
+constexpr int kMaxAggregates = 64;
 inline void coarse_layout(int nV, int want, int& nagg, int& per) {
-  if (want <= 0) want = std::max(1, (nV + 16383) / 16384);
-  want = std::min(want, 256);
+  if (want <= 0) want = std::min(32, std::max(4, nV / 1024));
+  want = std::min(want, kMaxAggregates);
   per = (nV + want - 1) / want;
   per = ((per + kJacVerts - 1) / kJacVerts) * kJacVerts;
   nagg = std::max(1, (nV + per - 1) / per);

@@ -70,8 +70,9 @@ def cpu_baseline(p, dX, vol, budget_s):
 def newton_steps(p, mesh, steps, pcg_max):
     """Newton-step wall time on the same beam (bent state, gravity, x = 0 clamped, PSD 1e-3, PCG to a
     1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
-    precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and
-    the learned Eisenstat diagonal (when trained weights are present)."""
+    precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form),
+    two-level block Jacobi (rigid-mode coarse space on vertex aggregates, also in FP64) and the
+    learned Eisenstat diagonal (when trained weights are present)."""
     import torch
 
     from bartels_amd import _lib, learned
@@ -93,7 +94,7 @@ def newton_steps(p, mesh, steps, pcg_max):
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
     # precision 1: FP32 Hessian / CG operator, FP64 element gradient; 2: FP32 element gradient too
     variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5),
-                ("fp32full_sgs", 2, 5), ("fp32_jacobi2l", 1, 7)]
+                ("fp32full_sgs", 2, 5), ("fp64_jacobi2l", 0, 7), ("fp32_jacobi2l", 1, 7)]
     if net is not None:
         variants.append(("fp32_learned_sgs", 1, net.precond))
     res, qs = {}, {}

@@ -307,8 +307,8 @@ typedef struct {
   const double* d_q_pred;
   double inertia;
   double relaxation;       /* BH_PRECOND_SGS: SSOR relaxation w in (0, 2), Dt = D / w (0 means 1) */
-  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L: number of aggregates (0: one per 16384 vertices, at
-                              least 1; at most 256) */
+  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L: number of aggregates (0: one per 1024 vertices,
+                              4 to 32; larger requests are capped at 64) */
 } bh_newton_opts;
 
 typedef struct {
