@@ -537,27 +537,30 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* 
 // ---- two-level block Jacobi (BH_PRECOND_JACOBI_2L) -------------------------------------------
 // M^-1 = D^-1 + Z E^-1 Z^T: block Jacobi plus an additive coarse correction on aggregates of
 // consecutive vertices (a multiple of the vector kernel's 64 vertices each, so every workgroup of
-// that kernel lies in one aggregate).  Z holds six columns per aggregate I, the rigid-body modes
-// about its centroid c_I of the current positions: translations e_l and rotations e_c x (q_v - c_I);
-// rows of fixed DOFs are zero.  E = Z^T A Z (6 nagg square, dense) is formed once per system and
-// inverted in place (Gauss-Jordan on the SPD matrix, no pivoting); per CG iteration the correction
-// costs a reduction of Z^T r per aggregate, one dense product with E^-1 and a vertex pass
-// u += Z y.  On slender, bending-dominated problems the rigid modes of slabs carry the slow
-// components that block Jacobi resolves last.  E is accumulated with atomic adds (LDS per
-// workgroup, then global), so its rounding, and with it the iterates, can differ between runs in
-// the last bits.
+// that kernel lies in one aggregate).  Z holds the affine modes of each aggregate I about the
+// centroid c_I of the current positions, twelve columns: translations e_l and the linear modes
+// e_l (q_v - c_I)_m (they span the rigid rotations too); rows of fixed DOFs are zero.
+// E = Z^T A Z (12 nagg square, dense) is formed once per system and inverted in place (Gauss-Jordan
+// on the SPD matrix, no pivoting); per CG iteration the correction costs a reduction of Z^T r per
+// aggregate, one dense product with E^-1 and a vertex pass u += Z y.  On slender,
+// bending-dominated problems the affine modes of slabs carry the slow components that block Jacobi
+// resolves last (on a 96x24x24 beam, 16 affine aggregates: 271 CG iterations, 32 rigid-mode
+// aggregates with the same coarse size: 322, block Jacobi alone: 2173).  E is accumulated with
+// atomic adds (LDS per workgroup, then global), so its rounding, and with it the iterates, can
+// differ between runs in the last bits.
+constexpr int kModes = 12;
 struct Coarse {
   int nagg = 0, per = 0, k = 0;
   double *cent, *E, *g, *y, *pzr;
 };
 
-// aggregate layout for nV vertices and a requested count (0: one per 1024 vertices, 4 to 32; on
-// the 6.07M-tet beam 16 to 128 aggregates gave the same iteration counts within 2%, and the dense
-// inverse of E, formed by one workgroup, costs ~20 ms per Newton step at 64 aggregates and ~170 ms
-// at 128, hence the cap)
-constexpr int kMaxAggregates = 64;
+// aggregate layout for nV vertices and a requested count (0: one per 2048 vertices, 4 to 16; on
+// the 6.07M-tet beam 16 to 128 rigid-mode aggregates gave the same iteration counts within 2%, and
+// the dense inverse of E, formed by one workgroup, costs ~20 ms per Newton step at k = 384 and
+// ~170 ms at k = 768, hence the cap)
+constexpr int kMaxAggregates = 32;
 inline void coarse_layout(int nV, int want, int& nagg, int& per) {
-  if (want <= 0) want = std::min(32, std::max(4, nV / 1024));
+  if (want <= 0) want = std::min(16, std::max(4, nV / 2048));
   want = std::min(want, kMaxAggregates);
   per = (nV + want - 1) / want;
   per = ((per + kJacVerts - 1) / kJacVerts) * kJacVerts;
@@ -566,8 +569,9 @@ inline void coarse_layout(int nV, int want, int& nagg, int& per) {
 
 int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   coarse_layout(m->nV, want, c.nagg, c.per);
-  c.k = 6 * c.nagg;
-  const size_t need = sizeof(double) * (3 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + 6 * (size_t)gj + 64);
+  c.k = kModes * c.nagg;
+  const size_t need =
+      sizeof(double) * (3 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64);
   if (m->cwsBytes < need) {
     if (m->cws) BH_CHECK(hipFree(m->cws));
     m->cws = nullptr;
@@ -604,20 +608,50 @@ __global__ void __launch_bounds__(256) k_fill_zero(int64_t n, double* __restrict
   if (i < n) a[i] = 0.0;
 }
 
-// E += Z^T A Z: one lane per row v (64 rows per workgroup, all in aggregate I); each block (v, n)
-// adds Z_v^T A_vn Z_n to E(I, J), J = aggregate of n.  The masked operator of the CG: fixed rows and
-// columns zero (Z's rows of fixed DOFs are zero), the diagonal block from hdiag when given.
+// B += A_vn Z_n for the affine modes of n (3 x 12, row-major): column l is A e_l, column 3 + 3 l + m
+// is A e_l d_m
+__device__ __forceinline__ void affine_right(const double A[9], const double d[3], double B[36]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const double a = A[3 * r + l];
+      B[12 * r + l] += a;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) B[12 * r + 3 + 3 * l + m] += a * d[m];
+    }
+}
+
+// dst (12 x 12) += Z_v^T B: row l is B's row l, row 3 + 3 l + m is d_m times B's row l
+__device__ __forceinline__ void affine_left_add(const double B[36], const double d[3], double* dst, int ld) {
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < kModes; ++c) {
+      const double b = B[12 * l + c];
+      atomicAdd(&dst[(size_t)l * ld + c], b);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) atomicAdd(&dst[(size_t)(3 + 3 * l + m) * ld + c], d[m] * b);
+    }
+}
+
+// E += Z^T A Z: one lane per row v (64 rows per workgroup, all in aggregate I).  The blocks of
+// neighbours n in the same aggregate are summed in registers (B = sum A_vn Z_n) and added once per
+// lane; the few neighbours in other aggregates go to LDS slots (J = I +- 1) or straight to global
+// memory.  The masked operator of the CG: fixed rows and columns zero (Z's rows of fixed DOFs are
+// zero), the diagonal block from hdiag when given.
 template <typename V>
 __global__ void __launch_bounds__(64)
     k_coarse_E(int nV, int per, int nagg, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                const V* __restrict__ vals, const double* __restrict__ hdiag, const uint8_t* __restrict__ fixed,
                const double* __restrict__ q, const double* __restrict__ cent, double* __restrict__ E) {
-  __shared__ double acc[3][36];  // J = I - 1, I, I + 1
-  for (int t = threadIdx.x; t < 108; t += 64) acc[t / 36][t % 36] = 0.0;
+  constexpr int kk2 = kModes * kModes;
+  __shared__ double acc[3][kk2];  // J = I - 1, I, I + 1
+  for (int t = threadIdx.x; t < 3 * kk2; t += 64) acc[t / kk2][t % kk2] = 0.0;
   __syncthreads();
   const int v = blockIdx.x * 64 + threadIdx.x;
   const int I = (blockIdx.x * 64) / per;
-  const int k = 6 * nagg;
+  const int k = kModes * nagg;
   if (v < nV) {
     const int b0 = bptr[v], nb = bptr[v + 1] - b0;
     double dv[3];
@@ -626,12 +660,15 @@ __global__ void __launch_bounds__(64)
     bool fv[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) fv[l] = fixed && fixed[3ll * v + l];
+    double Bs[36];  // same-aggregate sum
+#pragma unroll
+    for (int t = 0; t < 36; ++t) Bs[t] = 0.0;
     for (int r = 0; r < nb; ++r) {
       const int n = bcol[b0 + r];
       const int J = n / per;
       bool fn[3];
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) fn[kk] = fixed && fixed[3ll * n + kk];
+      for (int c = 0; c < 3; ++c) fn[c] = fixed && fixed[3ll * n + c];
       // masked entries are selected at the load (a conditional store into the array after the loads
       // left the fixed-column mask without effect in the compiled code)
       const double* src = (n == v && hdiag) ? hdiag + 9ll * v : nullptr;
@@ -639,49 +676,34 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int l = 0; l < 3; ++l)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-          const double a = src ? src[3 * l + kk] : (double)vals[9ll * b0 + 3ll * l * nb + 3 * r + kk];
-          A[3 * l + kk] = (fv[l] || fn[kk]) ? 0.0 : a;
+        for (int c = 0; c < 3; ++c) {
+          const double a = src ? src[3 * l + c] : (double)vals[9ll * b0 + 3ll * l * nb + 3 * r + c];
+          A[3 * l + c] = (fv[l] || fn[c]) ? 0.0 : a;
         }
       double dn[3];
 #pragma unroll
       for (int l = 0; l < 3; ++l) dn[l] = q[3ll * n + l] - cent[3 * J + l];
-      // B = A Z_n (3 x 6): translations, then rotations e_c x d_n
-      double B[3][6];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        B[l][0] = A[3 * l];
-        B[l][1] = A[3 * l + 1];
-        B[l][2] = A[3 * l + 2];
-        B[l][3] = -A[3 * l + 1] * dn[2] + A[3 * l + 2] * dn[1];  // e_x x d = (0, -dz, dy)
-        B[l][4] = A[3 * l] * dn[2] - A[3 * l + 2] * dn[0];       // e_y x d = (dz, 0, -dx)
-        B[l][5] = -A[3 * l] * dn[1] + A[3 * l + 1] * dn[0];      // e_z x d = (-dy, dx, 0)
-      }
-      // C = Z_v^T B (6 x 6)
-      double C[36];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        C[c] = B[0][c];
-        C[6 + c] = B[1][c];
-        C[12 + c] = B[2][c];
-        C[18 + c] = -dv[2] * B[1][c] + dv[1] * B[2][c];
-        C[24 + c] = dv[2] * B[0][c] - dv[0] * B[2][c];
-        C[30 + c] = -dv[1] * B[0][c] + dv[0] * B[1][c];
-      }
-      const int slot = J - I + 1;
-      if (slot >= 0 && slot <= 2) {
-#pragma unroll
-        for (int t = 0; t < 36; ++t) atomicAdd(&acc[slot][t], C[t]);
+      if (J == I) {
+        affine_right(A, dn, Bs);
       } else {
+        double B[36];
 #pragma unroll
-        for (int t = 0; t < 36; ++t) atomicAdd(&E[(size_t)(6 * I + t / 6) * k + 6 * J + t % 6], C[t]);
+        for (int t = 0; t < 36; ++t) B[t] = 0.0;
+        affine_right(A, dn, B);
+        const int slot = J - I + 1;
+        if (slot == 0 || slot == 2)
+          affine_left_add(B, dv, acc[slot], kModes);
+        else
+          affine_left_add(B, dv, E + (size_t)kModes * I * k + kModes * J, k);
       }
     }
+    affine_left_add(Bs, dv, acc[1], kModes);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < 108; t += 64) {
-    const int J = I - 1 + t / 36, e = t % 36;
-    if (J >= 0 && J < nagg) atomicAdd(&E[(size_t)(6 * I + e / 6) * k + 6 * J + e % 6], acc[t / 36][e]);
+  for (int t = threadIdx.x; t < 3 * kk2; t += 64) {
+    const int J = I - 1 + t / kk2, e = t % kk2;
+    if (J >= 0 && J < nagg)
+      atomicAdd(&E[(size_t)(kModes * I + e / kModes) * k + kModes * J + e % kModes], acc[t / kk2][e]);
   }
 }
 
@@ -718,21 +740,17 @@ __global__ void __launch_bounds__(1024) k_coarse_inv(int k, double* __restrict__
 }
 
 // the coarse right-hand side g = Z^T r: the partials of each aggregate's vector-kernel workgroups,
-// summed in workgroup order (one workgroup per aggregate)
+// summed in workgroup order (one workgroup per aggregate and mode)
 __global__ void __launch_bounds__(256) k_coarse_g(int per, int nblocks, const double* __restrict__ pzr,
                                                   double* __restrict__ g, const PcgState* __restrict__ st) {
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
-  const int w0 = blockIdx.x * (per / kJacVerts), w1 = min(nblocks, w0 + per / kJacVerts);
-  double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int w = w0 + threadIdx.x; w < w1; w += 256)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) a[c] += pzr[6ll * w + c];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    const double t = block_sum_256(a[c], sh);
-    if (threadIdx.x == 0) g[6 * blockIdx.x + c] = t;
-  }
+  const int I = blockIdx.x / kModes, c = blockIdx.x % kModes;
+  const int w0 = I * (per / kJacVerts), w1 = min(nblocks, w0 + per / kJacVerts);
+  double a = 0.0;
+  for (int w = w0 + threadIdx.x; w < w1; w += 256) a += pzr[(int64_t)kModes * w + c];
+  a = block_sum_256(a, sh);
+  if (threadIdx.x == 0) g[blockIdx.x] = a;
 }
 
 // y = E^-1 g (one wave per row, its lanes along the row: coalesced) and the partials of g.y
@@ -763,37 +781,43 @@ __global__ void __launch_bounds__(64 * kCoarseRowsPerBlock)
   }
 }
 
-// u += Z y (rows of fixed DOFs untouched)
-__global__ void __launch_bounds__(256) k_coarse_add(int nV, int per, const double* __restrict__ q,
-                                                    const double* __restrict__ cent, const double* __restrict__ y,
-                                                    const uint8_t* __restrict__ fixed, double* __restrict__ u,
-                                                    const PcgState* __restrict__ st) {
+// u += Z y (rows of fixed DOFs untouched): one thread per DOF row, 64 vertices per workgroup (the
+// vertex's positions meet in LDS, each read once)
+__global__ void __launch_bounds__(3 * kJacVerts)
+    k_coarse_add(int nV, int per, const double* __restrict__ q, const double* __restrict__ cent,
+                 const double* __restrict__ y, const uint8_t* __restrict__ fixed, double* __restrict__ u,
+                 const PcgState* __restrict__ st) {
+  __shared__ double qs[3 * kJacVerts];
   if (st && cg_skip(st)) return;
-  const int v = blockIdx.x * 256 + threadIdx.x;
-  if (v >= nV) return;
+  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
+  const bool in = i < 3ll * nV;
+  qs[threadIdx.x] = in ? q[i] : 0.0;
+  __syncthreads();
+  if (!in || (fixed && fixed[i])) return;
+  const int v = (int)(i / 3), l = (int)(i % 3);
   const int I = v / per;
-  const double* a = y + 6 * I;
-  double d[3];
-#pragma unroll
-  for (int l = 0; l < 3; ++l) d[l] = q[3ll * v + l] - cent[3 * I + l];
-  // a[0..2] + (a[3], a[4], a[5]) x d
-  const double z[3] = {a[0] + a[4] * d[2] - a[5] * d[1], a[1] + a[5] * d[0] - a[3] * d[2],
-                       a[2] + a[3] * d[1] - a[4] * d[0]};
-#pragma unroll
-  for (int l = 0; l < 3; ++l)
-    if (!(fixed && fixed[3ll * v + l])) u[3ll * v + l] += z[l];
+  const double* a = y + kModes * I;
+  const double* c = cent + 3 * I;
+  const int base = threadIdx.x - l;
+  u[i] += a[l] + a[3 + 3 * l] * (qs[base] - c[0]) + a[4 + 3 * l] * (qs[base + 1] - c[1]) +
+          a[5 + 3 * l] * (qs[base + 2] - c[2]);
 }
 
 // the vector kernel of block-Jacobi PCG plus the partials of Z^T r per workgroup (r of fixed DOFs
-// is zero: b is masked and the fixed rows of the operator are the identity)
+// is zero: b is masked and the fixed rows of the operator are the identity).  Z^T r of the
+// workgroup is the 3 x 4 sum of r_v (x) (1, d_v): every row writes r (1, d) to LDS, then 16 lanes
+// per entry sum it (4 values each, then a 16-lane butterfly)
 __global__ void __launch_bounds__(3 * kJacVerts)
     k_cg_vec_jacobi_2l(int nV, int start, int per, const double* __restrict__ b, const double* __restrict__ Dinv,
                        double* __restrict__ x, double* __restrict__ r, double* __restrict__ u, double* __restrict__ p,
                        double* __restrict__ s, const double* __restrict__ w, const double* __restrict__ q,
                        const double* __restrict__ cent, double* __restrict__ part_ru, double* __restrict__ part_rr,
                        double* __restrict__ pzr, const PcgState* __restrict__ st) {
+  static_assert(kModes * 16 == 3 * kJacVerts, "one 16-lane group per coarse entry");
   __shared__ double rs[3 * kJacVerts];
-  __shared__ double sh[4];
+  __shared__ double qs[3 * kJacVerts];
+  __shared__ double zs[3 * kJacVerts][4];
+  __shared__ double sh[3][2];
   if (!start && cg_skip(st)) return;
   const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
   const int64_t n = 3ll * nV;
@@ -816,9 +840,10 @@ __global__ void __launch_bounds__(3 * kJacVerts)
     r[i] = ri;
   }
   rs[threadIdx.x] = ri;
+  qs[threadIdx.x] = (i < n) ? q[i] : 0.0;
   __syncthreads();
   double ru = 0.0, rr = 0.0;
-  double zr[6] = {0, 0, 0, 0, 0, 0};
+  double z[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const int l = (int)(i % 3);
     const int base = threadIdx.x - l;
@@ -827,28 +852,35 @@ __global__ void __launch_bounds__(3 * kJacVerts)
     u[i] = ui;
     ru = ri * ui;
     rr = ri * ri;
-    zr[l] = ri;
-    if (l == 0) {
-      const int v = (int)(i / 3), I = v / per;
-      double d[3];
+    const int I = (int)(i / 3) / per;
+    z[0] = ri;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) d[c] = q[3ll * v + c] - cent[3 * I + c];
-      const double r0 = rs[base], r1 = rs[base + 1], r2 = rs[base + 2];
-      zr[3] = d[1] * r2 - d[2] * r1;  // (d x r)
-      zr[4] = d[2] * r0 - d[0] * r2;
-      zr[5] = d[0] * r1 - d[1] * r0;
-    }
+    for (int m = 0; m < 3; ++m) z[1 + m] = ri * (qs[base + m] - cent[3 * I + m]);
   }
-  ru = block_sum<3 * kJacVerts>(ru, sh);
-  rr = block_sum<3 * kJacVerts>(rr, sh);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) zs[threadIdx.x][c] = z[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    ru += __shfl_xor(ru, off, 64);
+    rr += __shfl_xor(rr, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6][0] = ru;
+    sh[threadIdx.x >> 6][1] = rr;
+  }
+  __syncthreads();
+  // entry Q = (l, c) of the 3 x 4 sum: lanes 16 Q .. 16 Q + 15, lane j takes vertices j, j + 16, ...
+  const int Q = threadIdx.x >> 4, j = threadIdx.x & 15;
+  const int l = Q >> 2, c = Q & 3;
+  double a = 0.0;
+#pragma unroll
+  for (int t = 0; t < kJacVerts / 16; ++t) a += zs[3 * (j + 16 * t) + l][c];
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+  if (j == 0) pzr[(int64_t)kModes * blockIdx.x + (c == 0 ? l : 3 + 3 * l + c - 1)] = a;
   if (threadIdx.x == 0) {
-    part_ru[blockIdx.x] = ru;
-    part_rr[blockIdx.x] = rr;
-  }
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    const double t = block_sum<3 * kJacVerts>(zr[c], sh);
-    if (threadIdx.x == 0) pzr[6ll * blockIdx.x + c] = t;
+    part_ru[blockIdx.x] = sh[0][0] + sh[1][0] + sh[2][0];
+    part_rr[blockIdx.x] = sh[0][1] + sh[1][1] + sh[2][1];
   }
 }
 
@@ -883,10 +915,10 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
     const PcgState* sk = start ? nullptr : w.st;  // kernels skip once the solve is done
     hipLaunchKernelGGL(k_cg_vec_jacobi_2l, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, start, c.per, w.b, w.Dinv, w.x,
                        w.r, w.z, w.p, w.s, w.Ap, q, (const double*)c.cent, pru, prr, c.pzr, w.st);
-    hipLaunchKernelGGL(k_coarse_g, dim3(c.nagg), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
+    hipLaunchKernelGGL(k_coarse_g, dim3(c.k), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
     hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
                        (const double*)c.g, c.y, pru + gj, sk);
-    hipLaunchKernelGGL(k_coarse_add, dim3((nV + 255) / 256), dim3(256), 0, s, nV, c.per, q, (const double*)c.cent,
+    hipLaunchKernelGGL(k_coarse_add, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, q, (const double*)c.cent,
                        (const double*)c.y, fixed, w.z, sk);
     hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, hdiag, fixed, w.z,
                        w.Ap, pwu, start ? (const PcgState*)nullptr : (const PcgState*)w.st);
