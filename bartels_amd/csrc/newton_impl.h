@@ -707,11 +707,16 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// E <- E^-1 in place (Gauss-Jordan without pivoting, one workgroup); a zero pivot (a mode with no
-// free DOF) gets a zero row and column
+// E <- E^-1 in place (Gauss-Jordan without pivoting, one workgroup).  A pivot at or below 1e-12
+// of its original diagonal entry belongs to a mode with no free DOF or one that the earlier modes
+// already span (an aggregate whose free vertices share a coordinate plane makes a linear mode a
+// multiple of a translation): it gets a zero row and column, which drops the mode from Z and keeps
+// the correction positive semi-definite.
 __global__ void __launch_bounds__(1024) k_coarse_inv(int k, double* __restrict__ E) {
-  extern __shared__ double col[];  // 2k: column p, row p
+  extern __shared__ double col[];  // 3k: column p, row p, the original diagonal
   double* row = col + k;
+  double* dg = col + 2 * k;
+  for (int i = threadIdx.x; i < k; i += 1024) dg[i] = E[(size_t)i * k + i];
   for (int p = 0; p < k; ++p) {
     for (int i = threadIdx.x; i < k; i += 1024) {
       col[i] = E[(size_t)i * k + p];
@@ -719,7 +724,7 @@ __global__ void __launch_bounds__(1024) k_coarse_inv(int k, double* __restrict__
     }
     __syncthreads();
     const double d = col[p];
-    const double piv = fabs(d) > 1e-300 ? 1.0 / d : 0.0;
+    const double piv = (d > 1e-12 * dg[p] && d > 0.0) ? 1.0 / d : 0.0;
     // wave wv takes rows wv, wv + 16, ...; its lanes the columns
     for (int i = (int)threadIdx.x >> 6; i < k; i += 16) {
       double* Ei = E + (size_t)i * k;
@@ -909,7 +914,7 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
   hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, s, kk, c.E);
   hipLaunchKernelGGL((k_coarse_E<V>), dim3((nV + 63) / 64), dim3(64), 0, s, nV, c.per, c.nagg, m->bptr, m->bcol, vals,
                      hdiag, fixed, q, (const double*)c.cent, c.E);
-  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(1024), sizeof(double) * 2 * c.k, s, c.k, c.E);
+  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(1024), sizeof(double) * 3 * c.k, s, c.k, c.E);
   BH_CHECK(hipGetLastError());
   auto precond_tail = [&](int start) {
     const PcgState* sk = start ? nullptr : w.st;  // kernels skip once the solve is done

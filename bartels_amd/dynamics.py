@@ -87,7 +87,8 @@ class BackwardEuler:
 
 
 VARIANTS = {"fp64_jacobi": (0, 0), "fp32_jacobi": (0, 1), "fp32_bic0": (2, 1), "fp32_sgs": (5, 1),
-            "fp32_learned_sgs": (6, 1), "fp32_learned_factor": (4, 1)}
+            "fp32_learned_sgs": (6, 1), "fp32_learned_factor": (4, 1), "fp64_jacobi2l": (7, 0),
+            "fp32_jacobi2l": (7, 1)}
 
 
 def run(steps, variant, grid=(256, 64, 64), weights=None, log=print):

@@ -64,7 +64,7 @@ def test_newton_config1_matches_direct_solve(oracle):
 
 
 def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_max=20000, gtol=1e-7, max_it=30,
-            handle=None, fallback=1, psd=False):
+            handle=None, fallback=1, psd=False, aggregates=0):
     import torch
 
     from bartels_amd import _lib
@@ -72,6 +72,7 @@ def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_ma
     lib = _lib.load()
     o = _lib.BhNewtonOpts()
     lib.bh_newton_default_opts(ctypes.byref(o))
+    o.coarse_aggregates = aggregates
     o.model = 0
     o.gradient_tol = gtol
     o.relative_tol = 1
@@ -119,7 +120,7 @@ def gravity_problem(oracle):
 
 def test_newton_precision_form_and_preconditioners(gravity_problem):
     """FP64 block-Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and two-level block
-    Jacobi (rigid-mode coarse correction), FP64 and the FP32 precision form, reach the oracle's
+    Jacobi (affine coarse correction), FP64 and the FP32 precision form, reach the oracle's
     equilibrium; the precision form within 1e-5 relative displacement.  IC(0), Gauss-Seidel and the
     two-level form need fewer CG iterations than block Jacobi."""
     from bartels_amd import _lib
@@ -157,6 +158,25 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
         assert results["fp64_2l"][2] < results["fp64_jacobi"][2]  # the coarse correction saves iterations
     finally:
         lib.bh_precond_destroy(P)
+
+
+def test_two_level_aggregate_counts(gravity_problem):
+    """Two-level block Jacobi with 1, 4, 16 and 64 requested aggregates (64 is capped at 32; the
+    aggregates are multiples of 64 vertices, so the last one may be short): every count reaches the
+    oracle's equilibrium to 1e-8 in FP64 and to 1e-5 in the precision form, with fewer CG iterations
+    than block Jacobi."""
+    p, mesh, q0, fixed, f, q_or = gravity_problem
+    u = np.linalg.norm(q_or - q0)
+    _, rep_j, _ = _newton(mesh, p, q0, fixed, f)
+    for agg in (1, 4, 16, 64):
+        for precision, bound in ((0, 1e-8), (1, 1e-5)):
+            q, rep, _ = _newton(mesh, p, q0, fixed, f, precond=7, precision=precision, aggregates=agg)
+            err = np.linalg.norm(q - q_or) / u
+            print(f"aggregates {agg} precision {precision}: rel err {err:.2e}, newton its {rep.iterations}, "
+                  f"pcg its {rep.pcg_iterations_total} (block Jacobi {rep_j.pcg_iterations_total})")
+            assert err < bound
+            assert rep.pcg_failures == 0
+            assert rep.pcg_iterations_total < rep_j.pcg_iterations_total
 
 
 def test_newton_jacobi_fallback_at_iteration_cap(gravity_problem):
