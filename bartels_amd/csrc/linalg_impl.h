@@ -67,9 +67,20 @@ __device__ __forceinline__ void inv3_block(double* A, int point, double* o) {
 // xor butterfly.  FP32 values are widened; products are summed in FP64.  One pass per block (the
 // loads of a row are a dependent chain; many resident blocks hide it).  hdiag (optional, FP64, 9 per
 // vertex) replaces the stored diagonal blocks.
-constexpr int kSpmvLanes = 16;
+// lanes per vertex: 16 with FP64 blocks, 8 with FP32 blocks (two blocks per lane on the Kuhn
+// meshes; on the 6.07M-tet beam the FP32 product got 5 % faster per Newton step with 8 lanes, the
+// FP64 one 16 % slower, and unrolling the block loop or 4 lanes were slower in both)
+template <typename V>
+constexpr int spmv_lanes() {
+  return sizeof(V) == 4 ? 8 : 16;
+}
+constexpr int kSpmvLanes = 16;  // the most lanes of any instance: sizes the partial-sum regions
 inline int spmv_blocks(int nV) {
   return (int)std::max<int64_t>(1, (kSpmvLanes * (int64_t)nV + kRedBlock - 1) / kRedBlock);
+}
+template <typename V>
+inline int spmv_blocks_for(int nV) {
+  return (int)std::max<int64_t>(1, (spmv_lanes<V>() * (int64_t)nV + kRedBlock - 1) / kRedBlock);
 }
 
 template <typename V>
@@ -78,15 +89,16 @@ __global__ void __launch_bounds__(kRedBlock)
                const V* __restrict__ vals, const double* __restrict__ hdiag, const uint8_t* __restrict__ fixed,
                const double* __restrict__ p,
                double* __restrict__ Ap, double* __restrict__ part, const PcgState* __restrict__ st) {
+  constexpr int L = spmv_lanes<V>();
   __shared__ double sh[4];
   if (st && cg_skip(st)) return;
   const int64_t gid = blockIdx.x * (int64_t)kRedBlock + threadIdx.x;
-  const int v = (int)(gid / kSpmvLanes), j = (int)(gid % kSpmvLanes);
+  const int v = (int)(gid / L), j = (int)(gid % L);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (v < nV) {
     const int b0 = bptr[v], nb = bptr[v + 1] - b0;
     const V* row = vals + 9ll * b0;
-    for (int r = j; r < nb; r += kSpmvLanes) {
+    for (int r = j; r < nb; r += L) {
       const int c = bcol[b0 + r];
       const double* xp = p + 3ll * c;
       const double x0 = xp[0], x1 = xp[1], x2 = xp[2];
@@ -106,7 +118,7 @@ __global__ void __launch_bounds__(kRedBlock)
     }
   }
 #pragma unroll
-  for (int off = kSpmvLanes / 2; off > 0; off >>= 1) {
+  for (int off = L / 2; off > 0; off >>= 1) {
     s0 += __shfl_xor(s0, off, 64);
     s1 += __shfl_xor(s1, off, 64);
     s2 += __shfl_xor(s2, off, 64);

@@ -1173,7 +1173,7 @@ int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t*
     const bool useRhs = rhs && j == 0;
     hipLaunchKernelGGL(k_mgn_probe_x, g256(3ll * nV), dim3(256), 0, s, nV, w.s, fixed, useRhs ? rhs : nullptr,
                        M->probeSolution, mgn_mix(seed + 0x1000ull * (uint64_t)j), x);
-    hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(spmv_blocks(nV)), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals,
+    hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(spmv_blocks_for<V>(nV)), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals,
                        (const double*)nullptr, fixed,
                        x, b, nullptr, nullptr);
     bs[j] = b;

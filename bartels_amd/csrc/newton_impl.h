@@ -458,9 +458,9 @@ int objective(bh_mesh* m, const bh_newton_opts* o, const double* params, const d
     // + 1/2 inertia d^T M d, d = q - q_pred (w.z, w.Ap are free outside the PCG solve)
     const int64_t n = 3ll * m->nV;
     hipLaunchKernelGGL(k_sub, dim3((n + 255) / 256), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
-    hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
-                       o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
-                       (const PcgState*)nullptr);
+    hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks_for<double>(m->nV)), dim3(kRedBlock), 0, s, m->nV,
+                       m->bptr, m->bcol, o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z, w.Ap,
+                       (double*)nullptr, (const PcgState*)nullptr);
     st = device_dot(w.z, w.Ap, n, w.part, w.scal + 9, s);
     if (st) return st;
     hipLaunchKernelGGL(k_axpy_to, dim3(1), dim3(1), 0, s, (int64_t)1, out, 0.5 * o->inertia, w.scal + 9, out);
@@ -504,7 +504,7 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* 
               NewtonWs& w, int* iters, int* done, hipStream_t s) {
   const int nV = m->nV;
   const int gj = jac_blocks(nV);
-  const int gs = spmv_blocks(nV);
+  const int gs = spmv_blocks_for<V>(nV);
   const int64_t R = part_region(nV);
   double* pru = w.part;
   double* prr = w.part + R;
@@ -897,7 +897,7 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
                  const double* q, NewtonWs& w, int* iters, int* done, hipStream_t s) {
   const int nV = m->nV;
   const int gj = jac_blocks(nV);
-  const int gs = spmv_blocks(nV);
+  const int gs = spmv_blocks_for<V>(nV);
   const int64_t R = part_region(nV);
   Coarse c;
   int st = coarse_workspace(m, o->coarse_aggregates, gj, c);
@@ -1239,8 +1239,8 @@ extern "C" {
 int bh_spmv_device(bh_mesh* m, const double* d_vals, const double* d_x, double* d_y, bh_stream_t s) {
   if (!m || !d_vals || !d_x || !d_y) return BH_BAD_ARG;
   if (m->nV == 0) return BH_OK;
-  hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, as_stream(s), m->nV, m->bptr,
-                     m->bcol, d_vals, (const double*)nullptr, nullptr, d_x, d_y, nullptr, nullptr);
+  hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks_for<double>(m->nV)), dim3(kRedBlock), 0, as_stream(s),
+                     m->nV, m->bptr, m->bcol, d_vals, (const double*)nullptr, nullptr, d_x, d_y, nullptr, nullptr);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -1311,9 +1311,9 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
     if (st) return st;
     if (o->d_mass_vals) {  // + inertia M (q - q_pred)
       hipLaunchKernelGGL(k_sub, dim3(gn), dim3(256), 0, s, n, q, o->d_q_pred, w.z);
-      hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks(m->nV)), dim3(kRedBlock), 0, s, m->nV, m->bptr, m->bcol,
-                         o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z, w.Ap, (double*)nullptr,
-                         (const PcgState*)nullptr);
+      hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(spmv_blocks_for<double>(m->nV)), dim3(kRedBlock), 0, s, m->nV,
+                         m->bptr, m->bcol, o->d_mass_vals, (const double*)nullptr, (const uint8_t*)nullptr, w.z,
+                         w.Ap, (double*)nullptr, (const PcgState*)nullptr);
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, s, n, o->inertia, w.Ap, w.g);
     }
     hipLaunchKernelGGL(k_mask_gradient, dim3(gn), dim3(256), 0, s, n, fixed, f, w.g);
