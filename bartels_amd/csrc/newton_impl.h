@@ -537,30 +537,51 @@ int pcg_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* 
 // ---- two-level block Jacobi (BH_PRECOND_JACOBI_2L) -------------------------------------------
 // M^-1 = D^-1 + Z E^-1 Z^T: block Jacobi plus an additive coarse correction on aggregates of
 // consecutive vertices (a multiple of the vector kernel's 64 vertices each, so every workgroup of
-// that kernel lies in one aggregate).  Z holds the affine modes of each aggregate I about the
-// centroid c_I of the current positions, twelve columns: translations e_l and the linear modes
-// e_l (q_v - c_I)_m (they span the rigid rotations too); rows of fixed DOFs are zero.
-// E = Z^T A Z (12 nagg square, dense) is formed once per system and inverted in place (Gauss-Jordan
-// on the SPD matrix, no pivoting); per CG iteration the correction costs a reduction of Z^T r per
-// aggregate, one dense product with E^-1 and a vertex pass u += Z y.  On slender,
-// bending-dominated problems the affine modes of slabs carry the slow components that block Jacobi
-// resolves last (on a 96x24x24 beam, 16 affine aggregates: 271 CG iterations, 32 rigid-mode
-// aggregates with the same coarse size: 322, block Jacobi alone: 2173).  E is accumulated with
-// atomic adds (LDS per workgroup, then global), so its rounding, and with it the iterates, can
-// differ between runs in the last bits.
-constexpr int kModes = 12;
+// that kernel lies in one aggregate).  Z holds the quadratic modes of each aggregate I: column
+// (l, i) is e_l m_i(d), d = s_I (q_v - c_I) the position about the aggregate's centroid scaled by its
+// inverse RMS radius, m = (1, d0, d1, d2, d0^2, d1^2, d2^2, d0 d1, d1 d2, d0 d2): thirty columns,
+// which contain the translations, rotations and linear modes and the bending of a slab.  Rows of
+// fixed DOFs are zero.  E = Z^T A Z (30 nagg square, dense) is formed once per system and inverted
+// in place (Gauss-Jordan on the SPD matrix, no pivoting); per CG iteration the correction costs a
+// reduction of Z^T r per aggregate, one dense product with E^-1 and a vertex pass u += Z y.  On a
+// 96x24x24 beam: block Jacobi 2173 CG iterations, 16 aggregates with the 12 affine modes 271, 40
+// affine aggregates 266, 8 quadratic aggregates 198, 16 quadratic 187.  E is accumulated with
+// atomic adds, so its rounding, and with it the iterates, can differ between runs in the last bits.
+constexpr int kMono = 10;
+constexpr int kModes = 3 * kMono;  // column (l, i) = kMono l + i
 struct Coarse {
   int nagg = 0, per = 0, k = 0;
-  double *cent, *E, *g, *y, *pzr;
+  double *cent, *E, *g, *y, *pzr;  // cent: c_I and s_I, four per aggregate
 };
 
-// aggregate layout for nV vertices and a requested count (0: one per 2048 vertices, 4 to 16; on
-// the 6.07M-tet beam 16 to 128 rigid-mode aggregates gave the same iteration counts within 2%, and
-// the dense inverse of E, formed by one workgroup, costs ~20 ms per Newton step at k = 384 and
-// ~170 ms at k = 768, hence the cap)
-constexpr int kMaxAggregates = 32;
+__device__ __forceinline__ void monomials(const double d[3], double m[kMono]) {
+  m[0] = 1.0;
+  m[1] = d[0];
+  m[2] = d[1];
+  m[3] = d[2];
+  m[4] = d[0] * d[0];
+  m[5] = d[1] * d[1];
+  m[6] = d[2] * d[2];
+  m[7] = d[0] * d[1];
+  m[8] = d[1] * d[2];
+  m[9] = d[0] * d[2];
+}
+
+// the scaled position of vertex v about aggregate I's centroid
+__device__ __forceinline__ void agg_coords(const double* __restrict__ cent, int I, double q0, double q1, double q2,
+                                           double d[3]) {
+  const double* c = cent + 4 * I;
+  d[0] = (q0 - c[0]) * c[3];
+  d[1] = (q1 - c[1]) * c[3];
+  d[2] = (q2 - c[2]) * c[3];
+}
+
+// aggregate layout for nV vertices and a requested count (0: one per 4096 vertices, 4 to 8; the
+// dense inverse of E is formed by one workgroup and grows as k^3: ~20 ms per Newton step at
+// k = 384, ~170 ms at k = 768, hence the cap at k = 480)
+constexpr int kMaxAggregates = 16;
 inline void coarse_layout(int nV, int want, int& nagg, int& per) {
-  if (want <= 0) want = std::min(16, std::max(4, nV / 2048));
+  if (want <= 0) want = std::min(8, std::max(4, nV / 4096));
   want = std::min(want, kMaxAggregates);
   per = (nV + want - 1) / want;
   per = ((per + kJacVerts - 1) / kJacVerts) * kJacVerts;
@@ -571,7 +592,7 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   coarse_layout(m->nV, want, c.nagg, c.per);
   c.k = kModes * c.nagg;
   const size_t need =
-      sizeof(double) * (3 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64);
+      sizeof(double) * (4 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64);
   if (m->cwsBytes < need) {
     if (m->cws) BH_CHECK(hipFree(m->cws));
     m->cws = nullptr;
@@ -579,7 +600,7 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
     m->cwsBytes = need;
   }
   double* q = m->cws;
-  c.cent = q; q += 3 * c.nagg;
+  c.cent = q; q += 4 * c.nagg;
   c.E = q; q += (size_t)c.k * c.k;
   c.g = q; q += c.k;
   c.y = q; q += c.k;
@@ -587,11 +608,14 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   return BH_OK;
 }
 
-// centroid of q over each aggregate (one workgroup per aggregate, fixed-order sums)
+// centroid c_I and inverse RMS radius s_I of each aggregate (one workgroup per aggregate, fixed-order
+// sums)
 __global__ void __launch_bounds__(256) k_agg_centroid(int nV, int per, const double* __restrict__ q,
                                                       double* __restrict__ cent) {
   __shared__ double sh[4];
+  __shared__ double cs[3];
   const int v0 = blockIdx.x * per, v1 = min(nV, v0 + per);
+  const double cnt = (double)max(1, v1 - v0);
   double a[3] = {0.0, 0.0, 0.0};
   for (int v = v0 + threadIdx.x; v < v1; v += 256)
 #pragma unroll
@@ -599,7 +623,22 @@ __global__ void __launch_bounds__(256) k_agg_centroid(int nV, int per, const dou
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     const double t = block_sum_256(a[l], sh);
-    if (threadIdx.x == 0) cent[3 * blockIdx.x + l] = t / (double)max(1, v1 - v0);
+    if (threadIdx.x == 0) cs[l] = t / cnt;
+  }
+  __syncthreads();
+  double r2 = 0.0;
+  for (int v = v0 + threadIdx.x; v < v1; v += 256)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const double d = q[3ll * v + l] - cs[l];
+      r2 += d * d;
+    }
+  r2 = block_sum_256(r2, sh);
+  if (threadIdx.x == 0) {
+    const double rms = sqrt(r2 / cnt);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) cent[4 * blockIdx.x + l] = cs[l];
+    cent[4 * blockIdx.x + 3] = rms > 0.0 ? 1.0 / rms : 1.0;
   }
 }
 
@@ -608,102 +647,116 @@ __global__ void __launch_bounds__(256) k_fill_zero(int64_t n, double* __restrict
   if (i < n) a[i] = 0.0;
 }
 
-// B += A_vn Z_n for the affine modes of n (3 x 12, row-major): column l is A e_l, column 3 + 3 l + m
-// is A e_l d_m
-__device__ __forceinline__ void affine_right(const double A[9], const double d[3], double B[36]) {
+// the masked block A_vn of the CG operator: fixed rows and columns zero (Z's rows of fixed DOFs are
+// zero), the diagonal block from hdiag when given.  Masked entries are selected at the load (a
+// conditional store into the array after the loads left the fixed-column mask without effect in the
+// compiled code).
+template <typename V>
+__device__ __forceinline__ void masked_row_block(const V* __restrict__ vals, const double* __restrict__ hdiag,
+                                                 const uint8_t* __restrict__ fixed, int v, int n, int b0, int nb,
+                                                 int r, const bool fv[3], double A[9]) {
+  bool fn[3];
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      const double a = A[3 * r + l];
-      B[12 * r + l] += a;
-#pragma unroll
-      for (int m = 0; m < 3; ++m) B[12 * r + 3 + 3 * l + m] += a * d[m];
-    }
-}
-
-// dst (12 x 12) += Z_v^T B: row l is B's row l, row 3 + 3 l + m is d_m times B's row l
-__device__ __forceinline__ void affine_left_add(const double B[36], const double d[3], double* dst, int ld) {
+  for (int c = 0; c < 3; ++c) fn[c] = fixed && fixed[3ll * n + c];
+  const double* src = (n == v && hdiag) ? hdiag + 9ll * v : nullptr;
 #pragma unroll
   for (int l = 0; l < 3; ++l)
 #pragma unroll
-    for (int c = 0; c < kModes; ++c) {
-      const double b = B[12 * l + c];
-      atomicAdd(&dst[(size_t)l * ld + c], b);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) atomicAdd(&dst[(size_t)(3 + 3 * l + m) * ld + c], d[m] * b);
+    for (int c = 0; c < 3; ++c) {
+      const double a = src ? src[3 * l + c] : (double)vals[9ll * b0 + 3ll * l * nb + 3 * r + c];
+      A[3 * l + c] = (fv[l] || fn[c]) ? 0.0 : a;
     }
 }
 
-// E += Z^T A Z: one lane per row v (64 rows per workgroup, all in aggregate I).  The blocks of
-// neighbours n in the same aggregate are summed in registers (B = sum A_vn Z_n) and added once per
-// lane; the few neighbours in other aggregates go to LDS slots (J = I +- 1) or straight to global
-// memory.  The masked operator of the CG: fixed rows and columns zero (Z's rows of fixed DOFs are
-// zero), the diagonal block from hdiag when given.
+// B += A_vn Z_n (3 x 30, row-major): column (l, i) is A e_l m_i(d_n)
+__device__ __forceinline__ void quad_right(const double A[9], const double mn[kMono], double B[3 * kModes]) {
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const double a = A[3 * rr + l];
+#pragma unroll
+      for (int i = 0; i < kMono; ++i) B[kModes * rr + kMono * l + i] += a * mn[i];
+    }
+}
+
+// E += Z^T A Z.  A workgroup takes 64 rows v of one aggregate I; for each neighbouring aggregate
+// J = I - 1, I, I + 1 every lane sums B_v = sum_{n in J} A_vn Z_n in registers, the 64 B_v and the
+// rows' monomials meet in LDS, and the block E(I, J) += sum_v Z_v^T B_v (entry ((l, i), col) =
+// sum_v m_i(v) B_v[l][col]) is contracted there, one atomic add per entry and workgroup.  Blocks of
+// aggregates farther away (none on slab meshes) are added per block.
 template <typename V>
 __global__ void __launch_bounds__(64)
     k_coarse_E(int nV, int per, int nagg, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                const V* __restrict__ vals, const double* __restrict__ hdiag, const uint8_t* __restrict__ fixed,
                const double* __restrict__ q, const double* __restrict__ cent, double* __restrict__ E) {
-  constexpr int kk2 = kModes * kModes;
-  __shared__ double acc[3][kk2];  // J = I - 1, I, I + 1
-  for (int t = threadIdx.x; t < 3 * kk2; t += 64) acc[t / kk2][t % kk2] = 0.0;
-  __syncthreads();
-  const int v = blockIdx.x * 64 + threadIdx.x;
+  __shared__ double ms[64][kMono];
+  __shared__ double bs[64][3 * kModes + 1];
+  const int t = threadIdx.x;
+  const int v = blockIdx.x * 64 + t;
   const int I = (blockIdx.x * 64) / per;
   const int k = kModes * nagg;
-  if (v < nV) {
-    const int b0 = bptr[v], nb = bptr[v + 1] - b0;
-    double dv[3];
+  const bool in = v < nV;
+  int b0 = 0, nb = 0;
+  bool fv[3] = {false, false, false};
+  double mv[kMono];
 #pragma unroll
-    for (int l = 0; l < 3; ++l) dv[l] = q[3ll * v + l] - cent[3 * I + l];
-    bool fv[3];
+  for (int i = 0; i < kMono; ++i) mv[i] = 0.0;
+  if (in) {
+    b0 = bptr[v];
+    nb = bptr[v + 1] - b0;
 #pragma unroll
     for (int l = 0; l < 3; ++l) fv[l] = fixed && fixed[3ll * v + l];
-    double Bs[36];  // same-aggregate sum
+    double d[3];
+    agg_coords(cent, I, q[3ll * v], q[3ll * v + 1], q[3ll * v + 2], d);
+    monomials(d, mv);
+  }
 #pragma unroll
-    for (int t = 0; t < 36; ++t) Bs[t] = 0.0;
+  for (int i = 0; i < kMono; ++i) ms[t][i] = mv[i];
+  for (int J = max(0, I - 1); J <= min(nagg - 1, I + 1); ++J) {
+    double B[3 * kModes];
+#pragma unroll
+    for (int e = 0; e < 3 * kModes; ++e) B[e] = 0.0;
     for (int r = 0; r < nb; ++r) {
       const int n = bcol[b0 + r];
-      const int J = n / per;
-      bool fn[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) fn[c] = fixed && fixed[3ll * n + c];
-      // masked entries are selected at the load (a conditional store into the array after the loads
-      // left the fixed-column mask without effect in the compiled code)
-      const double* src = (n == v && hdiag) ? hdiag + 9ll * v : nullptr;
+      if (n / per != J) continue;
       double A[9];
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double a = src ? src[3 * l + c] : (double)vals[9ll * b0 + 3ll * l * nb + 3 * r + c];
-          A[3 * l + c] = (fv[l] || fn[c]) ? 0.0 : a;
-        }
-      double dn[3];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) dn[l] = q[3ll * n + l] - cent[3 * J + l];
-      if (J == I) {
-        affine_right(A, dn, Bs);
-      } else {
-        double B[36];
-#pragma unroll
-        for (int t = 0; t < 36; ++t) B[t] = 0.0;
-        affine_right(A, dn, B);
-        const int slot = J - I + 1;
-        if (slot == 0 || slot == 2)
-          affine_left_add(B, dv, acc[slot], kModes);
-        else
-          affine_left_add(B, dv, E + (size_t)kModes * I * k + kModes * J, k);
-      }
+      masked_row_block(vals, hdiag, fixed, v, n, b0, nb, r, fv, A);
+      double d[3], mn[kMono];
+      agg_coords(cent, J, q[3ll * n], q[3ll * n + 1], q[3ll * n + 2], d);
+      monomials(d, mn);
+      quad_right(A, mn, B);
     }
-    affine_left_add(Bs, dv, acc[1], kModes);
+    __syncthreads();  // the previous contraction has read bs
+#pragma unroll
+    for (int e = 0; e < 3 * kModes; ++e) bs[t][e] = B[e];
+    __syncthreads();
+    for (int o = t; o < kModes * kModes; o += 64) {
+      const int row = o / kModes, col = o % kModes;
+      const int l = row / kMono, i = row % kMono;
+      double a = 0.0;
+      for (int s = 0; s < 64; ++s) a += ms[s][i] * bs[s][kModes * l + col];
+      atomicAdd(&E[(size_t)(kModes * I + row) * k + kModes * J + col], a);
+    }
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < 3 * kk2; t += 64) {
-    const int J = I - 1 + t / kk2, e = t % kk2;
-    if (J >= 0 && J < nagg)
-      atomicAdd(&E[(size_t)(kModes * I + e / kModes) * k + kModes * J + e % kModes], acc[t / kk2][e]);
+  // aggregates beyond the neighbours (general vertex orders): per block, straight to E
+  for (int r = 0; r < nb; ++r) {
+    const int n = bcol[b0 + r];
+    const int J = n / per;
+    if (J >= I - 1 && J <= I + 1) continue;
+    double A[9];
+    masked_row_block(vals, hdiag, fixed, v, n, b0, nb, r, fv, A);
+    double d[3], mn[kMono];
+    agg_coords(cent, J, q[3ll * n], q[3ll * n + 1], q[3ll * n + 2], d);
+    monomials(d, mn);
+    double B[3 * kModes];
+#pragma unroll
+    for (int e = 0; e < 3 * kModes; ++e) B[e] = 0.0;
+    quad_right(A, mn, B);
+    for (int l = 0; l < 3; ++l)
+      for (int i = 0; i < kMono; ++i)
+        for (int col = 0; col < kModes; ++col)
+          atomicAdd(&E[(size_t)(kModes * I + kMono * l + i) * k + kModes * J + col], mv[i] * B[kModes * l + col]);
   }
 }
 
@@ -801,27 +854,49 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   if (!in || (fixed && fixed[i])) return;
   const int v = (int)(i / 3), l = (int)(i % 3);
   const int I = v / per;
-  const double* a = y + kModes * I;
-  const double* c = cent + 3 * I;
   const int base = threadIdx.x - l;
-  u[i] += a[l] + a[3 + 3 * l] * (qs[base] - c[0]) + a[4 + 3 * l] * (qs[base + 1] - c[1]) +
-          a[5 + 3 * l] * (qs[base + 2] - c[2]);
+  double d[3], m[kMono];
+  agg_coords(cent, I, qs[base], qs[base + 1], qs[base + 2], d);
+  monomials(d, m);
+  const double* a = y + kModes * I + kMono * l;
+  double z = 0.0;
+#pragma unroll
+  for (int c = 0; c < kMono; ++c) z += a[c] * m[c];
+  u[i] += z;
+}
+
+// monomial c of d (one entry of monomials())
+__device__ __forceinline__ double monomial(const double d[3], int c) {
+  switch (c) {
+    case 0: return 1.0;
+    case 1: return d[0];
+    case 2: return d[1];
+    case 3: return d[2];
+    case 4: return d[0] * d[0];
+    case 5: return d[1] * d[1];
+    case 6: return d[2] * d[2];
+    case 7: return d[0] * d[1];
+    case 8: return d[1] * d[2];
+    default: return d[0] * d[2];
+  }
 }
 
 // the vector kernel of block-Jacobi PCG plus the partials of Z^T r per workgroup (r of fixed DOFs
 // is zero: b is masked and the fixed rows of the operator are the identity).  Z^T r of the
-// workgroup is the 3 x 4 sum of r_v (x) (1, d_v): every row writes r (1, d) to LDS, then 16 lanes
-// per entry sum it (4 values each, then a 16-lane butterfly)
+// workgroup is the 3 x 10 sum of r_v (x) m(d_v): r and q of the 64 vertices meet in LDS, six lanes
+// per entry sum it in strides (forming the monomial themselves), and one lane per entry adds the six
+// partials
 __global__ void __launch_bounds__(3 * kJacVerts)
     k_cg_vec_jacobi_2l(int nV, int start, int per, const double* __restrict__ b, const double* __restrict__ Dinv,
                        double* __restrict__ x, double* __restrict__ r, double* __restrict__ u, double* __restrict__ p,
                        double* __restrict__ s, const double* __restrict__ w, const double* __restrict__ q,
                        const double* __restrict__ cent, double* __restrict__ part_ru, double* __restrict__ part_rr,
                        double* __restrict__ pzr, const PcgState* __restrict__ st) {
-  static_assert(kModes * 16 == 3 * kJacVerts, "one 16-lane group per coarse entry");
+  constexpr int kSub = 6;
+  static_assert(kModes * kSub <= 3 * kJacVerts, "six lanes per coarse entry");
   __shared__ double rs[3 * kJacVerts];
   __shared__ double qs[3 * kJacVerts];
-  __shared__ double zs[3 * kJacVerts][4];
+  __shared__ double red[kModes][kSub];
   __shared__ double sh[3][2];
   if (!start && cg_skip(st)) return;
   const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
@@ -848,7 +923,6 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   qs[threadIdx.x] = (i < n) ? q[i] : 0.0;
   __syncthreads();
   double ru = 0.0, rr = 0.0;
-  double z[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     const int l = (int)(i % 3);
     const int base = threadIdx.x - l;
@@ -857,13 +931,7 @@ __global__ void __launch_bounds__(3 * kJacVerts)
     u[i] = ui;
     ru = ri * ui;
     rr = ri * ri;
-    const int I = (int)(i / 3) / per;
-    z[0] = ri;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) z[1 + m] = ri * (qs[base + m] - cent[3 * I + m]);
   }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) zs[threadIdx.x][c] = z[c];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     ru += __shfl_xor(ru, off, 64);
@@ -874,15 +942,27 @@ __global__ void __launch_bounds__(3 * kJacVerts)
     sh[threadIdx.x >> 6][1] = rr;
   }
   __syncthreads();
-  // entry Q = (l, c) of the 3 x 4 sum: lanes 16 Q .. 16 Q + 15, lane j takes vertices j, j + 16, ...
-  const int Q = threadIdx.x >> 4, j = threadIdx.x & 15;
-  const int l = Q >> 2, c = Q & 3;
-  double a = 0.0;
+  // entry Q = (l, c) of the 3 x 10 sum: lanes kSub Q .. kSub Q + kSub - 1, lane j takes vertices
+  // j, j + kSub, ...
+  if (threadIdx.x < kModes * kSub) {
+    const int Q = threadIdx.x / kSub, j = threadIdx.x % kSub;
+    const int l = Q / kMono, c = Q % kMono;
+    const int I = (blockIdx.x * kJacVerts) / per;  // the workgroup's aggregate
+    double a = 0.0;
+    for (int vv = j; vv < kJacVerts; vv += kSub) {
+      double d[3];
+      agg_coords(cent, I, qs[3 * vv], qs[3 * vv + 1], qs[3 * vv + 2], d);
+      a += rs[3 * vv + l] * monomial(d, c);  // r is zero beyond the mesh
+    }
+    red[Q][j] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < kModes) {
+    double a = 0.0;
 #pragma unroll
-  for (int t = 0; t < kJacVerts / 16; ++t) a += zs[3 * (j + 16 * t) + l][c];
-#pragma unroll
-  for (int off = 8; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-  if (j == 0) pzr[(int64_t)kModes * blockIdx.x + (c == 0 ? l : 3 + 3 * l + c - 1)] = a;
+    for (int j = 0; j < kSub; ++j) a += red[threadIdx.x][j];
+    pzr[(int64_t)kModes * blockIdx.x + threadIdx.x] = a;
+  }
   if (threadIdx.x == 0) {
     part_ru[blockIdx.x] = sh[0][0] + sh[1][0] + sh[2][0];
     part_rr[blockIdx.x] = sh[0][1] + sh[1][1] + sh[2][1];

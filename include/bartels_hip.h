@@ -189,10 +189,10 @@ typedef enum {
   BH_PRECOND_LEARNED_SGS = 6,  /* M = (Dt + L) Dt^-1 (Dt + L)^T with the block diagonal Dt written by the
                                   mesh-graph network opts.mgn (Dt = D at zero decoders) */
   BH_PRECOND_JACOBI_2L = 7     /* two-level: block Jacobi plus an additive coarse correction Z E^-1 Z^T,
-                                  Z the 12 affine modes (translations and linear maps about the
-                                  centroid of q) of each aggregate of consecutive vertices,
-                                  E = Z^T H Z; no precond_handle needed (coarse_aggregates sets the
-                                  aggregate count) */
+                                  Z the 30 quadratic modes (each displacement component times 1,
+                                  d, d^2 and the products of d, d = q about the centroid) of each
+                                  aggregate of consecutive vertices, E = Z^T H Z; no precond_handle
+                                  needed (coarse_aggregates sets the aggregate count) */
 } bh_precond_kind;
 
 /* Block lower-triangular preconditioner M = L L^T on the Hessian's own 3x3-block pattern, in a
@@ -308,8 +308,8 @@ typedef struct {
   const double* d_q_pred;
   double inertia;
   double relaxation;       /* BH_PRECOND_SGS: SSOR relaxation w in (0, 2), Dt = D / w (0 means 1) */
-  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L: number of aggregates (0: one per 2048 vertices,
-                              4 to 16; larger requests are capped at 32) */
+  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L: number of aggregates (0: one per 4096 vertices,
+                              4 to 8; larger requests are capped at 16) */
 } bh_newton_opts;
 
 typedef struct {
