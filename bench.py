@@ -71,7 +71,7 @@ def newton_steps(p, mesh, steps, pcg_max):
     """Newton-step wall time on the same beam (bent state, gravity, x = 0 clamped, PSD 1e-3, PCG to a
     1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
     precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form),
-    two-level block Jacobi (rigid-mode coarse space on vertex aggregates, also in FP64) and the
+    two-level block Jacobi (quadratic coarse space on vertex aggregates, also in FP64) and the
     learned Eisenstat diagonal (when trained weights are present)."""
     import torch
 

@@ -120,7 +120,7 @@ def gravity_problem(oracle):
 
 def test_newton_precision_form_and_preconditioners(gravity_problem):
     """FP64 block-Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and two-level block
-    Jacobi (affine coarse correction), FP64 and the FP32 precision form, reach the oracle's
+    Jacobi (quadratic coarse correction), FP64 and the FP32 precision form, reach the oracle's
     equilibrium; the precision form within 1e-5 relative displacement.  IC(0), Gauss-Seidel and the
     two-level form need fewer CG iterations than block Jacobi."""
     from bartels_amd import _lib
