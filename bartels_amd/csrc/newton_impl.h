@@ -371,23 +371,36 @@ __global__ void __launch_bounds__(256)
     k_implied_diag(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
                    const V* __restrict__ vals, const int32_t* __restrict__ vdeg, double psd_tol,
                    const double* __restrict__ mass, double inertia, double* __restrict__ hdiag) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nV) return;
-  const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+  // eight lanes per vertex, lane j takes blocks j, j + 8, ... of the block row (neighbouring 3-value
+  // pieces of each row), then a fixed xor butterfly over the eight lanes
+  constexpr int L = 8;
+  const int64_t gid = blockIdx.x * 256ll + threadIdx.x;
+  const int v = (int)(gid / L), j = (int)(gid % L);
   double S[9], Ms[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) S[i] = Ms[i] = 0.0;
-  for (int r = 0; r < nb; ++r) {
-    const bool off = bcol[b0 + r] != v;
+  if (v < nV) {
+    const int b0 = bptr[v], nb = bptr[v + 1] - b0;
+    for (int r = j; r < nb; r += L) {
+      const bool off = bcol[b0 + r] != v;
 #pragma unroll
-    for (int l = 0; l < 3; ++l)
+      for (int l = 0; l < 3; ++l)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int64_t at = 9ll * b0 + 3ll * l * nb + 3 * r + k;
-        if (off) S[3 * l + k] += (double)vals[at];
-        if (mass) Ms[3 * l + k] += mass[at];
-      }
+        for (int k = 0; k < 3; ++k) {
+          const int64_t at = 9ll * b0 + 3ll * l * nb + 3 * r + k;
+          if (off) S[3 * l + k] += (double)vals[at];
+          if (mass) Ms[3 * l + k] += mass[at];
+        }
+    }
   }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      S[i] += __shfl_xor(S[i], off, 64);
+      Ms[i] += __shfl_xor(Ms[i], off, 64);
+    }
+  if (v >= nV || j != 0) return;
   const double g = psd_tol * (double)vdeg[v];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
@@ -551,7 +564,8 @@ constexpr int kMono = 10;
 constexpr int kModes = 3 * kMono;  // column (l, i) = kMono l + i
 struct Coarse {
   int nagg = 0, per = 0, k = 0;
-  double *cent, *E, *g, *y, *pzr;  // cent: c_I and s_I, four per aggregate
+  double *cent, *E, *g, *y, *pzr;  // cent: c_I and s_I, four per aggregate; pzr mode-major
+  float* dinv;                     // the block-Jacobi inverse in FP32 (exactly symmetric blocks)
 };
 
 __device__ __forceinline__ void monomials(const double d[3], double m[kMono]) {
@@ -591,8 +605,8 @@ inline void coarse_layout(int nV, int want, int& nagg, int& per) {
 int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   coarse_layout(m->nV, want, c.nagg, c.per);
   c.k = kModes * c.nagg;
-  const size_t need =
-      sizeof(double) * (4 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64);
+  const size_t nd = 4 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64;
+  const size_t need = sizeof(double) * nd + sizeof(float) * 9 * (size_t)m->nV;
   if (m->cwsBytes < need) {
     if (m->cws) BH_CHECK(hipFree(m->cws));
     m->cws = nullptr;
@@ -605,6 +619,7 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   c.g = q; q += c.k;
   c.y = q; q += c.k;
   c.pzr = q;
+  c.dinv = (float*)(m->cws + nd);
   return BH_OK;
 }
 
@@ -645,6 +660,11 @@ __global__ void __launch_bounds__(256) k_agg_centroid(int nV, int per, const dou
 __global__ void __launch_bounds__(256) k_fill_zero(int64_t n, double* __restrict__ a) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_narrow(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // the masked block A_vn of the CG operator: fixed rows and columns zero (Z's rows of fixed DOFs are
@@ -806,7 +826,7 @@ __global__ void __launch_bounds__(256) k_coarse_g(int per, int nblocks, const do
   const int I = blockIdx.x / kModes, c = blockIdx.x % kModes;
   const int w0 = I * (per / kJacVerts), w1 = min(nblocks, w0 + per / kJacVerts);
   double a = 0.0;
-  for (int w = w0 + threadIdx.x; w < w1; w += 256) a += pzr[(int64_t)kModes * w + c];
+  for (int w = w0 + threadIdx.x; w < w1; w += 256) a += pzr[(int64_t)c * nblocks + w];
   a = block_sum_256(a, sh);
   if (threadIdx.x == 0) g[blockIdx.x] = a;
 }
@@ -887,7 +907,7 @@ __device__ __forceinline__ double monomial(const double d[3], int c) {
 // per entry sum it in strides (forming the monomial themselves), and one lane per entry adds the six
 // partials
 __global__ void __launch_bounds__(3 * kJacVerts)
-    k_cg_vec_jacobi_2l(int nV, int start, int per, const double* __restrict__ b, const double* __restrict__ Dinv,
+    k_cg_vec_jacobi_2l(int nV, int start, int per, const double* __restrict__ b, const float* __restrict__ Dinv,
                        double* __restrict__ x, double* __restrict__ r, double* __restrict__ u, double* __restrict__ p,
                        double* __restrict__ s, const double* __restrict__ w, const double* __restrict__ q,
                        const double* __restrict__ cent, double* __restrict__ part_ru, double* __restrict__ part_rr,
@@ -926,8 +946,8 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   if (i < n) {
     const int l = (int)(i % 3);
     const int base = threadIdx.x - l;
-    const double* D = Dinv + 3 * i;
-    const double ui = D[0] * rs[base] + D[1] * rs[base + 1] + D[2] * rs[base + 2];
+    const float* D = Dinv + 3 * i;
+    const double ui = (double)D[0] * rs[base] + (double)D[1] * rs[base + 1] + (double)D[2] * rs[base + 2];
     u[i] = ui;
     ru = ri * ui;
     rr = ri * ri;
@@ -961,7 +981,7 @@ __global__ void __launch_bounds__(3 * kJacVerts)
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < kSub; ++j) a += red[threadIdx.x][j];
-    pzr[(int64_t)kModes * blockIdx.x + threadIdx.x] = a;
+    pzr[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = a;  // mode-major: k_coarse_g reads runs
   }
   if (threadIdx.x == 0) {
     part_ru[blockIdx.x] = sh[0][0] + sh[1][0] + sh[2][0];
@@ -989,6 +1009,8 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
   double* pwu = w.part + 2 * R;
   hipLaunchKernelGGL(k_block_jacobi<V>, dim3((nV + 255) / 256), dim3(256), 0, s, nV, m->bptr, m->bcol, vals, hdiag,
                      fixed, 0, w.Dinv);
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((9ll * nV + 255) / 256)), dim3(256), 0, s, 9ll * nV,
+                     (const double*)w.Dinv, c.dinv);
   hipLaunchKernelGGL(k_agg_centroid, dim3(c.nagg), dim3(256), 0, s, nV, c.per, q, c.cent);
   const int64_t kk = (int64_t)c.k * c.k;
   hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, s, kk, c.E);
@@ -998,8 +1020,9 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
   BH_CHECK(hipGetLastError());
   auto precond_tail = [&](int start) {
     const PcgState* sk = start ? nullptr : w.st;  // kernels skip once the solve is done
-    hipLaunchKernelGGL(k_cg_vec_jacobi_2l, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, start, c.per, w.b, w.Dinv, w.x,
-                       w.r, w.z, w.p, w.s, w.Ap, q, (const double*)c.cent, pru, prr, c.pzr, w.st);
+    hipLaunchKernelGGL(k_cg_vec_jacobi_2l, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, start, c.per, w.b,
+                       (const float*)c.dinv, w.x, w.r, w.z, w.p, w.s, w.Ap, q, (const double*)c.cent, pru, prr,
+                       c.pzr, w.st);
     hipLaunchKernelGGL(k_coarse_g, dim3(c.k), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
     hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
                        (const double*)c.g, c.y, pru + gj, sk);
@@ -1255,9 +1278,9 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
   // precision form: FP64 diagonal blocks rebuilt from the FP32 off-diagonal blocks (exact row sums)
   const double* hdiag = nullptr;
   if (sizeof(V) == 4 && m->nV) {
-    hipLaunchKernelGGL(k_implied_diag<V>, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, m->bptr, m->bcol,
-                       (const V*)vals, m->vdeg, psd ? psd->tol : 0.0, o->d_mass_vals, o->d_mass_vals ? o->inertia : 0.0,
-                       w.hdiag);
+    hipLaunchKernelGGL(k_implied_diag<V>, dim3((unsigned)((8ll * m->nV + 255) / 256)), dim3(256), 0, s, m->nV,
+                       m->bptr, m->bcol, (const V*)vals, m->vdeg, psd ? psd->tol : 0.0, o->d_mass_vals,
+                       o->d_mass_vals ? o->inertia : 0.0, w.hdiag);
     BH_CHECK(hipGetLastError());
     hdiag = w.hdiag;
   }
