@@ -798,21 +798,38 @@ __global__ void __launch_bounds__(1024) k_coarse_inv(int k, double* __restrict__
     __syncthreads();
     const double d = col[p];
     const double piv = (d > 1e-12 * dg[p] && d > 0.0) ? 1.0 / d : 0.0;
-    // wave wv takes rows wv, wv + 16, ...; its lanes the columns
-    for (int i = (int)threadIdx.x >> 6; i < k; i += 16) {
-      double* Ei = E + (size_t)i * k;
-      const double ci = col[i];
-      for (int j = (int)threadIdx.x & 63; j < k; j += 64) {
-        if (i == p && j == p)
-          Ei[j] = piv;
-        else if (i == p)
-          Ei[j] = row[j] * piv;
-        else if (j == p)
-          Ei[j] = -ci * piv;
-        else
-          Ei[j] -= ci * row[j] * piv;
+    // wave wv takes rows wv, wv + 16, ...; its lanes the columns.  A thread loads a 4 x 4 tile of its
+    // entries before it updates them (16 loads in flight instead of one)
+    constexpr int R = 4, C = 4;
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    for (int i0 = wv; i0 < k; i0 += 16 * R)
+      for (int j0 = lane; j0 < k; j0 += 64 * C) {
+        double a[R][C];
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+#pragma unroll
+          for (int y = 0; y < C; ++y) {
+            const int i = i0 + 16 * x, j = j0 + 64 * y;
+            a[x][y] = (i < k && j < k) ? E[(size_t)i * k + j] : 0.0;
+          }
+#pragma unroll
+        for (int x = 0; x < R; ++x)
+#pragma unroll
+          for (int y = 0; y < C; ++y) {
+            const int i = i0 + 16 * x, j = j0 + 64 * y;
+            if (i >= k || j >= k) continue;
+            double e;
+            if (i == p && j == p)
+              e = piv;
+            else if (i == p)
+              e = row[j] * piv;
+            else if (j == p)
+              e = -col[i] * piv;
+            else
+              e = a[x][y] - col[i] * row[j] * piv;
+            E[(size_t)i * k + j] = e;
+          }
       }
-    }
     __syncthreads();
   }
 }
