@@ -179,6 +179,35 @@ def test_two_level_aggregate_counts(gravity_problem):
             assert rep.pcg_iterations_total < rep_j.pcg_iterations_total
 
 
+def test_two_level_scattered_vertex_order(gravity_problem):
+    """Two-level block Jacobi on the same beam with its vertices renumbered at random: most Hessian
+    blocks then couple aggregates that are not neighbours in the vertex order (added to E one block
+    at a time).  The coarse space is then a poor one, but the solve must still reach the oracle's
+    equilibrium to 1e-8, with no PCG failure."""
+    from bartels_amd import tetmesh as tm
+
+    p, mesh, q0, fixed, f, q_or = gravity_problem
+    nV = p.V.shape[0]
+    perm = np.random.default_rng(7).permutation(nV)  # new vertex i is old vertex perm[i]
+    inv = np.empty(nV, np.int64)
+    inv[perm] = np.arange(nV)
+    E2 = inv[p.E].astype(np.int32)
+    V2 = p.V[perm]
+    dX, vol = tm.linear_tetmesh_dphi_dX(V2, E2)
+    mesh2 = tm.TetMesh(nV, E2, dX, vol, V=V2)
+    try:
+        dof = (3 * perm[:, None] + np.arange(3)).ravel()  # new DOF j is old DOF dof[j]
+        _, rep_j, _ = _newton(mesh2, p, q0[dof], fixed[dof], f[dof])
+        q, rep, _ = _newton(mesh2, p, q0[dof], fixed[dof], f[dof], precond=7, aggregates=8)
+        err = np.linalg.norm(q - q_or[dof]) / np.linalg.norm(q_or - q0)
+        print(f"scattered order: rel err {err:.2e}, pcg its {rep.pcg_iterations_total} "
+              f"(block Jacobi {rep_j.pcg_iterations_total})")
+        assert err < 1e-8
+        assert rep.pcg_failures == 0
+    finally:
+        mesh2.close()
+
+
 def test_newton_jacobi_fallback_at_iteration_cap(gravity_problem):
     """a factor-preconditioned solve that hits its cap is redone with block-Jacobi PCG"""
     from bartels_amd import _lib
