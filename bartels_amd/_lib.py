@@ -73,6 +73,7 @@ class BhNewtonOpts(ctypes.Structure):
         ("inertia", c_double),
         ("relaxation", c_double),
         ("coarse_aggregates", c_int),
+        ("coarse_correction", c_int),
     ]
 
 

@@ -566,6 +566,7 @@ struct Coarse {
   int nagg = 0, per = 0, k = 0;
   double *cent, *E, *g, *y, *pzr;  // cent: c_I and s_I, four per aggregate; pzr mode-major
   float* dinv;                     // the block-Jacobi inverse in FP32 (exactly symmetric blocks)
+  int32_t* pos;                    // vertex -> position of a colour order (the factor solvers)
 };
 
 __device__ __forceinline__ void monomials(const double d[3], double m[kMono]) {
@@ -606,7 +607,7 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   coarse_layout(m->nV, want, c.nagg, c.per);
   c.k = kModes * c.nagg;
   const size_t nd = 4 * (size_t)c.nagg + (size_t)c.k * c.k + 2 * (size_t)c.k + kModes * (size_t)gj + 64;
-  const size_t need = sizeof(double) * nd + sizeof(float) * 9 * (size_t)m->nV;
+  const size_t need = sizeof(double) * nd + sizeof(float) * 9 * (size_t)m->nV + sizeof(int32_t) * (size_t)m->nV;
   if (m->cwsBytes < need) {
     if (m->cws) BH_CHECK(hipFree(m->cws));
     m->cws = nullptr;
@@ -620,6 +621,7 @@ int coarse_workspace(bh_mesh* m, int want, int gj, Coarse& c) {
   c.y = q; q += c.k;
   c.pzr = q;
   c.dinv = (float*)(m->cws + nd);
+  c.pos = (int32_t*)(c.dinv + 9 * (size_t)m->nV);
   return BH_OK;
 }
 
@@ -1006,6 +1008,93 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   }
 }
 
+// ---- the coarse correction for the colour-order (factor) solvers -----------------------------
+// Their vectors live in the solve order (position i holds vertex colVerts[i]); these kernels walk
+// the vertices in vertex order, 64 per workgroup as in the block-Jacobi form, and reach a vertex's
+// rows through pos = colVerts^-1.
+
+__global__ void __launch_bounds__(256) k_inverse_order(int nV, const int32_t* __restrict__ colVerts,
+                                                       int32_t* __restrict__ pos) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nV) pos[colVerts[i]] = i;
+}
+
+// the partials of Z^T r per workgroup of 64 vertices (r in the solve order), mode-major
+__global__ void __launch_bounds__(3 * kJacVerts)
+    k_coarse_zr_order(int nV, int per, const int32_t* __restrict__ pos, const double* __restrict__ r,
+                      const double* __restrict__ q, const double* __restrict__ cent, double* __restrict__ pzr,
+                      const PcgState* __restrict__ st) {
+  constexpr int kSub = 6;
+  __shared__ double rs[3 * kJacVerts];
+  __shared__ double qs[3 * kJacVerts];
+  __shared__ double red[kModes][kSub];
+  if (st && cg_skip(st)) return;
+  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
+  const bool in = i < 3ll * nV;
+  rs[threadIdx.x] = in ? r[3ll * pos[i / 3] + i % 3] : 0.0;
+  qs[threadIdx.x] = in ? q[i] : 0.0;
+  __syncthreads();
+  if (threadIdx.x < kModes * kSub) {
+    const int Q = threadIdx.x / kSub, j = threadIdx.x % kSub;
+    const int l = Q / kMono, c = Q % kMono;
+    const int I = (blockIdx.x * kJacVerts) / per;
+    double a = 0.0;
+    for (int vv = j; vv < kJacVerts; vv += kSub) {
+      double d[3];
+      agg_coords(cent, I, qs[3 * vv], qs[3 * vv + 1], qs[3 * vv + 2], d);
+      a += rs[3 * vv + l] * monomial(d, c);
+    }
+    red[Q][j] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < kModes) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < kSub; ++j) a += red[threadIdx.x][j];
+    pzr[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+  }
+}
+
+// u += Z y with u in the solve order (rows of fixed DOFs untouched)
+__global__ void __launch_bounds__(3 * kJacVerts)
+    k_coarse_add_order(int nV, int per, const int32_t* __restrict__ pos, const double* __restrict__ q,
+                       const double* __restrict__ cent, const double* __restrict__ y,
+                       const uint8_t* __restrict__ fixed, double* __restrict__ u, const PcgState* __restrict__ st) {
+  __shared__ double qs[3 * kJacVerts];
+  if (st && cg_skip(st)) return;
+  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
+  const bool in = i < 3ll * nV;
+  qs[threadIdx.x] = in ? q[i] : 0.0;
+  __syncthreads();
+  if (!in || (fixed && fixed[i])) return;
+  const int v = (int)(i / 3), l = (int)(i % 3);
+  const int I = v / per;
+  const int base = threadIdx.x - l;
+  double d[3], m[kMono];
+  agg_coords(cent, I, qs[base], qs[base + 1], qs[base + 2], d);
+  monomials(d, m);
+  const double* a = y + kModes * I + kMono * l;
+  double z = 0.0;
+#pragma unroll
+  for (int c = 0; c < kMono; ++c) z += a[c] * m[c];
+  u[3ll * pos[v] + l] += z;
+}
+
+// E = Z^T A Z and its inverse for the current system (vertex order, CSR operator)
+template <typename V>
+int coarse_setup(bh_mesh* m, const V* vals, const double* hdiag, const uint8_t* fixed, const double* q,
+                 const Coarse& c, hipStream_t s) {
+  const int nV = m->nV;
+  hipLaunchKernelGGL(k_agg_centroid, dim3(c.nagg), dim3(256), 0, s, nV, c.per, q, c.cent);
+  const int64_t kk = (int64_t)c.k * c.k;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, s, kk, c.E);
+  hipLaunchKernelGGL((k_coarse_E<V>), dim3((nV + 63) / 64), dim3(64), 0, s, nV, c.per, c.nagg, m->bptr, m->bcol, vals,
+                     hdiag, fixed, q, (const double*)c.cent, c.E);
+  hipLaunchKernelGGL(k_coarse_inv, dim3(1), dim3(1024), sizeof(double) * 3 * c.k, s, c.k, c.E);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
 // Two-level block-Jacobi PCG in vertex order on the CSR operator (FP64 or FP32 values, optional
 // FP64 diagonal).  The FP32 CSR operator is exactly symmetric when the Hessian came through the
 // PSD records; the element-math gathers leave it symmetric to FP32 rounding.
@@ -1227,7 +1316,7 @@ int pcg_solve_eisenstat(bh_mesh* m, const bh_newton_opts* o, const V* vals, cons
 // the forward sweep, r.z comes from the backward sweep.
 template <typename V>
 int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag, const uint8_t* fixed,
-                     const bh_precond* P, NewtonWs& w, int* iters, int* done, hipStream_t s) {
+                     const bh_precond* P, const double* q, NewtonWs& w, int* iters, int* done, hipStream_t s) {
   const int nV = m->nV;
   const int64_t n = 3ll * nV;
   const int gsp = sweep_grid(nV);
@@ -1244,23 +1333,47 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const d
   double *b = svec(P, SV_B), *x = svec(P, SV_X), *r = svec(P, SV_R), *p = svec(P, SV_P);
   double *sv = svec(P, SV_S), *u = svec(P, SV_U), *wv = svec(P, SV_W);
   int np[2] = {0, 0};
+  // optional coarse correction u += Z E^-1 Z^T r after the factor's solve (the factor's r.u partials
+  // come first in their region, the g.y partials after them)
+  Coarse c;
+  int gy = 0;
+  const int gj = jac_blocks(nV);
+  if (o->coarse_correction) {
+    if ((st = coarse_workspace(m, o->coarse_aggregates, gj, c))) return st;
+    gy = (c.k + kCoarseRowsPerBlock - 1) / kCoarseRowsPerBlock;
+    if (precond_sweep_blocks(P) + gy > R) return BH_UNSUPPORTED;
+    hipLaunchKernelGGL(k_inverse_order, dim3((nV + 255) / 256), dim3(256), 0, s, nV, P->colVerts, c.pos);
+    if ((st = coarse_setup<V>(m, vals, hdiag, fixed, q, c, s))) return st;
+  }
+  auto coarse_tail = [&](const PcgState* sk) {
+    if (!o->coarse_correction) return;
+    hipLaunchKernelGGL(k_coarse_zr_order, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, (const int32_t*)c.pos,
+                       (const double*)r, q, (const double*)c.cent, c.pzr, sk);
+    hipLaunchKernelGGL(k_coarse_g, dim3(c.k), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
+    hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
+                       (const double*)c.g, c.y, pru + np[1], sk);
+    hipLaunchKernelGGL(k_coarse_add_order, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, (const int32_t*)c.pos, q,
+                       (const double*)c.cent, (const double*)c.y, fixed, u, sk);
+  };
   hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)w.b, b);
   hipLaunchKernelGGL(k_cg_init, dim3(gi), dim3(kRedBlock), 0, s, n, b, x, r, p, sv, prr);
   CgVecs start{x, p, sv, u, wv, nullptr, nullptr};
   if ((st = precond_apply_solve(P, r, u, s, &start, 0, pru, np))) return st;
+  coarse_tail(nullptr);
   hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl, P->nuptr,
                      P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                      (const PcgState*)nullptr);
-  cg_reduce(w, nV, np[1], gi, gsp, 1, o, s);
+  cg_reduce(w, nV, np[1] + gy, gi, gsp, 1, o, s);
   CgVecs loop{x, p, sv, u, wv, prr, w.st};
   PcgState hs{};
   for (;;) {
     for (int k = 0; k < kPcgBatch; ++k) {
       if ((st = precond_apply_solve(P, r, u, s, &loop, 1, pru, np))) return st;
+      coarse_tail(w.st);
       hipLaunchKernelGGL(k_split_spmv<V>, dim3(gsp), dim3(256), 0, s, nV, P->nlptr, P->nlcol, (const V*)h.hl,
                          P->nuptr, P->nucol, (const V*)h.hu, h.ld, (const double*)P->hdiag, (const double*)u, wv, pwu,
                          w.st);
-      cg_reduce(w, nV, np[1], np[0], gsp, 0, o, s);
+      cg_reduce(w, nV, np[1] + gy, np[0], gsp, 0, o, s);
     }
     if ((st = cg_wait(w, &hs, s))) return st;
     if (hs.done) break;
@@ -1334,7 +1447,7 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
       if (!o->mgn) return BH_BAD_ARG;
       if ((st = mgn_factor<V>(o->mgn, o->precond_handle, vals, fixed, s))) return st;
     }
-    if ((st = pcg_solve_factor(m, o, vals, hdiag, fixed, o->precond_handle, w, pits, pdone, s))) return st;
+    if ((st = pcg_solve_factor(m, o, vals, hdiag, fixed, o->precond_handle, q, w, pits, pdone, s))) return st;
     if (*pdone == 2 && o->jacobi_fallback) {
       int p2 = 0;
       st = hdiag ? pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, 0, w, &p2, pdone, s)
@@ -1395,6 +1508,7 @@ void bh_newton_default_opts(bh_newton_opts* o) {
   o->inertia = 0.0;
   o->relaxation = 1.0;
   o->coarse_aggregates = 0;
+  o->coarse_correction = 0;
 }
 
 

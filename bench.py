@@ -71,8 +71,9 @@ def newton_steps(p, mesh, steps, pcg_max):
     """Newton-step wall time on the same beam (bent state, gravity, x = 0 clamped, PSD 1e-3, PCG to a
     1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
     precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form),
-    two-level block Jacobi (quadratic coarse space on vertex aggregates, also in FP64) and the
-    learned Eisenstat diagonal (when trained weights are present)."""
+    two-level block Jacobi (quadratic coarse space on vertex aggregates, also in FP64), block IC(0)
+    with the same coarse correction (also in FP64) and the learned Eisenstat diagonal (when trained
+    weights are present)."""
     import torch
 
     from bartels_amd import _lib, learned
@@ -94,11 +95,12 @@ def newton_steps(p, mesh, steps, pcg_max):
     net = learned.Network(weights=learned.DEFAULT_WEIGHTS) if os.path.exists(learned.DEFAULT_WEIGHTS) else None
     # precision 1: FP32 Hessian / CG operator, FP64 element gradient; 2: FP32 element gradient too
     variants = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5),
-                ("fp32full_sgs", 2, 5), ("fp64_jacobi2l", 0, 7), ("fp32_jacobi2l", 1, 7)]
+                ("fp32full_sgs", 2, 5), ("fp64_jacobi2l", 0, 7), ("fp32_jacobi2l", 1, 7),
+                ("fp64_bic0_2l", 0, 2, 1), ("fp32_bic0_2l", 1, 2, 1)]
     if net is not None:
         variants.append(("fp32_learned_sgs", 1, net.precond))
     res, qs = {}, {}
-    for name, prec, precond in variants:
+    for name, prec, precond, *coarse in variants:
         o = _lib.BhNewtonOpts()
         lib.bh_newton_default_opts(ctypes.byref(o))
         o.model = 0
@@ -108,6 +110,7 @@ def newton_steps(p, mesh, steps, pcg_max):
         o.pcg_tol = 1e-6
         o.pcg_max_iterations = pcg_max
         o.precond = precond
+        o.coarse_correction = coarse[0] if coarse else 0
         o.precond_handle = P
         o.element_precision = prec
         if net is not None:

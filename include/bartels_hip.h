@@ -308,8 +308,11 @@ typedef struct {
   const double* d_q_pred;
   double inertia;
   double relaxation;       /* BH_PRECOND_SGS: SSOR relaxation w in (0, 2), Dt = D / w (0 means 1) */
-  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L: number of aggregates (0: one per 4096 vertices,
-                              4 to 8; larger requests are capped at 16) */
+  int coarse_aggregates;   /* BH_PRECOND_JACOBI_2L and coarse_correction: number of aggregates (0: one
+                              per 4096 vertices, 4 to 8; larger requests are capped at 16) */
+  int coarse_correction;   /* 1: add the two-level coarse term Z E^-1 Z^T of BH_PRECOND_JACOBI_2L to the
+                              factor preconditioners (BH_PRECOND_BIC0, _FACTOR, _LEARNED); 0 (default):
+                              the factor alone */
 } bh_newton_opts;
 
 typedef struct {

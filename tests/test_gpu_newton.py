@@ -64,7 +64,7 @@ def test_newton_config1_matches_direct_solve(oracle):
 
 
 def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_max=20000, gtol=1e-7, max_it=30,
-            handle=None, fallback=1, psd=False, aggregates=0):
+            handle=None, fallback=1, psd=False, aggregates=0, coarse=0):
     import torch
 
     from bartels_amd import _lib
@@ -73,6 +73,7 @@ def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_ma
     o = _lib.BhNewtonOpts()
     lib.bh_newton_default_opts(ctypes.byref(o))
     o.coarse_aggregates = aggregates
+    o.coarse_correction = coarse
     o.model = 0
     o.gradient_tol = gtol
     o.relative_tol = 1
@@ -119,10 +120,11 @@ def gravity_problem(oracle):
 
 
 def test_newton_precision_form_and_preconditioners(gravity_problem):
-    """FP64 block-Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form) and two-level block
-    Jacobi (quadratic coarse correction), FP64 and the FP32 precision form, reach the oracle's
-    equilibrium; the precision form within 1e-5 relative displacement.  IC(0), Gauss-Seidel and the
-    two-level form need fewer CG iterations than block Jacobi."""
+    """FP64 block-Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form), two-level block
+    Jacobi (quadratic coarse correction) and block IC(0) with the same coarse correction, FP64 and the
+    FP32 precision form, reach the oracle's equilibrium; the precision form within 1e-5 relative
+    displacement.  IC(0), Gauss-Seidel and the two-level form need fewer CG iterations than block
+    Jacobi, and IC(0) with the coarse correction fewer than IC(0) alone."""
     from bartels_amd import _lib
 
     p, mesh, q0, fixed, f, q_or = gravity_problem
@@ -137,7 +139,9 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
                          ("fp32_bic0", dict(precision=1, precond=2, handle=P)),
                          ("fp32_sgs", dict(precision=1, precond=5, handle=P)),
                          ("fp32full_sgs", dict(precision=2, precond=5, handle=P)),
-                         ("fp64_2l", dict(precond=7)), ("fp32_2l", dict(precision=1, precond=7))):
+                         ("fp64_2l", dict(precond=7)), ("fp32_2l", dict(precision=1, precond=7)),
+                         ("fp64_bic0_2l", dict(precond=2, handle=P, coarse=1)),
+                         ("fp32_bic0_2l", dict(precision=1, precond=2, handle=P, coarse=1))):
             q, rep, tr = _newton(mesh, p, q0, fixed, f, **kw)
             err = np.linalg.norm(q - q_or) / u
             results[name] = (err, rep.iterations, rep.pcg_iterations_total, rep.gradient_norm / rep.gradient_norm0)
@@ -156,6 +160,10 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
         assert results["fp64_2l"][0] < 1e-8
         assert results["fp32_2l"][0] < 1e-5
         assert results["fp64_2l"][2] < results["fp64_jacobi"][2]  # the coarse correction saves iterations
+        assert results["fp64_bic0_2l"][0] < 1e-8
+        assert results["fp32_bic0_2l"][0] < 1e-5
+        assert results["fp64_bic0_2l"][2] < results["fp64_bic0"][2]
+        assert results["fp32_bic0_2l"][2] < results["fp32_bic0"][2]
     finally:
         lib.bh_precond_destroy(P)
 
