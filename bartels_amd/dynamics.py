@@ -46,7 +46,7 @@ class BackwardEuler:
         self.v = torch.zeros_like(self.x)
 
     def step(self, dt, precond=0, precision=0, mgn=None, gtol=1e-6, max_it=10, pcg_tol=1e-6, pcg_max=20000,
-             psd=True):
+             psd=True, coarse=0):
         import torch
 
         lib = self.lib
@@ -61,6 +61,7 @@ class BackwardEuler:
         o.pcg_tol = pcg_tol
         o.pcg_max_iterations = pcg_max
         o.precond = precond
+        o.coarse_correction = coarse
         o.precond_handle = self.P
         o.element_precision = precision
         if mgn is not None:
@@ -88,7 +89,7 @@ class BackwardEuler:
 
 VARIANTS = {"fp64_jacobi": (0, 0), "fp32_jacobi": (0, 1), "fp32_bic0": (2, 1), "fp32_sgs": (5, 1),
             "fp32_learned_sgs": (6, 1), "fp32_learned_factor": (4, 1), "fp64_jacobi2l": (7, 0),
-            "fp32_jacobi2l": (7, 1)}
+            "fp32_jacobi2l": (7, 1), "fp32_bic0_2l": (2, 1, 1)}
 
 
 def run(steps, variant, grid=(256, 64, 64), weights=None, log=print):
@@ -96,7 +97,7 @@ def run(steps, variant, grid=(256, 64, 64), weights=None, log=print):
 
     p = meshgen.config5(grid=grid)
     be = BackwardEuler(p)
-    precond, precision = VARIANTS[variant]
+    precond, precision, *coarse = VARIANTS[variant]
     net = None
     if precond in (4, 6):
         from . import learned
@@ -108,7 +109,8 @@ def run(steps, variant, grid=(256, 64, 64), weights=None, log=print):
     for k in range(steps):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        rep = be.step(p.dt, precond=precond, precision=precision, mgn=net.h if net else None)
+        rep = be.step(p.dt, precond=precond, precision=precision, mgn=net.h if net else None,
+                      coarse=coarse[0] if coarse else 0)
         torch.cuda.synchronize()
         rows.append(dict(step=k, newton=rep.iterations, pcg=rep.pcg_iterations_total, fallbacks=rep.pcg_fallbacks,
                          s=time.perf_counter() - t, tip_z=float(be.x[-1].item() - p.V.ravel()[-1])))
