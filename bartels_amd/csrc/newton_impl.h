@@ -904,20 +904,14 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   u[i] += z;
 }
 
-// monomial c of d (one entry of monomials())
+// monomial c of d (one entry of monomials()) as a product of two selected factors: the lanes of a
+// wave take different c, and a switch compiled to branches that the wave ran one after another
 __device__ __forceinline__ double monomial(const double d[3], int c) {
-  switch (c) {
-    case 0: return 1.0;
-    case 1: return d[0];
-    case 2: return d[1];
-    case 3: return d[2];
-    case 4: return d[0] * d[0];
-    case 5: return d[1] * d[1];
-    case 6: return d[2] * d[2];
-    case 7: return d[0] * d[1];
-    case 8: return d[1] * d[2];
-    default: return d[0] * d[2];
-  }
+  const int i1 = c == 0 ? -1 : c <= 3 ? c - 1 : c <= 6 ? c - 4 : c == 8 ? 1 : 0;
+  const int i2 = c <= 3 ? -1 : c <= 6 ? c - 4 : c == 7 ? 1 : 2;
+  const double a = i1 < 0 ? 1.0 : i1 == 0 ? d[0] : i1 == 1 ? d[1] : d[2];
+  const double e = i2 < 0 ? 1.0 : i2 == 0 ? d[0] : i2 == 1 ? d[1] : d[2];
+  return a * e;
 }
 
 // the vector kernel of block-Jacobi PCG plus the partials of Z^T r per workgroup (r of fixed DOFs
