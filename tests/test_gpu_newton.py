@@ -261,6 +261,16 @@ def test_newton_learned_preconditioner_untrained_is_block_jacobi(gravity_problem
         print(f"learned (untrained): err {err:.2e}, pcg {rep.pcg_iterations_total} vs jacobi {rj.pcg_iterations_total}")
         assert err < 1e-5
         assert abs(rep.pcg_iterations_total - rj.pcg_iterations_total) <= 0.10 * rj.pcg_iterations_total + 5
+        # with the coarse correction the untrained learned factor is two-level block Jacobi in the
+        # colour order: the same equilibrium, iteration counts within 10 % of the vertex-order form
+        q2, rep2, _ = _newton_mgn(mesh, p, q0, fixed, f, M, coarse=1, **o_extra)
+        _, r2l, _ = _newton(mesh, p, q0, fixed, f, precision=1, precond=7)
+        err2 = np.linalg.norm(q2 - q_or) / u
+        print(f"learned (untrained) + coarse: err {err2:.2e}, pcg {rep2.pcg_iterations_total} vs two-level "
+              f"{r2l.pcg_iterations_total}")
+        assert err2 < 1e-5
+        assert rep2.pcg_failures == 0
+        assert abs(rep2.pcg_iterations_total - r2l.pcg_iterations_total) <= 0.10 * r2l.pcg_iterations_total + 5
     finally:
         lib.bh_mgn_destroy(M)
         lib.bh_precond_destroy(P)
@@ -314,7 +324,7 @@ def test_precision_form_sgs_converges_to_fp64_equilibrium():
     assert r32.pcg_iterations_total < r64.pcg_iterations_total
 
 
-def _newton_mgn(mesh, p, q0, fixed, f, M, precond=4, precision=1, handle=None, pcg_tol=1e-10, max_it=30,
+def _newton_mgn(mesh, p, q0, fixed, f, M, precond=4, precision=1, handle=None, pcg_tol=1e-10, max_it=30, coarse=0,
                 gtol=1e-7):
     import torch
 
@@ -333,6 +343,7 @@ def _newton_mgn(mesh, p, q0, fixed, f, M, precond=4, precision=1, handle=None, p
     o.element_precision = precision
     o.precond_handle = handle
     o.mgn = M
+    o.coarse_correction = coarse
     dq = torch.tensor(q0, device="cuda")
     dparams = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
     dfixed = torch.tensor(fixed, device="cuda")
