@@ -878,12 +878,13 @@ __global__ void __launch_bounds__(64 * kCoarseRowsPerBlock)
   }
 }
 
-// u += Z y (rows of fixed DOFs untouched): one thread per DOF row, 64 vertices per workgroup (the
-// vertex's positions meet in LDS, each read once)
+// u += Z y (rows of fixed DOFs untouched): one thread per DOF row in vertex order, 64 vertices per
+// workgroup (the vertex's positions meet in LDS, each read once).  u is in vertex order, or in the
+// solve order of a colour-order solver when pos (vertex -> position) is given.
 __global__ void __launch_bounds__(3 * kJacVerts)
-    k_coarse_add(int nV, int per, const double* __restrict__ q, const double* __restrict__ cent,
-                 const double* __restrict__ y, const uint8_t* __restrict__ fixed, double* __restrict__ u,
-                 const PcgState* __restrict__ st) {
+    k_coarse_add(int nV, int per, const int32_t* __restrict__ pos, const double* __restrict__ q,
+                 const double* __restrict__ cent, const double* __restrict__ y, const uint8_t* __restrict__ fixed,
+                 double* __restrict__ u, const PcgState* __restrict__ st) {
   __shared__ double qs[3 * kJacVerts];
   if (st && cg_skip(st)) return;
   const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
@@ -901,7 +902,7 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   double z = 0.0;
 #pragma unroll
   for (int c = 0; c < kMono; ++c) z += a[c] * m[c];
-  u[i] += z;
+  u[pos ? 3ll * pos[v] + l : i] += z;
 }
 
 // monomial c of d (one entry of monomials()) as a product of two selected factors: the lanes of a
@@ -1049,31 +1050,6 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   }
 }
 
-// u += Z y with u in the solve order (rows of fixed DOFs untouched)
-__global__ void __launch_bounds__(3 * kJacVerts)
-    k_coarse_add_order(int nV, int per, const int32_t* __restrict__ pos, const double* __restrict__ q,
-                       const double* __restrict__ cent, const double* __restrict__ y,
-                       const uint8_t* __restrict__ fixed, double* __restrict__ u, const PcgState* __restrict__ st) {
-  __shared__ double qs[3 * kJacVerts];
-  if (st && cg_skip(st)) return;
-  const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
-  const bool in = i < 3ll * nV;
-  qs[threadIdx.x] = in ? q[i] : 0.0;
-  __syncthreads();
-  if (!in || (fixed && fixed[i])) return;
-  const int v = (int)(i / 3), l = (int)(i % 3);
-  const int I = v / per;
-  const int base = threadIdx.x - l;
-  double d[3], m[kMono];
-  agg_coords(cent, I, qs[base], qs[base + 1], qs[base + 2], d);
-  monomials(d, m);
-  const double* a = y + kModes * I + kMono * l;
-  double z = 0.0;
-#pragma unroll
-  for (int c = 0; c < kMono; ++c) z += a[c] * m[c];
-  u[3ll * pos[v] + l] += z;
-}
-
 // E = Z^T A Z and its inverse for the current system (vertex order, CSR operator)
 template <typename V>
 int coarse_setup(bh_mesh* m, const V* vals, const double* hdiag, const uint8_t* fixed, const double* q,
@@ -1126,8 +1102,8 @@ int pcg_solve_2l(bh_mesh* m, const bh_newton_opts* o, const V* vals, const doubl
     hipLaunchKernelGGL(k_coarse_g, dim3(c.k), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
     hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
                        (const double*)c.g, c.y, pru + gj, sk);
-    hipLaunchKernelGGL(k_coarse_add, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, q, (const double*)c.cent,
-                       (const double*)c.y, fixed, w.z, sk);
+    hipLaunchKernelGGL(k_coarse_add, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, (const int32_t*)nullptr, q,
+                       (const double*)c.cent, (const double*)c.y, fixed, w.z, sk);
     hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(gs), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals, hdiag, fixed, w.z,
                        w.Ap, pwu, start ? (const PcgState*)nullptr : (const PcgState*)w.st);
     cg_reduce(w, nV, gj + gy, gj, gs, start, o, s);
@@ -1346,7 +1322,7 @@ int pcg_solve_factor(bh_mesh* m, const bh_newton_opts* o, const V* vals, const d
     hipLaunchKernelGGL(k_coarse_g, dim3(c.k), dim3(256), 0, s, c.per, gj, (const double*)c.pzr, c.g, sk);
     hipLaunchKernelGGL(k_coarse_y, dim3(gy), dim3(64 * kCoarseRowsPerBlock), 0, s, c.k, (const double*)c.E,
                        (const double*)c.g, c.y, pru + np[1], sk);
-    hipLaunchKernelGGL(k_coarse_add_order, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, (const int32_t*)c.pos, q,
+    hipLaunchKernelGGL(k_coarse_add, dim3(gj), dim3(3 * kJacVerts), 0, s, nV, c.per, (const int32_t*)c.pos, q,
                        (const double*)c.cent, (const double*)c.y, fixed, u, sk);
   };
   hipLaunchKernelGGL(k_permute_rows, dim3(gr), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)w.b, b);
