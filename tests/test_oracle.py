@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle before it is trusted as the judge of the GPU path.
+"""Pins the FP64 CPU oracle before it is trusted as the checker of the GPU path.
 
 Each check is independent of the oracle's own formulas:
   * the reference's own fixtures (bunny mesh, mass matrix, legacy-NH Hessian, F = diag(2,4,3));
