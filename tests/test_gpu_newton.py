@@ -359,7 +359,9 @@ def _newton_mgn(mesh, p, q0, fixed, f, M, precond=4, precision=1, handle=None, p
 
 def test_backward_euler_steps_match_oracle(oracle):
     """three backward-Euler steps (dt = 0.01, gravity, x = 0 clamped) of a small beam: device Newton
-    with the inertia term (FP64, and the FP32 precision form) against the oracle's direct-solve BE"""
+    with the inertia term (FP64, and the FP32 precision form; block Jacobi, two-level block Jacobi and
+    block IC(0) with the coarse correction, whose E then carries the mass term) against the oracle's
+    direct-solve BE"""
     import torch
 
     from bartels_amd import _lib
@@ -385,7 +387,10 @@ def test_backward_euler_steps_match_oracle(oracle):
     v_or = np.zeros_like(x_or)
     # FP64: the oracle's energy-based Armijo test stalls near 1e-8 relative (rounding of the energy
     # differences), which sets the agreement floor
-    for prec, bound in ((0, 1e-7), (1, 1e-5), (2, 1e-5)):
+    P = ctypes.c_void_p()
+    assert lib.bh_precond_create(mesh.handle, ctypes.byref(P)) == 0
+    for prec, bound, precond, coarse in ((0, 1e-7, 0, 0), (1, 1e-5, 0, 0), (2, 1e-5, 0, 0), (0, 1e-7, 7, 0),
+                                         (1, 1e-5, 7, 0), (0, 1e-7, 2, 1), (1, 1e-5, 2, 1)):
         x = p.V.ravel().copy()
         v = np.zeros_like(x)
         xo, vo = x_or.copy(), v_or.copy()
@@ -402,6 +407,9 @@ def test_backward_euler_steps_match_oracle(oracle):
             o.max_iterations = 8
             o.pcg_tol = 1e-12
             o.element_precision = prec
+            o.precond = precond
+            o.precond_handle = P
+            o.coarse_correction = coarse
             dqp = torch.tensor(qp, device="cuda")
             o.d_mass_vals = ctypes.c_void_p(mvals.data_ptr())
             o.d_q_pred = ctypes.c_void_p(dqp.data_ptr())
@@ -417,5 +425,7 @@ def test_backward_euler_steps_match_oracle(oracle):
             v, x = (q - x) / dt, q
             vo, xo = (q_or - xo) / dt, q_or
         err = np.linalg.norm(x - xo) / np.linalg.norm(xo - p.V.ravel())
-        print(f"backward Euler, precision {prec}: relative displacement difference after 3 steps {err:.2e}")
+        print(f"backward Euler, precision {prec}, preconditioner {precond}, coarse {coarse}: relative displacement "
+              f"difference after 3 steps {err:.2e}")
         assert err < bound
+    lib.bh_precond_destroy(P)
