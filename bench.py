@@ -72,8 +72,8 @@ def newton_steps(p, mesh, steps, pcg_max):
     1e-6 relative residual): FP64 elements with block-Jacobi PCG (the plain solver) against the FP32
     precision form with block Jacobi, block IC(0), block symmetric Gauss-Seidel (Eisenstat form),
     two-level block Jacobi (quadratic coarse space on vertex aggregates, also in FP64), block IC(0)
-    with the same coarse correction (also in FP64) and the learned Eisenstat diagonal (when trained
-    weights are present)."""
+    with the same coarse correction (also in FP64), the learned factor with the coarse correction
+    (untrained network) and the learned Eisenstat diagonal (when trained weights are present)."""
     import torch
 
     from bartels_amd import _lib, learned
@@ -99,6 +99,10 @@ def newton_steps(p, mesh, steps, pcg_max):
                 ("fp64_bic0_2l", 0, 2, 1), ("fp32_bic0_2l", 1, 2, 1)]
     if net is not None:
         variants.append(("fp32_learned_sgs", 1, net.precond))
+    # the learned factor form with the coarse correction, untrained: its decoders are zero, so the
+    # network's factor is block Jacobi; the timing includes the network's inference per Newton step
+    net0 = learned.Network(form="factor")
+    variants.append(("fp32_learned_factor_untrained_2l", 1, net0.precond, 1))
     res, qs = {}, {}
     for name, prec, precond, *coarse in variants:
         o = _lib.BhNewtonOpts()
@@ -113,7 +117,9 @@ def newton_steps(p, mesh, steps, pcg_max):
         o.coarse_correction = coarse[0] if coarse else 0
         o.precond_handle = P
         o.element_precision = prec
-        if net is not None:
+        if precond == net0.precond:
+            o.mgn = net0.h
+        elif net is not None:
             o.mgn = net.h
         dq = torch.tensor(q0, device=dev)
         rep = _lib.BhNewtonReport()
@@ -135,6 +141,7 @@ def newton_steps(p, mesh, steps, pcg_max):
     lib.bh_precond_destroy(P)
     if net is not None:
         net.close()
+    net0.close()
     u = np.linalg.norm(qs["fp64_jacobi"] - q0)
     for name in qs:
         res[name]["rel_displacement_vs_fp64_jacobi"] = float(np.linalg.norm(qs[name] - qs["fp64_jacobi"]) / u)
