@@ -226,6 +226,9 @@ __device__ void cg_scalar_step(const double* __restrict__ part, int start, doubl
     if (rr == 0.0) {
       st->alpha = 0.0;
       st->done = 1;
+    } else if (!(g > 0.0)) {  // r.u <= 0 (or not finite): the preconditioner is not positive definite
+      st->alpha = 0.0;
+      st->done = 3;
     } else if (!(d > 0.0)) {  // non-positive curvature: take the preconditioned gradient
       st->alpha = 1.0;
       st->done = 3;
@@ -241,6 +244,9 @@ __device__ void cg_scalar_step(const double* __restrict__ part, int start, doubl
     st->done = 1;
   } else if (st->iters >= st->maxit) {
     st->done = 2;
+  } else if (!(g > 0.0)) {  // r.u <= 0 or not finite: indefinite preconditioner, keep the current iterate
+    st->alpha = 0.0;
+    st->done = 3;
   } else {
     const double beta = g / st->gamma;
     const double den = d - beta * g / st->alpha;  // = p'.H p' for the next direction
@@ -1363,6 +1369,28 @@ __global__ void __launch_bounds__(256) k_f64_to_f32(int64_t n, const double* __r
   if (i < n) b[i] = (float)a[i];
 }
 
+// The Jacobi fallback of the factor and Gauss-Seidel solvers: a solve that hit its iteration cap
+// (done 2) or broke down (done 3: r.u or p.Ap not positive or not finite, which an indefinite factor
+// or diagonal produces) is redone from zero with block-Jacobi PCG, whose preconditioner is the
+// (PSD-clamped) block diagonal of H itself.  split = 0 reuses the split Hessian the failed solve made
+// (precision form), 1 makes it.
+template <typename V>
+int jacobi_fallback_solve(bh_mesh* m, const bh_newton_opts* o, const V* vals, const double* hdiag,
+                          const uint8_t* fixed, int split, NewtonWs& w, int* pits, int* pdone, int* fallback,
+                          hipStream_t s) {
+  int p2 = 0, st;
+  if (hdiag) {
+    if (!o->precond_handle) return BH_BAD_ARG;
+    st = pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, split, w, &p2, pdone, s);
+  } else {
+    st = pcg_solve(m, o, vals, hdiag, fixed, w, &p2, pdone, s);
+  }
+  if (st) return st;
+  *pits += p2;
+  *fallback = 1;
+  return BH_OK;
+}
+
 // one Newton direction: H(q) (PSD-fixed) and PCG on H d = -g, with the chosen preconditioner
 template <typename V>
 int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* vals, const uint8_t* fixed,
@@ -1379,7 +1407,7 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
   const double* hdiag = nullptr;
   if (sizeof(V) == 4 && m->nV) {
     hipLaunchKernelGGL(k_implied_diag<V>, dim3((unsigned)((8ll * m->nV + 255) / 256)), dim3(256), 0, s, m->nV,
-                       m->bptr, m->bcol, (const V*)vals, m->vdeg, psd ? psd->tol : 0.0, o->d_mass_vals,
+                       m->bptr, m->bcol, (const V*)vals, m->vdeg, psd ? std::max(psd->tol, 0.0) : 0.0, o->d_mass_vals,
                        o->d_mass_vals ? o->inertia : 0.0, w.hdiag);
     BH_CHECK(hipGetLastError());
     hdiag = w.hdiag;
@@ -1390,14 +1418,8 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
     if ((st = pcg_solve_eisenstat(m, o, vals, hdiag, fixed, o->precond_handle,
                                   o->precond == BH_PRECOND_LEARNED_SGS ? 1 : 0, w, pits, pdone, s)))
       return st;
-    if (*pdone == 2 && o->jacobi_fallback) {
-      int p2 = 0;
-      st = hdiag ? pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, 0, w, &p2, pdone, s)
-                 : pcg_solve(m, o, vals, hdiag, fixed, w, &p2, pdone, s);
-      if (st) return st;
-      *pits += p2;
-      *fallback = 1;
-    }
+    if (*pdone != 1 && o->jacobi_fallback) return jacobi_fallback_solve(m, o, vals, hdiag, fixed, 0, w, pits, pdone,
+                                                                        fallback, s);
     return BH_OK;
   }
   if (o->precond == BH_PRECOND_BIC0 || o->precond == BH_PRECOND_FACTOR || o->precond == BH_PRECOND_LEARNED) {
@@ -1406,11 +1428,19 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
       // IC(0) may break down (non-positive pivots) on strongly heterogeneous problems: refactor with a
       // growing diagonal shift until every pivot is positive
       double shift = o->precond_shift;
+      int fails = 0;
       for (int attempt = 0;; ++attempt) {
-        int fails = 0;
+        fails = 0;
         if ((st = precond_factor_bic0(o->precond_handle, vals, hdiag, fixed, shift, &fails, s))) return st;
         if (fails == 0 || attempt >= 6) break;
         shift = (shift > 0.0) ? 4.0 * shift : 1e-3;
+      }
+      // pivots still not positive after the last shift: the factor holds infinite inverse blocks, so it
+      // is never applied -- block Jacobi solves this system, or the solve fails
+      if (fails) {
+        *pits = 0;
+        if (!o->jacobi_fallback) return BH_SOLVE_FAILED;
+        return jacobi_fallback_solve(m, o, vals, hdiag, fixed, 1, w, pits, pdone, fallback, s);
       }
     }
     if (o->precond == BH_PRECOND_LEARNED) {
@@ -1418,14 +1448,8 @@ int newton_direction(bh_mesh* m, const bh_newton_opts* o, const V* params, V* va
       if ((st = mgn_factor<V>(o->mgn, o->precond_handle, vals, fixed, s))) return st;
     }
     if ((st = pcg_solve_factor(m, o, vals, hdiag, fixed, o->precond_handle, q, w, pits, pdone, s))) return st;
-    if (*pdone == 2 && o->jacobi_fallback) {
-      int p2 = 0;
-      st = hdiag ? pcg_solve_split_jacobi(m, o, vals, hdiag, fixed, o->precond_handle, 0, w, &p2, pdone, s)
-                 : pcg_solve(m, o, vals, hdiag, fixed, w, &p2, pdone, s);
-      if (st) return st;
-      *pits += p2;
-      *fallback = 1;
-    }
+    if (*pdone != 1 && o->jacobi_fallback) return jacobi_fallback_solve(m, o, vals, hdiag, fixed, 0, w, pits, pdone,
+                                                                        fallback, s);
     return BH_OK;
   }
   if (o->precond == BH_PRECOND_JACOBI_2L) return pcg_solve_2l(m, o, vals, hdiag, fixed, q, w, pits, pdone, s);

@@ -290,8 +290,12 @@ typedef struct {
    * product, the residual check and the convergence test stay FP64; params stay FP64 at the
    * interface (converted once per solve). */
   int element_precision;
-  /* 1: when a factor-preconditioned PCG (BIC0 / FACTOR) hits pcg_max_iterations, solve again with
-   * block-Jacobi PCG from zero (the fallback), counted in bh_newton_report.pcg_fallbacks */
+  /* 1: when a factor or Gauss-Seidel preconditioned PCG (BIC0, FACTOR, LEARNED, SGS, LEARNED_SGS) hits
+   * pcg_max_iterations, or breaks down (r.u or p.Ap not positive or not finite: an indefinite factor or
+   * diagonal), or block IC(0) still has non-positive pivots after its last diagonal shift, solve again
+   * with block-Jacobi PCG from zero (the fallback), counted in bh_newton_report.pcg_fallbacks.
+   * 0: the capped or broken-down direction is used as it is (IC(0) with failed pivots returns
+   * BH_SOLVE_FAILED) */
   int jacobi_fallback;
   bh_mgn* mgn;             /* BH_PRECOND_LEARNED (precond_handle supplies the colour lists) */
   /* optional, called after every direction solve with device pointers: the Hessian values (FP32 in
