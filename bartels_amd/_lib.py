@@ -119,6 +119,7 @@ SIGNATURES = {
     "bh_linear_tetmesh_generic_elastic_dq": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_generic_elastic_dq2": (c_int, [_VP, _VP, _VP, _VP, P(BhPsd), _VP]),
     "bh_linear_tetmesh_q_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tetmesh_q_device_noalloc": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq2_device": (c_int, [_VP, c_int, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
     "bh_linear_tetmesh_mass_matrix_device": (c_int, [_VP, _VP, _VP, _VP]),

@@ -111,8 +111,10 @@ struct ProfScope {
   int k;
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
+  // m == nullptr: record nothing
   ProfScope(const bh_mesh* m_, int k_, hipStream_t s_) : m(m_), k(k_), s(s_) {
-    if (m->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
+    if (m && m->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, s);
   }
   ~ProfScope() {
     if (a && b) {
@@ -1117,33 +1119,73 @@ int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q
   }
 }
 
+}  // namespace
+
+// ---- the allocation-free energy entry point's scratch -----------------------------------------
+// The energy's per-workgroup partial sums live here for bh_linear_tetmesh_q_device_noalloc: a
+// file-scope device array (each GPU the library runs on holds its own copy until the process ends),
+// so that entry point calls no allocator.  Host code never uses the name `scratch` as an address
+// (on the host it is a stand-in, not the device address): it looks the address up per device with
+// hipGetSymbolAddress once, with that device current, and passes that pointer to the kernels.
+static constexpr int kNoAllocPartials = BH_Q_NOALLOC_MAX_TETS / (kEnergyBlock * kEnergyItems);
+static_assert(kNoAllocPartials * kEnergyBlock * kEnergyItems == BH_Q_NOALLOC_MAX_TETS,
+              "BH_Q_NOALLOC_MAX_TETS must be a whole number of energy workgroups");
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices];  // device address of `scratch`, per device ordinal
+
+// the current device's address of `scratch`, looked up on the first use on that device
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  BH_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) return BH_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    BH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return BH_DEVICE_ERROR;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BH_OK;
+}
+
+// partials: the per-workgroup partial sums (the mesh's own buffer, or the allocation-free entry
+// point's device address of `scratch`, which then also records no profiling events)
 template <int MODEL, typename T>
-int launch_energy(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, double* out, hipStream_t s) {
+int launch_energy(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, double* out, hipStream_t s,
+                  double* partials) {
   const int per = kEnergyBlock * kEnergyItems;
   const int grid = (m->nT + per - 1) / per;
-  ProfScope ps(m, BH_KT_ENERGY, s);
+  const bool own = partials == nullptr;
+  if (own) partials = m->partials;
+  ProfScope ps(own ? m : nullptr, BH_KT_ENERGY, s);
   if (grid > 0) {
     hipLaunchKernelGGL((k_energy<MODEL, T>), dim3(grid), dim3(kEnergyBlock), 0, s, view(m), q, params, mids,
-                       m->partials);
+                       partials);
     BH_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, m->partials, grid, out);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, grid, out);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
 
 template <typename T>
 int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q, const T* params, double* out,
-                    hipStream_t s) {
+                    hipStream_t s, double* partials = nullptr) {
   int st = ensure_geometry<T>(m, s);
   if (st) return st;
-  if (mids) return launch_energy<M_PER_TET>(m, q, params, mids, out, s);
+  if (mids) return launch_energy<M_PER_TET>(m, q, params, mids, out, s, partials);
   switch (model) {
-    case M_NH: return launch_energy<M_NH>(m, q, params, mids, out, s);
-    case M_STVK: return launch_energy<M_STVK>(m, q, params, mids, out, s);
-    case M_COROT: return launch_energy<M_COROT>(m, q, params, mids, out, s);
-    case M_ARAP: return launch_energy<M_ARAP>(m, q, params, mids, out, s);
-    case M_NH_LEGACY: return launch_energy<M_NH_LEGACY>(m, q, params, mids, out, s);
+    case M_NH: return launch_energy<M_NH>(m, q, params, mids, out, s, partials);
+    case M_STVK: return launch_energy<M_STVK>(m, q, params, mids, out, s, partials);
+    case M_COROT: return launch_energy<M_COROT>(m, q, params, mids, out, s, partials);
+    case M_ARAP: return launch_energy<M_ARAP>(m, q, params, mids, out, s, partials);
+    case M_NH_LEGACY: return launch_energy<M_NH_LEGACY>(m, q, params, mids, out, s, partials);
     default: return BH_BAD_ARG;
   }
 }
@@ -1306,6 +1348,9 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
   m->h_bptr.swap(bptr);
   m->h_bcol.swap(bcol);
   BH_CHECK(hipGetDevice(&m->device));
+  // set-up of the allocation-free energy entry point on this device (an address lookup, no allocation)
+  double* scr = nullptr;
+  if ((st = noalloc_scratch(&scr))) return st;
   return BH_OK;
 }
 
@@ -1447,6 +1492,22 @@ int bh_linear_tetmesh_q_device(bh_mesh* m, int model, const uint8_t* mids, const
                                double* d_energy, bh_stream_t s) {
   if (!m || !q || !params || !d_energy || check_model_args(model, mids)) return BH_BAD_ARG;
   return energy_dispatch(m, model, mids, q, params, d_energy, as_stream(s));
+}
+
+int bh_linear_tetmesh_q_device_noalloc(bh_mesh* m, int model, const uint8_t* mids, const double* q,
+                                       const double* params, double* d_energy, bh_stream_t s) {
+  if (!m || !q || !params || !d_energy || check_model_args(model, mids)) return BH_BAD_ARG;
+  if (m->nT > BH_Q_NOALLOC_MAX_TETS) {
+    std::fprintf(stderr, "bartels_hip: bh_linear_tetmesh_q_device_noalloc: %d tets exceed its maximum of %d\n",
+                 m->nT, BH_Q_NOALLOC_MAX_TETS);
+    return BH_UNSUPPORTED;
+  }
+  double* partials = nullptr;
+  int st = noalloc_scratch(&partials);
+  if (st) return st;
+  // FP64 geometry is resident from bh_mesh_create: ensure_geometry<double> allocates nothing, and
+  // every partial the reduction reads is written by this call's energy kernel (nothing to clear)
+  return energy_dispatch<double>(m, model, mids, q, params, d_energy, as_stream(s), partials);
 }
 
 int bh_linear_tetmesh_dq_device(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,

@@ -151,6 +151,20 @@ int bh_linear_tetmesh_dq_device(bh_mesh* mesh, int model, const uint8_t* d_model
 int bh_linear_tetmesh_dq2_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
                                  const double* d_params, const bh_psd* psd, double* d_vals, bh_stream_t stream);
 
+/* Allocation-free variant of bh_linear_tetmesh_q_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving or
+ * line-search loops, harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree: the
+ * per-workgroup partial sums of the energy live in a fixed device array inside the library
+ * (BH_Q_NOALLOC_MAX_TETS / 1024 doubles, 128 KiB, held by each GPU the library runs on until the
+ * process ends); the mesh (from bh_mesh_create, on the current device) and the inputs and output are
+ * the caller's.  Same kernels and summation order as bh_linear_tetmesh_q_device, so the result is
+ * bit-identical to it.  A mesh with more than BH_Q_NOALLOC_MAX_TETS tets is rejected with
+ * BH_UNSUPPORTED (and a message on stderr).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (concurrent streams, threads) race and must be serialised by the caller. */
+#define BH_Q_NOALLOC_MAX_TETS (16 * 1024 * 1024)
+int bh_linear_tetmesh_q_device_noalloc(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                       const double* d_params, double* d_energy, bh_stream_t stream);
+
 /* ---- precision form (device pointers) --------------------------------------------------------------
  * FP32 element path: FP32 copies of dphidX / volume (made on first use), FP32 params (nT x 2,
  * column-major), F formed from FP64 edge differences then FP32; J = det F, 1/J, log J and the
