@@ -29,6 +29,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <chrono>
 #include <limits>
 
 namespace {
@@ -624,6 +625,114 @@ void psd_fix(double* A, int n, double tol) {
     }
 }
 
+// The same spectral function by the method of Eigen's SelfAdjointEigenSolver (which the reference
+// calls, src/simple_psd_fix.cpp:9): Householder tridiagonalisation with the reflections accumulated,
+// then implicit symmetric QR steps with the Wilkinson shift (Golub & Van Loan, Matrix Computations,
+// 4th ed., Algorithms 8.3.1-8.3.3), eigenvectors accumulated by the Givens rotations.  Fixed-size
+// arrays, no allocation: the CPU-baseline timing build uses it; tests pin it against psd_fix and
+// numpy's eigh.
+template <int N>
+void sym_eigen_ql(const double* A /*N x N col-major, symmetric*/, double* lam, double* Vout /*col-major*/) {
+  double a[N][N], Q[N][N];
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      a[i][j] = A[i + N * j];
+      Q[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  // tridiagonalisation: for column k, the reflection H = I - 2 v v^T / v^T v (v zero above k+1) maps
+  // a(k+1:, k) onto a multiple of e_{k+1}; A <- H A H by the symmetric rank-2 update, Q <- Q H
+  for (int k = 0; k < N - 2; ++k) {
+    double nrm = 0.0;
+    for (int i = k + 1; i < N; ++i) nrm += a[i][k] * a[i][k];
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) continue;
+    const double alpha = (a[k + 1][k] > 0.0) ? -nrm : nrm;
+    double v[N] = {};
+    for (int i = k + 1; i < N; ++i) v[i] = a[i][k];
+    v[k + 1] -= alpha;
+    double vv = 0.0;
+    for (int i = k + 1; i < N; ++i) vv += v[i] * v[i];
+    if (vv == 0.0) continue;
+    double pvec[N], K = 0.0;
+    for (int i = 0; i < N; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < N; ++j) t += a[i][j] * v[j];
+      pvec[i] = 2.0 * t / vv;
+    }
+    for (int i = k + 1; i < N; ++i) K += v[i] * pvec[i];
+    K /= vv;
+    double w[N];
+    for (int i = 0; i < N; ++i) w[i] = pvec[i] - K * v[i];
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) a[i][j] -= v[i] * w[j] + w[i] * v[j];
+    for (int i = 0; i < N; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < N; ++j) t += Q[i][j] * v[j];
+      t *= 2.0 / vv;
+      for (int j = k + 1; j < N; ++j) Q[i][j] -= t * v[j];
+    }
+  }
+  double d[N], e[N];
+  for (int i = 0; i < N; ++i) d[i] = a[i][i];
+  for (int i = 0; i + 1 < N; ++i) e[i] = a[i + 1][i];
+  e[N - 1] = 0.0;
+  // implicit symmetric QR on the unreduced trailing block [l, m]
+  const double eps = std::numeric_limits<double>::epsilon();
+  int guard = 0;
+  for (int m = N - 1; m > 0 && guard < 60 * N;) {
+    if (std::fabs(e[m - 1]) <= eps * (std::fabs(d[m - 1]) + std::fabs(d[m]))) {
+      e[m - 1] = 0.0;
+      --m;
+      continue;
+    }
+    int l = m - 1;
+    while (l > 0 && std::fabs(e[l - 1]) > eps * (std::fabs(d[l - 1]) + std::fabs(d[l]))) --l;
+    ++guard;
+    const double dd = 0.5 * (d[m - 1] - d[m]), b = e[m - 1];
+    const double den = dd + ((dd >= 0.0) ? 1.0 : -1.0) * std::hypot(dd, b);
+    const double mu = d[m] - ((den != 0.0) ? b * b / den : 0.0);
+    double x = d[l] - mu, z = e[l], bulge = 0.0;
+    for (int k = l; k < m; ++k) {
+      const double r = std::hypot(x, z);
+      const double c = (r != 0.0) ? x / r : 1.0, sn = (r != 0.0) ? z / r : 0.0;
+      if (k > l) e[k - 1] = r;
+      const double ak = d[k], bk = e[k], ck = d[k + 1];
+      d[k] = c * c * ak + 2.0 * c * sn * bk + sn * sn * ck;
+      d[k + 1] = sn * sn * ak - 2.0 * c * sn * bk + c * c * ck;
+      e[k] = (c * c - sn * sn) * bk + c * sn * (ck - ak);
+      if (k + 1 < m) {
+        bulge = sn * e[k + 1];
+        e[k + 1] *= c;
+      }
+      for (int i = 0; i < N; ++i) {
+        const double qk = Q[i][k], qk1 = Q[i][k + 1];
+        Q[i][k] = c * qk + sn * qk1;
+        Q[i][k + 1] = -sn * qk + c * qk1;
+      }
+      x = e[k];
+      z = bulge;
+    }
+  }
+  for (int i = 0; i < N; ++i) {
+    lam[i] = d[i];
+    for (int j = 0; j < N; ++j) Vout[i + N * j] = Q[i][j];
+  }
+}
+
+template <int N>
+void psd_fix_ql(double* A, double tol) {
+  double lam[N], V[N * N];
+  sym_eigen_ql<N>(A, lam, V);
+  for (int k = 0; k < N; ++k) lam[k] = (lam[k] < tol) ? tol : lam[k];
+  for (int c = 0; c < N; ++c)
+    for (int r = c; r < N; ++r) {
+      double sum = 0.0;
+      for (int k = 0; k < N; ++k) sum += V[r + N * k] * lam[k] * V[c + N * k];
+      A[r + N * c] = sum;
+      A[c + N * r] = sum;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // assembly: 144 triplets per tet, then setFromTriplets semantics (duplicates summed left to right
 // in insertion order = ascending element order, explicit zeros kept, column-major compressed,
@@ -836,6 +945,44 @@ or_csc* or_tetmesh_dq2(int model, int nV, int nT, const int* E, const double* q,
   set_from_triplets(M, 3 * (int64_t)nV, trips);
   return export_csc(M);
 }
+
+// The CPU baseline's timing mode of the Hessian above: the same per-element algorithm (index-functor
+// B, dense B^T d2psi B, 144 triplets per tet) with simple_psd_fix by tridiagonalisation + implicit QR
+// (fixed-size, no allocation), elements split over nthreads OpenMP threads in contiguous ranges.
+// Each tet writes its 144 triplets at offset 144 e of one preallocated list, so the list is in
+// element order whatever the thread count, then one serial setFromTriplets fold (Eigen's is serial).
+// times (optional): [element and triplet seconds, setFromTriplets seconds].
+or_csc* or_tetmesh_dq2_timed(int model, int nV, int nT, const int* E, const double* q, const double* dphidX,
+                             const double* vol, const double* params, int psd, double psd_tol, int nthreads,
+                             double* times) {
+  std::vector<Triplet> trips((size_t)nT * 144);
+  const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int e = 0; e < nT; ++e) {
+    Tet t;
+    double K[144];
+    load_tet(t, e, nT, E, q, dphidX, vol, params);
+    element_hessian(t, model, 1, K);
+    if (psd) psd_fix_ql<12>(K, psd_tol);
+    Triplet* out = trips.data() + (size_t)e * 144;
+    for (int jj = 0; jj < 12; ++jj)
+      for (int ii = 0; ii < 12; ++ii)
+        *out++ = {3 * E[e + (int64_t)(ii / 3) * nT] + ii % 3, 3 * E[e + (int64_t)(jj / 3) * nT] + jj % 3,
+                  K[ii + 12 * jj]};
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  CSC M;
+  set_from_triplets(M, 3 * (int64_t)nV, trips);
+  const auto t2 = std::chrono::steady_clock::now();
+  if (times) {
+    times[0] = std::chrono::duration<double>(t1 - t0).count();
+    times[1] = std::chrono::duration<double>(t2 - t1).count();
+  }
+  return export_csc(M);
+}
+
+// 12x12 simple_psd_fix by the tridiagonalisation + QR eigensolver (tests compare it with or_psd_fix)
+void or_psd_fix_ql(double* A, double tol) { psd_fix_ql<12>(A, tol); }
 
 // Consistent mass matrix (m/10 diagonal, m/20 off-diagonal identity blocks).
 // Reference: src/linear_tet_mass_matrix.cpp:8-44, src/linear_tetmesh_mass_matrix.cpp:7-24.

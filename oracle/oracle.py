@@ -27,13 +27,53 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load():
-    global _lib
+_lib_fast = None
+
+
+def host_cpu():
+    """(model name, logical CPUs usable by this process, a short tag of the model and its flags)"""
+    import hashlib
+
+    model, flags = "unknown", ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count() or 1
+    return model, ncpu, hashlib.sha1((model + flags).encode()).hexdigest()[:10]
+
+
+def fast_lib_path():
+    return os.path.join(HERE, f"liboracle_fast_{host_cpu()[2]}.so")
+
+
+def load(fast=False):
+    """the checker build (-O2), or with fast=True the CPU-baseline timing build of the same source
+    (-O3 -march=native; built on the host that runs it, since -march=native is host-specific)"""
+    global _lib, _lib_fast
+    if fast:
+        if _lib_fast is None:
+            path = fast_lib_path()
+            if not os.path.exists(path):
+                subprocess.run(["make", "-s", "-C", HERE, "fast", "FAST_LIB=" + os.path.basename(path)], check=True)
+            _lib_fast = _declare(ctypes.CDLL(path))
+        return _lib_fast
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB):
         build()
-    L = ctypes.CDLL(LIB)
+    _lib = _declare(ctypes.CDLL(LIB))
+    return _lib
+
+
+def _declare(L):
     vp, i, d, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
     L.or_dphidX_volume.argtypes = [i, vp, i, vp, vp, vp]
     L.or_dphidX_volume.restype = i
@@ -63,7 +103,9 @@ def load():
     L.or_linesearch_backtracking.restype = i
     L.or_band_solve.argtypes = [i64, vp, vp, vp, vp, vp]
     L.or_band_solve.restype = i
-    _lib = L
+    L.or_psd_fix_ql.argtypes = [vp, d]
+    L.or_tetmesh_dq2_timed.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, d, i, vp]
+    L.or_tetmesh_dq2_timed.restype = ctypes.POINTER(_CSC)
     return L
 
 
@@ -126,6 +168,26 @@ def svd3(F):
     U, S, V = np.zeros(9), np.zeros(3), np.zeros(9)
     load().or_svd3(_p(f), _p(U), _p(S), _p(V))
     return U.reshape(3, 3, order="F"), S, V.reshape(3, 3, order="F")
+
+
+def psd_fix_ql(A, tol):
+    """12x12 simple_psd_fix by tridiagonalisation + implicit QR (the timing build's eigensolver)"""
+    A = np.asfortranarray(np.array(A, dtype=np.float64))
+    assert A.shape == (12, 12)
+    load().or_psd_fix_ql(_p(A), tol)
+    return A
+
+
+def tetmesh_dq2_timed(model, nV, E, q, dX, vol, params, psd=False, psd_tol=1e-3, threads=1, fast=True):
+    """the CPU baseline's timing mode of tetmesh_dq2: (H, [element + triplet seconds, setFromTriplets
+    seconds]) with the elements split over `threads` OpenMP threads"""
+    L = load(fast=fast)
+    E, dX, params = _F(E, np.int32), _F(dX, np.float64), _F(params, np.float64)
+    q, vol = np.ascontiguousarray(q, dtype=np.float64), np.ascontiguousarray(vol, dtype=np.float64)
+    times = np.zeros(2)
+    ptr = L.or_tetmesh_dq2_timed(model, nV, E.shape[0], _p(E), _p(q), _p(dX), _p(vol), _p(params), 1 if psd else 0,
+                                 psd_tol, threads, _p(times))
+    return _take_csc(ptr), times
 
 
 def psd_fix(A, tol):

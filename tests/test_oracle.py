@@ -276,3 +276,64 @@ def test_backward_euler_free_fall_translation(oracle):
     qp = x1 + dt * v1
     x2, _, _, _ = oracle.newton_be(0, nV, E, dX, vol, params, qp, qp, dt, rho, None, f, tol=1e-12, max_it=5)
     assert np.abs(x2 - x1 - 2 * dt * dt * np.tile(g, nV)).max() < 1e-12
+
+
+def test_psd_fix_ql_matches_eigh(oracle):
+    """The CPU baseline's tridiagonalisation + implicit QR simple_psd_fix against numpy's eigh spectral
+    function and the checker's Jacobi version: random symmetric matrices (indefinite, wide scale
+    ranges, repeated eigenvalues, already-diagonal and already-tridiagonal inputs) and real element
+    Hessians of strongly distorted tets."""
+    rng = np.random.default_rng(3)
+
+    def spectral(A, tol):
+        w, V = np.linalg.eigh(A)
+        return (V * np.maximum(w, tol)) @ V.T
+
+    cases = []
+    for k in range(20):
+        A = rng.standard_normal((12, 12))
+        cases.append(((A + A.T) * 10.0 ** rng.uniform(-4, 6), 10.0 ** rng.uniform(-3, 2)))
+    Q, _ = np.linalg.qr(rng.standard_normal((12, 12)))
+    w = np.array([3.0, 3.0, 3.0, -1.0, -1.0, 0.0, 0.0, 0.0, 5.0, 7.0, 7.0, 1e-9])
+    cases.append(((Q * w) @ Q.T, 1e-3))  # repeated eigenvalues
+    cases.append((np.diag(rng.standard_normal(12)), 0.1))  # already diagonal
+    off = rng.standard_normal(11)
+    T = np.diag(rng.standard_normal(12)) + np.diag(off, 1) + np.diag(off, -1)
+    T[4, 5] = T[5, 4] = 0.0  # a split tridiagonal
+    cases.append((T, 0.2))
+    V, E = meshgen.beam_grid(4, 3, 3, h=0.02)
+    q = (V + 0.3 * 0.02 * rng.standard_normal(V.shape)).ravel()
+    dX, vol = oracle.dphidX_volume(V, E)
+    lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+    params = meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu)
+    for e in range(0, E.shape[0], 7):
+        cases.append((oracle.element_hessian(0, e, E, q, dX, vol, params), 1e-3))
+    for A, tol in cases:
+        ref = spectral(A, tol)
+        got = oracle.psd_fix_ql(A, tol)
+        scale = max(np.abs(A).max(), tol)
+        assert np.abs(got - ref).max() < 1e-13 * scale * 12
+        assert np.abs(got - oracle.psd_fix(A, tol)).max() < 1e-12 * scale
+        assert np.array_equal(got, got.T)
+
+
+def test_timed_dq2_matches_checker(oracle):
+    """the CPU baseline's timing mode (QL simple_psd_fix, triplets at 144 e, OpenMP over elements)
+    assembles the checker's matrix: same pattern, values to rounding, identical bits for 1 and 4
+    threads (the triplet list is in element order whatever the thread count)"""
+    import scipy.sparse as sp
+
+    V, E = meshgen.beam_grid(8, 4, 4, h=0.02)
+    rng = np.random.default_rng(5)
+    q = (V + 0.2 * 0.02 * rng.standard_normal(V.shape)).ravel()
+    dX, vol = oracle.dphidX_volume(V, E)
+    lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+    params = meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu)
+    nV = V.shape[0]
+    H_ref = oracle.tetmesh_dq2(0, nV, E, q, dX, vol, params, psd=True, psd_tol=1e-3)
+    H1, t1 = oracle.tetmesh_dq2_timed(0, nV, E, q, dX, vol, params, psd=True, psd_tol=1e-3, threads=1, fast=False)
+    H4, t4 = oracle.tetmesh_dq2_timed(0, nV, E, q, dX, vol, params, psd=True, psd_tol=1e-3, threads=4, fast=False)
+    assert np.array_equal(H1.indptr, H_ref.indptr) and np.array_equal(H1.indices, H_ref.indices)
+    assert sp.linalg.norm(H1 - H_ref) / sp.linalg.norm(H_ref) < 1e-13
+    assert np.array_equal(H1.data.view(np.uint64), H4.data.view(np.uint64))
+    assert t1[0] > 0 and t1[1] > 0
