@@ -148,6 +148,10 @@ SIGNATURES = {
     "bh_mgn_train_step_device": (c_int, [_VP, _VP, _VP, _VP, _VP, c_int, ctypes.c_uint64, c_double, P(c_double), _VP]),
     "bh_mgn_factor_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mgn_debug_buffers": (c_int, [_VP, P(_VP), P(_VP), P(_VP), P(_VP)]),
+    "bh_read_tetgen": (c_int, [ctypes.c_char_p, ctypes.c_char_p, P(c_int), P(c_int), _VP, _VP]),
+    "bh_read_medit": (c_int, [ctypes.c_char_p, P(c_int), P(c_int), _VP, _VP]),
+    "bh_save_market": (c_int, [ctypes.c_char_p, c_int64, c_int64, _VP, _VP, _VP]),
+    "bh_load_market": (c_int, [ctypes.c_char_p, P(c_int64), P(c_int64), P(c_int64), _VP, _VP, _VP]),
     "bh_mesh_set_profiling": (c_int, [_VP, c_int]),
     "bh_mesh_psd_count": (c_int, [_VP, P(c_int64)]),
     "bh_mesh_kernel_times": (c_int, [_VP, _VP, _VP]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libbartels_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "bartels_hip.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "bartels_hip.hip"), os.path.join(HERE, "csrc", "io.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "elements.h"), os.path.join(HERE, "csrc", "newton_impl.h"),
            os.path.join(ROOT, "include", "bartels_hip.h")]
 

@@ -141,29 +141,12 @@ def jitter(V, E, h, amount, seed, max_rounds=50):
 
 
 def read_mesh(path):
-    """MEDIT .mesh reader (vertices and tetrahedra; 1-based indices in the file)."""
-    with open(path) as f:
-        toks = f.read().split()
-    i, V, T = 0, None, None
-    while i < len(toks):
-        t = toks[i]
-        if t == "Vertices":
-            n = int(toks[i + 1])
-            arr = np.array(toks[i + 2:i + 2 + 4 * n], dtype=np.float64).reshape(n, 4)
-            V = arr[:, :3].copy()
-            i += 2 + 4 * n
-        elif t == "Tetrahedra":
-            n = int(toks[i + 1])
-            arr = np.array(toks[i + 2:i + 2 + 5 * n], dtype=np.int64).reshape(n, 5)
-            T = (arr[:, :4] - 1).astype(np.int32)
-            i += 2 + 5 * n
-        elif t in ("Triangles", "Edges"):
-            n = int(toks[i + 1])
-            w = 4 if t == "Triangles" else 3
-            i += 2 + w * n
-        else:
-            i += 1
-    return V, T
+    """MEDIT .mesh reader (vertices and tetrahedra, returned 0-based): the library's C++ reader,
+    bh_read_medit."""
+    from .io import read_mesh as _read
+
+    V, E = _read(path)
+    return np.ascontiguousarray(V), np.ascontiguousarray(E)
 
 
 # ---- the configurations --------------------------------------------------------------------------

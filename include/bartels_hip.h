@@ -118,6 +118,30 @@ typedef struct bh_tuning {
 } bh_tuning;
 int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
 
+/* ---- file formats (host only; no device call) ------------------------------------------------
+ * Two-call convention: with NULL output arrays only the sizes are returned; called again with arrays
+ * of those sizes they are filled.  V nV x 3 double and E nT x 4 int32, column-major (Eigen's layout). */
+
+/* Replaces sim::read_tetgen (src/read_tetgen.cpp:105-124, loadTet :32-99): TetGen .node / .ele.  Lines
+ * shorter than 3 characters and '#' comments are skipped; each record's leading index is dropped and the
+ * vertex indices are returned as written in the file (the reference does not renumber them; TetGen
+ * files written with -z are 0-based).  Only the first 4 nodes of each element are kept. */
+int bh_read_tetgen(const char* node_file, const char* ele_file, int* nV, int* nT, double* V, int32_t* E);
+/* Replaces igl::readMESH as the reference's tests use it (tests/test_assemble.cpp:25,
+ * tests/test_constitutive_model.cpp:26): MEDIT .mesh, "Vertices" and "Tetrahedra" sections (other
+ * sections skipped), indices returned 0-based. */
+int bh_read_medit(const char* path, int* nV, int* nT, double* V, int32_t* E);
+/* Replaces Eigen::saveMarket for the reference's CSC matrices (the format of the files in data/matrices):
+ * "%%MatrixMarket matrix coordinate  real general", "rows cols nnz", then one "row col value" line per
+ * stored entry (explicit zeros kept), 1-based, column by column with rows ascending, values "%.64e". */
+int bh_save_market(const char* path, int64_t rows, int64_t cols, const int32_t* colptr, const int32_t* rowind,
+                   const double* vals);
+/* Replaces Eigen::loadMarket (tests/test_assemble.cpp:33): coordinate real general or symmetric (the
+ * mirror entries added); duplicates summed in file order, rows sorted within each column (the semantics
+ * of setFromTriplets).  colptr: cols + 1 entries, rowind / vals: nnz entries. */
+int bh_load_market(const char* path, int64_t* rows, int64_t* cols, int64_t* nnz, int32_t* colptr, int32_t* rowind,
+                   double* vals);
+
 /* ---- energy / gradient / Hessian (host arrays) -------------------------------------------- */
 
 /* Replaces sim::linear_tetmesh_{neohookean,stvk,corotational,arap}_q
