@@ -87,7 +87,7 @@ class Solver:
         self.q0 = p.V.ravel().copy()
 
     def newton(self, precond, mgn=None, precision=1, max_it=4, pcg_tol=1e-6, pcg_max=5000, on_system=None,
-               q0=None, psd=True, model=None):
+               q0=None, psd=True, model=None, coarse=0):
         import torch
 
         lib = self.lib
@@ -108,6 +108,7 @@ class Solver:
         o.precond_handle = self.P
         o.element_precision = precision
         o.jacobi_fallback = 1
+        o.coarse_correction = coarse
         if mgn is not None:
             o.mgn = mgn
         cb = _lib.ON_SYSTEM(on_system) if on_system else _lib.ON_SYSTEM()
@@ -125,7 +126,7 @@ class Solver:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         if st not in (_lib.BH_OK, _lib.BH_MAXIT):
-            raise _lib.BartelsError(f"bh_newton_static_device failed with status {st}")
+            raise _lib.BartelsError(st, "bh_newton_static_device")
         return dq, rep, trace.reshape(-1, 4)[:rep.iterations], wall
 
     def close(self):
@@ -174,13 +175,13 @@ class Network:
 
 
 def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, form="sgs", probe_solution=True,
-          log=print):
+          log=print, ny_range=(24, 41)):
     net = Network(seed=seed + 12345, form=form, probe_solution=probe_solution)
     rng = np.random.default_rng(seed)
     history = []
     t0 = time.perf_counter()
     for i in range(solves):
-        ny = int(rng.integers(24, 41))
+        ny = int(rng.integers(*ny_range))
         grid = (4 * ny, ny, ny)
         p = random_problem(grid, seed=1000 + 7 * i + seed)
         S = Solver(p)
@@ -213,22 +214,30 @@ def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, form="s
     return history
 
 
-def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="sgs", log=print):
-    """held-out 32x32x128 problems: PCG iterations and time to 1e-6 over the first Newton steps,
+def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="sgs", coarse=0, log=print,
+             grid=(128, 32, 32)):
+    """held-out problems (32x32x128 by default): PCG iterations and time to 1e-6 over the first Newton steps,
     block-Jacobi PCG, block symmetric Gauss-Seidel and the learned preconditioner (all in the precision
-    form)"""
+    form); coarse = 1 adds the two-level coarse correction to the learned factor (and evaluates two-level
+    block Jacobi beside it).  Every solve reports its inner PCG failures (after the Jacobi fallback)."""
     net = Network(weights=weights, form=form)
     rows = []
     for i in range(problems):
-        p = random_problem((128, 32, 32), seed=90000 + seed + 13 * i)
+        p = random_problem(grid, seed=90000 + seed + 13 * i)
         S = Solver(p)
         _, rj, _, wj = S.newton(0, precision=1, max_it=newton_its, pcg_max=pcg_max)
-        _, rl, _, wl = S.newton(net.precond, mgn=net.h, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        _, rl, _, wl = S.newton(net.precond, mgn=net.h, precision=1, max_it=newton_its, pcg_max=pcg_max,
+                                coarse=coarse)
         _, rb, _, wb = S.newton(5, precision=1, max_it=newton_its, pcg_max=pcg_max)
+        row = dict(problem=i, jacobi_pcg=rj.pcg_iterations_total, learned_pcg=rl.pcg_iterations_total,
+                   sgs_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, sgs_s=wb,
+                   fallbacks=rl.pcg_fallbacks, learned_failures=rl.pcg_failures, newton=rl.iterations,
+                   learned_rel_gradient=rl.gradient_norm / rl.gradient_norm0 if rl.gradient_norm0 else 0.0)
+        if coarse:
+            _, r2, _, w2 = S.newton(7, precision=1, max_it=newton_its, pcg_max=pcg_max)
+            row.update(jacobi2l_pcg=r2.pcg_iterations_total, jacobi2l_s=w2)
         S.close()
-        rows.append(dict(problem=i, jacobi_pcg=rj.pcg_iterations_total, learned_pcg=rl.pcg_iterations_total,
-                         sgs_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, sgs_s=wb,
-                         fallbacks=rl.pcg_fallbacks))
+        rows.append(row)
         log(json.dumps(rows[-1]))
     net.close()
     return rows

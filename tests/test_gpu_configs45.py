@@ -1,0 +1,153 @@
+"""Configurations 4 and 5 on the GPU, at reduced size.
+
+Configuration 5: backward-Euler steps of the clamped beam falling under gravity with per-tet Young's
+modulus log-uniform in [1e5, 1e7] Pa, solved with the learned directions (the learned block factor
+with the two-level coarse correction, and the learned Eisenstat diagonal), the Jacobi fallback on,
+against the oracle's backward Euler with the direct (banded LU) solve
+(matlab/examples/test_tet_materials.m:84-112): within 1e-5 in the precision form, and in FP64 within
+1e-7 or three times the oracle's own floor on this problem, whichever is larger.
+
+Configuration 4: the learned preconditioner's protocol end to end -- training through the plain
+solver's Newton systems (bh_newton_opts.on_system) on jittered beams with random materials and end
+tractions, then a held-out 32x32x128 problem solved with block-Jacobi, block Gauss-Seidel and the
+learned preconditioner; every learned solve must reach its 1e-6 relative residual, directly or
+through the fallback."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from bartels_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _network(form, seed):
+    """the shipped weights when present, else a network with every parameter (decoders included)
+    perturbed at random, so that inference writes a factor unlike the untrained block Jacobi / SGS"""
+    from bartels_amd import learned
+
+    if os.path.exists(learned.DEFAULT_WEIGHTS) and form == "factor":
+        return learned.Network(weights=learned.DEFAULT_WEIGHTS, form=form), "shipped"
+    net = learned.Network(seed=seed, form=form)
+    w = net.params()
+    w = w + 0.02 * np.random.default_rng(seed).standard_normal(w.size).astype(np.float32)
+    from bartels_amd import _lib
+
+    _lib.check(net.lib.bh_mgn_set_params(net.h, w.ctypes.data_as(ctypes.c_void_p)), "set_params")
+    return net, "random"
+
+
+def test_config5_backward_euler_heterogeneous_learned(oracle):
+    import torch
+
+    from bartels_amd import _lib
+    from bartels_amd import tetmesh as tm
+
+    p = meshgen.config5(grid=(12, 4, 4))
+    nV, nT = p.V.shape[0], p.E.shape[0]
+    dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
+    mesh = tm.TetMesh(nV, p.E, dX, vol, V=p.V)
+    fixed = np.zeros(3 * nV, np.uint8)
+    fixed[(3 * p.fixed[:, None] + np.arange(3)).ravel()] = 1
+    f = oracle.const_ext_force(nV, p.E, vol, p.density * p.gravity)
+    dt = p.dt
+    lib = _lib.load()
+    dens = torch.full((nT,), p.density, dtype=torch.float64, device="cuda")
+    mvals = torch.empty(mesh.nnz, dtype=torch.float64, device="cuda")
+    assert lib.bh_linear_tetmesh_mass_matrix_device(mesh.handle, ctypes.c_void_p(dens.data_ptr()),
+                                                    ctypes.c_void_p(mvals.data_ptr()), None) == 0
+    dparams = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
+    dfixed = torch.tensor(fixed, device="cuda")
+    df = torch.tensor(f, device="cuda")
+    P = ctypes.c_void_p()
+    assert lib.bh_precond_create(mesh.handle, ctypes.byref(P)) == 0
+    nets = {form: _network(form, 11 + k) for k, form in enumerate(("factor", "sgs"))}
+    # oracle trajectories: three steps, direct solve.  On this soft, heterogeneous beam the reference's
+    # energy-based Armijo test stalls at |g| ~ 7e-11, where accepted steps of ~1e-3 of a rounding-level
+    # Newton direction still move q: two terminations of the same oracle (|g| < 1e-9 within 12 iterations,
+    # and 30 iterations aiming at 1e-11) end ~3e-7 apart.  That spread is the FP64 floor of the comparison.
+    def oracle_traj(tol, max_it):
+        xo, vo = p.V.ravel().copy(), np.zeros(3 * nV)
+        for step in range(3):
+            qpo = xo + dt * vo
+            q_or, _, _, _ = oracle.newton_be(0, nV, p.E, dX, vol, p.params, qpo, qpo, dt, p.density, fixed, f,
+                                             tol=tol, max_it=max_it)
+            vo, xo = (q_or - xo) / dt, q_or
+        return xo
+
+    traj = [oracle_traj(1e-9, 12)]
+    u = np.linalg.norm(traj[-1] - p.V.ravel())
+    floor = np.linalg.norm(oracle_traj(1e-11, 30) - traj[-1]) / u
+    print(f"oracle FP64 floor (two terminations): {floor:.2e}")
+    try:
+        # (name, precision, preconditioner, coarse correction, network form, bound)
+        b64 = max(1e-7, 3.0 * floor)
+        variants = [("fp64_jacobi", 0, 0, 0, None, b64), ("fp64_learned_factor_2l", 0, 4, 1, "factor", b64),
+                    ("fp32_learned_factor_2l", 1, 4, 1, "factor", 1e-5), ("fp32_learned_sgs", 1, 6, 0, "sgs", 1e-5),
+                    ("fp32_bic0_2l", 1, 2, 1, None, 1e-5)]
+        for name, prec, precond, coarse, form, bound in variants:
+            x, v = p.V.ravel().copy(), np.zeros(3 * nV)
+            fb = pits = 0
+            for step in range(3):
+                qp = x + dt * v
+                o = _lib.BhNewtonOpts()
+                lib.bh_newton_default_opts(ctypes.byref(o))
+                o.model = 0
+                o.gradient_tol = 1e-10
+                o.relative_tol = 1
+                o.max_iterations = 8
+                o.pcg_tol = 1e-12
+                o.element_precision = prec
+                o.precond = precond
+                o.precond_handle = P
+                o.coarse_correction = coarse
+                o.jacobi_fallback = 1
+                if form is not None:
+                    o.mgn = nets[form][0].h
+                dqp = torch.tensor(qp, device="cuda")
+                o.d_mass_vals = ctypes.c_void_p(mvals.data_ptr())
+                o.d_q_pred = ctypes.c_void_p(dqp.data_ptr())
+                o.inertia = 1.0 / dt ** 2
+                dq = torch.tensor(qp, device="cuda")
+                rep = _lib.BhNewtonReport()
+                st = lib.bh_newton_static_device(mesh.handle, ctypes.byref(o), ctypes.c_void_p(dparams.data_ptr()),
+                                                 ctypes.c_void_p(dfixed.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+                                                 ctypes.c_void_p(dq.data_ptr()), ctypes.byref(rep), None,
+                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+                assert st in (_lib.BH_OK, _lib.BH_MAXIT), st
+                assert rep.pcg_failures == 0  # every direction reached its tolerance, directly or by fallback
+                fb += rep.pcg_fallbacks
+                pits += rep.pcg_iterations_total
+                q = dq.cpu().numpy()
+                v, x = (q - x) / dt, q
+            err = np.linalg.norm(x - traj[-1]) / u
+            net_kind = nets[form][1] if form else "-"
+            print(f"config 5 (heterogeneous E), {name} (network {net_kind}): displacement vs oracle after 3 steps "
+                  f"{err:.2e}, CG iterations {pits}, Jacobi fallbacks {fb}")
+            assert err < bound
+    finally:
+        for net, _ in nets.values():
+            net.close()
+        lib.bh_precond_destroy(P)
+        mesh.close()
+
+
+def test_config4_protocol_reduced(tmp_path):
+    """train on three jittered beams of 48..64 x 12..16 x 12..16 vertices (random materials, random end
+    tractions; two Newton systems each, the learned factor form), then solve one held-out 32x32x128
+    problem (the configuration's evaluation size) with block Jacobi, block Gauss-Seidel, two-level
+    block Jacobi and the learned factor with the coarse correction"""
+    from bartels_amd import learned
+
+    out = str(tmp_path / "weights.f32")
+    hist = learned.train(3, out, seed=4, steps=2, newton_its=2, probes=4, form="factor", ny_range=(12, 17),
+                         log=lambda s: print(s, flush=True))
+    assert len(hist) == 3 and all(h["loss_last"] is not None and np.isfinite(h["loss_last"]) for h in hist)
+    rows = learned.evaluate(out, problems=1, newton_its=2, form="factor", coarse=1, log=lambda s: print(s, flush=True))
+    r = rows[0]
+    print(f"held-out 32x32x128: CG iterations block Jacobi {r['jacobi_pcg']}, block SGS {r['sgs_pcg']}, two-level "
+          f"block Jacobi {r['jacobi2l_pcg']}, learned factor + coarse {r['learned_pcg']} (fallbacks {r['fallbacks']})")
+    assert r["learned_failures"] == 0
+    assert r["newton"] == 2
