@@ -104,6 +104,7 @@ SIGNATURES = {
     "bh_linear_tetmesh_dphi_dX": (c_int, [c_int, _VP, c_int, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dphi_dX_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mesh_create": (c_int, [c_int, c_int, _VP, _VP, _VP, P(c_void_p)]),
+    "bh_mesh_create_device": (c_int, [c_int, c_int, _VP, _VP, _VP, _VP, P(c_void_p)]),
     "bh_mesh_destroy": (c_int, [_VP]),
     "bh_mesh_nnz": (c_int, [_VP, P(c_int64)]),
     "bh_mesh_pattern": (c_int, [_VP, _VP, _VP]),

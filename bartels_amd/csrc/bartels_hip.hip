@@ -1238,6 +1238,8 @@ int stage_inputs(bh_mesh* m, const double* q, const double* params, const uint8_
 // host-side pattern construction
 namespace {
 
+int finish_mesh(bh_mesh* m);
+
 int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* volume) {
   const int nV = m->nV, nT = m->nT;
   // incidence lists (ascending element order by construction)
@@ -1337,6 +1339,14 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
     for (int v = 0; v < nV; ++v) vdeg[v] = incPtr[v + 1] - incPtr[v];
     if ((st = upload(&m->vdeg, vdeg.data(), vdeg.size()))) return st;
   }
+  m->h_bptr.swap(bptr);
+  m->h_bcol.swap(bcol);
+  return finish_mesh(m);
+}
+
+// the per-call buffers of a mesh whose pattern and incidence lists are built
+int finish_mesh(bh_mesh* m) {
+  const int nV = m->nV, nT = m->nT;
   const int per = kEnergyBlock * kEnergyItems;
   m->nPartials = std::max(1, (nT + per - 1) / per);
   BH_CHECK(hipMalloc(&m->partials, sizeof(double) * m->nPartials));
@@ -1345,13 +1355,10 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
   BH_CHECK(hipMalloc(&m->stage_params, sizeof(double) * 2 * std::max(1, nT)));
   BH_CHECK(hipMalloc(&m->stage_out, sizeof(double) * 3 * std::max(1, nV)));
   BH_CHECK(hipMalloc(&m->stage_ids, std::max(1, nT)));
-  m->h_bptr.swap(bptr);
-  m->h_bcol.swap(bcol);
   BH_CHECK(hipGetDevice(&m->device));
   // set-up of the allocation-free energy entry point on this device (an address lookup, no allocation)
   double* scr = nullptr;
-  if ((st = noalloc_scratch(&scr))) return st;
-  return BH_OK;
+  return noalloc_scratch(&scr);
 }
 
 void free_mesh(bh_mesh* m) {
@@ -1729,3 +1736,4 @@ int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
 #include "precond_impl.h"
 #include "mgn_impl.h"
 #include "newton_impl.h"
+#include "setup_impl.h"

@@ -84,6 +84,25 @@ class TetMesh:
             Vc = _colmajor(np.asarray(V, dtype=np.float64).reshape(self.nV, 3), np.float64)
             check(lib.bh_mesh_set_rest_positions(h, _ptr(Vc)), "bh_mesh_set_rest_positions")
 
+    @classmethod
+    def from_device(cls, nV, E, V=None, dphidX=None, volume=None):
+        """the same mesh built on the device (bh_mesh_create_device) from torch tensors on cuda:0: E
+        (nT x 4 column-major int32, flattened), and either dphidX (nT x 12 column-major) and volume, or
+        V (nV x 3 column-major) to compute them on the device"""
+        self = cls.__new__(cls)
+        lib = _lib.load()
+        dp = cls._dp
+        nT = E.numel() // 4
+        h = ctypes.c_void_p()
+        check(lib.bh_mesh_create_device(int(nV), int(nT), dp(E), dp(V), dp(dphidX), dp(volume), ctypes.byref(h)),
+              "bh_mesh_create_device")
+        self._h, self._lib, self.nV, self.nT = h, lib, int(nV), int(nT)
+        nnz = ctypes.c_int64()
+        check(lib.bh_mesh_nnz(h, ctypes.byref(nnz)), "bh_mesh_nnz")
+        self.nnz = int(nnz.value)
+        self._pattern = None
+        return self
+
     @property
     def handle(self):
         return self._h

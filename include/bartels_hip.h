@@ -85,6 +85,14 @@ int bh_linear_tetmesh_dphi_dX_device(bh_mesh* mesh, const double* d_V, double* d
  * CSR slots.  Copies E, dphidX and volume (host arrays, reference layouts) to the device.
  * Replaces the per-call triplet list of sim::assemble (src/assemble.cpp:36-97). */
 int bh_mesh_create(int nV, int nT, const int* E, const double* dphidX, const double* volume, bh_mesh** mesh);
+/* The same mesh state built on the device from device arrays (no host copy of E): the incidence lists by
+ * a stable radix sort of the (vertex, element) pairs, the pattern by a sort-unique of the 16 nT vertex
+ * pairs, the ranks by binary search -- the same arrays as bh_mesh_create, bit for bit.  Replaces the
+ * reference's per-call triplet pattern (src/assemble.cpp:57-94) and, when d_dphidX and d_volume are
+ * NULL, sim::linear_tetmesh_dphi_dX (src/linear_tetmesh_dphi_dX.cpp:5-18) with the signed volume from
+ * d_V (nV x 3, column-major).  Synchronous. */
+int bh_mesh_create_device(int nV, int nT, const int32_t* d_E, const double* d_V, const double* d_dphidX,
+                          const double* d_volume, bh_mesh** mesh);
 int bh_mesh_destroy(bh_mesh* mesh);
 int bh_mesh_nnz(const bh_mesh* mesh, int64_t* nnz);
 /* rowptr: 3*nV + 1 entries, colind: nnz entries (host arrays). */
