@@ -83,6 +83,7 @@ struct bh_mesh {
   // host-API staging
   double* stage_q = nullptr;
   double* stage_params = nullptr;
+  int stageParamCols = 2;
   double* stage_out = nullptr;
   uint8_t* stage_ids = nullptr;
   double* stage_vals = nullptr;
@@ -275,6 +276,13 @@ __device__ __forceinline__ void select_row(const T phi[4][3], int a, T* pa) {
   for (int j = 0; j < 3; ++j) pa[j] = (a == 0) ? phi[0][j] : (a == 1) ? phi[1][j] : (a == 2) ? phi[2][j] : phi[3][j];
 }
 
+// the muscle fibre's columns of a tet's parameters (M_NH_MUSCLE: C, D, a, d0, d1, d2)
+template <typename T>
+__device__ __forceinline__ void load_fibre(const T* __restrict__ params, int nT, int e, T fib[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) fib[c] = params[(int64_t)(2 + c) * nT + e];
+}
+
 // ---- energy ----------------------------------------------------------------------------------
 template <int MODEL, typename T>
 __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __restrict__ q,
@@ -285,6 +293,11 @@ __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __
   const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
   const T* Gp = (sizeof(T) == 4) ? G : nullptr;
   double v;
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    return (psi<M_NH>(F, p0, p1, Gp) + fibre_energy(F, fib)) * (double)load_vol<T>(m, e);
+  }
   switch (model_at<MODEL>(mids, e)) {
     case M_NH: v = psi<M_NH>(F, p0, p1, Gp); break;
     case M_STVK: v = psi<M_STVK>(F, p0, p1, Gp); break;
@@ -353,7 +366,12 @@ __global__ void __launch_bounds__(256)
     load_FG(m, q, load_E(m, e), phi, F, G);
     const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
     const T* Gp = (sizeof(T) == 4) ? G : nullptr;
-    switch (model_at<MODEL>(mids, e)) {
+    if constexpr (MODEL == M_NH_MUSCLE) {
+      T fib[4];
+      load_fibre(params, m.nT, e, fib);
+      piola<M_NH>(F, p0, p1, P, Gp);
+      fibre_piola_add(F, fib, P);
+    } else switch (model_at<MODEL>(mids, e)) {
       case M_NH: piola<M_NH>(F, p0, p1, P, Gp); break;
       case M_STVK: piola<M_STVK>(F, p0, p1, P, Gp); break;
       case M_COROT: piola<M_COROT>(F, p0, p1, P); break;
@@ -419,6 +437,11 @@ __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __re
   T pa[3];
   select_row(phi, a, pa);
   pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    set_fibre(P, fib);
+  }
 #pragma unroll
   for (int mm = 0; mm < 4; ++mm) hess_block<MODEL>(P, pa, phi[mm], blk[mm]);
 }
@@ -451,6 +474,11 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
   T pa[3];
   select_row(phi, a, pa);
   pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    set_fibre(P, fib);
+  }
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     T pm[3];
@@ -484,6 +512,11 @@ __device__ __forceinline__ void tet_block1(const MeshView& m, const double* __re
   select_row(phi, a, pa);
   select_row(phi, mm, pm);
   pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    set_fibre(P, fib);
+  }
   hess_block<MODEL>(P, pa, pm, blk[0]);
 }
 
@@ -514,7 +547,9 @@ __global__ void __launch_bounds__(kGatherThreads)
       const int e = code >> 2, a = code & 3;
       const uint32_t rk = m.incR[idx];
       T blk[4][9];
-      switch (model_at<MODEL>(mids, e)) {
+      if constexpr (MODEL == M_NH_MUSCLE) {
+        tet_blocks<M_NH_MUSCLE>(m, q, params, e, a, stretchRef, blk);
+      } else switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_blocks<M_NH>(m, q, params, e, a, stretchRef, blk); break;
         case M_STVK: tet_blocks<M_STVK>(m, q, params, e, a, stretchRef, blk); break;
         case M_COROT: tet_blocks<M_COROT>(m, q, params, e, a, stretchRef, blk); break;
@@ -569,7 +604,9 @@ __global__ void __launch_bounds__(kGatherThreads)
       const int e = code >> 2, a = code & 3;
       const uint32_t rk = m.incR[idx];
       T blk[2][9];
-      switch (model_at<MODEL>(mids, e)) {
+      if constexpr (MODEL == M_NH_MUSCLE) {
+        tet_blocks2<M_NH_MUSCLE>(m, q, params, e, a, h, stretchRef, blk);
+      } else switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_blocks2<M_NH>(m, q, params, e, a, h, stretchRef, blk); break;
         case M_STVK: tet_blocks2<M_STVK>(m, q, params, e, a, h, stretchRef, blk); break;
         case M_COROT: tet_blocks2<M_COROT>(m, q, params, e, a, h, stretchRef, blk); break;
@@ -626,7 +663,9 @@ __global__ void __launch_bounds__(kGatherThreads)
       const int e = code >> 2, a = code & 3;
       const uint32_t rk = m.incR[idx];
       T blk[1][9];
-      switch (model_at<MODEL>(mids, e)) {
+      if constexpr (MODEL == M_NH_MUSCLE) {
+        tet_block1<M_NH_MUSCLE>(m, q, params, e, a, c, stretchRef, blk);
+      } else switch (model_at<MODEL>(mids, e)) {
         case M_NH: tet_block1<M_NH>(m, q, params, e, a, c, stretchRef, blk); break;
         case M_STVK: tet_block1<M_STVK>(m, q, params, e, a, c, stretchRef, blk); break;
         case M_COROT: tet_block1<M_COROT>(m, q, params, e, a, c, stretchRef, blk); break;
@@ -669,13 +708,20 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
   pre_hessian<MODEL>(P, p0, p1, stretchRef);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    set_fibre(P, fib);
+  }
   reduced_packed<MODEL>(P, R, M);
 }
 
 template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const T* params,
                                                    const uint8_t* mids, int e, bool stretchRef, T M[45]) {
-  switch (model_at<MODEL>(mids, e)) {
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    reduced_M<M_NH_MUSCLE>(m, q, params, e, stretchRef, M);
+  } else switch (model_at<MODEL>(mids, e)) {
     case M_NH: reduced_M<M_NH>(m, q, params, e, stretchRef, M); break;
     case M_STVK: reduced_M<M_STVK>(m, q, params, e, stretchRef, M); break;
     case M_COROT: reduced_M<M_COROT>(m, q, params, e, stretchRef, M); break;
@@ -939,7 +985,7 @@ __global__ void __launch_bounds__(256) k_to_f32(int64_t n, const double* __restr
 }
 
 // ---------------------------------------------------------------------------------------------
-bool is_model(int model) { return model >= 0 && model <= 4; }
+bool is_model(int model) { return model >= 0 && model <= 5; }
 
 int max_lds_bytes() {
   static int lim = -1;
@@ -1115,6 +1161,7 @@ int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q
     case M_COROT: return hessian_model<M_COROT>(m, q, params, mids, psd, vals, s);
     case M_ARAP: return hessian_model<M_ARAP>(m, q, params, mids, psd, vals, s);
     case M_NH_LEGACY: return hessian_model<M_NH_LEGACY>(m, q, params, mids, psd, vals, s);
+    case M_NH_MUSCLE: return hessian_model<M_NH_MUSCLE>(m, q, params, mids, psd, vals, s);
     default: return BH_BAD_ARG;
   }
 }
@@ -1186,6 +1233,7 @@ int energy_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q,
     case M_COROT: return launch_energy<M_COROT>(m, q, params, mids, out, s, partials);
     case M_ARAP: return launch_energy<M_ARAP>(m, q, params, mids, out, s, partials);
     case M_NH_LEGACY: return launch_energy<M_NH_LEGACY>(m, q, params, mids, out, s, partials);
+    case M_NH_MUSCLE: return launch_energy<M_NH_MUSCLE>(m, q, params, mids, out, s, partials);
     default: return BH_BAD_ARG;
   }
 }
@@ -1212,6 +1260,7 @@ int gradient_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* 
     case M_COROT: return launch_gradient<M_COROT>(m, q, params, mids, g, s);
     case M_ARAP: return launch_gradient<M_ARAP>(m, q, params, mids, g, s);
     case M_NH_LEGACY: return launch_gradient<M_NH_LEGACY>(m, q, params, mids, g, s);
+    case M_NH_MUSCLE: return launch_gradient<M_NH_MUSCLE>(m, q, params, mids, g, s);
     default: return BH_BAD_ARG;
   }
 }
@@ -1223,11 +1272,21 @@ int check_model_args(int model, const uint8_t* mids) {
 
 hipStream_t as_stream(bh_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-// host-API staging: copy q / params / ids in, run, copy out
-int stage_inputs(bh_mesh* m, const double* q, const double* params, const uint8_t* ids, hipStream_t s) {
+// host-API staging: copy q / params (param_cols(model) columns) / ids in, run, copy out
+int stage_inputs(bh_mesh* m, const double* q, const double* params, const uint8_t* ids, hipStream_t s,
+                 int model = M_NH) {
   if (q) BH_CHECK(hipMemcpyAsync(m->stage_q, q, sizeof(double) * 3 * (size_t)m->nV, hipMemcpyHostToDevice, s));
+  const int cols = ids ? 2 : param_cols(model);
+  if (cols > m->stageParamCols) {  // the muscle model's six columns: the staging buffer grows once
+    BH_CHECK(hipStreamSynchronize(s));
+    BH_CHECK(hipFree(m->stage_params));
+    m->stage_params = nullptr;
+    BH_CHECK(hipMalloc(&m->stage_params, sizeof(double) * cols * std::max<size_t>(1, m->nT)));
+    m->stageParamCols = cols;
+  }
   if (params)
-    BH_CHECK(hipMemcpyAsync(m->stage_params, params, sizeof(double) * 2 * (size_t)m->nT, hipMemcpyHostToDevice, s));
+    BH_CHECK(hipMemcpyAsync(m->stage_params, params, sizeof(double) * cols * (size_t)m->nT, hipMemcpyHostToDevice,
+                            s));
   if (ids) BH_CHECK(hipMemcpyAsync(m->stage_ids, ids, (size_t)m->nT, hipMemcpyHostToDevice, s));
   return BH_OK;
 }
@@ -1550,7 +1609,7 @@ int bh_linear_tetmesh_dq2_f32_device(bh_mesh* m, int model, const uint8_t* mids,
 int bh_linear_tetmesh_q(bh_mesh* m, int model, const double* q, const double* params, double* energy) {
   if (!m || !q || !params || !energy || !is_model(model)) return BH_BAD_ARG;
   hipStream_t s = nullptr;
-  int st = stage_inputs(m, q, params, nullptr, s);
+  int st = stage_inputs(m, q, params, nullptr, s, model);
   if (st) return st;
   st = energy_dispatch(m, model, nullptr, m->stage_q, m->stage_params, m->d_energy, s);
   if (st) return st;
@@ -1562,7 +1621,7 @@ int bh_linear_tetmesh_q(bh_mesh* m, int model, const double* q, const double* pa
 int bh_linear_tetmesh_dq(bh_mesh* m, int model, const double* q, const double* params, double* g) {
   if (!m || !q || !params || !g || !is_model(model)) return BH_BAD_ARG;
   hipStream_t s = nullptr;
-  int st = stage_inputs(m, q, params, nullptr, s);
+  int st = stage_inputs(m, q, params, nullptr, s, model);
   if (st) return st;
   st = gradient_dispatch(m, model, nullptr, m->stage_q, m->stage_params, m->stage_out, s);
   if (st) return st;
@@ -1582,7 +1641,7 @@ int bh_linear_tetmesh_dq2(bh_mesh* m, int model, const double* q, const double* 
   hipStream_t s = nullptr;
   int st = ensure_stage_vals(m);
   if (st) return st;
-  st = stage_inputs(m, q, params, nullptr, s);
+  st = stage_inputs(m, q, params, nullptr, s, model);
   if (st) return st;
   st = hessian_dispatch(m, model, nullptr, m->stage_q, m->stage_params, psd, m->stage_vals, s);
   if (st) return st;

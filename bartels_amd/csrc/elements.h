@@ -36,7 +36,18 @@
 
 namespace bh {
 
-enum : int { M_NH = 0, M_STVK = 1, M_COROT = 2, M_ARAP = 3, M_NH_LEGACY = 4, M_MASS = 100, M_PER_TET = 255 };
+enum : int {
+  M_NH = 0,
+  M_STVK = 1,
+  M_COROT = 2,
+  M_ARAP = 3,
+  M_NH_LEGACY = 4,
+  M_NH_MUSCLE = 5,  // Neo-Hookean plus a muscle fibre; params nT x 6: C, D, a, d0, d1, d2
+  M_MASS = 100,
+  M_PER_TET = 255
+};
+// parameter columns a model reads (nT x cols, column-major)
+__host__ __device__ constexpr int param_cols(int model) { return model == M_NH_MUSCLE ? 6 : 2; }
 
 template <typename T>
 __host__ __device__ __forceinline__ T det3(const T* F) {
@@ -210,7 +221,39 @@ struct Pre {
   T U[9], V[9];      // frame of G (reference layout: G = F^T)
   T psi2[9];         // vol * d2psi/ds2
   T A[3], B[3];      // vol * A_ij, vol * B_ij for pairs (0,1), (0,2), (1,2)
+  // muscle fibre (M_NH_MUSCLE)
+  T fa;              // vol * a
+  T fd[3];           // fibre direction d
 };
+
+// ---- muscle fibre: psi = a/2 |F d|^2 (src/psi_muscle_fiber_F.cpp:5-10), P = a (F d) d^T
+// (src/dpsi_muscle_fiber_dF.cpp, a D^T D vec F), d2psi = a D^T D (src/d2psi_muscle_fiber_dF2.cpp), with
+// D vec(F) = F d.  fib = {a, d0, d1, d2}.
+template <typename T>
+__host__ __device__ __forceinline__ double fibre_energy(const T* F, const T* fib) {
+  double Fd[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    Fd[i] = (double)F[i] * fib[1] + (double)F[i + 3] * fib[2] + (double)F[i + 6] * fib[3];
+  return 0.5 * (double)fib[0] * (Fd[0] * Fd[0] + Fd[1] * Fd[1] + Fd[2] * Fd[2]);
+}
+template <typename T>
+__host__ __device__ __forceinline__ void fibre_piola_add(const T* F, const T* fib, T* P) {
+  T Fd[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Fd[i] = F[i] * fib[1] + F[i + 3] * fib[2] + F[i + 6] * fib[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P[i + 3 * j] += fib[0] * Fd[i] * fib[1 + j];
+}
+template <typename T>
+__host__ __device__ __forceinline__ void set_fibre(Pre<T>& P, const T* fib) {
+  P.fa = P.vol * fib[0];
+  P.fd[0] = fib[1];
+  P.fd[1] = fib[2];
+  P.fd[2] = fib[3];
+}
 
 // stretch-energy derivatives in singular values. Corotational: params [lambda, mu]
 // (src/dpsi_corotational_dS.cpp:6-13); ARAP: k = params(0) (src/dpsi_arap_dS.cpp:6-12).
@@ -233,7 +276,7 @@ template <int MODEL, typename T>
 __host__ __device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool stretch_ref) {
   const T* F = P.F;
   const T vol = P.vol;
-  if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
+  if (MODEL == M_NH || MODEL == M_NH_LEGACY || MODEL == M_NH_MUSCLE) {
     double J;
     if constexpr (sizeof(T) == 8) {
       cof3(F, P.cof);
@@ -246,7 +289,7 @@ __host__ __device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool
     }
     const double iJ = 1.0 / J;
     const double C = p0, D = p1, vd = vol;
-    if (MODEL == M_NH) {
+    if (MODEL == M_NH || MODEL == M_NH_MUSCLE) {
       P.a0 = (T)(vd * 2.0 * C);                              // 2C
       P.a1 = (T)(vd * (2.0 * C * iJ * iJ + 2.0 * D));        // alpha
       P.a2 = (T)(vd * (2.0 * D * (J - 1.0) - 2.0 * C * iJ)); // beta
@@ -316,7 +359,7 @@ __host__ __device__ __forceinline__ void pre_hessian(Pre<T>& P, T p0, T p1, bool
 // 3x3 block K_am (row-major out[3l + k]).
 template <int MODEL, typename T>
 __host__ __device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa, const T* pm, T* out) {
-  if (MODEL == M_NH || MODEL == M_NH_LEGACY) {
+  if (MODEL == M_NH || MODEL == M_NH_LEGACY || MODEL == M_NH_MUSCLE) {
     T ca[3], cm[3], x[3], w[3];
     mv3(P.cof, pa, ca);
     mv3(P.cof, pm, cm);
@@ -341,6 +384,12 @@ __host__ __device__ __forceinline__ void hess_block(const Pre<T>& P, const T* pa
       for (int l = 0; l < 3; ++l)
 #pragma unroll
         for (int k = 0; k < 3; ++k) out[3 * l + k] += P.a3 * (ua[l] * cm[k] + ca[l] * um[k]);
+    }
+    if (MODEL == M_NH_MUSCLE) {  // + vol a (pa.d)(pm.d) I
+      const T f = P.fa * dot3(pa, P.fd) * dot3(pm, P.fd);
+      out[0] += f;
+      out[4] += f;
+      out[8] += f;
     }
   } else if (MODEL == M_STVK) {
     T ua[3], um[3], Sm[3];

@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(256)
 struct NewtonWs {
   double *vals, *g, *b, *x, *r, *z, *p, *s, *Ap, *qt, *Dinv, *hdiag, *part, *scal;
   float* vals32;    // precision form: FP32 Hessian values
-  float* params32;  // precision form: FP32 parameters (2 nT)
+  float* params32;  // precision form: FP32 parameters (up to 6 nT: param_cols of the model)
   PcgState* st;
   int* arrived;     // workgroups of the reduction done so far (zero between reductions)
 };
@@ -427,7 +427,7 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   const int64_t n = 3ll * m->nV;
   const int64_t nParts = 3 * part_region(m->nV) + 3 * kRed1Blocks + 16;
   const size_t need = sizeof(double) * ((size_t)m->nnz + 9 * (size_t)n + 18 * (size_t)m->nV + nParts + 16) +
-                      sizeof(float) * ((size_t)m->nnz + 2 * (size_t)m->nT) + sizeof(PcgState) + 512;
+                      sizeof(float) * ((size_t)m->nnz + 6 * (size_t)m->nT) + sizeof(PcgState) + 512;
   bool fresh = false;
   if (m->wsBytes < need) {
     if (m->ws) BH_CHECK(hipFree(m->ws));
@@ -453,7 +453,7 @@ int newton_workspace(bh_mesh* m, NewtonWs& w) {
   w.scal = q; q += 16;
   float* f = reinterpret_cast<float*>(q);
   w.vals32 = f; f += m->nnz;
-  w.params32 = f; f += 2ll * m->nT;
+  w.params32 = f; f += 6ll * m->nT;
   w.st = reinterpret_cast<PcgState*>(reinterpret_cast<uintptr_t>(f + 64) & ~uintptr_t(63));
   w.arrived = reinterpret_cast<int*>(w.st + 1);
   if (fresh) BH_CHECK(hipMemset(w.arrived, 0, sizeof(int)));
@@ -1528,8 +1528,8 @@ int bh_newton_static_device(bh_mesh* m, const bh_newton_opts* o, const double* p
   const bool f32g = o->element_precision == 2;
   if (o->element_precision < 0 || o->element_precision > 2) return BH_BAD_ARG;
   if (f32) {
-    hipLaunchKernelGGL(k_f64_to_f32, dim3((2ll * m->nT + 255) / 256), dim3(256), 0, s, 2ll * m->nT, params,
-                       w.params32);
+    const int64_t np = (int64_t)(o->d_model_ids ? 2 : param_cols(o->model)) * m->nT;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, np, params, w.params32);
     BH_CHECK(hipGetLastError());
   }
   do {

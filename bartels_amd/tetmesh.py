@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib
 from ._lib import check
 
-NEOHOOKEAN, STVK, COROTATIONAL, ARAP, NEOHOOKEAN_LEGACY = 0, 1, 2, 3, 4
+NEOHOOKEAN, STVK, COROTATIONAL, ARAP, NEOHOOKEAN_LEGACY, NEOHOOKEAN_MUSCLE = 0, 1, 2, 3, 4, 5
 MODEL_PER_TET = 255
 
 
@@ -306,3 +306,26 @@ def linear_tetmesh_generic_elastic_dq(V, E, q, model_ids, dphidX, volume, params
 
 def linear_tetmesh_generic_elastic_dq2(V, E, q, model_ids, dphidX, volume, params, callback=None):
     return _mesh_for(V, E, dphidX, volume).dq2(MODEL_PER_TET, q, params, callback, model_ids=model_ids)
+
+
+# matlab/linear_tetmesh_neohookean_muscle_{q,dq,dq2}.cpp: Neo-Hookean (params_nh nT x 2) plus a muscle fibre
+# per tet (directions nT x 3, params_a nT x 1); the element callback of dq2 as above
+def muscle_params(directions, params_nh, params_a):
+    """the nT x 6 parameter array of NEOHOOKEAN_MUSCLE: [C, D, a, d0, d1, d2]"""
+    nh = np.asarray(params_nh, dtype=np.float64).reshape(-1, 2)
+    a = np.asarray(params_a, dtype=np.float64).reshape(-1, 1)
+    d = np.asarray(directions, dtype=np.float64).reshape(-1, 3)
+    return np.asfortranarray(np.hstack([nh, a[:, :1], d]))
+
+
+def linear_tetmesh_neohookean_muscle_q(V, E, q, dphidX, volume, directions, params_nh, params_a):
+    return _mesh_for(V, E, dphidX, volume).q(NEOHOOKEAN_MUSCLE, q, muscle_params(directions, params_nh, params_a))
+
+
+def linear_tetmesh_neohookean_muscle_dq(V, E, q, dphidX, volume, directions, params_nh, params_a):
+    return _mesh_for(V, E, dphidX, volume).dq(NEOHOOKEAN_MUSCLE, q, muscle_params(directions, params_nh, params_a))
+
+
+def linear_tetmesh_neohookean_muscle_dq2(V, E, q, dphidX, volume, directions, params_nh, params_a, callback=None):
+    return _mesh_for(V, E, dphidX, volume).dq2(NEOHOOKEAN_MUSCLE, q, muscle_params(directions, params_nh, params_a),
+                                               callback)

@@ -240,6 +240,9 @@ def main():
                     help="only the Newton-step comparison (for the kernel profile of the solvers)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference point for the HBM byte counters)")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="three calls each of dq2 + PSD, dq2, dq and q (FP64), then the FP32 dq2 + PSD and dq2, "
+                         "and a 1 GiB device copy, for rocprofv3 (no timing output)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU sharding is a design, not code); "
@@ -275,6 +278,27 @@ def main():
 
     def step():
         mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, callback=psd, stream=stream)
+
+    if args.profile_kernels:
+        par32 = params.to(torch.float32)
+        H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
+        calls = [step, lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream),
+                 lambda: mesh.dq_device(tm.NEOHOOKEAN, q, params, g, stream=stream),
+                 lambda: mesh.q_device(tm.NEOHOOKEAN, q, params, e, stream=stream),
+                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream),
+                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream)]
+        for fn in calls:
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profiled": ["dq2_psd", "dq2", "dq", "q", "dq2_f32_psd", "dq2_f32", "copy_1GiB"],
+                          "calls_each": 3, "tets": nT, "nnz": nnz}), flush=True)
+        return
 
     for _ in range(args.warmup):
         step()

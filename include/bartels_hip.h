@@ -12,6 +12,8 @@
  *             BH_NEOHOOKEAN, BH_NEOHOOKEAN_LEGACY : [C, D]       = [mu/2, lambda/2]
  *             BH_STVK, BH_COROTATIONAL            : [lambda, mu]
  *             BH_ARAP                             : [k, unused]
+ *           nT x 6 for BH_NEOHOOKEAN_MUSCLE: [C, D, a, d0, d1, d2] (the MATLAB gateway's params_nh,
+ *           params_a and directions side by side)
  * Sparse outputs use the reference's pattern: full 3x3 blocks for every pair of vertices that
  * share a tet, explicit zeros kept, sorted int32 indices.  The pattern is symmetric, so the CSR
  * arrays returned here equal the CSC arrays of Eigen::SparseMatrix<double> (outerIndexPtr,
@@ -48,6 +50,11 @@ typedef enum {
   BH_COROTATIONAL = 2,      /* psi = mu sum (s-1)^2 + lambda/2 (sum s - 3)^2  src/psi_corotational_S.cpp:5-9 */
   BH_ARAP = 3,              /* psi = k sum (s-1)^2                         src/psi_arap_S.cpp:5-11 */
   BH_NEOHOOKEAN_LEGACY = 4, /* psi = C(J^{-2/3} tr F^T F - 3) + D (J-1)^2  src/psi_neohookean_F.cpp:8-14 (commented form) */
+  BH_NEOHOOKEAN_MUSCLE = 5, /* Neo-Hookean plus a muscle fibre, psi + a/2 |F d|^2 per tet: the generic_elastic
+                               functor of matlab/linear_tetmesh_neohookean_muscle_{q,dq,dq2}.cpp (:53-77 for dq2)
+                               with src/psi_muscle_fiber_F.cpp:5-10, src/dpsi_muscle_fiber_dF.cpp,
+                               src/d2psi_muscle_fiber_dF2.cpp; params nT x 6 [C, D, a, d0, d1, d2]; not
+                               available as a per-tet model id */
   BH_MODEL_PER_TET = 255    /* per-tet model ids passed separately (generic_elastic) */
 } bh_model;
 

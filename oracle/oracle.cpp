@@ -34,7 +34,7 @@
 
 namespace {
 
-enum Model { NEOHOOKEAN = 0, STVK = 1, COROTATIONAL = 2, ARAP = 3, NEOHOOKEAN_LEGACY = 4 };
+enum Model { NEOHOOKEAN = 0, STVK = 1, COROTATIONAL = 2, ARAP = 3, NEOHOOKEAN_LEGACY = 4, NEOHOOKEAN_MUSCLE = 5 };
 
 inline double det3(const double* F) {
   return F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
@@ -302,8 +302,37 @@ void dsvd(double dU[3][3][9], double dS[3][3][3], double dV[3][3][9], const doub
 
 // Reference: src/psi_neohookean_F.cpp:16-31 (active), src/psi_stvk_F.cpp:5-13,
 // src/psi_stretch_F.cpp:7-22.
+// muscle fibre of the reference's MATLAB neohookean_muscle gateways (matlab/linear_tetmesh_neohookean_muscle_q.cpp,
+// _dq.cpp, _dq2.cpp:53-77): Neo-Hookean with params (p[0], p[1]) plus a fibre with stiffness p[2] along the
+// direction (p[3], p[4], p[5]): psi_muscle_fiber_F = 0.5 a d^T F^T F d (src/psi_muscle_fiber_F.cpp:5-10),
+// dpsi = a D^T D vec F (src/dpsi_muscle_fiber_dF.cpp), d2psi = a D^T D (src/d2psi_muscle_fiber_dF2.cpp), with
+// D = flatten_multiply_right(d) the 3x9 matrix of vec(F) -> F d (include/flatten_multiply.h:44-88).
+void fibre_D(const double* d, double* D /*3x9 col-major*/) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 9; ++c) D[r + 3 * c] = (c % 3 == r) ? d[c / 3] : 0.0;
+}
+double psi_fibre(const double* F, const double* p) {
+  double D[27], Fd[3] = {0, 0, 0};
+  fibre_D(p + 3, D);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 9; ++c) Fd[r] += D[r + 3 * c] * F[c];
+  return 0.5 * p[2] * (Fd[0] * Fd[0] + Fd[1] * Fd[1] + Fd[2] * Fd[2]);
+}
+void d2psi_fibre(const double* p, double* H /*9x9 col-major*/) {
+  double D[27];
+  fibre_D(p + 3, D);
+  for (int a = 0; a < 9; ++a)
+    for (int b = 0; b < 9; ++b) {
+      double s = 0;
+      for (int r = 0; r < 3; ++r) s += D[r + 3 * a] * D[r + 3 * b];
+      H[a + 9 * b] = p[2] * s;
+    }
+}
+
 double psi_F(int model, const double* F, const double* p) {
   switch (model) {
+    case NEOHOOKEAN_MUSCLE:
+      return psi_F(NEOHOOKEAN, F, p) + psi_fibre(F, p);
     case NEOHOOKEAN: {
       double J = det3(F), I1 = 0;
       for (int k = 0; k < 9; ++k) I1 += F[k] * F[k];
@@ -339,6 +368,14 @@ double psi_F(int model, const double* F, const double* p) {
 // src/dpsi_stretch_dF.cpp:7-47 (inversion flip of negative singular values when det <= -1e-10).
 void dpsi_F(int model, const double* F, const double* p, double* P) {
   switch (model) {
+    case NEOHOOKEAN_MUSCLE: {
+      dpsi_F(NEOHOOKEAN, F, p, P);
+      double H[81];
+      d2psi_fibre(p, H);  // the fibre energy is quadratic in vec F: its gradient is (a D^T D) vec F
+      for (int a = 0; a < 9; ++a)
+        for (int b = 0; b < 9; ++b) P[a] += H[a + 9 * b] * F[b];
+      return;
+    }
     case NEOHOOKEAN: {
       double c[9];
       cofactor(F, c);
@@ -403,6 +440,13 @@ void dpsi_F(int model, const double* F, const double* p, double* P) {
 // (src/d2psi_stretch_dF2.cpp:36-45), i.e. the Hessian evaluated at F^T.
 void d2psi_F(int model, const double* F, const double* p, int ref_layout, double* H) {
   switch (model) {
+    case NEOHOOKEAN_MUSCLE: {
+      d2psi_F(NEOHOOKEAN, F, p, ref_layout, H);
+      double Hf[81];
+      d2psi_fibre(p, Hf);
+      for (int k = 0; k < 81; ++k) H[k] += Hf[k];
+      return;
+    }
     case NEOHOOKEAN: {
       // d2psi = 2C I + (2C/J^2 + 2D) g g^T + (2D(J-1) - 2C/J) d2J,  g = vec(cof F)
       double c[9];
@@ -527,11 +571,14 @@ struct Tet {
   double qe[12];
   double F[9];
   double vol;
-  double p[2];
+  double p[6];
 };
 
+// parameter columns: 6 for the muscle model with one model for all tets, else 2
+inline int cols_of(int model, const unsigned char* ids) { return (!ids && model == NEOHOOKEAN_MUSCLE) ? 6 : 2; }
+
 void load_tet(Tet& t, int e, int nT, const int* E, const double* q, const double* dphidX, const double* vol,
-              const double* params) {
+              const double* params, int cols = 2) {
   for (int k = 0; k < 12; ++k) t.phi[k] = dphidX[e + (int64_t)k * nT];
   for (int a = 0; a < 4; ++a) {
     int v = E[e + (int64_t)a * nT];
@@ -544,8 +591,7 @@ void load_tet(Tet& t, int e, int nT, const int* E, const double* q, const double
     t.F[r] = s;
   }
   t.vol = vol[e];
-  t.p[0] = params[e];
-  t.p[1] = params[e + (int64_t)nT];
+  for (int c = 0; c < 6; ++c) t.p[c] = (c < cols) ? params[e + (int64_t)c * nT] : 0.0;
 }
 
 // K = B^T d2psi B * vol  (12x12 col-major); B materialised once per element, as the reference's
@@ -861,7 +907,7 @@ void or_element_hessian(int model, int nT, int e, const int* E, const double* q,
                         const double* vol, const double* params, int ref_layout, int psd, double psd_tol,
                         double* K) {
   Tet t;
-  load_tet(t, e, nT, E, q, dphidX, vol, params);
+  load_tet(t, e, nT, E, q, dphidX, vol, params, cols_of(model, nullptr));
   element_hessian(t, model, ref_layout, K);
   if (psd) psd_fix(K, 12, psd_tol);
 }
@@ -874,7 +920,7 @@ double or_tetmesh_q(int model, int nV, int nT, const int* E, const double* q, co
   double energy = 0.0;
   for (int e = 0; e < nT; ++e) {
     Tet t;
-    load_tet(t, e, nT, E, q, dphidX, vol, params);
+    load_tet(t, e, nT, E, q, dphidX, vol, params, cols_of(model, model_ids));
     double v = psi_F(model_of(model, model_ids, e), t.F, t.p) * t.vol;
     energy = (e == 0) ? v : energy + v;
   }
@@ -888,7 +934,7 @@ void or_tetmesh_dq(int model, int nV, int nT, const int* E, const double* q, con
   std::memset(g, 0, sizeof(double) * 3 * (size_t)nV);
   for (int e = 0; e < nT; ++e) {
     Tet t;
-    load_tet(t, e, nT, E, q, dphidX, vol, params);
+    load_tet(t, e, nT, E, q, dphidX, vol, params, cols_of(model, model_ids));
     double ge[12];
     element_gradient(t, model_of(model, model_ids, e), ge);
     for (int a = 0; a < 4; ++a)
@@ -936,7 +982,7 @@ or_csc* or_tetmesh_dq2(int model, int nV, int nT, const int* E, const double* q,
   double K[144];
   for (int e = 0; e < nT; ++e) {
     Tet t;
-    load_tet(t, e, nT, E, q, dphidX, vol, params);
+    load_tet(t, e, nT, E, q, dphidX, vol, params, cols_of(model, model_ids));
     element_hessian(t, model_of(model, model_ids, e), ref_layout, K);
     if (psd) psd_fix(K, 12, psd_tol);
     push_element(trips, E, nT, e, K);
@@ -961,7 +1007,7 @@ or_csc* or_tetmesh_dq2_timed(int model, int nV, int nT, const int* E, const doub
   for (int e = 0; e < nT; ++e) {
     Tet t;
     double K[144];
-    load_tet(t, e, nT, E, q, dphidX, vol, params);
+    load_tet(t, e, nT, E, q, dphidX, vol, params, cols_of(model, nullptr));
     element_hessian(t, model, 1, K);
     if (psd) psd_fix_ql<12>(K, psd_tol);
     Triplet* out = trips.data() + (size_t)e * 144;

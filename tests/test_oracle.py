@@ -128,16 +128,53 @@ def test_rest_state_and_frame_invariance(oracle, model):
 
 
 # ---- mesh level: finite differences and invariants -------------------------------------------
+def muscle_columns(nT, lam, seed):
+    """[a, d0, d1, d2] of the muscle fibre per tet: stiffness of the order of lambda, random unit directions"""
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((nT, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.hstack([(np.asarray(lam) * (0.5 + rng.random(nT)))[:, None], d])
+
+
 def small_problem(model, seed=0, grid=(5, 3, 3), amp=0.08):
     V, E = meshgen.beam_grid(*grid, h=0.1)
     rng = np.random.default_rng(seed)
     q = (V + amp * 0.1 * rng.standard_normal(V.shape)).ravel()
     lam, mu = meshgen.emu_to_lame(1e5 * (1 + rng.random(E.shape[0])), 0.3 + 0.1 * rng.random(E.shape[0]))
+    if model == 5:  # Neo-Hookean plus a muscle fibre: nT x 6 [C, D, a, d]
+        nh = meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu)
+        return V, E, q, np.asfortranarray(np.hstack([nh, muscle_columns(E.shape[0], lam, seed + 1)]))
     ids = np.full(E.shape[0], model, dtype=np.uint8)
     return V, E, q, meshgen.model_params(ids, lam, mu)
 
 
-@pytest.mark.parametrize("model", MODELS)
+def test_muscle_known_answers(oracle):
+    """the fibre term of the muscle model (src/psi_muscle_fiber_F.cpp:5-10: 0.5 a d^T F^T F d) at
+    F = diag(2, 4, 3) with d = e_y, a = 3: psi = 24, P = a (F d) d^T = 12 at (1, 1), d2psi = a at the
+    three entries (i + 3, i + 3); added to the Neo-Hookean part exactly"""
+    F = np.diag([2.0, 4.0, 3.0])
+    p = np.array([0.0, 0.0, 3.0, 0.0, 1.0, 0.0])
+    assert oracle.psi(5, F, p) == 24.0
+    P = oracle.dpsi(5, F, p)
+    Pref = np.zeros((3, 3))
+    Pref[1, 1] = 12.0
+    assert np.array_equal(P, Pref)
+    H = oracle.d2psi(5, F, p, 1)
+    Href = np.zeros((9, 9))
+    for i in range(3):
+        Href[i + 3, i + 3] = 3.0
+    assert np.array_equal(H, Href)
+    pn = np.array([3e5, 7e5, 3.0, 0.0, 1.0, 0.0])
+    assert abs(oracle.psi(5, F, pn) - oracle.psi(0, F, pn[:2]) - 24.0) < 1e-9
+    assert np.abs(oracle.dpsi(5, F, pn) - oracle.dpsi(0, F, pn[:2]) - Pref).max() < 1e-9
+    assert np.abs(oracle.d2psi(5, F, pn, 1) - oracle.d2psi(0, F, pn[:2], 1) - Href).max() < 1e-9
+    # a general direction: psi = a/2 |F d|^2 for d not along an axis
+    d = np.array([0.3, -0.5, 0.8])
+    pd = np.array([0.0, 0.0, 1.7, *d])
+    assert abs(oracle.psi(5, F, pd) - 0.5 * 1.7 * np.sum((F @ d) ** 2)) < 1e-13
+
+
+@pytest.mark.parametrize("model", MODELS + [5])
 def test_mesh_fd(oracle, model):
     V, E, q, params = small_problem(model)
     dX, vol = oracle.dphidX_volume(V, E)
