@@ -93,6 +93,7 @@ struct bh_mesh {
   double* psdDM = nullptr;  // nT x 45: the tets' records M' (packed reduced 9x9)
   float* psdDM32 = nullptr;
   double* tetQ = nullptr;   // nT x 12: Q of Phi = Q R, tet-contiguous (made on first use)
+  double* tetForces = nullptr;  // nT x 12: the gradient's per-tet corner forces, corner-major (first use)
   float* tetQ32 = nullptr;
   // solver workspace (lazily allocated)
   double* ws = nullptr;
@@ -349,42 +350,61 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restri
 }
 
 // ---- gradient (vertex gather, ascending element order per vertex; FP64 accumulation) ----------
+// Two passes, so that each tet's inputs are read once (a vertex gather of the element math reads every
+// tet once per corner).  k_tet_forces: one lane per tet forms P and writes the tet's four corner forces
+// vol P phi_a, rounded to FP64 exactly as the sum takes them, corner-major (component c of corner a of
+// tet e at f[(3a + c) nT + e]: a wave's stores are contiguous).  k_gather_forces: one lane per vertex
+// adds the forces of its incident tets in ascending element order (the reference's += order,
+// src/assemble.cpp:99-132); the lanes of a wave read the same corner of consecutive tets.
 template <int MODEL, typename T>
 __global__ void __launch_bounds__(256)
-    k_gradient(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-               const uint8_t* __restrict__ mids, double* __restrict__ g) {
+    k_tet_forces(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                 const uint8_t* __restrict__ mids, double* __restrict__ f) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  T phi[4][3], F[9], G[9], P[9];
+  load_phi(m, e, phi);
+  load_FG(m, q, load_E(m, e), phi, F, G);
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  const T* Gp = (sizeof(T) == 4) ? G : nullptr;
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    T fib[4];
+    load_fibre(params, m.nT, e, fib);
+    piola<M_NH>(F, p0, p1, P, Gp);
+    fibre_piola_add(F, fib, P);
+  } else switch (model_at<MODEL>(mids, e)) {
+    case M_NH: piola<M_NH>(F, p0, p1, P, Gp); break;
+    case M_STVK: piola<M_STVK>(F, p0, p1, P, Gp); break;
+    case M_COROT: piola<M_COROT>(F, p0, p1, P); break;
+    case M_ARAP: piola<M_ARAP>(F, p0, p1, P); break;
+    default: piola<M_NH_LEGACY>(F, p0, p1, P); break;
+  }
+  const T vl = load_vol<T>(m, e);
+  const int64_t nT = m.nT;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T fa[3];
+    mv3(P, phi[a], fa);  // (B^T vec P)_a = P phi_a
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[(3 * a + c) * nT + e] = (double)(vl * fa[c]);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_gather_forces(MeshView m, const double* __restrict__ f, double* __restrict__ g) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= m.nV) return;
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   const IncList L = inc_list(m, v);
+  const int64_t nT = m.nT;
   for (int k = 0; k < L.len; ++k) {
     const int code = m.incE[L.first + 64ll * k];
     if (code < 0) break;
     const int e = code >> 2, a = code & 3;
-    T phi[4][3], F[9], G[9], P[9];
-    load_phi(m, e, phi);
-    load_FG(m, q, load_E(m, e), phi, F, G);
-    const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
-    const T* Gp = (sizeof(T) == 4) ? G : nullptr;
-    if constexpr (MODEL == M_NH_MUSCLE) {
-      T fib[4];
-      load_fibre(params, m.nT, e, fib);
-      piola<M_NH>(F, p0, p1, P, Gp);
-      fibre_piola_add(F, fib, P);
-    } else switch (model_at<MODEL>(mids, e)) {
-      case M_NH: piola<M_NH>(F, p0, p1, P, Gp); break;
-      case M_STVK: piola<M_STVK>(F, p0, p1, P, Gp); break;
-      case M_COROT: piola<M_COROT>(F, p0, p1, P); break;
-      case M_ARAP: piola<M_ARAP>(F, p0, p1, P); break;
-      default: piola<M_NH_LEGACY>(F, p0, p1, P); break;
-    }
-    const T vl = load_vol<T>(m, e);
-    T pa[3], f[3];
-    select_row(phi, a, pa);
-    mv3(P, pa, f);  // (B^T vec P)_a = P phi_a
-    g0 += (double)(vl * f[0]);
-    g1 += (double)(vl * f[1]);
-    g2 += (double)(vl * f[2]);
+    const double* fa = f + 3 * a * nT + e;
+    g0 += fa[0];
+    g1 += fa[nT];
+    g2 += fa[2 * nT];
   }
   g[3ll * v] = g0;
   g[3ll * v + 1] = g1;
@@ -1242,8 +1262,13 @@ template <int MODEL, typename T>
 int launch_gradient(bh_mesh* m, const double* q, const T* params, const uint8_t* mids, double* g, hipStream_t s) {
   const int grid = (m->nV + 255) / 256;
   if (grid == 0) return BH_OK;
+  if (!m->tetForces) BH_CHECK(hipMalloc(&m->tetForces, sizeof(double) * 12 * std::max<size_t>(1, m->nT)));
   ProfScope ps(m, BH_KT_GRADIENT, s);
-  hipLaunchKernelGGL((k_gradient<MODEL, T>), dim3(grid), dim3(256), 0, s, view(m), q, params, mids, g);
+  if (m->nT) {
+    hipLaunchKernelGGL((k_tet_forces<MODEL, T>), dim3((m->nT + 255) / 256), dim3(256), 0, s, view(m), q, params, mids,
+                       m->tetForces);
+  }
+  hipLaunchKernelGGL(k_gather_forces, dim3(grid), dim3(256), 0, s, view(m), (const double*)m->tetForces, g);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -1425,7 +1450,7 @@ void free_mesh(bh_mesh* m) {
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->vdeg, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
                   m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32, m->tetQ, m->tetQ32,
-                  m->cws};
+                  m->cws, m->tetForces};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
