@@ -15,7 +15,9 @@
 //   d_v = h_v W_dv + b_dv (6),  O_k = e_k W_do + b_do (9)       (decoders start at zero)
 // Factor (normalised): Lhat_vv = C_v T_v with T_v lower triangular, T_ii = exp(d), strict lower = d;
 //   Lhat_vn = O_k.  At zero decoders Lhat Lhat^T = blockdiag(Hn_vv): block Jacobi.  The factor of H
-//   is L = S Lhat (L_vv = s_v Lhat_vv, L_vn = s_v Lhat_vn).
+//   is L = S Lhat (L_vv = s_v Lhat_vv, L_vn = s_v Lhat_vn).  The tied factor form (form 2) adds
+//   Hn_vn Lhat_nn^{-T} to O_k: at zero decoders L L^T = (D + L_H) D^-1 (D + L_H)^T, block symmetric
+//   Gauss-Seidel as a factor (L_H the strictly lower blocks of H in the colour order, D its diagonal).
 // Loss of a system H x = b:  with xh = S x, bh = S^{-1} b,  |Lhat Lhat^T xh - bh|^2 / |bh|^2.
 // Training uses probe systems xh = w, bh = Hn w (w the normalised right-hand side of the plain
 // solver's Newton system and standard normal vectors): a Hutchinson estimate of the relative
@@ -41,7 +43,8 @@ struct bh_mgn {
   int64_t pvSize = 0;
   double* probe = nullptr;  // training probes: x_j, b_j
   size_t probeBytes = 0;
-  int form = 0;             // 0: general factor; 1: Eisenstat (symmetric Gauss-Seidel) form, O tied to H
+  int form = 0;             // 0: general factor; 1: Eisenstat (symmetric Gauss-Seidel) form, O tied to H;
+                            // 2: tied factor, O = Hn_vn Lhat_nn^{-T} + the pair decoder's output
   int probeSolution = 0;    // 1: the vector handed to a training step is a solution x of H x = b (probe 0)
   int energyLoss = 0;       // 1 (Eisenstat form): the energy-ratio loss instead of the residual loss
 };
@@ -680,10 +683,12 @@ __device__ __forceinline__ void ltri_inv(const float* L, float* I) {  // row-maj
   I[6] = i20; I[7] = i21; I[8] = i22;
 }
 
+// the tied blocks O = B Xi^T (B = Hn_vn, Xi = Lhat_nn^{-1}); add = 1 adds them to the pair decoder's
+// output already in O (the tied factor form), add = 0 replaces it (the Eisenstat form)
 __global__ void __launch_bounds__(256)
     k_mgn_tied_O(int64_t nP, const int32_t* __restrict__ pv, const int32_t* __restrict__ lcol,
                  const float* __restrict__ fp, const float* __restrict__ C, const float* __restrict__ d,
-                 float* __restrict__ O) {
+                 float* __restrict__ O, int add) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= nP) return;
   const int n = lcol[k];
@@ -695,29 +700,39 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int l = 0; l < 3; ++l)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) O[9 * k + 3 * l + c] = B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
+    for (int c = 0; c < 3; ++c) {
+      const float t = B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
+      O[9 * k + 3 * l + c] = add ? O[9 * k + 3 * l + c] + t : t;
+    }
   (void)pv;
 }
 
 // the tied blocks' share of the gradient of Lhat_nn, gathered over the pairs in which n is the lower
 // vertex (its upper list): with O = B X^{-T}, dL/dX = -X^{-T} G^T O for G = dL/dO; added (through
-// X = C T(d)) to gd_n.  gO is cleared afterwards: the pair decoder does not act in this form.
+// X = C T(d)) to gd_n.  In the Eisenstat form gO is cleared afterwards (the pair decoder does not act
+// there); in the tied factor form it also reaches the pair decoder, whose output is added to O.
 __global__ void __launch_bounds__(256)
     k_mgn_tied_bwd(int nV, const int32_t* __restrict__ uptr, const int32_t* __restrict__ ulink,
-                   const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ O,
+                   const float* __restrict__ C, const float* __restrict__ d, const float* __restrict__ fp,
                    const float* __restrict__ gO, float* __restrict__ gd) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nV) return;
   float X[9], Xi[9];
   mgn_lvv(C + 6ll * n, d + 6ll * n, X);
   ltri_inv(X, Xi);
-  float A[9];  // sum over pairs of G^T O
+  float A[9];  // sum over pairs of G^T O (O the tied blocks B Xi^T, formed again from the pair features)
 #pragma unroll
   for (int e = 0; e < 9; ++e) A[e] = 0.f;
   for (int t = uptr[n]; t < uptr[n + 1]; ++t) {
     const int64_t k = ulink[t];
     const float* G = gO + 9 * k;
-    const float* Ok = O + 9 * k;
+    const float* B = fp + (int64_t)kPairIn * k;
+    float Ok[9];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        Ok[3 * l + c] = B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -1027,8 +1042,9 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
   const int nV = P->mesh->nV;
   const int64_t nP = P->nlower;
   if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
-  if (M->form == 1 && nP)
-    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O);
+  if ((M->form == 1 || M->form == 2) && nP)
+    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O,
+                       M->form == 2 ? 1 : 0);
   const MgnLayout L = mgn_layout();
   const int nb = (int)((nV + 255) / 256);
   double* part = reinterpret_cast<double*>(M->part);
@@ -1059,9 +1075,9 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
     }
   }
   BH_CHECK(hipGetLastError());
-  if (backward && M->form == 1 && !M->energyLoss) {
-    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.O, w.gO, w.gd);
-    if (nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
+  if (backward && (M->form == 1 || M->form == 2) && !M->energyLoss) {
+    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.fp, w.gO, w.gd);
+    if (nP && M->form == 1) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
     BH_CHECK(hipGetLastError());
   }
   if (backward) {
@@ -1205,6 +1221,9 @@ int mgn_factor(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hi
   mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
   if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
   const int nV = P->mesh->nV;
+  if (M->form == 2 && P->nlower)  // tied factor form: L_vn = s_v (Hn_vn Lhat_nn^{-T} + O_k)
+    hipLaunchKernelGGL(k_mgn_tied_O, g256(P->nlower), dim3(256), 0, s, P->nlower, M->pv, P->lcol, w.fp, w.C, w.d,
+                       w.O, 1);
   hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O, w.s,
                      P->lval, P->dinv);
   BH_CHECK(hipGetLastError());
@@ -1325,7 +1344,7 @@ int bh_mgn_destroy(bh_mgn* M) {
 }
 
 int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution, int energy_loss) {
-  if (!M || form < 0 || form > 1 || (energy_loss && form != 1)) return BH_BAD_ARG;
+  if (!M || form < 0 || form > 2 || (energy_loss && form != 1)) return BH_BAD_ARG;
   M->form = form;
   M->probeSolution = probe_is_solution ? 1 : 0;
   M->energyLoss = energy_loss ? 1 : 0;

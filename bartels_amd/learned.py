@@ -135,8 +135,8 @@ class Solver:
             self.P = ctypes.c_void_p()
 
 
-FORMS = {"factor": 0, "sgs": 1}
-PRECOND = {"factor": 4, "sgs": 6}  # BH_PRECOND_LEARNED, BH_PRECOND_LEARNED_SGS
+FORMS = {"factor": 0, "sgs": 1, "tied": 2}
+PRECOND = {"factor": 4, "sgs": 6, "tied": 4}  # BH_PRECOND_LEARNED, BH_PRECOND_LEARNED_SGS, BH_PRECOND_LEARNED
 
 
 class Network:

@@ -282,7 +282,10 @@ int bh_mgn_create(uint64_t seed, bh_mgn** mgn);
 int bh_mgn_destroy(bh_mgn* mgn);
 /* The preconditioner form the loss and training steps use: 0 the general block lower-triangular factor
  * (BH_PRECOND_LEARNED), 1 the Eisenstat form M = (Dt + L) Dt^-1 (Dt + L)^T whose off-diagonal blocks
- * are tied to H (BH_PRECOND_LEARNED_SGS; only the node decoder acts).  probe_is_solution = 1: the vector
+ * are tied to H (BH_PRECOND_LEARNED_SGS; only the node decoder acts), 2 the tied factor
+ * (BH_PRECOND_LEARNED): L_vv from the node decoder as in form 0, L_vn = H_vn L_nn^-T + s_v O_k with O_k
+ * the pair decoder's output, so that at zero decoders L L^T is block symmetric Gauss-Seidel and the
+ * coarse correction applies.  probe_is_solution = 1: the vector
  * bh_mgn_train_step_device receives is a solution x of the Newton system (probe 0 is x itself) instead of
  * its right-hand side.  No counterpart in the reference (the learned preconditioner is this build's
  * addition).  energy_loss = 1 (form 1 only): the loss is the mean over probes x of

@@ -116,7 +116,7 @@ def forward(P, g, xn, fp):
     return d, O
 
 
-def factor(C, d, O, g=None, fp=None):
+def factor(C, d, O, g=None, fp=None, add=False):
     """normalised factor blocks: Lvv (nV x 3 x 3) = C T(d), Lk (nP x 3 x 3) = O; or, in the Eisenstat
     (symmetric Gauss-Seidel) form (g and the pair features fp given), Lk = Hn_vn Lvv_n^{-T} with Hn_vn
     the first 9 pair features, so that L L^T = (Dt + L_H) Dt^{-1} (Dt + L_H)^T, Dt = Lvv Lvv^T"""
@@ -134,7 +134,9 @@ def factor(C, d, O, g=None, fp=None):
     pn = torch.as_tensor(g.lcol.astype(np.int64))
     Hn = fp[:, :9].reshape(-1, 3, 3)
     Xinv_t = torch.linalg.inv(Lvv[pn]).transpose(1, 2).contiguous()
-    return Lvv, torch.bmm(Hn, Xinv_t)
+    tied = torch.bmm(Hn, Xinv_t)
+    # add: the tied factor form, the pair decoder's output added to the tied blocks
+    return Lvv, (tied + O.reshape(-1, 3, 3)) if add else tied
 
 
 def apply_LLt(g, Lvv, Lk, u):
@@ -148,13 +150,16 @@ def apply_LLt(g, Lvv, Lk, u):
     return z
 
 
-def loss(P, g, feats, x, b, tied=False):
-    """|Lhat Lhat^T xh - bh|^2 / |bh|^2 with xh = s x, bh = b / s (tied: the Eisenstat form)"""
+def loss(P, g, feats, x, b, tied=False, tied_factor=False):
+    """|Lhat Lhat^T xh - bh|^2 / |bh|^2 with xh = s x, bh = b / s (tied: the Eisenstat form; tied_factor:
+    the tied factor form, Lk = Hn_vn Lvv_n^{-T} + O)"""
     s, C, xn, fp = (torch.as_tensor(a) for a in feats)
     d, O = forward(P, g, xn, fp)
-    Lvv, Lk = factor(C, d, O, g, fp) if tied else factor(C, d, O)
-    if tied:
+    if tied or tied_factor:
+        Lvv, Lk = factor(C, d, O, g, fp, add=tied_factor)
         O = Lk.reshape(-1, 9)
+    else:
+        Lvv, Lk = factor(C, d, O)
     xh = torch.as_tensor(x.reshape(-1, 3)) * s[:, None]
     bh = torch.as_tensor(b.reshape(-1, 3)) / s[:, None]
     z = apply_LLt(g, Lvv, Lk, xh)

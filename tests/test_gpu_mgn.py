@@ -91,11 +91,11 @@ def make_mgn(lib, seed=7):
     return M, n.value
 
 
-def _gradient_check(S, tied):
+def _gradient_check(S, tied, tied_factor=False):
     lib = S["lib"]
     M, n = make_mgn(lib)
     try:
-        assert lib.bh_mgn_set_form(M, 1 if tied else 0, 0, 0) == 0
+        assert lib.bh_mgn_set_form(M, 2 if tied_factor else (1 if tied else 0), 0, 0) == 0
         lay, ntot = R.layout()
         assert n == ntot
         rng = np.random.default_rng(1)
@@ -127,7 +127,7 @@ def _gradient_check(S, tied):
         O_gpu = d2h(bufs[3].value, 9 * g.nP, np.float32).reshape(-1, 9)
         # reference
         Pt = {k: v.clone().requires_grad_(True) for k, v in R.unflatten(torch.tensor(w, dtype=torch.float64)).items()}
-        L, d_ref, O_ref = R.loss(Pt, g, S["feats"], S["x"], S["b"], tied=tied)
+        L, d_ref, O_ref = R.loss(Pt, g, S["feats"], S["x"], S["b"], tied=tied, tied_factor=tied_factor)
         L.backward()
         g_ref = np.zeros(n)
         for k, (o, sh) in lay.items():
@@ -162,6 +162,12 @@ def test_mgn_tied_form_gradient_matches_reference(system):
     """Eisenstat (symmetric Gauss-Seidel) form: off-diagonal blocks Hn_vn Lhat_nn^{-T}, the gradient through
     them into the node decoder, against FP64 torch"""
     _gradient_check(system, tied=True)
+
+
+def test_mgn_tied_factor_gradient_matches_reference(system):
+    """tied factor form: off-diagonal blocks Hn_vn Lhat_nn^{-T} plus the pair decoder's output (block
+    symmetric Gauss-Seidel at zero decoders), the gradient into both decoders, against FP64 torch"""
+    _gradient_check(system, tied=False, tied_factor=True)
 
 
 def test_mgn_energy_loss_gradient_matches_reference(system):
