@@ -106,6 +106,7 @@ SIGNATURES = {
     "bh_mesh_create": (c_int, [c_int, c_int, _VP, _VP, _VP, P(c_void_p)]),
     "bh_mesh_create_device": (c_int, [c_int, c_int, _VP, _VP, _VP, _VP, P(c_void_p)]),
     "bh_mesh_destroy": (c_int, [_VP]),
+    "bh_mesh_trim": (c_int, [_VP]),
     "bh_mesh_nnz": (c_int, [_VP, P(c_int64)]),
     "bh_mesh_pattern": (c_int, [_VP, _VP, _VP]),
     "bh_mesh_block_pattern_device": (c_int, [_VP, P(c_void_p), P(c_void_p), P(c_int64)]),

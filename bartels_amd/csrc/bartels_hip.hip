@@ -1534,6 +1534,25 @@ int bh_mesh_destroy(bh_mesh* m) {
   return BH_OK;
 }
 
+int bh_mesh_trim(bh_mesh* m) {
+  if (!m) return BH_BAD_ARG;
+  int prev = 0;
+  BH_CHECK(hipGetDevice(&prev));
+  BH_CHECK(hipSetDevice(m->device));
+  BH_CHECK(hipDeviceSynchronize());
+  void** bufs[] = {(void**)&m->psdDM, (void**)&m->psdDM32, (void**)&m->psdFlag, (void**)&m->psdCount,
+                   (void**)&m->tetQ,  (void**)&m->tetQ32,  (void**)&m->tetForces, (void**)&m->stage_vals,
+                   (void**)&m->ws,    (void**)&m->cws,     (void**)&m->dX32,    (void**)&m->vol32};
+  for (void** b : bufs) {
+    if (*b) BH_CHECK(hipFree(*b));
+    *b = nullptr;
+  }
+  m->wsBytes = 0;
+  m->cwsBytes = 0;
+  BH_CHECK(hipSetDevice(prev));
+  return BH_OK;
+}
+
 int bh_mesh_nnz(const bh_mesh* m, int64_t* nnz) {
   if (!m || !nnz) return BH_BAD_ARG;
   *nnz = m->nnz;

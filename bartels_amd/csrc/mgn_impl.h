@@ -29,6 +29,7 @@ struct bh_mgn {
   int64_t nparam = 0;
   float* w = nullptr;   // parameters
   float* g = nullptr;   // gradient accumulator
+  float* gsave = nullptr;  // the accumulator before the latest system's backward (restored if that fails)
   float* m1 = nullptr;  // Adam moments
   float* m2 = nullptr;
   int64_t step = 0;
@@ -47,6 +48,8 @@ struct bh_mgn {
                             // 2: tied factor, O = Hn_vn Lhat_nn^{-T} + the pair decoder's output
   int probeSolution = 0;    // 1: the vector handed to a training step is a solution x of H x = b (probe 0)
   int energyLoss = 0;       // 1 (Eisenstat form): the energy-ratio loss instead of the residual loss
+  int teacher = 0;          // 1 (factor forms): the IC(0) teacher loss (training steps factor the Hessian)
+  int kaporin = 0;          // 1 (factor forms): the Kaporin condition number of L L^T against H
 };
 
 namespace {
@@ -683,6 +686,117 @@ __device__ __forceinline__ void ltri_inv(const float* L, float* I) {  // row-maj
   I[6] = i20; I[7] = i21; I[8] = i22;
 }
 
+// The IC(0) teacher loss of the factor forms: |Lhat - T|_F^2 / |T|_F^2 over the blocks of the factor's
+// pattern, T the block IC(0) factor of the same Hessian (bh_precond's lval / dinv after
+// precond_factor_bic0) in the normalised space, T_vv = inv(dinv_v) / s_v, T_k = lval_k / s_v; entries in
+// the row or column of a fixed component are left out (both factors are the identity there).
+__device__ __forceinline__ void teacher_blocks(int v, const float* __restrict__ dinv, float sv, float T[9]) {
+  const float* a = dinv + 6ll * v;
+  const float I[9] = {a[0], 0.f, 0.f, a[1], a[2], 0.f, a[3], a[4], a[5]};
+  ltri_inv(I, T);
+  const float is = 1.f / sv;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) T[i] *= is;
+}
+
+__global__ void __launch_bounds__(256)
+    k_mgn_teacher_fwd(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                      const uint8_t* __restrict__ fixed, const float* __restrict__ C, const float* __restrict__ d,
+                      const float* __restrict__ O, const float* __restrict__ s, const float* __restrict__ lval,
+                      const float* __restrict__ dinv, double* __restrict__ part) {
+  __shared__ double sh[8];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double num = 0.0, den = 0.0;
+  if (v < nV) {
+    const float sv = s[v];
+    bool fv[3] = {false, false, false};
+    if (fixed)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+    float L[9], T[9];
+    mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+    teacher_blocks(v, dinv, sv, T);
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c <= l; ++c)
+        if (!fv[l] && !fv[c]) {
+          const double e = (double)L[3 * l + c] - T[3 * l + c];
+          num += e * e;
+          den += (double)T[3 * l + c] * T[3 * l + c];
+        }
+    const float is = 1.f / sv;
+    for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+      const int n = lcol[k];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        if (fixed && (fv[i / 3] || fixed[3ll * n + i % 3])) continue;
+        const double t = (double)lval[9ll * k + i] * is;
+        const double e = (double)O[9ll * k + i] - t;
+        num += e * e;
+        den += t * t;
+      }
+    }
+  }
+  num = block_reduce_sum(num, sh);
+  __syncthreads();
+  den = block_reduce_sum(den, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = num;
+    part[2 * blockIdx.x + 1] = den;
+  }
+}
+
+// gradient of the teacher loss: gO_k = 2 (O_k - T_k) / |T|^2 (masked), gd_v through Lhat_vv = C_v T(d_v)
+__global__ void __launch_bounds__(256)
+    k_mgn_teacher_bwd(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                      const uint8_t* __restrict__ fixed, const float* __restrict__ C, const float* __restrict__ d,
+                      const float* __restrict__ O, const float* __restrict__ s, const float* __restrict__ lval,
+                      const float* __restrict__ dinv, const double* __restrict__ sc, float* __restrict__ gO,
+                      float* __restrict__ gd) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const float scale = sc[1] > 0.0 ? (float)(2.0 / sc[1]) : 0.f;
+  const float sv = s[v];
+  bool fv[3] = {false, false, false};
+  if (fixed)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+  const float is = 1.f / sv;
+  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+    const int n = lcol[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const bool mk = fixed && (fv[i / 3] || fixed[3ll * n + i % 3]);
+      gO[9ll * k + i] = mk ? 0.f : scale * (O[9ll * k + i] - lval[9ll * k + i] * is);
+    }
+  }
+  float L[9], T[9], G[9];
+  mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+  teacher_blocks(v, dinv, sv, T);
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) G[3 * l + c] = (c <= l && !fv[l] && !fv[c]) ? scale * (L[3 * l + c] - T[3 * l + c]) : 0.f;
+  // Lhat = C T: gT = C^T G (lower part kept), then through T(d)
+  const float* Cv = C + 6ll * v;
+  const float Cm[9] = {Cv[0], 0.f, 0.f, Cv[1], Cv[2], 0.f, Cv[3], Cv[4], Cv[5]};
+  float gT[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * G[j] + Cm[3 + i] * G[3 + j] + Cm[6 + i] * G[6 + j];
+  const float* dv = d + 6ll * v;
+  float* g = gd + 6ll * v;
+  g[0] = gT[0] * dtexp(dv[0]);
+  g[1] = gT[3];
+  g[2] = gT[4] * dtexp(dv[2]);
+  g[3] = gT[6];
+  g[4] = gT[7];
+  g[5] = gT[8] * dtexp(dv[5]);
+}
+
+
 // the tied blocks O = B Xi^T (B = Hn_vn, Xi = Lhat_nn^{-1}); add = 1 adds them to the pair decoder's
 // output already in O (the tied factor form), add = 0 replaces it (the Eisenstat form)
 __global__ void __launch_bounds__(256)
@@ -862,6 +976,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// sum of squares of a float vector in FP64, one block (fixed order): the finiteness check of a gradient
+__global__ void __launch_bounds__(256) k_sumsq_f(int64_t n, const float* __restrict__ a, double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) acc += (double)a[i] * (double)a[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 __global__ void __launch_bounds__(256) k_zero_f(int64_t n, float* __restrict__ a) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.f;
@@ -1026,61 +1154,13 @@ int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fi
   return BH_OK;
 }
 
-// Mean loss over ns systems (b_j, x_j) and, with backward, its gradient accumulated into M->g:
-// one forward, the factor-side gradients of every system summed, one backward through the network.
-constexpr int kMaxSystems = 16;
-
-template <typename V>
-int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, int ns, const double* const* bs,
-             const double* const* xs, int backward, double* loss, hipStream_t s) {
-  if (ns < 1 || ns > kMaxSystems) return BH_BAD_ARG;
-  int st = mgn_prepare(M, P);
-  if (st) return st;
-  MgnWs w;
-  mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
-  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+// The network's backward pass from the factor-side gradients w.gd (node decoder outputs) and w.gO (pair
+// decoder outputs): the decoders, the message-passing layers in reverse, the encoders; the weight
+// gradients are accumulated into M->g.
+int mgn_network_backward(bh_mgn* M, const bh_precond* P, MgnWs& w, hipStream_t s) {
   const int nV = P->mesh->nV;
   const int64_t nP = P->nlower;
-  if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
-  if ((M->form == 1 || M->form == 2) && nP)
-    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O,
-                       M->form == 2 ? 1 : 0);
   const MgnLayout L = mgn_layout();
-  const int nb = (int)((nV + 255) / 256);
-  double* part = reinterpret_cast<double*>(M->part);
-  if (M->energyLoss) {
-    for (int j = 0; j < ns; ++j) {
-      hipLaunchKernelGGL(k_mgn_energy_fwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.fp, w.C, w.d,
-                         w.s, xs[j], bs[j], w.y, part);
-      hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
-      if (backward)
-        hipLaunchKernelGGL(k_mgn_energy_bwd, g256(nV), dim3(256), 0, s, nV, w.C, w.d, w.s, xs[j], w.y,
-                           M->dscal + 2 * j, ns, j > 0 ? 1 : 0, w.gd);
-    }
-    if (backward && nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
-  }
-  for (int j = 0; j < ns && !M->energyLoss; ++j) {
-    // y = Lhat^T xh ; r = Lhat y - bh
-    hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, xs[j],
-                       1, (const float*)nullptr, w.y);
-    hipLaunchKernelGGL(k_mgn_resid, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.O, w.s, w.y, bs[j],
-                       w.r, part);
-    hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
-    if (backward) {
-      hipLaunchKernelGGL(k_mgn_gz, g256(3ll * nV), dim3(256), 0, s, 3ll * nV, w.r, M->dscal + 2 * j, ns, w.gz);
-      hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s,
-                         (const double*)nullptr, 0, w.gz, w.gy);
-      hipLaunchKernelGGL(k_mgn_gfactor, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.s, xs[j], w.y,
-                         w.gz, w.gy, j > 0 ? 1 : 0, w.gO, w.gd);
-    }
-  }
-  BH_CHECK(hipGetLastError());
-  if (backward && (M->form == 1 || M->form == 2) && !M->energyLoss) {
-    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.fp, w.gO, w.gd);
-    if (nP && M->form == 1) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
-    BH_CHECK(hipGetLastError());
-  }
-  if (backward) {
     float* G = M->g;
     float* wp = M->part;  // weight-gradient partials (after the loss partials were consumed)
     auto outer = [&](auto xwTag, auto gwTag, int64_t rows, Seg3 S, const float* g, float* gW, float* gb) {
@@ -1139,7 +1219,97 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
       outer(I10{}, I32{}, nP, plain(w.fp, kPairIn), w.ghid, G + L.penc.W1, G + L.penc.b1);
     }
     BH_CHECK(hipGetLastError());
+    return BH_OK;
+}
+
+// Mean loss over ns systems (b_j, x_j) and, with backward, its gradient accumulated into M->g:
+// one forward, the factor-side gradients of every system summed, one backward through the network.
+constexpr int kMaxSystems = 16;
+
+// The network's backward for one system with a guard: the accumulated gradient is saved first, and if
+// the accumulation is not finite afterwards it is restored and BH_NONFINITE returned, so one diverged
+// system drops only its own contribution to a minibatch.  dscal[2 kMaxSystems] holds the check's sum.
+int mgn_backward_checked(bh_mgn* M, const bh_precond* P, MgnWs& w, hipStream_t s) {
+  const size_t bytes = sizeof(float) * (size_t)M->nparam;
+  BH_CHECK(hipMemcpyAsync(M->gsave, M->g, bytes, hipMemcpyDeviceToDevice, s));
+  int st = mgn_network_backward(M, P, w, s);
+  if (st) return st;
+  hipLaunchKernelGGL(k_sumsq_f, dim3(1), dim3(256), 0, s, M->nparam, (const float*)M->g, M->dscal + 2 * kMaxSystems);
+  BH_CHECK(hipGetLastError());
+  double g2 = 0.0;
+  BH_CHECK(hipMemcpyAsync(&g2, M->dscal + 2 * kMaxSystems, sizeof(double), hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  if (std::isfinite(g2)) return BH_OK;
+  BH_CHECK(hipMemcpyAsync(M->g, M->gsave, bytes, hipMemcpyDeviceToDevice, s));
+  return BH_NONFINITE;
+}
+
+// the accumulated gradient as it was before the latest guarded backward
+int mgn_restore_gradient(bh_mgn* M, hipStream_t s) {
+  BH_CHECK(hipMemcpyAsync(M->g, M->gsave, sizeof(float) * (size_t)M->nparam, hipMemcpyDeviceToDevice, s));
+  return BH_OK;
+}
+
+template <typename V>
+int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, int ns, const double* const* bs,
+             const double* const* xs, int backward, double* loss, hipStream_t s) {
+  if (ns < 1 || ns > kMaxSystems) return BH_BAD_ARG;
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  MgnWs w;
+  mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
+  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  const int nV = P->mesh->nV;
+  const int64_t nP = P->nlower;
+  if ((nV + 255) / 256 > 65536) return BH_UNSUPPORTED;  // loss partials
+  if ((M->form == 1 || M->form == 2) && nP)
+    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O,
+                       M->form == 2 ? 1 : 0);
+  const MgnLayout L = mgn_layout();
+  const int nb = (int)((nV + 255) / 256);
+  double* part = reinterpret_cast<double*>(M->part);
+  if (M->energyLoss) {
+    for (int j = 0; j < ns; ++j) {
+      hipLaunchKernelGGL(k_mgn_energy_fwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.fp, w.C, w.d,
+                         w.s, xs[j], bs[j], w.y, part);
+      hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
+      if (backward)
+        hipLaunchKernelGGL(k_mgn_energy_bwd, g256(nV), dim3(256), 0, s, nV, w.C, w.d, w.s, xs[j], w.y,
+                           M->dscal + 2 * j, ns, j > 0 ? 1 : 0, w.gd);
+    }
+    if (backward && nP) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
   }
+  if (M->teacher) {
+    hipLaunchKernelGGL(k_mgn_teacher_fwd, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O, w.s,
+                       P->lval, P->dinv, part);
+    hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal);
+    if (backward)
+      hipLaunchKernelGGL(k_mgn_teacher_bwd, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O,
+                         w.s, P->lval, P->dinv, M->dscal, w.gO, w.gd);
+    ns = 1;
+  }
+  for (int j = 0; j < ns && !M->energyLoss && !M->teacher; ++j) {
+    // y = Lhat^T xh ; r = Lhat y - bh
+    hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s, xs[j],
+                       1, (const float*)nullptr, w.y);
+    hipLaunchKernelGGL(k_mgn_resid, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.O, w.s, w.y, bs[j],
+                       w.r, part);
+    hipLaunchKernelGGL(k_mgn_loss_sum, dim3(1), dim3(256), 0, s, nb, part, M->dscal + 2 * j);
+    if (backward) {
+      hipLaunchKernelGGL(k_mgn_gz, g256(3ll * nV), dim3(256), 0, s, 3ll * nV, w.r, M->dscal + 2 * j, ns, w.gz);
+      hipLaunchKernelGGL(k_mgn_lt, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ucol, P->ulink, w.C, w.d, w.O, w.s,
+                         (const double*)nullptr, 0, w.gz, w.gy);
+      hipLaunchKernelGGL(k_mgn_gfactor, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, w.C, w.d, w.s, xs[j], w.y,
+                         w.gz, w.gy, j > 0 ? 1 : 0, w.gO, w.gd);
+    }
+  }
+  BH_CHECK(hipGetLastError());
+  if (backward && (M->form == 1 || M->form == 2) && !M->energyLoss) {
+    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.fp, w.gO, w.gd);
+    if (nP && M->form == 1) hipLaunchKernelGGL(k_zero_f, g256(9 * nP), dim3(256), 0, s, 9 * nP, w.gO);
+    BH_CHECK(hipGetLastError());
+  }
+  if (backward && (st = mgn_backward_checked(M, P, w, s))) return st;
   if (loss) {
     double h[2 * kMaxSystems];
     BH_CHECK(hipMemcpyAsync(h, M->dscal, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, s));
@@ -1158,12 +1328,237 @@ int mgn_loss(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed
   return BH_OK;
 }
 
+// ---- the Kaporin loss of the factor forms -----------------------------------------------------------
+// log kappa(M^-1 H) = log(tr(M^-1 H) / n) - log det(M^-1 H) / n for M = L L^T, Kaporin's condition number
+// (I. E. Kaporin, "New convergence results and preconditioning strategies for the conjugate gradient
+// method", Numer. Linear Algebra Appl. 1(2), 1994), which bounds the CG iteration count.  With
+// log det(M^-1 H) = log det H - 2 sum log L_ii (log det H does not depend on the network) and the
+// Hutchinson estimate tr(L^-1 H L^-T) = mean over probes z of f = u^T H u, u = L^-T z (z standard normal on
+// the free DOFs), the loss is log(mean f / n) + (2 / n) sum log L_ii.  Gradient on the factor's pattern:
+// d f / d L = -2 u y^T with y = L^-1 H u, and d log L_ii / d L_ii = 1 / L_ii.  Everything runs in the
+// preconditioner's solve order with its sweeps, after the network's factor is installed; the gradient
+// then maps to the normalised factor (L = S Lhat: dLhat = s_v dL on the rows of v).  n = 3 nV.
+
+__global__ void __launch_bounds__(256)
+    k_kap_probe(int nV, const int32_t* __restrict__ colVerts, const uint8_t* __restrict__ fixed, uint64_t seed,
+                double* __restrict__ z) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3ll * nV) return;
+  const int64_t g = 3ll * colVerts[t / 3] + t % 3;  // the DOF in vertex order: the draw does not depend on
+                                                      // the colouring
+  const uint64_t a = mgn_mix(seed ^ mgn_mix((uint64_t)(2 * g))), b = mgn_mix(seed ^ mgn_mix((uint64_t)(2 * g + 1)));
+  const double u1 = ((double)(a >> 11) + 1.0) * (1.0 / 9007199254740993.0), u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+  z[t] = (fixed && fixed[g]) ? 0.0 : sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// per-block partial sums of a . b (fixed partition: deterministic)
+__global__ void __launch_bounds__(256) k_kap_dot(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[8];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += a[i] * b[i];
+  acc = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+__global__ void __launch_bounds__(256) k_kap_sum(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sh[8];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += part[i];
+  acc = block_reduce_sum(acc, sh);
+  if (threadIdx.x == 0) *out = acc;
+}
+
+// accumulates sum over probes of u_i y_n^T on the lower pairs (vertex-order entry nlsrc[t]) and u_i y_i^T
+// on the diagonal blocks (9 per vertex), positions i in the solve order
+__global__ void __launch_bounds__(256)
+    k_kap_accum(int nV, const int32_t* __restrict__ colVerts, const int32_t* __restrict__ nlptr,
+                const int32_t* __restrict__ nlcol, const int32_t* __restrict__ nlsrc, const double* __restrict__ u,
+                const double* __restrict__ y, int first, float* __restrict__ gO, float* __restrict__ gD) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nV) return;
+  const double u0 = u[3ll * i], u1 = u[3ll * i + 1], u2 = u[3ll * i + 2];
+  const double ui[3] = {u0, u1, u2};
+  for (int t = nlptr[i]; t < nlptr[i + 1]; ++t) {
+    const int n = nlcol[t];
+    float* o = gO + 9ll * nlsrc[t];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float a = (float)(ui[l] * y[3ll * n + c]);
+        o[3 * l + c] = first ? a : o[3 * l + c] + a;
+      }
+  }
+  float* dd = gD + 9ll * colVerts[i];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float a = (float)(ui[l] * y[3ll * i + c]);
+      dd[3 * l + c] = first ? a : dd[3 * l + c] + a;
+    }
+}
+
+// the loss gradient on the normalised factor and through Lhat_vv = C_v T(d_v); partial sums of
+// log Lhat_ll over the free components (for the reported loss)
+__global__ void __launch_bounds__(256)
+    k_kap_final(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ lcol,
+                const uint8_t* __restrict__ fixed, const float* __restrict__ C, const float* __restrict__ d,
+                const float* __restrict__ s, float gscale, float dscale, float* __restrict__ gO,
+                const float* __restrict__ gD, float* __restrict__ gd, double* __restrict__ part) {
+  __shared__ double sh[8];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double ld = 0.0;
+  if (v < nV) {
+    const float sv = s[v];
+    bool fv[3] = {false, false, false};
+    if (fixed)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+    for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
+      const int n = lcol[k];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const bool mk = fixed && (fv[i / 3] || fixed[3ll * n + i % 3]);
+        gO[9ll * k + i] = mk ? 0.f : gscale * sv * gO[9ll * k + i];
+      }
+    }
+    float L[9], G[9];
+    mgn_lvv(C + 6ll * v, d + 6ll * v, L);
+    const float* A = gD + 9ll * v;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const bool keep = c <= l && !fv[l] && !fv[c];
+        G[3 * l + c] = keep ? gscale * sv * A[3 * l + c] + (l == c ? dscale / L[3 * l + c] : 0.f) : 0.f;
+      }
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+      if (!fv[l]) ld += log((double)L[4 * l]);
+    const float* Cv = C + 6ll * v;
+    const float Cm[9] = {Cv[0], 0.f, 0.f, Cv[1], Cv[2], 0.f, Cv[3], Cv[4], Cv[5]};
+    float gT[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gT[3 * i + j] = Cm[i] * G[j] + Cm[3 + i] * G[3 + j] + Cm[6 + i] * G[6 + j];
+    const float* dv = d + 6ll * v;
+    float* g = gd + 6ll * v;
+    g[0] = gT[0] * dtexp(dv[0]);
+    g[1] = gT[3];
+    g[2] = gT[4] * dtexp(dv[2]);
+    g[3] = gT[6];
+    g[4] = gT[7];
+    g[5] = gT[8] * dtexp(dv[5]);
+  }
+  ld = block_reduce_sum(ld, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = ld;
+}
+
+constexpr int kKapDotBlocks = 256;
+
+// One training step of the Kaporin loss on a Hessian (the factor forms): forward, the network's factor
+// into P, nprobe probes, the gradient, the network's backward, Adam.
+template <typename V>
+int mgn_kaporin_step(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, int nprobe, uint64_t seed,
+                     double lr, double* loss, hipStream_t s) {
+  int st = mgn_prepare(M, P);
+  if (st) return st;
+  const bh_mesh* m = P->mesh;
+  const int nV = m->nV;
+  const int64_t nP = P->nlower, n3 = 3ll * nV;
+  MgnWs w;
+  mgn_ws_carve(M->ws, nV, nP, w);
+  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  if (M->form == 2 && nP)
+    hipLaunchKernelGGL(k_mgn_tied_O, g256(nP), dim3(256), 0, s, nP, M->pv, P->lcol, w.fp, w.C, w.d, w.O, 1);
+  hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O, w.s,
+                     P->lval, P->dinv);
+  BH_CHECK(hipGetLastError());
+  if ((st = precond_finalize(P, s))) return st;
+  double* z = svec(P, SV_Z);
+  double* u = svec(P, SV_U);
+  double* uv = svec(P, SV_T);
+  double* huv = svec(P, SV_V);
+  double* hu = svec(P, SV_W);
+  double* y = svec(P, SV_Y);
+  double* part = reinterpret_cast<double*>(M->part);
+  float* gD = w.gm;  // 9 per vertex (free until the network's backward)
+  const int g3 = (int)((n3 + 255) / 256);
+  const int nc = P->ncolors;
+  for (int j = 0; j < nprobe; ++j) {
+    hipLaunchKernelGGL(k_kap_probe, dim3(g3), dim3(256), 0, s, nV, P->colVerts, fixed,
+                       mgn_mix(seed + 0x1000ull * (uint64_t)j), z);
+    for (int c = nc - 1; c >= 0; --c) {  // u = L^-T z
+      const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
+      if (!n) continue;
+      hipLaunchKernelGGL(k_sweep_bwd, dim3(sweep_grid(n)), dim3(256), 0, s, P->h_colPtr[c], n, P->nuptr, P->nucol,
+                         P->uvs, P->dinvs, (const double*)z, u, (const double*)nullptr, (double*)nullptr,
+                         (const PcgState*)nullptr);
+    }
+    hipLaunchKernelGGL(k_permute_rows, dim3(g3), dim3(256), 0, s, nV, P->colVerts, 0, (const double*)u, uv);
+    hipLaunchKernelGGL(k_pcg_spmv<V>, dim3(spmv_blocks_for<V>(nV)), dim3(kRedBlock), 0, s, nV, m->bptr, m->bcol, vals,
+                       (const double*)nullptr, fixed, (const double*)uv, huv, (double*)nullptr,
+                       (const PcgState*)nullptr);
+    hipLaunchKernelGGL(k_permute_rows, dim3(g3), dim3(256), 0, s, nV, P->colVerts, 1, (const double*)huv, hu);
+    hipLaunchKernelGGL(k_kap_dot, dim3(kKapDotBlocks), dim3(256), 0, s, n3, (const double*)u, (const double*)hu,
+                       part);
+    hipLaunchKernelGGL(k_kap_sum, dim3(1), dim3(256), 0, s, kKapDotBlocks, (const double*)part, M->dscal + j);
+    CgVecs none{};
+    for (int c = 0; c < nc; ++c) {  // y = L^-1 H u
+      const int n = P->h_colPtr[c + 1] - P->h_colPtr[c];
+      if (!n) continue;
+      hipLaunchKernelGGL(k_sweep_fwd, dim3(sweep_grid(n)), dim3(256), 0, s, P->h_colPtr[c], n, P->nlptr, P->nlcol,
+                         P->lvs, P->dinvs, hu, y, none, 0);
+    }
+    hipLaunchKernelGGL(k_kap_accum, g256(nV), dim3(256), 0, s, nV, P->colVerts, P->nlptr, P->nlcol, P->nlsrc,
+                       (const double*)u, (const double*)y, j == 0 ? 1 : 0, w.gO, gD);
+    BH_CHECK(hipGetLastError());
+  }
+  double f[kMaxSystems];
+  BH_CHECK(hipMemcpyAsync(f, M->dscal, sizeof(double) * nprobe, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  double tr = 0.0;
+  for (int j = 0; j < nprobe; ++j) tr += f[j];
+  tr /= nprobe;
+  if (!(tr > 0.0) || !std::isfinite(tr)) return BH_NONFINITE;  // an indefinite or broken factor: no step
+  const double n = (double)n3;
+  const int nb = (int)((nV + 255) / 256);
+  hipLaunchKernelGGL(k_kap_final, dim3(nb), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.s,
+                     (float)(-2.0 / (tr * nprobe)), (float)(2.0 / n), w.gO, (const float*)gD, w.gd, part);
+  hipLaunchKernelGGL(k_kap_sum, dim3(1), dim3(256), 0, s, nb, (const double*)part, M->dscal + kMaxSystems);
+  if (M->form == 2)
+    hipLaunchKernelGGL(k_mgn_tied_bwd, g256(nV), dim3(256), 0, s, nV, P->uptr, P->ulink, w.C, w.d, w.fp, w.gO, w.gd);
+  BH_CHECK(hipGetLastError());
+  if ((st = mgn_backward_checked(M, P, w, s))) return st;
+  double logdet = 0.0;
+  BH_CHECK(hipMemcpyAsync(&logdet, M->dscal + kMaxSystems, sizeof(double), hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  const double lv = std::log(tr / n) + 2.0 * logdet / n;
+  if (loss) *loss = lv;
+  if (!std::isfinite(lv)) {  // drop this system's gradient, keep the accumulation before it
+    if ((st = mgn_restore_gradient(M, s))) return st;
+    return BH_NONFINITE;
+  }
+  if (lr == 0.0) return BH_OK;  // the gradient stays accumulated in M->g
+  M->step += 1;
+  const double b1 = 0.9, b2 = 0.999;
+  const double c1 = 1.0 - std::pow(b1, (double)M->step), c2 = 1.0 - std::pow(b2, (double)M->step);
+  hipLaunchKernelGGL(k_mgn_adam, g256(M->nparam), dim3(256), 0, s, M->nparam, M->w, M->g, M->m1, M->m2, (float)lr,
+                     (float)b1, (float)b2, 1e-8f, (float)c1, (float)c2);
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
 // One training step on a Hessian: nprobe probe systems (the normalised right-hand side first when
 // rhs is given, then standard normal vectors drawn from seed), loss, backward, Adam.
 template <typename V>
 int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, const double* rhs, int nprobe,
                    uint64_t seed, double lr, double* loss, hipStream_t s) {
   if (nprobe < 1 || nprobe > kMaxSystems) return BH_BAD_ARG;
+  if (M->kaporin) return mgn_kaporin_step<V>(M, const_cast<bh_precond*>(P), vals, fixed, nprobe, seed, lr, loss, s);
   int st = mgn_prepare(M, P);
   if (st) return st;
   const bh_mesh* m = P->mesh;
@@ -1196,14 +1591,26 @@ int mgn_train_step(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t*
     xs[j] = x;
   }
   BH_CHECK(hipGetLastError());
+  if (M->teacher) {
+    // the teacher: block IC(0) of this Hessian (diagonal-shift retry as the solver does), into P
+    double shift = 0.0;
+    for (int attempt = 0;; ++attempt) {
+      int fails = 0;
+      if ((st = precond_factor_bic0(const_cast<bh_precond*>(P), vals, (const double*)nullptr, fixed, shift, &fails, s)))
+        return st;
+      if (fails == 0) break;
+      if (attempt >= 6) return BH_SOLVE_FAILED;
+      shift = (shift > 0.0) ? 4.0 * shift : 1e-3;
+    }
+  }
   double lv = 0.0;
-  if ((st = mgn_loss<V>(M, P, vals, fixed, nprobe, bs, xs, 1, &lv, s))) return st;
+  if ((st = mgn_loss<V>(M, P, vals, fixed, M->teacher ? 1 : nprobe, bs, xs, 1, &lv, s))) return st;
   if (loss) *loss = lv;
-  if (!std::isfinite(lv)) {  // a diverged step: drop its gradient, keep the parameters
-    hipLaunchKernelGGL(k_zero_f, g256(M->nparam), dim3(256), 0, s, M->nparam, M->g);
-    BH_CHECK(hipGetLastError());
+  if (!std::isfinite(lv)) {  // a diverged step: drop its gradient (the accumulation before it stays)
+    if ((st = mgn_restore_gradient(M, s))) return st;
     return BH_NONFINITE;
   }
+  if (lr == 0.0) return BH_OK;  // the gradient stays accumulated in M->g
   M->step += 1;
   const double b1 = 0.9, b2 = 0.999;
   const double c1 = 1.0 - std::pow(b1, (double)M->step), c2 = 1.0 - std::pow(b2, (double)M->step);
@@ -1319,9 +1726,9 @@ int bh_mgn_create(uint64_t seed, bh_mgn** out) {
   mgn_init_host(h, seed);
   const size_t bytes = sizeof(float) * (size_t)M->nparam;
   const size_t partBytes = sizeof(float) * (size_t)kOuterBlocks * (3 * kW * kW + kW) + sizeof(double) * 2 * 65536;
-  if (hipMalloc(&M->w, bytes) != hipSuccess || hipMalloc(&M->g, bytes) != hipSuccess ||
+  if (hipMalloc(&M->w, bytes) != hipSuccess || hipMalloc(&M->g, bytes) != hipSuccess || hipMalloc(&M->gsave, bytes) != hipSuccess ||
       hipMalloc(&M->m1, bytes) != hipSuccess || hipMalloc(&M->m2, bytes) != hipSuccess ||
-      hipMalloc(&M->part, partBytes) != hipSuccess || hipMalloc(&M->dscal, sizeof(double) * 2 * kMaxSystems) != hipSuccess) {
+      hipMalloc(&M->part, partBytes) != hipSuccess || hipMalloc(&M->dscal, sizeof(double) * (2 * kMaxSystems + 1)) != hipSuccess) {
     delete M;
     return BH_DEVICE_ERROR;
   }
@@ -1336,7 +1743,7 @@ int bh_mgn_create(uint64_t seed, bh_mgn** out) {
 int bh_mgn_destroy(bh_mgn* M) {
   if (!M) return BH_BAD_ARG;
   (void)hipDeviceSynchronize();
-  void* ptrs[] = {M->w, M->g, M->m1, M->m2, M->part, M->dscal, M->ws, M->pv, M->probe};
+  void* ptrs[] = {M->w, M->g, M->gsave, M->m1, M->m2, M->part, M->dscal, M->ws, M->pv, M->probe};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete M;
@@ -1344,10 +1751,15 @@ int bh_mgn_destroy(bh_mgn* M) {
 }
 
 int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution, int energy_loss) {
-  if (!M || form < 0 || form > 2 || (energy_loss && form != 1)) return BH_BAD_ARG;
+  // energy_loss: 0 the residual loss, 1 the energy-ratio loss (form 1), 2 the IC(0) teacher (forms 0, 2)
+  if (!M || form < 0 || form > 2 || energy_loss < 0 || energy_loss > 3 || (energy_loss == 1 && form != 1) ||
+      (energy_loss >= 2 && form == 1))
+    return BH_BAD_ARG;
   M->form = form;
+  M->teacher = energy_loss == 2 ? 1 : 0;
+  M->kaporin = energy_loss == 3 ? 1 : 0;
   M->probeSolution = probe_is_solution ? 1 : 0;
-  M->energyLoss = energy_loss ? 1 : 0;
+  M->energyLoss = energy_loss == 1 ? 1 : 0;
   return BH_OK;
 }
 
@@ -1388,7 +1800,7 @@ int bh_mgn_loss_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint
 int bh_mgn_train_step_device(bh_mgn* M, bh_precond* P, const float* d_vals, const uint8_t* d_fixed_dof,
                              const double* d_rhs, int nprobe, uint64_t seed, double lr, double* loss,
                              bh_stream_t stream) {
-  if (!M || !P || !d_vals || !(lr > 0)) return BH_BAD_ARG;
+  if (!M || !P || !d_vals || !(lr >= 0)) return BH_BAD_ARG;
   if (P->mesh->nV == 0) return BH_OK;
   return mgn_train_step<float>(M, P, d_vals, d_fixed_dof, d_rhs, nprobe, seed, lr, loss, as_stream(stream));
 }

@@ -1,22 +1,26 @@
 """Training and evaluation of the learned preconditioner (the mesh-graph network of
 csrc/mgn_impl.h) on the Newton systems that the plain solver generates.
 
-Training problems: jittered beams of 24x24x96 ... 40x40x160 vertices, per-tet Young's modulus
-log-uniform in [1e5, 1e7] Pa, Poisson ratio uniform in [0.30, 0.48], 70 / 20 / 10 % Neo-Hookean /
-StVK / corotational tets, x = 0 clamped, gravity plus a random traction of magnitude up to 5e4 Pa in
-a random direction on the free end.  Each problem is solved by the precision-form Newton with
-block symmetric Gauss-Seidel PCG to 1e-6 relative residual; every Hessian it assembles is handed
-(on the device, through bh_newton_opts.on_system) to a few Adam steps of the network on probe
-systems: the Newton direction the solver found (a smooth vector: the modes CG converges on last)
-and standard normal vectors (a Hutchinson estimate of the relative Frobenius error of M).
+Problems: jittered beams of 96x24x24 ... 160x40x40 vertices, per-tet Young's modulus log-uniform in
+[1e5, 1e7] Pa, Poisson ratio uniform in [0.30, 0.48], 70 / 20 / 10 % Neo-Hookean / StVK / corotational
+tets, x = 0 clamped, gravity plus a random traction of magnitude up to 5e4 Pa in a random direction on
+the free end.  Each problem is solved by the precision-form Newton with block symmetric Gauss-Seidel
+PCG to 1e-6 relative residual, and the FP32 Hessians it assembles (handed over on the device through
+bh_newton_opts.on_system) are the training systems.
 
-Two forms (bh_mgn_set_form): "sgs" (default), the Eisenstat form M = (Dt + L) Dt^-1 (Dt + L)^T with
-the network writing the block diagonal Dt (Dt = D, block symmetric Gauss-Seidel, at zero decoders),
-solved with BH_PRECOND_LEARNED_SGS; and "factor", a general block lower-triangular factor
-(BH_PRECOND_LEARNED).
+The shipped weights (mgn_weights.f32) are the tied factor form (bh_mgn_set_form form 2: block
+symmetric Gauss-Seidel as a factor at zero decoders, solved with BH_PRECOND_LEARNED and the two-level
+coarse correction) trained on the Kaporin condition number (loss 3) of L L^T against each Hessian:
+150 problems, their first three Newton systems each (450 systems), 30 epochs in minibatches of 8
+systems, Adam at 3e-4, 8 Hutchinson probes per system:
 
-    python -m bartels_amd.learned train --solves 120 --out weights.f32
-    python -m bartels_amd.learned eval --weights weights.f32 --problems 10
+    python -m bartels_amd.learned fit --problems 150 --epochs 30 --out weights.f32
+    python -m bartels_amd.learned eval --weights weights.f32 --problems 6
+
+The older online trainer (`train`: a few Adam steps per system as the solver meets it, residual /
+energy / teacher losses) stays for the other forms: "sgs", the Eisenstat form M = (Dt + L) Dt^-1
+(Dt + L)^T with the network writing the block diagonal Dt (BH_PRECOND_LEARNED_SGS), and "factor", a
+general block lower-triangular factor (block Jacobi at zero decoders).
 """
 import argparse
 import ctypes
@@ -32,6 +36,8 @@ from . import tetmesh as tm
 
 H = meshgen.H_DEFAULT
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mgn_weights.f32")
+# the form the shipped weights were trained for (the tied factor: block SGS as a factor at zero decoders)
+DEFAULT_FORM = "tied"
 
 
 def random_problem(grid, seed):
@@ -139,15 +145,23 @@ FORMS = {"factor": 0, "sgs": 1, "tied": 2}
 PRECOND = {"factor": 4, "sgs": 6, "tied": 4}  # BH_PRECOND_LEARNED, BH_PRECOND_LEARNED_SGS, BH_PRECOND_LEARNED
 
 
+LOSSES = {"residual": 0, "energy": 1, "teacher": 2, "kaporin": 3}
+
+
 class Network:
-    def __init__(self, seed=1, weights=None, form="sgs", probe_solution=True, energy_loss=True):
+    def __init__(self, seed=1, weights=None, form="sgs", probe_solution=True, energy_loss=True, loss=None):
+        """loss: "residual", "energy" (the Eisenstat form) or "teacher" (the factor forms: the IC(0) factor
+        of each training Hessian as the target); default "energy" for the Eisenstat form with energy_loss,
+        else "residual" """
         self.lib = _lib.load()
         self.h = ctypes.c_void_p()
         _lib.check(self.lib.bh_mgn_create(seed, ctypes.byref(self.h)), "bh_mgn_create")
         self.form = form
         self.precond = PRECOND[form]
-        _lib.check(self.lib.bh_mgn_set_form(self.h, FORMS[form], 1 if probe_solution else 0,
-                                            1 if (energy_loss and form == "sgs") else 0), "bh_mgn_set_form")
+        if loss is None:
+            loss = "energy" if (energy_loss and form == "sgs") else "residual"
+        _lib.check(self.lib.bh_mgn_set_form(self.h, FORMS[form], 1 if probe_solution else 0, LOSSES[loss]),
+                   "bh_mgn_set_form")
         n = ctypes.c_int64()
         self.lib.bh_mgn_num_params(self.h, ctypes.byref(n))
         self.n = n.value
@@ -175,8 +189,10 @@ class Network:
 
 
 def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, form="sgs", probe_solution=True,
-          log=print, ny_range=(24, 41)):
-    net = Network(seed=seed + 12345, form=form, probe_solution=probe_solution)
+          log=print, ny_range=(24, 41), loss=None, init=None):
+    net = Network(seed=seed + 12345, form=form, probe_solution=probe_solution, loss=loss)
+    if init is not None:
+        net.load(init)
     rng = np.random.default_rng(seed)
     history = []
     t0 = time.perf_counter()
@@ -214,6 +230,68 @@ def train(solves, out, seed=0, steps=4, lr=1e-3, newton_its=4, probes=4, form="s
     return history
 
 
+def collect_systems(problems, seed=0, newton_its=3, ny_range=(24, 29), log=print):
+    """the training set: the FP32 Newton systems (Hessian values on the device) that the plain solver (block
+    symmetric Gauss-Seidel PCG, precision form) meets on `problems` random beams; [(Solver, values)]"""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library is linked against
+    systems = []
+    for i in range(problems):
+        ny = int(rng.integers(*ny_range))
+        S = Solver(random_problem((4 * ny, ny, ny), seed=2000 + 7 * i + seed))
+        caught = []
+
+        def on_system(user, it, v32, v64, b, x):
+            t = torch.empty(S.mesh.nnz, dtype=torch.float32, device="cuda")
+            hip.hipMemcpy(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(v32), ctypes.c_size_t(4 * S.mesh.nnz), 3)
+            caught.append(t)
+
+        S.newton(5, precision=1, max_it=newton_its, on_system=on_system)
+        torch.cuda.synchronize()
+        S.mesh.trim()  # training needs the pattern and the preconditioner lists only
+        systems += [(S, t) for t in caught]
+        log(f"problem {i}: grid {(4 * ny, ny, ny)}, {len(caught)} systems")
+    return systems
+
+
+def train_batched(systems, out, epochs=30, batch=6, lr=3e-4, probes=8, form="tied", loss="kaporin", seed=0,
+                  log=print, init=None):
+    """minibatch training over a fixed set of Newton systems: per system the loss gradient is accumulated
+    (bh_mgn_train_step_device with lr = 0), one Adam step per `batch` systems, `epochs` passes in a fresh
+    random order each"""
+    lib = _lib.load()
+    net = Network(seed=seed + 12345, form=form, probe_solution=False, loss=loss)
+    if init is not None:
+        net.load(init)
+    rng = np.random.default_rng(seed)
+    history = []
+    for ep in range(epochs):
+        order = rng.permutation(len(systems))
+        losses, dropped = [], 0
+        for bi, idx in enumerate(order):
+            S, vals = systems[idx]
+            L = ctypes.c_double()
+            st = lib.bh_mgn_train_step_device(net.h, S.P, ctypes.c_void_p(vals.data_ptr()),
+                                              ctypes.c_void_p(S.dfixed.data_ptr()), None, probes,
+                                              (seed * 1000003 + 7919 * ep + 31 * bi) & (2**64 - 1), 0.0,
+                                              ctypes.byref(L), None)
+            if st == _lib.BH_NONFINITE:
+                dropped += 1
+                continue
+            _lib.check(st, "mgn_train_step")
+            losses.append(L.value)
+            if (bi + 1) % batch == 0 or bi + 1 == len(order):
+                _lib.check(lib.bh_mgn_adam_step(net.h, lr, 0.9, 0.999, 1e-8, None), "bh_mgn_adam_step")
+        history.append(dict(epoch=ep, loss=float(np.mean(losses)) if losses else None, dropped=dropped))
+        mean = f"{history[-1]['loss']:.5f}" if losses else "n/a"
+        log(f"epoch {ep}: mean loss {mean}, dropped {dropped}")
+    net.save(out)
+    net.close()
+    return history
+
+
 def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="sgs", coarse=0, log=print,
              grid=(128, 32, 32)):
     """held-out problems (32x32x128 by default): PCG iterations and time to 1e-6 over the first Newton steps,
@@ -235,7 +313,9 @@ def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="
                    learned_rel_gradient=rl.gradient_norm / rl.gradient_norm0 if rl.gradient_norm0 else 0.0)
         if coarse:
             _, r2, _, w2 = S.newton(7, precision=1, max_it=newton_its, pcg_max=pcg_max)
-            row.update(jacobi2l_pcg=r2.pcg_iterations_total, jacobi2l_s=w2)
+            _, r3, _, w3 = S.newton(2, precision=1, max_it=newton_its, pcg_max=pcg_max, coarse=1)
+            row.update(jacobi2l_pcg=r2.pcg_iterations_total, jacobi2l_s=w2, bic0_2l_pcg=r3.pcg_iterations_total,
+                       bic0_2l_s=w3)
         S.close()
         rows.append(row)
         log(json.dumps(rows[-1]))
@@ -246,7 +326,19 @@ def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
-    a = sub.add_parser("train")
+    f = sub.add_parser("fit", help="minibatch training over a fixed set of Newton systems (the shipped recipe)")
+    f.add_argument("--problems", type=int, default=150)
+    f.add_argument("--newton-its", type=int, default=3)
+    f.add_argument("--epochs", type=int, default=30)
+    f.add_argument("--batch", type=int, default=8)
+    f.add_argument("--lr", type=float, default=3e-4)
+    f.add_argument("--probes", type=int, default=8)
+    f.add_argument("--form", choices=sorted(FORMS), default=DEFAULT_FORM)
+    f.add_argument("--loss", choices=["kaporin", "teacher", "residual"], default="kaporin")
+    f.add_argument("--data-seed", type=int, default=1)
+    f.add_argument("--seed", type=int, default=0)
+    f.add_argument("--out", default=DEFAULT_WEIGHTS)
+    a = sub.add_parser("train", help="online training, a few Adam steps per system as the solver meets it")
     a.add_argument("--solves", type=int, default=120)
     a.add_argument("--steps", type=int, default=4)
     a.add_argument("--lr", type=float, default=1e-3)
@@ -258,20 +350,28 @@ def main():
     a.add_argument("--out", default=DEFAULT_WEIGHTS)
     b = sub.add_parser("eval")
     b.add_argument("--weights", default=DEFAULT_WEIGHTS)
-    b.add_argument("--problems", type=int, default=10)
-    b.add_argument("--form", choices=sorted(FORMS), default="sgs")
+    b.add_argument("--problems", type=int, default=6)
+    b.add_argument("--form", choices=sorted(FORMS), default=DEFAULT_FORM)
+    b.add_argument("--coarse", type=int, default=1)
+    b.add_argument("--seed", type=int, default=555)
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)  # noqa: E731
-    if args.cmd == "train":
+    if args.cmd == "fit":
+        systems = collect_systems(args.problems, seed=args.data_seed, newton_its=args.newton_its, ny_range=(24, 41),
+                                  log=log)
+        train_batched(systems, args.out, epochs=args.epochs, batch=args.batch, lr=args.lr, probes=args.probes,
+                      form=args.form, loss=args.loss, seed=args.seed, log=log)
+        for S, _ in systems:
+            S.close()
+    elif args.cmd == "train":
         train(args.solves, args.out, seed=args.seed, steps=args.steps, lr=args.lr, probes=args.probes,
               form=args.form, probe_solution=args.probe == "solution", log=log)
     else:
-        rows = evaluate(args.weights, args.problems, form=args.form, log=log)
-        j = np.array([r["jacobi_pcg"] for r in rows], float)
-        l = np.array([r["learned_pcg"] for r in rows], float)
-        b = np.array([r["sgs_pcg"] for r in rows], float)
-        log(json.dumps(dict(mean_jacobi_pcg=j.mean(), mean_learned_pcg=l.mean(), mean_sgs_pcg=b.mean(),
-                            sgs_over_learned=(b / l).mean(), jacobi_over_learned=(j / l).mean())))
+        rows = evaluate(args.weights, args.problems, form=args.form, coarse=args.coarse, seed=args.seed,
+                        newton_its=2, log=log)
+        keys = ["jacobi_pcg", "sgs_pcg", "learned_pcg", "learned_s"] + (
+            ["jacobi2l_pcg", "bic0_2l_pcg", "jacobi2l_s", "bic0_2l_s"] if args.coarse else [])
+        log(json.dumps({"mean_" + k: float(np.mean([r[k] for r in rows])) for k in keys}))
 
 
 if __name__ == "__main__":

@@ -107,6 +107,11 @@ class TetMesh:
     def handle(self):
         return self._h
 
+    def trim(self):
+        """free the work buffers the mesh keeps between calls (remade on next use): bh_mesh_trim"""
+        if self._h:
+            _lib.check(self._lib.bh_mesh_trim(self._h), "bh_mesh_trim")
+
     def close(self):
         if self._h:
             self._lib.bh_mesh_destroy(self._h)

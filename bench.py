@@ -112,10 +112,11 @@ class NewtonBench:
                                                      ctypes.c_void_p(self.f.data_ptr()), None)
         self.P = ctypes.c_void_p()
         _lib.check(lib.bh_precond_create(mesh.handle, ctypes.byref(self.P)), "bh_precond_create")
-        self.net = (learned.Network(weights=learned.DEFAULT_WEIGHTS)
+        # the shipped weights (the tied factor form) and their untrained twin: at zero decoders the tied
+        # factor is block symmetric Gauss-Seidel written as a factor
+        self.net = (learned.Network(weights=learned.DEFAULT_WEIGHTS, form=learned.DEFAULT_FORM)
                     if os.path.exists(learned.DEFAULT_WEIGHTS) else None)
-        # the learned factor form, untrained: its decoders are zero, so its factor is block Jacobi
-        self.net0 = learned.Network(form="factor")
+        self.net0 = learned.Network(form=learned.DEFAULT_FORM)
 
     def close(self):
         self.lib.bh_precond_destroy(self.P)
@@ -161,17 +162,14 @@ class NewtonBench:
                                       rel_gradient=rep.gradient_norm / rep.gradient_norm0 if rep.gradient_norm0 else 0.0)
 
     def variants(self):
-        v = [("fp64_jacobi", 0, 0), ("fp32_jacobi", 1, 0), ("fp32_bic0", 1, 2), ("fp32_sgs", 1, 5),
-             ("fp32full_sgs", 2, 5), ("fp64_jacobi2l", 0, 7), ("fp32_jacobi2l", 1, 7), ("fp64_bic0_2l", 0, 2, 1),
-             ("fp32_bic0_2l", 1, 2, 1), ("fp32_learned_factor_untrained_2l", 1, self.net0.precond, 1)]
+        """(name, element precision, preconditioner, coarse correction, network handle or None)"""
+        v = [("fp64_jacobi", 0, 0, 0, None), ("fp32_jacobi", 1, 0, 0, None), ("fp32_bic0", 1, 2, 0, None),
+             ("fp32_sgs", 1, 5, 0, None), ("fp32full_sgs", 2, 5, 0, None), ("fp64_jacobi2l", 0, 7, 0, None),
+             ("fp32_jacobi2l", 1, 7, 0, None), ("fp64_bic0_2l", 0, 2, 1, None), ("fp32_bic0_2l", 1, 2, 1, None),
+             ("fp32_learned_untrained_2l", 1, self.net0.precond, 1, self.net0.h)]
         if self.net is not None:
-            v.append(("fp32_learned_2l", 1, self.net.precond, 1))
+            v.append(("fp32_learned_2l", 1, self.net.precond, 1, self.net.h))
         return v
-
-    def mgn_for(self, precond):
-        if precond == self.net0.precond and (self.net is None or self.net.precond != precond):
-            return self.net0.h
-        return self.net.h if self.net is not None and precond == self.net.precond else None
 
 
 # the FP64 judge of the Newton comparisons: FP64 elements, block IC(0) with the coarse correction, PCG to
@@ -190,9 +188,8 @@ def newton_steps(p, mesh, steps, pcg_max, solve_tol=1e-6, solve_max_it=40, solve
     try:
         qj, rj = nb.run(JUDGE[1], JUDGE[2], JUDGE[3], pcg_tol=1e-12, max_it=steps)
         res[JUDGE[0]] = rj
-        for name, prec, precond, *coarse in nb.variants():
-            qs[name], res[name] = nb.run(prec, precond, coarse[0] if coarse else 0, pcg_tol=1e-6, max_it=steps,
-                                         mgn=nb.mgn_for(precond))
+        for name, prec, precond, coarse, mgn in nb.variants():
+            qs[name], res[name] = nb.run(prec, precond, coarse, pcg_tol=1e-6, max_it=steps, mgn=mgn)
         u = np.linalg.norm(qj - nb.q0)
         for name in qs:
             res[name]["rel_displacement_vs_fp64_judge"] = float(np.linalg.norm(qs[name] - qj) / u)
@@ -203,16 +200,15 @@ def newton_steps(p, mesh, steps, pcg_max, solve_tol=1e-6, solve_max_it=40, solve
         # time to solution (relative gradient solve_tol)
         sol = {}
         if solve_variants is None:
-            solve_variants = ["fp64_jacobi", "fp32_bic0_2l", "fp32_jacobi2l"] + (
+            solve_variants = ["fp64_jacobi", "fp32_bic0_2l", "fp32_jacobi2l", "fp32_learned_untrained_2l"] + (
                 ["fp32_learned_2l"] if nb.net is not None else [])
         qsj, sj = nb.run(JUDGE[1], JUDGE[2], JUDGE[3], pcg_tol=1e-12, max_it=solve_max_it, gtol=solve_tol)
         sol[JUDGE[0]] = sj
         uj = np.linalg.norm(qsj - nb.q0)
         table = {v[0]: v for v in nb.variants()}
         for name in solve_variants:
-            _, prec, precond, *coarse = table[name]
-            qv, rv = nb.run(prec, precond, coarse[0] if coarse else 0, pcg_tol=1e-6, max_it=solve_max_it,
-                            gtol=solve_tol, mgn=nb.mgn_for(precond))
+            _, prec, precond, coarse, mgn = table[name]
+            qv, rv = nb.run(prec, precond, coarse, pcg_tol=1e-6, max_it=solve_max_it, gtol=solve_tol, mgn=mgn)
             rv["rel_displacement_vs_fp64_judge"] = float(np.linalg.norm(qv - qsj) / uj)
             sol[name] = rv
         for name in sol:

@@ -101,6 +101,12 @@ int bh_mesh_create(int nV, int nT, const int* E, const double* dphidX, const dou
 int bh_mesh_create_device(int nV, int nT, const int32_t* d_E, const double* d_V, const double* d_dphidX,
                           const double* d_volume, bh_mesh** mesh);
 int bh_mesh_destroy(bh_mesh* mesh);
+/* Frees the buffers the mesh makes on first use and keeps between calls -- the PSD records and Q
+ * factors, the gradient's corner forces, the Newton and coarse-space workspaces, the FP32 geometry, the
+ * host-API value staging -- after a device synchronisation.  The next call that needs one makes it
+ * again.  For callers that hold many meshes between solves.  No reference counterpart: the reference
+ * keeps no state between calls. */
+int bh_mesh_trim(bh_mesh* mesh);
 int bh_mesh_nnz(const bh_mesh* mesh, int64_t* nnz);
 /* rowptr: 3*nV + 1 entries, colind: nnz entries (host arrays). */
 int bh_mesh_pattern(const bh_mesh* mesh, int32_t* rowptr, int32_t* colind);
@@ -289,7 +295,11 @@ int bh_mgn_destroy(bh_mgn* mgn);
  * bh_mgn_train_step_device receives is a solution x of the Newton system (probe 0 is x itself) instead of
  * its right-hand side.  No counterpart in the reference (the learned preconditioner is this build's
  * addition).  energy_loss = 1 (form 1 only): the loss is the mean over probes x of
- * (log(x^T M x) - log(x^T H x))^2 instead of |M x - H x|^2 / |H x|^2.  A training step whose loss is not
+ * (log(x^T M x) - log(x^T H x))^2 instead of |M x - H x|^2 / |H x|^2.  energy_loss = 2 (forms 0 and 2):
+ * the IC(0) teacher, |Lhat - T|_F^2 / |T|_F^2 with T the block IC(0) factor of the same Hessian in the
+ * diagonally scaled space; bh_mgn_train_step_device then factors each Hessian it is given (no probes).
+ * energy_loss = 3 (forms 0 and 2): Kaporin's condition number of L L^T against H (see
+ * bh_mgn_train_step_device).  A training step whose loss is not
  * finite returns BH_NONFINITE and leaves the parameters. */
 int bh_mgn_set_form(bh_mgn* mgn, int form, int probe_is_solution, int energy_loss);
 int bh_mgn_num_params(const bh_mgn* mgn, int64_t* nparams);
@@ -304,7 +314,11 @@ int bh_mgn_loss_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, co
 /* One training step on a Hessian: nprobe (<= 16) probe systems -- the normalised right-hand side
  * d_rhs first when given, then standard normal vectors drawn from seed -- give a Hutchinson estimate
  * of |L L^T - H|_F^2 / |H|_F^2 in the diagonally scaled norm; backward and one Adam step (lr,
- * beta 0.9 / 0.999, eps 1e-8).  loss (host, optional) synchronises. */
+ * beta 0.9 / 0.999, eps 1e-8).  With the Kaporin loss (bh_mgn_set_form, energy_loss = 3; factor forms)
+ * the probes instead give the Hutchinson estimate of tr((L L^T)^-1 H) in
+ * log(tr / n) + (2 / n) log det L (n = 3 nV), Kaporin's condition number up to log det H / n.
+ * lr = 0: the gradient is accumulated and no Adam step taken (bh_mgn_get_params returns it).
+ * loss (host, optional) synchronises. */
 int bh_mgn_train_step_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, const uint8_t* d_fixed_dof,
                              const double* d_rhs, int nprobe, uint64_t seed, double lr, double* loss,
                              bh_stream_t stream);

@@ -128,3 +128,42 @@ def test_device_setup_at_scale():
     t_host, t_dev, t_devV = _compare(p.V, p.E, p.q)
     print(f"64x64x256 beam ({p.E.shape[0]} tets): bh_mesh_create {t_host:.3f} s, bh_mesh_create_device "
           f"{t_dev:.3f} s, with dphidX and volume from V {t_devV:.3f} s")
+
+
+def test_trim_then_recompute_is_identical():
+    """bh_mesh_trim frees the buffers a mesh keeps between calls; every output made after it (the PSD
+    records and Q factors, the corner forces and the FP32 copies are made again) equals the output
+    made before it to the last bit, and trimming twice or before any call is harmless"""
+    import torch
+
+    from bartels_amd import tetmesh as tm
+
+    V, E = meshgen.beam_grid(16, 6, 6, 0.05)
+    V = meshgen.jitter(V, E, 0.05, 0.1, 3)
+    nV = V.shape[0]
+    dX, vol = tm.linear_tetmesh_dphi_dX(V, E)
+    mesh = tm.TetMesh(nV, E, dX, vol, V=V)
+    try:
+        mesh.trim()  # nothing made yet
+        lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+        params = torch.tensor(np.asfortranarray(meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu))
+                              .ravel(order="F"), device="cuda")
+        rng = np.random.default_rng(5)
+        q = torch.tensor(V.ravel() + 0.001 * rng.standard_normal(3 * nV), device="cuda")
+        before = _outputs(mesh, q, params)
+        assert all(bool(torch.isfinite(t).all()) for t in before)
+        H32 = torch.empty(mesh.nnz, dtype=torch.float32, device="cuda")
+        mesh.dq2_f32_device(0, q, params.to(torch.float32), H32, callback=tm.simple_psd_fix(1e-3))
+        torch.cuda.synchronize()
+        h32 = H32.clone()
+        mesh.trim()
+        mesh.trim()
+        after = _outputs(mesh, q, params)
+        mesh.dq2_f32_device(0, q, params.to(torch.float32), H32, callback=tm.simple_psd_fix(1e-3))
+        torch.cuda.synchronize()
+        for name, a, b in zip(("dq2", "dq2 + PSD", "dq", "q"), before, after):
+            print(f"{name}: identical after trim: {bool(torch.equal(a, b))}")
+            assert torch.equal(a, b)
+        assert torch.equal(h32, H32)
+    finally:
+        mesh.close()
