@@ -397,14 +397,37 @@ __global__ void __launch_bounds__(256)
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   const IncList L = inc_list(m, v);
   const int64_t nT = m.nT;
-  for (int k = 0; k < L.len; ++k) {
-    const int code = m.incE[L.first + 64ll * k];
-    if (code < 0) break;
-    const int e = code >> 2, a = code & 3;
-    const double* fa = f + 3 * a * nT + e;
-    g0 += fa[0];
-    g1 += fa[nT];
-    g2 += fa[2 * nT];
+  // four steps per round: their codes, then their forces, are loaded together (independent loads in
+  // flight), and the sums still take the forces in ascending element order, stopping at the list's
+  // first negative code
+  constexpr int U = 4;
+  for (int k = 0; k < L.len; k += U) {
+    int cd[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) cd[j] = (k + j < L.len) ? m.incE[L.first + 64ll * (k + j)] : -1;
+    double fx[U][3];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (cd[j] >= 0) {
+        const double* fa = f + 3 * (cd[j] & 3) * nT + (cd[j] >> 2);
+        fx[j][0] = fa[0];
+        fx[j][1] = fa[nT];
+        fx[j][2] = fa[2 * nT];
+      } else {
+        fx[j][0] = fx[j][1] = fx[j][2] = 0.0;
+      }
+    }
+    bool done = false;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      done = done || cd[j] < 0;
+      if (!done) {
+        g0 += fx[j][0];
+        g1 += fx[j][1];
+        g2 += fx[j][2];
+      }
+    }
+    if (done) break;
   }
   g[3ll * v] = g0;
   g[3ll * v + 1] = g1;
@@ -509,9 +532,9 @@ __device__ __forceinline__ void tet_blocks2(const MeshView& m, const double* __r
 
 // one block of the vertex's block row: corner mm
 template <int MODEL, typename T>
-__device__ __forceinline__ void tet_block1(const MeshView& m, const double* __restrict__ q,
-                                           const T* __restrict__ params, int e, int a, int mm, bool stretch_ref,
-                                           T blk[1][9]) {
+__device__ __forceinline__ void tet_block1e(const MeshView& m, const double* __restrict__ q,
+                                            const T* __restrict__ params, int e, int4 ev, int a, int mm,
+                                            bool stretch_ref, T blk[1][9]) {
   if (MODEL == M_MASS) {
     const T mass = params[e] * load_vol<T>(m, e);
     const T c = (mm == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
@@ -524,7 +547,7 @@ __device__ __forceinline__ void tet_block1(const MeshView& m, const double* __re
   load_phi(m, e, phi);
   {
     T Gh[9];
-    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+    load_FG(m, q, ev, phi, P.F, Gh);
   }
   P.vol = load_vol<T>(m, e);
   const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
@@ -538,6 +561,13 @@ __device__ __forceinline__ void tet_block1(const MeshView& m, const double* __re
     set_fibre(P, fib);
   }
   hess_block<MODEL>(P, pa, pm, blk[0]);
+}
+
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_block1(const MeshView& m, const double* __restrict__ q,
+                                           const T* __restrict__ params, int e, int a, int mm, bool stretch_ref,
+                                           T blk[1][9]) {
+  tet_block1e<MODEL>(m, q, params, e, load_E(m, e), a, mm, stretch_ref, blk);
 }
 
 template <int MODEL, int VPW, typename T>
@@ -676,28 +706,41 @@ __global__ void __launch_bounds__(kGatherThreads)
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
     const IncList L = inc_list(m, v);
-    for (int k = 0; k < L.len; ++k) {
-      const int64_t idx = L.first + 64ll * k;
-      const int code = m.incE[idx];
-      if (code < 0) break;
-      const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incR[idx];
+    // pipelined walk: the code of step k + 2 and the vertex ids of step k + 1 are loaded during step k,
+    // so a step waits on one level of loads (its q, geometry and parameters) instead of three (code,
+    // then vertex ids, then q); lists are valid up to their first negative code
+    int cA = L.len > 0 ? m.incE[L.first] : -1;
+    uint32_t rA = cA >= 0 ? m.incR[L.first] : 0u;
+    int4 eA = cA >= 0 ? load_E(m, cA >> 2) : make_int4(0, 0, 0, 0);
+    int cB = (cA >= 0 && L.len > 1) ? m.incE[L.first + 64] : -1;
+    uint32_t rB = cB >= 0 ? m.incR[L.first + 64] : 0u;
+    for (int k = 0; cA >= 0; ++k) {
+      const int64_t iC = L.first + 64ll * (k + 2);
+      const int cC = (cB >= 0 && k + 2 < L.len) ? m.incE[iC] : -1;
+      const uint32_t rC = cC >= 0 ? m.incR[iC] : 0u;
+      const int4 eB = cB >= 0 ? load_E(m, cB >> 2) : make_int4(0, 0, 0, 0);
+      const int e = cA >> 2, a = cA & 3;
       T blk[1][9];
       if constexpr (MODEL == M_NH_MUSCLE) {
-        tet_block1<M_NH_MUSCLE>(m, q, params, e, a, c, stretchRef, blk);
+        tet_block1e<M_NH_MUSCLE>(m, q, params, e, eA, a, c, stretchRef, blk);
       } else switch (model_at<MODEL>(mids, e)) {
-        case M_NH: tet_block1<M_NH>(m, q, params, e, a, c, stretchRef, blk); break;
-        case M_STVK: tet_block1<M_STVK>(m, q, params, e, a, c, stretchRef, blk); break;
-        case M_COROT: tet_block1<M_COROT>(m, q, params, e, a, c, stretchRef, blk); break;
-        case M_ARAP: tet_block1<M_ARAP>(m, q, params, e, a, c, stretchRef, blk); break;
-        case M_MASS: tet_block1<M_MASS>(m, q, params, e, a, c, stretchRef, blk); break;
-        default: tet_block1<M_NH_LEGACY>(m, q, params, e, a, c, stretchRef, blk); break;
+        case M_NH: tet_block1e<M_NH>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        case M_STVK: tet_block1e<M_STVK>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        case M_COROT: tet_block1e<M_COROT>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        case M_ARAP: tet_block1e<M_ARAP>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        case M_MASS: tet_block1e<M_MASS>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        default: tet_block1e<M_NH_LEGACY>(m, q, params, e, eA, a, c, stretchRef, blk); break;
       }
-      T* dst = row + 3 * ((rk >> (8 * c)) & 0xffu);
+      T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
 #pragma unroll
       for (int l = 0; l < 3; ++l)
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[0][3 * l + kk];
+      cA = cB;
+      rA = rB;
+      eA = eB;
+      cB = cC;
+      rB = rC;
     }
   }
   __syncthreads();
@@ -884,9 +927,65 @@ __global__ void __launch_bounds__(256) k_tet_q(MeshView m, T* __restrict__ Qout)
 // diagonal relies on it).  LPV lanes per vertex: lane c owns
 // corners c, c + LPV, ...; as in the element gathers every LDS slot receives its contributions in
 // ascending element order.  No element math and no q, model or parameter reads here.
+//
+// The walk over a vertex's incident tets is pipelined: the incidence code of step k + 2 is loaded
+// during step k, and one value from each cache line of step k + 1's record and Q is loaded too, so those
+// lines are on their way to L2 / L1 while step k is computed (full register buffers for the next
+// record would exceed the register budget of two waves per SIMD).  The touched values are folded into
+// a value that is stored only when the host passes touchOut != 0 (it never does), which keeps the
+// loads alive at the cost of one add each.
+template <int NB, int LPV, typename T>
+__device__ __forceinline__ void rec_load(const T* __restrict__ tq, const T* __restrict__ rec, int code, int c, T S[45],
+                                         T ql[NB][3], T qh[NB][3]) {
+  const int e = code >> 2, a = code & 3;
+  const T* Sp = rec + 45ll * e;
+#pragma unroll
+  for (int i = 0; i < 45; ++i) S[i] = Sp[i];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int mm = c + LPV * j;
+    const int lo = min(a, mm), hi = max(a, mm);
+    // the two rows of Q by address (no per-lane selection among four loaded rows)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      ql[j][l] = tq[12ll * e + 3 * lo + l];
+      qh[j][l] = tq[12ll * e + 3 * hi + l];
+    }
+  }
+}
+
+// one value from each 128-byte line of tet e's record (45 values) and Q (12 values)
+template <typename T>
+__device__ __forceinline__ T rec_touch(const T* __restrict__ tq, const T* __restrict__ rec, int e) {
+  constexpr int per = 128 / sizeof(T);
+  const T* Sp = rec + 45ll * e;
+  const T* Qp = tq + 12ll * e;
+  T t = Sp[0] + Sp[44] + Qp[0] + Qp[11];
+#pragma unroll
+  for (int i = per; i < 44; i += per) t += Sp[i];
+  return t;
+}
+
+template <int NB, int LPV, typename T>
+__device__ __forceinline__ void rec_accum(int code, uint32_t rk, int c, int nb, T* row, const T S[45],
+                                          const T ql[NB][3], const T qh[NB][3], T tolPos) {
+  const int a = code & 3;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int mm = c + LPV * j;
+    T blk[9];
+    rec_block(ql[j], qh[j], a, mm, S, tolPos, blk);
+    T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
+  }
+}
+
 template <int VPW, int LPV, typename T>
 __global__ void __launch_bounds__(kGatherThreads)
-    k_hessian_rec(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd,
+    k_hessian_rec(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd, int touchOut,
                   T* __restrict__ vals) {
   constexpr int NB = 4 / LPV;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -902,40 +1001,36 @@ __global__ void __launch_bounds__(kGatherThreads)
   __syncthreads();
   const int vl = (int)threadIdx.x / LPV, c = (int)threadIdx.x % LPV;
   const int v = v0 + vl;
+  T sink = T(0);
   if (vl < VPW && v < m.nV) {
     const int nb = m.bptr[v + 1] - m.bptr[v];
     T* row = acc + 9 * (m.bptr[v] - b0);
     const IncList L = inc_list(m, v);
-    for (int k = 0; k < L.len; ++k) {
-      const int64_t idx = L.first + 64ll * k;
-      const int code = m.incE[idx];
-      if (code < 0) break;
-      const int e = code >> 2, a = code & 3;
-      const uint32_t rk = m.incR[idx];
-      const T* S = rec + 45ll * e;
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        const int mm = c + LPV * j;
-        const int lo = min(a, mm), hi = max(a, mm);
-        // the two rows of Q by address (no per-lane selection among four loaded rows)
-        T ql[3], qh[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          ql[l] = tq[12ll * e + 3 * lo + l];
-          qh[l] = tq[12ll * e + 3 * hi + l];
-        }
-        T blk[9];
-        rec_block(ql, qh, a, mm, S, tolPos, blk);
-        T* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-#pragma unroll
-          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
-      }
+    // lists are valid up to their first negative code (padding)
+    int cA = L.len > 0 ? m.incE[L.first] : -1;
+    uint32_t rA = cA >= 0 ? m.incR[L.first] : 0u;
+    int cB = (cA >= 0 && L.len > 1) ? m.incE[L.first + 64] : -1;
+    uint32_t rB = cB >= 0 ? m.incR[L.first + 64] : 0u;
+    T touched = T(0);
+    for (int k = 0; cA >= 0; ++k) {
+      T S[45], ql[NB][3], qh[NB][3];
+      rec_load<NB, LPV>(tq, rec, cA, c, S, ql, qh);
+      sink += touched;
+      const int64_t iC = L.first + 64ll * (k + 2);
+      const int cC = (cB >= 0 && k + 2 < L.len) ? m.incE[iC] : -1;
+      const uint32_t rC = cC >= 0 ? m.incR[iC] : 0u;
+      touched = cB >= 0 ? rec_touch(tq, rec, cB >> 2) : T(0);
+      rec_accum<NB, LPV>(cA, rA, c, nb, row, S, ql, qh, tolPos);
+      cA = cB;
+      rA = rB;
+      cB = cC;
+      rB = rC;
     }
+    sink += touched;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+  if (touchOut && sink == T(0)) vals[seg0] = sink;
 }
 
 // ---- mass / external force / setup / spmv -----------------------------------------------------------
@@ -1109,7 +1204,7 @@ int launch_hessian_rec(const bh_mesh* m, const T* tq, const T* rec, T tolPos, T*
     const size_t lds = gather_lds<VPW, T>(m);
     ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
     hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
-                       m->xcdGroups, vals);
+                       m->xcdGroups, 0, vals);
     BH_CHECK(hipGetLastError());
     return BH_OK;
   });
