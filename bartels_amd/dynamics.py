@@ -3,7 +3,12 @@ matlab/examples/test_tet_materials.m:97-112, solved in position form): every ste
 bh_newton_static_device call with the inertia term 1/(2 dt^2) (q - q_pred)^T M (q - q_pred),
 q_pred = q_t + dt v_t, then v_{t+1} = (q_{t+1} - q_t) / dt.
 
-    python -m bartels_amd.dynamics --steps 5 --variant fp32_sgs
+The falling clamped beam with per-tet Young's modulus log-uniform in [1e5, 1e7] Pa (meshgen.config5):
+every variant solves each Newton direction by PCG to a 1e-6 relative residual with the Jacobi
+fallback on; --judge steps the FP64 judge (FP64 elements, block IC(0) with the coarse correction,
+PCG to 1e-12) in lockstep and reports each step's displacement difference against it.
+
+    python -m bartels_amd.dynamics --steps 50 --variant fp32_learned_2l --judge
 """
 import argparse
 import ctypes
@@ -46,7 +51,7 @@ class BackwardEuler:
         self.v = torch.zeros_like(self.x)
 
     def step(self, dt, precond=0, precision=0, mgn=None, gtol=1e-6, max_it=10, pcg_tol=1e-6, pcg_max=20000,
-             psd=True, coarse=0):
+             psd=True, coarse=0, fallback=1):
         import torch
 
         lib = self.lib
@@ -62,6 +67,7 @@ class BackwardEuler:
         o.pcg_max_iterations = pcg_max
         o.precond = precond
         o.coarse_correction = coarse
+        o.jacobi_fallback = fallback
         o.precond_handle = self.P
         o.element_precision = precision
         if mgn is not None:
@@ -87,39 +93,60 @@ class BackwardEuler:
             self.P = ctypes.c_void_p()
 
 
-VARIANTS = {"fp64_jacobi": (0, 0), "fp32_jacobi": (0, 1), "fp32_bic0": (2, 1), "fp32_sgs": (5, 1),
-            "fp32_learned_sgs": (6, 1), "fp32_learned_factor": (4, 1), "fp64_jacobi2l": (7, 0),
-            "fp32_jacobi2l": (7, 1), "fp32_bic0_2l": (2, 1, 1)}
+# name: (preconditioner, element precision, coarse correction, network form or None, shipped weights)
+VARIANTS = {"fp64_jacobi": (0, 0, 0, None, False), "fp32_jacobi": (0, 1, 0, None, False),
+            "fp32_bic0": (2, 1, 0, None, False), "fp32_sgs": (5, 1, 0, None, False),
+            "fp64_jacobi2l": (7, 0, 0, None, False), "fp32_jacobi2l": (7, 1, 0, None, False),
+            "fp32_bic0_2l": (2, 1, 1, None, False), "fp32_learned_2l": (4, 1, 1, "tied", True),
+            "fp32_learned_untrained_2l": (4, 1, 1, "tied", False), "fp32_learned_sgs": (6, 1, 0, "sgs", False)}
+# the FP64 judge: FP64 elements, block IC(0) with the coarse correction, PCG to 1e-12
+JUDGE = (2, 0, 1, 1e-12)
 
 
-def run(steps, variant, grid=(256, 64, 64), weights=None, log=print):
+def run(steps, variant, grid=(256, 64, 64), weights=None, judge=False, log=print):
+    """`steps` backward-Euler steps of the configuration-5 beam with one solver variant (and, with judge,
+    the FP64 judge in lockstep); returns the final positions, per-step rows and the variant's wall time"""
     import torch
+
+    from . import learned
 
     p = meshgen.config5(grid=grid)
     be = BackwardEuler(p)
-    precond, precision, *coarse = VARIANTS[variant]
+    bj = BackwardEuler(p) if judge else None
+    precond, precision, coarse, form, shipped = VARIANTS[variant]
     net = None
-    if precond in (4, 6):
-        from . import learned
-
-        w = weights or learned.DEFAULT_WEIGHTS
-        net = learned.Network(weights=w if os.path.exists(w) else None, form="sgs" if precond == 6 else "factor")
+    if form is not None:
+        w = weights or (learned.DEFAULT_WEIGHTS if shipped else None)
+        net = learned.Network(weights=w, form=form)
+    X = torch.tensor(p.V.ravel(), device="cuda")
     rows = []
-    t0 = time.perf_counter()
+    wall = 0.0
     for k in range(steps):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        rep = be.step(p.dt, precond=precond, precision=precision, mgn=net.h if net else None,
-                      coarse=coarse[0] if coarse else 0)
+        rep = be.step(p.dt, precond=precond, precision=precision, mgn=net.h if net else None, coarse=coarse)
         torch.cuda.synchronize()
-        rows.append(dict(step=k, newton=rep.iterations, pcg=rep.pcg_iterations_total, fallbacks=rep.pcg_fallbacks,
-                         s=time.perf_counter() - t, tip_z=float(be.x[-1].item() - p.V.ravel()[-1])))
+        dt_s = time.perf_counter() - t
+        wall += dt_s
+        row = dict(step=k, newton=rep.iterations, pcg=rep.pcg_iterations_total, fallbacks=rep.pcg_fallbacks,
+                   pcg_failures=rep.pcg_failures, converged=bool(rep.converged), s=dt_s,
+                   tip_z=float(be.x[-1].item() - p.V.ravel()[-1]))
+        if bj is not None:
+            t = time.perf_counter()
+            rj = bj.step(p.dt, precond=JUDGE[0], precision=JUDGE[1], coarse=JUDGE[2], pcg_tol=JUDGE[3])
+            torch.cuda.synchronize()
+            u = float(torch.linalg.norm(bj.x - X))
+            row.update(judge_s=time.perf_counter() - t, judge_newton=rj.iterations, judge_pcg=rj.pcg_iterations_total,
+                       rel_displacement_vs_fp64_judge=float(torch.linalg.norm(be.x - bj.x)) / u if u > 0 else 0.0)
+        rows.append(row)
         log(json.dumps(rows[-1]))
     x = be.x.cpu().numpy()
     be.close()
+    if bj is not None:
+        bj.close()
     if net:
         net.close()
-    return x, rows, time.perf_counter() - t0
+    return x, rows, wall
 
 
 def main():
@@ -128,10 +155,19 @@ def main():
     ap.add_argument("--variant", default="fp64_jacobi", choices=sorted(VARIANTS))
     ap.add_argument("--grid", default="256,64,64")
     ap.add_argument("--save", default="")
+    ap.add_argument("--judge", action="store_true", help="step the FP64 judge in lockstep and compare")
     a = ap.parse_args()
     grid = tuple(int(v) for v in a.grid.split(","))
-    x, rows, wall = run(a.steps, a.variant, grid=grid, log=lambda s: print(s, flush=True))
-    print(json.dumps(dict(variant=a.variant, steps=a.steps, wall_s=wall, s_per_step=wall / max(1, a.steps))))
+    x, rows, wall = run(a.steps, a.variant, grid=grid, judge=a.judge, log=lambda s: print(s, flush=True))
+    summary = dict(variant=a.variant, grid=grid, steps=a.steps, wall_s=wall, s_per_step=wall / max(1, a.steps),
+                   pcg_total=sum(r["pcg"] for r in rows), fallbacks=sum(r["fallbacks"] for r in rows),
+                   pcg_failures=sum(r["pcg_failures"] for r in rows),
+                   all_converged=all(r["converged"] for r in rows))
+    if a.judge:
+        summary.update(judge_s_per_step=sum(r["judge_s"] for r in rows) / max(1, a.steps),
+                       max_rel_displacement_vs_fp64_judge=max(r["rel_displacement_vs_fp64_judge"] for r in rows),
+                       final_rel_displacement_vs_fp64_judge=rows[-1]["rel_displacement_vs_fp64_judge"] if rows else 0)
+    print(json.dumps(summary))
     if a.save:
         np.save(a.save, x)
 

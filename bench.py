@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix), AMD published
 
 
 def algorithmic_bytes(nV, nT, nnz):
@@ -382,6 +383,11 @@ def main():
     gather_ms = kt["hessian_gather"]
     achieved = B["dq2"] * nT / (gather_ms * 1e-3) / 1e9
     traffic = load_traffic("hessian_gather_psd")
+    achieved_call = B["dq2"] * nT / (ms_step * 1e-3) / 1e9
+    # the PSD pass is bound by FP64 VALU issue: its FP64 operations per tet from rocprofv3 counters
+    # (64 lanes per issued wave instruction, FMA counted as 2), divided by its live duration
+    psd_flops = load_traffic("psd_pass_fp64_flops_per_tet")
+    psd_ms = kt.get("psd")
     out = {
         "metric": "Mtets/s for neohookean_dq2 assembly",
         "value": nT / (ms_step * 1e-3) / 1e6,
@@ -409,6 +415,18 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes_per_tet": B["dq2"],
+            "call_achieved": achieved_call,
+            "call_frac": achieved_call / HBM_PEAK_GBS,
+            "call_note": "the same algorithmic bytes over the whole call (PSD pass + gather)",
+        },
+        "roofline_psd_pass": None if not (psd_flops and psd_ms) else {
+            "kernel": "k_psd_fused (reduced 9x9 M, eigenvalues, clamp, records)",
+            "bound": "fp64 valu",
+            "flops_per_tet": psd_flops,
+            "achieved": psd_flops * nT / (psd_ms * 1e-3) / 1e12,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": psd_flops * nT / (psd_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         },
         "kernels_ms": {k: v for k, v in kt.items() if v is not None},
         "extra": {
