@@ -42,6 +42,8 @@ class BhTuning(ctypes.Structure):
 # void (*on_system)(void* user, int iteration, const float* vals32, const double* vals64, const double* b,
 #                   const double* x)
 ON_SYSTEM = ctypes.CFUNCTYPE(None, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p)
+# bh_element_callback: (element_matrix double[144], element, user)
+ELEMENT_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), c_int, c_void_p)
 
 
 class BhNewtonOpts(ctypes.Structure):
@@ -117,6 +119,8 @@ SIGNATURES = {
     "bh_linear_tetmesh_q": (c_int, [_VP, c_int, _VP, _VP, P(c_double)]),
     "bh_linear_tetmesh_dq": (c_int, [_VP, c_int, _VP, _VP, _VP]),
     "bh_linear_tetmesh_dq2": (c_int, [_VP, c_int, _VP, _VP, P(BhPsd), _VP]),
+    "bh_linear_tetmesh_dq2_callback": (c_int, [_VP, c_int, _VP, _VP, _VP, ELEMENT_CALLBACK, _VP, _VP]),
+    "bh_linear_tetmesh_dq2_callback_device": (c_int, [_VP, c_int, _VP, _VP, _VP, ELEMENT_CALLBACK, _VP, _VP, _VP]),
     "bh_linear_tetmesh_generic_elastic_q": (c_int, [_VP, _VP, _VP, _VP, P(c_double)]),
     "bh_linear_tetmesh_generic_elastic_dq": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_linear_tetmesh_generic_elastic_dq2": (c_int, [_VP, _VP, _VP, _VP, P(BhPsd), _VP]),

@@ -1033,6 +1033,112 @@ __global__ void __launch_bounds__(kGatherThreads)
   if (touchOut && sink == T(0)) vals[seg0] = sink;
 }
 
+// ---- element callback path (the reference's ElementMatrixCallback on the Hessian) -------------------
+// k_elem_hessian: one lane per tet of [e0, e0 + n), the tet's 12x12 element Hessian vol B^T H B written
+// column-major with DOF 3 a + c (corner a, component c): the matrix sim::assemble hands to the
+// reference's callback before it emits the tet's triplets (src/linear_tetmesh_neohookean_dq2.cpp:17-23,
+// src/assemble.cpp:57-70).  Its 16 blocks are the blocks the element-math gathers fold.
+template <int MODEL>
+__device__ __forceinline__ void elem_hessian12(const MeshView& m, const double* __restrict__ q,
+                                               const double* __restrict__ params, int e, bool stretchRef,
+                                               double* __restrict__ out) {
+  double phi[4][3];
+  Pre<double> P;
+  load_phi(m, e, phi);
+  {
+    double Gh[9];
+    load_FG(m, q, load_E(m, e), phi, P.F, Gh);
+  }
+  P.vol = load_vol<double>(m, e);
+  const double p0 = params[e], p1 = params[(int64_t)m.nT + e];
+  pre_hessian<MODEL>(P, p0, p1, stretchRef);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    double fib[4];
+    load_fibre(params, m.nT, e, fib);
+    set_fibre(P, fib);
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int mm = 0; mm < 4; ++mm) {
+      double pa[3], pm[3], blk[9];
+      select_row(phi, a, pa);
+      select_row(phi, mm, pm);
+      hess_block<MODEL>(P, pa, pm, blk);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[(3 * mm + k) * 12 + 3 * a + l] = blk[3 * l + k];
+    }
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(64)
+    k_elem_hessian(MeshView m, const double* __restrict__ q, const double* __restrict__ params,
+                   const uint8_t* __restrict__ mids, int e0, int n, int stretchRef, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = e0 + i;
+  double* o = out + 144ll * i;
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    elem_hessian12<M_NH_MUSCLE>(m, q, params, e, stretchRef != 0, o);
+  } else switch (model_at<MODEL>(mids, e)) {
+    case M_NH: elem_hessian12<M_NH>(m, q, params, e, stretchRef != 0, o); break;
+    case M_STVK: elem_hessian12<M_STVK>(m, q, params, e, stretchRef != 0, o); break;
+    case M_COROT: elem_hessian12<M_COROT>(m, q, params, e, stretchRef != 0, o); break;
+    case M_ARAP: elem_hessian12<M_ARAP>(m, q, params, e, stretchRef != 0, o); break;
+    default: elem_hessian12<M_NH_LEGACY>(m, q, params, e, stretchRef != 0, o); break;
+  }
+}
+
+// Assembly of the (callback-modified) element matrices of tets [e0, e1): the vertex gather of the
+// other assembly kernels, reading block (a, m) of tet e from its staged 12x12; the workgroup's CSR
+// segment starts from the values of the earlier chunks (accumulate = 1), so every slot still folds its
+// contributions in ascending element order across chunks.
+template <int VPW, int LPV>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_staged(MeshView m, const double* __restrict__ stage, int e0, int e1, int accumulate, int xcd,
+                     double* __restrict__ vals) {
+  constexpr int NB = 4 / LPV;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* acc = reinterpret_cast<double*>(smem);
+  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
+  if (grp * VPW >= m.nV) return;
+  const int v0 = grp * VPW;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = accumulate ? vals[seg0 + i] : 0.0;
+  __syncthreads();
+  const int vl = (int)threadIdx.x / LPV, c = (int)threadIdx.x % LPV;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    double* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    for (int k = 0; k < L.len; ++k) {
+      const int64_t idx = L.first + 64ll * k;
+      const int code = m.incE[idx];
+      if (code < 0) break;
+      const int e = code >> 2, a = code & 3;
+      if (e < e0) continue;  // the list is in ascending element order
+      if (e >= e1) break;
+      const uint32_t rk = m.incR[idx];
+      const double* S = stage + 144ll * (e - e0);
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int mm = c + LPV * j;
+        double* dst = row + 3 * ((rk >> (8 * mm)) & 0xffu);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += S[(3 * mm + kk) * 12 + 3 * a + l];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
 // ---- mass / external force / setup / spmv -----------------------------------------------------------
 __global__ void __launch_bounds__(256) k_ext_force(MeshView m, double t0, double t1, double t2, double* __restrict__ f) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1279,6 +1385,68 @@ int hessian_dispatch(bh_mesh* m, int model, const uint8_t* mids, const double* q
     case M_NH_MUSCLE: return hessian_model<M_NH_MUSCLE>(m, q, params, mids, psd, vals, s);
     default: return BH_BAD_ARG;
   }
+}
+
+// The Hessian with a host element callback, in chunks of at most kCallbackChunk tets: the chunk's element
+// matrices on the device, copied to pinned host memory, handed to the callback one tet at a time in
+// ascending element order, copied back and folded into the CSR values by the staged gather.
+constexpr int kCallbackChunk = 1 << 18;  // 302 MB of staged 12x12 matrices
+
+struct CallbackStage {
+  double* d = nullptr;
+  double* h = nullptr;
+  ~CallbackStage() {
+    if (d) (void)hipFree(d);
+    if (h) (void)hipHostFree(h);
+  }
+};
+
+int hessian_callback(bh_mesh* m, int model, const uint8_t* mids, const double* q, const double* params,
+                     bh_element_callback cb, void* user, double* vals, hipStream_t s) {
+  if (m->nV == 0) return BH_OK;
+  int st = ensure_geometry<double>(m, s);
+  if (st) return st;
+  const int chunk = std::max(1, std::min(m->nT, kCallbackChunk));
+  CallbackStage buf;
+  BH_CHECK(hipMalloc(&buf.d, sizeof(double) * 144 * (size_t)chunk));
+  BH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&buf.h), sizeof(double) * 144 * (size_t)chunk, hipHostMallocDefault));
+  const int eff = mids ? (int)M_PER_TET : model;
+  for (int e0 = 0; e0 < m->nT || e0 == 0; e0 += chunk) {
+    const int n = std::min(chunk, m->nT - e0);
+    if (n > 0) {
+      const dim3 grid((n + 63) / 64), blk(64);
+      const int sr = m->opts.stretch_reference_layout;
+      switch (eff) {
+        case M_PER_TET: hipLaunchKernelGGL(k_elem_hessian<M_PER_TET>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_NH: hipLaunchKernelGGL(k_elem_hessian<M_NH>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_STVK: hipLaunchKernelGGL(k_elem_hessian<M_STVK>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_COROT: hipLaunchKernelGGL(k_elem_hessian<M_COROT>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_ARAP: hipLaunchKernelGGL(k_elem_hessian<M_ARAP>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_NH_LEGACY: hipLaunchKernelGGL(k_elem_hessian<M_NH_LEGACY>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        case M_NH_MUSCLE: hipLaunchKernelGGL(k_elem_hessian<M_NH_MUSCLE>, grid, blk, 0, s, view(m), q, params, mids, e0, n, sr, buf.d); break;
+        default: return BH_BAD_ARG;
+      }
+      BH_CHECK(hipGetLastError());
+      BH_CHECK(hipMemcpyAsync(buf.h, buf.d, sizeof(double) * 144 * (size_t)n, hipMemcpyDeviceToHost, s));
+      BH_CHECK(hipStreamSynchronize(s));
+      for (int i = 0; i < n; ++i) cb(buf.h + 144ll * i, e0 + i, user);
+      BH_CHECK(hipMemcpyAsync(buf.d, buf.h, sizeof(double) * 144 * (size_t)n, hipMemcpyHostToDevice, s));
+    }
+    st = pick_gather_shape<double>(m, [&](auto vpw, auto lpv) {
+      constexpr int VPW = decltype(vpw)::value, LPV = decltype(lpv)::value;
+      auto kern = k_hessian_staged<VPW, LPV>;
+      gather_lds_attr(kern);
+      const size_t lds = gather_lds<VPW, double>(m);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m),
+                         (const double*)buf.d, e0, e0 + std::max(n, 0), e0 > 0 ? 1 : 0, m->xcdGroups, vals);
+      BH_CHECK(hipGetLastError());
+      return BH_OK;
+    });
+    if (st) return st;
+    if (m->nT == 0) break;
+  }
+  BH_CHECK(hipStreamSynchronize(s));  // the staging buffers are freed on return
+  return BH_OK;
 }
 
 }  // namespace
@@ -1783,6 +1951,32 @@ int bh_linear_tetmesh_dq2(bh_mesh* m, int model, const double* q, const double* 
   st = stage_inputs(m, q, params, nullptr, s, model);
   if (st) return st;
   st = hessian_dispatch(m, model, nullptr, m->stage_q, m->stage_params, psd, m->stage_vals, s);
+  if (st) return st;
+  BH_CHECK(hipMemcpyAsync(vals, m->stage_vals, sizeof(double) * m->nnz, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tetmesh_dq2_callback_device(bh_mesh* m, int model, const uint8_t* d_model_ids, const double* d_q,
+                                          const double* d_params, bh_element_callback cb, void* user, double* d_vals,
+                                          bh_stream_t s) {
+  if (!m || !d_q || !d_params || !d_vals || !cb || check_model_args(model, d_model_ids)) return BH_BAD_ARG;
+  return hessian_callback(m, model, d_model_ids, d_q, d_params, cb, user, d_vals, as_stream(s));
+}
+
+int bh_linear_tetmesh_dq2_callback(bh_mesh* m, int model, const uint8_t* model_ids, const double* q,
+                                   const double* params, bh_element_callback cb, void* user, double* vals) {
+  if (!m || !q || !params || !vals || !cb || check_model_args(model, model_ids)) return BH_BAD_ARG;
+  if (model_ids)
+    for (int e = 0; e < m->nT; ++e)
+      if (!is_model(model_ids[e]) || model_ids[e] == M_NH_MUSCLE) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  int st = ensure_stage_vals(m);
+  if (st) return st;
+  st = stage_inputs(m, q, params, model_ids, s, model_ids ? M_NH : model);
+  if (st) return st;
+  st = hessian_callback(m, model, model_ids ? m->stage_ids : nullptr, m->stage_q, m->stage_params, cb, user,
+                        m->stage_vals, s);
   if (st) return st;
   BH_CHECK(hipMemcpyAsync(vals, m->stage_vals, sizeof(double) * m->nnz, hipMemcpyDeviceToHost, s));
   BH_CHECK(hipStreamSynchronize(s));

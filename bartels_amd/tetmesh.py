@@ -43,7 +43,18 @@ def _psd_struct(callback):
         return None
     if isinstance(callback, simple_psd_fix):
         return _lib.BhPsd(1, callback.tol)
-    raise NotImplementedError("only simple_psd_fix(tol) element callbacks run on the GPU path")
+    raise TypeError("the device entry points take simple_psd_fix(tol); other element callbacks go through "
+                    "TetMesh.dq2 / the linear_tetmesh_*_dq2 functions (host callback path)")
+
+
+def _element_callback(callback):
+    """a Python callable f(H) that changes the 12x12 element matrix H (a numpy view, column-major as the
+    reference's Eigen::Matrix12d) in place, wrapped as a bh_element_callback"""
+    def trampoline(ptr, element, user):
+        H = np.ctypeslib.as_array(ptr, shape=(144,)).reshape(12, 12, order="F")
+        callback(H)
+
+    return _lib.ELEMENT_CALLBACK(trampoline)
 
 
 def linear_tetmesh_dphi_dX(V, E):
@@ -167,6 +178,15 @@ class TetMesh:
         q = np.ascontiguousarray(q, dtype=np.float64)
         params = _colmajor(params, np.float64)
         vals = np.zeros(self.nnz)
+        if callback is not None and not isinstance(callback, simple_psd_fix):
+            # an arbitrary element callback: element matrices made on the device, changed on the host
+            cb = _element_callback(callback)
+            ids = np.ascontiguousarray(model_ids, dtype=np.uint8) if model_ids is not None else None
+            check(self._lib.bh_linear_tetmesh_dq2_callback(self._h, MODEL_PER_TET if ids is not None else model,
+                                                           _ptr(ids) if ids is not None else None, _ptr(q),
+                                                           _ptr(params), cb, None, _ptr(vals)),
+                  "bh_linear_tetmesh_dq2_callback")
+            return vals
         psd = _psd_struct(callback)
         pp = ctypes.byref(psd) if psd is not None else None
         if model_ids is not None:
@@ -184,6 +204,9 @@ class TetMesh:
         rowptr, colind = self.pattern()
         vals = self.dq2_values(model, q, params, callback, model_ids)
         n = 3 * self.nV
+        if callback is not None and not isinstance(callback, simple_psd_fix):
+            # a host callback may leave the element matrices unsymmetric: convert the CSR values exactly
+            return sp.csr_matrix((vals, colind, rowptr), shape=(n, n)).tocsc()
         # the pattern is symmetric: CSR arrays read as CSC give the transpose of the values, i.e.
         # H itself up to the rounding asymmetry of the element blocks
         return sp.csc_matrix((vals, colind, rowptr), shape=(n, n))

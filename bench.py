@@ -219,6 +219,66 @@ def newton_steps(p, mesh, steps, pcg_max, solve_tol=1e-6, solve_max_it=40, solve
     return dict(per_step=res, to_solution=dict(rel_gradient_tol=solve_tol, results=sol))
 
 
+def config3_leg(stream):
+    """Configuration 3 at full size: the jittered 48x48x192 beam (2.5M tets) with 70 / 20 / 10 %
+    Neo-Hookean / StVK / corotational tets and per-tet materials, compressed to J in [0.2, 1.5].
+    Times the FP64 and FP32 (precision form) assembly with per-tet models, with and without
+    simple_psd_fix(1e-3), and reports the FP32 outputs' relative error against the FP64 ones."""
+    import torch
+
+    from bartels_amd import meshgen
+    from bartels_amd import tetmesh as tm
+
+    p = meshgen.config3()
+    dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
+    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol, V=p.V)
+    dev = torch.device("cuda:0")
+    q = torch.tensor(p.q, device=dev)
+    params = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device=dev)
+    par32 = params.to(torch.float32)
+    ids = torch.tensor(p.model_ids, device=dev)
+    nnz, nV, nT = mesh.nnz, mesh.nV, mesh.nT
+    psd = tm.simple_psd_fix(1e-3)
+    M = tm.MODEL_PER_TET
+    H = torch.empty(nnz, dtype=torch.float64, device=dev)
+    H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
+    g, g32 = (torch.empty(3 * nV, dtype=torch.float64, device=dev) for _ in range(2))
+    e, e32 = (torch.empty(1, dtype=torch.float64, device=dev) for _ in range(2))
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a.double() - b) / torch.linalg.norm(b))
+
+    out = dict(workload=f"config 3: jittered 48x48x192 beam, mixed NH / StVK / corotational, J in [0.2, 1.5]",
+               tets=nT, nnz=nnz)
+    for tag, cb in (("psd", psd), ("nopsd", None)):
+        out[f"dq2_{tag}_fp64_ms"] = timed(lambda: mesh.dq2_device(M, q, params, H, callback=cb, model_ids=ids,
+                                                                  stream=stream))
+        out[f"dq2_{tag}_fp32_ms"] = timed(lambda: mesh.dq2_f32_device(M, q, par32, H32, callback=cb, model_ids=ids,
+                                                                      stream=stream))
+        out[f"dq2_{tag}_fp32_rel_error"] = rel(H32, H)
+    out["dq_fp64_ms"] = timed(lambda: mesh.dq_device(M, q, params, g, model_ids=ids, stream=stream))
+    out["dq_fp32_ms"] = timed(lambda: mesh.dq_f32_device(M, q, par32, g32, model_ids=ids, stream=stream))
+    out["dq_fp32_rel_error"] = rel(g32, g)
+    out["q_fp64_ms"] = timed(lambda: mesh.q_device(M, q, params, e, model_ids=ids, stream=stream))
+    out["q_fp32_ms"] = timed(lambda: mesh.q_f32_device(M, q, par32, e32, model_ids=ids, stream=stream))
+    out["q_fp32_rel_error"] = rel(e32, e)
+    out["dq2_psd_fp64_mtets_s"] = nT / (out["dq2_psd_fp64_ms"] * 1e-3) / 1e6
+    out["dq2_psd_fp32_mtets_s"] = nT / (out["dq2_psd_fp32_ms"] * 1e-3) / 1e6
+    mesh.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +289,7 @@ def main():
     ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1, 2 or 4 lanes per vertex "
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-config3", action="store_true", help="skip the 48x48x192 mixed-model assembly leg")
     ap.add_argument("--newton-steps", type=int, default=2,
                     help="Newton steps per solver variant in the Newton-step comparison (0: skip)")
     ap.add_argument("--pcg-max", type=int, default=20000)
@@ -445,6 +506,8 @@ def main():
             "setup_s": setup_s,
         },
     }
+    if not args.no_config3:
+        out["extra"]["config3"] = config3_leg(stream)
     if args.newton_steps > 0:
         del H, g
         nres = newton_steps(p, mesh, args.newton_steps, args.pcg_max)

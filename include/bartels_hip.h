@@ -177,6 +177,26 @@ int bh_linear_tetmesh_dq(bh_mesh* mesh, int model, const double* q, const double
 int bh_linear_tetmesh_dq2(bh_mesh* mesh, int model, const double* q, const double* params, const bh_psd* psd,
                           double* vals);
 
+/* An arbitrary element-matrix callback, the reference's ElementMatrixCallback on the Hessian
+ * (include/EigenTypes.h:133 default_callback; called in src/linear_tetmesh_neohookean_dq2.cpp:17-23 and
+ * its siblings): a host function called once per tet, in ascending element order, with the tet's
+ * 12x12 element Hessian vol B^T d2psi B (column-major, row / column 3 a + c for corner a, component c,
+ * the order sim::assemble maps to 3 E(e, a) + c, src/assemble.cpp:57-70), which it may change in place;
+ * the changed matrices are assembled.  element is the tet's index. */
+typedef void (*bh_element_callback)(double* element_matrix, int element, void* user);
+/* Replaces sim::linear_tetmesh_*_dq2 with a callback other than simple_psd_fix (bh_psd covers that one
+ * on the device).  The element matrices are made on the device in chunks of up to 262,144 tets, copied
+ * to pinned host memory, handed to cb, copied back and folded by the vertex gather, every CSR slot in
+ * ascending element order across chunks: with a callback that changes nothing the values equal
+ * bh_linear_tetmesh_dq2 without PSD to rounding (the same closed-form element blocks).  model_ids NULL: model for every tet (0..5);
+ * otherwise per-tet ids 0..4.  Synchronous; the callback runs on the calling thread. */
+int bh_linear_tetmesh_dq2_callback(bh_mesh* mesh, int model, const uint8_t* model_ids, const double* q,
+                                   const double* params, bh_element_callback cb, void* user, double* vals);
+/* The same with device inputs and output; returns after the last chunk is assembled. */
+int bh_linear_tetmesh_dq2_callback_device(bh_mesh* mesh, int model, const uint8_t* d_model_ids, const double* d_q,
+                                          const double* d_params, bh_element_callback cb, void* user,
+                                          double* d_vals, bh_stream_t stream);
+
 /* Replaces sim::linear_tetmesh_generic_elastic_{q,dq,dq2} (src/linear_tetmesh_generic_elastic_q.cpp:5-30,
  * _dq.cpp:5-36, _dq2.cpp:8-38) for the built-in models: the user functor becomes a per-tet model id
  * (uint8, bh_model values 0..4), params stay nT x 2 per tet. */
