@@ -36,7 +36,8 @@ class BhTuning(ctypes.Structure):
     """schedule knobs of a mesh: gather lanes and XCD grouping (same per-slot fold order), and the vertex
     colouring of the colour sweeps (0 keep, 1 DSatur, 2 greedy) and the lanes per vertex of the Gauss-Seidel
     sweeps (0 keep, 4, 8)"""
-    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int), ("coloring", c_int), ("sweep_lanes", c_int)]
+    _fields_ = [("gather_lanes", c_int), ("xcd_groups", c_int), ("coloring", c_int), ("sweep_lanes", c_int),
+                ("gather_order", c_int)]
 
 
 # void (*on_system)(void* user, int iteration, const float* vals32, const double* vals64, const double* b,

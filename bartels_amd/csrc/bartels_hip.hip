@@ -73,6 +73,11 @@ struct bh_mesh {
   int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
   int gatherLanes = 4;                            // lanes per vertex in the assembly gather (1, 2 or 4)
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
+  // visiting order of the gather workgroups (Morton order of the vertex groups' rest centroids; made when
+  // rest positions are set): ord16 over groups of 16 vertices, ord64 over groups of 64
+  int32_t* ord16 = nullptr;
+  int32_t* ord64 = nullptr;
+  int useOrder = 0;                               // visit the groups in that order (bh_tuning.gather_order)
   int coloring = 1;                               // colour sweeps: 1 DSatur, 2 greedy in vertex order
   int sweepLanes = 4;                             // lanes per vertex in the Gauss-Seidel sweeps (4 or 8)
   bh_precond* solveOrder = nullptr;               // colour lists for the precision form's block Jacobi
@@ -90,9 +95,9 @@ struct bh_mesh {
   // PSD
   uint8_t* psdFlag = nullptr;
   int32_t* psdCount = nullptr;
-  double* psdDM = nullptr;  // nT x 45: the tets' records M' (packed reduced 9x9)
+  double* psdDM = nullptr;  // nT x 48: the tets' records M' (packed reduced 9x9, quad layout)
   float* psdDM32 = nullptr;
-  double* tetQ = nullptr;   // nT x 12: Q of Phi = Q R, tet-contiguous (made on first use)
+  double* tetQ = nullptr;   // nT x 16: Q of Phi = Q R, tet-contiguous, quad layout (made on first use)
   double* tetForces = nullptr;  // nT x 12: the gradient's per-tet corner forces, corner-major (first use)
   float* tetQ32 = nullptr;
   // solver workspace (lazily allocated)
@@ -153,11 +158,14 @@ struct MeshView {
   const int32_t* __restrict__ bptr;
   const int32_t* __restrict__ bcol;
   const double* __restrict__ X;  // rest positions (3 nV, interleaved) or null
+  const int32_t* __restrict__ ord16;  // gather visiting orders (or null: vertex order)
+  const int32_t* __restrict__ ord64;
 };
 
 MeshView view(const bh_mesh* m) {
+  const bool ord = m->useOrder != 0;
   return MeshView{m->nV, m->nT, m->E, m->dX, m->dX32, m->vol, m->vol32, m->gBase, m->gLen, m->incE, m->incR,
-                  m->bptr, m->bcol, m->X};
+                  m->bptr, m->bcol, m->X, ord ? m->ord16 : nullptr, ord ? m->ord64 : nullptr};
 }
 
 // the interleaved incidence list of vertex v: entries at first + 64 k, k < len (code < 0 ends it)
@@ -443,14 +451,32 @@ __global__ void __launch_bounds__(256)
 // form is 9 nb doubles (1080 B at valence 14), so LDS caps one-lane workgroups of 64 vertices at
 // two per CU: occupancy and latency, not HBM, bound this kernel.
 
-// Workgroup -> vertex group.  Workgroups are dealt round-robin to the 8 XCDs (each with its own
-// L2); with xcd set, XCD x takes the contiguous run of groups [x per, (x+1) per), so the groups
-// that share tets (neighbouring ids) meet in the same L2.  The grid is then 8 per workgroups and
-// the surplus ones return at once.
-__device__ __forceinline__ int gather_group(int xcd, int ngroups) {
-  if (!xcd) return (int)blockIdx.x;
-  const int per = (ngroups + 7) >> 3;
-  return (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+// Workgroup -> vertex group.  The hardware deals workgroups round-robin to the 8 XCDs, each with its
+// own L2.  With xcd set, XCD x takes the contiguous run [x per, (x+1) per) of the visiting order, so
+// the groups that share tets meet in the same L2 (the grid is then 8 per workgroups; the surplus ones
+// return at once).  The visiting order is vertex order, or, when the mesh has one and it is switched on
+// (bh_tuning.gather_order), the Morton order of the groups' rest centroids: neighbouring groups then run
+// at nearly the same time on the same XCD.  On the 64x64x256 beam that cut the record gather's HBM
+// reads from 7.1 to 4.2 GB per call and left its time unchanged (the gathers are bound by cache-line
+// lookups in L1, not by HBM), so it is off by default.  Groups of 32 and 64 vertices follow the order
+// of the 64-vertex groups, split into consecutive sub-groups.
+__host__ __device__ __forceinline__ int gather_units(int nV, int vpw, bool ordered) {
+  return (!ordered || vpw == 16) ? (nV + vpw - 1) / vpw : ((nV + 63) / 64) * (64 / vpw);
+}
+
+// first vertex of this workgroup's group, or -1 for a surplus workgroup
+template <int VPW>
+__device__ __forceinline__ int gather_first(const MeshView& m, int xcd) {
+  const int32_t* ord = (VPW == 16) ? m.ord16 : m.ord64;
+  const int n = gather_units(m.nV, VPW, ord != nullptr);
+  int idx = (int)blockIdx.x;
+  if (xcd) idx = (int)(blockIdx.x & 7u) * ((n + 7) >> 3) + (int)(blockIdx.x >> 3);
+  if (idx >= n) return -1;
+  if (!ord) return idx * VPW;
+  if (VPW == 16) return ord[idx] * 16;
+  constexpr int SUB = 64 / VPW;
+  const int v0 = ord[idx / SUB] * 64 + (idx % SUB) * VPW;
+  return v0 < m.nV ? v0 : -1;
 }
 
 template <int MODEL, typename T>
@@ -576,9 +602,8 @@ __global__ void __launch_bounds__(kGatherThreads)
               const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
-  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
-  if (grp * VPW >= m.nV) return;
-  const int v0 = grp * VPW;
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
@@ -632,9 +657,8 @@ __global__ void __launch_bounds__(kGatherThreads)
                    const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
-  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
-  if (grp * VPW >= m.nV) return;
-  const int v0 = grp * VPW;
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
@@ -691,9 +715,8 @@ __global__ void __launch_bounds__(kGatherThreads)
                    const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
-  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
-  if (grp * VPW >= m.nV) return;
-  const int v0 = grp * VPW;
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
@@ -813,21 +836,56 @@ constexpr int kPsdPending = 1;
 // coalesced block.
 constexpr int kRecPitch = 65;  // LDS pitch of one record entry over the 64 lanes (odd: no bank conflicts)
 
+// Record layout in HBM.  A tet's record (M' packed, 45 values) takes kRecW = 48 elements and its Q
+// (4 x 3) kQW = 16, both tet-contiguous.  The four lanes of a vertex in the gather load a record
+// together: lane j takes the 16-byte chunks j, j + 4, j + 8, ..., so each load instruction of the quad
+// touches a single cache line (the gather is bound by cache-line lookups, not bytes), and the lanes
+// then hand the values to each other over DPP.  Value v of M' sits in lane v & 3 as its slot v >> 2;
+// row r of Q sits in lane r (slots 0..2), so the lane of corner c holds Q[c] itself.  Slot s of lane j
+// is element s % W of the lane's chunk s / W (W elements per 16-byte chunk).
+constexpr int kRecW = 48;
+constexpr int kQW = 16;
+template <typename T>
+__host__ __device__ constexpr int chunk_off(int j, int s) {
+  constexpr int W = 16 / (int)sizeof(T);
+  return 4 * W * (s / W) + W * j + s % W;
+}
+template <typename T>
+__host__ __device__ constexpr int rec_off(int v) {
+  return chunk_off<T>(v & 3, v >> 2);
+}
+template <typename T>
+__host__ __device__ constexpr int q_off(int r, int l) {
+  return chunk_off<T>(r, l);
+}
+// value index of record element o (or -1 for padding)
+template <typename T>
+__host__ __device__ constexpr int rec_value(int o) {
+  constexpr int W = 16 / (int)sizeof(T);
+  const int t = o / (4 * W), j = (o % (4 * W)) / W, w = o % W;
+  const int v = 4 * (W * t + w) + j;
+  return v < 45 ? v : -1;
+}
+
 template <int MODEL, typename T>
 __global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
                 int32_t* __restrict__ count, T* __restrict__ rec) {
-  __shared__ T stash[45 * kRecPitch];
+  // the records in LDS in their HBM element order (entry o of the lane's tet at o * kRecPitch + lane)
+  __shared__ T stash[kRecW * kRecPitch];
   const int lane = (int)threadIdx.x;
   const int e0 = (int)blockIdx.x * 64;
   const int e = e0 + lane;
   bool clamped = false;
+#pragma unroll
+  for (int o = 0; o < kRecW; ++o)
+    if (rec_value<T>(o) < 0) stash[o * kRecPitch + lane] = T(0);
   if (e < m.nT) {
     T M[45];
     reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
 #pragma unroll
-    for (int i = 0; i < 45; ++i) stash[i * kRecPitch + lane] = M[i];
+    for (int i = 0; i < 45; ++i) stash[rec_off<T>(i) * kRecPitch + lane] = M[i];
     int fl = 0;
     if constexpr (sizeof(T) == 8) {
       // the QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the
@@ -836,7 +894,7 @@ __global__ void __launch_bounds__(64, 2)
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
-          for (int j = i; j < 9; ++j) stash[sym9(i, j) * kRecPitch + lane] += w[i] * w[j];
+          for (int j = i; j < 9; ++j) stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] += w[i] * w[j];
       });
       clamped = k > 0;
       if (k > 3) fl = kPsdPending;
@@ -851,7 +909,7 @@ __global__ void __launch_bounds__(64, 2)
       for (int i = 0; i < 9; ++i)
 #pragma unroll
         for (int j = i; j < 9; ++j) {
-          T acc = stash[sym9(i, j) * kRecPitch + lane];
+          T acc = stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane];
           if (k > 3) {
             acc += dm[sym9(i, j)];
           } else {
@@ -859,18 +917,26 @@ __global__ void __launch_bounds__(64, 2)
             for (int c = 0; c < 3; ++c)
               if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
           }
-          stash[sym9(i, j) * kRecPitch + lane] = acc;
+          stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] = acc;
         }
     }
     flag[e] = (uint8_t)fl;
   }
   count_flagged(clamped, count);
   __syncthreads();
-  const int n = 45 * min(64, m.nT - e0);
-  T* out = rec + 45ll * e0;
+  // element idx = kRecW t + o of the block: (t, o) advanced by 64 = kRecW + 16 per step, no division
+  const int n = kRecW * min(64, m.nT - e0);
+  T* out = rec + (int64_t)kRecW * e0;
+  static_assert(kRecW == 48, "write-out stride assumes 64 = kRecW + 16");
+  int t = lane >= kRecW ? 1 : 0, o = lane - kRecW * t;
   for (int idx = lane; idx < n; idx += 64) {
-    const int t = idx / 45;
-    out[idx] = stash[(idx - 45 * t) * kRecPitch + t];
+    out[idx] = stash[o * kRecPitch + t];
+    o += 16;
+    t += 1;
+    if (o >= kRecW) {
+      o -= kRecW;
+      t += 1;
+    }
   }
 }
 
@@ -885,16 +951,16 @@ __global__ void __launch_bounds__(64)
   if (e >= m.nT || flag[e] != kPsdPending) return;
   T M[45], dm[45];
   reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
-  T* r = rec + 45ll * e;
+  T* r = rec + (int64_t)kRecW * e;
 #pragma unroll
-  for (int i = 0; i < 45; ++i) r[i] = M[i];
+  for (int i = 0; i < 45; ++i) r[rec_off<T>(i)] = M[i];
   const int k = psd_correction(M, tol, dm);
   __asm__ volatile("" ::: "memory");
 #pragma unroll
   for (int i = 0; i < 9; ++i)
 #pragma unroll
     for (int j = i; j < 9; ++j) {
-      T acc = r[sym9(i, j)];
+      T acc = r[rec_off<T>(sym9(i, j))];
       if (k > 3) {
         acc += dm[sym9(i, j)];
       } else {
@@ -902,12 +968,12 @@ __global__ void __launch_bounds__(64)
         for (int c = 0; c < 3; ++c)
           if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
       }
-      r[sym9(i, j)] = acc;
+      r[rec_off<T>(sym9(i, j))] = acc;
     }
   flag[e] = 0;
 }
 
-// Q of each tet's Phi = Q R (12 values, corner-major), stored tet-contiguous for the gather
+// Q of each tet's Phi = Q R (kQW elements per tet in the quad layout: row c in lane c's slots)
 template <typename T>
 __global__ void __launch_bounds__(256) k_tet_q(MeshView m, T* __restrict__ Qout) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -915,10 +981,15 @@ __global__ void __launch_bounds__(256) k_tet_q(MeshView m, T* __restrict__ Qout)
   T phi[4][3], Q[4][3], R[3][3];
   load_phi(m, e, phi);
   phi_qr(phi, Q, R);
+  T out[kQW];
+#pragma unroll
+  for (int i = 0; i < kQW; ++i) out[i] = T(0);
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int l = 0; l < 3; ++l) Qout[12ll * e + 3 * c + l] = Q[c][l];
+    for (int l = 0; l < 3; ++l) out[q_off<T>(c, l)] = Q[c][l];
+#pragma unroll
+  for (int i = 0; i < kQW; ++i) Qout[(int64_t)kQW * e + i] = out[i];
 }
 
 // Assembly gather from the PSD records: the (a, m) block of tet e is tolPos I3 + sum_{p,r}
@@ -938,9 +1009,9 @@ template <int NB, int LPV, typename T>
 __device__ __forceinline__ void rec_load(const T* __restrict__ tq, const T* __restrict__ rec, int code, int c, T S[45],
                                          T ql[NB][3], T qh[NB][3]) {
   const int e = code >> 2, a = code & 3;
-  const T* Sp = rec + 45ll * e;
+  const T* Sp = rec + (int64_t)kRecW * e;
 #pragma unroll
-  for (int i = 0; i < 45; ++i) S[i] = Sp[i];
+  for (int i = 0; i < 45; ++i) S[i] = Sp[rec_off<T>(i)];
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     const int mm = c + LPV * j;
@@ -948,21 +1019,21 @@ __device__ __forceinline__ void rec_load(const T* __restrict__ tq, const T* __re
     // the two rows of Q by address (no per-lane selection among four loaded rows)
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      ql[j][l] = tq[12ll * e + 3 * lo + l];
-      qh[j][l] = tq[12ll * e + 3 * hi + l];
+      ql[j][l] = tq[(int64_t)kQW * e + q_off<T>(lo, l)];
+      qh[j][l] = tq[(int64_t)kQW * e + q_off<T>(hi, l)];
     }
   }
 }
 
-// one value from each 128-byte line of tet e's record (45 values) and Q (12 values)
+// one value from each 128-byte line of tet e's record (kRecW elements) and Q (kQW elements)
 template <typename T>
 __device__ __forceinline__ T rec_touch(const T* __restrict__ tq, const T* __restrict__ rec, int e) {
   constexpr int per = 128 / sizeof(T);
-  const T* Sp = rec + 45ll * e;
-  const T* Qp = tq + 12ll * e;
-  T t = Sp[0] + Sp[44] + Qp[0] + Qp[11];
+  const T* Sp = rec + (int64_t)kRecW * e;
+  const T* Qp = tq + (int64_t)kQW * e;
+  T t = Sp[0] + Sp[kRecW - 1] + Qp[0] + Qp[kQW - 1];
 #pragma unroll
-  for (int i = per; i < 44; i += per) t += Sp[i];
+  for (int i = per; i < kRecW - 1; i += per) t += Sp[i];
   return t;
 }
 
@@ -990,9 +1061,8 @@ __global__ void __launch_bounds__(kGatherThreads)
   constexpr int NB = 4 / LPV;
   extern __shared__ __align__(16) unsigned char smem[];
   T* acc = reinterpret_cast<T*>(smem);
-  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
-  if (grp * VPW >= m.nV) return;
-  const int v0 = grp * VPW;
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
@@ -1031,6 +1101,126 @@ __global__ void __launch_bounds__(kGatherThreads)
   __syncthreads();
   for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
   if (touchOut && sink == T(0)) vals[seg0] = sink;
+}
+
+// value of lane K of this lane's quad (DPP quad_perm broadcast; all four lanes of the quad active)
+template <int K>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), K * 0x55, 0xf, 0xf, true));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, K * 0x55, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), K * 0x55, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+// the 16-byte chunks lane j of a quad loads of a tet's record (NR) and Q (NQ)
+template <typename T>
+struct QuadChunks {
+  static constexpr int W = 16 / (int)sizeof(T);
+  static constexpr int NR = kRecW / (4 * W);
+  static constexpr int NQ = kQW / (4 * W);
+  T r[NR * W];
+  T q[NQ * W];
+};
+
+template <typename T>
+__device__ __forceinline__ void quad_load(const T* __restrict__ rec, const T* __restrict__ tq, int e, int j,
+                                          QuadChunks<T>& c) {
+  using C = QuadChunks<T>;
+  using V = std::conditional_t<sizeof(T) == 8, double2, float4>;
+  const V* rp = reinterpret_cast<const V*>(rec + (int64_t)kRecW * e) + j;
+  const V* qp = reinterpret_cast<const V*>(tq + (int64_t)kQW * e) + j;
+#pragma unroll
+  for (int t = 0; t < C::NR; ++t) *reinterpret_cast<V*>(&c.r[C::W * t]) = rp[4 * t];
+#pragma unroll
+  for (int t = 0; t < C::NQ; ++t) *reinterpret_cast<V*>(&c.q[C::W * t]) = qp[4 * t];
+}
+
+// M' (packed, 45 values) in every lane of the quad from the lanes' slots: value v from lane v & 3
+template <typename T, int V = 0>
+__device__ __forceinline__ void quad_expand(const T* r, T S[45]) {
+  if constexpr (V < 45) {
+    S[V] = quad_bcast<V & 3>(r[V >> 2]);
+    quad_expand<T, V + 1>(r, S);
+  }
+}
+
+// Assembly gather from the PSD records with four lanes per vertex (16 vertices per workgroup): lane c
+// of a vertex's quad owns corner c's block, as in k_hessian_rec, but the quad loads each tet's record
+// and Q once, cooperatively (one cache line per load instruction and quad), and passes the values
+// around over DPP; Q[a] (a: the vertex's corner, the same in all four lanes) comes from lane a.  The next
+// tet's chunks are loaded while the current one is folded; incidence codes run two steps ahead.  Same
+// blocks, bit for bit, and the same ascending-element fold per slot as k_hessian_rec.
+template <typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_recq(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd,
+                   T* __restrict__ vals) {
+  constexpr int VPW = 16;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 2, c = (int)threadIdx.x & 3;
+  const int v = v0 + vl;
+  if (v < m.nV) {  // the same for the four lanes of a quad, as is every branch below
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    int cA = L.len > 0 ? m.incE[L.first] : -1;
+    uint32_t rA = cA >= 0 ? m.incR[L.first] : 0u;
+    int cB = (cA >= 0 && L.len > 1) ? m.incE[L.first + 64] : -1;
+    uint32_t rB = cB >= 0 ? m.incR[L.first + 64] : 0u;
+    QuadChunks<T> cur;
+    if (cA >= 0) quad_load(rec, tq, cA >> 2, c, cur);
+    for (int k = 0; cA >= 0; ++k) {
+      QuadChunks<T> nxt;
+      if (cB >= 0) quad_load(rec, tq, cB >> 2, c, nxt);
+      const int64_t iC = L.first + 64ll * (k + 2);
+      const int cC = (cB >= 0 && k + 2 < L.len) ? m.incE[iC] : -1;
+      const uint32_t rC = cC >= 0 ? m.incR[iC] : 0u;
+      T S[45];
+      quad_expand<T>(cur.r, S);
+      const int a = cA & 3;
+      // Q[a] from lane a of the quad (a is uniform within the quad), Q[c] is this lane's own
+      const int src = ((int)threadIdx.x & ~3) | a;
+      T qa[3], qc[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        qc[l] = cur.q[l];
+        qa[l] = __shfl(cur.q[l], src, 64);
+      }
+      const bool low = c < a;
+      T ql[3], qh[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        ql[l] = low ? qc[l] : qa[l];
+        qh[l] = low ? qa[l] : qc[l];
+      }
+      T blk[9];
+      rec_block(ql, qh, a, c, S, tolPos, blk);
+      T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
+      cur = nxt;
+      cA = cB;
+      rA = rB;
+      cB = cC;
+      rB = rC;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
 }
 
 // ---- element callback path (the reference's ElementMatrixCallback on the Hessian) -------------------
@@ -1100,9 +1290,8 @@ __global__ void __launch_bounds__(kGatherThreads)
   constexpr int NB = 4 / LPV;
   extern __shared__ __align__(16) unsigned char smem[];
   double* acc = reinterpret_cast<double*>(smem);
-  const int grp = gather_group(xcd, (m.nV + VPW - 1) / VPW);
-  if (grp * VPW >= m.nV) return;
-  const int v0 = grp * VPW;
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
   const int vend = min(v0 + VPW, m.nV);
   const int b0 = m.bptr[v0];
   const int64_t seg0 = 9ll * b0;
@@ -1251,7 +1440,8 @@ void gather_lds_attr(K kern) {
 }
 
 int gather_grid(const bh_mesh* m, int vpw) {
-  const int groups = (m->nV + vpw - 1) / vpw;
+  const bool ordered = m->useOrder && ((vpw == 16) ? m->ord16 : m->ord64);
+  const int groups = gather_units(m->nV, vpw, ordered);
   return m->xcdGroups ? 8 * ((groups + 7) / 8) : groups;
 }
 
@@ -1305,12 +1495,19 @@ template <typename T>
 int launch_hessian_rec(const bh_mesh* m, const T* tq, const T* rec, T tolPos, T* vals, hipStream_t s) {
   return pick_gather_shape<T>(m, [&](auto vpw, auto lpv) {
     constexpr int VPW = decltype(vpw)::value, LPV = decltype(lpv)::value;
-    auto kern = k_hessian_rec<VPW, LPV, T>;
-    gather_lds_attr(kern);
     const size_t lds = gather_lds<VPW, T>(m);
     ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
-    hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
-                       m->xcdGroups, 0, vals);
+    if constexpr (LPV == 4) {
+      auto kern = k_hessian_recq<T>;
+      gather_lds_attr(kern);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
+                         m->xcdGroups, vals);
+    } else {
+      auto kern = k_hessian_rec<VPW, LPV, T>;
+      gather_lds_attr(kern);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
+                         m->xcdGroups, 0, vals);
+    }
     BH_CHECK(hipGetLastError());
     return BH_OK;
   });
@@ -1321,7 +1518,7 @@ template <typename T>
 int ensure_tet_q(bh_mesh* m, const T** tq, hipStream_t s) {
   void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->tetQ) : reinterpret_cast<void**>(&m->tetQ32);
   if (!*buf) {
-    BH_CHECK(hipMalloc(buf, sizeof(T) * 12 * std::max<size_t>(1, m->nT)));
+    BH_CHECK(hipMalloc(buf, sizeof(T) * kQW * std::max<size_t>(1, m->nT)));
     if (m->nT) {
       hipLaunchKernelGGL((k_tet_q<T>), dim3((m->nT + 255) / 256), dim3(256), 0, s, view(m), reinterpret_cast<T*>(*buf));
       BH_CHECK(hipGetLastError());
@@ -1341,7 +1538,7 @@ int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* 
     BH_CHECK(hipMalloc(&m->psdCount, sizeof(int32_t)));
   }
   void** buf = (sizeof(T) == 8) ? reinterpret_cast<void**>(&m->psdDM) : reinterpret_cast<void**>(&m->psdDM32);
-  if (!*buf) BH_CHECK(hipMalloc(buf, sizeof(T) * 45 * std::max<size_t>(1, m->nT)));
+  if (!*buf) BH_CHECK(hipMalloc(buf, sizeof(T) * kRecW * std::max<size_t>(1, m->nT)));
   *rec = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
   ProfScope ps(m, BH_KT_PSD, s);
@@ -1708,12 +1905,70 @@ int finish_mesh(bh_mesh* m) {
   return noalloc_scratch(&scr);
 }
 
+// spreads the low 10 bits of x to every third bit (Morton interleave)
+uint32_t spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// The gathers' visiting orders from rest positions Xi (3 nV, interleaved, host): for groups of 16 and of
+// 64 consecutive vertices, the group ids sorted by the Morton code of the group's centroid (10 bits per
+// axis over the centroids' bounding cube; ties by group id).  The order changes which workgroup runs
+// when, never what a workgroup computes, so every result is the same to the last bit.
+int build_gather_order(bh_mesh* m, const double* Xi) {
+  for (int gs : {16, 64}) {
+    int32_t** dst = (gs == 16) ? &m->ord16 : &m->ord64;
+    const int ng = (m->nV + gs - 1) / gs;
+    if (ng == 0) continue;
+    std::vector<double> c(3 * (size_t)ng, 0.0);
+    for (int g = 0; g < ng; ++g) {
+      const int v0 = g * gs, v1 = std::min(m->nV, v0 + gs);
+      for (int v = v0; v < v1; ++v)
+        for (int k = 0; k < 3; ++k) c[3 * (size_t)g + k] += Xi[3 * (size_t)v + k];
+      for (int k = 0; k < 3; ++k) c[3 * (size_t)g + k] /= (double)(v1 - v0);
+    }
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = c[k];
+    for (int g = 0; g < ng; ++g)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], c[3 * (size_t)g + k]);
+        hi[k] = std::max(hi[k], c[3 * (size_t)g + k]);
+      }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+    if (!std::isfinite(span)) return BH_NONFINITE;
+    const double sc = span > 0.0 ? 1023.999 / span : 0.0;
+    std::vector<std::pair<uint32_t, int32_t>> key(ng);
+    for (int g = 0; g < ng; ++g) {
+      uint32_t code = 0;
+      for (int k = 0; k < 3; ++k)
+        code |= spread3((uint32_t)((c[3 * (size_t)g + k] - lo[k]) * sc)) << k;
+      key[g] = {code, g};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> ord(ng);
+    for (int g = 0; g < ng; ++g) ord[g] = key[g].second;
+    if (!*dst) BH_CHECK(hipMalloc(dst, sizeof(int32_t) * ng));
+    BH_CHECK(hipMemcpy(*dst, ord.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice));
+  }
+  return BH_OK;
+}
+
+void drop_gather_order(bh_mesh* m) {
+  if (m->ord16) (void)hipFree(m->ord16);
+  if (m->ord64) (void)hipFree(m->ord64);
+  m->ord16 = m->ord64 = nullptr;
+}
+
 void free_mesh(bh_mesh* m) {
   clear_profile(m);
   void* ptrs[] = {m->E, m->dX, m->vol, m->gBase, m->gLen, m->incE, m->incR, m->bptr, m->bcol, m->vdeg, m->partials,
                   m->d_energy, m->stage_q, m->stage_params, m->stage_out, m->stage_ids, m->stage_vals,
                   m->psdFlag, m->psdCount, m->psdDM, m->X, m->ws, m->dX32, m->vol32, m->psdDM32, m->tetQ, m->tetQ32,
-                  m->cws, m->tetForces};
+                  m->cws, m->tetForces, m->ord16, m->ord64};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (m->own) (void)hipStreamDestroy(m->own);
@@ -2062,6 +2317,7 @@ int bh_mesh_set_rest_positions(bh_mesh* m, const double* V) {
   if (!V) {
     if (m->X) BH_CHECK(hipFree(m->X));
     m->X = nullptr;
+    drop_gather_order(m);
     return BH_OK;
   }
   std::vector<double> h(3 * (size_t)m->nV);
@@ -2069,7 +2325,7 @@ int bh_mesh_set_rest_positions(bh_mesh* m, const double* V) {
     for (int c = 0; c < 3; ++c) h[3 * (size_t)v + c] = V[v + (size_t)c * m->nV];
   if (!m->X) BH_CHECK(hipMalloc(&m->X, sizeof(double) * std::max<size_t>(3, h.size())));
   if (!h.empty()) BH_CHECK(hipMemcpy(m->X, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-  return BH_OK;
+  return build_gather_order(m, h.data());
 }
 
 int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
@@ -2084,6 +2340,8 @@ int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
   if (t->sweep_lanes != 0 && t->sweep_lanes != 4 && t->sweep_lanes != 8) return BH_BAD_ARG;
+  if (t->gather_order < 0 || t->gather_order > 2) return BH_BAD_ARG;
+  if (t->gather_order) m->useOrder = t->gather_order == 1;
   if (t->sweep_lanes) m->sweepLanes = t->sweep_lanes;
   if (t->coloring && t->coloring != m->coloring) {
     m->coloring = t->coloring;

