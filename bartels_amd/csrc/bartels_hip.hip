@@ -930,7 +930,7 @@ __global__ void __launch_bounds__(64, 2)
   static_assert(kRecW == 48, "write-out stride assumes 64 = kRecW + 16");
   int t = lane >= kRecW ? 1 : 0, o = lane - kRecW * t;
   for (int idx = lane; idx < n; idx += 64) {
-    out[idx] = stash[o * kRecPitch + t];
+    __builtin_nontemporal_store(stash[o * kRecPitch + t], &out[idx]);  // streamed: read back by the gather
     o += 16;
     t += 1;
     if (o >= kRecW) {

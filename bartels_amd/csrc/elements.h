@@ -805,15 +805,16 @@ __host__ __device__ __forceinline__ int sturm_below(const T d[9], const T e[9], 
 }
 
 // The eigenvalues of the tridiagonal (d, e) below x0, by implicit QL that hunts them first: each
-// deflation at the top starts with two zero-shift sweeps (the top then converges toward the
+// deflation at the top starts with three zero-shift sweeps (the top then converges toward the
 // eigenvalue nearest zero -- below a small x0 these are the few smallest in magnitude, well separated
 // from the stiff rest) and continues with the Wilkinson shift of tridiag_ql; it stops as soon as the
 // deflated values below x0 number what the Sturm count of T - x0 I says.  Returns ldone: d[0 .. ldone)
 // are eigenvalues, and every eigenvalue below x0 is among them; with ldone = 9 all of d are.  The
 // remaining block's diagonal entries d[ldone ..] are not eigenvalues (its eigenvalues are all >= x0).
 // Compared with the full QL on the 64x64x256 beam's reduced matrices (two eigenvalues below 1e-3 in
-// every tet): 51 rotations and 9 sweeps per matrix instead of 103 and 28, same eigenvalues to
-// 1e-15 of |M|.
+// every tet): 51 rotations and 9 sweeps per matrix instead of 103 and 28, the same count in every
+// tet of a wave (two zero-shift sweeps: 51 on average but up to 55 in a wave), and the same
+// eigenvalues to 1e-15 of |M|.
 template <typename T>
 __host__ __device__ __forceinline__ int tridiag_ql_below(T d[9], T e[9], T x0) {
   constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
@@ -835,7 +836,7 @@ __host__ __device__ __forceinline__ int tridiag_ql_below(T d[9], T e[9], T x0) {
 #pragma unroll
       for (int j = l + 1; j < 8; ++j) dm = (j == m) ? d[j] : dm;
       T g = dm;  // zero shift
-      if (iter >= 2) {
+      if (iter >= 3) {
         const T gg = (d[l + 1] - d[l]) / (T(2) * e[l]);
         const T r0 = sqrt(gg * gg + T(1));
         g = dm - d[l] + e[l] / (gg + (gg >= T(0) ? r0 : -r0));
@@ -1061,13 +1062,17 @@ __host__ __device__ __forceinline__ void deflate9(T x[9], const T y[9]) {
 // Clamp correction of a reduced matrix M (packed upper triangle, destroyed) when at most three
 // eigenvalues lie below tol, without forming the eigenvector matrix: Householder tridiagonalisation
 // (reflectors kept), the eigenvalues below tol by tridiag_ql_below on a copy of (d, e) (no rotations
-// applied to vectors), then for each eigenvalue mu_j < tol three steps of inverse iteration on the tridiagonal
+// applied to vectors), then for each eigenvalue mu_j < tol kInvIt steps of inverse iteration on the tridiagonal
 // form from a fixed start vector, each followed by Gram-Schmidt against the vectors already found
 // (members of a cluster then span its invariant subspace; the clamp only needs that subspace, and
 // the error any rotation within a cluster leaves in dM is bounded by the cluster's width), and the
 // result mapped back through the reflectors.  Hands each column w_j = sqrt(tol - mu_j) v_j to
 // emit(j, w_j) as soon as it is formed and returns k; for k > 3 it emits nothing (the caller runs
 // psd_correction on the matrix again).
+// Inverse-iteration steps per vector.  On the 64x64x256 beam's reduced matrices the clamp is within
+// 8e-16 of |M| of the full eigensolver's after two steps (1.9e-15 after one); three change nothing.
+constexpr int kInvIt = 2;
+
 template <typename T, typename Emit>
 __host__ __device__ inline int psd_correction_low(T a[45], T tol, Emit&& emit) {
   constexpr T kEps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07);
@@ -1121,11 +1126,11 @@ __host__ __device__ inline int psd_correction_low(T a[45], T tol, Emit&& emit) {
 #pragma unroll
       for (int i = 0; i < 9; ++i) x[i] = start[9 * j + i];
 #pragma unroll
-      for (int it = 0; it < 3; ++it) {
+      for (int it = 0; it < kInvIt; ++it) {
         tridiag_shift_solve(d, e, mu, pmin, x);
 #pragma unroll
         for (int q = 0; q < j; ++q) deflate9(x, y[q]);
-        if (it == 2) {
+        if (it == kInvIt - 1) {
 #pragma unroll
           for (int q = 0; q < j; ++q) deflate9(x, y[q]);
         }
