@@ -77,7 +77,7 @@ struct bh_mesh {
   // rest positions are set): ord16 over groups of 16 vertices, ord64 over groups of 64
   int32_t* ord16 = nullptr;
   int32_t* ord64 = nullptr;
-  int useOrder = 0;                               // visit the groups in that order (bh_tuning.gather_order)
+  int useOrder = 2;  // visit the groups in that order: 0 never, 1 in every gather, 2 in the PSD-record gathers
   int coloring = 1;                               // colour sweeps: 1 DSatur, 2 greedy in vertex order
   int sweepLanes = 4;                             // lanes per vertex in the Gauss-Seidel sweeps (4 or 8)
   bh_precond* solveOrder = nullptr;               // colour lists for the precision form's block Jacobi
@@ -162,8 +162,9 @@ struct MeshView {
   const int32_t* __restrict__ ord64;
 };
 
-MeshView view(const bh_mesh* m) {
-  const bool ord = m->useOrder != 0;
+// ordered: with the gathers' visiting order (when the mesh has one)
+MeshView view(const bh_mesh* m, bool ordered = false) {
+  const bool ord = ordered && m->useOrder != 0;
   return MeshView{m->nV, m->nT, m->E, m->dX, m->dX32, m->vol, m->vol32, m->gBase, m->gLen, m->incE, m->incR,
                   m->bptr, m->bcol, m->X, ord ? m->ord16 : nullptr, ord ? m->ord64 : nullptr};
 }
@@ -1439,10 +1440,15 @@ void gather_lds_attr(K kern) {
   });
 }
 
-int gather_grid(const bh_mesh* m, int vpw) {
-  const bool ordered = m->useOrder && ((vpw == 16) ? m->ord16 : m->ord64);
-  const int groups = gather_units(m->nV, vpw, ordered);
-  return m->xcdGroups ? 8 * ((groups + 7) / 8) : groups;
+// whether a gather of kind rec (PSD-record gather) or not visits the groups in the mesh's Morton order
+bool gather_ordered(const bh_mesh* m, int vpw, bool rec) {
+  return (m->useOrder == 1 || (rec && m->useOrder == 2)) && ((vpw == 16) ? m->ord16 : m->ord64);
+}
+// XCD runs: on with the Morton order (neighbouring groups then share an L2), else the mesh's setting
+int gather_xcd(const bh_mesh* m, int vpw, bool rec) { return gather_ordered(m, vpw, rec) ? 1 : m->xcdGroups; }
+int gather_grid(const bh_mesh* m, int vpw, bool rec = false) {
+  const int groups = gather_units(m->nV, vpw, gather_ordered(m, vpw, rec));
+  return gather_xcd(m, vpw, rec) ? 8 * ((groups + 7) / 8) : groups;
 }
 
 template <int VPW, typename T>
@@ -1459,8 +1465,8 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const
   gather_lds_attr(kern);
   const size_t lds = gather_lds<VPW, T>(m);
   ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
-  hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), q, params, mids,
-                     m->opts.stretch_reference_layout, m->xcdGroups, vals);
+  hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m, gather_ordered(m, VPW, false)),
+                     q, params, mids, m->opts.stretch_reference_layout, gather_xcd(m, VPW, false), vals);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -1500,13 +1506,13 @@ int launch_hessian_rec(const bh_mesh* m, const T* tq, const T* rec, T tolPos, T*
     if constexpr (LPV == 4) {
       auto kern = k_hessian_recq<T>;
       gather_lds_attr(kern);
-      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
-                         m->xcdGroups, vals);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW, true)), dim3(kGatherThreads), lds, s,
+                         view(m, gather_ordered(m, VPW, true)), tq, rec, tolPos, gather_xcd(m, VPW, true), vals);
     } else {
       auto kern = k_hessian_rec<VPW, LPV, T>;
       gather_lds_attr(kern);
-      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m), tq, rec, tolPos,
-                         m->xcdGroups, 0, vals);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW, true)), dim3(kGatherThreads), lds, s,
+                         view(m, gather_ordered(m, VPW, true)), tq, rec, tolPos, gather_xcd(m, VPW, true), 0, vals);
     }
     BH_CHECK(hipGetLastError());
     return BH_OK;
@@ -1634,8 +1640,9 @@ int hessian_callback(bh_mesh* m, int model, const uint8_t* mids, const double* q
       auto kern = k_hessian_staged<VPW, LPV>;
       gather_lds_attr(kern);
       const size_t lds = gather_lds<VPW, double>(m);
-      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s, view(m),
-                         (const double*)buf.d, e0, e0 + std::max(n, 0), e0 > 0 ? 1 : 0, m->xcdGroups, vals);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW)), dim3(kGatherThreads), lds, s,
+                         view(m, gather_ordered(m, VPW, false)), (const double*)buf.d, e0, e0 + std::max(n, 0),
+                         e0 > 0 ? 1 : 0, gather_xcd(m, VPW, false), vals);
       BH_CHECK(hipGetLastError());
       return BH_OK;
     });
@@ -2340,8 +2347,8 @@ int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
   if (t->sweep_lanes != 0 && t->sweep_lanes != 4 && t->sweep_lanes != 8) return BH_BAD_ARG;
-  if (t->gather_order < 0 || t->gather_order > 2) return BH_BAD_ARG;
-  if (t->gather_order) m->useOrder = t->gather_order == 1;
+  if (t->gather_order < 0 || t->gather_order > 3) return BH_BAD_ARG;
+  if (t->gather_order) m->useOrder = t->gather_order == 1 ? 1 : t->gather_order == 2 ? 0 : 2;
   if (t->sweep_lanes) m->sweepLanes = t->sweep_lanes;
   if (t->coloring && t->coloring != m->coloring) {
     m->coloring = t->coloring;

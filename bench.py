@@ -428,7 +428,7 @@ def main():
     par32 = params.to(torch.float32)
     H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
     for lanes in (1, 2, 4):
-        for xcd, order in ((0, 2), (1, 2), (1, 1)):
+        for xcd, order in ((0, 2), (1, 2), (0, 1)):
             tune(lanes, xcd, order)
             key = f"lanes{lanes}_xcd{xcd}_{'morton' if order == 1 else 'vertex'}"
             variants[key + "_psd"] = timed_kernels(step)
@@ -437,7 +437,7 @@ def main():
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
             variants[key + "_fp32_nopsd"] = timed_kernels(
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
-    tune(args.gather_lanes or 4, 0, 2)  # back to the library default (or the requested shape)
+    tune(args.gather_lanes or 4, 0, 3)  # back to the library default (or the requested shape)
     del H32
 
     B = algorithmic_bytes(nV, nT, nnz)

@@ -136,9 +136,9 @@ typedef struct bh_tuning {
                        bh_precond_create uses for the colour sweeps */
   int sweep_lanes;  /* 0 keep, 4 (default) or 8: lanes per vertex in the symmetric Gauss-Seidel sweeps of
                        preconditioners created afterwards */
-  int gather_order; /* 0 keep, 1 on, 2 off (default): visit the gathers' vertex groups in the Morton order
-                       of their rest centroids (made when rest positions are set) instead of vertex order;
-                       best with xcd_groups 1 */
+  int gather_order; /* 0 keep, 1 every gather, 2 none, 3 the PSD-record gathers only (default): visit the
+                       vertex groups in the Morton order of their rest centroids (made when rest positions
+                       are set), dealt to the XCDs in contiguous runs, instead of vertex order */
 } bh_tuning;
 int bh_mesh_set_tuning(bh_mesh* mesh, const bh_tuning* tuning);
 

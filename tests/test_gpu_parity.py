@@ -342,7 +342,7 @@ def test_gather_variants_agree(tm, fp32, psd):
     outs = {}
     for lanes in (1, 2, 4):
         for xcd in (0, 1):
-            for order in (1, 2):
+            for order in (1, 2, 3):
                 t = _lib.BhTuning(lanes, xcd, 0, 0, order)
                 assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(t)) == 0
                 H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
@@ -350,7 +350,7 @@ def test_gather_variants_agree(tm, fp32, psd):
                 fn(0, q, params, H, callback=tm.simple_psd_fix(1e-3) if psd else None)
                 torch.cuda.synchronize()
                 outs[(lanes, xcd, order)] = H.cpu().numpy().astype(np.float64)
-    assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(4, 1, 0, 0, 1))) == 0
+    assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(4, 0, 0, 0, 3))) == 0
     ref = outs[(1, 0, 2)]
     worst = 0.0
     for k, v in outs.items():
