@@ -1153,8 +1153,13 @@ __device__ __forceinline__ void quad_expand(const T* r, T S[45]) {
 // of a vertex's quad owns corner c's block, as in k_hessian_rec, but the quad loads each tet's record
 // and Q once, cooperatively (one cache line per load instruction and quad), and passes the values
 // around over DPP; Q[a] (a: the vertex's corner, the same in all four lanes) comes from lane a.  The next
-// tet's chunks are loaded while the current one is folded; incidence codes run two steps ahead.  Same
-// blocks, bit for bit, and the same ascending-element fold per slot as k_hessian_rec.
+// tet's chunks are loaded while the current one is folded; incidence codes run two steps ahead.
+// Each lane forms the tet's (lo, hi) block, lo = min(a, c), hi = max(a, c), as
+// sum_{p,r} (Q[lo][p] Q[hi][r]) M'(p, r) and adds it to the slot transposed when a > c (by the LDS
+// addressing), so the (c, a) contribution is bit for bit the transpose of the (a, c) one; a diagonal
+// block (c = a) is added as formed and each vertex's diagonal slot is symmetrised, (B + B^T) / 2, once
+// its fold is complete.  Every off-diagonal slot is the ascending-element fold of k_hessian_rec's
+// blocks (to rounding: the sum is ordered differently), and the assembled matrix is exactly symmetric.
 template <typename T>
 __global__ void __launch_bounds__(kGatherThreads)
     k_hessian_recq(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd,
@@ -1172,9 +1177,11 @@ __global__ void __launch_bounds__(kGatherThreads)
   __syncthreads();
   const int vl = (int)threadIdx.x >> 2, c = (int)threadIdx.x & 3;
   const int v = v0 + vl;
+  int selfRank = -1, nb = 0;  // the vertex's rank in its own neighbour list, once it has a tet
+  T* row = acc;
   if (v < m.nV) {  // the same for the four lanes of a quad, as is every branch below
-    const int nb = m.bptr[v + 1] - m.bptr[v];
-    T* row = acc + 9 * (m.bptr[v] - b0);
+    nb = m.bptr[v + 1] - m.bptr[v];
+    row = acc + 9 * (m.bptr[v] - b0);
     const IncList L = inc_list(m, v);
     int cA = L.len > 0 ? m.incE[L.first] : -1;
     uint32_t rA = cA >= 0 ? m.incR[L.first] : 0u;
@@ -1206,19 +1213,45 @@ __global__ void __launch_bounds__(kGatherThreads)
         ql[l] = low ? qc[l] : qa[l];
         qh[l] = low ? qa[l] : qc[l];
       }
-      T blk[9];
-      rec_block(ql, qh, a, c, S, tolPos, blk);
+      T P[3][3];
+#pragma unroll
+      for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) P[pp][rr] = ql[pp] * qh[rr];
+      // entry (l, k) of the (lo, hi) block lands at (l, k) of the slot, or at (k, l) when a > c
+      const int rs = (a > c) ? 1 : 3 * nb, cs = (a > c) ? 3 * nb : 1;
       T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
 #pragma unroll
       for (int l = 0; l < 3; ++l)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
+        for (int kk = 0; kk < 3; ++kk) {
+          T x = (l == kk) ? tolPos : T(0);
+#pragma unroll
+          for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) x += P[pp][rr] * S[sym9(3 * pp + l, 3 * rr + kk)];
+          dst[l * rs + kk * cs] += x;
+        }
+      selfRank = (int)((rA >> (8 * a)) & 0xffu);
       cur = nxt;
       cA = cB;
       rA = rB;
       cB = cC;
       rB = rC;
     }
+  }
+  __syncthreads();
+  if (c == 0 && selfRank >= 0) {  // the diagonal slot, symmetrised once its fold is complete
+    T* d = row + 3 * selfRank;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int kk = l + 1; kk < 3; ++kk) {
+        const T x = d[3 * l * nb + kk], y = d[3 * kk * nb + l];
+        const T sym = T(0.5) * (x + y);
+        d[3 * l * nb + kk] = sym;
+        d[3 * kk * nb + l] = sym;
+      }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
