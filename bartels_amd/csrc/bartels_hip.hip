@@ -480,6 +480,19 @@ __device__ __forceinline__ int gather_first(const MeshView& m, int xcd) {
   return v0 < m.nV ? v0 : -1;
 }
 
+// *p += x in LDS.  FP64: one LDS atomic add (ds_add_f64, no return value, so no wait for the old
+// value): the gather's workgroup is a single wave, whose LDS instructions execute in program order, and
+// no two lanes of one instruction add to the same slot, so every slot still receives its contributions
+// in the order the wave issues them, with the rounding of a read-add-write.  FP32: a read-add-write
+// (ds_add_f32 made the FP32 record gather 7x slower: 4.45 against 0.64 ms at 6.07M tets).
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T x) {
+  if constexpr (sizeof(T) == 8)
+    (void)__hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    *p += x;
+}
+
 template <int MODEL, typename T>
 __device__ __forceinline__ void tet_blocks(const MeshView& m, const double* __restrict__ q,
                                            const T* __restrict__ params, int e, int a, bool stretch_ref,
@@ -639,7 +652,7 @@ __global__ void __launch_bounds__(kGatherThreads)
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[mm][3 * l + kk];
+          for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[mm][3 * l + kk]);
       }
     }
   }
@@ -696,7 +709,7 @@ __global__ void __launch_bounds__(kGatherThreads)
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[j][3 * l + kk];
+          for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[j][3 * l + kk]);
       }
     }
   }
@@ -759,7 +772,7 @@ __global__ void __launch_bounds__(kGatherThreads)
 #pragma unroll
       for (int l = 0; l < 3; ++l)
 #pragma unroll
-        for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[0][3 * l + kk];
+        for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[0][3 * l + kk]);
       cA = cB;
       rA = rB;
       eA = eB;
@@ -1051,7 +1064,7 @@ __device__ __forceinline__ void rec_accum(int code, uint32_t rk, int c, int nb, 
 #pragma unroll
     for (int l = 0; l < 3; ++l)
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += blk[3 * l + kk];
+      for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[3 * l + kk]);
   }
 }
 
@@ -1102,19 +1115,6 @@ __global__ void __launch_bounds__(kGatherThreads)
   __syncthreads();
   for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
   if (touchOut && sink == T(0)) vals[seg0] = sink;
-}
-
-// *p += x in LDS.  FP64: one LDS atomic add (ds_add_f64, no return value, so no wait for the old
-// value): the gather's workgroup is a single wave, whose LDS instructions execute in program order, and
-// no two lanes of one instruction add to the same slot, so every slot still receives its contributions
-// in the order the wave issues them, with the rounding of a read-add-write.  FP32: a read-add-write
-// (ds_add_f32 made the FP32 record gather 7x slower: 4.45 against 0.64 ms at 6.07M tets).
-template <typename T>
-__device__ __forceinline__ void lds_add(T* p, T x) {
-  if constexpr (sizeof(T) == 8)
-    (void)__hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  else
-    *p += x;
 }
 
 // value of lane K of this lane's quad (DPP quad_perm broadcast; all four lanes of the quad active)
@@ -1367,7 +1367,7 @@ __global__ void __launch_bounds__(kGatherThreads)
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-          for (int kk = 0; kk < 3; ++kk) dst[3 * l * nb + kk] += S[(3 * mm + kk) * 12 + 3 * a + l];
+          for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], S[(3 * mm + kk) * 12 + 3 * a + l]);
       }
     }
   }
