@@ -105,6 +105,20 @@ class BartelsError(RuntimeError):
 _VP = c_void_p
 SIGNATURES = {
     "bh_linear_tetmesh_dphi_dX": (c_int, [c_int, _VP, c_int, _VP, _VP, _VP]),
+    "bh_linear_tri2dmesh_dphi_dX": (c_int, [c_int, _VP, c_int, _VP, _VP]),
+    "bh_linear_tri3dmesh_dphi_dX": (c_int, [c_int, _VP, c_int, _VP, _VP]),
+    "bh_trimesh_create": (c_int, [c_int, c_int, c_int, _VP, P(c_void_p)]),
+    "bh_trimesh_destroy": (c_int, [_VP]),
+    "bh_trimesh_nnz": (c_int, [_VP, P(c_int64)]),
+    "bh_trimesh_pattern": (c_int, [_VP, _VP, _VP]),
+    "bh_linear_tri2dmesh_q": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, P(c_double)]),
+    "bh_linear_tri2dmesh_dq": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tri2dmesh_dq2": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, P(BhPsd), _VP]),
+    "bh_linear_tri2dmesh_q_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tri2dmesh_dq_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "bh_linear_tri2dmesh_dq2_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
+    "bh_linear_tri2dmesh_mass_matrix": (c_int, [_VP, _VP, _VP, _VP]),
+    "bh_linear_tri3dmesh_stvk_dq2": (c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, P(BhPsd), _VP]),
     "bh_linear_tetmesh_dphi_dX_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mesh_create": (c_int, [c_int, c_int, _VP, _VP, _VP, P(c_void_p)]),
     "bh_mesh_create_device": (c_int, [c_int, c_int, _VP, _VP, _VP, _VP, P(c_void_p)]),

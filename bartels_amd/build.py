@@ -12,6 +12,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libbartels_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "bartels_hip.hip"), os.path.join(HERE, "csrc", "io.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "elements.h"), os.path.join(HERE, "csrc", "newton_impl.h"),
+           os.path.join(HERE, "csrc", "tri_impl.h"),
            os.path.join(ROOT, "include", "bartels_hip.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -2440,3 +2440,4 @@ int bh_mesh_kernel_times(bh_mesh* m, double* ms, int64_t* launches) {
 #include "mgn_impl.h"
 #include "newton_impl.h"
 #include "setup_impl.h"
+#include "tri_impl.h"

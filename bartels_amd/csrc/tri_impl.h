@@ -1,0 +1,652 @@
+// Triangle siblings of the tet path (included at the end of bartels_hip.hip; same translation unit):
+// the 2-D plane-strain triangles of sim::linear_tri2dmesh_{neohookean,stvk,corotational,arap}_{q,dq,dq2}
+// and linear_tri2dmesh_mass_matrix, and the StVK shell triangle of sim::linear_tri3dmesh_stvk_dq2.
+//
+// tri2d (src/linear_tri2d_neohookean_dq2.cpp:5-35 and siblings): the reference embeds the triangle
+// in 3-D, F3 = [[F2, 0], [0, 1]] with F2 = sum_a x_a phi_a^T (plane strain), and evaluates the 3-D
+// energies.  So psi, the stress and the element Hessian are the tet path's closed forms at F3, with
+// the shape gradients padded to 3-vectors (phi_a, 0); the Hessian's 2x2 block (a, m) is the leading
+// 2x2 of the 3-D block.
+// tri3d (src/linear_tri3d_stvk_dq2.cpp:6-40): F3 = unflatten(B q_e + vec(n N^T)) with
+// B(i + 3j, 3a + c) = [i == c] phi(a, j) + N[j] dndq(i, 3a + c); the element Hessian is the dense
+// B^T d2psi B vol with d2psi assembled from the closed-form blocks at unit vectors.
+//
+// Layout: the element arrays are the reference's (E nT x 3, dphidX nT x 6 or nT x 9 with Phi(m, j) at
+// column m + 3j, N and n nT x 3, dndq nT x 27, all column-major).  The mesh handle holds only the
+// pattern: block rows of dim x dim blocks (sorted neighbours, the vertex itself included), CSR row
+// dim v + l at dim^2 bptr[v] + dim l nb(v), the block of rank r at + dim r (the reference's CSC by
+// symmetry), and per-vertex incidence lists in ascending element order.  Assembly: one lane per
+// element writes its element matrix (with simple_psd_fix when asked: the full 9x9 eigensolver, a 6x6
+// padded with zeros), then one lane per vertex folds the blocks of its rows in ascending element
+// order (the order of Eigen's setFromTriplets on the reference's triplet list).  These meshes are
+// small next to the tet path's; the kernels are plain, not tuned.
+
+struct bh_trimesh {
+  int dim = 2, nV = 0, nT = 0;
+  int64_t nnz = 0, nblocks = 0;
+  std::vector<int32_t> h_bptr, h_bcol;
+  int32_t* E = nullptr;     // nT x 3, column-major
+  int32_t* bptr = nullptr;  // nV + 1
+  int32_t* bcol = nullptr;
+  int32_t* iptr = nullptr;  // nV + 1: incidence list of v at [iptr[v], iptr[v + 1])
+  int32_t* inc = nullptr;   // (e << 2) | corner, ascending e
+  uint32_t* irank = nullptr;  // ranks of the element's 3 vertices in v's neighbour list, 8 bits each
+};
+
+namespace {
+
+constexpr int kTriModels = 4;  // neohookean, stvk, corotational, arap
+
+void free_trimesh(bh_trimesh* m) {
+  void* ptrs[] = {m->E, m->bptr, m->bcol, m->iptr, m->inc, m->irank};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+// device array freed on scope exit
+template <typename T>
+struct DevArr {
+  T* p = nullptr;
+  ~DevArr() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    BH_CHECK(hipMalloc(&p, sizeof(T) * std::max<size_t>(1, n)));
+    return BH_OK;
+  }
+  int upload(const T* h, size_t n, hipStream_t s) {
+    int st = alloc(n);
+    if (st) return st;
+    if (n) BH_CHECK(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return BH_OK;
+  }
+};
+
+int build_trimesh(bh_trimesh* m, const int* E) {
+  const int nV = m->nV, nT = m->nT;
+  // vertex pairs of every element (3 x 3, the vertex itself included), sorted and unique per vertex
+  std::vector<int64_t> keys;
+  keys.reserve(9 * (size_t)nT);
+  for (int e = 0; e < nT; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        keys.push_back((int64_t)E[e + (int64_t)a * nT] * nV + E[e + (int64_t)b * nT]);
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  m->h_bptr.assign(nV + 1, 0);
+  m->h_bcol.resize(keys.size());
+  for (size_t k = 0; k < keys.size(); ++k) {
+    m->h_bptr[keys[k] / nV + 1]++;
+    m->h_bcol[k] = (int32_t)(keys[k] % nV);
+  }
+  for (int v = 0; v < nV; ++v) m->h_bptr[v + 1] += m->h_bptr[v];
+  m->nblocks = (int64_t)keys.size();
+  m->nnz = m->nblocks * m->dim * m->dim;
+  if (m->nnz >= (1ll << 31)) return BH_UNSUPPORTED;
+  // incidence lists in ascending element order, with the ranks of the element's vertices
+  std::vector<int32_t> iptr(nV + 1, 0), inc(3 * (size_t)nT);
+  std::vector<uint32_t> irank(3 * (size_t)nT);
+  for (int e = 0; e < nT; ++e)
+    for (int a = 0; a < 3; ++a) iptr[E[e + (int64_t)a * nT] + 1]++;
+  for (int v = 0; v < nV; ++v) iptr[v + 1] += iptr[v];
+  std::vector<int32_t> fill(iptr.begin(), iptr.end() - 1);
+  for (int e = 0; e < nT; ++e)
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      const int32_t* b0 = m->h_bcol.data() + m->h_bptr[v];
+      const int32_t* b1 = m->h_bcol.data() + m->h_bptr[v + 1];
+      if (b1 - b0 > 256) return BH_UNSUPPORTED;  // ranks are packed in 8 bits
+      uint32_t rk = 0;
+      for (int b = 0; b < 3; ++b) {
+        const int n = E[e + (int64_t)b * nT];
+        rk |= (uint32_t)(std::lower_bound(b0, b1, n) - b0) << (8 * b);
+      }
+      const int k = fill[v]++;
+      inc[k] = (e << 2) | a;
+      irank[k] = rk;
+    }
+  hipStream_t s = nullptr;
+  BH_CHECK(hipMalloc(&m->E, sizeof(int32_t) * std::max<size_t>(1, 3 * (size_t)nT)));
+  BH_CHECK(hipMalloc(&m->bptr, sizeof(int32_t) * (nV + 1)));
+  BH_CHECK(hipMalloc(&m->bcol, sizeof(int32_t) * std::max<size_t>(1, keys.size())));
+  BH_CHECK(hipMalloc(&m->iptr, sizeof(int32_t) * (nV + 1)));
+  BH_CHECK(hipMalloc(&m->inc, sizeof(int32_t) * std::max<size_t>(1, inc.size())));
+  BH_CHECK(hipMalloc(&m->irank, sizeof(uint32_t) * std::max<size_t>(1, irank.size())));
+  if (nT) BH_CHECK(hipMemcpyAsync(m->E, E, sizeof(int32_t) * 3 * (size_t)nT, hipMemcpyHostToDevice, s));
+  BH_CHECK(hipMemcpyAsync(m->bptr, m->h_bptr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice, s));
+  if (!keys.empty())
+    BH_CHECK(hipMemcpyAsync(m->bcol, m->h_bcol.data(), sizeof(int32_t) * keys.size(), hipMemcpyHostToDevice, s));
+  BH_CHECK(hipMemcpyAsync(m->iptr, iptr.data(), sizeof(int32_t) * (nV + 1), hipMemcpyHostToDevice, s));
+  if (nT) {
+    BH_CHECK(hipMemcpyAsync(m->inc, inc.data(), sizeof(int32_t) * inc.size(), hipMemcpyHostToDevice, s));
+    BH_CHECK(hipMemcpyAsync(m->irank, irank.data(), sizeof(uint32_t) * irank.size(), hipMemcpyHostToDevice, s));
+  }
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+// the clamped spectral function of a symmetric n x n (n <= 9, column-major, in place): eigenvalues
+// below tol raised to tol (src/simple_psd_fix.cpp:5-21); a matrix smaller than 9 x 9 is padded with
+// zeros, whose block is decoupled from it, so the clamp of the padded matrix restricted to the leading
+// n x n is the clamp of the matrix
+__device__ __forceinline__ void tri_psd(double* K, int n, double tol) {
+  double a[45], d[9], Z[81];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) a[sym9(i, j)] = (i < n && j < n) ? K[i + n * j] : 0.0;
+  sym9_eigen(a, d, Z);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = d[i] < tol ? tol : d[i];
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) s += Z[r + 9 * i] * d[i] * Z[c + 9 * i];
+      K[r + n * c] = s;
+    }
+}
+
+// F3 of tri2d element e (column-major, F3[i + 3 j]) and its padded shape gradients
+__device__ __forceinline__ void tri2_load(int nT, const int32_t* __restrict__ E, const double* __restrict__ q,
+                                          const double* __restrict__ dX, int e, double phi[3][3], double F[9]) {
+  double x[3][2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int v = E[e + (int64_t)a * nT];
+    x[a][0] = q[2ll * v];
+    x[a][1] = q[2ll * v + 1];
+    phi[a][0] = dX[e + (int64_t)a * nT];
+    phi[a][1] = dX[e + (int64_t)(a + 3) * nT];
+    phi[a][2] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) F[i + 3 * j] = x[0][i] * phi[0][j] + x[1][i] * phi[1][j] + x[2][i] * phi[2][j];
+  F[8] = 1.0;
+}
+
+template <int MODEL>
+__device__ double tri2_energy(const double* F, double p0, double p1) {
+  return psi<MODEL>(F, p0, p1);
+}
+
+template <int MODEL>
+__device__ void tri2_forces(const double* F, const double phi[3][3], double vol, double p0, double p1, double* f) {
+  double P[9];
+  piola<MODEL>(F, p0, p1, P);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) f[2 * a + c] = vol * (P[c] * phi[a][0] + P[c + 3] * phi[a][1]);
+}
+
+template <int MODEL>
+__device__ void tri2_hessian(const double* F, const double phi[3][3], double vol, double p0, double p1, double* K) {
+  Pre<double> P;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) P.F[i] = F[i];
+  P.vol = vol;
+  pre_hessian<MODEL>(P, p0, p1, true);
+  for (int a = 0; a < 3; ++a)
+    for (int mm = 0; mm < 3; ++mm) {
+      double blk[9];
+      hess_block<MODEL>(P, phi[a], phi[mm], blk);
+#pragma unroll
+      for (int l = 0; l < 2; ++l)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) K[(2 * a + l) + 6 * (2 * mm + k)] = blk[3 * l + k];
+    }
+}
+
+// what the per-element kernel leaves: 0 energy (1 value), 1 forces (6), 2 Hessian (36), 3 mass (36)
+template <int MODEL>
+__device__ void tri2_element(int what, int nT, const int32_t* E, const double* q, const double* dX,
+                             const double* vol, const double* params, int e, double* out) {
+  double phi[3][3], F[9];
+  tri2_load(nT, E, q, dX, e, phi, F);
+  const double p0 = params[e], p1 = params[e + (int64_t)nT];
+  if (what == 0)
+    out[0] = tri2_energy<MODEL>(F, p0, p1) * vol[e];
+  else if (what == 1)
+    tri2_forces<MODEL>(F, phi, vol[e], p0, p1, out);
+  else
+    tri2_hessian<MODEL>(F, phi, vol[e], p0, p1, out);
+}
+
+// per element: what (see tri2_element) into out + stride * e; simple_psd_fix(tol) when psd
+__global__ void __launch_bounds__(64)
+    k_tri2_element(int what, int model, int nT, const int32_t* __restrict__ E, const double* __restrict__ q,
+                   const double* __restrict__ dX, const double* __restrict__ vol, const double* __restrict__ params,
+                   int psd, double tol, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  const int stride = what == 0 ? 1 : what == 1 ? 6 : 36;
+  double* o = out + (int64_t)stride * e;
+  if (what == 3) {  // consistent mass: (rho vol / 6) I2 on the diagonal blocks, (rho vol / 12) I2 off them
+    const double mass = params[e] * vol[e];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o[r + 6 * c] = (r % 2 == c % 2) ? ((r / 2 == c / 2) ? mass / 6.0 : mass / 12.0) : 0.0;
+    return;
+  }
+  double K[36];
+  double* dst = what == 2 ? K : o;
+  switch (model) {
+    case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, dst); break;
+    case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, dst); break;
+    case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, dst); break;
+    default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, dst); break;
+  }
+  if (what == 2) {
+    if (psd) tri_psd(K, 6, tol);
+#pragma unroll
+    for (int i = 0; i < 36; ++i) o[i] = K[i];
+  }
+}
+
+// tri3d StVK shell element Hessian (9 x 9) of element e
+__global__ void __launch_bounds__(64)
+    k_tri3_stvk_hessian(int nT, const int32_t* __restrict__ E, const double* __restrict__ q,
+                        const double* __restrict__ dX, const double* __restrict__ N, const double* __restrict__ n,
+                        const double* __restrict__ dndq, const double* __restrict__ vol,
+                        const double* __restrict__ params, int psd, double tol, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  double B[81], qe[9], Ne[3], ne[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Ne[k] = N[e + (int64_t)k * nT];
+    ne[k] = n[e + (int64_t)k * nT];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int v = E[e + (int64_t)a * nT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) qe[3 * a + c] = q[3ll * v + c];
+  }
+  for (int col = 0; col < 9; ++col) {
+    const int a = col / 3, c = col % 3;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int i = r % 3, j = r / 3;
+      B[r + 9 * col] = ((i == c) ? dX[e + (int64_t)(a + 3 * j) * nT] : 0.0) +
+                       Ne[j] * dndq[e + (int64_t)(i + 3 * col) * nT];
+    }
+  }
+  Pre<double> P;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    double s = ne[r % 3] * Ne[r / 3];
+    for (int col = 0; col < 9; ++col) s += B[r + 9 * col] * qe[col];
+    P.F[r] = s;
+  }
+  P.vol = 1.0;
+  pre_hessian<M_STVK>(P, params[e], params[e + (int64_t)nT], true);
+  // d2psi (9 x 9, index l + 3 j) from the closed-form blocks at unit vectors
+  double H[81];
+  for (int j = 0; j < 3; ++j)
+    for (int jp = 0; jp < 3; ++jp) {
+      double ej[3] = {0.0, 0.0, 0.0}, ejp[3] = {0.0, 0.0, 0.0};
+      ej[j] = 1.0;
+      ejp[jp] = 1.0;
+      double blk[9];
+      hess_block<M_STVK>(P, ej, ejp, blk);
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) H[(l + 3 * j) + 9 * (k + 3 * jp)] = blk[3 * l + k];
+    }
+  const double v = vol[e];
+  double K[81];
+  for (int c = 0; c < 9; ++c) {
+    double hb[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s += H[r + 9 * k] * B[k + 9 * c];
+      hb[r] = s;
+    }
+    for (int r = 0; r < 9; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s += B[k + 9 * r] * hb[k];
+      K[r + 9 * c] = s * v;
+    }
+  }
+  if (psd) tri_psd(K, 9, tol);
+  double* o = out + 81ll * e;
+  for (int i = 0; i < 81; ++i) o[i] = K[i];
+}
+
+// energy: fixed-partition partial sums of the per-element energies (block b sums its contiguous range
+// in index order), then one workgroup sums the partials in block order
+__global__ void __launch_bounds__(256) k_tri_sum(int64_t n, const double* __restrict__ x, int nparts,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int64_t per = (n + nparts - 1) / nparts;
+  const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min(n, i0 + per);
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += x[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// gradient: g[dim v + c] = sum over v's incident elements, ascending, of the element's corner force
+__global__ void __launch_bounds__(256)
+    k_tri_gather_vec(int nV, int dim, const int32_t* __restrict__ iptr, const int32_t* __restrict__ inc,
+                     const double* __restrict__ f, double* __restrict__ g) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const int ne = 3 * dim;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = iptr[v]; k < iptr[v + 1]; ++k) {
+    const int code = inc[k], e = code >> 2, a = code & 3;
+    for (int c = 0; c < dim; ++c) acc[c] += f[(int64_t)ne * e + dim * a + c];
+  }
+  for (int c = 0; c < dim; ++c) g[(int64_t)dim * v + c] = acc[c];
+}
+
+// Hessian / mass: the rows of vertex v, folded from the element matrices (ne x ne, column-major) of its
+// incident elements in ascending element order; each lane owns its vertex's rows
+__global__ void __launch_bounds__(256)
+    k_tri_gather_mat(int nV, int dim, const int32_t* __restrict__ bptr, const int32_t* __restrict__ iptr,
+                     const int32_t* __restrict__ inc, const uint32_t* __restrict__ irank,
+                     const double* __restrict__ K, double* __restrict__ vals) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const int ne = 3 * dim, nb = bptr[v + 1] - bptr[v];
+  double* row = vals + (int64_t)dim * dim * bptr[v];
+  for (int i = 0; i < dim * dim * nb; ++i) row[i] = 0.0;
+  for (int k = iptr[v]; k < iptr[v + 1]; ++k) {
+    const int code = inc[k], e = code >> 2, a = code & 3;
+    const uint32_t rk = irank[k];
+    const double* Ke = K + (int64_t)ne * ne * e;
+    for (int mm = 0; mm < 3; ++mm) {
+      double* dst = row + dim * ((rk >> (8 * mm)) & 0xffu);
+      for (int l = 0; l < dim; ++l)
+        for (int c = 0; c < dim; ++c) dst[dim * l * nb + c] += Ke[(dim * a + l) + ne * (dim * mm + c)];
+    }
+  }
+}
+
+bool tri2_model_ok(int model) { return model >= 0 && model < kTriModels; }
+
+// the element matrices of kind what (2 Hessian, 3 mass) folded into vals; inputs on the device
+int tri2_matrix(bh_trimesh* m, int what, int model, const double* q, const double* dX, const double* vol,
+                const double* params, const bh_psd* psd, double* vals, hipStream_t s) {
+  if (m->nV == 0) return BH_OK;
+  DevArr<double> K;
+  int st = K.alloc(36 * (size_t)m->nT);
+  if (st) return st;
+  if (m->nT) {
+    hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, what, model, m->nT, m->E, q, dX, vol,
+                       params, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K.p);
+    BH_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_tri_gather_mat, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 2, m->bptr, m->iptr, m->inc,
+                     m->irank, (const double*)K.p, vals);
+  BH_CHECK(hipGetLastError());
+  BH_CHECK(hipStreamSynchronize(s));  // K is freed on return
+  return BH_OK;
+}
+
+// host arrays staged to the device: q (dim nV), dX (nT x cols), vol (nT), params (nT x 2)
+struct TriStage {
+  DevArr<double> q, dX, vol, params;
+  int up(const bh_trimesh* m, const double* hq, const double* hdX, int cols, const double* hvol, const double* hp,
+         hipStream_t s) {
+    int st;
+    if (hq && (st = q.upload(hq, (size_t)m->dim * m->nV, s))) return st;
+    if (hdX && (st = dX.upload(hdX, (size_t)cols * m->nT, s))) return st;
+    if (hvol && (st = vol.upload(hvol, (size_t)m->nT, s))) return st;
+    if (hp && (st = params.upload(hp, 2 * (size_t)m->nT, s))) return st;
+    return BH_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int bh_linear_tri2dmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX) {
+  if (nV < 0 || nT < 0 || (nV && !V) || (nT && (!E || !dphidX))) return BH_BAD_ARG;
+  int status = BH_OK;
+  for (int e = 0; e < nT; ++e) {
+    double x[3][2];
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      if (v < 0 || v >= nV) return BH_BAD_ARG;
+      for (int c = 0; c < 2; ++c) x[a][c] = V[v + (int64_t)c * nV];
+    }
+    const double a00 = x[1][0] - x[0][0], a10 = x[1][1] - x[0][1], a01 = x[2][0] - x[0][0], a11 = x[2][1] - x[0][1];
+    const double det = a00 * a11 - a01 * a10;
+    const double inv[2][2] = {{a11 / det, -a01 / det}, {-a10 / det, a00 / det}};
+    for (int j = 0; j < 2; ++j) {
+      dphidX[e + (int64_t)(1 + 3 * j) * nT] = inv[0][j];
+      dphidX[e + (int64_t)(2 + 3 * j) * nT] = inv[1][j];
+      dphidX[e + (int64_t)(0 + 3 * j) * nT] = -(inv[0][j] + inv[1][j]);
+      if (!std::isfinite(inv[0][j]) || !std::isfinite(inv[1][j])) status = BH_NONFINITE;
+    }
+  }
+  return status;
+}
+
+int bh_linear_tri3dmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX) {
+  if (nV < 0 || nT < 0 || (nV && !V) || (nT && (!E || !dphidX))) return BH_BAD_ARG;
+  int status = BH_OK;
+  for (int e = 0; e < nT; ++e) {
+    double x[3][3];
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      if (v < 0 || v >= nV) return BH_BAD_ARG;
+      for (int c = 0; c < 3; ++c) x[a][c] = V[v + (int64_t)c * nV];
+    }
+    double T[3][2], G[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int c = 0; c < 3; ++c) {
+      T[c][0] = x[1][c] - x[0][c];
+      T[c][1] = x[2][c] - x[0][c];
+    }
+    for (int r = 0; r < 2; ++r)
+      for (int t = 0; t < 2; ++t)
+        for (int c = 0; c < 3; ++c) G[r][t] += T[c][r] * T[c][t];
+    const double det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+    const double Gi[2][2] = {{G[1][1] / det, -G[0][1] / det}, {-G[1][0] / det, G[0][0] / det}};
+    for (int j = 0; j < 3; ++j) {
+      const double r0 = Gi[0][0] * T[j][0] + Gi[0][1] * T[j][1], r1 = Gi[1][0] * T[j][0] + Gi[1][1] * T[j][1];
+      dphidX[e + (int64_t)(1 + 3 * j) * nT] = r0;
+      dphidX[e + (int64_t)(2 + 3 * j) * nT] = r1;
+      dphidX[e + (int64_t)(0 + 3 * j) * nT] = -(r0 + r1);
+      if (!std::isfinite(r0) || !std::isfinite(r1)) status = BH_NONFINITE;
+    }
+  }
+  return status;
+}
+
+int bh_trimesh_create(int dim, int nV, int nT, const int* E, bh_trimesh** out) {
+  if (!out || (dim != 2 && dim != 3) || nV < 0 || nT < 0 || (nT && !E)) return BH_BAD_ARG;
+  *out = nullptr;
+  if ((int64_t)nT >= (1ll << 29)) return BH_UNSUPPORTED;
+  for (int64_t i = 0; i < 3ll * nT; ++i)
+    if (E[i] < 0 || E[i] >= nV) return BH_BAD_ARG;
+  bh_trimesh* m = new bh_trimesh();
+  m->dim = dim;
+  m->nV = nV;
+  m->nT = nT;
+  const int st = build_trimesh(m, E);
+  if (st != BH_OK) {
+    free_trimesh(m);
+    delete m;
+    return st;
+  }
+  *out = m;
+  return BH_OK;
+}
+
+int bh_trimesh_destroy(bh_trimesh* m) {
+  if (!m) return BH_BAD_ARG;
+  (void)hipDeviceSynchronize();
+  free_trimesh(m);
+  delete m;
+  return BH_OK;
+}
+
+int bh_trimesh_nnz(const bh_trimesh* m, int64_t* nnz) {
+  if (!m || !nnz) return BH_BAD_ARG;
+  *nnz = m->nnz;
+  return BH_OK;
+}
+
+int bh_trimesh_pattern(const bh_trimesh* m, int32_t* rowptr, int32_t* colind) {
+  if (!m || !rowptr || (m->nnz && !colind)) return BH_BAD_ARG;
+  const int d = m->dim;
+  int64_t k = 0;
+  rowptr[0] = 0;
+  for (int v = 0; v < m->nV; ++v)
+    for (int l = 0; l < d; ++l) {
+      for (int b = m->h_bptr[v]; b < m->h_bptr[v + 1]; ++b)
+        for (int c = 0; c < d; ++c) colind[k++] = d * m->h_bcol[b] + c;
+      rowptr[d * v + l + 1] = (int32_t)k;
+    }
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_q_device(bh_trimesh* m, int model, const double* q, const double* dphidX,
+                                 const double* volume, const double* params, double* d_energy, bh_stream_t stream) {
+  if (!m || m->dim != 2 || !tri2_model_ok(model) || !d_energy || (m->nT && (!q || !dphidX || !volume || !params)))
+    return BH_BAD_ARG;
+  hipStream_t s = as_stream(stream);
+  constexpr int kParts = 256;
+  DevArr<double> ee, part;
+  int st;
+  if ((st = ee.alloc(m->nT)) || (st = part.alloc(kParts))) return st;
+  if (m->nT) {
+    hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, 0, model, m->nT, m->E, q, dphidX,
+                       volume, params, 0, 0.0, ee.p);
+    BH_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_tri_sum, dim3(kParts), dim3(256), 0, s, (int64_t)m->nT, (const double*)ee.p, kParts, part.p);
+  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(256), 0, s, (int64_t)kParts, (const double*)part.p, 1, d_energy);
+  BH_CHECK(hipGetLastError());
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_dq_device(bh_trimesh* m, int model, const double* q, const double* dphidX,
+                                  const double* volume, const double* params, double* d_g, bh_stream_t stream) {
+  if (!m || m->dim != 2 || !tri2_model_ok(model) || !d_g || (m->nT && (!q || !dphidX || !volume || !params)))
+    return BH_BAD_ARG;
+  hipStream_t s = as_stream(stream);
+  DevArr<double> f;
+  int st = f.alloc(6 * (size_t)m->nT);
+  if (st) return st;
+  if (m->nT) {
+    hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, 1, model, m->nT, m->E, q, dphidX,
+                       volume, params, 0, 0.0, f.p);
+    BH_CHECK(hipGetLastError());
+  }
+  if (m->nV) {
+    hipLaunchKernelGGL(k_tri_gather_vec, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 2, m->iptr, m->inc,
+                       (const double*)f.p, d_g);
+    BH_CHECK(hipGetLastError());
+  }
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_dq2_device(bh_trimesh* m, int model, const double* q, const double* dphidX,
+                                   const double* volume, const double* params, const bh_psd* psd, double* d_vals,
+                                   bh_stream_t stream) {
+  if (!m || m->dim != 2 || !tri2_model_ok(model) || !d_vals || (m->nT && (!q || !dphidX || !volume || !params)))
+    return BH_BAD_ARG;
+  return tri2_matrix(m, 2, model, q, dphidX, volume, params, psd, d_vals, as_stream(stream));
+}
+
+int bh_linear_tri2dmesh_q(bh_trimesh* m, int model, const double* q, const double* dphidX, const double* volume,
+                          const double* params, double* energy) {
+  if (!m || !energy) return BH_BAD_ARG;
+  TriStage S;
+  DevArr<double> e;
+  int st;
+  if ((st = S.up(m, q, dphidX, 6, volume, params, nullptr)) || (st = e.alloc(1))) return st;
+  if ((st = bh_linear_tri2dmesh_q_device(m, model, S.q.p, S.dX.p, S.vol.p, S.params.p, e.p, nullptr))) return st;
+  BH_CHECK(hipMemcpy(energy, e.p, sizeof(double), hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_dq(bh_trimesh* m, int model, const double* q, const double* dphidX, const double* volume,
+                           const double* params, double* g) {
+  if (!m || !g) return BH_BAD_ARG;
+  TriStage S;
+  DevArr<double> d;
+  int st;
+  if ((st = S.up(m, q, dphidX, 6, volume, params, nullptr)) || (st = d.alloc(2 * (size_t)m->nV))) return st;
+  if ((st = bh_linear_tri2dmesh_dq_device(m, model, S.q.p, S.dX.p, S.vol.p, S.params.p, d.p, nullptr))) return st;
+  if (m->nV) BH_CHECK(hipMemcpy(g, d.p, sizeof(double) * 2 * (size_t)m->nV, hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_dq2(bh_trimesh* m, int model, const double* q, const double* dphidX, const double* volume,
+                            const double* params, const bh_psd* psd, double* vals) {
+  if (!m || !vals) return BH_BAD_ARG;
+  TriStage S;
+  DevArr<double> d;
+  int st;
+  if ((st = S.up(m, q, dphidX, 6, volume, params, nullptr)) || (st = d.alloc((size_t)m->nnz))) return st;
+  if ((st = bh_linear_tri2dmesh_dq2_device(m, model, S.q.p, S.dX.p, S.vol.p, S.params.p, psd, d.p, nullptr))) return st;
+  if (m->nnz) BH_CHECK(hipMemcpy(vals, d.p, sizeof(double) * (size_t)m->nnz, hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+int bh_linear_tri2dmesh_mass_matrix(bh_trimesh* m, const double* density, const double* volume, double* vals) {
+  if (!m || m->dim != 2 || !vals || (m->nT && (!density || !volume))) return BH_BAD_ARG;
+  hipStream_t s = nullptr;
+  DevArr<double> rho, vol, d;
+  int st;
+  if ((st = rho.upload(density, (size_t)m->nT, s)) || (st = vol.upload(volume, (size_t)m->nT, s)) ||
+      (st = d.alloc((size_t)m->nnz)))
+    return st;
+  // the mass kind reads params[e] as the density
+  if ((st = tri2_matrix(m, 3, M_NH, nullptr, nullptr, vol.p, rho.p, nullptr, d.p, s))) return st;
+  if (m->nnz) BH_CHECK(hipMemcpy(vals, d.p, sizeof(double) * (size_t)m->nnz, hipMemcpyDeviceToHost));
+  return BH_OK;
+}
+
+int bh_linear_tri3dmesh_stvk_dq2(bh_trimesh* m, const double* q, const double* dphidX, const double* N,
+                                 const double* n, const double* dndq, const double* volume, const double* params,
+                                 const bh_psd* psd, double* vals) {
+  if (!m || m->dim != 3 || !vals || (m->nT && (!q || !dphidX || !N || !n || !dndq || !volume || !params)))
+    return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  hipStream_t s = nullptr;
+  TriStage S;
+  DevArr<double> dN, dn, dD, K, d;
+  int st;
+  if ((st = S.up(m, q, dphidX, 9, volume, params, s)) || (st = dN.upload(N, 3 * (size_t)m->nT, s)) ||
+      (st = dn.upload(n, 3 * (size_t)m->nT, s)) || (st = dD.upload(dndq, 27 * (size_t)m->nT, s)) ||
+      (st = K.alloc(81 * (size_t)m->nT)) || (st = d.alloc((size_t)m->nnz)))
+    return st;
+  if (m->nT) {
+    hipLaunchKernelGGL(k_tri3_stvk_hessian, dim3((m->nT + 63) / 64), dim3(64), 0, s, m->nT, m->E, S.q.p, S.dX.p, dN.p,
+                       dn.p, dD.p, S.vol.p, S.params.p, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K.p);
+    BH_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_tri_gather_mat, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 3, m->bptr, m->iptr, m->inc,
+                     m->irank, (const double*)K.p, d.p);
+  BH_CHECK(hipGetLastError());
+  if (m->nnz) BH_CHECK(hipMemcpyAsync(vals, d.p, sizeof(double) * (size_t)m->nnz, hipMemcpyDeviceToHost, s));
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
+}  // extern "C"

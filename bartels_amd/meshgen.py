@@ -119,6 +119,32 @@ def beam_grid(nx, ny, nz, h=H_DEFAULT):
     return V, E.reshape(-1, 4)
 
 
+def strip2d(nx, ny, h=H_DEFAULT, seed=None, jitter_amount=0.0):
+    """2-D triangle strip: nx x ny vertices (x the long axis), spacing h, each cell split along its
+    rising diagonal into two counter-clockwise triangles; optional uniform jitter of the vertices by
+    +-jitter_amount h (seeded), which keeps every triangle counter-clockwise for amounts below 0.25."""
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    V = np.stack([ix.ravel() * h, iy.ravel() * h], axis=1).astype(np.float64)
+    cx, cy = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="ij")
+    cx, cy = cx.ravel(), cy.ravel()
+
+    def vid(dx, dy):
+        return (cx + dx) * ny + (cy + dy)
+
+    E = np.concatenate([np.stack([vid(0, 0), vid(1, 0), vid(1, 1)], 1),
+                        np.stack([vid(0, 0), vid(1, 1), vid(0, 1)], 1)]).astype(np.int32)
+    if seed is not None and jitter_amount > 0:
+        V = V + (2.0 * uniform(seed, V.size).reshape(V.shape) - 1.0) * jitter_amount * h
+    return V, E
+
+
+def triangle_areas(V, E):
+    """areas of 2-D triangles (positive for counter-clockwise ones)"""
+    a, b, c = (V[E[:, k]] for k in range(3))
+    u, w = b - a, c - a
+    return 0.5 * (u[:, 0] * w[:, 1] - u[:, 1] * w[:, 0])
+
+
 def signed_volumes(V, E):
     a, b, c, d = (V[E[:, k]] for k in range(4))
     return -np.einsum("ij,ij->i", a - d, np.cross(b - d, c - d)) / 6.0

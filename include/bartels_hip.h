@@ -464,6 +464,57 @@ int bh_mesh_psd_count(bh_mesh* mesh, int64_t* count);
 /* Device sparse helpers on the mesh pattern (values in bh_mesh_pattern order). */
 int bh_spmv_device(bh_mesh* mesh, const double* d_vals, const double* d_x, double* d_y, bh_stream_t stream);
 
+/* ---- triangle siblings: 2-D plane-strain triangles (tri2d) and the StVK shell triangle (tri3d) ----
+ * Element arrays in the reference's layouts (column-major): E nT x 3, dphidX nT x 6 (tri2d) or nT x 9
+ * (tri3d) with Phi(m, j) = d phi_m / d X_j at column m + 3j, volume nT (the element area times the
+ * thickness the caller folds in), params nT x 2 with the tet conventions ([C, D] Neo-Hookean, [lambda,
+ * mu] StVK and corotational, [k, .] ARAP), q dim nV interleaved.  The Hessian values are in
+ * bh_trimesh_pattern order: full dim x dim blocks per vertex pair sharing an element, explicit zeros
+ * kept, sorted indices (the reference's CSC, by symmetry).  Models: BH_NEOHOOKEAN, BH_STVK,
+ * BH_COROTATIONAL, BH_ARAP.  The elements evaluate the 3-D energies at F3 = [[F2, 0], [0, 1]]. */
+typedef struct bh_trimesh bh_trimesh;
+/* Replaces sim::linear_tri2dmesh_dphi_dX (src/linear_tri2dmesh_dphi_dX.cpp:5-16, element
+ * src/linear_tri2d_dphi_dX.cpp:5-17).  V nV x 2.  BH_NONFINITE for a degenerate triangle. */
+int bh_linear_tri2dmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX);
+/* Replaces sim::linear_tri3dmesh_dphi_dX (src/linear_tri3dmesh_dphi_dX.cpp:5-19, element
+ * src/linear_tri3d_dphi_dX.cpp:5-19): rows 1..2 of Phi = (T^T T)^-1 T^T, T = [x1 - x0, x2 - x0]. */
+int bh_linear_tri3dmesh_dphi_dX(int nV, const double* V, int nT, const int* E, double* dphidX);
+/* The pattern and incidence lists of a triangle mesh with dim (2 or 3) DOFs per vertex, built once
+ * (the reference rebuilds its triplet list per call, src/assemble.cpp:36-97). */
+int bh_trimesh_create(int dim, int nV, int nT, const int* E, bh_trimesh** mesh);
+int bh_trimesh_destroy(bh_trimesh* mesh);
+int bh_trimesh_nnz(const bh_trimesh* mesh, int64_t* nnz);
+/* rowptr dim nV + 1 entries, colind nnz entries (host arrays). */
+int bh_trimesh_pattern(const bh_trimesh* mesh, int32_t* rowptr, int32_t* colind);
+/* Replace sim::linear_tri2dmesh_{neohookean,stvk,corotational,arap}_q / _dq / _dq2
+ * (src/linear_tri2dmesh_neohookean_q.cpp, _dq.cpp, _dq2.cpp:5-24 and siblings; elements
+ * src/linear_tri2d_*.cpp) on host arrays; dq2 with the simple_psd_fix callback as bh_psd. */
+int bh_linear_tri2dmesh_q(bh_trimesh* mesh, int model, const double* q, const double* dphidX, const double* volume,
+                          const double* params, double* energy);
+int bh_linear_tri2dmesh_dq(bh_trimesh* mesh, int model, const double* q, const double* dphidX, const double* volume,
+                           const double* params, double* g);
+int bh_linear_tri2dmesh_dq2(bh_trimesh* mesh, int model, const double* q, const double* dphidX, const double* volume,
+                            const double* params, const bh_psd* psd, double* vals);
+/* The same on device arrays (d_energy: one double on the device); they synchronise the stream. */
+int bh_linear_tri2dmesh_q_device(bh_trimesh* mesh, int model, const double* d_q, const double* d_dphidX,
+                                 const double* d_volume, const double* d_params, double* d_energy, bh_stream_t stream);
+int bh_linear_tri2dmesh_dq_device(bh_trimesh* mesh, int model, const double* d_q, const double* d_dphidX,
+                                  const double* d_volume, const double* d_params, double* d_g, bh_stream_t stream);
+int bh_linear_tri2dmesh_dq2_device(bh_trimesh* mesh, int model, const double* d_q, const double* d_dphidX,
+                                   const double* d_volume, const double* d_params, const bh_psd* psd, double* d_vals,
+                                   bh_stream_t stream);
+/* Replaces sim::linear_tri2dmesh_mass_matrix (src/linear_tri2dmesh_mass_matrix.cpp, element
+ * src/linear_tri2d_mass_matrix.cpp:5-29): blocks rho vol / 6 I2 on the diagonal, rho vol / 12 I2 off it. */
+int bh_linear_tri2dmesh_mass_matrix(bh_trimesh* mesh, const double* density, const double* volume, double* vals);
+/* Replaces sim::linear_tri3dmesh_stvk_dq2 (src/linear_tri3dmesh_stvk_dq2.cpp:5-30, element
+ * src/linear_tri3d_stvk_dq2.cpp:6-40) on a dim-3 mesh: per element the rest normal N and current
+ * normal n (nT x 3) and the normal's derivative dndq (nT x 27, the 3 x 9 matrix column-major), as the
+ * reference takes them; F3 = B q_e + vec(n N^T), the Hessian B^T d2psi B vol (the normal's second
+ * derivative ignored, as in the reference). */
+int bh_linear_tri3dmesh_stvk_dq2(bh_trimesh* mesh, const double* q, const double* dphidX, const double* N,
+                                 const double* n, const double* dndq, const double* volume, const double* params,
+                                 const bh_psd* psd, double* vals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1300,3 +1300,290 @@ int or_band_solve(int64_t n, const int32_t* colptr, const int32_t* rowind, const
 }
 
 }  // extern "C"
+
+// =============================================================================================
+// Triangle siblings (the tri2d plane-strain elements and the tri3d StVK shell element): the
+// reference's dense-B algorithm, element by element, then the same setFromTriplets fold.
+namespace {
+
+// tri2d: F3 = unflatten<3,3>(B qe + e_9) with B (9 x 6) = flatten_multiply_right<3x3>([dphidX | 0]) P,
+// P embedding (x, y) -> (x, y, 0): B(i + 3j, 2a + c) = [i == c][j < 2] Phi(a, j).
+// Reference: src/linear_tri2d_neohookean_q.cpp:5-13 (and its dq / dq2 and model siblings).
+struct Tri2 {
+  double B[54];  // 9 x 6 column-major
+  double F[9];
+  double vol;
+  double p[2];
+};
+
+void load_tri2(Tri2& t, int e, int nT, const int* E, const double* q, const double* dphidX, const double* vol,
+               const double* params) {
+  double phi[6], qe[6];
+  for (int k = 0; k < 6; ++k) phi[k] = dphidX[e + (int64_t)k * nT];  // Phi(m, j) at m + 3j
+  for (int a = 0; a < 3; ++a) {
+    const int v = E[e + (int64_t)a * nT];
+    for (int c = 0; c < 2; ++c) qe[2 * a + c] = q[2 * (int64_t)v + c];
+  }
+  for (int col = 0; col < 6; ++col) {
+    const int a = col / 2, c = col % 2;
+    for (int r = 0; r < 9; ++r) {
+      const int i = r % 3, j = r / 3;
+      t.B[r + 9 * col] = (i == c && j < 2) ? phi[a + 3 * j] : 0.0;
+    }
+  }
+  for (int r = 0; r < 9; ++r) {
+    double s = (r == 8) ? 1.0 : 0.0;
+    for (int col = 0; col < 6; ++col) s += t.B[r + 9 * col] * qe[col];
+    t.F[r] = s;
+  }
+  t.vol = vol[e];
+  t.p[0] = params[e];
+  t.p[1] = params[e + (int64_t)nT];
+}
+
+// tri3d StVK shell: B (9 x 9) = flatten_multiply_right<3x3>(dphidX) + flatten_multiply_right<3x1>(N^T)
+// unflatten<3,9>(dndq), F3 = unflatten<3,3>(B qe + vec(n N^T)).  Reference: src/linear_tri3d_stvk_dq2.cpp:6-40.
+struct Tri3 {
+  double B[81];  // 9 x 9 column-major
+  double F[9];
+  double vol;
+  double p[2];
+};
+
+void load_tri3(Tri3& t, int e, int nT, const int* E, const double* q, const double* dphidX, const double* N,
+               const double* n, const double* dndq, const double* vol, const double* params) {
+  double phi[9], qe[9], Ne[3], ne[3], D[27];
+  for (int k = 0; k < 9; ++k) phi[k] = dphidX[e + (int64_t)k * nT];  // Phi(m, j) at m + 3j
+  for (int k = 0; k < 3; ++k) {
+    Ne[k] = N[e + (int64_t)k * nT];
+    ne[k] = n[e + (int64_t)k * nT];
+  }
+  for (int k = 0; k < 27; ++k) D[k] = dndq[e + (int64_t)k * nT];  // D(i, c) at i + 3c
+  for (int a = 0; a < 3; ++a) {
+    const int v = E[e + (int64_t)a * nT];
+    for (int c = 0; c < 3; ++c) qe[3 * a + c] = q[3 * (int64_t)v + c];
+  }
+  for (int col = 0; col < 9; ++col) {
+    const int a = col / 3, c = col % 3;
+    for (int r = 0; r < 9; ++r) {
+      const int i = r % 3, j = r / 3;
+      t.B[r + 9 * col] = ((i == c) ? phi[a + 3 * j] : 0.0) + Ne[j] * D[i + 3 * col];
+    }
+  }
+  for (int r = 0; r < 9; ++r) {
+    const int i = r % 3, j = r / 3;
+    double s = ne[i] * Ne[j];
+    for (int col = 0; col < 9; ++col) s += t.B[r + 9 * col] * qe[col];
+    t.F[r] = s;
+  }
+  t.vol = vol[e];
+  t.p[0] = params[e];
+  t.p[1] = params[e + (int64_t)nT];
+}
+
+// K = B^T H B vol (ne x ne, column-major) for B 9 x ne
+void btHb(const double* B, int ne, const double* H, double vol, double* K) {
+  double HB[81];
+  for (int r = 0; r < 9; ++r)
+    for (int c = 0; c < ne; ++c) {
+      double s = 0;
+      for (int k = 0; k < 9; ++k) s += H[r + 9 * k] * B[k + 9 * c];
+      HB[r + 9 * c] = s;
+    }
+  for (int r = 0; r < ne; ++r)
+    for (int c = 0; c < ne; ++c) {
+      double s = 0;
+      for (int k = 0; k < 9; ++k) s += B[k + 9 * r] * HB[k + 9 * c];
+      K[r + ne * c] = s * vol;
+    }
+}
+
+// triplets of an ne x ne element matrix of a dim-DOF mesh with 3-vertex elements, in the reference's
+// loop order (src/assemble.cpp:68-74)
+void push_tri(std::vector<Triplet>& trips, const int* E, int nT, int e, int dim, const double* K) {
+  const int ne = 3 * dim;
+  for (int jj = 0; jj < ne; ++jj)
+    for (int ii = 0; ii < ne; ++ii)
+      trips.push_back({dim * E[e + (int64_t)(ii / dim) * nT] + ii % dim, dim * E[e + (int64_t)(jj / dim) * nT] + jj % dim,
+                       K[ii + ne * jj]});
+}
+
+}  // namespace
+
+extern "C" {
+
+// dphidX (nT x 6 col-major, Phi(m, j) at m + 3j) of 2-D triangles.  Reference:
+// src/linear_tri2dmesh_dphi_dX.cpp:5-16, src/linear_tri2d_dphi_dX.cpp:5-17.
+int or_tri2d_dphidX(int nV, const double* V /*nV x 2*/, int nT, const int* E, double* dphidX) {
+  for (int e = 0; e < nT; ++e) {
+    double x[3][2];
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      if (v < 0 || v >= nV) return -1;
+      for (int c = 0; c < 2; ++c) x[a][c] = V[v + (int64_t)c * nV];
+    }
+    // A = [x1 - x0, x2 - x0] (columns), rows 1..2 of Phi = A^-1, row 0 = -(row 1 + row 2)
+    const double a00 = x[1][0] - x[0][0], a10 = x[1][1] - x[0][1], a01 = x[2][0] - x[0][0], a11 = x[2][1] - x[0][1];
+    const double det = a00 * a11 - a01 * a10;
+    const double inv[2][2] = {{a11 / det, -a01 / det}, {-a10 / det, a00 / det}};
+    for (int j = 0; j < 2; ++j) {
+      dphidX[e + (int64_t)(1 + 3 * j) * nT] = inv[0][j];
+      dphidX[e + (int64_t)(2 + 3 * j) * nT] = inv[1][j];
+      dphidX[e + (int64_t)(0 + 3 * j) * nT] = -(inv[0][j] + inv[1][j]);
+    }
+  }
+  return 0;
+}
+
+// dphidX (nT x 9, Phi(m, j) at m + 3j) of triangles in 3-D: rows 1..2 of Phi = (T^T T)^-1 T^T for
+// T = [x1 - x0, x2 - x0], row 0 = -(row 1 + row 2).  Reference: src/linear_tri3dmesh_dphi_dX.cpp:5-19,
+// src/linear_tri3d_dphi_dX.cpp:5-19.
+int or_tri3d_dphidX(int nV, const double* V /*nV x 3*/, int nT, const int* E, double* dphidX) {
+  for (int e = 0; e < nT; ++e) {
+    double x[3][3];
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
+      if (v < 0 || v >= nV) return -1;
+      for (int c = 0; c < 3; ++c) x[a][c] = V[v + (int64_t)c * nV];
+    }
+    double T[3][2];
+    for (int c = 0; c < 3; ++c) {
+      T[c][0] = x[1][c] - x[0][c];
+      T[c][1] = x[2][c] - x[0][c];
+    }
+    double G[2][2] = {{0, 0}, {0, 0}};
+    for (int r = 0; r < 2; ++r)
+      for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 3; ++c) G[r][s] += T[c][r] * T[c][s];
+    const double det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+    const double Gi[2][2] = {{G[1][1] / det, -G[0][1] / det}, {-G[1][0] / det, G[0][0] / det}};
+    for (int j = 0; j < 3; ++j) {
+      double rows[2];
+      for (int m = 0; m < 2; ++m) rows[m] = Gi[m][0] * T[j][0] + Gi[m][1] * T[j][1];
+      dphidX[e + (int64_t)(1 + 3 * j) * nT] = rows[0];
+      dphidX[e + (int64_t)(2 + 3 * j) * nT] = rows[1];
+      dphidX[e + (int64_t)(0 + 3 * j) * nT] = -(rows[0] + rows[1]);
+    }
+  }
+  return 0;
+}
+
+// sim::linear_tri2dmesh_<model>_q: fold from element 0 (src/linear_tri2dmesh_neohookean_q.cpp, src/assemble.cpp:134-150)
+double or_tri2dmesh_q(int model, int nV, int nT, const int* E, const double* q, const double* dphidX,
+                      const double* vol, const double* params) {
+  (void)nV;
+  double energy = 0.0;
+  for (int e = 0; e < nT; ++e) {
+    Tri2 t;
+    load_tri2(t, e, nT, E, q, dphidX, vol, params);
+    const double v = psi_F(model, t.F, t.p) * t.vol;
+    energy = (e == 0) ? v : energy + v;
+  }
+  return energy;
+}
+
+// sim::linear_tri2dmesh_<model>_dq: B^T vec(dpsi/dF) vol, += in ascending element order (src/linear_tri2d_neohookean_dq.cpp)
+void or_tri2dmesh_dq(int model, int nV, int nT, const int* E, const double* q, const double* dphidX,
+                     const double* vol, const double* params, double* g) {
+  std::memset(g, 0, sizeof(double) * 2 * (size_t)nV);
+  for (int e = 0; e < nT; ++e) {
+    Tri2 t;
+    load_tri2(t, e, nT, E, q, dphidX, vol, params);
+    double P[9];
+    dpsi_F(model, t.F, t.p, P);
+    for (int col = 0; col < 6; ++col) {
+      double s = 0;
+      for (int k = 0; k < 9; ++k) s += t.B[k + 9 * col] * P[k];
+      g[2 * (int64_t)E[e + (int64_t)(col / 2) * nT] + col % 2] += s * t.vol;
+    }
+  }
+}
+
+// per-element 6x6 of tri2d element e (B^T d2psi B vol), optional simple_psd_fix
+void or_tri2d_element_hessian(int model, int nT, int e, const int* E, const double* q, const double* dphidX,
+                              const double* vol, const double* params, int psd, double psd_tol, double* K) {
+  Tri2 t;
+  load_tri2(t, e, nT, E, q, dphidX, vol, params);
+  double H[81];
+  d2psi_F(model, t.F, t.p, 1, H);
+  btHb(t.B, 6, H, t.vol, K);
+  if (psd) psd_fix(K, 6, psd_tol);
+}
+
+// sim::linear_tri2dmesh_<model>_dq2 with the optional simple_psd_fix callback (src/linear_tri2dmesh_neohookean_dq2.cpp:5-24)
+or_csc* or_tri2dmesh_dq2(int model, int nV, int nT, const int* E, const double* q, const double* dphidX,
+                         const double* vol, const double* params, int psd, double psd_tol) {
+  std::vector<Triplet> trips;
+  trips.reserve((size_t)nT * 36);
+  double K[36];
+  for (int e = 0; e < nT; ++e) {
+    or_tri2d_element_hessian(model, nT, e, E, q, dphidX, vol, params, psd, psd_tol, K);
+    push_tri(trips, E, nT, e, 2, K);
+  }
+  CSC M;
+  set_from_triplets(M, 2 * (int64_t)nV, trips);
+  return export_csc(M);
+}
+
+// sim::linear_tri2dmesh_mass_matrix: blocks (rho vol / 6) I2 on the diagonal, (rho vol / 12) I2 off it
+// (src/linear_tri2d_mass_matrix.cpp:5-29, src/linear_tri2dmesh_mass_matrix.cpp)
+or_csc* or_tri2dmesh_mass(int nV, int nT, const int* E, const double* density, const double* vol) {
+  std::vector<Triplet> trips;
+  trips.reserve((size_t)nT * 36);
+  for (int e = 0; e < nT; ++e) {
+    const double mass = density[e] * vol[e], c0 = (1.0 / 6.0) * mass, c1 = (1.0 / 12.0) * mass;
+    double K[36];
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) K[r + 6 * c] = (r % 2 == c % 2) ? ((r / 2 == c / 2) ? c0 : c1) : 0.0;
+    push_tri(trips, E, nT, e, 2, K);
+  }
+  CSC M;
+  set_from_triplets(M, 2 * (int64_t)nV, trips);
+  return export_csc(M);
+}
+
+// the tri3d shell's element gradient B^T vec(dpsi_stvk/dF) vol (test helper: no reference counterpart;
+// with the per-element N, n, dndq held fixed, F3 is affine in q and this is the derivative of
+// psi(F3) vol, whose derivative in turn is the reference's B^T H B vol)
+void or_tri3d_stvk_element_gradient(int nT, int e, const int* E, const double* q, const double* dphidX,
+                                    const double* N, const double* n, const double* dndq, const double* vol,
+                                    const double* params, double* ge) {
+  Tri3 t;
+  load_tri3(t, e, nT, E, q, dphidX, N, n, dndq, vol, params);
+  double P[9];
+  dpsi_F(STVK, t.F, t.p, P);
+  for (int col = 0; col < 9; ++col) {
+    double s = 0;
+    for (int k = 0; k < 9; ++k) s += t.B[k + 9 * col] * P[k];
+    ge[col] = s * t.vol;
+  }
+}
+
+void or_tri3d_stvk_element_hessian(int nT, int e, const int* E, const double* q, const double* dphidX,
+                                   const double* N, const double* n, const double* dndq, const double* vol,
+                                   const double* params, int psd, double psd_tol, double* K) {
+  Tri3 t;
+  load_tri3(t, e, nT, E, q, dphidX, N, n, dndq, vol, params);
+  double H[81];
+  d2psi_F(STVK, t.F, t.p, 1, H);
+  btHb(t.B, 9, H, t.vol, K);
+  if (psd) psd_fix(K, 9, psd_tol);
+}
+
+// sim::linear_tri3dmesh_stvk_dq2 (src/linear_tri3dmesh_stvk_dq2.cpp:5-30, src/linear_tri3d_stvk_dq2.cpp:6-40)
+or_csc* or_tri3dmesh_stvk_dq2(int nV, int nT, const int* E, const double* q, const double* dphidX, const double* N,
+                              const double* n, const double* dndq, const double* vol, const double* params, int psd,
+                              double psd_tol) {
+  std::vector<Triplet> trips;
+  trips.reserve((size_t)nT * 81);
+  double K[81];
+  for (int e = 0; e < nT; ++e) {
+    or_tri3d_stvk_element_hessian(nT, e, E, q, dphidX, N, n, dndq, vol, params, psd, psd_tol, K);
+    push_tri(trips, E, nT, e, 3, K);
+  }
+  CSC M;
+  set_from_triplets(M, 3 * (int64_t)nV, trips);
+  return export_csc(M);
+}
+
+}  // extern "C"

@@ -61,8 +61,8 @@ def load(fast=False):
     if fast:
         if _lib_fast is None:
             path = fast_lib_path()
-            if not os.path.exists(path):
-                subprocess.run(["make", "-s", "-C", HERE, "fast", "FAST_LIB=" + os.path.basename(path)], check=True)
+            # make rebuilds it when oracle.cpp is newer than a copy left by an earlier build
+            subprocess.run(["make", "-s", "-C", HERE, "fast", "FAST_LIB=" + os.path.basename(path)], check=True)
             _lib_fast = _declare(ctypes.CDLL(path))
         return _lib_fast
     if _lib is not None:
@@ -106,6 +106,22 @@ def _declare(L):
     L.or_psd_fix_ql.argtypes = [vp, d]
     L.or_tetmesh_dq2_timed.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, d, i, vp]
     L.or_tetmesh_dq2_timed.restype = ctypes.POINTER(_CSC)
+    L.or_tri2d_dphidX.argtypes = [i, vp, i, vp, vp]
+    L.or_tri2d_dphidX.restype = i
+    L.or_tri3d_dphidX.argtypes = [i, vp, i, vp, vp]
+    L.or_tri3d_dphidX.restype = i
+    L.or_tri2dmesh_q.argtypes = [i, i, i, vp, vp, vp, vp, vp]
+    L.or_tri2dmesh_q.restype = d
+    L.or_tri2dmesh_dq.argtypes = [i, i, i, vp, vp, vp, vp, vp, vp]
+    L.or_tri2d_element_hessian.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, d, vp]
+    L.or_tri2dmesh_dq2.argtypes = [i, i, i, vp, vp, vp, vp, vp, i, d]
+    L.or_tri2dmesh_dq2.restype = ctypes.POINTER(_CSC)
+    L.or_tri2dmesh_mass.argtypes = [i, i, vp, vp, vp]
+    L.or_tri2dmesh_mass.restype = ctypes.POINTER(_CSC)
+    L.or_tri3d_stvk_element_gradient.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.or_tri3d_stvk_element_hessian.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, i, d, vp]
+    L.or_tri3dmesh_stvk_dq2.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp, vp, i, d]
+    L.or_tri3dmesh_stvk_dq2.restype = ctypes.POINTER(_CSC)
     return L
 
 
@@ -315,3 +331,84 @@ def band_solve(A, b):
     if load().or_band_solve(A.shape[0], _p(colptr), _p(rowind), _p(vals), _p(b), _p(x)) != 0:
         raise RuntimeError("band solve failed")
     return x
+
+
+# ---- triangle siblings (tri2d plane-strain elements, tri3d StVK shell) -----------------------------
+def tri2d_dphidX(V, E):
+    """nT x 6 (Phi(m, j) at column m + 3j) of 2-D triangles (src/linear_tri2dmesh_dphi_dX.cpp)"""
+    V, E = _F(V, np.float64), _F(E, np.int32)
+    out = np.zeros((E.shape[0], 6), order="F")
+    assert load().or_tri2d_dphidX(V.shape[0], _p(V), E.shape[0], _p(E), _p(out)) == 0
+    return out
+
+
+def tri3d_dphidX(V, E):
+    """nT x 9 (Phi(m, j) at column m + 3j) of triangles in 3-D (src/linear_tri3dmesh_dphi_dX.cpp)"""
+    V, E = _F(V, np.float64), _F(E, np.int32)
+    out = np.zeros((E.shape[0], 9), order="F")
+    assert load().or_tri3d_dphidX(V.shape[0], _p(V), E.shape[0], _p(E), _p(out)) == 0
+    return out
+
+
+def _tri_args(E, q, dX, vol, params):
+    return (_F(E, np.int32), np.ascontiguousarray(q, dtype=np.float64), _F(dX, np.float64),
+            np.ascontiguousarray(vol, dtype=np.float64), _F(params, np.float64))
+
+
+def tri2dmesh_q(model, nV, E, q, dX, vol, params):
+    E, q, dX, vol, params = _tri_args(E, q, dX, vol, params)
+    return load().or_tri2dmesh_q(model, nV, E.shape[0], _p(E), _p(q), _p(dX), _p(vol), _p(params))
+
+
+def tri2dmesh_dq(model, nV, E, q, dX, vol, params):
+    E, q, dX, vol, params = _tri_args(E, q, dX, vol, params)
+    g = np.zeros(2 * nV)
+    load().or_tri2dmesh_dq(model, nV, E.shape[0], _p(E), _p(q), _p(dX), _p(vol), _p(params), _p(g))
+    return g
+
+
+def tri2d_element_hessian(model, e, E, q, dX, vol, params, psd=False, psd_tol=1e-3):
+    E, q, dX, vol, params = _tri_args(E, q, dX, vol, params)
+    K = np.zeros((6, 6), order="F")
+    load().or_tri2d_element_hessian(model, E.shape[0], e, _p(E), _p(q), _p(dX), _p(vol), _p(params),
+                                    1 if psd else 0, psd_tol, _p(K))
+    return K
+
+
+def tri2dmesh_dq2(model, nV, E, q, dX, vol, params, psd=False, psd_tol=1e-3):
+    E, q, dX, vol, params = _tri_args(E, q, dX, vol, params)
+    return _take_csc(load().or_tri2dmesh_dq2(model, nV, E.shape[0], _p(E), _p(q), _p(dX), _p(vol), _p(params),
+                                             1 if psd else 0, psd_tol))
+
+
+def tri2dmesh_mass(nV, E, density, vol):
+    E = _F(E, np.int32)
+    density, vol = np.ascontiguousarray(density, dtype=np.float64), np.ascontiguousarray(vol, dtype=np.float64)
+    return _take_csc(load().or_tri2dmesh_mass(nV, E.shape[0], _p(E), _p(density), _p(vol)))
+
+
+def _tri3_args(E, q, dX, N, n, dndq, vol, params):
+    E, q, dX, vol, params = _tri_args(E, q, dX, vol, params)
+    return E, q, dX, _F(N, np.float64), _F(n, np.float64), _F(dndq, np.float64), vol, params
+
+
+def tri3d_stvk_element_gradient(e, E, q, dX, N, n, dndq, vol, params):
+    E, q, dX, N, n, dndq, vol, params = _tri3_args(E, q, dX, N, n, dndq, vol, params)
+    g = np.zeros(9)
+    load().or_tri3d_stvk_element_gradient(E.shape[0], e, _p(E), _p(q), _p(dX), _p(N), _p(n), _p(dndq), _p(vol),
+                                          _p(params), _p(g))
+    return g
+
+
+def tri3d_stvk_element_hessian(e, E, q, dX, N, n, dndq, vol, params, psd=False, psd_tol=1e-3):
+    E, q, dX, N, n, dndq, vol, params = _tri3_args(E, q, dX, N, n, dndq, vol, params)
+    K = np.zeros((9, 9), order="F")
+    load().or_tri3d_stvk_element_hessian(E.shape[0], e, _p(E), _p(q), _p(dX), _p(N), _p(n), _p(dndq), _p(vol),
+                                         _p(params), 1 if psd else 0, psd_tol, _p(K))
+    return K
+
+
+def tri3dmesh_stvk_dq2(nV, E, q, dX, N, n, dndq, vol, params, psd=False, psd_tol=1e-3):
+    E, q, dX, N, n, dndq, vol, params = _tri3_args(E, q, dX, N, n, dndq, vol, params)
+    return _take_csc(load().or_tri3dmesh_stvk_dq2(nV, E.shape[0], _p(E), _p(q), _p(dX), _p(N), _p(n), _p(dndq),
+                                                  _p(vol), _p(params), 1 if psd else 0, psd_tol))
