@@ -161,6 +161,7 @@ SIGNATURES = {
     "bh_mgn_create": (c_int, [ctypes.c_uint64, P(_VP)]),
     "bh_mgn_destroy": (c_int, [_VP]),
     "bh_mgn_set_form": (c_int, [_VP, c_int, c_int, c_int]),
+    "bh_mgn_set_inference": (c_int, [_VP, c_int]),
     "bh_mgn_num_params": (c_int, [_VP, P(c_int64)]),
     "bh_mgn_get_params": (c_int, [_VP, _VP, _VP]),
     "bh_mgn_set_params": (c_int, [_VP, _VP]),
@@ -169,6 +170,7 @@ SIGNATURES = {
     "bh_mgn_train_step_device": (c_int, [_VP, _VP, _VP, _VP, _VP, c_int, ctypes.c_uint64, c_double, P(c_double), _VP]),
     "bh_mgn_factor_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mgn_debug_buffers": (c_int, [_VP, P(_VP), P(_VP), P(_VP), P(_VP)]),
+    "bh_mgn_debug_latents": (c_int, [_VP, c_int, P(_VP), P(_VP), P(_VP)]),
     "bh_read_tetgen": (c_int, [ctypes.c_char_p, ctypes.c_char_p, P(c_int), P(c_int), _VP, _VP]),
     "bh_read_medit": (c_int, [ctypes.c_char_p, P(c_int), P(c_int), _VP, _VP]),
     "bh_save_market": (c_int, [ctypes.c_char_p, c_int64, c_int64, _VP, _VP, _VP]),

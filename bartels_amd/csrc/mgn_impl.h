@@ -50,6 +50,7 @@ struct bh_mgn {
   int energyLoss = 0;       // 1 (Eisenstat form): the energy-ratio loss instead of the residual loss
   int teacher = 0;          // 1 (factor forms): the IC(0) teacher loss (training steps factor the Hessian)
   int kaporin = 0;          // 1 (factor forms): the Kaporin condition number of L L^T against H
+  int inferBf16 = 1;        // inference MLPs: 1 BF16 on the matrix cores, 0 the FP32 training forward
 };
 
 namespace {
@@ -316,6 +317,109 @@ __global__ void __launch_bounds__(256)
   for (int j = 0; j < kW; ++j) op[j] = o[j];
 }
 
+// ---- inference MLP on the matrix cores ---------------------------------------------------------------
+// out = [resid +] relu(x W1 + b1) W2 + b2 with x, W1, W2 and the hidden activations rounded to BF16
+// (round to nearest even) and FP32 accumulation; the residual stream stays FP32.  A wave computes
+// 32-row tiles transposed, H^T = W1^T X^T by v_mfma_f32_32x32x16_bf16 (A = W1^T, 32 outputs x 16
+// inputs per chunk; B = X^T, 16 inputs x 32 rows): lane l ends with row l % 32 of the tile and, in
+// accumulator register r, hidden feature 8 (r / 4) + 4 (l / 32) + r % 4.  Registers 0-7 and 8-15 are
+// then the B operands of O^T = W2^T H^T as they stand, with W2's rows taken in the same order (a
+// reduction index may be permuted freely when A and B agree on it).  The output lands in the same
+// layout: four float4 pieces per lane.  Weights and biases are held in registers for the whole
+// grid-stride loop over tiles.
+typedef __bf16 bh_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float bh_f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaBlocks = 2048;  // grid cap of the tile loop: 8 workgroups of 4 waves per CU
+
+// inputs 16 c + 8 h .. 16 c + 8 h + 7 of row r (zero past IN and for rows past the end)
+template <int IN>
+__device__ __forceinline__ bh_bf16x8 mlp_in_chunk(const Seg3& S, int64_t r, bool live, int c, int h) {
+  bh_bf16x8 b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) b[i] = (__bf16)0.f;
+  if (!live) return b;
+  if constexpr (IN <= 16) {
+    const int64_t row = S.idx[0] ? (int64_t)S.idx[0][r] : r;
+    const float* src = S.p[0] + row * IN;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (8 * h + i < IN) b[i] = (__bf16)src[8 * h + i];
+  } else {
+    const int sg = c >> 1;
+    const int64_t row = S.idx[sg] ? (int64_t)S.idx[sg][r] : r;
+    const float4* src = reinterpret_cast<const float4*>(S.p[sg] + row * kW + 16 * (c & 1) + 8 * h);
+    const float4 u = src[0], v = src[1];
+    b[0] = (__bf16)u.x; b[1] = (__bf16)u.y; b[2] = (__bf16)u.z; b[3] = (__bf16)u.w;
+    b[4] = (__bf16)v.x; b[5] = (__bf16)v.y; b[6] = (__bf16)v.z; b[7] = (__bf16)v.w;
+  }
+  return b;
+}
+
+template <int IN>
+__global__ void __launch_bounds__(256)
+    k_mlp_mfma(int64_t rows, Seg3 S, const float* __restrict__ W1, const float* __restrict__ b1,
+               const float* __restrict__ W2, const float* __restrict__ b2, const float* __restrict__ resid,
+               float* __restrict__ out) {
+  static_assert(IN <= 16 || IN % kW == 0, "inputs: one segment of at most 16 or segments of 32");
+  constexpr int NC = (IN + 15) / 16;
+  const int lane = threadIdx.x & 63, m = lane & 31, h = lane >> 5;
+  bh_bf16x8 A1[NC], A2[2];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = 16 * c + 8 * h + i;
+      A1[c][i] = (__bf16)(k < IN ? W1[k * kW + m] : 0.f);
+    }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) A2[c][i] = (__bf16)W2[(16 * c + 8 * (i >> 2) + 4 * h + (i & 3)) * kW + m];
+  float c1[16], c2[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int f = 8 * (r >> 2) + 4 * h + (r & 3);
+    c1[r] = b1[f];
+    c2[r] = b2[f];
+  }
+  const int64_t ntile = (rows + 31) / 32;
+  for (int64_t t = blockIdx.x * 4ll + (threadIdx.x >> 6); t < ntile; t += 4ll * gridDim.x) {
+    const int64_t r = 32 * t + m;
+    const bool live = r < rows;
+    bh_bf16x8 X[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) X[c] = mlp_in_chunk<IN>(S, r, live, c, h);
+    bh_f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = c1[i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1[c], X[c], acc, 0, 0, 0);
+    bh_bf16x8 H0, H1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      H0[i] = (__bf16)fmaxf(acc[i], 0.f);
+      H1[i] = (__bf16)fmaxf(acc[8 + i], 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = c2[i];
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2[0], H0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2[1], H1, acc, 0, 0, 0);
+    if (live) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int64_t off = r * kW + 8 * g + 4 * h;
+        float4 o = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+        if (resid) {
+          const float4 x = *reinterpret_cast<const float4*>(resid + off);
+          o.x += x.x; o.y += x.y; o.z += x.z; o.w += x.w;
+        }
+        *reinterpret_cast<float4*>(out + off) = o;
+      }
+    }
+  }
+}
+
 // given gout: ghid = (gout W2^T) * [hid > 0]; gin = ghid W1^T (IN values per row)
 template <int IN>
 __global__ void __launch_bounds__(256)
@@ -416,13 +520,27 @@ __global__ void __launch_bounds__(256) k_outer_reduce(int nb, int XW, int GW, co
 __global__ void __launch_bounds__(256)
     k_mgn_aggregate(int nV, const int32_t* __restrict__ lptr, const int32_t* __restrict__ uptr,
                     const int32_t* __restrict__ ulink, const float* __restrict__ e, float* __restrict__ m) {
+  // 8 lanes per vertex, 4 features each; every row is added in list order (lower list, then upper),
+  // with up to four rows' loads in flight ahead of their sums
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int v = (int)(gid / kW), j = (int)(gid % kW);
+  const int v = (int)(gid >> 3), j = 4 * (int)(gid & 7);
   if (v >= nV) return;
-  float a = 0.f;
-  for (int k = lptr[v]; k < lptr[v + 1]; ++k) a += e[(int64_t)k * kW + j];
-  for (int t = uptr[v]; t < uptr[v + 1]; ++t) a += e[(int64_t)ulink[t] * kW + j];
-  m[(int64_t)v * kW + j] = a;
+  const float4* E = reinterpret_cast<const float4*>(e + j);
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto add = [&](const float4& x) {
+    a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
+  };
+  const int l0 = lptr[v], l1 = lptr[v + 1], u0 = uptr[v], u1 = uptr[v + 1];
+  const int n = (l1 - l0) + (u1 - u0);
+  auto row = [&](int t) -> int64_t { return t < l1 - l0 ? (int64_t)(l0 + t) : (int64_t)ulink[u0 + t - (l1 - l0)]; };
+  int t = 0;
+  for (; t + 4 <= n; t += 4) {
+    const float4 x0 = E[row(t) * (kW / 4)], x1 = E[row(t + 1) * (kW / 4)];
+    const float4 x2 = E[row(t + 2) * (kW / 4)], x3 = E[row(t + 3) * (kW / 4)];
+    add(x0); add(x1); add(x2); add(x3);
+  }
+  for (; t < n; ++t) add(E[row(t) * (kW / 4)]);
+  *reinterpret_cast<float4*>(m + (int64_t)v * kW + j) = a;
 }
 
 // pair gradients from a per-vertex gradient: ge_k (+)= gm_v + gm_n
@@ -1026,6 +1144,57 @@ __global__ void __launch_bounds__(256)
   x[i] = (fixed && fixed[i]) ? 0.0 : w / (double)s[v];
 }
 
+// Inference: the pair decoder, the tied blocks and the factor's lower blocks in one pass, one thread
+// per lower pair k = (v, n): O_k = e_k W_do + b_do (+ Hn_vn Lhat_nn^{-T} in the tied factor form, added
+// after the decoder as the training path adds it), written to O (the forward's pair output) and, scaled
+// by s_v with the fixed components' rows and columns zeroed, to lval.  The same arithmetic in the same
+// order as k_linear<9>, k_mgn_tied_O and the pair part of k_mgn_factor_out.
+__global__ void __launch_bounds__(256)
+    k_mgn_pair_factor(int64_t nP, const int32_t* __restrict__ pv, const int32_t* __restrict__ lcol,
+                      const uint8_t* __restrict__ fixed, const float* __restrict__ e, const float* __restrict__ Wdo,
+                      const float* __restrict__ bdo, const float* __restrict__ fp, const float* __restrict__ C,
+                      const float* __restrict__ d, const float* __restrict__ s, int tied, float* __restrict__ O,
+                      float* __restrict__ lval) {
+  __shared__ float sW[kW * 9 + 9];
+  for (int i = threadIdx.x; i < kW * 9; i += blockDim.x) sW[i] = Wdo[i];
+  if (threadIdx.x < 9) sW[kW * 9 + threadIdx.x] = bdo[threadIdx.x];
+  __syncthreads();
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nP) return;
+  float o[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) o[j] = sW[kW * 9 + j];
+  const float4* x4 = reinterpret_cast<const float4*>(e + k * kW);
+#pragma unroll
+  for (int q = 0; q < kW / 4; ++q) {
+    const float4 x = x4[q];
+    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 9; ++j) o[j] += xs[u] * sW[(4 * q + u) * 9 + j];
+  }
+  const int v = pv[k], n = lcol[k];
+  if (tied) {
+    float X[9], Xi[9];
+    mgn_lvv(C + 6ll * n, d + 6ll * n, X);
+    ltri_inv(X, Xi);
+    const float* B = fp + (int64_t)kPairIn * k;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        o[3 * l + c] += B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
+  }
+  const float sv = s[v];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    O[9 * k + i] = o[i];
+    const bool fz = fixed && (fixed[3ll * v + i / 3] || fixed[3ll * n + i % 3]);
+    lval[9 * k + i] = fz ? 0.f : sv * o[i];
+  }
+}
+
 // The factor of H for bh_precond: L_vn = s_v O_k, inverse of L_vv = s_v Lhat_vv (packed lower).
 // Fixed DOFs get an identity row and column (zero off-diagonal entries, unit diagonal), so the
 // preconditioner maps vectors that vanish on the fixed DOFs to vectors that vanish there too, as
@@ -1035,6 +1204,7 @@ __global__ void __launch_bounds__(256)
                      const uint8_t* __restrict__ fixed, const float* __restrict__ C, const float* __restrict__ d,
                      const float* __restrict__ O, const float* __restrict__ s, float* __restrict__ lval,
                      float* __restrict__ dinv) {
+  // lval null: the lower blocks were written by k_mgn_pair_factor
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nV) return;
   const float sv = s[v];
@@ -1042,6 +1212,7 @@ __global__ void __launch_bounds__(256)
   if (fixed)
 #pragma unroll
     for (int l = 0; l < 3; ++l) fv[l] = fixed[3ll * v + l] != 0;
+  if (lval)
   for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
     const int n = lcol[k];
 #pragma unroll
@@ -1113,8 +1284,11 @@ int mgn_prepare(bh_mgn* M, const bh_precond* P) {
   return BH_OK;
 }
 
+// infer: the pass feeds no backward, so the MLPs may run in BF16 on the matrix cores (M->inferBf16)
+// and keep no hidden activations
 template <typename V>
-int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, MgnWs& w, hipStream_t s) {
+int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fixed, MgnWs& w, hipStream_t s,
+                bool infer = false) {
   const bh_mesh* m = P->mesh;
   const int nV = m->nV;
   const int64_t nP = P->nlower;
@@ -1129,8 +1303,15 @@ int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fi
   auto mlp = [&](auto inTag, int64_t rows, Seg3 S, const MlpOff& o, const float* resid, float* hid, float* out) {
     constexpr int IN = decltype(inTag)::value;
     if (rows == 0) return;
-    hipLaunchKernelGGL((k_mlp_fwd<IN>), g256(rows), dim3(256), 0, s, rows, S, W + o.W1, W + o.b1, W + o.W2, W + o.b2,
-                       resid, hid, out);
+    if (infer && M->inferBf16) {
+      const int64_t tiles = (rows + 31) / 32;
+      const unsigned nb = (unsigned)std::min<int64_t>((tiles + 3) / 4, kMfmaBlocks);
+      hipLaunchKernelGGL((k_mlp_mfma<IN>), dim3(nb), dim3(256), 0, s, rows, S, W + o.W1, W + o.b1, W + o.W2,
+                         W + o.b2, resid, out);
+    } else {
+      hipLaunchKernelGGL((k_mlp_fwd<IN>), g256(rows), dim3(256), 0, s, rows, S, W + o.W1, W + o.b1, W + o.W2,
+                         W + o.b2, resid, hid, out);
+    }
   };
   using I8 = std::integral_constant<int, kNodeIn>;
   using I10 = std::integral_constant<int, kPairIn>;
@@ -1143,13 +1324,14 @@ int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fi
   for (int l = 0; l < kLayers; ++l) {
     mlp(I96{}, nP, Seg3{{w.e[l], w.h[l], w.h[l]}, {nullptr, M->pv, P->lcol}}, L.edge[l], w.e[l],
         w.hidE[l + 1], w.e[l + 1]);
-    hipLaunchKernelGGL(k_mgn_aggregate, g256((int64_t)nV * kW), dim3(256), 0, s, nV, P->lptr, P->uptr, P->ulink,
+    hipLaunchKernelGGL(k_mgn_aggregate, g256((int64_t)nV * 8), dim3(256), 0, s, nV, P->lptr, P->uptr, P->ulink,
                        w.e[l + 1], w.m[l]);
     mlp(I64{}, nV, Seg3{{w.h[l], w.m[l], nullptr}, {nullptr, nullptr, nullptr}}, L.node[l], w.h[l],
         w.hidN[l + 1], w.h[l + 1]);
   }
   hipLaunchKernelGGL(k_linear<6>, g256(nV), dim3(256), 0, s, (int64_t)nV, w.h[kLayers], W + L.Wdv, W + L.bdv, w.d);
-  if (nP) hipLaunchKernelGGL(k_linear<9>, g256(nP), dim3(256), 0, s, nP, w.e[kLayers], W + L.Wdo, W + L.bdo, w.O);
+  if (nP && !infer)  // (inference applies the pair decoder in k_mgn_pair_factor, or does not need it)
+    hipLaunchKernelGGL(k_linear<9>, g256(nP), dim3(256), 0, s, nP, w.e[kLayers], W + L.Wdo, W + L.bdo, w.O);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
@@ -1626,13 +1808,15 @@ int mgn_factor(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hi
   if (st) return st;
   MgnWs w;
   mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
-  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  if ((st = mgn_forward(M, P, vals, fixed, w, s, true))) return st;
   const int nV = P->mesh->nV;
-  if (M->form == 2 && P->nlower)  // tied factor form: L_vn = s_v (Hn_vn Lhat_nn^{-T} + O_k)
-    hipLaunchKernelGGL(k_mgn_tied_O, g256(P->nlower), dim3(256), 0, s, P->nlower, M->pv, P->lcol, w.fp, w.C, w.d,
-                       w.O, 1);
+  const MgnLayout L = mgn_layout();
+  if (P->nlower)  // tied factor form: L_vn = s_v (Hn_vn Lhat_nn^{-T} + O_k); general form: s_v O_k
+    hipLaunchKernelGGL(k_mgn_pair_factor, g256(P->nlower), dim3(256), 0, s, P->nlower, M->pv, P->lcol, fixed,
+                       w.e[kLayers], M->w + L.Wdo, M->w + L.bdo, w.fp, w.C, w.d, w.s, M->form == 2 ? 1 : 0, w.O,
+                       P->lval);
   hipLaunchKernelGGL(k_mgn_factor_out, g256(nV), dim3(256), 0, s, nV, P->lptr, P->lcol, fixed, w.C, w.d, w.O, w.s,
-                     P->lval, P->dinv);
+                     (float*)nullptr, P->dinv);
   BH_CHECK(hipGetLastError());
   return precond_finalize(P, s);
 }
@@ -1675,7 +1859,7 @@ int mgn_sgs(bh_mgn* M, bh_precond* P, const V* vals, const uint8_t* fixed, hipSt
   if (st) return st;
   MgnWs w;
   mgn_ws_carve(M->ws, P->mesh->nV, P->nlower, w);
-  if ((st = mgn_forward(M, P, vals, fixed, w, s))) return st;
+  if ((st = mgn_forward(M, P, vals, fixed, w, s, true))) return st;
   const int nV = P->mesh->nV;
   hipLaunchKernelGGL(k_mgn_sgs_out, g256(nV), dim3(256), 0, s, nV, P->colVerts, fixed, w.C, w.d, w.s,
                      (const double*)P->hdiag, P->dt, P->dtinv, P->kk);
@@ -1763,6 +1947,12 @@ int bh_mgn_set_form(bh_mgn* M, int form, int probe_is_solution, int energy_loss)
   return BH_OK;
 }
 
+int bh_mgn_set_inference(bh_mgn* M, int precision) {
+  if (!M || precision < 0 || precision > 1) return BH_BAD_ARG;
+  M->inferBf16 = precision;
+  return BH_OK;
+}
+
 int bh_mgn_num_params(const bh_mgn* M, int64_t* n) {
   if (!M || !n) return BH_BAD_ARG;
   *n = M->nparam;
@@ -1820,6 +2010,16 @@ int bh_mgn_factor_device(bh_mgn* M, bh_precond* P, const float* d_vals, const ui
   if (!M || !P || !d_vals) return BH_BAD_ARG;
   if (P->mesh->nV == 0) return BH_OK;
   return mgn_factor<float>(M, P, d_vals, d_fixed_dof, as_stream(stream));
+}
+
+int bh_mgn_debug_latents(bh_mgn* M, int layer, const float** h, const float** e, const float** m) {
+  if (!M || !M->ws || layer < -1 || layer > kLayers) return BH_BAD_ARG;
+  MgnWs w;
+  mgn_ws_carve(M->ws, M->wsV, M->wsP, w);
+  if (h) *h = layer < 0 ? w.xn : w.h[layer];
+  if (e) *e = layer < 0 ? w.fp : w.e[layer];
+  if (m) *m = (layer >= 0 && layer < kLayers) ? w.m[layer] : nullptr;
+  return BH_OK;
 }
 
 int bh_mgn_debug_buffers(bh_mgn* M, const float** s, const float** C, const float** d, const float** O) {

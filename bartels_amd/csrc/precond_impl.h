@@ -526,28 +526,31 @@ __global__ void __launch_bounds__(256) k_copy_blocks(int64_t nb, int64_t ld, con
   for (int q = 0; q < 9; ++q) dst[q * ld + t] = src[q * ld + k];
 }
 
-// the factor in the solve order, one thread per position i (vertex colVerts[i]): the 6 packed inverse
-// entries of L_ii, the lower blocks and, for the upper list, the lower blocks (n, i) of the neighbours
+// the factor in the solve order, one thread per stored value so that the writes are contiguous: the
+// 6 packed inverse entries of L_ii per position i (vertex colVerts[i]), then the 9 values of every
+// lower block in the solve order, then those of the upper lists (the lower blocks (n, i) of the
+// neighbours, in each position's upper-list order)
 __global__ void __launch_bounds__(256)
-    k_factor_solve_order(int nV, const int32_t* __restrict__ colVerts, const float* __restrict__ dinv,
-                         const float* __restrict__ lval, const int32_t* __restrict__ nlptr,
-                         const int32_t* __restrict__ nlsrc, const int32_t* __restrict__ nuptr,
+    k_factor_solve_order(int nV, int64_t nl, const int32_t* __restrict__ colVerts, const float* __restrict__ dinv,
+                         const float* __restrict__ lval, const int32_t* __restrict__ nlsrc,
                          const int32_t* __restrict__ nulink, float* __restrict__ dinvs, float* __restrict__ lv,
                          float* __restrict__ uv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nV) return;
-  const float* a = dinv + 6ll * colVerts[i];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) dinvs[6ll * i + k] = a[k];
-  for (int t = nlptr[i]; t < nlptr[i + 1]; ++t) {
-    const float* b = lval + 9ll * nlsrc[t];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) lv[9ll * t + e] = b[e];
+  int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g < 6ll * nV) {
+    const int i = (int)(g / 6), q = (int)(g - 6ll * i);
+    dinvs[g] = dinv[6ll * colVerts[i] + q];
+    return;
   }
-  for (int t = nuptr[i]; t < nuptr[i + 1]; ++t) {
-    const float* b = lval + 9ll * nlsrc[nulink[t]];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) uv[9ll * t + e] = b[e];
+  g -= 6ll * nV;
+  if (g < 9 * nl) {
+    const int64_t t = g / 9;
+    lv[g] = lval[9ll * nlsrc[t] + (g - 9 * t)];
+    return;
+  }
+  g -= 9 * nl;
+  if (g < 9 * nl) {
+    const int64_t t = g / 9;
+    uv[g] = lval[9ll * nlsrc[nulink[t]] + (g - 9 * t)];
   }
 }
 
@@ -898,9 +901,10 @@ SplitH<V> split_view(const bh_precond* P) {
 // after every change of lval / dinv: the factor in the solve order (both list orders)
 int precond_finalize(bh_precond* P, hipStream_t s) {
   const int nV = P->mesh->nV;
-  if (nV)
-    hipLaunchKernelGGL(k_factor_solve_order, dim3((nV + 255) / 256), dim3(256), 0, s, nV, P->colVerts, P->dinv,
-                       P->lval, P->nlptr, P->nlsrc, P->nuptr, P->nulink, P->dinvs, P->lvs, P->uvs);
+  const int64_t n = 6ll * nV + 18 * P->nlower;
+  if (n)
+    hipLaunchKernelGGL(k_factor_solve_order, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nV, P->nlower,
+                       P->colVerts, P->dinv, P->lval, P->nlsrc, P->nulink, P->dinvs, P->lvs, P->uvs);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }

@@ -168,6 +168,10 @@ class Network:
         if weights is not None:
             self.load(weights)
 
+    def set_inference(self, bf16=True):
+        """the MLPs of inference: BF16 on the matrix cores (the default) or the FP32 training forward"""
+        _lib.check(self.lib.bh_mgn_set_inference(self.h, 1 if bf16 else 0), "bh_mgn_set_inference")
+
     def params(self):
         w = np.empty(self.n, np.float32)
         _lib.check(self.lib.bh_mgn_get_params(self.h, w.ctypes.data_as(ctypes.c_void_p), None), "get_params")

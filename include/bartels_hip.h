@@ -325,6 +325,11 @@ int bh_mgn_destroy(bh_mgn* mgn);
  * bh_mgn_train_step_device).  A training step whose loss is not
  * finite returns BH_NONFINITE and leaves the parameters. */
 int bh_mgn_set_form(bh_mgn* mgn, int form, int probe_is_solution, int energy_loss);
+/* The MLPs of inference (bh_mgn_factor_device and the Newton directions BH_PRECOND_LEARNED /
+ * BH_PRECOND_LEARNED_SGS): precision 1 (the default) BF16 inputs, weights and hidden activations on the
+ * matrix cores with FP32 accumulation and an FP32 residual stream; 0 the FP32 forward that training uses.
+ * Training and the loss always run FP32.  No counterpart in the reference. */
+int bh_mgn_set_inference(bh_mgn* mgn, int precision);
 int bh_mgn_num_params(const bh_mgn* mgn, int64_t* nparams);
 /* host copies of the parameters and of the accumulated gradient (either may be NULL) */
 int bh_mgn_get_params(const bh_mgn* mgn, float* params, float* grad);
@@ -353,6 +358,10 @@ int bh_mgn_factor_device(bh_mgn* mgn, bh_precond* precond, const float* d_vals, 
 /* device views of the last forward pass: scales s (nV), chol of the normalised diagonal blocks
  * (6 per vertex, packed lower), decoder outputs d (6 per vertex) and O (9 per lower pair) */
 int bh_mgn_debug_buffers(bh_mgn* mgn, const float** s, const float** C, const float** d, const float** O);
+/* device views of the last forward pass's latents: layer -1 the node (8 per vertex) and pair (10 per
+ * lower pair) features; layer 0..3 the node latents h (32 per vertex) and pair latents e (32 per pair)
+ * after that many message-passing layers, and (layers 0..2) the messages m the next node update reads */
+int bh_mgn_debug_latents(bh_mgn* mgn, int layer, const float** h, const float** e, const float** m);
 
 typedef struct {
   int model;               /* bh_model, or BH_MODEL_PER_TET with model_ids */

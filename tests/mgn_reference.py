@@ -97,20 +97,29 @@ def features(g, vals, fixed):
     return s, C, xn, fp
 
 
-def mlp(P, prefix, x):
-    return torch.relu(x @ P[prefix + ".W1"] + P[prefix + ".b1"]) @ P[prefix + ".W2"] + P[prefix + ".b2"]
+def bf16(x):
+    """x rounded to BF16 (nearest even), kept in x's dtype"""
+    return x.to(torch.bfloat16).to(x.dtype)
 
 
-def forward(P, g, xn, fp):
+def mlp(P, prefix, x, round_bf16=False):
+    """Linear -> ReLU -> Linear; round_bf16: the MLP's input, weights and hidden activations rounded to
+    BF16 (the biases and the sums stay wide), the arithmetic of the library's BF16 inference"""
+    r = bf16 if round_bf16 else (lambda t: t)
+    hid = torch.relu(r(x) @ r(P[prefix + ".W1"]) + P[prefix + ".b1"])
+    return r(hid) @ r(P[prefix + ".W2"]) + P[prefix + ".b2"]
+
+
+def forward(P, g, xn, fp, round_bf16=False):
     """decoder outputs d (nV x 6) and O (nP x 9)"""
     pv = torch.as_tensor(g.pv)
     pn = torch.as_tensor(g.lcol.astype(np.int64))
-    h = mlp(P, "nenc", xn)
-    e = mlp(P, "penc", fp)
+    h = mlp(P, "nenc", xn, round_bf16)
+    e = mlp(P, "penc", fp, round_bf16)
     for l in range(LAYERS):
-        e = e + mlp(P, f"edge{l}", torch.cat([e, h[pv], h[pn]], dim=1))
+        e = e + mlp(P, f"edge{l}", torch.cat([e, h[pv], h[pn]], dim=1), round_bf16)
         m = torch.zeros_like(h).index_add(0, pv, e).index_add(0, pn, e)
-        h = h + mlp(P, f"node{l}", torch.cat([h, m], dim=1))
+        h = h + mlp(P, f"node{l}", torch.cat([h, m], dim=1), round_bf16)
     d = h @ P["Wdv"] + P["bdv"]
     O = e @ P["Wdo"] + P["bdo"]
     return d, O

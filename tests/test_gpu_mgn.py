@@ -230,6 +230,7 @@ def test_mgn_factor_inference_matches_reference(system):
             o, sh = lay[k]
             w[o:o + int(np.prod(sh))] = 0.02 * rng.standard_normal(int(np.prod(sh)))
         lib.bh_mgn_set_params(M, w.ctypes.data_as(ctypes.c_void_p))
+        assert lib.bh_mgn_set_inference(M, 0) == 0  # the FP32 forward (BF16: the test below)
         assert lib.bh_mgn_factor_device(M, S["P"], vp(S["vals"]), vp(S["dfixed"]), None) == 0
         g = S["g"]
         s, C, xn, fp = (torch.as_tensor(a) for a in S["feats"])
@@ -243,6 +244,114 @@ def test_mgn_factor_inference_matches_reference(system):
         assert lib.bh_precond_apply_device(S["P"], vp(dr), vp(dz), None) == 0
         err = np.linalg.norm(dz.cpu().numpy() - z_ref) / np.linalg.norm(z_ref)
         print(f"learned factor application vs dense solve: {err:.1e}")
+        assert err < 1e-3
+    finally:
+        lib.bh_mgn_destroy(M)
+
+
+def _bf16_stage_check(S, M, n):
+    """every MLP of the last forward against the FP64 torch restatement fed with the kernel's own FP32
+    inputs of that stage, rounded to BF16 as the kernel rounds them: the remaining differences are
+    the FP32 accumulation order and the rare hidden activation it moves across a BF16 rounding
+    boundary"""
+    lib, g = S["lib"], S["g"]
+    w = np.empty(n, np.float32)
+    lib.bh_mgn_get_params(M, w.ctypes.data_as(ctypes.c_void_p), None)
+    Pw = R.unflatten(torch.tensor(w, dtype=torch.float64))
+    nV, nP = g.nV, g.nP
+
+    def lat(layer):
+        p = [ctypes.c_void_p() for _ in range(3)]
+        assert lib.bh_mgn_debug_latents(M, layer, *[ctypes.byref(x) for x in p]) == 0
+        wh, we = (8, 10) if layer < 0 else (32, 32)
+        h = torch.tensor(d2h(p[0].value, wh * nV, np.float32).reshape(nV, wh), dtype=torch.float64)
+        e = torch.tensor(d2h(p[1].value, we * nP, np.float32).reshape(nP, we), dtype=torch.float64)
+        m = (torch.tensor(d2h(p[2].value, 32 * nV, np.float32).reshape(nV, 32), dtype=torch.float64)
+             if p[2].value else None)
+        return h, e, m
+
+    pv = torch.as_tensor(g.pv)
+    pn = torch.as_tensor(g.lcol.astype(np.int64))
+    xn, fp, _ = lat(-1)
+    h, e, _ = lat(0)
+    worst = {}
+
+    def check(name, got, want):
+        err = (torch.linalg.norm(got - want) / torch.linalg.norm(want)).item()
+        worst[name] = err
+        assert err < 1e-4, (name, err)
+
+    check("node encoder", h, R.mlp(Pw, "nenc", xn, True))
+    check("pair encoder", e, R.mlp(Pw, "penc", fp, True))
+    for l in range(R.LAYERS):
+        h1, e1, _ = lat(l + 1)
+        _, _, m = lat(l)
+        check(f"edge {l}", e1, e + R.mlp(Pw, f"edge{l}", torch.cat([e, h[pv], h[pn]], dim=1), True))
+        check(f"messages {l}", m, torch.zeros_like(h).index_add(0, pv, e1).index_add(0, pn, e1))
+        check(f"node {l}", h1, h + R.mlp(Pw, f"node{l}", torch.cat([h, m], dim=1), True))
+        h, e = h1, e1
+    return worst
+
+
+def test_mgn_bf16_inference_matches_reference(system):
+    """BF16 inference on the matrix cores (the default of bh_mgn_factor_device) with the shipped trained
+    weights: each MLP against the BF16-rounding restatement on the kernel's own inputs; the decoder
+    outputs of the whole network against the unrounded FP64 restatement (the size of the BF16 effect,
+    reported) and the FP32 forward against the same (tight); the factor written from the BF16 outputs
+    against a dense solve"""
+    import os
+
+    import bartels_amd
+
+    S = system
+    lib = S["lib"]
+    M, n = make_mgn(lib, seed=5)
+    try:
+        w = np.fromfile(os.path.join(os.path.dirname(bartels_amd.__file__), "mgn_weights.f32"), np.float32)
+        assert w.size == n
+        lib.bh_mgn_set_params(M, w.ctypes.data_as(ctypes.c_void_p))
+        assert lib.bh_mgn_set_form(M, 2, 0, 3) == 0  # the tied factor form the weights were trained in
+        g = S["g"]
+        outs = {}
+        for prec in (0, 1):
+            assert lib.bh_mgn_set_inference(M, prec) == 0
+            assert lib.bh_mgn_factor_device(M, S["P"], vp(S["vals"]), vp(S["dfixed"]), None) == 0
+            bufs = [ctypes.c_void_p() for _ in range(4)]
+            lib.bh_mgn_debug_buffers(M, *[ctypes.byref(p) for p in bufs])
+            outs[prec] = (d2h(bufs[2].value, 6 * g.nV, np.float32).reshape(-1, 6),
+                          d2h(bufs[3].value, 9 * g.nP, np.float32).reshape(-1, 9))
+        # the BF16 forward is the last one: its stages
+        worst = _bf16_stage_check(S, M, n)
+        print("BF16 MLP stages vs rounding restatement: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+        s, C, xn, fp = (torch.as_tensor(a) for a in S["feats"])
+        Pw = R.unflatten(torch.tensor(w, dtype=torch.float64))
+        d, O = R.forward(Pw, g, xn, fp)
+        _, Lk = R.factor(C, d, O, g, fp, add=True)  # the O buffer ends as Hn_vn Lhat_nn^-T + O_k
+        d_f, O_f = d.detach().numpy(), Lk.detach().numpy().reshape(-1, 9)
+
+        def rel(a, b):
+            return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+        (d_b, O_b), (d_32, O_32) = outs[1], outs[0]
+        e_b = max(rel(d_b, d_f), rel(O_b, O_f))
+        e_32 = max(rel(d_32, d_f), rel(O_32, O_f))
+        print(f"decoder outputs vs unrounded FP64 restatement: BF16 inference {e_b:.1e}, FP32 inference {e_32:.1e}")
+        assert e_32 < 1e-4 and e_b < 0.1
+        # the factor written from the BF16 outputs (the device's own d and tied pair blocks)
+        T = torch.zeros(g.nV, 3, 3, dtype=torch.float64)
+        dd = torch.tensor(d_b, dtype=torch.float64)
+        T[:, 0, 0], T[:, 1, 0], T[:, 1, 1] = torch.exp(dd[:, 0]), dd[:, 1], torch.exp(dd[:, 2])
+        T[:, 2, 0], T[:, 2, 1], T[:, 2, 2] = dd[:, 3], dd[:, 4], torch.exp(dd[:, 5])
+        Lvv = torch.bmm(C, T).numpy()
+        Ld = R.dense_factor(g, S["feats"][0], Lvv, O_b.astype(np.float64).reshape(-1, 3, 3), S["fixed"])
+        rng = np.random.default_rng(4)
+        r = rng.standard_normal(3 * g.nV)
+        z_ref = np.linalg.solve(Ld @ Ld.T, r)
+        dr = torch.tensor(r, device="cuda")
+        dz = torch.empty_like(dr)
+        assert lib.bh_precond_apply_device(S["P"], vp(dr), vp(dz), None) == 0
+        err = np.linalg.norm(dz.cpu().numpy() - z_ref) / np.linalg.norm(z_ref)
+        print(f"BF16 learned factor application vs dense solve: {err:.1e}")
         assert err < 1e-3
     finally:
         lib.bh_mgn_destroy(M)
