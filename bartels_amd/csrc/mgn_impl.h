@@ -191,26 +191,38 @@ __global__ void __launch_bounds__(256)
   x[7] = (float)(bptr[v + 1] - bptr[v] - 1) * (1.f / 16.f);
 }
 
+// the rows [k0, k0 + cnt) of a row-major array of W floats per row, written from LDS (row t of the block
+// at stage[W t]) as one contiguous run
+template <int W>
+__device__ __forceinline__ void store_rows(const float* __restrict__ stage, float* __restrict__ dst, int64_t k0,
+                                           int cnt) {
+  float* out = dst + W * k0;
+  for (int i = threadIdx.x; i < W * cnt; i += blockDim.x) out[i] = stage[i];
+}
+
+// pair features, one thread per lower pair k = (v, n) (v = pv[k]); the 10 values of the workgroup's 256
+// pairs leave through LDS as one contiguous run
 template <typename V>
 __global__ void __launch_bounds__(256)
-    k_mgn_pairfeat(int nV, const int32_t* __restrict__ bptr, const V* __restrict__ vals,
-                   const uint8_t* __restrict__ fixed, const int32_t* __restrict__ lptr,
-                   const int32_t* __restrict__ lcol, const int32_t* __restrict__ lslot, const float* __restrict__ s,
-                   float* __restrict__ fp) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nV) return;
-  const float sv = s[v];
-  for (int k = lptr[v]; k < lptr[v + 1]; ++k) {
-    const int n = lcol[k];
+    k_mgn_pairfeat(int64_t nP, const int32_t* __restrict__ pv, const int32_t* __restrict__ bptr,
+                   const V* __restrict__ vals, const uint8_t* __restrict__ fixed, const int32_t* __restrict__ lcol,
+                   const int32_t* __restrict__ lslot, const float* __restrict__ s, float* __restrict__ fp) {
+  __shared__ float stage[256 * kPairIn];
+  const int64_t k0 = blockIdx.x * (int64_t)blockDim.x;
+  const int64_t k = k0 + threadIdx.x;
+  if (k < nP) {
+    const int v = pv[k], n = lcol[k];
     float A[9];
     mgn_block(vals, bptr, v, lslot[k], fixed, n, A);
-    const float sn = s[n];
+    const float sv = s[v], sn = s[n];
     const float inv = 1.f / (sv * sn);
-    float* f = fp + (int64_t)kPairIn * k;
+    float* f = stage + kPairIn * threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 9; ++i) f[i] = A[i] * inv;
     f[9] = logf(sv / sn);
   }
+  __syncthreads();
+  store_rows<kPairIn>(stage, fp, k0, (int)min<int64_t>(blockDim.x, nP - k0));
 }
 
 __global__ void __launch_bounds__(256) k_mgn_pair_owner(int nV, const int32_t* __restrict__ lptr,
@@ -1156,43 +1168,50 @@ __global__ void __launch_bounds__(256)
                       const float* __restrict__ d, const float* __restrict__ s, int tied, float* __restrict__ O,
                       float* __restrict__ lval) {
   __shared__ float sW[kW * 9 + 9];
+  __shared__ float stage[2][256 * 9];  // O and lval rows of the workgroup's pairs
   for (int i = threadIdx.x; i < kW * 9; i += blockDim.x) sW[i] = Wdo[i];
   if (threadIdx.x < 9) sW[kW * 9 + threadIdx.x] = bdo[threadIdx.x];
   __syncthreads();
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= nP) return;
-  float o[9];
+  const int64_t k0 = blockIdx.x * (int64_t)blockDim.x;
+  const int64_t k = k0 + threadIdx.x;
+  if (k < nP) {
+    float o[9];
 #pragma unroll
-  for (int j = 0; j < 9; ++j) o[j] = sW[kW * 9 + j];
-  const float4* x4 = reinterpret_cast<const float4*>(e + k * kW);
+    for (int j = 0; j < 9; ++j) o[j] = sW[kW * 9 + j];
+    const float4* x4 = reinterpret_cast<const float4*>(e + k * kW);
 #pragma unroll
-  for (int q = 0; q < kW / 4; ++q) {
-    const float4 x = x4[q];
-    const float xs[4] = {x.x, x.y, x.z, x.w};
+    for (int q = 0; q < kW / 4; ++q) {
+      const float4 x = x4[q];
+      const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int j = 0; j < 9; ++j) o[j] += xs[u] * sW[(4 * q + u) * 9 + j];
+        for (int j = 0; j < 9; ++j) o[j] += xs[u] * sW[(4 * q + u) * 9 + j];
+    }
+    const int v = pv[k], n = lcol[k];
+    if (tied) {
+      float X[9], Xi[9];
+      mgn_lvv(C + 6ll * n, d + 6ll * n, X);
+      ltri_inv(X, Xi);
+      const float* B = fp + (int64_t)kPairIn * k;
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          o[3 * l + c] += B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
+    }
+    const float sv = s[v];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      stage[0][9 * threadIdx.x + i] = o[i];
+      const bool fz = fixed && (fixed[3ll * v + i / 3] || fixed[3ll * n + i % 3]);
+      stage[1][9 * threadIdx.x + i] = fz ? 0.f : sv * o[i];
+    }
   }
-  const int v = pv[k], n = lcol[k];
-  if (tied) {
-    float X[9], Xi[9];
-    mgn_lvv(C + 6ll * n, d + 6ll * n, X);
-    ltri_inv(X, Xi);
-    const float* B = fp + (int64_t)kPairIn * k;
-#pragma unroll
-    for (int l = 0; l < 3; ++l)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        o[3 * l + c] += B[3 * l] * Xi[3 * c] + B[3 * l + 1] * Xi[3 * c + 1] + B[3 * l + 2] * Xi[3 * c + 2];
-  }
-  const float sv = s[v];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    O[9 * k + i] = o[i];
-    const bool fz = fixed && (fixed[3ll * v + i / 3] || fixed[3ll * n + i % 3]);
-    lval[9 * k + i] = fz ? 0.f : sv * o[i];
-  }
+  __syncthreads();
+  const int cnt = (int)min<int64_t>(blockDim.x, nP - k0);
+  store_rows<9>(stage[0], O, k0, cnt);
+  store_rows<9>(stage[1], lval, k0, cnt);
 }
 
 // The factor of H for bh_precond: L_vn = s_v O_k, inverse of L_vv = s_v Lhat_vv (packed lower).
@@ -1296,9 +1315,10 @@ int mgn_forward(bh_mgn* M, const bh_precond* P, const V* vals, const uint8_t* fi
   const float* W = M->w;
   hipLaunchKernelGGL((k_mgn_prep<V>), g256(nV), dim3(256), 0, s, nV, m->bptr, vals, fixed, P->dslot, P->lptr, P->lcol,
                      P->lslot, w.s, w.C, w.xn);
-  hipLaunchKernelGGL((k_mgn_pairfeat<V>), g256(nV), dim3(256), 0, s, nV, m->bptr, vals, fixed, P->lptr, P->lcol,
-                     P->lslot, w.s, w.fp);
   hipLaunchKernelGGL(k_mgn_pair_owner, g256(nV), dim3(256), 0, s, nV, P->lptr, M->pv);
+  if (nP)
+    hipLaunchKernelGGL((k_mgn_pairfeat<V>), g256(nP), dim3(256), 0, s, nP, (const int32_t*)M->pv, m->bptr, vals, fixed,
+                       P->lcol, P->lslot, w.s, w.fp);
   BH_CHECK(hipGetLastError());
   auto mlp = [&](auto inTag, int64_t rows, Seg3 S, const MlpOff& o, const float* resid, float* hid, float* out) {
     constexpr int IN = decltype(inTag)::value;

@@ -911,16 +911,6 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   u[pos ? 3ll * pos[v] + l : i] += z;
 }
 
-// monomial c of d (one entry of monomials()) as a product of two selected factors: the lanes of a
-// wave take different c, and a switch compiled to branches that the wave ran one after another
-__device__ __forceinline__ double monomial(const double d[3], int c) {
-  const int i1 = c == 0 ? -1 : c <= 3 ? c - 1 : c <= 6 ? c - 4 : c == 8 ? 1 : 0;
-  const int i2 = c <= 3 ? -1 : c <= 6 ? c - 4 : c == 7 ? 1 : 2;
-  const double a = i1 < 0 ? 1.0 : i1 == 0 ? d[0] : i1 == 1 ? d[1] : d[2];
-  const double e = i2 < 0 ? 1.0 : i2 == 0 ? d[0] : i2 == 1 ? d[1] : d[2];
-  return a * e;
-}
-
 // the vector kernel of block-Jacobi PCG plus the partials of Z^T r per workgroup (r of fixed DOFs
 // is zero: b is masked and the fixed rows of the operator are the identity).  Z^T r of the
 // workgroup is the 3 x 10 sum of r_v (x) m(d_v): r and q of the 64 vertices meet in LDS, six lanes
@@ -936,9 +926,11 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   static_assert(kModes * kSub <= 3 * kJacVerts, "six lanes per coarse entry");
   __shared__ double rs[3 * kJacVerts];
   __shared__ double qs[3 * kJacVerts];
+  __shared__ double ms[kJacVerts][kMono + 1];  // the monomials of each vertex, formed once
   __shared__ double red[kModes][kSub];
   __shared__ double sh[3][2];
   if (!start && cg_skip(st)) return;
+  const int I = (blockIdx.x * kJacVerts) / per;  // the workgroup's aggregate
   const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
   const int64_t n = 3ll * nV;
   double ri = 0.0;
@@ -962,6 +954,13 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   rs[threadIdx.x] = ri;
   qs[threadIdx.x] = (i < n) ? q[i] : 0.0;
   __syncthreads();
+  if (threadIdx.x < kJacVerts) {
+    double d[3], mm[kMono];
+    agg_coords(cent, I, qs[3 * threadIdx.x], qs[3 * threadIdx.x + 1], qs[3 * threadIdx.x + 2], d);
+    monomials(d, mm);
+#pragma unroll
+    for (int c = 0; c < kMono; ++c) ms[threadIdx.x][c] = mm[c];
+  }
   double ru = 0.0, rr = 0.0;
   if (i < n) {
     const int l = (int)(i % 3);
@@ -987,13 +986,8 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   if (threadIdx.x < kModes * kSub) {
     const int Q = threadIdx.x / kSub, j = threadIdx.x % kSub;
     const int l = Q / kMono, c = Q % kMono;
-    const int I = (blockIdx.x * kJacVerts) / per;  // the workgroup's aggregate
     double a = 0.0;
-    for (int vv = j; vv < kJacVerts; vv += kSub) {
-      double d[3];
-      agg_coords(cent, I, qs[3 * vv], qs[3 * vv + 1], qs[3 * vv + 2], d);
-      a += rs[3 * vv + l] * monomial(d, c);  // r is zero beyond the mesh
-    }
+    for (int vv = j; vv < kJacVerts; vv += kSub) a += rs[3 * vv + l] * ms[vv][c];  // r is zero beyond the mesh
     red[Q][j] = a;
   }
   __syncthreads();
@@ -1028,23 +1022,28 @@ __global__ void __launch_bounds__(3 * kJacVerts)
   constexpr int kSub = 6;
   __shared__ double rs[3 * kJacVerts];
   __shared__ double qs[3 * kJacVerts];
+  __shared__ double ms[kJacVerts][kMono + 1];  // the monomials of each vertex, formed once
   __shared__ double red[kModes][kSub];
   if (st && cg_skip(st)) return;
+  const int I = (blockIdx.x * kJacVerts) / per;
   const int64_t i = blockIdx.x * (int64_t)(3 * kJacVerts) + threadIdx.x;
   const bool in = i < 3ll * nV;
   rs[threadIdx.x] = in ? r[3ll * pos[i / 3] + i % 3] : 0.0;
   qs[threadIdx.x] = in ? q[i] : 0.0;
   __syncthreads();
+  if (threadIdx.x < kJacVerts) {
+    double d[3], mm[kMono];
+    agg_coords(cent, I, qs[3 * threadIdx.x], qs[3 * threadIdx.x + 1], qs[3 * threadIdx.x + 2], d);
+    monomials(d, mm);
+#pragma unroll
+    for (int c = 0; c < kMono; ++c) ms[threadIdx.x][c] = mm[c];
+  }
+  __syncthreads();
   if (threadIdx.x < kModes * kSub) {
     const int Q = threadIdx.x / kSub, j = threadIdx.x % kSub;
     const int l = Q / kMono, c = Q % kMono;
-    const int I = (blockIdx.x * kJacVerts) / per;
     double a = 0.0;
-    for (int vv = j; vv < kJacVerts; vv += kSub) {
-      double d[3];
-      agg_coords(cent, I, qs[3 * vv], qs[3 * vv + 1], qs[3 * vv + 2], d);
-      a += rs[3 * vv + l] * monomial(d, c);
-    }
+    for (int vv = j; vv < kJacVerts; vv += kSub) a += rs[3 * vv + l] * ms[vv][c];
     red[Q][j] = a;
   }
   __syncthreads();
