@@ -1238,11 +1238,28 @@ __global__ void __launch_bounds__(kGatherThreads)
       for (int l = 0; l < 3; ++l)
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) {
-          T x = (l == kk) ? tolPos : T(0);
+          T x;
+          if constexpr (sizeof(T) == 8) {
+            x = (l == kk) ? tolPos : T(0);
 #pragma unroll
-          for (int pp = 0; pp < 3; ++pp)
+            for (int pp = 0; pp < 3; ++pp)
 #pragma unroll
-            for (int rr = 0; rr < 3; ++rr) x += P[pp][rr] * S[sym9(3 * pp + l, 3 * rr + kk)];
+              for (int rr = 0; rr < 3; ++rr) x += P[pp][rr] * S[sym9(3 * pp + l, 3 * rr + kk)];
+          } else {
+            // FP32: Q[lo]'s contraction first, then Q[hi]'s (the rounding of rec_block).  The FP32
+            // operator's soft modes are sensitive to it: forming Q[lo][p] Q[hi][r] first doubled the
+            // precision-form Newton step's distance from the FP64 solution (5.6e-6 -> 1.2e-5)
+            T t[3];
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+              T sum = T(0);
+#pragma unroll
+              for (int pp = 0; pp < 3; ++pp) sum += ql[pp] * S[sym9(3 * pp + l, 3 * rr + kk)];
+              t[rr] = sum;
+            }
+            x = qh[0] * t[0] + qh[1] * t[1] + qh[2] * t[2];
+            if (l == kk) x += tolPos;
+          }
           lds_add(&dst[l * rs + kk * cs], x);
         }
       selfRank = (int)((rA >> (8 * a)) & 0xffu);
