@@ -297,12 +297,18 @@ def train_batched(systems, out, epochs=30, batch=6, lr=3e-4, probes=8, form="tie
 
 
 def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="sgs", coarse=0, log=print,
-             grid=(128, 32, 32)):
+             grid=(128, 32, 32), twin=False, bf16=True):
     """held-out problems (32x32x128 by default): PCG iterations and time to 1e-6 over the first Newton steps,
     block-Jacobi PCG, block symmetric Gauss-Seidel and the learned preconditioner (all in the precision
     form); coarse = 1 adds the two-level coarse correction to the learned factor (and evaluates two-level
-    block Jacobi beside it).  Every solve reports its inner PCG failures (after the Jacobi fallback)."""
+    block Jacobi beside it).  Every solve reports its inner PCG failures (after the Jacobi fallback).
+    twin: also the same form with zero decoders (the untrained network); bf16: the network's inference
+    precision (BF16 on the matrix cores, else FP32)."""
     net = Network(weights=weights, form=form)
+    net.set_inference(bf16)
+    net0 = Network(form=form) if twin else None
+    if net0 is not None:
+        net0.set_inference(bf16)
     rows = []
     for i in range(problems):
         p = random_problem(grid, seed=90000 + seed + 13 * i)
@@ -315,6 +321,10 @@ def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="
                    sgs_pcg=rb.pcg_iterations_total, jacobi_s=wj, learned_s=wl, sgs_s=wb,
                    fallbacks=rl.pcg_fallbacks, learned_failures=rl.pcg_failures, newton=rl.iterations,
                    learned_rel_gradient=rl.gradient_norm / rl.gradient_norm0 if rl.gradient_norm0 else 0.0)
+        if net0 is not None:
+            _, rt, _, wt = S.newton(net0.precond, mgn=net0.h, precision=1, max_it=newton_its, pcg_max=pcg_max,
+                                    coarse=coarse)
+            row.update(twin_pcg=rt.pcg_iterations_total, twin_s=wt, twin_fallbacks=rt.pcg_fallbacks)
         if coarse:
             _, r2, _, w2 = S.newton(7, precision=1, max_it=newton_its, pcg_max=pcg_max)
             _, r3, _, w3 = S.newton(2, precision=1, max_it=newton_its, pcg_max=pcg_max, coarse=1)
@@ -324,6 +334,8 @@ def evaluate(weights, problems=10, seed=777, newton_its=3, pcg_max=20000, form="
         rows.append(row)
         log(json.dumps(rows[-1]))
     net.close()
+    if net0 is not None:
+        net0.close()
     return rows
 
 
@@ -358,6 +370,8 @@ def main():
     b.add_argument("--form", choices=sorted(FORMS), default=DEFAULT_FORM)
     b.add_argument("--coarse", type=int, default=1)
     b.add_argument("--seed", type=int, default=555)
+    b.add_argument("--twin", action="store_true", help="also the untrained network (zero decoders)")
+    b.add_argument("--fp32", action="store_true", help="FP32 inference instead of BF16 on the matrix cores")
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)  # noqa: E731
     if args.cmd == "fit":
@@ -372,10 +386,20 @@ def main():
               form=args.form, probe_solution=args.probe == "solution", log=log)
     else:
         rows = evaluate(args.weights, args.problems, form=args.form, coarse=args.coarse, seed=args.seed,
-                        newton_its=2, log=log)
-        keys = ["jacobi_pcg", "sgs_pcg", "learned_pcg", "learned_s"] + (
-            ["jacobi2l_pcg", "bic0_2l_pcg", "jacobi2l_s", "bic0_2l_s"] if args.coarse else [])
-        log(json.dumps({"mean_" + k: float(np.mean([r[k] for r in rows])) for k in keys}))
+                        newton_its=2, log=log, twin=args.twin, bf16=not args.fp32)
+        keys = ["jacobi_pcg", "sgs_pcg", "learned_pcg", "learned_s", "jacobi_s", "sgs_s"] + (
+            ["jacobi2l_pcg", "bic0_2l_pcg", "jacobi2l_s", "bic0_2l_s"] if args.coarse else []) + (
+            ["twin_pcg", "twin_s"] if args.twin else [])
+        summary = {"problems": len(rows), "inference": "fp32" if args.fp32 else "bf16"}
+        summary.update({"mean_" + k: float(np.mean([r[k] for r in rows])) for k in keys})
+        summary["learned_fallbacks"] = int(sum(r["fallbacks"] for r in rows))
+        summary["learned_failures"] = int(sum(r["learned_failures"] for r in rows))
+        if args.twin:
+            summary["learned_fewer_than_twin"] = int(sum(r["learned_pcg"] < r["twin_pcg"] for r in rows))
+        if args.coarse:
+            summary["learned_fewer_than_jacobi2l"] = int(sum(r["learned_pcg"] < r["jacobi2l_pcg"] for r in rows))
+            summary["learned_faster_than_jacobi2l"] = int(sum(r["learned_s"] < r["jacobi2l_s"] for r in rows))
+        log(json.dumps(summary))
 
 
 if __name__ == "__main__":
