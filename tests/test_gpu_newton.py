@@ -64,7 +64,7 @@ def test_newton_config1_matches_direct_solve(oracle):
 
 
 def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_max=20000, gtol=1e-7, max_it=30,
-            handle=None, fallback=1, psd=False, aggregates=0, coarse=0):
+            handle=None, fallback=1, psd=False, aggregates=0, coarse=0, psd_tol=1e-3):
     import torch
 
     from bartels_amd import _lib
@@ -83,7 +83,7 @@ def _newton(mesh, p, q0, fixed, f, precond=0, precision=0, pcg_tol=1e-10, pcg_ma
     o.precond = precond
     o.element_precision = precision
     o.jacobi_fallback = fallback
-    o.psd = _lib.BhPsd(1 if psd else 0, 1e-3)
+    o.psd = _lib.BhPsd(1 if psd else 0, psd_tol)
     if handle is not None:
         o.precond_handle = handle
     dq = torch.tensor(q0, device="cuda")
@@ -164,6 +164,37 @@ def test_newton_precision_form_and_preconditioners(gravity_problem):
         assert results["fp32_bic0_2l"][0] < 1e-5
         assert results["fp64_bic0_2l"][2] < results["fp64_bic0"][2]
         assert results["fp32_bic0_2l"][2] < results["fp32_bic0"][2]
+    finally:
+        lib.bh_precond_destroy(P)
+
+
+@pytest.mark.parametrize("tol", [0.0, -1e-3])
+def test_precision_form_nonpositive_psd_tol(gravity_problem, tol):
+    """simple_psd_fix with tol <= 0 adds nothing to the translation modes, so the precision form's rebuilt
+    FP64 diagonal (exact row sums plus max(tol, 0) per incident tet) must add nothing either: the FP32
+    Newton solves with block Jacobi and with IC(0) plus the coarse correction reach the oracle's
+    equilibrium as the FP64 solve with the same clamp does.  (The clamp makes the iteration a modified
+    Newton method, linearly convergent, and the reference's Armijo test, f(x + a d) <= f(x) + c g.d
+    without the step length, stops resolving the energy decrease at rounding level: every solve settles
+    about 2e-8 from the equilibrium before the 1e-7 relative gradient, so 30 iterations are compared.)"""
+    p, mesh, q0, fixed, f, q_or = gravity_problem
+    from bartels_amd import _lib
+
+    u = np.linalg.norm(q_or - q0)
+    lib = _lib.load()
+    P = ctypes.c_void_p()
+    assert lib.bh_precond_create(mesh.handle, ctypes.byref(P)) == 0
+    try:
+        res = {}
+        for name, kw in (("fp64_jacobi", dict()), ("fp32_jacobi", dict(precision=1)),
+                         ("fp32_bic0_2l", dict(precision=1, precond=2, handle=P, coarse=1))):
+            q, rep, _ = _newton(mesh, p, q0, fixed, f, psd=True, psd_tol=tol, **kw)
+            res[name] = (np.linalg.norm(q - q_or) / u, rep.pcg_fallbacks)
+            print(f"tol {tol} {name}: displacement {res[name][0]:.2e}, newton its {rep.iterations}, "
+                  f"fallbacks {rep.pcg_fallbacks}")
+        assert res["fp64_jacobi"][0] < 1e-7
+        assert res["fp32_jacobi"][0] < 1e-5 and res["fp32_bic0_2l"][0] < 1e-5
+        assert res["fp32_bic0_2l"][1] == 0
     finally:
         lib.bh_precond_destroy(P)
 
