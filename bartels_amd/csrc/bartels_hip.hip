@@ -845,9 +845,10 @@ constexpr int kPsdPending = 1;
 // stressed tet) the record becomes M + W W^T, W's columns from inverse iteration without an
 // eigenvector matrix, which keeps the pass at half the registers of the full eigensolver; tets with
 // more are marked pending.  FP32: a Cholesky test of M - tol I, then the full eigensolver on the
-// tets that fail it.  The records of the workgroup's 64 tets are built in LDS (M parked there while the
-// eigensolver overwrites it, each column added as it is formed) and leave as one contiguous,
-// coalesced block.
+// tets that fail it.  FP64 forms M again after the eigensolver (which overwrites it) and adds W W^T in
+// registers; the records leave through LDS in 16-entry rounds, one cache line per tet and round.  FP32
+// builds the workgroup's 64 records in LDS (M parked there during the eigensolver) and writes them as
+// one contiguous block.
 constexpr int kRecPitch = 65;  // LDS pitch of one record entry over the 64 lanes (odd: no bank conflicts)
 
 // Record layout in HBM.  A tet's record (M' packed, 45 values) takes kRecW = 48 elements and its Q
@@ -886,70 +887,114 @@ __global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
                 int32_t* __restrict__ count, T* __restrict__ rec) {
-  // the records in LDS in their HBM element order (entry o of the lane's tet at o * kRecPitch + lane)
-  __shared__ T stash[kRecW * kRecPitch];
+  // FP64: the correction columns W (27 values per tet), then one 16-entry round of the workgroup's
+  // records at a time (14 KB, so that the registers, not LDS, bound the occupancy); FP32: the whole
+  // records, M parked there during the eigensolver
+  constexpr int kRows = sizeof(T) == 8 ? 27 : kRecW;
+  __shared__ T stash[kRows * kRecPitch];  // entry o of the lane's tet at o * kRecPitch + lane
   const int lane = (int)threadIdx.x;
   const int e0 = (int)blockIdx.x * 64;
   const int e = e0 + lane;
   bool clamped = false;
-#pragma unroll
-  for (int o = 0; o < kRecW; ++o)
-    if (rec_value<T>(o) < 0) stash[o * kRecPitch + lane] = T(0);
-  if (e < m.nT) {
+  if constexpr (sizeof(T) == 8) {
+    // The QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the rotational
+    // modes put nearly every tet of a mildly stressed mesh below it).  The eigensolver overwrites M, so
+    // M is formed again afterwards (the same arithmetic: the same bits) and the columns of W, kept in
+    // registers, are added to it in the order the solver produced them.
     T M[45];
-    reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+    if (e < m.nT) {
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+      const int k = psd_correction_low(M, tol, [&](int j, const T* w) {
 #pragma unroll
-    for (int i = 0; i < 45; ++i) stash[rec_off<T>(i) * kRecPitch + lane] = M[i];
-    int fl = 0;
-    if constexpr (sizeof(T) == 8) {
-      // the QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the
-      // rotational modes put nearly every tet of a mildly stressed mesh below it)
-      const int k = psd_correction_low(M, tol, [&](int, const T* w) {
+        for (int i = 0; i < 9; ++i) stash[(9 * j + i) * kRecPitch + lane] = w[i];  // W's column j
+      });
+      clamped = k > 0;
+      // the element's inputs are loaded again (L1 / L2 hits) after the eigensolver rather than kept live
+      // through it: the tet index of the second formation is made to depend on the solver's result, so
+      // the compiler cannot hoist those loads above it
+      int e2 = e;
+      __asm__ volatile("" : "+v"(e2) : "v"(k));
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e2, stretchRef != 0, M);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c < k && k <= 3) {
+          T w[9];
+#pragma unroll
+          for (int i = 0; i < 9; ++i) w[i] = stash[(9 * c + i) * kRecPitch + lane];
+#pragma unroll
+          for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = i; j < 9; ++j) M[sym9(i, j)] += w[i] * w[j];
+        }
+      flag[e] = (uint8_t)(k > 3 ? kPsdPending : 0);  // pending records are rewritten by k_psd_full
+    }
+    count_flagged(clamped, count);
+    // three rounds of 16 entries: round r covers elements [16 r, 16 r + 16) of each 48-element record,
+    // one 128-byte line per tet; lane L of store i writes element (64 i + L) mod 16 of tet (64 i + L) / 16
+#pragma unroll
+    for (int r = 0; r < kRecW / 16; ++r) {
+      __syncthreads();  // (r = 0: every lane has read its W columns)
+#pragma unroll
+      for (int o = 0; o < 16; ++o) {
+        const int v = rec_value<T>(16 * r + o);
+        stash[o * kRecPitch + lane] = (e < m.nT && v >= 0) ? M[v < 0 ? 0 : v] : T(0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int idx = lane + 64 * i, t = idx >> 4, o = idx & 15;
+        if (e0 + t < m.nT)
+          __builtin_nontemporal_store(stash[o * kRecPitch + t], &rec[(int64_t)kRecW * (e0 + t) + 16 * r + o]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int o = 0; o < kRecW; ++o)
+      if (rec_value<T>(o) < 0) stash[o * kRecPitch + lane] = T(0);
+    if (e < m.nT) {
+      T M[45];
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+#pragma unroll
+      for (int i = 0; i < 45; ++i) stash[rec_off<T>(i) * kRecPitch + lane] = M[i];
+      if (!shifted_pd(M, tol)) {
+        // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
+        // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
+        // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
+        clamped = true;
+        T dm[45];
+        const int k = psd_correction(M, tol, dm);
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
-          for (int j = i; j < 9; ++j) stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] += w[i] * w[j];
-      });
-      clamped = k > 0;
-      if (k > 3) fl = kPsdPending;
-    } else if (!shifted_pd(M, tol)) {
-      // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
-      // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
-      // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
-      clamped = true;
-      T dm[45];
-      const int k = psd_correction(M, tol, dm);
+          for (int j = i; j < 9; ++j) {
+            T acc = stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane];
+            if (k > 3) {
+              acc += dm[sym9(i, j)];
+            } else {
 #pragma unroll
-      for (int i = 0; i < 9; ++i)
-#pragma unroll
-        for (int j = i; j < 9; ++j) {
-          T acc = stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane];
-          if (k > 3) {
-            acc += dm[sym9(i, j)];
-          } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
+              for (int c = 0; c < 3; ++c)
+                if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
+            }
+            stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] = acc;
           }
-          stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] = acc;
-        }
+      }
+      flag[e] = 0;
     }
-    flag[e] = (uint8_t)fl;
-  }
-  count_flagged(clamped, count);
-  __syncthreads();
-  // element idx = kRecW t + o of the block: (t, o) advanced by 64 = kRecW + 16 per step, no division
-  const int n = kRecW * min(64, m.nT - e0);
-  T* out = rec + (int64_t)kRecW * e0;
-  static_assert(kRecW == 48, "write-out stride assumes 64 = kRecW + 16");
-  int t = lane >= kRecW ? 1 : 0, o = lane - kRecW * t;
-  for (int idx = lane; idx < n; idx += 64) {
-    __builtin_nontemporal_store(stash[o * kRecPitch + t], &out[idx]);  // streamed: read back by the gather
-    o += 16;
-    t += 1;
-    if (o >= kRecW) {
-      o -= kRecW;
+    count_flagged(clamped, count);
+    __syncthreads();
+    // element idx = kRecW t + o of the block: (t, o) advanced by 64 = kRecW + 16 per step, no division
+    const int n = kRecW * min(64, m.nT - e0);
+    T* out = rec + (int64_t)kRecW * e0;
+    static_assert(kRecW == 48, "write-out stride assumes 64 = kRecW + 16");
+    int t = lane >= kRecW ? 1 : 0, o = lane - kRecW * t;
+    for (int idx = lane; idx < n; idx += 64) {
+      __builtin_nontemporal_store(stash[o * kRecPitch + t], &out[idx]);  // streamed: read back by the gather
+      o += 16;
       t += 1;
+      if (o >= kRecW) {
+        o -= kRecW;
+        t += 1;
+      }
     }
   }
 }
