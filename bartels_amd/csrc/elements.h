@@ -968,13 +968,18 @@ __host__ __device__ inline int psd_correction(T a[45], T tol, T* out) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) k += (d[i] < tol) ? 1 : 0;
   if (k <= 3) {
+    // column j of out for the j-th eigenvalue below tol; the column is chosen by predication over
+    // compile-time indices (a run-time index would put out in scratch memory)
     int j = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       if (d[i] < tol) {
         const T w = sqrt(tol - d[i]);
 #pragma unroll
-        for (int r = 0; r < 9; ++r) out[9 * j + r] = w * Z[r + 9 * i];
+        for (int jj = 0; jj < 3; ++jj)
+          if (j == jj)
+#pragma unroll
+            for (int r = 0; r < 9; ++r) out[9 * jj + r] = w * Z[r + 9 * i];
         ++j;
       }
     }
