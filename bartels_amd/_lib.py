@@ -119,6 +119,7 @@ SIGNATURES = {
     "bh_linear_tri2dmesh_dq2_device": (c_int, [_VP, c_int, _VP, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
     "bh_linear_tri2dmesh_mass_matrix": (c_int, [_VP, _VP, _VP, _VP]),
     "bh_linear_tri3dmesh_stvk_dq2": (c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, P(BhPsd), _VP]),
+    "bh_linear_tri3dmesh_stvk_dq2_device": (c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, P(BhPsd), _VP, _VP]),
     "bh_linear_tetmesh_dphi_dX_device": (c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bh_mesh_create": (c_int, [c_int, c_int, _VP, _VP, _VP, P(c_void_p)]),
     "bh_mesh_create_device": (c_int, [c_int, c_int, _VP, _VP, _VP, _VP, P(c_void_p)]),

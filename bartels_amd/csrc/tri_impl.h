@@ -31,6 +31,9 @@ struct bh_trimesh {
   int32_t* iptr = nullptr;  // nV + 1: incidence list of v at [iptr[v], iptr[v + 1])
   int32_t* inc = nullptr;   // (e << 2) | corner, ascending e
   uint32_t* irank = nullptr;  // ranks of the element's 3 vertices in v's neighbour list, 8 bits each
+  double* work = nullptr;     // element results between the two passes (grown on demand, kept)
+  size_t workN = 0;
+  int maxSeg64 = 0;           // most pattern blocks in the rows of 64 consecutive vertices
 };
 
 namespace {
@@ -38,9 +41,24 @@ namespace {
 constexpr int kTriModels = 4;  // neohookean, stvk, corotational, arap
 
 void free_trimesh(bh_trimesh* m) {
-  void* ptrs[] = {m->E, m->bptr, m->bcol, m->iptr, m->inc, m->irank};
+  void* ptrs[] = {m->E, m->bptr, m->bcol, m->iptr, m->inc, m->irank, m->work};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+}
+
+// the mesh's workspace of at least n doubles (calls on one mesh share it; each call synchronises its
+// stream before it returns, so calls one after another are safe, calls overlapping on one mesh are not)
+int tri_work(bh_trimesh* m, size_t n, double** out) {
+  n = std::max<size_t>(1, n);
+  if (m->workN < n) {
+    if (m->work) BH_CHECK(hipFree(m->work));
+    m->work = nullptr;
+    m->workN = 0;
+    BH_CHECK(hipMalloc(&m->work, sizeof(double) * n));
+    m->workN = n;
+  }
+  *out = m->work;
+  return BH_OK;
 }
 
 // device array freed on scope exit
@@ -80,6 +98,8 @@ int build_trimesh(bh_trimesh* m, const int* E) {
     m->h_bcol[k] = (int32_t)(keys[k] % nV);
   }
   for (int v = 0; v < nV; ++v) m->h_bptr[v + 1] += m->h_bptr[v];
+  for (int v0 = 0; v0 < nV; v0 += 64)
+    m->maxSeg64 = std::max(m->maxSeg64, m->h_bptr[std::min(v0 + 64, nV)] - m->h_bptr[v0]);
   m->nblocks = (int64_t)keys.size();
   m->nnz = m->nblocks * m->dim * m->dim;
   if (m->nnz >= (1ll << 31)) return BH_UNSUPPORTED;
@@ -129,21 +149,24 @@ int build_trimesh(bh_trimesh* m, const int* E) {
 // below tol raised to tol (src/simple_psd_fix.cpp:5-21); a matrix smaller than 9 x 9 is padded with
 // zeros, whose block is decoupled from it, so the clamp of the padded matrix restricted to the leading
 // n x n is the clamp of the matrix
-__device__ __forceinline__ void tri_psd(double* K, int n, double tol) {
+template <int N>
+__device__ __forceinline__ void tri_psd(double* K, double tol) {
   double a[45], d[9], Z[81];
 #pragma unroll
   for (int i = 0; i < 9; ++i)
 #pragma unroll
-    for (int j = i; j < 9; ++j) a[sym9(i, j)] = (i < n && j < n) ? K[i + n * j] : 0.0;
+    for (int j = i; j < 9; ++j) a[sym9(i, j)] = (i < N && j < N) ? K[i + N * j] : 0.0;
   sym9_eigen(a, d, Z);
 #pragma unroll
   for (int i = 0; i < 9; ++i) d[i] = d[i] < tol ? tol : d[i];
-  for (int r = 0; r < n; ++r)
-    for (int c = 0; c < n; ++c) {
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
       double s = 0.0;
 #pragma unroll
       for (int i = 0; i < 9; ++i) s += Z[r + 9 * i] * d[i] * Z[c + 9 * i];
-      K[r + n * c] = s;
+      K[r + N * c] = s;
     }
 }
 
@@ -222,106 +245,142 @@ __global__ void __launch_bounds__(64)
     k_tri2_element(int what, int model, int nT, const int32_t* __restrict__ E, const double* __restrict__ q,
                    const double* __restrict__ dX, const double* __restrict__ vol, const double* __restrict__ params,
                    int psd, double tol, double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nT) return;
-  const int stride = what == 0 ? 1 : what == 1 ? 6 : 36;
-  double* o = out + (int64_t)stride * e;
-  if (what == 3) {  // consistent mass: (rho vol / 6) I2 on the diagonal blocks, (rho vol / 12) I2 off them
-    const double mass = params[e] * vol[e];
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) o[r + 6 * c] = (r % 2 == c % 2) ? ((r / 2 == c / 2) ? mass / 6.0 : mass / 12.0) : 0.0;
+  __shared__ double stage[64 * 37];  // matrices (what 2, 3): the workgroup's 64 element matrices, pitch 37
+  const int t = (int)threadIdx.x;
+  const int e0 = blockIdx.x * blockDim.x;
+  const int e = e0 + t;
+  if (what < 2) {  // (what is the same for the whole grid)
+    if (e >= nT) return;
+    double* o = out + (int64_t)(what == 0 ? 1 : 6) * e;
+    switch (model) {
+      case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, o); break;
+      case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, o); break;
+      case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, o); break;
+      default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, o); break;
+    }
     return;
   }
-  double K[36];
-  double* dst = what == 2 ? K : o;
-  switch (model) {
-    case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, dst); break;
-    case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, dst); break;
-    case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, dst); break;
-    default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, dst); break;
-  }
-  if (what == 2) {
-    if (psd) tri_psd(K, 6, tol);
+  if (e < nT) {
+    double K[36];
+    if (what == 3) {  // consistent mass: (rho vol / 6) I2 on the diagonal blocks, (rho vol / 12) I2 off them
+      const double mass = params[e] * vol[e];
 #pragma unroll
-    for (int i = 0; i < 36; ++i) o[i] = K[i];
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          K[r + 6 * c] = (r % 2 == c % 2) ? ((r / 2 == c / 2) ? mass / 6.0 : mass / 12.0) : 0.0;
+    } else {
+      switch (model) {
+        case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, K); break;
+        case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, K); break;
+        case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, K); break;
+        default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, K); break;
+      }
+      if (psd) tri_psd<6>(K, tol);
+    }
+#pragma unroll
+    for (int i = 0; i < 36; ++i) stage[37 * t + i] = K[i];
+  }
+  __syncthreads();
+  // the block's matrices are contiguous in out: element e0 + u's entry i at 36 (e0 + u) + i
+  const int cnt = 36 * min(64, nT - e0);
+  double* out0 = out + 36ll * e0;
+  for (int idx = t; idx < cnt; idx += 64) {
+    const int u = idx / 36, i = idx - 36 * u;
+    out0[idx] = stage[37 * u + i];
   }
 }
 
-// tri3d StVK shell element Hessian (9 x 9) of element e
+// tri3d StVK shell element Hessian (9 x 9) of element e.  Column 3 a + c of B is vec(e_c phi_a^T +
+// D_col N^T) (D_col = dndq's column 3 a + c), so by bilinearity K(3 a1 + c1, 3 a2 + c2) = vol (H(phi_a1,
+// phi_a2)[c1][c2] + D2 . H(phi_a1, N)[c1][.] + D1 . H(phi_a2, N)[c2][.] + D1^T H(N, N) D2), with
+// H(x, y)[l][k] = d2psi(e_l x^T, e_k y^T) the closed-form block (hess_block) -- the reference's
+// B^T d2psi B vol without forming B.  The workgroup's 64 matrices leave through LDS as one contiguous block.
 __global__ void __launch_bounds__(64)
     k_tri3_stvk_hessian(int nT, const int32_t* __restrict__ E, const double* __restrict__ q,
                         const double* __restrict__ dX, const double* __restrict__ N, const double* __restrict__ n,
                         const double* __restrict__ dndq, const double* __restrict__ vol,
                         const double* __restrict__ params, int psd, double tol, double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nT) return;
-  double B[81], qe[9], Ne[3], ne[3];
+  __shared__ double stage[64 * 82];
+  const int t = (int)threadIdx.x;
+  const int e0 = blockIdx.x * blockDim.x;
+  const int e = e0 + t;
+  if (e < nT) {
+    double qe[9], Ne[3], ne[3], phi[3][3], D[27];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    Ne[k] = N[e + (int64_t)k * nT];
-    ne[k] = n[e + (int64_t)k * nT];
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int v = E[e + (int64_t)a * nT];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) qe[3 * a + c] = q[3ll * v + c];
-  }
-  for (int col = 0; col < 9; ++col) {
-    const int a = col / 3, c = col % 3;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int i = r % 3, j = r / 3;
-      B[r + 9 * col] = ((i == c) ? dX[e + (int64_t)(a + 3 * j) * nT] : 0.0) +
-                       Ne[j] * dndq[e + (int64_t)(i + 3 * col) * nT];
+    for (int k = 0; k < 3; ++k) {
+      Ne[k] = N[e + (int64_t)k * nT];
+      ne[k] = n[e + (int64_t)k * nT];
     }
-  }
-  Pre<double> P;
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    double s = ne[r % 3] * Ne[r / 3];
-    for (int col = 0; col < 9; ++col) s += B[r + 9 * col] * qe[col];
-    P.F[r] = s;
-  }
-  P.vol = 1.0;
-  pre_hessian<M_STVK>(P, params[e], params[e + (int64_t)nT], true);
-  // d2psi (9 x 9, index l + 3 j) from the closed-form blocks at unit vectors
-  double H[81];
-  for (int j = 0; j < 3; ++j)
-    for (int jp = 0; jp < 3; ++jp) {
-      double ej[3] = {0.0, 0.0, 0.0}, ejp[3] = {0.0, 0.0, 0.0};
-      ej[j] = 1.0;
-      ejp[jp] = 1.0;
-      double blk[9];
-      hess_block<M_STVK>(P, ej, ejp, blk);
+    for (int a = 0; a < 3; ++a) {
+      const int v = E[e + (int64_t)a * nT];
 #pragma unroll
-      for (int l = 0; l < 3; ++l)
+      for (int c = 0; c < 3; ++c) qe[3 * a + c] = q[3ll * v + c];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) H[(l + 3 * j) + 9 * (k + 3 * jp)] = blk[3 * l + k];
+      for (int j = 0; j < 3; ++j) phi[a][j] = dX[e + (int64_t)(a + 3 * j) * nT];
     }
-  const double v = vol[e];
-  double K[81];
-  for (int c = 0; c < 9; ++c) {
-    double hb[9];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
+    for (int k = 0; k < 27; ++k) D[k] = dndq[e + (int64_t)k * nT];  // D[i + 3 col]
+    // F3 = B q_e + vec(n N^T): F(i, j) = n_i N_j + sum_a phi_a[j] q_a[i] + N_j (D q_e)_i
+    Pre<double> P;
+    double Dq[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) s += H[r + 9 * k] * B[k + 9 * c];
-      hb[r] = s;
+      for (int col = 0; col < 9; ++col) s += D[i + 3 * col] * qe[col];
+      Dq[i] = s;
     }
-    for (int r = 0; r < 9; ++r) {
-      double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) s += B[k + 9 * r] * hb[k];
-      K[r + 9 * c] = s * v;
-    }
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        P.F[i + 3 * j] = ne[i] * Ne[j] + phi[0][j] * qe[i] + phi[1][j] * qe[3 + i] + phi[2][j] * qe[6 + i] +
+                         Ne[j] * Dq[i];
+    P.vol = 1.0;
+    pre_hessian<M_STVK>(P, params[e], params[e + (int64_t)nT], true);
+    double HPN[3][9], HNN[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) hess_block<M_STVK>(P, phi[a], Ne, HPN[a]);
+    hess_block<M_STVK>(P, Ne, Ne, HNN);
+    const double v = vol[e];
+    double K[81];
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+      for (int a2 = a1; a2 < 3; ++a2) {
+        double HPP[9];
+        hess_block<M_STVK>(P, phi[a1], phi[a2], HPP);
+#pragma unroll
+        for (int c1 = 0; c1 < 3; ++c1)
+#pragma unroll
+          for (int c2 = 0; c2 < 3; ++c2) {
+            const double* D1 = D + 3 * (3 * a1 + c1);
+            const double* D2 = D + 3 * (3 * a2 + c2);
+            double x = HPP[3 * c1 + c2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x += D2[k] * HPN[a1][3 * c1 + k] + D1[k] * HPN[a2][3 * c2 + k];
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+#pragma unroll
+              for (int k = 0; k < 3; ++k) x += D1[l] * HNN[3 * l + k] * D2[k];
+            const int r = 3 * a1 + c1, c = 3 * a2 + c2;
+            K[r + 9 * c] = x * v;
+            K[c + 9 * r] = x * v;
+          }
+      }
+    if (psd) tri_psd<9>(K, tol);
+#pragma unroll
+    for (int i = 0; i < 81; ++i) stage[82 * t + i] = K[i];
   }
-  if (psd) tri_psd(K, 9, tol);
-  double* o = out + 81ll * e;
-  for (int i = 0; i < 81; ++i) o[i] = K[i];
+  __syncthreads();
+  const int cnt = 81 * min(64, nT - e0);
+  double* out0 = out + 81ll * e0;
+  for (int idx = t; idx < cnt; idx += 64) {
+    const int u = idx / 81, i = idx - 81 * u;
+    out0[idx] = stage[82 * u + i];
+  }
 }
 
 // energy: fixed-partition partial sums of the per-element energies (block b sums its contiguous range
@@ -380,24 +439,82 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same fold with 64 vertices per workgroup and their rows accumulated in LDS: four lanes per vertex,
+// lane mm of a vertex adding column block mm of each incident element's matrix (the element's three
+// vertices are distinct, so the lanes of one step touch distinct slots; the steps run in ascending
+// element order, the lanes of a vertex sharing a wave), then the rows leave as one contiguous run.  The
+// sums are the same as k_tri_gather_mat's, bit for bit.
+template <int DIM>
+__global__ void __launch_bounds__(256)
+    k_tri_gather_lds(int nV, const int32_t* __restrict__ bptr, const int32_t* __restrict__ iptr,
+                     const int32_t* __restrict__ inc, const uint32_t* __restrict__ irank,
+                     const double* __restrict__ K, double* __restrict__ vals) {
+  extern __shared__ double acc[];
+  constexpr int NE = 3 * DIM;
+  const int v0 = blockIdx.x * 64;
+  const int vend = min(v0 + 64, nV);
+  const int b0 = bptr[v0];
+  const int seglen = DIM * DIM * (bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  const int v = v0 + ((int)threadIdx.x >> 2), mm = (int)threadIdx.x & 3;
+  if (v < nV && mm < 3) {
+    const int nb = bptr[v + 1] - bptr[v];
+    double* row = acc + DIM * DIM * (bptr[v] - b0);
+    for (int k = iptr[v]; k < iptr[v + 1]; ++k) {
+      const int code = inc[k], e = code >> 2, a = code & 3;
+      const double* Ke = K + (int64_t)NE * NE * e + DIM * a + NE * DIM * mm;
+      double* dst = row + DIM * ((irank[k] >> (8 * mm)) & 0xffu);
+#pragma unroll
+      for (int l = 0; l < DIM; ++l)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) dst[DIM * l * nb + c] += Ke[l + NE * c];
+    }
+  }
+  __syncthreads();
+  double* out = vals + (int64_t)DIM * DIM * b0;
+  for (int i = threadIdx.x; i < seglen; i += blockDim.x) out[i] = acc[i];
+}
+
+// the fold of the element matrices K (ne x ne per element) into vals: the LDS gather when the rows of 64
+// vertices fit, else one lane per vertex in global memory
+int tri_fold(const bh_trimesh* m, const double* K, double* vals, hipStream_t s) {
+  const size_t lds = sizeof(double) * (size_t)m->dim * m->dim * m->maxSeg64;
+  const unsigned g64 = (unsigned)((m->nV + 63) / 64);
+  if (lds <= (size_t)max_lds_bytes()) {
+    if (m->dim == 2) {
+      gather_lds_attr(k_tri_gather_lds<2>);
+      hipLaunchKernelGGL(k_tri_gather_lds<2>, dim3(g64), dim3(256), lds, s, m->nV, m->bptr, m->iptr, m->inc,
+                         m->irank, K, vals);
+    } else {
+      gather_lds_attr(k_tri_gather_lds<3>);
+      hipLaunchKernelGGL(k_tri_gather_lds<3>, dim3(g64), dim3(256), lds, s, m->nV, m->bptr, m->iptr, m->inc,
+                         m->irank, K, vals);
+    }
+  } else {
+    hipLaunchKernelGGL(k_tri_gather_mat, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, m->dim, m->bptr, m->iptr,
+                       m->inc, m->irank, K, vals);
+  }
+  BH_CHECK(hipGetLastError());
+  return BH_OK;
+}
+
 bool tri2_model_ok(int model) { return model >= 0 && model < kTriModels; }
 
 // the element matrices of kind what (2 Hessian, 3 mass) folded into vals; inputs on the device
 int tri2_matrix(bh_trimesh* m, int what, int model, const double* q, const double* dX, const double* vol,
                 const double* params, const bh_psd* psd, double* vals, hipStream_t s) {
   if (m->nV == 0) return BH_OK;
-  DevArr<double> K;
-  int st = K.alloc(36 * (size_t)m->nT);
+  double* K = nullptr;
+  int st = tri_work(m, 36 * (size_t)m->nT, &K);
   if (st) return st;
   if (m->nT) {
     hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, what, model, m->nT, m->E, q, dX, vol,
-                       params, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K.p);
+                       params, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K);
     BH_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_tri_gather_mat, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 2, m->bptr, m->iptr, m->inc,
-                     m->irank, (const double*)K.p, vals);
-  BH_CHECK(hipGetLastError());
-  BH_CHECK(hipStreamSynchronize(s));  // K is freed on return
+  if ((st = tri_fold(m, K, vals, s))) return st;
+  BH_CHECK(hipStreamSynchronize(s));
   return BH_OK;
 }
 
@@ -527,16 +644,17 @@ int bh_linear_tri2dmesh_q_device(bh_trimesh* m, int model, const double* q, cons
     return BH_BAD_ARG;
   hipStream_t s = as_stream(stream);
   constexpr int kParts = 256;
-  DevArr<double> ee, part;
-  int st;
-  if ((st = ee.alloc(m->nT)) || (st = part.alloc(kParts))) return st;
+  double* ee = nullptr;
+  int st = tri_work(m, (size_t)m->nT + kParts, &ee);
+  if (st) return st;
+  double* part = ee + m->nT;
   if (m->nT) {
     hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, 0, model, m->nT, m->E, q, dphidX,
-                       volume, params, 0, 0.0, ee.p);
+                       volume, params, 0, 0.0, ee);
     BH_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_tri_sum, dim3(kParts), dim3(256), 0, s, (int64_t)m->nT, (const double*)ee.p, kParts, part.p);
-  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(256), 0, s, (int64_t)kParts, (const double*)part.p, 1, d_energy);
+  hipLaunchKernelGGL(k_tri_sum, dim3(kParts), dim3(256), 0, s, (int64_t)m->nT, (const double*)ee, kParts, part);
+  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(256), 0, s, (int64_t)kParts, (const double*)part, 1, d_energy);
   BH_CHECK(hipGetLastError());
   BH_CHECK(hipStreamSynchronize(s));
   return BH_OK;
@@ -547,17 +665,17 @@ int bh_linear_tri2dmesh_dq_device(bh_trimesh* m, int model, const double* q, con
   if (!m || m->dim != 2 || !tri2_model_ok(model) || !d_g || (m->nT && (!q || !dphidX || !volume || !params)))
     return BH_BAD_ARG;
   hipStream_t s = as_stream(stream);
-  DevArr<double> f;
-  int st = f.alloc(6 * (size_t)m->nT);
+  double* f = nullptr;
+  int st = tri_work(m, 6 * (size_t)m->nT, &f);
   if (st) return st;
   if (m->nT) {
     hipLaunchKernelGGL(k_tri2_element, dim3((m->nT + 63) / 64), dim3(64), 0, s, 1, model, m->nT, m->E, q, dphidX,
-                       volume, params, 0, 0.0, f.p);
+                       volume, params, 0, 0.0, f);
     BH_CHECK(hipGetLastError());
   }
   if (m->nV) {
     hipLaunchKernelGGL(k_tri_gather_vec, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 2, m->iptr, m->inc,
-                       (const double*)f.p, d_g);
+                       (const double*)f, d_g);
     BH_CHECK(hipGetLastError());
   }
   BH_CHECK(hipStreamSynchronize(s));
@@ -622,6 +740,26 @@ int bh_linear_tri2dmesh_mass_matrix(bh_trimesh* m, const double* density, const 
   return BH_OK;
 }
 
+int bh_linear_tri3dmesh_stvk_dq2_device(bh_trimesh* m, const double* q, const double* dphidX, const double* N,
+                                        const double* n, const double* dndq, const double* volume,
+                                        const double* params, const bh_psd* psd, double* d_vals, bh_stream_t stream) {
+  if (!m || m->dim != 3 || !d_vals || (m->nT && (!q || !dphidX || !N || !n || !dndq || !volume || !params)))
+    return BH_BAD_ARG;
+  if (m->nV == 0) return BH_OK;
+  hipStream_t s = as_stream(stream);
+  double* K = nullptr;
+  int st = tri_work(m, 81 * (size_t)m->nT, &K);
+  if (st) return st;
+  if (m->nT) {
+    hipLaunchKernelGGL(k_tri3_stvk_hessian, dim3((m->nT + 63) / 64), dim3(64), 0, s, m->nT, m->E, q, dphidX, N, n,
+                       dndq, volume, params, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K);
+    BH_CHECK(hipGetLastError());
+  }
+  if ((st = tri_fold(m, K, d_vals, s))) return st;
+  BH_CHECK(hipStreamSynchronize(s));
+  return BH_OK;
+}
+
 int bh_linear_tri3dmesh_stvk_dq2(bh_trimesh* m, const double* q, const double* dphidX, const double* N,
                                  const double* n, const double* dndq, const double* volume, const double* params,
                                  const bh_psd* psd, double* vals) {
@@ -630,22 +768,16 @@ int bh_linear_tri3dmesh_stvk_dq2(bh_trimesh* m, const double* q, const double* d
   if (m->nV == 0) return BH_OK;
   hipStream_t s = nullptr;
   TriStage S;
-  DevArr<double> dN, dn, dD, K, d;
+  DevArr<double> dN, dn, dD, d;
   int st;
   if ((st = S.up(m, q, dphidX, 9, volume, params, s)) || (st = dN.upload(N, 3 * (size_t)m->nT, s)) ||
       (st = dn.upload(n, 3 * (size_t)m->nT, s)) || (st = dD.upload(dndq, 27 * (size_t)m->nT, s)) ||
-      (st = K.alloc(81 * (size_t)m->nT)) || (st = d.alloc((size_t)m->nnz)))
+      (st = d.alloc((size_t)m->nnz)))
     return st;
-  if (m->nT) {
-    hipLaunchKernelGGL(k_tri3_stvk_hessian, dim3((m->nT + 63) / 64), dim3(64), 0, s, m->nT, m->E, S.q.p, S.dX.p, dN.p,
-                       dn.p, dD.p, S.vol.p, S.params.p, (psd && psd->enabled) ? 1 : 0, psd ? psd->tol : 0.0, K.p);
-    BH_CHECK(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_tri_gather_mat, dim3((m->nV + 255) / 256), dim3(256), 0, s, m->nV, 3, m->bptr, m->iptr, m->inc,
-                     m->irank, (const double*)K.p, d.p);
-  BH_CHECK(hipGetLastError());
-  if (m->nnz) BH_CHECK(hipMemcpyAsync(vals, d.p, sizeof(double) * (size_t)m->nnz, hipMemcpyDeviceToHost, s));
-  BH_CHECK(hipStreamSynchronize(s));
+  if ((st = bh_linear_tri3dmesh_stvk_dq2_device(m, S.q.p, S.dX.p, dN.p, dn.p, dD.p, S.vol.p, S.params.p, psd, d.p,
+                                                nullptr)))
+    return st;
+  if (m->nnz) BH_CHECK(hipMemcpy(vals, d.p, sizeof(double) * (size_t)m->nnz, hipMemcpyDeviceToHost));
   return BH_OK;
 }
 

@@ -137,6 +137,50 @@ class TriMesh:
         return self._csc(vals)
 
 
+def _dp(t):
+    """device pointer of a torch tensor (or None)"""
+    import ctypes
+
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import ctypes
+
+    return None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+
+
+class TriMeshDevice(TriMesh):
+    """The same calls on device arrays (torch tensors, FP64, the reference's column-major layouts
+    flattened: dphidX.ravel(order="F") etc.); each call synchronises its stream."""
+
+    def q_device(self, model, q, dphidX, volume, params, out, stream=None):
+        check(self.lib.bh_linear_tri2dmesh_q_device(self.handle, model, _dp(q), _dp(dphidX), _dp(volume), _dp(params),
+                                                    _dp(out), _stream(stream)), "bh_linear_tri2dmesh_q_device")
+
+    def dq_device(self, model, q, dphidX, volume, params, out, stream=None):
+        check(self.lib.bh_linear_tri2dmesh_dq_device(self.handle, model, _dp(q), _dp(dphidX), _dp(volume),
+                                                     _dp(params), _dp(out), _stream(stream)),
+              "bh_linear_tri2dmesh_dq_device")
+
+    def dq2_device(self, model, q, dphidX, volume, params, out, callback=None, stream=None):
+        import ctypes
+
+        psd = _psd_struct(callback)
+        check(self.lib.bh_linear_tri2dmesh_dq2_device(self.handle, model, _dp(q), _dp(dphidX), _dp(volume),
+                                                      _dp(params), ctypes.byref(psd) if psd is not None else None,
+                                                      _dp(out), _stream(stream)), "bh_linear_tri2dmesh_dq2_device")
+
+    def stvk_shell_dq2_device(self, q, dphidX, N, n, dndq, volume, params, out, callback=None, stream=None):
+        import ctypes
+
+        psd = _psd_struct(callback)
+        check(self.lib.bh_linear_tri3dmesh_stvk_dq2_device(self.handle, _dp(q), _dp(dphidX), _dp(N), _dp(n), _dp(dndq),
+                                                           _dp(volume), _dp(params),
+                                                           ctypes.byref(psd) if psd is not None else None, _dp(out),
+                                                           _stream(stream)), "bh_linear_tri3dmesh_stvk_dq2_device")
+
+
 def _make(model):
     def q_fn(V, E, q, dphidX, volume, params):
         return TriMesh(2, np.asarray(V).shape[0], E).q(model, q, dphidX, volume, params)

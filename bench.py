@@ -279,6 +279,165 @@ def config3_leg(stream):
     return out
 
 
+def _event_timer(stream):
+    import torch
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    return timed
+
+
+def muscle_leg(p, mesh, stream):
+    """The muscle-fibre model (Neo-Hookean plus a/2 |F d|^2 per tet) on the same 6.07M-tet beam and
+    deformation: dq2 with and without simple_psd_fix(1e-3), FP64, fibres along the beam axis tilted per
+    tet (a = 1e5 Pa)"""
+    import torch
+
+    from bartels_amd import tetmesh as tm
+
+    nT = p.E.shape[0]
+    rng = np.random.default_rng(5)
+    d = np.c_[np.ones(nT), 0.2 * rng.standard_normal(nT), 0.2 * rng.standard_normal(nT)]
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    par = tm.muscle_params(d, p.params, np.full(nT, 1e5))
+    dev = torch.device("cuda:0")
+    q = torch.tensor(p.q, device=dev)
+    dpar = torch.tensor(np.asfortranarray(par).ravel(order="F"), device=dev)
+    H = torch.empty(mesh.nnz, dtype=torch.float64, device=dev)
+    timed = _event_timer(stream)
+    M = tm.NEOHOOKEAN_MUSCLE
+    ms_psd = timed(lambda: mesh.dq2_device(M, q, dpar, H, callback=tm.simple_psd_fix(1e-3), stream=stream))
+    ms = timed(lambda: mesh.dq2_device(M, q, dpar, H, stream=stream))
+    del H
+    return dict(workload="NEOHOOKEAN_MUSCLE dq2 on the 64x64x256 bend, fibres along x tilted per tet, a = 1e5 Pa",
+                dq2_psd_ms=ms_psd, dq2_psd_mtets_s=nT / (ms_psd * 1e-3) / 1e6, dq2_ms=ms,
+                dq2_mtets_s=nT / (ms * 1e-3) / 1e6)
+
+
+def triangle_leg(stream, cpu=True, n=1536):
+    """The triangle siblings at scale, inputs resident on the device: the 2-D plane-strain Neo-Hookean
+    strip of n x n vertices (2 (n - 1)^2 triangles) under a smooth bend + stretch -- q, dq, dq2 with and
+    without simple_psd_fix(1e-3) -- and the StVK shell Hessian on the same strip embedded in 3-D (flat
+    rest state, a sine bend out of the plane; per-element rest normal N = e_z, the current normal n and its
+    exact derivative dndq formed on the host, as the reference's callers hand them over).  Algorithmic
+    bytes per element: E, dphidX, area, params, the vertices' q once and every CSR value written once.
+    The oracle's CPU time (the reference algorithm) on a 224 x 224 sample of the strip sits beside it."""
+    import torch
+
+    from bartels_amd import meshgen
+    from bartels_amd import tetmesh as tm
+    from bartels_amd import trimesh as trm
+
+    h = 1e-3
+    V, E = meshgen.strip2d(n, n, h)
+    nV, nT = V.shape[0], E.shape[0]
+    L = (n - 1) * h
+    lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+    dev = torch.device("cuda:0")
+    timed = _event_timer(stream)
+
+    def col(a):
+        return torch.tensor(np.asfortranarray(a).ravel(order="F"), device=dev)
+
+    out = dict(workload=f"{n} x {n} vertex strip, {nT} triangles", triangles=nT, vertices=nV)
+    # tri2d: plane-strain Neo-Hookean
+    x = V.copy()
+    x[:, 0] *= 1.02
+    x[:, 1] += 0.05 * L * np.sin(np.pi * V[:, 0] / L)
+    par = meshgen.model_params(np.zeros(nT, np.uint8), lam, mu)
+    dX = trm.linear_tri2dmesh_dphi_dX(V, E)
+    area = meshgen.triangle_areas(V, E)
+    m2 = trm.TriMeshDevice(2, nV, E)
+    q, dXd, vol, pd = torch.tensor(x.ravel(), device=dev), col(dX), torch.tensor(area, device=dev), col(par)
+    Hd = torch.empty(m2.nnz, dtype=torch.float64, device=dev)
+    g = torch.empty(2 * nV, dtype=torch.float64, device=dev)
+    e = torch.empty(1, dtype=torch.float64, device=dev)
+    psd = tm.simple_psd_fix(1e-3)
+    b_in = 12 + 48 + 8 + 16 + 16.0 * nV / nT
+    r2 = dict(nnz=m2.nnz)
+    r2["q_ms"] = timed(lambda: m2.q_device(0, q, dXd, vol, pd, e, stream=stream))
+    r2["dq_ms"] = timed(lambda: m2.dq_device(0, q, dXd, vol, pd, g, stream=stream))
+    r2["dq2_ms"] = timed(lambda: m2.dq2_device(0, q, dXd, vol, pd, Hd, stream=stream))
+    r2["dq2_psd_ms"] = timed(lambda: m2.dq2_device(0, q, dXd, vol, pd, Hd, callback=psd, stream=stream))
+    bytes_dq2 = (b_in + 8.0 * m2.nnz / nT) * nT
+    r2["dq2_psd_mtris_s"] = nT / (r2["dq2_psd_ms"] * 1e-3) / 1e6
+    r2["algorithmic_bytes"] = dict(q=b_in * nT, dq=(b_in + 16.0 * nV / nT) * nT, dq2=bytes_dq2)
+    r2["hbm_frac"] = dict(q=b_in * nT / (r2["q_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          dq=(b_in + 16.0 * nV / nT) * nT / (r2["dq_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          dq2=bytes_dq2 / (r2["dq2_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          dq2_psd=bytes_dq2 / (r2["dq2_psd_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    out["tri2d_neohookean"] = r2
+    del Hd, g
+    m2.close()
+    # tri3d: StVK shell
+    V3 = np.c_[V, np.zeros(nV)]
+    x3 = V3.copy()
+    x3[:, 2] = 0.05 * L * np.sin(np.pi * V[:, 0] / L)
+    X0, X1, X2 = (x3[E[:, k]] for k in range(3))
+    e1, e2 = X1 - X0, X2 - X0
+    c = np.cross(e1, e2)
+    cn = np.linalg.norm(c, axis=1)
+    nrm = c / cn[:, None]
+    Pn = (np.eye(3)[None] - nrm[:, :, None] * nrm[:, None, :]) / cn[:, None, None]
+
+    def skew(v):
+        z = np.zeros(v.shape[0])
+        return np.stack([np.stack([z, -v[:, 2], v[:, 1]], 1), np.stack([v[:, 2], z, -v[:, 0]], 1),
+                         np.stack([-v[:, 1], v[:, 0], z], 1)], 1)
+
+    dc1, dc2 = -skew(e2), skew(e1)  # d(e1 x e2)/d x1, / d x2
+    dcs = [-(dc1 + dc2), dc1, dc2]
+    dndq = np.concatenate([Pn @ d_ for d_ in dcs], axis=2)  # nT x 3 x 9, column 3a + c
+    dX3 = trm.linear_tri3dmesh_dphi_dX(V3, E)
+    par3 = meshgen.model_params(np.ones(nT, np.uint8), lam, mu)  # StVK [lambda, mu]
+    m3 = trm.TriMeshDevice(3, nV, E)
+    Nd = col(np.tile([0.0, 0.0, 1.0], (nT, 1)))
+    nd = col(nrm)
+    Dd = col(dndq.reshape(nT, 27, order="F"))
+    q3, dX3d, p3 = torch.tensor(x3.ravel(), device=dev), col(dX3), col(par3)
+    H3 = torch.empty(m3.nnz, dtype=torch.float64, device=dev)
+    r3 = dict(nnz=m3.nnz)
+    r3["dq2_ms"] = timed(lambda: m3.stvk_shell_dq2_device(q3, dX3d, Nd, nd, Dd, vol, p3, H3, stream=stream))
+    r3["dq2_psd_ms"] = timed(lambda: m3.stvk_shell_dq2_device(q3, dX3d, Nd, nd, Dd, vol, p3, H3, callback=psd,
+                                                             stream=stream))
+    b3 = (12 + 72 + 24 + 24 + 216 + 8 + 16 + 24.0 * nV / nT + 8.0 * m3.nnz / nT) * nT
+    r3["algorithmic_bytes"] = b3
+    r3["hbm_frac"] = dict(dq2=b3 / (r3["dq2_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          dq2_psd=b3 / (r3["dq2_psd_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    r3["dq2_psd_mtris_s"] = nT / (r3["dq2_psd_ms"] * 1e-3) / 1e6
+    out["tri3d_stvk_shell"] = r3
+    del H3
+    m3.close()
+    if cpu:
+        from oracle import oracle as O
+
+        O.load(fast=True)
+        ns = 224
+        Vs, Es = meshgen.strip2d(ns, ns, h)
+        xs = Vs.copy()
+        xs[:, 0] *= 1.02
+        xs[:, 1] += 0.05 * (ns - 1) * h * np.sin(np.pi * Vs[:, 0] / ((ns - 1) * h))
+        dXs = trm.linear_tri2dmesh_dphi_dX(Vs, Es)
+        ps = meshgen.model_params(np.zeros(Es.shape[0], np.uint8), lam, mu)
+        t0 = time.perf_counter()
+        O.tri2dmesh_dq2(0, Vs.shape[0], Es, xs.ravel(), dXs, meshgen.triangle_areas(Vs, Es), ps, psd=True, psd_tol=1e-3)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle_tri2d_dq2_psd"] = dict(triangles=int(Es.shape[0]), seconds=dt,
+                                                mtris_s=Es.shape[0] / dt / 1e6, threads=1,
+                                                what="the oracle's restatement of the reference (dense B^T H B, "
+                                                     "12x12-padded PSD, sort-and-fold), one host thread")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +449,8 @@ def main():
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-config3", action="store_true", help="skip the 48x48x192 mixed-model assembly leg")
+    ap.add_argument("--no-siblings", action="store_true",
+                    help="skip the muscle-fibre model and the triangle siblings (tri2d, tri3d shell) legs")
     ap.add_argument("--newton-steps", type=int, default=2,
                     help="Newton steps per solver variant in the Newton-step comparison (0: skip)")
     ap.add_argument("--pcg-max", type=int, default=20000)
@@ -510,6 +671,9 @@ def main():
     }
     if not args.no_config3:
         out["extra"]["config3"] = config3_leg(stream)
+    if not args.no_siblings:
+        out["extra"]["muscle"] = muscle_leg(p, mesh, stream)
+        out["extra"]["triangles"] = triangle_leg(stream, cpu=not args.no_cpu_baseline)
     if args.newton_steps > 0:
         del H, g
         nres = newton_steps(p, mesh, args.newton_steps, args.pcg_max)

@@ -523,6 +523,13 @@ int bh_linear_tri2dmesh_mass_matrix(bh_trimesh* mesh, const double* density, con
 int bh_linear_tri3dmesh_stvk_dq2(bh_trimesh* mesh, const double* q, const double* dphidX, const double* N,
                                  const double* n, const double* dndq, const double* volume, const double* params,
                                  const bh_psd* psd, double* vals);
+/* The same on device arrays; synchronises the stream.  The triangle calls keep their element results
+ * in a workspace of the mesh handle: calls one after another on a handle are safe, overlapping ones
+ * are not. */
+int bh_linear_tri3dmesh_stvk_dq2_device(bh_trimesh* mesh, const double* d_q, const double* d_dphidX,
+                                        const double* d_N, const double* d_n, const double* d_dndq,
+                                        const double* d_volume, const double* d_params, const bh_psd* psd,
+                                        double* d_vals, bh_stream_t stream);
 
 #ifdef __cplusplus
 }
