@@ -16,10 +16,10 @@
 // pattern: block rows of dim x dim blocks (sorted neighbours, the vertex itself included), CSR row
 // dim v + l at dim^2 bptr[v] + dim l nb(v), the block of rank r at + dim r (the reference's CSC by
 // symmetry), and per-vertex incidence lists in ascending element order.  Assembly: one lane per
-// element writes its element matrix (with simple_psd_fix when asked: the full 9x9 eigensolver, a 6x6
-// padded with zeros), then one lane per vertex folds the blocks of its rows in ascending element
-// order (the order of Eigen's setFromTriplets on the reference's triplet list).  These meshes are
-// small next to the tet path's; the kernels are plain, not tuned.
+// element forms its element matrix (with simple_psd_fix when asked: the full 9x9 eigensolver, a 6x6
+// padded with zeros) and the workgroup's matrices leave through LDS as one contiguous block; then 64
+// vertices per workgroup fold the blocks of their rows in LDS in ascending element order (the order
+// of Eigen's setFromTriplets on the reference's triplet list) and write them out contiguously.
 
 struct bh_trimesh {
   int dim = 2, nV = 0, nT = 0;
