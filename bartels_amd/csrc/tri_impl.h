@@ -225,6 +225,144 @@ __device__ void tri2_hessian(const double* F, const double phi[3][3], double vol
     }
 }
 
+// Cyclic Jacobi on a symmetric 4x4 (A destroyed): eigenvalues d, eigenvectors in the columns of V
+// (Golub & Van Loan, Matrix Computations, 4th ed., section 8.5.2); every index is a compile-time
+// constant after unrolling, so A and V stay in registers.
+__device__ __forceinline__ void jacobi4(double A[4][4], double d[4], double V[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    double off = 0.0, dia = 0.0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      dia += A[p][p] * A[p][p];
+#pragma unroll
+      for (int q = p + 1; q < 4; ++q) off += A[p][q] * A[p][q];
+    }
+    if (off <= 1e-34 * dia) break;  // converged (quadratically: a few sweeps)
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = p + 1; q < 4; ++q) {
+        const double apq = A[p][q];
+        if (fabs(apq) <= 1e-300) continue;
+        const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // A <- A J (columns p, q)
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // A <- J^T A (rows p, q)
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = A[i][i];
+}
+
+// The tri2d element Hessian with simple_psd_fix: as for the tets, the three shape gradients sum to
+// zero, so with the thin QR Phi = Q R (Q 3 x 2) the 6x6 K = (Q x I2) M (Q x I2)^T with the 4x4
+// M(l + 2p, k + 2r) = the leading 2x2 of the closed-form block at (R_p, R_r) (rows of R), and the two
+// translations in K's null space.  The clamp is then K' = (Q x I2) M' (Q x I2)^T + max(tol, 0)/3 (1 1^T x I2)
+// with M' = M with its eigenvalues raised to tol: the eigenvalues of K are those of M and the two zeros
+// of the translations (eigenvectors (1, 1, 1) / sqrt 3 x e_i), which the clamp lifts to tol.
+template <int MODEL>
+__device__ void tri2_hessian_psd(const double* F, const double phi[3][3], double vol, double p0, double p1,
+                                 double tol, double* K) {
+  double Q[3][2], R[2][2];
+  {
+    double r00 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r00 += phi[a][0] * phi[a][0];
+    r00 = sqrt(r00);
+    double r01 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      Q[a][0] = phi[a][0] / r00;
+      r01 += Q[a][0] * phi[a][1];
+    }
+    double u[3], r11 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      u[a] = phi[a][1] - r01 * Q[a][0];
+      r11 += u[a] * u[a];
+    }
+    r11 = sqrt(r11);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Q[a][1] = u[a] / r11;
+    R[0][0] = r00;
+    R[0][1] = r01;
+    R[1][0] = 0.0;
+    R[1][1] = r11;
+  }
+  Pre<double> P;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) P.F[i] = F[i];
+  P.vol = vol;
+  pre_hessian<MODEL>(P, p0, p1, true);
+  double M[4][4];
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp)
+#pragma unroll
+    for (int rr = pp; rr < 2; ++rr) {
+      const double rp[3] = {R[pp][0], R[pp][1], 0.0}, rq[3] = {R[rr][0], R[rr][1], 0.0};
+      double blk[9];
+      hess_block<MODEL>(P, rp, rq, blk);
+#pragma unroll
+      for (int l = 0; l < 2; ++l)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          M[l + 2 * pp][k + 2 * rr] = blk[3 * l + k];
+          M[k + 2 * rr][l + 2 * pp] = blk[3 * l + k];
+        }
+    }
+  double d[4], V[4][4];
+  jacobi4(M, d, V);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) d[i] = d[i] < tol ? tol : d[i];
+  double Mp[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s += V[i][c] * d[c] * V[j][c];
+      Mp[i][j] = s;
+    }
+  const double tp = (tol > 0.0 ? tol : 0.0) / 3.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int l = 0; l < 2; ++l)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          double s = (l == k) ? tp : 0.0;
+#pragma unroll
+          for (int pp = 0; pp < 2; ++pp)
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr) s += Q[a][pp] * Q[b][rr] * Mp[l + 2 * pp][k + 2 * rr];
+          K[(2 * a + l) + 6 * (2 * b + k)] = s;
+        }
+}
+
 // what the per-element kernel leaves: 0 energy (1 value), 1 forces (6), 2 Hessian (36), 3 mass (36)
 template <int MODEL>
 __device__ void tri2_element(int what, int nT, const int32_t* E, const double* q, const double* dX,
@@ -270,13 +408,24 @@ __global__ void __launch_bounds__(64)
         for (int c = 0; c < 6; ++c)
           K[r + 6 * c] = (r % 2 == c % 2) ? ((r / 2 == c / 2) ? mass / 6.0 : mass / 12.0) : 0.0;
     } else {
-      switch (model) {
-        case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, K); break;
-        case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, K); break;
-        case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, K); break;
-        default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, K); break;
+      if (psd) {
+        double phi[3][3], F[9];
+        tri2_load(nT, E, q, dX, e, phi, F);
+        const double p0 = params[e], p1 = params[e + (int64_t)nT], v = vol[e];
+        switch (model) {
+          case M_NH: tri2_hessian_psd<M_NH>(F, phi, v, p0, p1, tol, K); break;
+          case M_STVK: tri2_hessian_psd<M_STVK>(F, phi, v, p0, p1, tol, K); break;
+          case M_COROT: tri2_hessian_psd<M_COROT>(F, phi, v, p0, p1, tol, K); break;
+          default: tri2_hessian_psd<M_ARAP>(F, phi, v, p0, p1, tol, K); break;
+        }
+      } else {
+        switch (model) {
+          case M_NH: tri2_element<M_NH>(what, nT, E, q, dX, vol, params, e, K); break;
+          case M_STVK: tri2_element<M_STVK>(what, nT, E, q, dX, vol, params, e, K); break;
+          case M_COROT: tri2_element<M_COROT>(what, nT, E, q, dX, vol, params, e, K); break;
+          default: tri2_element<M_ARAP>(what, nT, E, q, dX, vol, params, e, K); break;
+        }
       }
-      if (psd) tri_psd<6>(K, tol);
     }
 #pragma unroll
     for (int i = 0; i < 36; ++i) stage[37 * t + i] = K[i];
