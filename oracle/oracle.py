@@ -67,6 +67,10 @@ def load(fast=False):
         return _lib_fast
     if _lib is not None:
         return _lib
+    if os.environ.get("BH_ORACLE_SANITIZE") == "1":  # the AddressSanitizer / UBSan build (host checks)
+        subprocess.run(["make", "-s", "-C", HERE, "sanitize"], check=True)
+        _lib = _declare(ctypes.CDLL(os.path.join(HERE, "liboracle_san.so")))
+        return _lib
     if not os.path.exists(LIB):
         build()
     _lib = _declare(ctypes.CDLL(LIB))
