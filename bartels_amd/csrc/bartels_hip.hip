@@ -70,8 +70,10 @@ struct bh_mesh {
   int32_t* bptr = nullptr;
   int32_t* bcol = nullptr;
   int32_t* vdeg = nullptr;  // tets incident to each vertex
-  int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0;  // max values per gather workgroup
-  int gatherLanes = 4;                            // lanes per vertex in the assembly gather (1, 2 or 4)
+  int maxSeg64 = 0, maxSeg32 = 0, maxSeg16 = 0, maxSeg8 = 0;  // max values per gather workgroup
+  // lanes per vertex in the assembly gathers (1, 2, 4 or 8); 0: 8 in the element-math gathers, 4 in the
+  // record gathers (the fastest of each on the 64x64x256 beam)
+  int gatherLanes = 0;
   int xcdGroups = 0;                              // deal vertex groups to XCDs in contiguous runs
   // visiting order of the gather workgroups (Morton order of the vertex groups' rest centroids; made when
   // rest positions are set): ord16 over groups of 16 vertices, ord64 over groups of 64
@@ -160,13 +162,14 @@ struct MeshView {
   const double* __restrict__ X;  // rest positions (3 nV, interleaved) or null
   const int32_t* __restrict__ ord16;  // gather visiting orders (or null: vertex order)
   const int32_t* __restrict__ ord64;
+  const int32_t* __restrict__ vdeg;   // incidence-list length of each vertex
 };
 
 // ordered: with the gathers' visiting order (when the mesh has one)
 MeshView view(const bh_mesh* m, bool ordered = false) {
   const bool ord = ordered && m->useOrder != 0;
   return MeshView{m->nV, m->nT, m->E, m->dX, m->dX32, m->vol, m->vol32, m->gBase, m->gLen, m->incE, m->incR,
-                  m->bptr, m->bcol, m->X, ord ? m->ord16 : nullptr, ord ? m->ord64 : nullptr};
+                  m->bptr, m->bcol, m->X, ord ? m->ord16 : nullptr, ord ? m->ord64 : nullptr, m->vdeg};
 }
 
 // the interleaved incidence list of vertex v: entries at first + 64 k, k < len (code < 0 ends it)
@@ -177,6 +180,19 @@ struct IncList {
 __device__ __forceinline__ IncList inc_list(const MeshView& m, int v) {
   const int g = v >> 6;
   return IncList{m.gBase[g] + (v & 63), m.gLen[g]};
+}
+
+// value of lane K of this lane's quad (DPP quad_perm broadcast; all four lanes of the quad active)
+template <int K>
+__device__ __forceinline__ float quad_bcast(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), K * 0x55, 0xf, 0xf, true));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, K * 0x55, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), K * 0x55, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
 // ---- tet loads -------------------------------------------------------------------------------
@@ -211,16 +227,7 @@ __device__ __forceinline__ T load_vol(const MeshView& m, int e) {
 // FP32: F = sum_{m>=1} (x_m - x_0) phi_m^T with the edges formed in FP64 before the cast, so the
 //       beam's absolute coordinates (metres) do not swamp the millimetre edges in FP32
 template <typename T>
-__device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, const T phi[4][3], T* F) {
-  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
-  double x[4][3];
-#pragma unroll
-  for (int mm = 0; mm < 4; ++mm) {
-    const double* qp = q + 3ll * vid[mm];
-    x[mm][0] = qp[0];
-    x[mm][1] = qp[1];
-    x[mm][2] = qp[2];
-  }
+__device__ __forceinline__ void F_from_x(const double x[4][3], const T phi[4][3], T* F) {
   if constexpr (sizeof(T) == 8) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -240,6 +247,37 @@ __device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, co
   }
 }
 
+template <typename T>
+__device__ __forceinline__ void load_F(const double* __restrict__ q, int4 ev, const T phi[4][3], T* F) {
+  const int vid[4] = {ev.x, ev.y, ev.z, ev.w};
+  double x[4][3];
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) {
+    const double* qp = q + 3ll * vid[mm];
+    x[mm][0] = qp[0];
+    x[mm][1] = qp[1];
+    x[mm][2] = qp[2];
+  }
+  F_from_x(x, phi, F);
+}
+
+// FP32 with rest positions: F and G from the corners' displacements u (FP64)
+template <typename T>
+__device__ __forceinline__ void FG_from_u(const double u[4][3], const T phi[4][3], T* F, T* G) {
+  T ed[3][3];
+#pragma unroll
+  for (int mm = 0; mm < 3; ++mm)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(u[mm + 1][i] - u[0][i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      G[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
+      F[i + 3 * j] = G[i + 3 * j] + ((i == j) ? T(1) : T(0));
+    }
+}
+
 // F and G = F - I.  FP64: G from F.  FP32 with rest positions: G from displacement edges,
 // G = sum_{m>=1} ((x_m - x_0) - (X_m - X_0)) phi_m^T with the edges formed in FP64, so G keeps its
 // relative accuracy at small strain (F = I + G; stress and energy then use G); without rest
@@ -255,18 +293,7 @@ __device__ __forceinline__ void load_FG(const MeshView& m, const double* __restr
       for (int mm = 0; mm < 4; ++mm)
 #pragma unroll
         for (int i = 0; i < 3; ++i) u[mm][i] = q[3ll * vid[mm] + i] - m.X[3ll * vid[mm] + i];
-      T ed[3][3];
-#pragma unroll
-      for (int mm = 0; mm < 3; ++mm)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ed[mm][i] = (T)(u[mm + 1][i] - u[0][i]);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          G[i + 3 * j] = ed[0][i] * phi[1][j] + ed[1][i] * phi[2][j] + ed[2][i] * phi[3][j];
-          F[i + 3 * j] = G[i + 3 * j] + ((i == j) ? T(1) : T(0));
-        }
+      FG_from_u(u, phi, F, G);
       return;
     }
   }
@@ -603,6 +630,86 @@ __device__ __forceinline__ void tet_block1e(const MeshView& m, const double* __r
   hess_block<MODEL>(P, pa, pm, blk[0]);
 }
 
+// rows[K][.] = v[.] of lane K of this lane's quad
+template <typename T, typename U>
+__device__ __forceinline__ void quad_rows(const U v[3], T rows[4][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    rows[0][j] = quad_bcast<0>(v[j]);
+    rows[1][j] = quad_bcast<1>(v[j]);
+    rows[2][j] = quad_bcast<2>(v[j]);
+    rows[3][j] = quad_bcast<3>(v[j]);
+  }
+}
+
+// One tet's inputs in all four lanes of a quad: lane c loads its own corner's vertex position (vid: the
+// tet's vertex c) and shape-gradient row, and one of vol / params[e] / params[nT + e] (lanes 0, 1, 2);
+// the quad hands them around over DPP, and every lane forms F exactly as load_FG does.  A quarter of
+// the loads of every lane reading the whole tet.
+template <typename T>
+__device__ __forceinline__ void quad_tet_inputs(const MeshView& m, const double* __restrict__ q,
+                                                const T* __restrict__ params, int e, int vid, int c, T phi[4][3],
+                                                T F[9], T& vol, T& p0, T& p1) {
+  const int64_t nT = m.nT;
+  T pc[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if constexpr (sizeof(T) == 8)
+      pc[j] = m.dX[e + (c + 4 * j) * nT];
+    else
+      pc[j] = m.dX32[e + (c + 4 * j) * nT];
+  }
+  const bool rel = sizeof(T) == 4 && m.X != nullptr;  // FP32 with rest positions: displacements
+  double xc[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) xc[i] = rel ? q[3ll * vid + i] - m.X[3ll * vid + i] : q[3ll * vid + i];
+  const T* sp = (c == 0) ? ((sizeof(T) == 8) ? reinterpret_cast<const T*>(m.vol) : reinterpret_cast<const T*>(m.vol32))
+                         : params + ((c == 1) ? 0 : nT);
+  const T sv = (c < 3) ? sp[e] : T(0);
+  quad_rows(pc, phi);
+  double x[4][3];
+  quad_rows(xc, x);
+  vol = quad_bcast<0>(sv);
+  p0 = quad_bcast<1>(sv);
+  p1 = quad_bcast<2>(sv);
+  if (rel) {
+    T G[9];
+    FG_from_u(x, phi, F, G);
+  } else {
+    F_from_x(x, phi, F);
+  }
+}
+
+// tet_block1e with the tet's inputs loaded cooperatively by the quad (lane c: corner c, vid: the tet's
+// vertex c); the same arithmetic, so the same block to the last bit.  All four lanes of the quad take
+// part (they hold the same tet).
+template <int MODEL, typename T>
+__device__ __forceinline__ void tet_block1q(const MeshView& m, const double* __restrict__ q,
+                                            const T* __restrict__ params, int e, int vid, int a, int c,
+                                            bool stretch_ref, T blk[1][9]) {
+  if (MODEL == M_MASS) {
+    const T mass = params[e] * load_vol<T>(m, e);
+    const T cm = (c == a) ? mass * T(1.0 / 10.0) : mass * T(1.0 / 20.0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) blk[0][i] = (i % 4 == 0) ? cm : T(0);
+    return;
+  }
+  T phi[4][3];
+  Pre<T> P;
+  T p0, p1;
+  quad_tet_inputs(m, q, params, e, vid, c, phi, P.F, P.vol, p0, p1);
+  T pa[3], pm[3];
+  select_row(phi, a, pa);
+  select_row(phi, c, pm);
+  pre_hessian<MODEL>(P, p0, p1, stretch_ref);
+  if constexpr (MODEL == M_NH_MUSCLE) {
+    const T f = params[(int64_t)(2 + c) * m.nT + e];
+    const T fib[4] = {quad_bcast<0>(f), quad_bcast<1>(f), quad_bcast<2>(f), quad_bcast<3>(f)};
+    set_fibre(P, fib);
+  }
+  hess_block<MODEL>(P, pa, pm, blk[0]);
+}
+
 template <int MODEL, typename T>
 __device__ __forceinline__ void tet_block1(const MeshView& m, const double* __restrict__ q,
                                            const T* __restrict__ params, int e, int a, int mm, bool stretch_ref,
@@ -746,27 +853,28 @@ __global__ void __launch_bounds__(kGatherThreads)
     // pipelined walk: the code of step k + 2 and the vertex ids of step k + 1 are loaded during step k,
     // so a step waits on one level of loads (its q, geometry and parameters) instead of three (code,
     // then vertex ids, then q); lists are valid up to their first negative code
+    const int64_t nT = m.nT;
     int cA = L.len > 0 ? m.incE[L.first] : -1;
     uint32_t rA = cA >= 0 ? m.incR[L.first] : 0u;
-    int4 eA = cA >= 0 ? load_E(m, cA >> 2) : make_int4(0, 0, 0, 0);
+    int vA = cA >= 0 ? m.E[(cA >> 2) + c * nT] : 0;  // this lane's corner of the tet
     int cB = (cA >= 0 && L.len > 1) ? m.incE[L.first + 64] : -1;
     uint32_t rB = cB >= 0 ? m.incR[L.first + 64] : 0u;
     for (int k = 0; cA >= 0; ++k) {
       const int64_t iC = L.first + 64ll * (k + 2);
       const int cC = (cB >= 0 && k + 2 < L.len) ? m.incE[iC] : -1;
       const uint32_t rC = cC >= 0 ? m.incR[iC] : 0u;
-      const int4 eB = cB >= 0 ? load_E(m, cB >> 2) : make_int4(0, 0, 0, 0);
+      const int vB = cB >= 0 ? m.E[(cB >> 2) + c * nT] : 0;
       const int e = cA >> 2, a = cA & 3;
       T blk[1][9];
       if constexpr (MODEL == M_NH_MUSCLE) {
-        tet_block1e<M_NH_MUSCLE>(m, q, params, e, eA, a, c, stretchRef, blk);
+        tet_block1q<M_NH_MUSCLE>(m, q, params, e, vA, a, c, stretchRef, blk);
       } else switch (model_at<MODEL>(mids, e)) {
-        case M_NH: tet_block1e<M_NH>(m, q, params, e, eA, a, c, stretchRef, blk); break;
-        case M_STVK: tet_block1e<M_STVK>(m, q, params, e, eA, a, c, stretchRef, blk); break;
-        case M_COROT: tet_block1e<M_COROT>(m, q, params, e, eA, a, c, stretchRef, blk); break;
-        case M_ARAP: tet_block1e<M_ARAP>(m, q, params, e, eA, a, c, stretchRef, blk); break;
-        case M_MASS: tet_block1e<M_MASS>(m, q, params, e, eA, a, c, stretchRef, blk); break;
-        default: tet_block1e<M_NH_LEGACY>(m, q, params, e, eA, a, c, stretchRef, blk); break;
+        case M_NH: tet_block1q<M_NH>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        case M_STVK: tet_block1q<M_STVK>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        case M_COROT: tet_block1q<M_COROT>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        case M_ARAP: tet_block1q<M_ARAP>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        case M_MASS: tet_block1q<M_MASS>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        default: tet_block1q<M_NH_LEGACY>(m, q, params, e, vA, a, c, stretchRef, blk); break;
       }
       T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
 #pragma unroll
@@ -775,7 +883,85 @@ __global__ void __launch_bounds__(kGatherThreads)
         for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[0][3 * l + kk]);
       cA = cB;
       rA = rB;
-      eA = eB;
+      vA = vB;
+      cB = cC;
+      rB = rC;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
+// Same gather with eight lanes per vertex (eight vertices per 64-lane workgroup): the two quads of a
+// vertex take incidences k and k + 1 of its list at each step, lane c of each quad the block of corner
+// c.  The two blocks a step makes for one slot (when both tets hold that neighbour) go into LDS in list
+// order: the first quad's adds are issued before the second's, and a wave's LDS instructions execute
+// in issue order.  Half the LDS per workgroup of the quad kernel, so twice the waves per CU.
+template <int MODEL, int VPW, typename T>
+__global__ void __launch_bounds__(kGatherThreads)
+    k_hessian_oct(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
+                  const uint8_t* __restrict__ mids, int stretchRef, int xcd, T* __restrict__ vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 3, h = ((int)threadIdx.x >> 2) & 1, c = (int)threadIdx.x & 3;
+  const int v = v0 + vl;
+  if (vl < VPW && v < m.nV) {
+    const int nb = m.bptr[v + 1] - m.bptr[v];
+    T* row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    const int deg = m.vdeg[v];  // entries [0, deg) of the list are the vertex's incidences
+    // pipelined walk as in the quad kernel: this lane's code of the step after next and the vertex ids
+    // of the next step are loaded during a step
+    const int64_t nT = m.nT;
+    int cA = h < deg ? m.incE[L.first + 64ll * h] : -1;
+    uint32_t rA = cA >= 0 ? m.incR[L.first + 64ll * h] : 0u;
+    int vA = cA >= 0 ? m.E[(cA >> 2) + c * nT] : 0;  // this lane's corner of the tet
+    int cB = 2 + h < deg ? m.incE[L.first + 64ll * (2 + h)] : -1;
+    uint32_t rB = cB >= 0 ? m.incR[L.first + 64ll * (2 + h)] : 0u;
+    for (int k = 0; k < deg; k += 2) {
+      const int iC = k + 4 + h;
+      const int cC = iC < deg ? m.incE[L.first + 64ll * iC] : -1;
+      const uint32_t rC = cC >= 0 ? m.incR[L.first + 64ll * iC] : 0u;
+      const int vB = cB >= 0 ? m.E[(cB >> 2) + c * nT] : 0;
+      T blk[1][9];
+      if (cA >= 0) {  // the same in the four lanes of a quad
+        const int e = cA >> 2, a = cA & 3;
+        if constexpr (MODEL == M_NH_MUSCLE) {
+          tet_block1q<M_NH_MUSCLE>(m, q, params, e, vA, a, c, stretchRef, blk);
+        } else switch (model_at<MODEL>(mids, e)) {
+          case M_NH: tet_block1q<M_NH>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+          case M_STVK: tet_block1q<M_STVK>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+          case M_COROT: tet_block1q<M_COROT>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+          case M_ARAP: tet_block1q<M_ARAP>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+          case M_MASS: tet_block1q<M_MASS>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+          default: tet_block1q<M_NH_LEGACY>(m, q, params, e, vA, a, c, stretchRef, blk); break;
+        }
+      }
+      T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
+      if (h == 0 && cA >= 0) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[0][3 * l + kk]);
+      }
+      __asm__ volatile("" ::: "memory");  // keep the second quad's adds after the first's
+      if (h == 1 && cA >= 0) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int kk = 0; kk < 3; ++kk) lds_add(&dst[3 * l * nb + kk], blk[0][3 * l + kk]);
+      }
+      cA = cB;
+      rA = rB;
+      vA = vB;
       cB = cC;
       rB = rC;
     }
@@ -1162,19 +1348,6 @@ __global__ void __launch_bounds__(kGatherThreads)
   if (touchOut && sink == T(0)) vals[seg0] = sink;
 }
 
-// value of lane K of this lane's quad (DPP quad_perm broadcast; all four lanes of the quad active)
-template <int K>
-__device__ __forceinline__ float quad_bcast(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), K * 0x55, 0xf, 0xf, true));
-}
-template <int K>
-__device__ __forceinline__ double quad_bcast(double x) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, K * 0x55, 0xf, 0xf, true);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), K * 0x55, 0xf, 0xf, true);
-  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
-}
-
 // the 16-byte chunks lane j of a quad loads of a tet's record (NR) and Q (NQ)
 template <typename T>
 struct QuadChunks {
@@ -1205,6 +1378,73 @@ __device__ __forceinline__ void quad_expand(const T* r, T S[45]) {
     S[V] = quad_bcast<V & 3>(r[V >> 2]);
     quad_expand<T, V + 1>(r, S);
   }
+}
+
+// The (lo, hi) block of the tet whose chunks the quad holds, for corner c of a vertex at corner a (lo =
+// min(a, c), hi = max(a, c)): sum_{p,r} (Q[lo][p] Q[hi][r]) M'(p, r), plus tolPos on the diagonal.
+// Q[a] comes from lane a of the quad (a is uniform within the quad), Q[c] is this lane's own.
+template <typename T>
+__device__ __forceinline__ void recq_block(const QuadChunks<T>& cur, int a, int c, T tolPos, T x[9]) {
+  T S[45];
+  quad_expand<T>(cur.r, S);
+  const int src = ((int)threadIdx.x & ~3) | a;
+  T qa[3], qc[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    qc[l] = cur.q[l];
+    qa[l] = __shfl(cur.q[l], src, 64);
+  }
+  const bool low = c < a;
+  T ql[3], qh[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    ql[l] = low ? qc[l] : qa[l];
+    qh[l] = low ? qa[l] : qc[l];
+  }
+  T P[3][3];
+#pragma unroll
+  for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) P[pp][rr] = ql[pp] * qh[rr];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      T y;
+      if constexpr (sizeof(T) == 8) {
+        y = (l == kk) ? tolPos : T(0);
+#pragma unroll
+        for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+          for (int rr = 0; rr < 3; ++rr) y += P[pp][rr] * S[sym9(3 * pp + l, 3 * rr + kk)];
+      } else {
+        // FP32: Q[lo]'s contraction first, then Q[hi]'s (the rounding of rec_block).  The FP32
+        // operator's soft modes are sensitive to it: forming Q[lo][p] Q[hi][r] first doubled the
+        // precision-form Newton step's distance from the FP64 solution (5.6e-6 -> 1.2e-5)
+        T t[3];
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+          T sum = T(0);
+#pragma unroll
+          for (int pp = 0; pp < 3; ++pp) sum += ql[pp] * S[sym9(3 * pp + l, 3 * rr + kk)];
+          t[rr] = sum;
+        }
+        y = qh[0] * t[0] + qh[1] * t[1] + qh[2] * t[2];
+        if (l == kk) y += tolPos;
+      }
+      x[3 * l + kk] = y;
+    }
+}
+
+// adds the block to its slot (dst: the slot's first entry in the vertex's LDS row of nb blocks): entry
+// (l, k) at (l, k), or at (k, l) when a > c
+template <typename T>
+__device__ __forceinline__ void recq_add(T* dst, const T x[9], int a, int c, int nb) {
+  const int rs = (a > c) ? 1 : 3 * nb, cs = (a > c) ? 3 * nb : 1;
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) lds_add(&dst[l * rs + kk * cs], x[3 * l + kk]);
 }
 
 // Assembly gather from the PSD records with four lanes per vertex (16 vertices per workgroup): lane c
@@ -1253,60 +1493,10 @@ __global__ void __launch_bounds__(kGatherThreads)
       const int64_t iC = L.first + 64ll * (k + 2);
       const int cC = (cB >= 0 && k + 2 < L.len) ? m.incE[iC] : -1;
       const uint32_t rC = cC >= 0 ? m.incR[iC] : 0u;
-      T S[45];
-      quad_expand<T>(cur.r, S);
       const int a = cA & 3;
-      // Q[a] from lane a of the quad (a is uniform within the quad), Q[c] is this lane's own
-      const int src = ((int)threadIdx.x & ~3) | a;
-      T qa[3], qc[3];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        qc[l] = cur.q[l];
-        qa[l] = __shfl(cur.q[l], src, 64);
-      }
-      const bool low = c < a;
-      T ql[3], qh[3];
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        ql[l] = low ? qc[l] : qa[l];
-        qh[l] = low ? qa[l] : qc[l];
-      }
-      T P[3][3];
-#pragma unroll
-      for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-        for (int rr = 0; rr < 3; ++rr) P[pp][rr] = ql[pp] * qh[rr];
-      // entry (l, k) of the (lo, hi) block lands at (l, k) of the slot, or at (k, l) when a > c
-      const int rs = (a > c) ? 1 : 3 * nb, cs = (a > c) ? 3 * nb : 1;
-      T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-          T x;
-          if constexpr (sizeof(T) == 8) {
-            x = (l == kk) ? tolPos : T(0);
-#pragma unroll
-            for (int pp = 0; pp < 3; ++pp)
-#pragma unroll
-              for (int rr = 0; rr < 3; ++rr) x += P[pp][rr] * S[sym9(3 * pp + l, 3 * rr + kk)];
-          } else {
-            // FP32: Q[lo]'s contraction first, then Q[hi]'s (the rounding of rec_block).  The FP32
-            // operator's soft modes are sensitive to it: forming Q[lo][p] Q[hi][r] first doubled the
-            // precision-form Newton step's distance from the FP64 solution (5.6e-6 -> 1.2e-5)
-            T t[3];
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-              T sum = T(0);
-#pragma unroll
-              for (int pp = 0; pp < 3; ++pp) sum += ql[pp] * S[sym9(3 * pp + l, 3 * rr + kk)];
-              t[rr] = sum;
-            }
-            x = qh[0] * t[0] + qh[1] * t[1] + qh[2] * t[2];
-            if (l == kk) x += tolPos;
-          }
-          lds_add(&dst[l * rs + kk * cs], x);
-        }
+      T x[9];
+      recq_block(cur, a, c, tolPos, x);
+      recq_add(row + 3 * ((rA >> (8 * c)) & 0xffu), x, a, c, nb);
       selfRank = (int)((rA >> (8 * a)) & 0xffu);
       cur = nxt;
       cA = cB;
@@ -1317,6 +1507,80 @@ __global__ void __launch_bounds__(kGatherThreads)
   }
   __syncthreads();
   if (c == 0 && selfRank >= 0) {  // the diagonal slot, symmetrised once its fold is complete
+    T* d = row + 3 * selfRank;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+      for (int kk = l + 1; kk < 3; ++kk) {
+        const T x = d[3 * l * nb + kk], y = d[3 * kk * nb + l];
+        const T sym = T(0.5) * (x + y);
+        d[3 * l * nb + kk] = sym;
+        d[3 * kk * nb + l] = sym;
+      }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) vals[seg0 + i] = acc[i];
+}
+
+// The same gather with eight lanes per vertex (eight vertices per workgroup): the vertex's two quads take
+// incidences k and k + 1 at each step (quad h: k + h), each as in k_hessian_recq, and the first quad's
+// LDS adds are issued before the second's, so every slot still folds in list order.  Half the LDS per
+// workgroup of k_hessian_recq.
+template <typename T>
+__global__ void __launch_bounds__(kGatherThreads, 3)  // FP64: 168 VGPRs, three waves per SIMD
+    k_hessian_reco(MeshView m, const T* __restrict__ tq, const T* __restrict__ rec, T tolPos, int xcd,
+                   T* __restrict__ vals) {
+  constexpr int VPW = 8;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* acc = reinterpret_cast<T*>(smem);
+  const int v0 = gather_first<VPW>(m, xcd);
+  if (v0 < 0) return;
+  const int vend = min(v0 + VPW, m.nV);
+  const int b0 = m.bptr[v0];
+  const int64_t seg0 = 9ll * b0;
+  const int seglen = 9 * (m.bptr[vend] - b0);
+  for (int i = threadIdx.x; i < seglen; i += kGatherThreads) acc[i] = T(0);
+  __syncthreads();
+  const int vl = (int)threadIdx.x >> 3, h = ((int)threadIdx.x >> 2) & 1, c = (int)threadIdx.x & 3;
+  const int v = v0 + vl;
+  int selfRank = -1, nb = 0;
+  T* row = acc;
+  if (v < m.nV) {  // the same for the eight lanes of a vertex
+    nb = m.bptr[v + 1] - m.bptr[v];
+    row = acc + 9 * (m.bptr[v] - b0);
+    const IncList L = inc_list(m, v);
+    const int deg = m.vdeg[v];  // entries [0, deg) of the list are the vertex's incidences
+    int cA = h < deg ? m.incE[L.first + 64ll * h] : -1;
+    uint32_t rA = cA >= 0 ? m.incR[L.first + 64ll * h] : 0u;
+    int cB = 2 + h < deg ? m.incE[L.first + 64ll * (2 + h)] : -1;
+    uint32_t rB = cB >= 0 ? m.incR[L.first + 64ll * (2 + h)] : 0u;
+    QuadChunks<T> cur;
+    if (cA >= 0) quad_load(rec, tq, cA >> 2, c, cur);
+    for (int k = 0; k < deg; k += 2) {  // the quad's own incidence (k + h) may be past the list's end
+      QuadChunks<T> nxt;
+      if (cB >= 0) quad_load(rec, tq, cB >> 2, c, nxt);
+      const int iC = k + 4 + h;
+      const int cC = iC < deg ? m.incE[L.first + 64ll * iC] : -1;
+      const uint32_t rC = cC >= 0 ? m.incR[L.first + 64ll * iC] : 0u;
+      const int a = cA & 3;
+      T x[9];
+      if (cA >= 0) {
+        recq_block(cur, a, c, tolPos, x);
+        selfRank = (int)((rA >> (8 * a)) & 0xffu);
+      }
+      T* dst = row + 3 * ((rA >> (8 * c)) & 0xffu);
+      if (h == 0 && cA >= 0) recq_add(dst, x, a, c, nb);
+      __asm__ volatile("" ::: "memory");  // keep the second quad's adds after the first's
+      if (h == 1 && cA >= 0) recq_add(dst, x, a, c, nb);
+      cur = nxt;
+      cA = cB;
+      rA = rB;
+      cB = cC;
+      rB = rC;
+    }
+  }
+  __syncthreads();
+  if ((threadIdx.x & 7) == 0 && selfRank >= 0) {  // the diagonal slot, symmetrised once its fold is complete
     T* d = row + 3 * selfRank;
 #pragma unroll
     for (int l = 0; l < 3; ++l)
@@ -1561,15 +1825,23 @@ int gather_grid(const bh_mesh* m, int vpw, bool rec = false) {
 
 template <int VPW, typename T>
 size_t gather_lds(const bh_mesh* m) {
-  const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : m->maxSeg16;
+  const int seg = (VPW == 64) ? m->maxSeg64 : (VPW == 32) ? m->maxSeg32 : (VPW == 16) ? m->maxSeg16 : m->maxSeg8;
   return sizeof(T) * (size_t)std::max(seg, 1);
 }
 
 template <int MODEL, int VPW, int LPV, typename T>
 int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T* vals,
                        hipStream_t s) {
-  auto kern = (LPV == 1) ? k_hessian<MODEL, VPW, T>
-              : (LPV == 2) ? k_hessian_pair<MODEL, VPW, T> : k_hessian_quad<MODEL, VPW, T>;
+  auto kern = [] {
+    if constexpr (LPV == 1)
+      return k_hessian<MODEL, VPW, T>;
+    else if constexpr (LPV == 2)
+      return k_hessian_pair<MODEL, VPW, T>;
+    else if constexpr (LPV == 4)
+      return k_hessian_quad<MODEL, VPW, T>;
+    else
+      return k_hessian_oct<MODEL, VPW, T>;
+  }();
   gather_lds_attr(kern);
   const size_t lds = gather_lds<VPW, T>(m);
   ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
@@ -1580,15 +1852,21 @@ int launch_hessian_vpw(const bh_mesh* m, const double* q, const T* params, const
 }
 
 // Picks the workgroup shape (vertices per workgroup, lanes per vertex) from the mesh's tuning and
-// the LDS the densest vertex group needs, and calls launch(std::integral_constant VPW, LPV).
-template <typename T, typename F>
+// the LDS the densest vertex group needs, and calls launch(std::integral_constant VPW, LPV).  DEFLANES:
+// the lanes per vertex when the mesh leaves the choice to the gather; OCT: the caller has an eight-lane
+// kernel (else eight lanes fall back to four).
+template <typename T, int DEFLANES, bool OCT, typename F>
 int pick_gather_shape(const bh_mesh* m, F&& launch) {
   using std::integral_constant;
   const size_t lim = (size_t)max_lds_bytes();
   const size_t s64 = sizeof(T) * (size_t)m->maxSeg64, s32 = sizeof(T) * (size_t)m->maxSeg32,
-               s16 = sizeof(T) * (size_t)m->maxSeg16;
-  if (m->gatherLanes == 4 && s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 4>());
-  if (m->gatherLanes == 2) {
+               s16 = sizeof(T) * (size_t)m->maxSeg16, s8 = sizeof(T) * (size_t)m->maxSeg8;
+  const int lanes = m->gatherLanes ? m->gatherLanes : DEFLANES;
+  if constexpr (OCT) {
+    if (lanes == 8 && s8 <= lim) return launch(integral_constant<int, 8>(), integral_constant<int, 8>());
+  }
+  if ((lanes == 4 || lanes == 8) && s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 4>());
+  if (lanes == 2) {
     if (s32 <= lim / 2) return launch(integral_constant<int, 32>(), integral_constant<int, 2>());
     if (s16 <= lim) return launch(integral_constant<int, 16>(), integral_constant<int, 2>());
   }
@@ -1600,18 +1878,23 @@ int pick_gather_shape(const bh_mesh* m, F&& launch) {
 
 template <int MODEL, typename T>
 int launch_hessian(const bh_mesh* m, const double* q, const T* params, const uint8_t* mids, T* vals, hipStream_t s) {
-  return pick_gather_shape<T>(m, [&](auto vpw, auto lpv) {
+  return pick_gather_shape<T, 8, true>(m, [&](auto vpw, auto lpv) {
     return launch_hessian_vpw<MODEL, decltype(vpw)::value, decltype(lpv)::value, T>(m, q, params, mids, vals, s);
   });
 }
 
 template <typename T>
 int launch_hessian_rec(const bh_mesh* m, const T* tq, const T* rec, T tolPos, T* vals, hipStream_t s) {
-  return pick_gather_shape<T>(m, [&](auto vpw, auto lpv) {
+  return pick_gather_shape<T, 4, true>(m, [&](auto vpw, auto lpv) {
     constexpr int VPW = decltype(vpw)::value, LPV = decltype(lpv)::value;
     const size_t lds = gather_lds<VPW, T>(m);
     ProfScope ps(m, BH_KT_HESSIAN_GATHER, s);
-    if constexpr (LPV == 4) {
+    if constexpr (LPV == 8) {
+      auto kern = k_hessian_reco<T>;
+      gather_lds_attr(kern);
+      hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW, true)), dim3(kGatherThreads), lds, s,
+                         view(m, gather_ordered(m, VPW, true)), tq, rec, tolPos, gather_xcd(m, VPW, true), vals);
+    } else if constexpr (LPV == 4) {
       auto kern = k_hessian_recq<T>;
       gather_lds_attr(kern);
       hipLaunchKernelGGL(kern, dim3(gather_grid(m, VPW, true)), dim3(kGatherThreads), lds, s,
@@ -1743,7 +2026,7 @@ int hessian_callback(bh_mesh* m, int model, const uint8_t* mids, const double* q
       for (int i = 0; i < n; ++i) cb(buf.h + 144ll * i, e0 + i, user);
       BH_CHECK(hipMemcpyAsync(buf.d, buf.h, sizeof(double) * 144 * (size_t)n, hipMemcpyHostToDevice, s));
     }
-    st = pick_gather_shape<double>(m, [&](auto vpw, auto lpv) {
+    st = pick_gather_shape<double, 4, false>(m, [&](auto vpw, auto lpv) {
       constexpr int VPW = decltype(vpw)::value, LPV = decltype(lpv)::value;
       auto kern = k_hessian_staged<VPW, LPV>;
       gather_lds_attr(kern);
@@ -1957,6 +2240,7 @@ int build_mesh(bh_mesh* m, const int* E, const double* dphidX, const double* vol
   m->maxSeg64 = maxseg(64);
   m->maxSeg32 = maxseg(32);
   m->maxSeg16 = maxseg(16);
+  m->maxSeg8 = maxseg(8);
   m->nblocks = nblocks;
   m->nnz = 9 * nblocks;
   // interleaved incidence lists (groups of 64 vertices, slot-major)
@@ -2444,14 +2728,16 @@ int bh_mesh_set_rest_positions(bh_mesh* m, const double* V) {
 }
 
 int bh_mesh_set_gather_lanes(bh_mesh* m, int lanes) {
-  if (!m || (lanes != 1 && lanes != 2 && lanes != 4)) return BH_BAD_ARG;
+  if (!m || (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)) return BH_BAD_ARG;
   m->gatherLanes = lanes;
   return BH_OK;
 }
 
 int bh_mesh_set_tuning(bh_mesh* m, const bh_tuning* t) {
   if (!m || !t) return BH_BAD_ARG;
-  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2 && t->gather_lanes != 4) return BH_BAD_ARG;
+  if (t->gather_lanes != 0 && t->gather_lanes != 1 && t->gather_lanes != 2 && t->gather_lanes != 4 &&
+      t->gather_lanes != 8)
+    return BH_BAD_ARG;
   if (t->xcd_groups < -1 || t->xcd_groups > 1) return BH_BAD_ARG;
   if (t->coloring < 0 || t->coloring > 2) return BH_BAD_ARG;
   if (t->sweep_lanes != 0 && t->sweep_lanes != 4 && t->sweep_lanes != 8) return BH_BAD_ARG;

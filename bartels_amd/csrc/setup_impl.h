@@ -284,6 +284,8 @@ int build_mesh_device(bh_mesh* m, hipStream_t s) {
   m->maxSeg64 = hstats[0];
   m->maxSeg32 = hstats[1];
   m->maxSeg16 = hstats[2];
+  for (int v0 = 0; v0 < nV; v0 += 8)
+    m->maxSeg8 = std::max(m->maxSeg8, 9 * (m->h_bptr[std::min(v0 + 8, nV)] - m->h_bptr[v0]));
   m->nblocks = nb;
   m->nnz = 9 * nb;
   return BH_OK;

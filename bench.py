@@ -445,7 +445,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", default="256,64,64", help="vertices along x,y,z")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1, 2 or 4 lanes per vertex "
+    ap.add_argument("--gather-lanes", type=int, default=0, help="assembly kernel shape: 1, 2, 4 or 8 lanes per vertex "
                     "(0: the library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-config3", action="store_true", help="skip the 48x48x192 mixed-model assembly leg")
@@ -588,7 +588,7 @@ def main():
     variants = {}
     par32 = params.to(torch.float32)
     H32 = torch.empty(nnz, dtype=torch.float32, device=dev)
-    for lanes in (1, 2, 4):
+    for lanes in (1, 2, 4, 8):
         for xcd, order in ((0, 2), (1, 2), (0, 1)):
             tune(lanes, xcd, order)
             key = f"lanes{lanes}_xcd{xcd}_{'morton' if order == 1 else 'vertex'}"
@@ -598,7 +598,8 @@ def main():
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, callback=psd, stream=stream))
             variants[key + "_fp32_nopsd"] = timed_kernels(
                 lambda: mesh.dq2_f32_device(tm.NEOHOOKEAN, q, par32, H32, stream=stream))
-    tune(args.gather_lanes or 4, 0, 3)  # back to the library default (or the requested shape)
+    tune(0, 0, 3)  # back to the library default (or the requested shape)
+    assert lib.bh_mesh_set_gather_lanes(mesh.handle, args.gather_lanes) == 0
     del H32
 
     B = algorithmic_bytes(nV, nT, nnz)

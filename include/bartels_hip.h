@@ -122,15 +122,17 @@ int bh_mesh_set_element_opts(bh_mesh* mesh, const bh_element_opts* opts);
  * counterpart in the reference (its V argument is passed on every call and unused by the
  * element math). */
 int bh_mesh_set_rest_positions(bh_mesh* mesh, const double* V);
-/* Assembly-kernel shape: 1, 2 or 4 (the default) lanes per vertex (with 2, each lane owns two of the
- * four corner blocks of every incident tet; with 4, one).  Same per-slot fold order. */
+/* Assembly-kernel shape: 1, 2, 4 or 8 lanes per vertex (with 2, each lane owns two of the four corner
+ * blocks of every incident tet; with 4, one; with 8, the vertex's two quads take two incidences per
+ * step), or 0, the default: 8 in the element-math gathers, 4 in the PSD-record gathers.  The callback
+ * path's gather takes 4 for 8.  Same per-slot fold order. */
 int bh_mesh_set_gather_lanes(bh_mesh* mesh, int lanes);
 
 /* Kernel-schedule knobs of a mesh (no reference counterpart: the reference has one CPU schedule).
  * Every setting keeps each slot's ascending-element fold, so results agree to the last bits; a
  * fixed setting is bitwise reproducible. */
 typedef struct bh_tuning {
-  int gather_lanes; /* 0 keep, else 1, 2 or 4 lanes per vertex in the assembly gather */
+  int gather_lanes; /* 0 keep, else 1, 2, 4 or 8 lanes per vertex in the assembly gathers */
   int xcd_groups;   /* -1 keep, 0 off, 1 on: deal vertex groups to the 8 XCDs in contiguous runs */
   int coloring;     /* 0 keep, 1 DSatur (default), 2 greedy in vertex order: the vertex colouring that
                        bh_precond_create uses for the colour sweeps */

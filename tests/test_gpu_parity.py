@@ -301,7 +301,7 @@ def test_assembled_hessian_exactly_symmetric(tm, fp32, psd):
     from bartels_amd import _lib
 
     lib = _lib.load()
-    for lanes in (1, 2, 4):
+    for lanes in (1, 2, 4, 8):
         assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, 0))) == 0
         H = torch.empty(mesh.nnz, dtype=params.dtype, device="cuda")
         fn = mesh.dq2_f32_device if fp32 else mesh.dq2_device
@@ -321,10 +321,11 @@ def test_assembled_hessian_exactly_symmetric(tm, fp32, psd):
 @pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("psd", [True, False])
 def test_gather_variants_agree(tm, fp32, psd):
-    """every kernel schedule -- 1, 2 or 4 lanes per vertex, XCD grouping on/off, vertex groups visited in
-    vertex order or in the Morton order of their rest centroids -- keeps each slot's ascending-element
-    fold and the same element math: same results to rounding across lane counts, and bitwise equal for
-    one lane count whatever the visiting order and XCD grouping"""
+    """every kernel schedule -- 1, 2, 4 or 8 lanes per vertex, XCD grouping on/off, vertex groups visited
+    in vertex order or in the Morton order of their rest centroids -- keeps each slot's ascending-element
+    fold and the same element math: same results to rounding across lane counts, bitwise equal for one
+    lane count whatever the visiting order and XCD grouping, and bitwise equal for 4 and 8 lanes (the
+    same block arithmetic; the two incidences an 8-lane step takes fold in list order)"""
     import ctypes
 
     import torch
@@ -340,7 +341,7 @@ def test_gather_variants_agree(tm, fp32, psd):
         params = params.to(torch.float32)
     lib = _lib.load()
     outs = {}
-    for lanes in (1, 2, 4):
+    for lanes in (1, 2, 4, 8):
         for xcd in (0, 1):
             for order in (1, 2, 3):
                 t = _lib.BhTuning(lanes, xcd, 0, 0, order)
@@ -358,4 +359,6 @@ def test_gather_variants_agree(tm, fp32, psd):
         worst = max(worst, d)
         print(f"schedule (lanes, xcd, order) {k}: max rel diff {d:.2e}, bitwise equal: {np.array_equal(v, ref)}")
         assert np.array_equal(v, outs[(k[0], 0, 2)]), f"schedule {k} differs from vertex order"
+        if k[0] == 8:
+            assert np.array_equal(v, outs[(4,) + k[1:]]), f"schedule {k} differs from four lanes"
     assert worst < (1e-6 if fp32 else 1e-14)
