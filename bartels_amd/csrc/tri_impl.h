@@ -225,46 +225,47 @@ __device__ void tri2_hessian(const double* F, const double phi[3][3], double vol
     }
 }
 
-// Cyclic Jacobi on a symmetric 4x4 (A destroyed): eigenvalues d, eigenvectors in the columns of V
+// Cyclic Jacobi on a symmetric N x N (A destroyed): eigenvalues d, eigenvectors in the columns of V
 // (Golub & Van Loan, Matrix Computations, 4th ed., section 8.5.2); every index is a compile-time
 // constant after unrolling, so A and V stay in registers.
-__device__ __forceinline__ void jacobi4(double A[4][4], double d[4], double V[4][4]) {
+template <int N, int SWEEPS>
+__device__ __forceinline__ void jacobi_sym(double A[N][N], double d[N], double V[N][N]) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 8; ++sweep) {
+    for (int j = 0; j < N; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < SWEEPS; ++sweep) {
     double off = 0.0, dia = 0.0;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < N; ++p) {
       dia += A[p][p] * A[p][p];
 #pragma unroll
-      for (int q = p + 1; q < 4; ++q) off += A[p][q] * A[p][q];
+      for (int q = p + 1; q < N; ++q) off += A[p][q] * A[p][q];
     }
     if (off <= 1e-34 * dia) break;  // converged (quadratically: a few sweeps)
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+    for (int p = 0; p < N - 1; ++p)
 #pragma unroll
-      for (int q = p + 1; q < 4; ++q) {
+      for (int q = p + 1; q < N; ++q) {
         const double apq = A[p][q];
         if (fabs(apq) <= 1e-300) continue;
         const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
         const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
         const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {  // A <- A J (columns p, q)
+        for (int k = 0; k < N; ++k) {  // A <- A J (columns p, q)
           const double akp = A[k][p], akq = A[k][q];
           A[k][p] = c * akp - s * akq;
           A[k][q] = s * akp + c * akq;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {  // A <- J^T A (rows p, q)
+        for (int k = 0; k < N; ++k) {  // A <- J^T A (rows p, q)
           const double apk = A[p][k], aqk = A[q][k];
           A[p][k] = c * apk - s * aqk;
           A[q][k] = s * apk + c * aqk;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < N; ++k) {
           const double vkp = V[k][p], vkq = V[k][q];
           V[k][p] = c * vkp - s * vkq;
           V[k][q] = s * vkp + c * vkq;
@@ -272,7 +273,7 @@ __device__ __forceinline__ void jacobi4(double A[4][4], double d[4], double V[4]
       }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) d[i] = A[i][i];
+  for (int i = 0; i < N; ++i) d[i] = A[i][i];
 }
 
 // The tri2d element Hessian with simple_psd_fix: as for the tets, the three shape gradients sum to
@@ -332,7 +333,7 @@ __device__ void tri2_hessian_psd(const double* F, const double phi[3][3], double
         }
     }
   double d[4], V[4][4];
-  jacobi4(M, d, V);
+  jacobi_sym<4, 8>(M, d, V);
 #pragma unroll
   for (int i = 0; i < 4; ++i) d[i] = d[i] < tol ? tol : d[i];
   double Mp[4][4];
@@ -440,6 +441,85 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// The shell clamp through a 6x6 reduced matrix.  With consistent inputs (shape gradients and normal
+// derivatives that sum to zero over the three vertices) the 9x9 K has the three translations
+// (1, 1, 1) / sqrt 3 x e_c in its null space, so with U (3 x 2) a fixed orthonormal basis of the
+// complement of (1, 1, 1), K = (U x I3) M (U x I3)^T, M = (U x I3)^T K (U x I3) (6x6), and
+// simple_psd_fix(K, tol) = (U x I3) M' (U x I3)^T + max(tol, 0) / 3 (1 1^T x I3), M' = M with its
+// eigenvalues raised to tol (cyclic Jacobi).  When K does not annihilate the translations (to 1e-12 of
+// its largest entry) the full 9x9 clamp runs instead.
+__device__ __forceinline__ void tri3_psd(double* K, double tol) {
+  double kmax = 0.0, tres = 0.0;
+#pragma unroll
+  for (int i = 0; i < 81; ++i) kmax = fmax(kmax, fabs(K[i]));
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) tres = fmax(tres, fabs(K[r + 9 * dd] + K[r + 9 * (3 + dd)] + K[r + 9 * (6 + dd)]));
+  if (!(tres <= 1e-12 * kmax)) {
+    tri_psd<9>(K, tol);
+    return;
+  }
+  constexpr double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;  // 1/sqrt(2), 1/sqrt(6)
+  const double U[3][2] = {{s2, s6}, {-s2, s6}, {0.0, -2.0 * s6}};
+  double M[6][6];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) s += U[a][i] * U[b][j] * K[(3 * a + c) + 9 * (3 * b + dd)];
+          M[3 * i + c][3 * j + dd] = s;
+        }
+#pragma unroll
+  for (int r = 0; r < 6; ++r)  // exactly symmetric
+#pragma unroll
+    for (int c = r + 1; c < 6; ++c) {
+      const double x = 0.5 * (M[r][c] + M[c][r]);
+      M[r][c] = x;
+      M[c][r] = x;
+    }
+  double d[6], V[6][6];
+  jacobi_sym<6, 12>(M, d, V);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) d[i] = d[i] < tol ? tol : d[i];
+  double Mp[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s += V[i][c] * d[c] * V[j][c];
+      Mp[i][j] = s;
+      Mp[j][i] = s;
+    }
+  const double tp = (tol > 0.0 ? tol : 0.0) / 3.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) {
+          double s = (c == dd) ? tp : 0.0;
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) s += U[a][i] * U[b][j] * Mp[3 * i + c][3 * j + dd];
+          K[(3 * a + c) + 9 * (3 * b + dd)] = s;
+          K[(3 * b + dd) + 9 * (3 * a + c)] = s;
+        }
+}
+
 // tri3d StVK shell element Hessian (9 x 9) of element e.  Column 3 a + c of B is vec(e_c phi_a^T +
 // D_col N^T) (D_col = dndq's column 3 a + c), so by bilinearity K(3 a1 + c1, 3 a2 + c2) = vol (H(phi_a1,
 // phi_a2)[c1][c2] + D2 . H(phi_a1, N)[c1][.] + D1 . H(phi_a2, N)[c2][.] + D1^T H(N, N) D2), with
@@ -519,7 +599,7 @@ __global__ void __launch_bounds__(64)
             K[c + 9 * r] = x * v;
           }
       }
-    if (psd) tri_psd<9>(K, tol);
+    if (psd) tri3_psd(K, tol);
 #pragma unroll
     for (int i = 0; i < 81; ++i) stage[82 * t + i] = K[i];
   }

@@ -100,3 +100,44 @@ def test_tri3d_stvk_shell_parity(tri, oracle, psd):
     e = sp.linalg.norm(H_g - H_o) / sp.linalg.norm(H_o)
     print(f"tri3d StVK shell (psd={psd}): Hessian {e:.2e}")
     assert e < (1e-11 if psd else 1e-12)
+
+
+@pytest.mark.parametrize("tol", [1e-3, 1e2, 0.0, -1e-3])
+def test_tri3d_shell_clamp_consistent_normals(tri, oracle, tol):
+    """the shell clamp with the geometric normal derivatives (they sum to zero over the three vertices,
+    as the shape gradients do), so the element Hessians annihilate the translations and the clamp runs
+    through the 6x6 reduced matrix; against the oracle's 9x9 eigensolver, at tolerances that clamp
+    nothing, some or most of the spectrum, and that leave the translations at zero"""
+    rng = np.random.default_rng(37)
+    V2, E = meshgen.strip2d(12, 5, h=0.05, seed=37, jitter_amount=0.1)
+    V = np.column_stack([V2, 0.02 * np.sin(V2[:, 0] * 9.0)])
+    nV, nT = V.shape[0], E.shape[0]
+    dX = tri.linear_tri3dmesh_dphi_dX(V, E)
+    N = np.cross(V[E[:, 1]] - V[E[:, 0]], V[E[:, 2]] - V[E[:, 0]])
+    N /= np.linalg.norm(N, axis=1)[:, None]
+    x = V + 0.01 * rng.standard_normal(V.shape)
+    x[:, 0] *= 0.97  # some compression: negative eigenvalues to clamp
+    q = x.ravel()
+    e1, e2 = x[E[:, 1]] - x[E[:, 0]], x[E[:, 2]] - x[E[:, 0]]
+    c = np.cross(e1, e2)
+    cn = np.linalg.norm(c, axis=1)
+    n = c / cn[:, None]
+    Pn = (np.eye(3)[None] - n[:, :, None] * n[:, None, :]) / cn[:, None, None]
+
+    def skew(v):
+        z = np.zeros(v.shape[0])
+        return np.stack([np.stack([z, -v[:, 2], v[:, 1]], 1), np.stack([v[:, 2], z, -v[:, 0]], 1),
+                         np.stack([-v[:, 1], v[:, 0], z], 1)], 1)
+
+    dc1, dc2 = -skew(e2), skew(e1)
+    dndq = np.concatenate([Pn @ d_ for d_ in (-(dc1 + dc2), dc1, dc2)], axis=2).reshape(nT, 27, order="F")
+    vol = 0.5 * np.linalg.norm(np.cross(V[E[:, 1]] - V[E[:, 0]], V[E[:, 2]] - V[E[:, 0]]), axis=1) * 1e-3
+    lam, mu = meshgen.emu_to_lame(np.full(nT, 1e6), np.full(nT, 0.3))
+    params = np.stack([lam, mu], 1)
+    H_g = tri.linear_tri3dmesh_stvk_dq2(V, E, q, dX, N, n, dndq, vol, params, tri.simple_psd_fix(tol))
+    H_o = oracle.tri3dmesh_stvk_dq2(nV, E, q, dX, N, n, dndq, vol, params, psd=True, psd_tol=tol)
+    H_raw = oracle.tri3dmesh_stvk_dq2(nV, E, q, dX, N, n, dndq, vol, params, psd=False)
+    e = sp.linalg.norm(H_g - H_o) / sp.linalg.norm(H_o)
+    moved = sp.linalg.norm(H_o - H_raw) / sp.linalg.norm(H_raw)
+    print(f"tri3d shell clamp, consistent normals, tol {tol}: Hessian {e:.2e} (the clamp moved H by {moved:.2e})")
+    assert e < 1e-11
