@@ -649,9 +649,8 @@ __device__ __forceinline__ void quad_rows(const U v[3], T rows[4][3]) {
 template <typename T>
 __device__ __forceinline__ void quad_tet_inputs(const MeshView& m, const double* __restrict__ q,
                                                 const T* __restrict__ params, int e, int vid, int c, T phi[4][3],
-                                                T F[9], T& vol, T& p0, T& p1) {
+                                                T pc[3], T F[9], T& vol, T& p0, T& p1) {
   const int64_t nT = m.nT;
-  T pc[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     if constexpr (sizeof(T) == 8)
@@ -682,7 +681,8 @@ __device__ __forceinline__ void quad_tet_inputs(const MeshView& m, const double*
 
 // tet_block1e with the tet's inputs loaded cooperatively by the quad (lane c: corner c, vid: the tet's
 // vertex c); the same arithmetic, so the same block to the last bit.  All four lanes of the quad take
-// part (they hold the same tet).
+// part (they hold the same tet).  The lane's own shape-gradient row is phi_c; phi_a (a is the same in
+// the four lanes) comes from lane a over ds_bpermute.
 template <int MODEL, typename T>
 __device__ __forceinline__ void tet_block1q(const MeshView& m, const double* __restrict__ q,
                                             const T* __restrict__ params, int e, int vid, int a, int c,
@@ -694,13 +694,14 @@ __device__ __forceinline__ void tet_block1q(const MeshView& m, const double* __r
     for (int i = 0; i < 9; ++i) blk[0][i] = (i % 4 == 0) ? cm : T(0);
     return;
   }
-  T phi[4][3];
+  T phi[4][3], pm[3];
   Pre<T> P;
   T p0, p1;
-  quad_tet_inputs(m, q, params, e, vid, c, phi, P.F, P.vol, p0, p1);
-  T pa[3], pm[3];
-  select_row(phi, a, pa);
-  select_row(phi, c, pm);
+  quad_tet_inputs(m, q, params, e, vid, c, phi, pm, P.F, P.vol, p0, p1);
+  T pa[3];
+  const int src = ((int)threadIdx.x & ~3) | a;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) pa[j] = __shfl(pm[j], src, 64);
   pre_hessian<MODEL>(P, p0, p1, stretch_ref);
   if constexpr (MODEL == M_NH_MUSCLE) {
     const T f = params[(int64_t)(2 + c) * m.nT + e];
