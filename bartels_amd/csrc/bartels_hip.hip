@@ -644,8 +644,8 @@ __device__ __forceinline__ void quad_rows(const U v[3], T rows[4][3]) {
 
 // One tet's inputs in all four lanes of a quad: lane c loads its own corner's vertex position (vid: the
 // tet's vertex c) and shape-gradient row, and one of vol / params[e] / params[nT + e] (lanes 0, 1, 2);
-// the quad hands them around over DPP, and every lane forms F exactly as load_FG does.  A quarter of
-// the loads of every lane reading the whole tet.
+// the quad hands them around over DPP (pc: this lane's own row), and every lane forms F exactly as
+// load_FG does.  A quarter of the load instructions of every lane reading the whole tet.
 template <typename T>
 __device__ __forceinline__ void quad_tet_inputs(const MeshView& m, const double* __restrict__ q,
                                                 const T* __restrict__ params, int e, int vid, int c, T phi[4][3],
