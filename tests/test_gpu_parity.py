@@ -362,3 +362,43 @@ def test_gather_variants_agree(tm, fp32, psd):
         if k[0] == 8:
             assert np.array_equal(v, outs[(4,) + k[1:]]), f"schedule {k} differs from four lanes"
     assert worst < (1e-6 if fp32 else 1e-14)
+
+
+@pytest.mark.parametrize("psd", [False, True])
+def test_gather_shapes_small_and_ragged(tm, oracle, psd):
+    """every gather shape (1, 2, 4, 8 lanes per vertex) on small ragged meshes -- one tet; two tets
+    sharing a face (odd vertex degrees, so an eight-lane step has one idle quad); a tet fan around an
+    edge; a mesh with an unused vertex -- against the oracle, with and without the clamp"""
+    import ctypes
+
+    from bartels_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(5)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 1]], float) * 0.1
+    fan = np.array([[0, 0, 0], [0, 0, 0.1]] + [[0.1 * np.cos(t), 0.1 * np.sin(t), 0.05]
+                                               for t in np.linspace(0, 2 * np.pi, 7)[:-1]])
+    cases = {
+        "one tet": (base[:4], np.array([[0, 1, 2, 3]])),
+        "two tets sharing a face": (base, np.array([[0, 1, 2, 3], [1, 2, 3, 4]])),
+        "fan of six tets around an edge": (fan, np.array([[0, 1, 2 + k, 2 + (k + 1) % 6] for k in range(6)])),
+        "unused vertex": (np.vstack([base, [[0.5, 0.5, 0.5]]]), np.array([[0, 1, 2, 3], [1, 2, 3, 4]])),
+    }
+    lam, mu = meshgen.emu_to_lame(2e6, 0.45)
+    for name, (V, E) in cases.items():
+        E = np.array(E, dtype=np.int32)
+        d = np.einsum("ij,ij->i", np.cross(V[E[:, 1]] - V[E[:, 0]], V[E[:, 2]] - V[E[:, 0]]), V[E[:, 3]] - V[E[:, 0]])
+        E[d < 0] = E[d < 0][:, [0, 1, 3, 2]]  # positively oriented
+        mesh, dX, vol = make_mesh(tm, V, E)
+        q = (V + 0.02 * rng.standard_normal(V.shape)).ravel()
+        params = meshgen.model_params(np.zeros(E.shape[0], np.uint8), lam, mu)
+        H_or = oracle.tetmesh_dq2(0, V.shape[0], E, q, dX, vol, params, psd=psd, psd_tol=1e-3)
+        outs = {}
+        for lanes in (1, 2, 4, 8):
+            assert lib.bh_mesh_set_tuning(mesh.handle, ctypes.byref(_lib.BhTuning(lanes, 0))) == 0
+            H = csr_from_mesh(mesh, mesh.dq2_values(0, q, params, callback=tm.simple_psd_fix(1e-3) if psd else None))
+            outs[lanes] = H
+            e = sp.linalg.norm(H - H_or) / sp.linalg.norm(H_or)
+            print(f"{name}, psd={psd}, lanes {lanes}: {e:.2e}")
+            assert e < 1e-12, (name, lanes)
+        assert (outs[8] != outs[4]).nnz == 0, name  # four and eight lanes: the same bits
