@@ -320,6 +320,15 @@ __device__ __forceinline__ void load_fibre(const T* __restrict__ params, int nT,
   for (int c = 0; c < 4; ++c) fib[c] = params[(int64_t)(2 + c) * nT + e];
 }
 
+// Logical block of a one-dimensional launch of 8 ceil(nblocks / 8) workgroups, dealt to the XCDs in
+// contiguous runs: the hardware hands workgroup b to XCD b mod 8, so logical block (b mod 8) per + b / 8
+// gives XCD x the run [x per, (x + 1) per) of consecutive tet ranges, whose shared vertices then meet in
+// one L2.  Returns nblocks or more for a surplus workgroup.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  return (int)(blockIdx.x & 7u) * ((nblocks + 7) >> 3) + (int)(blockIdx.x >> 3);
+}
+__host__ __forceinline__ int xcd_grid(int nblocks) { return 8 * ((nblocks + 7) / 8); }
+
 // ---- energy ----------------------------------------------------------------------------------
 template <int MODEL, typename T>
 __device__ __forceinline__ double tet_energy(const MeshView& m, const double* __restrict__ q,
@@ -363,9 +372,11 @@ __device__ __forceinline__ double block_reduce_sum(double v, double* sh) {
 template <int MODEL, typename T>
 __global__ void __launch_bounds__(kEnergyBlock)
     k_energy(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
-             const uint8_t* __restrict__ mids, double* __restrict__ partials) {
+             const uint8_t* __restrict__ mids, int nblk, double* __restrict__ partials) {
   __shared__ double sh[kEnergyBlock / 64];
-  const int base = blockIdx.x * kEnergyBlock * kEnergyItems + threadIdx.x;
+  const int lb = xcd_block(nblk);
+  if (lb >= nblk) return;
+  const int base = lb * kEnergyBlock * kEnergyItems + threadIdx.x;
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < kEnergyItems; ++i) {
@@ -373,7 +384,7 @@ __global__ void __launch_bounds__(kEnergyBlock)
     if (e < m.nT) acc += tet_energy<MODEL>(m, q, params, mids, e);
   }
   const double s = block_reduce_sum(acc, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) partials[lb] = s;
 }
 
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ partials, int n,
@@ -1073,14 +1084,16 @@ template <int MODEL, typename T>
 __global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
-                int32_t* __restrict__ count, T* __restrict__ rec) {
+                int32_t* __restrict__ count, int nblk, T* __restrict__ rec) {
   // FP64: the correction columns W (27 values per tet), then one 16-entry round of the workgroup's
   // records at a time (14 KB, so that the registers, not LDS, bound the occupancy); FP32: the whole
   // records, M parked there during the eigensolver
   constexpr int kRows = sizeof(T) == 8 ? 27 : kRecW;
   __shared__ T stash[kRows * kRecPitch];  // entry o of the lane's tet at o * kRecPitch + lane
   const int lane = (int)threadIdx.x;
-  const int e0 = (int)blockIdx.x * 64;
+  const int lb = xcd_block(nblk);  // consecutive tet blocks on one XCD: the second formation's re-reads
+  if (lb >= nblk) return;
+  const int e0 = lb * 64;
   const int e = e0 + lane;
   bool clamped = false;
   if constexpr (sizeof(T) == 8) {
@@ -1940,8 +1953,9 @@ int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* 
   *rec = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
   ProfScope ps(m, BH_KT_PSD, s);
-  hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
-                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, *rec);
+  const int nblk = (m->nT + 63) / 64;
+  hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3(xcd_grid(nblk)), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, nblk, *rec);
   BH_CHECK(hipGetLastError());
   hipLaunchKernelGGL((k_psd_full<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
                      m->opts.stretch_reference_layout, m->psdFlag, *rec);
@@ -2091,8 +2105,8 @@ int launch_energy(bh_mesh* m, const double* q, const T* params, const uint8_t* m
   if (own) partials = m->partials;
   ProfScope ps(own ? m : nullptr, BH_KT_ENERGY, s);
   if (grid > 0) {
-    hipLaunchKernelGGL((k_energy<MODEL, T>), dim3(grid), dim3(kEnergyBlock), 0, s, view(m), q, params, mids,
-                       partials);
+    hipLaunchKernelGGL((k_energy<MODEL, T>), dim3(xcd_grid(grid)), dim3(kEnergyBlock), 0, s, view(m), q, params,
+                       mids, grid, partials);
     BH_CHECK(hipGetLastError());
   }
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, s, partials, grid, out);
