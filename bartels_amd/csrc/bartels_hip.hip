@@ -438,9 +438,10 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(256)
-    k_gather_forces(MeshView m, const double* __restrict__ f, double* __restrict__ g) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= m.nV) return;
+    k_gather_forces(MeshView m, const double* __restrict__ f, int nblk, double* __restrict__ g) {
+  const int lb = xcd_block(nblk);  // neighbouring vertex blocks, which read the same tets' forces, on one XCD
+  const int v = lb * 256 + threadIdx.x;  // blockDim 256
+  if (lb >= nblk || v >= m.nV) return;
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   const IncList L = inc_list(m, v);
   const int64_t nT = m.nT;
@@ -2141,7 +2142,8 @@ int launch_gradient(bh_mesh* m, const double* q, const T* params, const uint8_t*
     hipLaunchKernelGGL((k_tet_forces<MODEL, T>), dim3((m->nT + 255) / 256), dim3(256), 0, s, view(m), q, params, mids,
                        m->tetForces);
   }
-  hipLaunchKernelGGL(k_gather_forces, dim3(grid), dim3(256), 0, s, view(m), (const double*)m->tetForces, g);
+  hipLaunchKernelGGL(k_gather_forces, dim3(xcd_grid(grid)), dim3(256), 0, s, view(m), (const double*)m->tetForces, grid,
+                     g);
   BH_CHECK(hipGetLastError());
   return BH_OK;
 }
