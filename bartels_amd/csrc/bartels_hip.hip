@@ -993,10 +993,9 @@ __global__ void __launch_bounds__(kGatherThreads)
 // full tridiagonal-QL eigensolver in a second pass for k > 3.  The assembly gather then reads only
 // the records and the tets' Q: the per-tet element math runs once, not once per corner.
 
-// T: the arithmetic (and the mesh data read: FP64 or the FP32 copies); TP: the parameters' type
-template <int MODEL, typename T, typename TP = T>
+template <int MODEL, typename T>
 __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __restrict__ q,
-                                          const TP* __restrict__ params, int e, bool stretchRef, T M[45]) {
+                                          const T* __restrict__ params, int e, bool stretchRef, T M[45]) {
   T phi[4][3], Q[4][3], R[3][3];
   Pre<T> P;
   load_phi(m, e, phi);
@@ -1005,29 +1004,28 @@ __device__ __forceinline__ void reduced_M(const MeshView& m, const double* __res
     load_FG(m, q, load_E(m, e), phi, P.F, Gh);
   }
   P.vol = load_vol<T>(m, e);
-  const T p0 = (T)params[e], p1 = (T)params[(int64_t)m.nT + e];
+  const T p0 = params[e], p1 = params[(int64_t)m.nT + e];
   phi_qr(phi, Q, R);
   pre_hessian<MODEL>(P, p0, p1, stretchRef);
   if constexpr (MODEL == M_NH_MUSCLE) {
     T fib[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) fib[c] = (T)params[(int64_t)(2 + c) * m.nT + e];
+    load_fibre(params, m.nT, e, fib);
     set_fibre(P, fib);
   }
   reduced_packed<MODEL>(P, R, M);
 }
 
-template <int MODEL, typename T, typename TP = T>
-__device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const TP* params,
+template <int MODEL, typename T>
+__device__ __forceinline__ void reduced_M_dispatch(const MeshView& m, const double* q, const T* params,
                                                    const uint8_t* mids, int e, bool stretchRef, T M[45]) {
   if constexpr (MODEL == M_NH_MUSCLE) {
-    reduced_M<M_NH_MUSCLE, T, TP>(m, q, params, e, stretchRef, M);
+    reduced_M<M_NH_MUSCLE>(m, q, params, e, stretchRef, M);
   } else switch (model_at<MODEL>(mids, e)) {
-    case M_NH: reduced_M<M_NH, T, TP>(m, q, params, e, stretchRef, M); break;
-    case M_STVK: reduced_M<M_STVK, T, TP>(m, q, params, e, stretchRef, M); break;
-    case M_COROT: reduced_M<M_COROT, T, TP>(m, q, params, e, stretchRef, M); break;
-    case M_ARAP: reduced_M<M_ARAP, T, TP>(m, q, params, e, stretchRef, M); break;
-    default: reduced_M<M_NH_LEGACY, T, TP>(m, q, params, e, stretchRef, M); break;
+    case M_NH: reduced_M<M_NH>(m, q, params, e, stretchRef, M); break;
+    case M_STVK: reduced_M<M_STVK>(m, q, params, e, stretchRef, M); break;
+    case M_COROT: reduced_M<M_COROT>(m, q, params, e, stretchRef, M); break;
+    case M_ARAP: reduced_M<M_ARAP>(m, q, params, e, stretchRef, M); break;
+    default: reduced_M<M_NH_LEGACY>(m, q, params, e, stretchRef, M); break;
   }
 }
 
@@ -1088,54 +1086,50 @@ __global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
                 int32_t* __restrict__ count, int nblk, T* __restrict__ rec) {
-  // The pass computes in FP64 for both record types (the precision form's FP32 records are the FP64
-  // M' rounded once, from the mesh's FP64 geometry and the caller's FP32 parameters).  LDS: the
-  // correction columns W (27 values per tet), then the records on their way out: FP64 one 16-entry
-  // round of the workgroup's records at a time, FP32 all 48 entries in one round (14 KB either way,
-  // so that the registers, not LDS, bound the occupancy)
-  constexpr int kRows = 27;
-  __shared__ double stash[kRows * kRecPitch];  // entry o of the lane's tet at o * kRecPitch + lane
-  static_assert(kRecW * kRecPitch * sizeof(float) <= sizeof(stash), "FP32 records fit the stash");
+  // FP64: the correction columns W (27 values per tet), then one 16-entry round of the workgroup's
+  // records at a time (14 KB, so that the registers, not LDS, bound the occupancy); FP32: the whole
+  // records, M parked there during the eigensolver
+  constexpr int kRows = sizeof(T) == 8 ? 27 : kRecW;
+  __shared__ T stash[kRows * kRecPitch];  // entry o of the lane's tet at o * kRecPitch + lane
   const int lane = (int)threadIdx.x;
   const int lb = xcd_block(nblk);  // consecutive tet blocks on one XCD: the second formation's re-reads
   if (lb >= nblk) return;
   const int e0 = lb * 64;
   const int e = e0 + lane;
-  const double tl = (double)tol;
   bool clamped = false;
-  // The QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the rotational
-  // modes put nearly every tet of a mildly stressed mesh below it).  The eigensolver overwrites M, so
-  // M is formed again afterwards (the same arithmetic: the same bits) and the columns of W, parked in
-  // LDS, are added to it in the order the solver produced them.
-  double M[45];
-  if (e < m.nT) {
-    reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
-    const int k = psd_correction_low(M, tl, [&](int j, const double* w) {
-#pragma unroll
-      for (int i = 0; i < 9; ++i) stash[(9 * j + i) * kRecPitch + lane] = w[i];  // W's column j
-    });
-    clamped = k > 0;
-    // the element's inputs are loaded again (L1 / L2 hits) after the eigensolver rather than kept live
-    // through it: the tet index of the second formation is made to depend on the solver's result, so
-    // the compiler cannot hoist those loads above it
-    int e2 = e;
-    __asm__ volatile("" : "+v"(e2) : "v"(k));
-    reduced_M_dispatch<MODEL>(m, q, params, mids, e2, stretchRef != 0, M);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      if (c < k && k <= 3) {
-        double w[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) w[i] = stash[(9 * c + i) * kRecPitch + lane];
-#pragma unroll
-        for (int i = 0; i < 9; ++i)
-#pragma unroll
-          for (int j = i; j < 9; ++j) M[sym9(i, j)] += w[i] * w[j];
-      }
-    flag[e] = (uint8_t)(k > 3 ? kPsdPending : 0);  // pending records are rewritten by k_psd_full
-  }
-  count_flagged(clamped, count);
   if constexpr (sizeof(T) == 8) {
+    // The QL eigenvalues decide (no separate positive-definiteness test: with tol = 1e-3 the rotational
+    // modes put nearly every tet of a mildly stressed mesh below it).  The eigensolver overwrites M, so
+    // M is formed again afterwards (the same arithmetic: the same bits) and the columns of W, kept in
+    // registers, are added to it in the order the solver produced them.
+    T M[45];
+    if (e < m.nT) {
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+      const int k = psd_correction_low(M, tol, [&](int j, const T* w) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) stash[(9 * j + i) * kRecPitch + lane] = w[i];  // W's column j
+      });
+      clamped = k > 0;
+      // the element's inputs are loaded again (L1 / L2 hits) after the eigensolver rather than kept live
+      // through it: the tet index of the second formation is made to depend on the solver's result, so
+      // the compiler cannot hoist those loads above it
+      int e2 = e;
+      __asm__ volatile("" : "+v"(e2) : "v"(k));
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e2, stretchRef != 0, M);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c < k && k <= 3) {
+          T w[9];
+#pragma unroll
+          for (int i = 0; i < 9; ++i) w[i] = stash[(9 * c + i) * kRecPitch + lane];
+#pragma unroll
+          for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = i; j < 9; ++j) M[sym9(i, j)] += w[i] * w[j];
+        }
+      flag[e] = (uint8_t)(k > 3 ? kPsdPending : 0);  // pending records are rewritten by k_psd_full
+    }
+    count_flagged(clamped, count);
     // three rounds of 16 entries: round r covers elements [16 r, 16 r + 16) of each 48-element record,
     // one 128-byte line per tet; lane L of store i writes element (64 i + L) mod 16 of tet (64 i + L) / 16
 #pragma unroll
@@ -1144,7 +1138,7 @@ __global__ void __launch_bounds__(64, 2)
 #pragma unroll
       for (int o = 0; o < 16; ++o) {
         const int v = rec_value<T>(16 * r + o);
-        stash[o * kRecPitch + lane] = (e < m.nT && v >= 0) ? M[v < 0 ? 0 : v] : 0.0;
+        stash[o * kRecPitch + lane] = (e < m.nT && v >= 0) ? M[v < 0 ? 0 : v] : T(0);
       }
       __syncthreads();
 #pragma unroll
@@ -1155,13 +1149,39 @@ __global__ void __launch_bounds__(64, 2)
       }
     }
   } else {
-    float* fst = reinterpret_cast<float*>(stash);
-    __syncthreads();  // every lane has read its W columns
 #pragma unroll
-    for (int o = 0; o < kRecW; ++o) {
-      const int v = rec_value<T>(o);
-      fst[o * kRecPitch + lane] = (e < m.nT && v >= 0) ? (float)M[v < 0 ? 0 : v] : 0.0f;
+    for (int o = 0; o < kRecW; ++o)
+      if (rec_value<T>(o) < 0) stash[o * kRecPitch + lane] = T(0);
+    if (e < m.nT) {
+      T M[45];
+      reduced_M_dispatch<MODEL>(m, q, params, mids, e, stretchRef != 0, M);
+#pragma unroll
+      for (int i = 0; i < 45; ++i) stash[rec_off<T>(i) * kRecPitch + lane] = M[i];
+      if (!shifted_pd(M, tol)) {
+        // FP32: the full eigensolver.  Inverse iteration's vectors (Gram-Schmidt in FP32) carry ~1e-6
+        // of |M|, which on stiff tets is of the order of tol itself and leaves the clamped record
+        // short of positive definite; the orthogonal eigenvector matrix keeps that error at ~1e-7.
+        clamped = true;
+        T dm[45];
+        const int k = psd_correction(M, tol, dm);
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+          for (int j = i; j < 9; ++j) {
+            T acc = stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane];
+            if (k > 3) {
+              acc += dm[sym9(i, j)];
+            } else {
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                if (c < k) acc += dm[9 * c + i] * dm[9 * c + j];
+            }
+            stash[rec_off<T>(sym9(i, j)) * kRecPitch + lane] = acc;
+          }
+      }
+      flag[e] = 0;
     }
+    count_flagged(clamped, count);
     __syncthreads();
     // element idx = kRecW t + o of the block: (t, o) advanced by 64 = kRecW + 16 per step, no division
     const int n = kRecW * min(64, m.nT - e0);
@@ -1169,7 +1189,7 @@ __global__ void __launch_bounds__(64, 2)
     static_assert(kRecW == 48, "write-out stride assumes 64 = kRecW + 16");
     int t = lane >= kRecW ? 1 : 0, o = lane - kRecW * t;
     for (int idx = lane; idx < n; idx += 64) {
-      __builtin_nontemporal_store(fst[o * kRecPitch + t], &out[idx]);  // streamed: read back by the gather
+      __builtin_nontemporal_store(stash[o * kRecPitch + t], &out[idx]);  // streamed: read back by the gather
       o += 16;
       t += 1;
       if (o >= kRecW) {
