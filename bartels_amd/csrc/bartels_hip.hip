@@ -1085,16 +1085,14 @@ template <int MODEL, typename T>
 __global__ void __launch_bounds__(64, 2)
     k_psd_fused(MeshView m, const double* __restrict__ q, const T* __restrict__ params,
                 const uint8_t* __restrict__ mids, T tol, int stretchRef, uint8_t* __restrict__ flag,
-                int32_t* __restrict__ count, int nblk, T* __restrict__ rec) {
+                int32_t* __restrict__ count, T* __restrict__ rec) {
   // FP64: the correction columns W (27 values per tet), then one 16-entry round of the workgroup's
   // records at a time (14 KB, so that the registers, not LDS, bound the occupancy); FP32: the whole
   // records, M parked there during the eigensolver
   constexpr int kRows = sizeof(T) == 8 ? 27 : kRecW;
   __shared__ T stash[kRows * kRecPitch];  // entry o of the lane's tet at o * kRecPitch + lane
   const int lane = (int)threadIdx.x;
-  const int lb = xcd_block(nblk);  // consecutive tet blocks on one XCD: the second formation's re-reads
-  if (lb >= nblk) return;
-  const int e0 = lb * 64;
+  const int e0 = (int)blockIdx.x * 64;
   const int e = e0 + lane;
   bool clamped = false;
   if constexpr (sizeof(T) == 8) {
@@ -1954,9 +1952,8 @@ int run_psd_passes(bh_mesh* m, const double* q, const T* params, const uint8_t* 
   *rec = reinterpret_cast<T*>(*buf);
   BH_CHECK(hipMemsetAsync(m->psdCount, 0, sizeof(int32_t), s));
   ProfScope ps(m, BH_KT_PSD, s);
-  const int nblk = (m->nT + 63) / 64;
-  hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3(xcd_grid(nblk)), dim3(64), 0, s, view(m), q, params, mids, tol,
-                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, nblk, *rec);
+  hipLaunchKernelGGL((k_psd_fused<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
+                     m->opts.stretch_reference_layout, m->psdFlag, m->psdCount, *rec);
   BH_CHECK(hipGetLastError());
   hipLaunchKernelGGL((k_psd_full<MODEL, T>), dim3((m->nT + 63) / 64), dim3(64), 0, s, view(m), q, params, mids, tol,
                      m->opts.stretch_reference_layout, m->psdFlag, *rec);
